@@ -1,0 +1,81 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "rte-rrtmgp-cpp_amd")
+for p in (PKG, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+AVAILABLE = ["h2o", "co2", "o3", "n2o", "ch4"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP) device")
+    import build_oracle  # noqa: E402  (oracle recipe, test infrastructure)
+    src = os.path.join(ROOT, "oracle", "rrtmgp_oracle.cpp")
+    if (not os.path.exists(build_oracle.LIB)
+            or os.path.getmtime(build_oracle.LIB) < os.path.getmtime(src)):
+        build_oracle.build()
+
+
+@pytest.fixture(scope="session")
+def syn():
+    from rrtmgp_amd import synthetic
+    return synthetic
+
+
+@pytest.fixture(scope="session")
+def kd_lw(syn):
+    return syn.make_kdist(True, syn.SEED_KDIST_LW)
+
+
+@pytest.fixture(scope="session")
+def kd_sw(syn):
+    return syn.make_kdist(False, syn.SEED_KDIST_SW)
+
+
+@pytest.fixture(scope="session")
+def lut_lw(syn):
+    return syn.make_cloud_lut(True, syn.SEED_CLOUD_LW)
+
+
+@pytest.fixture(scope="session")
+def lut_sw(syn):
+    return syn.make_cloud_lut(False, syn.SEED_CLOUD_SW)
+
+
+@pytest.fixture(scope="session")
+def O():
+    import oracle
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def okd_lw(O, kd_lw):
+    return O.Kdist(kd_lw, AVAILABLE)
+
+
+@pytest.fixture(scope="session")
+def okd_sw(O, kd_sw):
+    return O.Kdist(kd_sw, AVAILABLE)
+
+
+@pytest.fixture(scope="session")
+def atm_c0(syn):
+    """Config-0 shaped clear-sky profiles, reduced to 16 columns for test speed."""
+    return syn.make_atmosphere(syn.SEED_ATMOS[0], 16, 60)
+
+
+@pytest.fixture(scope="session")
+def atm_cloudy(syn):
+    return syn.make_atmosphere(syn.SEED_ATMOS[1], 24, 128, cloudy=True)
+
+
+def rel_err(a, b, floor=0.0):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)) if a.size else 0.0
