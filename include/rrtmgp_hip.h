@@ -1,0 +1,339 @@
+/*
+ * rrtmgp_hip.h -- C ABI of the MI355X (gfx950) RTE+RRTMGP column-radiation kernels.
+ *
+ * This is the thin layer the C++ class API (Gas_optics_rrtmgp_gpu, Cloud_optics_gpu,
+ * Rte_lw_gpu, Rte_sw_gpu, Fluxes_*_gpu) calls, and the surface an FFI binds.  It replaces the
+ * reference's two internal kernel boundaries:
+ *   - the CUDA launcher namespaces of include_kernels_cuda/*.h (device pointers), and
+ *   - the Fortran C-ABI of include/rrtmgp_kernels.h:31-200 (CPU path).
+ * Conventions (all entry points):
+ *   - arrays are device pointers in the reference's Fortran layouts (first index fastest,
+ *     e.g. tau(ncol, nlay, ngpt) has idx = icol + ncol*(ilay + nlay*igpt));
+ *   - index tables (band_lims_gpt, gpoint_flavor, flavor, ...) hold 1-based values as in the
+ *     reference; Bool is signed char (RTE_USE_CBOOL);
+ *   - extents are int64_t; every array offset is computed in 64 bits;
+ *   - stream is a hipStream_t (NULL = default stream); launches are asynchronous;
+ *   - the return value is RRTMGP_OK or an error code; rrtmgp_hip_last_error() gives the
+ *     message of the last failure on the calling thread;
+ *   - _f32 / _f64 suffixes select single / double precision of the real arrays.
+ */
+#ifndef RRTMGP_HIP_H
+#define RRTMGP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum
+{
+    RRTMGP_OK = 0,
+    RRTMGP_ERR_ARG = 1,      /* invalid argument (std::invalid_argument / runtime_error in C++) */
+    RRTMGP_ERR_RANGE = 2,    /* input out of the table range (std::range_error in C++)          */
+    RRTMGP_ERR_HIP = 3,      /* HIP runtime error                                               */
+    RRTMGP_ERR_RUNTIME = 4   /* any other failure                                               */
+};
+
+#define RRTMGP_MAX_GAS 32
+
+typedef signed char rrtmgp_bool;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Runtime helpers (FFI callers without their own HIP binding).  No reference counterpart:     */
+/* the reference allocates through Tools_gpu::allocate_gpu (include/tools_gpu.h:72-99).        */
+/* ------------------------------------------------------------------------------------------ */
+const char* rrtmgp_hip_last_error(void);
+int rrtmgp_hip_device_count(int* n);
+int rrtmgp_hip_set_device(int device);
+int rrtmgp_hip_device_name(char* buf, int len);
+int rrtmgp_hip_malloc(void** ptr, size_t bytes);
+int rrtmgp_hip_free(void* ptr);
+int rrtmgp_hip_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int rrtmgp_hip_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int rrtmgp_hip_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int rrtmgp_hip_memset(void* ptr, int value, size_t bytes, void* stream);
+int rrtmgp_hip_stream_create(void** stream);
+int rrtmgp_hip_stream_destroy(void* stream);
+int rrtmgp_hip_stream_synchronize(void* stream);
+int rrtmgp_hip_device_synchronize(void);
+int rrtmgp_hip_event_create(void** event);
+int rrtmgp_hip_event_destroy(void* event);
+int rrtmgp_hip_event_record(void* event, void* stream);
+int rrtmgp_hip_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Device-resident k-distribution, as prepared by the Gas_optics_rrtmgp_gpu constructor        */
+/* (reference: src_cuda/Gas_optics_rrtmgp.cu:779-802 uploads the same tables).  Real-valued    */
+/* pointers hold float or double matching the entry-point suffix.                              */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+    int64_t ngas, nflav, neta, npres, ntemp, ngpt, nbnd, nplanck;
+    int64_t idx_h2o;                       /* 1-based index of h2o in the gas list          */
+    double press_ref_log_delta, temp_ref_min, temp_ref_delta, press_ref_trop_log;
+    double totplnk_delta;
+    int64_t nminor_lower, nminor_upper;
+    const int* flavor;                     /* (2, nflav)                                    */
+    const int* gpoint_flavor;              /* (2, ngpt)                                     */
+    const int* band_lims_gpt;              /* (2, nbnd)                                     */
+    const void* press_ref_log;             /* (npres)                                       */
+    const void* temp_ref;                  /* (ntemp)                                       */
+    const void* vmr_ref;                   /* (2, ngas+1, ntemp), gas 0 = dry air           */
+    const void* kmajor;                    /* (ntemp, neta, npres+1, ngpt)                  */
+    const void* kminor_lower;              /* (ntemp, neta, ncontrib_lower)                 */
+    const void* kminor_upper;
+    const int* minor_limits_gpt_lower;     /* (2, nminor)                                   */
+    const int* minor_limits_gpt_upper;
+    const rrtmgp_bool* minor_scales_with_density_lower;
+    const rrtmgp_bool* minor_scales_with_density_upper;
+    const rrtmgp_bool* scale_by_complement_lower;
+    const rrtmgp_bool* scale_by_complement_upper;
+    const int* idx_minor_lower;            /* 1-based gas index                             */
+    const int* idx_minor_upper;
+    const int* idx_minor_scaling_lower;    /* 1-based gas index, <= 0 for none              */
+    const int* idx_minor_scaling_upper;
+    const int* kminor_start_lower;         /* 1-based contributor offset                    */
+    const int* kminor_start_upper;
+    const int* minor_band_start_lower;     /* (nbnd+1) offsets into minor_band_list_lower   */
+    const int* minor_band_list_lower;      /* 0-based minor intervals touching each band     */
+    const int* minor_band_start_upper;
+    const int* minor_band_list_upper;
+    const void* totplnk;                   /* (nplanck, nbnd)          LW only              */
+    const void* planck_frac;               /* (ntemp, neta, npres+1, ngpt)  LW only         */
+    const void* krayl;                     /* (ntemp, neta, ngpt, 2)   SW only              */
+} rrtmgp_kdist_dev;
+
+/* Volume mixing ratios in the k-distribution's gas order; dim1 is 1 or ncol, dim2 is 1 or   */
+/* nlay (the broadcast rules of Gas_optics_rrtmgp::compute_gas_taus,                          */
+/* src/Gas_optics_rrtmgp.cpp:1121-1154).                                                     */
+typedef struct
+{
+    int64_t ngas;
+    const void* vmr[RRTMGP_MAX_GAS];
+    int64_t dim1[RRTMGP_MAX_GAS];
+    int64_t dim2[RRTMGP_MAX_GAS];
+} rrtmgp_gas_vmr;
+
+#define RRTMGP_DECLARE_BOTH(DECL) DECL(f32, float) DECL(f64, double)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Gas optics                                                                                  */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Fused gas optics, longwave: col_gas fill + interpolation + major/minor absorption + Planck  */
+/* sources in one pass, optionally adding band-resolved cloud absorption (add_to of a 1scl).   */
+/* Replaces Gas_optics_rrtmgp_gpu::gas_optics LW (src_cuda/Gas_optics_rrtmgp.cu:907-943):     */
+/* fill_gases_kernel, interpolation, zero_array, compute_tau_absorption, compute_planck_source */
+/* (+ inc_1scalar_by_1scalar_bybnd when cloud_tau != NULL).  sfc_lay is 1-based               */
+/* (src/Gas_optics_rrtmgp.cpp:1333).                                                          */
+#define RRTMGP_DECL_GAS_OPTICS_LW(S, T)                                                        \
+    int rrtmgp_hip_gas_optics_lw_##S(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay,  \
+        const T* play, const T* tlay, const T* tlev, const T* tsfc, const T* col_dry,          \
+        const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau,                          \
+        T* tau, T* lay_src, T* lev_src_inc, T* lev_src_dec, T* sfc_src, T* sfc_src_jac,        \
+        void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_LW)
+
+/* Fused gas optics, shortwave: absorption + Rayleigh + combine_abs_and_rayleigh (CPU        */
+/* semantics: ssa = tau_ray/tau if tau > 2*epsilon, src/Gas_optics_rrtmgp.cpp:365-385), g = 0, */
+/* optionally followed by inc_2stream_by_2stream_bybnd of band cloud properties.  Replaces     */
+/* Gas_optics_rrtmgp_gpu::gas_optics SW (src_cuda/Gas_optics_rrtmgp.cu:947-977).             */
+#define RRTMGP_DECL_GAS_OPTICS_SW(S, T)                                                        \
+    int rrtmgp_hip_gas_optics_sw_##S(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay,  \
+        const T* play, const T* tlay, const T* col_dry, const rrtmgp_gas_vmr* gases,           \
+        const T* cloud_tau, const T* cloud_ssa, const T* cloud_g,                              \
+        T* tau, T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_SW)
+
+/* Gas_optics_rrtmgp_kernels_cuda::interpolation (include_kernels_cuda/                       */
+/* gas_optics_rrtmgp_kernels_cuda.h:42-61; Fortran rrtmgp_interpolation, rrtmgp_kernels.h:60-79)*/
+/* Materializes jtemp, jpress, tropo (ncol,nlay), jeta (2,ncol,nlay,nflav), col_mix (2,...),  */
+/* fminor (2,2,...), fmajor (2,2,2,...) from col_gas (ncol,nlay,ngas+1; gas 0 = col_dry).     */
+#define RRTMGP_DECL_INTERPOLATION(S, T)                                                        \
+    int rrtmgp_hip_interpolation_##S(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay,  \
+        const T* play, const T* tlay, const T* col_gas, int* jtemp, T* fmajor, T* fminor,      \
+        T* col_mix, rrtmgp_bool* tropo, int* jeta, int* jpress, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_INTERPOLATION)
+
+/* Gas_optics_rrtmgp_kernels_cuda::compute_tau_absorption (gas_optics_rrtmgp_kernels_cuda.h:  */
+/* 81-115; Fortran rrtmgp_compute_tau_absorption, rrtmgp_kernels.h:81-108): tau += major +   */
+/* minor (lower set where tropo, upper set elsewhere) from materialized interpolation.         */
+#define RRTMGP_DECL_TAU_ABS(S, T)                                                              \
+    int rrtmgp_hip_compute_tau_absorption_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,       \
+        int64_t nlay, const rrtmgp_bool* tropo, const T* col_mix, const T* fmajor,             \
+        const T* fminor, const T* play, const T* tlay, const T* col_gas, const int* jeta,      \
+        const int* jtemp, const int* jpress, T* tau, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_TAU_ABS)
+
+/* Gas_optics_rrtmgp_kernels_cuda::compute_tau_rayleigh (gas_optics_rrtmgp_kernels_cuda.h:   */
+/* 68-79; Fortran rrtmgp_compute_tau_rayleigh, rrtmgp_kernels.h:129-138).                     */
+#define RRTMGP_DECL_TAU_RAY(S, T)                                                              \
+    int rrtmgp_hip_compute_tau_rayleigh_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,         \
+        int64_t nlay, const T* col_dry, const T* col_gas, const T* fminor, const int* jeta,    \
+        const rrtmgp_bool* tropo, const int* jtemp, T* tau_rayleigh, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_TAU_RAY)
+
+/* Gas_optics_rrtmgp_kernels_cuda::combine_abs_and_rayleigh (gas_optics_rrtmgp_kernels_cuda.h:*/
+/* 63-66), with the CPU threshold 2*epsilon (src/Gas_optics_rrtmgp.cpp:365-385).              */
+#define RRTMGP_DECL_COMBINE(S, T)                                                              \
+    int rrtmgp_hip_combine_abs_and_rayleigh_##S(int64_t ncol, int64_t nlay, int64_t ngpt,     \
+        const T* tau_abs, const T* tau_rayleigh, T* tau, T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_COMBINE)
+
+/* Gas_optics_rrtmgp_kernels_cuda::compute_planck_source (gas_optics_rrtmgp_kernels_cuda.h:   */
+/* 117-131; Fortran rrtmgp_compute_Planck_source, rrtmgp_kernels.h:119-127), delta_Tsurf = 1.  */
+#define RRTMGP_DECL_PLANCK(S, T)                                                               \
+    int rrtmgp_hip_compute_planck_source_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,        \
+        int64_t nlay, const T* tlay, const T* tlev, const T* tsfc, int sfc_lay,                \
+        const T* fmajor, const int* jeta, const rrtmgp_bool* tropo, const int* jtemp,          \
+        const int* jpress, T* sfc_src, T* lay_src, T* lev_src_inc, T* lev_src_dec,             \
+        T* sfc_src_jac, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_PLANCK)
+
+/* Gas_optics_rrtmgp_gpu::get_col_dry (src_cuda/Gas_optics_rrtmgp.cu:806-903;                */
+/* CPU src/Gas_optics_rrtmgp.cpp:764-792).                                                     */
+#define RRTMGP_DECL_COL_DRY(S, T)                                                              \
+    int rrtmgp_hip_get_col_dry_##S(int64_t ncol, int64_t nlay, const T* vmr_h2o,             \
+        int64_t h2o_dim1, int64_t h2o_dim2, const T* plev, T* col_dry, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_COL_DRY)
+
+/* fill_gases_kernel (src_cuda/Gas_optics_rrtmgp.cu:392-422): col_gas(ncol,nlay,ngas+1) with  */
+/* gas 0 = col_dry and gas i = vmr_i * col_dry.                                               */
+#define RRTMGP_DECL_FILL_GASES(S, T)                                                           \
+    int rrtmgp_hip_fill_col_gas_##S(int64_t ncol, int64_t nlay, const rrtmgp_gas_vmr* gases,  \
+        const T* col_dry, T* col_gas, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_FILL_GASES)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Cloud optics (Cloud_optics_gpu::cloud_optics, src_cuda/Cloud_optics.cu:181-329; CPU       */
+/* src/Cloud_optics.cpp:72-232).  LUTs are (nsize, nbnd) device arrays (ice: the roughness    */
+/* category the host class selected).  two_stream = 0 writes the 1scl absorption tau only;    */
+/* delta_scale != 0 applies delta_scale_2str_k to the 2str result in the same pass.           */
+/* ------------------------------------------------------------------------------------------ */
+#define RRTMGP_DECL_CLOUD(S, T)                                                                \
+    int rrtmgp_hip_cloud_optics_##S(int64_t ncol, int64_t nlay, int64_t nbnd,                 \
+        int64_t liq_nsteps, T liq_step_size, T radliq_lwr, const T* lut_extliq,                \
+        const T* lut_ssaliq, const T* lut_asyliq, int64_t ice_nsteps, T ice_step_size,         \
+        T radice_lwr, const T* lut_extice, const T* lut_ssaice, const T* lut_asyice,           \
+        const T* clwp, const T* ciwp, const T* reliq, const T* reice, int two_stream,          \
+        int delta_scale, T* tau, T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_CLOUD)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Optical-property arithmetic (Optical_props_kernels_cuda, include_kernels_cuda/             */
+/* optical_props_kernels_cuda.h:31-55; Fortran rte_increment_*, rte_inc_*_bybnd,              */
+/* rte_delta_scale_2str_k, rrtmgp_kernels.h:177-199).  eps = 3*tiny.  ssa/g NULL = 1scl.      */
+/* ------------------------------------------------------------------------------------------ */
+#define RRTMGP_DECL_OPTPROPS(S, T)                                                             \
+    int rrtmgp_hip_increment_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau_inout,      \
+        T* ssa_inout, T* g_inout, const T* tau_in, const T* ssa_in, const T* g_in,             \
+        void* stream);                                                                         \
+    int rrtmgp_hip_inc_bybnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau_inout,      \
+        T* ssa_inout, T* g_inout, const T* tau_in, const T* ssa_in, const T* g_in,             \
+        int64_t nbnd, const int* band_lims_gpt, void* stream);                                 \
+    int rrtmgp_hip_delta_scale_2str_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau,     \
+        T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_OPTPROPS)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Solvers (Rte_solver_kernels_cuda, include_kernels_cuda/rte_solver_kernels_cuda.h:31-65;    */
+/* Fortran rte_lw_solver_noscat / rte_sw_solver_2stream / apply_BC_*, rrtmgp_kernels.h:     */
+/* 140-175).                                                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* apply_BC: flux_dn(top) = 0 | inc_flux | inc_flux * factor(icol).  inc_flux (ncol, ngpt).  */
+#define RRTMGP_DECL_APPLY_BC(S, T)                                                             \
+    int rrtmgp_hip_apply_bc_##S(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,       \
+        const T* inc_flux, const T* factor, T* flux_dn, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_APPLY_BC)
+
+/* expand_and_transpose (src_cuda/Rte_lw.cu:36-56): (nbnd, ncol) -> (ncol, ngpt).             */
+#define RRTMGP_DECL_EXPAND(S, T)                                                               \
+    int rrtmgp_hip_expand_and_transpose_##S(int64_t ncol, int64_t nbnd, int64_t ngpt,         \
+        const int* band_lims_gpt, const T* arr_in, T* arr_out, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_EXPAND)
+
+/* lw_solver_noscat with Gaussian quadrature (rte_solver_kernels_cuda.h:47-58; CPU           */
+/* src_kernels/mo_rte_solver_kernels.F90:58-376).  secants (ncol, ngpt, nmus), weights (nmus) */
+/* are device arrays; sfc_emis, sfc_src, inc_flux (NULL = 0) are (ncol, ngpt).  With          */
+/* do_broadband the spectral fluxes are not returned (flux_up/flux_dn may be NULL) and         */
+/* broadband_up/dn (ncol, nlev) hold the g-point sums.  do_jacobians adds the broadband       */
+/* surface-temperature Jacobian of flux_up into flux_up_jac (ncol, nlev).                      */
+#define RRTMGP_DECL_LW_SOLVER(S, T)                                                            \
+    int rrtmgp_hip_lw_solver_noscat_##S(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,\
+        int64_t nmus, const T* secants, const T* weights, const T* tau, const T* lay_source,   \
+        const T* lev_source_inc, const T* lev_source_dec, const T* sfc_emis,                   \
+        const T* sfc_src, const T* inc_flux, T* flux_up, T* flux_dn, int do_broadband,         \
+        T* broadband_up, T* broadband_dn, int do_jacobians, const T* sfc_src_jac,              \
+        T* flux_up_jac, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_SOLVER)
+
+/* sw_solver_2stream (rte_solver_kernels_cuda.h:36-44; CPU F90:512-605, 996-1227).  mu0      */
+/* (ncol); sfc_alb_dir/dif, inc_flux_dir, inc_flux_dif (ncol, ngpt).  flux_dn is total        */
+/* (diffuse + direct).  do_broadband as for the LW solver.                                     */
+#define RRTMGP_DECL_SW_SOLVER(S, T)                                                            \
+    int rrtmgp_hip_sw_solver_2stream_##S(int64_t ncol, int64_t nlay, int64_t ngpt,            \
+        int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,                    \
+        const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, T* flux_up,         \
+        T* flux_dn, T* flux_dir, int has_dif_bc, const T* inc_flux_dif, int do_broadband,      \
+        T* broadband_up, T* broadband_dn, T* broadband_dir, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Flux reductions (Fluxes_kernels_cuda, include_kernels_cuda/fluxes_kernels_cuda.h:31-50;    */
+/* Fortran rte_sum_broadband / rte_net_broadband_precalc / sum_byband / net_byband_precalc,   */
+/* rrtmgp_kernels.h:34-52).  By-band sums use the CPU semantics (1-based inclusive band_lims, */
+/* mo_fluxes_byband_kernels.F90:22-79).                                                       */
+/* ------------------------------------------------------------------------------------------ */
+#define RRTMGP_DECL_FLUXES(S, T)                                                               \
+    int rrtmgp_hip_sum_broadband_##S(int64_t ncol, int64_t nlev, int64_t ngpt,                \
+        const T* gpt_flux, T* flux, void* stream);                                             \
+    int rrtmgp_hip_net_broadband_##S(int64_t ncol, int64_t nlev, const T* flux_dn,            \
+        const T* flux_up, T* flux_net, void* stream);                                          \
+    int rrtmgp_hip_sum_byband_##S(int64_t ncol, int64_t nlev, int64_t ngpt, int64_t nbnd,     \
+        const int* band_lims, const T* gpt_flux, T* bnd_flux, void* stream);                   \
+    int rrtmgp_hip_net_byband_full_##S(int64_t ncol, int64_t nlev, int64_t ngpt,              \
+        int64_t nbnd, const int* band_lims, const T* gpt_flux_dn, const T* gpt_flux_up,        \
+        T* bnd_flux_net, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_FLUXES)
+
+/* Heating rate, no reference counterpart: hr(ncol, nlay) [K/day] =                          */
+/* g0/cp * (net(k) - net(k+1)) / (p(k+1) - p(k)) from flux_net = dn - up (ncol, nlev); the    */
+/* flux and pressure differences are taken in double precision for both suffixes.             */
+#define RRTMGP_DECL_HEATING(S, T)                                                              \
+    int rrtmgp_hip_heating_rate_##S(int64_t ncol, int64_t nlay, const T* flux_up,             \
+        const T* flux_dn, const T* plev, T* heating_rate, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_HEATING)
+
+/* spread_col_kernel (src_cuda/Gas_optics_rrtmgp.cu:40-68): dst(icol, igpt) = src(igpt), and  */
+/* the per-column scaling of the driver (toa_src * tsi_scaling, src_test/Radiation_solver.cpp: */
+/* 739-743): arr(icol, j) *= factor(icol) for j < n2.                                          */
+#define RRTMGP_DECL_SPREAD(S, T)                                                               \
+    int rrtmgp_hip_spread_col_##S(int64_t ncol, int64_t ngpt, const T* src, T* dst,           \
+        void* stream);                                                                         \
+    int rrtmgp_hip_scale_by_col_##S(int64_t ncol, int64_t n2, T* arr, const T* factor,        \
+        void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SPREAD)
+
+/* Column-block copy (the subset/get_from_subset kernels of the reference driver,             */
+/* src_kernels_cuda/subset_kernels.cu:1-98, include/Array.h:296-335): for nrows rows of       */
+/* (ncol, rows) arrays, dst(dst_col0 + i, r) = src(src_col0 + i, r) for i < ncopy.            */
+#define RRTMGP_DECL_COPY_COLS(S, T)                                                            \
+    int rrtmgp_hip_copy_cols_##S(const T* src, int64_t src_ncol, int64_t src_col0, T* dst,    \
+        int64_t dst_ncol, int64_t dst_col0, int64_t ncopy, int64_t nrows, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_COPY_COLS)
+
+/* Input validation of Gas_optics_rrtmgp::gas_optics (src/Gas_optics_rrtmgp.cpp:813-826):     */
+/* count of values outside [lo, hi] (or below lo when hi is NaN), written to *count (host).    */
+/* Synchronizes the stream.                                                                    */
+#define RRTMGP_DECL_COUNT(S, T)                                                                \
+    int rrtmgp_hip_count_outside_##S(int64_t n, const T* arr, T lo, T hi, int64_t* count,      \
+        void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_COUNT)
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,91 @@
+// Shared device-side helpers of the RRTMGP HIP kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <limits>
+#include <string>
+#include <type_traits>
+
+#include "rrtmgp_hip.h"
+
+namespace rrtmgp_hip
+{
+using Bool = signed char;   // RTE_USE_CBOOL semantics of the reference (include/types.h:13-17)
+
+// Error state shared by all entry points (thread-local message, status codes of rrtmgp_hip.h).
+void set_error(const std::string& msg);
+int check_hip(hipError_t e, const char* what);
+
+#define RRTMGP_TRY(expr)                                                        \
+    do { const int rc_ = (expr); if (rc_ != RRTMGP_OK) return rc_; } while (0)
+
+#define RRTMGP_HIP(call) RRTMGP_TRY(::rrtmgp_hip::check_hip((call), #call))
+
+// Launch-status check: launch errors surface as hipGetLastError.
+#define RRTMGP_LAUNCHED(name) RRTMGP_TRY(::rrtmgp_hip::check_hip(hipGetLastError(), name))
+
+inline int arg_error(const std::string& msg)
+{
+    set_error(msg);
+    return RRTMGP_ERR_ARG;
+}
+
+inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+inline unsigned grid1(int64_t n, int block)
+{
+    return static_cast<unsigned>((n + block - 1) / block);
+}
+
+// Keeps memory freed by hipFreeAsync in the device's default pool instead of returning it to
+// the driver at every synchronization, so repeated solver calls do not re-map scratch.
+int retain_pool_memory();
+
+// Device allocation of scratch memory on a stream (stream-ordered allocator).
+template<typename T>
+int scratch_alloc(T** p, size_t n, hipStream_t s)
+{
+    if (n == 0) { *p = nullptr; return RRTMGP_OK; }
+    RRTMGP_TRY(retain_pool_memory());
+    return check_hip(hipMallocAsync(reinterpret_cast<void**>(p), n * sizeof(T), s), "hipMallocAsync");
+}
+
+template<typename T>
+void scratch_free(T* p, hipStream_t s)
+{
+    if (p) (void)hipFreeAsync(p, s);
+}
+
+// Precision-dependent constants.
+template<typename Real> struct Limits;
+template<> struct Limits<float>
+{
+    static constexpr float eps = FLT_EPSILON;
+    static constexpr float tiny = FLT_MIN;
+    static constexpr float k_min = 1.e-4f;     // two-stream k floor in single precision
+};
+template<> struct Limits<double>
+{
+    static constexpr double eps = DBL_EPSILON;
+    static constexpr double tiny = DBL_MIN;
+    static constexpr double k_min = 1.e-12;
+};
+
+// 8 XCDs with private L2 each: work-group i is dispatched to XCD i % 8.  Remap the flat
+// work-group id so that consecutive work units (which share k-table slices) land on one XCD.
+constexpr int n_xcd = 8;
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t flat_id, int64_t n_units)
+{
+    const int64_t per_xcd = (n_units + n_xcd - 1) / n_xcd;
+    return (flat_id % n_xcd) * per_xcd + flat_id / n_xcd;
+}
+
+inline unsigned xcd_grid(int64_t n_units)
+{
+    return static_cast<unsigned>(((n_units + n_xcd - 1) / n_xcd) * n_xcd);
+}
+}  // namespace rrtmgp_hip

@@ -1,0 +1,405 @@
+// Cloud optics, optical-property arithmetic, flux reductions and heating rates for gfx950.
+//
+// Element-wise kernels: one thread per element of the reference layout, columns fastest, so
+// every access is coalesced.  Arithmetic follows src/Cloud_optics.cpp:72-232,
+// src_kernels_cuda/optical_props_kernels.cu:30-160 (eps = 3*tiny, launchers :80,123,172),
+// src_kernels_cuda/fluxes_kernels.cu:26-62 and src_kernels/mo_fluxes_byband_kernels.F90:22-79.
+#include "common.h"
+
+namespace rrtmgp_hip
+{
+// Table lookup of compute_all_from_table (src/Cloud_optics.cpp:80-101) for one element.
+template<typename Real>
+__device__ __forceinline__ void cloud_lookup(Real cwp, Real re, int64_t nsteps, Real step, Real offset,
+                                             const Real* et, const Real* st, const Real* at, int64_t ibnd,
+                                             Real& tau, Real& taussa, Real& taussag)
+{
+    if (cwp > Real(0))
+    {
+        // Clamped below as well: an effective radius under the table's lower bound would
+        // otherwise index before the table (the reference reads out of bounds there).
+        const int index = max(1, min(int((re - offset) / step) + 1, int(nsteps) - 1));
+        const Real fint = (re - offset) / step - Real(index - 1);
+        const int64_t b = nsteps * ibnd;
+        const Real e0 = et[b + index - 1], s0 = st[b + index - 1], a0 = at[b + index - 1];
+        tau = cwp * (e0 + fint * (et[b + index] - e0));
+        taussa = tau * (s0 + fint * (st[b + index] - s0));
+        taussag = taussa * (a0 + fint * (at[b + index] - a0));
+    }
+    else
+    {
+        tau = Real(0); taussa = Real(0); taussag = Real(0);
+    }
+}
+
+template<typename Real>
+__global__ void cloud_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd,
+                                    int64_t liq_nsteps, Real liq_step, Real liq_lwr,
+                                    const Real* __restrict__ extl, const Real* __restrict__ ssal,
+                                    const Real* __restrict__ asyl, int64_t ice_nsteps, Real ice_step,
+                                    Real ice_lwr, const Real* __restrict__ exti, const Real* __restrict__ ssai,
+                                    const Real* __restrict__ asyi, const Real* __restrict__ clwp,
+                                    const Real* __restrict__ ciwp, const Real* __restrict__ rel,
+                                    const Real* __restrict__ rei, int two_stream, int delta_scale,
+                                    Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ g)
+{
+    const int64_t ncl = ncol * nlay;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncl) return;
+    const Real lw = clwp[i], iw = ciwp[i], rl = rel[i], ri = rei[i];
+    const Real eps = Limits<Real>::eps;
+    const Real op_eps = Real(3) * Limits<Real>::tiny;
+    for (int64_t b = 0; b < nbnd; ++b)
+    {
+        Real lt, ls, lg, it, is, ig;
+        cloud_lookup(lw, rl, liq_nsteps, liq_step, liq_lwr, extl, ssal, asyl, b, lt, ls, lg);
+        cloud_lookup(iw, ri, ice_nsteps, ice_step, ice_lwr, exti, ssai, asyi, b, it, is, ig);
+        const int64_t o = i + ncl * b;
+        if (two_stream)
+        {
+            // Cloud_optics::cloud_optics 2str (src/Cloud_optics.cpp:159-171).
+            Real t = lt + it;
+            const Real ts = ls + is, tsg = lg + ig;
+            Real s = ts / max(t, eps);
+            Real gg = tsg / max(ts, eps);
+            if (delta_scale)
+            {
+                // delta_scale_2str_k_kernel (optical_props_kernels.cu:139-160).
+                const Real f = gg * gg, wf = s * f;
+                t *= (Real(1) - wf);
+                s = (s - wf) / max(op_eps, Real(1) - wf);
+                gg = (gg - f) / max(op_eps, Real(1) - f);
+            }
+            tau[o] = t; ssa[o] = s; g[o] = gg;
+        }
+        else
+            tau[o] = (lt - ls) + (it - is);   // 1scl (:219-230)
+    }
+}
+
+template<typename Real>
+__global__ void increment_kernel(int64_t n, Real* __restrict__ tau1, Real* __restrict__ ssa1,
+                                 Real* __restrict__ g1, const Real* __restrict__ tau2,
+                                 const Real* __restrict__ ssa2, const Real* __restrict__ g2)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!ssa1) { tau1[i] = tau1[i] + tau2[i]; return; }
+    const Real eps = Real(3) * Limits<Real>::tiny;
+    const Real t1 = tau1[i], t2 = tau2[i], s1 = ssa1[i], s2 = ssa2[i];
+    const Real tau12 = t1 + t2;
+    const Real tauscat12 = (t1 * s1) + (t2 * s2);
+    g1[i] = ((t1 * s1 * g1[i]) + (t2 * s2 * g2[i])) / max(tauscat12, eps);
+    ssa1[i] = tauscat12 / max(eps, tau12);
+    tau1[i] = tau12;
+}
+
+// inc_*_bybnd: thread per (col, lay, gpt); the band of a g-point is found by scanning the
+// (few) band limits, as the reference kernel does (optical_props_kernels.cu:86-95).
+template<typename Real>
+__global__ void inc_bybnd_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, Real* __restrict__ tau1,
+                                 Real* __restrict__ ssa1, Real* __restrict__ g1,
+                                 const Real* __restrict__ tau2, const Real* __restrict__ ssa2,
+                                 const Real* __restrict__ g2, int64_t nbnd, const int* __restrict__ band_lims)
+{
+    const int64_t ncl = ncol * nlay;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncl * ngpt) return;
+    const int64_t cl = i % ncl;
+    const int gpt1 = int(i / ncl) + 1;
+    const Real eps = Real(3) * Limits<Real>::tiny;
+    for (int64_t b = 0; b < nbnd; ++b)
+    {
+        if (gpt1 >= band_lims[2 * b] && gpt1 <= band_lims[2 * b + 1])
+        {
+            const int64_t ib = cl + ncl * b;
+            if (!ssa1) { tau1[i] = tau1[i] + tau2[ib]; continue; }
+            const Real t1 = tau1[i], t2 = tau2[ib], s1 = ssa1[i], s2 = ssa2[ib];
+            const Real tau12 = t1 + t2;
+            const Real tauscat12 = (t1 * s1) + (t2 * s2);
+            g1[i] = ((t1 * s1 * g1[i]) + (t2 * s2 * g2[ib])) / max(tauscat12, eps);
+            ssa1[i] = tauscat12 / max(eps, tau12);
+            tau1[i] = tau12;
+        }
+    }
+}
+
+template<typename Real>
+__global__ void delta_scale_kernel(int64_t n, Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ g)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Real eps = Real(3) * Limits<Real>::tiny;
+    const Real gv = g[i], sv = ssa[i];
+    const Real f = gv * gv, wf = sv * f;
+    tau[i] *= (Real(1) - wf);
+    ssa[i] = (sv - wf) / max(eps, Real(1) - wf);
+    g[i] = (gv - f) / max(eps, Real(1) - f);
+}
+
+// Broadband sum over g-points in order (fluxes_kernels.cu:26-43).
+template<typename Real>
+__global__ void sum_broadband_kernel(int64_t ncv, int64_t ngpt, const Real* __restrict__ f, Real* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncv) return;
+    Real s = Real(0);
+    for (int64_t g = 0; g < ngpt; ++g) s += f[i + ncv * g];
+    out[i] = s;
+}
+
+template<typename Real>
+__global__ void net_kernel(int64_t n, const Real* __restrict__ dn, const Real* __restrict__ up, Real* __restrict__ net)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    net[i] = dn[i] - up[i];
+}
+
+// sum_byband (mo_fluxes_byband_kernels.F90:22-41).
+template<typename Real>
+__global__ void sum_byband_kernel(int64_t ncv, int64_t nbnd, const int* __restrict__ bl,
+                                  const Real* __restrict__ f, Real* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncv * nbnd) return;
+    const int64_t cv = i % ncv, b = i / ncv;
+    const int gs = bl[2 * b], ge = bl[2 * b + 1];
+    Real s = f[cv + ncv * (gs - 1)];
+    for (int g = gs; g < ge; ++g) s += f[cv + ncv * g];
+    out[i] = s;
+}
+
+// net_byband_full (mo_fluxes_byband_kernels.F90:46-69): (net + dn) - up in g-point order.
+template<typename Real>
+__global__ void net_byband_full_kernel(int64_t ncv, int64_t nbnd, const int* __restrict__ bl,
+                                       const Real* __restrict__ dn, const Real* __restrict__ up,
+                                       Real* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncv * nbnd) return;
+    const int64_t cv = i % ncv, b = i / ncv;
+    const int gs = bl[2 * b], ge = bl[2 * b + 1];
+    Real s = dn[cv + ncv * (gs - 1)] - up[cv + ncv * (gs - 1)];
+    for (int g = gs; g < ge; ++g) s = s + dn[cv + ncv * g] - up[cv + ncv * g];
+    out[i] = s;
+}
+
+// Heating rate with the net-flux and pressure differences in double precision.
+template<typename Real>
+__global__ void heating_rate_kernel(int64_t ncol, int64_t nlay, const Real* __restrict__ up,
+                                    const Real* __restrict__ dn, const Real* __restrict__ plev,
+                                    Real* __restrict__ hr)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * nlay) return;
+    const int64_t j = i + ncol;
+    const double net_a = double(dn[i]) - double(up[i]);
+    const double net_b = double(dn[j]) - double(up[j]);
+    const double g0 = 9.80665, cp = 1004.64, sec_per_day = 86400.0;
+    hr[i] = Real(g0 / cp * (net_a - net_b) / (double(plev[j]) - double(plev[i])) * sec_per_day);
+}
+
+// expand_and_transpose (src_cuda/Rte_lw.cu:36-56): (nbnd, ncol) -> (ncol, ngpt).
+template<typename Real>
+__global__ void expand_kernel(int64_t ncol, int64_t nbnd, int64_t ngpt, const int* __restrict__ bl,
+                              const Real* __restrict__ in, Real* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * ngpt) return;
+    const int64_t icol = i % ncol;
+    const int gpt1 = int(i / ncol) + 1;
+    for (int64_t b = 0; b < nbnd; ++b)
+        if (gpt1 >= bl[2 * b] && gpt1 <= bl[2 * b + 1]) out[i] = in[b + nbnd * icol];
+}
+
+// apply_BC variants (rte_solver_kernels.cu:342-379): 0, inc_flux, inc_flux*factor(icol).
+template<typename Real>
+__global__ void apply_bc_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
+                                const Real* __restrict__ inc, const Real* __restrict__ factor,
+                                Real* __restrict__ flux_dn)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * ngpt) return;
+    const int64_t icol = i % ncol, igpt = i / ncol;
+    const int64_t o = icol + (top_at_1 ? 0 : nlay * ncol) + igpt * ncol * (nlay + 1);
+    Real v = inc ? inc[i] : Real(0);
+    if (inc && factor) v = inc[i] * factor[icol];
+    flux_dn[o] = v;
+}
+template<typename Real>
+__global__ void spread_col_kernel(int64_t ncol, int64_t ngpt, const Real* __restrict__ src, Real* __restrict__ dst)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * ngpt) return;
+    dst[i] = src[i / ncol];
+}
+
+template<typename Real>
+__global__ void scale_by_col_kernel(int64_t ncol, int64_t n2, Real* __restrict__ arr, const Real* __restrict__ f)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * n2) return;
+    arr[i] *= f[i % ncol];
+}
+
+template<typename Real>
+__global__ void copy_cols_kernel(const Real* __restrict__ src, int64_t src_ncol, int64_t src_col0,
+                                 Real* __restrict__ dst, int64_t dst_ncol, int64_t dst_col0,
+                                 int64_t ncopy, int64_t nrows)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncopy * nrows) return;
+    const int64_t c = i % ncopy, r = i / ncopy;
+    dst[dst_col0 + c + dst_ncol * r] = src[src_col0 + c + src_ncol * r];
+}
+}  // namespace rrtmgp_hip
+
+using namespace rrtmgp_hip;
+
+#define RRTMGP_DEF_OPTPROPS(S, T)                                                                   \
+extern "C" int rrtmgp_hip_cloud_optics_##S(int64_t ncol, int64_t nlay, int64_t nbnd,              \
+    int64_t liq_nsteps, T liq_step_size, T radliq_lwr, const T* lut_extliq, const T* lut_ssaliq,    \
+    const T* lut_asyliq, int64_t ice_nsteps, T ice_step_size, T radice_lwr, const T* lut_extice,    \
+    const T* lut_ssaice, const T* lut_asyice, const T* clwp, const T* ciwp, const T* reliq,         \
+    const T* reice, int two_stream, int delta_scale, T* tau, T* ssa, T* g, void* stream)            \
+{                                                                                                    \
+    if (ncol <= 0 || nlay <= 0 || nbnd <= 0) return arg_error("cloud_optics: empty extents");      \
+    if (liq_nsteps < 2 || ice_nsteps < 2) return arg_error("cloud_optics: LUT needs >= 2 sizes");  \
+    if (two_stream && (!ssa || !g)) return arg_error("cloud_optics: ssa/g required for 2str");     \
+    const int64_t n = ncol * nlay;                                                                   \
+    hipLaunchKernelGGL(cloud_optics_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),\
+                       ncol, nlay, nbnd, liq_nsteps, liq_step_size, radliq_lwr, lut_extliq,         \
+                       lut_ssaliq, lut_asyliq, ice_nsteps, ice_step_size, radice_lwr, lut_extice,   \
+                       lut_ssaice, lut_asyice, clwp, ciwp, reliq, reice, two_stream, delta_scale,   \
+                       tau, ssa, g);                                                                \
+    RRTMGP_LAUNCHED("cloud_optics_kernel");                                                         \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_increment_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau_inout,   \
+    T* ssa_inout, T* g_inout, const T* tau_in, const T* ssa_in, const T* g_in, void* stream)        \
+{                                                                                                    \
+    const int64_t n = ncol * nlay * ngpt;                                                            \
+    hipLaunchKernelGGL(increment_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
+                       n, tau_inout, ssa_inout, g_inout, tau_in, ssa_in, g_in);                     \
+    RRTMGP_LAUNCHED("increment_kernel");                                                            \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_inc_bybnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau_inout,   \
+    T* ssa_inout, T* g_inout, const T* tau_in, const T* ssa_in, const T* g_in, int64_t nbnd,        \
+    const int* band_lims_gpt, void* stream)                                                          \
+{                                                                                                    \
+    const int64_t n = ncol * nlay * ngpt;                                                            \
+    hipLaunchKernelGGL(inc_bybnd_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
+                       ncol, nlay, ngpt, tau_inout, ssa_inout, g_inout, tau_in, ssa_in, g_in, nbnd, \
+                       band_lims_gpt);                                                               \
+    RRTMGP_LAUNCHED("inc_bybnd_kernel");                                                            \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_delta_scale_2str_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau,  \
+    T* ssa, T* g, void* stream)                                                                      \
+{                                                                                                    \
+    const int64_t n = ncol * nlay * ngpt;                                                            \
+    hipLaunchKernelGGL(delta_scale_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), \
+                       n, tau, ssa, g);                                                              \
+    RRTMGP_LAUNCHED("delta_scale_kernel");                                                          \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_sum_broadband_##S(int64_t ncol, int64_t nlev, int64_t ngpt,             \
+    const T* gpt_flux, T* flux, void* stream)                                                        \
+{                                                                                                    \
+    const int64_t n = ncol * nlev;                                                                   \
+    hipLaunchKernelGGL(sum_broadband_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                  \
+                       as_stream(stream), n, ngpt, gpt_flux, flux);                                 \
+    RRTMGP_LAUNCHED("sum_broadband_kernel");                                                        \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_net_broadband_##S(int64_t ncol, int64_t nlev, const T* flux_dn,         \
+    const T* flux_up, T* flux_net, void* stream)                                                     \
+{                                                                                                    \
+    const int64_t n = ncol * nlev;                                                                   \
+    hipLaunchKernelGGL(net_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), n,      \
+                       flux_dn, flux_up, flux_net);                                                  \
+    RRTMGP_LAUNCHED("net_kernel");                                                                  \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_sum_byband_##S(int64_t ncol, int64_t nlev, int64_t ngpt, int64_t nbnd,  \
+    const int* band_lims, const T* gpt_flux, T* bnd_flux, void* stream)                             \
+{                                                                                                    \
+    const int64_t n = ncol * nlev * nbnd;                                                            \
+    hipLaunchKernelGGL(sum_byband_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),  \
+                       ncol * nlev, nbnd, band_lims, gpt_flux, bnd_flux);                           \
+    RRTMGP_LAUNCHED("sum_byband_kernel");                                                           \
+    (void)ngpt;                                                                                      \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_net_byband_full_##S(int64_t ncol, int64_t nlev, int64_t ngpt,           \
+    int64_t nbnd, const int* band_lims, const T* gpt_flux_dn, const T* gpt_flux_up,                 \
+    T* bnd_flux_net, void* stream)                                                                   \
+{                                                                                                    \
+    const int64_t n = ncol * nlev * nbnd;                                                            \
+    hipLaunchKernelGGL(net_byband_full_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                \
+                       as_stream(stream), ncol * nlev, nbnd, band_lims, gpt_flux_dn, gpt_flux_up,   \
+                       bnd_flux_net);                                                                \
+    RRTMGP_LAUNCHED("net_byband_full_kernel");                                                      \
+    (void)ngpt;                                                                                      \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_heating_rate_##S(int64_t ncol, int64_t nlay, const T* flux_up,          \
+    const T* flux_dn, const T* plev, T* heating_rate, void* stream)                                 \
+{                                                                                                    \
+    const int64_t n = ncol * nlay;                                                                   \
+    hipLaunchKernelGGL(heating_rate_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                   \
+                       as_stream(stream), ncol, nlay, flux_up, flux_dn, plev, heating_rate);        \
+    RRTMGP_LAUNCHED("heating_rate_kernel");                                                         \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_expand_and_transpose_##S(int64_t ncol, int64_t nbnd, int64_t ngpt,      \
+    const int* band_lims_gpt, const T* arr_in, T* arr_out, void* stream)                            \
+{                                                                                                    \
+    const int64_t n = ncol * ngpt;                                                                   \
+    hipLaunchKernelGGL(expand_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),      \
+                       ncol, nbnd, ngpt, band_lims_gpt, arr_in, arr_out);                           \
+    RRTMGP_LAUNCHED("expand_kernel");                                                               \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_apply_bc_##S(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,    \
+    const T* inc_flux, const T* factor, T* flux_dn, void* stream)                                   \
+{                                                                                                    \
+    const int64_t n = ncol * ngpt;                                                                   \
+    hipLaunchKernelGGL(apply_bc_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),    \
+                       ncol, nlay, ngpt, top_at_1, inc_flux, factor, flux_dn);                      \
+    RRTMGP_LAUNCHED("apply_bc_kernel");                                                             \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_spread_col_##S(int64_t ncol, int64_t ngpt, const T* src, T* dst,      \
+    void* stream)                                                                                    \
+{                                                                                                    \
+    const int64_t n = ncol * ngpt;                                                                   \
+    hipLaunchKernelGGL(spread_col_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),  \
+                       ncol, ngpt, src, dst);                                                        \
+    RRTMGP_LAUNCHED("spread_col_kernel");                                                           \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_scale_by_col_##S(int64_t ncol, int64_t n2, T* arr, const T* factor,     \
+    void* stream)                                                                                    \
+{                                                                                                    \
+    const int64_t n = ncol * n2;                                                                     \
+    hipLaunchKernelGGL(scale_by_col_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                   \
+                       as_stream(stream), ncol, n2, arr, factor);                                   \
+    RRTMGP_LAUNCHED("scale_by_col_kernel");                                                         \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_copy_cols_##S(const T* src, int64_t src_ncol, int64_t src_col0, T* dst, \
+    int64_t dst_ncol, int64_t dst_col0, int64_t ncopy, int64_t nrows, void* stream)                \
+{                                                                                                    \
+    const int64_t n = ncopy * nrows;                                                                 \
+    if (n == 0) return RRTMGP_OK;                                                                    \
+    hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
+                       src, src_ncol, src_col0, dst, dst_ncol, dst_col0, ncopy, nrows);             \
+    RRTMGP_LAUNCHED("copy_cols_kernel");                                                            \
+    return RRTMGP_OK;                                                                                \
+}
+
+RRTMGP_DEF_OPTPROPS(f32, float)
+RRTMGP_DEF_OPTPROPS(f64, double)

@@ -1,0 +1,104 @@
+// Column-radiation drivers on the GPU (reference include_test/Radiation_solver.h:33-233,
+// src_test/Radiation_solver.cu:419-950).  The call sequence is the reference's
+// (gas optics -> cloud optics -> add_to -> rte -> flux reduction); the cloud add_to is folded
+// into the gas-optics pass, the SW cloud delta-scaling into the cloud-optics pass, and
+// broadband fluxes are reduced inside the solver (CPU semantics) unless band fluxes are asked
+// for.  Workspaces persist across calls; columns are processed in blocks of n_col_block.
+#ifndef RRTMGP_AMD_RADIATION_SOLVER_H
+#define RRTMGP_AMD_RADIATION_SOLVER_H
+
+#include <memory>
+
+#include "Array.h"
+#include "Cloud_optics.h"
+#include "Fluxes.h"
+#include "Gas_concs.h"
+#include "Gas_optics_rrtmgp.h"
+#include "Optical_props.h"
+#include "Source_functions.h"
+#include "types.h"
+
+class Radiation_solver_longwave_gpu
+{
+    public:
+        Radiation_solver_longwave_gpu(std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist,
+                                      std::shared_ptr<Cloud_optics_gpu> cloud_optics);
+
+        // Arrays are device arrays of the full domain: (ncol, nlay|nlev) profiles, t_sfc (ncol),
+        // emis_sfc (nbnd, ncol); col_dry may be empty (computed from h2o); cloud arrays are used
+        // when switch_cloud_optics.  Outputs (ncol, nlev[, nbnd]).
+        void solve_gpu(
+                const bool switch_cloud_optics, const bool switch_output_bnd_fluxes,
+                const Gas_concs_gpu& gas_concs,
+                const Array_gpu<Float,2>& p_lay, const Array_gpu<Float,2>& p_lev,
+                const Array_gpu<Float,2>& t_lay, const Array_gpu<Float,2>& t_lev,
+                const Array_gpu<Float,2>& col_dry, const Array_gpu<Float,1>& t_sfc,
+                const Array_gpu<Float,2>& emis_sfc,
+                const Array_gpu<Float,2>& lwp, const Array_gpu<Float,2>& iwp,
+                const Array_gpu<Float,2>& rel, const Array_gpu<Float,2>& rei,
+                Array_gpu<Float,2>& lw_flux_up, Array_gpu<Float,2>& lw_flux_dn,
+                Array_gpu<Float,2>& lw_flux_net,
+                Array_gpu<Float,3>& lw_bnd_flux_up, Array_gpu<Float,3>& lw_bnd_flux_dn,
+                Array_gpu<Float,3>& lw_bnd_flux_net);
+
+        void set_n_col_block(int n) { n_col_block = n; }
+        int get_n_gpt() const { return kdist->get_ngpt(); }
+        int get_n_bnd() const { return kdist->get_nband(); }
+        Gas_optics_rrtmgp_gpu& get_kdist() { return *kdist; }
+
+    private:
+        std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist;
+        std::shared_ptr<Cloud_optics_gpu> cloud_optics;
+        int n_col_block = 1 << 30;
+        int ws_ncol = -1, ws_nlay = -1, ws_bnd = -1;
+        std::unique_ptr<Optical_props_arry_gpu> optical_props;
+        std::unique_ptr<Optical_props_1scl_gpu> cloud_props;
+        std::unique_ptr<Source_func_lw_gpu> sources;
+        Array_gpu<Float,3> gpt_flux_up, gpt_flux_dn;
+        std::unique_ptr<Fluxes_byband_gpu> bnd_fluxes;
+        Array_gpu<Float,2> col_dry_ws;
+};
+
+class Radiation_solver_shortwave_gpu
+{
+    public:
+        Radiation_solver_shortwave_gpu(std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist,
+                                       std::shared_ptr<Cloud_optics_gpu> cloud_optics);
+
+        // sfc_alb_dir/dif (nbnd, ncol), tsi_scaling and mu0 (ncol).
+        void solve_gpu(
+                const bool switch_cloud_optics, const bool switch_output_bnd_fluxes,
+                const bool switch_delta_cloud,
+                const Gas_concs_gpu& gas_concs,
+                const Array_gpu<Float,2>& p_lay, const Array_gpu<Float,2>& p_lev,
+                const Array_gpu<Float,2>& t_lay, const Array_gpu<Float,2>& t_lev,
+                const Array_gpu<Float,2>& col_dry,
+                const Array_gpu<Float,2>& sfc_alb_dir, const Array_gpu<Float,2>& sfc_alb_dif,
+                const Array_gpu<Float,1>& tsi_scaling, const Array_gpu<Float,1>& mu0,
+                const Array_gpu<Float,2>& lwp, const Array_gpu<Float,2>& iwp,
+                const Array_gpu<Float,2>& rel, const Array_gpu<Float,2>& rei,
+                Array_gpu<Float,2>& sw_flux_up, Array_gpu<Float,2>& sw_flux_dn,
+                Array_gpu<Float,2>& sw_flux_dn_dir, Array_gpu<Float,2>& sw_flux_net,
+                Array_gpu<Float,3>& sw_bnd_flux_up, Array_gpu<Float,3>& sw_bnd_flux_dn,
+                Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net);
+
+        void set_n_col_block(int n) { n_col_block = n; }
+        int get_n_gpt() const { return kdist->get_ngpt(); }
+        int get_n_bnd() const { return kdist->get_nband(); }
+        Float get_tsi() const { return kdist->get_tsi(); }
+        Gas_optics_rrtmgp_gpu& get_kdist() { return *kdist; }
+
+    private:
+        std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist;
+        std::shared_ptr<Cloud_optics_gpu> cloud_optics;
+        int n_col_block = 1 << 30;
+        int ws_ncol = -1, ws_nlay = -1, ws_bnd = -1;
+        std::unique_ptr<Optical_props_arry_gpu> optical_props;
+        std::unique_ptr<Optical_props_2str_gpu> cloud_props;
+        Array_gpu<Float,2> toa_src;
+        Array_gpu<Float,3> gpt_flux_up, gpt_flux_dn, gpt_flux_dir;
+        std::unique_ptr<Fluxes_byband_gpu> bnd_fluxes;
+        Array_gpu<Float,2> col_dry_ws;
+};
+
+#endif

@@ -1,0 +1,34 @@
+// Longwave source-function holder (reference include/Source_functions.h:73-99).
+#ifndef RRTMGP_AMD_SOURCE_FUNCTIONS_H
+#define RRTMGP_AMD_SOURCE_FUNCTIONS_H
+
+#include "Array.h"
+#include "Optical_props.h"
+#include "types.h"
+
+class Source_func_lw_gpu : public Optical_props_gpu
+{
+    public:
+        Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& optical_props);
+
+        Array_gpu<Float,2>& get_sfc_source()     { return sfc_source;     }
+        Array_gpu<Float,2>& get_sfc_source_jac() { return sfc_source_jac; }
+        Array_gpu<Float,3>& get_lay_source()     { return lay_source;     }
+        Array_gpu<Float,3>& get_lev_source_inc() { return lev_source_inc; }
+        Array_gpu<Float,3>& get_lev_source_dec() { return lev_source_dec; }
+
+        const Array_gpu<Float,2>& get_sfc_source()     const { return sfc_source;     }
+        const Array_gpu<Float,2>& get_sfc_source_jac() const { return sfc_source_jac; }
+        const Array_gpu<Float,3>& get_lay_source()     const { return lay_source;     }
+        const Array_gpu<Float,3>& get_lev_source_inc() const { return lev_source_inc; }
+        const Array_gpu<Float,3>& get_lev_source_dec() const { return lev_source_dec; }
+
+    private:
+        Array_gpu<Float,2> sfc_source;
+        Array_gpu<Float,2> sfc_source_jac;
+        Array_gpu<Float,3> lay_source;
+        Array_gpu<Float,3> lev_source_inc;
+        Array_gpu<Float,3> lev_source_dec;
+};
+
+#endif

@@ -1,0 +1,76 @@
+// Longwave driver (reference src_cuda/Rte_lw.cu:81-152, CPU src/Rte_lw.cpp:90-162).
+#include "Rte_lw.h"
+
+#include "Optical_props.h"
+#include "Source_functions.h"
+#include "hip_dispatch.h"
+
+using rrtmgp_cpp::hip_check;
+using rrtmgp_cpp::stream;
+
+namespace
+{
+// Gaussian quadrature secants and weights for 1-4 angles (src/Rte_lw.cpp:100-113).
+const Float gauss_Ds[4][4] = {{1.66, 0., 0., 0.},
+                              {1.18350343, 2.81649655, 0., 0.},
+                              {1.09719858, 1.69338507, 4.70941630, 0.},
+                              {1.06056257, 1.38282560, 2.40148179, 7.15513024}};
+const Float gauss_wts[4][4] = {{0.5, 0., 0., 0.},
+                               {0.3180413817, 0.1819586183, 0., 0.},
+                               {0.2009319137, 0.2292411064, 0.0698269799, 0.},
+                               {0.1355069134, 0.2034645680, 0.1298475476, 0.0311809710}};
+}
+
+void Rte_lw_gpu::rte_lw(
+        const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
+        const Bool top_at_1,
+        const Source_func_lw_gpu& sources,
+        const Array_gpu<Float,2>& sfc_emis,
+        const Array_gpu<Float,2>& inc_flux,
+        Array_gpu<Float,3>& gpt_flux_up,
+        Array_gpu<Float,3>& gpt_flux_dn,
+        const int n_gauss_angles)
+{
+    const int ncol = optical_props->get_ncol();
+    const int nlay = optical_props->get_nlay();
+    const int ngpt = optical_props->get_ngpt();
+    if (n_gauss_angles < 1 || n_gauss_angles > 4)
+        throw std::runtime_error("n_gauss_angles must be between 1 and 4");
+
+    Array_gpu<Float,2> sfc_emis_gpt({ncol, ngpt});
+    expand_and_transpose(optical_props, sfc_emis, sfc_emis_gpt);
+
+    const int nmus = n_gauss_angles;
+    Array<Float,3> secants_h({ncol, ngpt, nmus});
+    Array<Float,1> weights_h({nmus});
+    for (int imu = 0; imu < nmus; ++imu)
+    {
+        weights_h({imu + 1}) = gauss_wts[nmus - 1][imu];
+        std::fill(secants_h.v().begin() + size_t(imu) * ncol * ngpt,
+                  secants_h.v().begin() + size_t(imu + 1) * ncol * ngpt, gauss_Ds[nmus - 1][imu]);
+    }
+    Array_gpu<Float,3> secants(secants_h);
+    Array_gpu<Float,1> weights(weights_h);
+
+    // CPU semantics: a (ncol, nlev, 1) output array selects broadband fluxes (src/Rte_lw.cpp:137).
+    const bool do_broadband = (gpt_flux_up.dim(3) == 1) && (ngpt > 1);
+    const Float* inc = (inc_flux.size() == 0) ? nullptr : inc_flux.ptr();
+    hip_check(K<Float>::lw_solver_noscat(
+            ncol, nlay, ngpt, top_at_1, nmus, secants.ptr(), weights.ptr(),
+            optical_props->get_tau().ptr(), sources.get_lay_source().ptr(),
+            sources.get_lev_source_inc().ptr(), sources.get_lev_source_dec().ptr(),
+            sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc,
+            do_broadband ? nullptr : gpt_flux_up.ptr(), do_broadband ? nullptr : gpt_flux_dn.ptr(),
+            do_broadband, do_broadband ? gpt_flux_up.ptr() : nullptr,
+            do_broadband ? gpt_flux_dn.ptr() : nullptr, 0, nullptr, nullptr, stream()));
+}
+
+void Rte_lw_gpu::expand_and_transpose(
+        const std::unique_ptr<Optical_props_arry_gpu>& ops,
+        const Array_gpu<Float,2> arr_in,
+        Array_gpu<Float,2>& arr_out)
+{
+    hip_check(K<Float>::expand_and_transpose(arr_in.dim(2), ops->get_nband(), ops->get_ngpt(),
+                                             ops->get_band_lims_gpoint_gpu().ptr(), arr_in.ptr(),
+                                             arr_out.ptr(), stream()));
+}

@@ -1,0 +1,52 @@
+// Shortwave driver (reference src_cuda/Rte_sw.cu:116-161, CPU src/Rte_sw.cpp:97-145).
+#include "Rte_sw.h"
+
+#include "Optical_props.h"
+#include "hip_dispatch.h"
+
+using rrtmgp_cpp::hip_check;
+using rrtmgp_cpp::stream;
+
+void Rte_sw_gpu::rte_sw(
+        const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
+        const Bool top_at_1,
+        const Array_gpu<Float,1>& mu0,
+        const Array_gpu<Float,2>& inc_flux_dir,
+        const Array_gpu<Float,2>& sfc_alb_dir,
+        const Array_gpu<Float,2>& sfc_alb_dif,
+        const Array_gpu<Float,2>& inc_flux_dif,
+        Array_gpu<Float,3>& gpt_flux_up,
+        Array_gpu<Float,3>& gpt_flux_dn,
+        Array_gpu<Float,3>& gpt_flux_dir)
+{
+    const int ncol = optical_props->get_ncol();
+    const int nlay = optical_props->get_nlay();
+    const int ngpt = optical_props->get_ngpt();
+
+    Array_gpu<Float,2> sfc_alb_dir_gpt({ncol, ngpt});
+    Array_gpu<Float,2> sfc_alb_dif_gpt({ncol, ngpt});
+    expand_and_transpose(optical_props, sfc_alb_dir, sfc_alb_dir_gpt);
+    expand_and_transpose(optical_props, sfc_alb_dif, sfc_alb_dif_gpt);
+
+    const bool has_dif_bc = inc_flux_dif.size() > 0;
+    const bool do_broadband = (gpt_flux_up.dim(3) == 1) && (ngpt > 1);
+    hip_check(K<Float>::sw_solver_2stream(
+            ncol, nlay, ngpt, top_at_1, optical_props->get_tau().ptr(), optical_props->get_ssa().ptr(),
+            optical_props->get_g().ptr(), mu0.ptr(), sfc_alb_dir_gpt.ptr(), sfc_alb_dif_gpt.ptr(),
+            inc_flux_dir.ptr(),
+            do_broadband ? nullptr : gpt_flux_up.ptr(), do_broadband ? nullptr : gpt_flux_dn.ptr(),
+            do_broadband ? nullptr : gpt_flux_dir.ptr(), has_dif_bc,
+            has_dif_bc ? inc_flux_dif.ptr() : nullptr, do_broadband,
+            do_broadband ? gpt_flux_up.ptr() : nullptr, do_broadband ? gpt_flux_dn.ptr() : nullptr,
+            do_broadband ? gpt_flux_dir.ptr() : nullptr, stream()));
+}
+
+void Rte_sw_gpu::expand_and_transpose(
+        const std::unique_ptr<Optical_props_arry_gpu>& ops,
+        const Array_gpu<Float,2> arr_in,
+        Array_gpu<Float,2>& arr_out)
+{
+    hip_check(K<Float>::expand_and_transpose(arr_in.dim(2), ops->get_nband(), ops->get_ngpt(),
+                                             ops->get_band_lims_gpoint_gpu().ptr(), arr_in.ptr(),
+                                             arr_out.ptr(), stream()));
+}

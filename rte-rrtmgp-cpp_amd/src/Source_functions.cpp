@@ -1,0 +1,11 @@
+// Longwave source holder (reference src_cuda/Source_functions.cu:30-40).
+#include "Source_functions.h"
+
+Source_func_lw_gpu::Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& p) :
+    Optical_props_gpu(p),
+    sfc_source({n_col, p.get_ngpt()}),
+    sfc_source_jac({n_col, p.get_ngpt()}),
+    lay_source({n_col, n_lay, p.get_ngpt()}),
+    lev_source_inc({n_col, n_lay, p.get_ngpt()}),
+    lev_source_dec({n_col, n_lay, p.get_ngpt()})
+{}

@@ -1,0 +1,245 @@
+"""GPU parity: the HIP path (through the class-API facade and the thin C ABI) against the FP64
+CPU oracle on the same seeded inputs.
+
+Bars: interpolation indices and flags bit-exact (FP64 build); fluxes and heating rates within
+1e-4 relative in FP64 (the implementation reaches ~1e-12); FP32 fluxes within 0.05 W m-2 of
+the FP64 oracle where stated.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import AVAILABLE, rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def G():
+    from rrtmgp_amd import api, device
+    if device.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return api
+
+
+@pytest.fixture(scope="module")
+def gkd(G, kd_lw, kd_sw):
+    return {(lw, p): G.KDist(kd_lw if lw else kd_sw, AVAILABLE, precision=p)
+            for lw in (True, False) for p in (8, 4)}
+
+
+def _host(d):
+    return d.to_host().astype(np.float64)
+
+
+def test_init_tables_match_oracle(gkd, okd_lw, okd_sw):
+    for lw, okd in ((True, okd_lw), (False, okd_sw)):
+        k = gkd[(lw, 8)]
+        assert k.gas_names == okd.gas_names and k.nflav == okd.nflav
+        for name in ("flavor", "gpoint_flavor", "minor_limits_gpt_lower", "minor_limits_gpt_upper",
+                     "kminor_start_lower", "kminor_start_upper", "idx_minor_lower", "idx_minor_upper",
+                     "idx_minor_scaling_lower", "idx_minor_scaling_upper"):
+            assert np.array_equal(k.get_int(name), okd.get_int(name)), name
+        assert np.array_equal(k.get_real("kmajor"), okd.get_real("kmajor"))
+        assert np.array_equal(k.get_real("kminor_lower"), okd.get_real("kminor_lower"))
+
+
+def test_init_minor_reduction_matches_oracle(G, O, kd_lw):
+    avail = [g for g in AVAILABLE if g != "ch4"]
+    k, o = G.KDist(kd_lw, avail), O.Kdist(kd_lw, avail)
+    for name in ("minor_limits_gpt_lower", "kminor_start_lower", "idx_minor_lower"):
+        assert np.array_equal(k.get_int(name), o.get_int(name))
+    assert np.array_equal(k.get_real("kminor_lower"), o.get_real("kminor_lower"))
+    with pytest.raises(RuntimeError, match="required gas co2 is missing"):
+        G.KDist(kd_lw, ["h2o", "o3", "n2o", "ch4"])
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_interpolation_indices_bit_exact(G, gkd, okd_lw, syn, O, top_at_1):
+    atm = syn.make_atmosphere(syn.SEED_ATMOS[0], 128, 60, top_at_1=top_at_1)
+    k = gkd[(True, 8)]
+    ncol, nlay = atm.p_lay.shape
+    col_dry = O.get_col_dry(atm.vmr["h2o"], atm.p_lev)
+    ref = O.gas_optics(okd_lw, atm, col_dry=col_dry, interp=True)
+    gc = G.GasConcs(atm.vmr)
+    dev = G.DeviceArray
+    cd = dev.from_host(col_dry)
+    col_gas = dev((ncol, nlay, k.ngas + 1))
+    k._check(k.L.rrtmgp_api_kdist_fill_col_gas(k.h, ncol, nlay, gc.h, cd.ptr, col_gas.ptr))
+    nf = k.nflav
+    out = {"jtemp": dev((ncol, nlay), np.int32), "jpress": dev((ncol, nlay), np.int32),
+           "tropo": dev((ncol, nlay), np.int8), "jeta": dev((2, ncol, nlay, nf), np.int32),
+           "col_mix": dev((2, ncol, nlay, nf)), "fmajor": dev((2, 2, 2, ncol, nlay, nf)),
+           "fminor": dev((2, 2, ncol, nlay, nf))}
+    play, tlay = dev.from_host(atm.p_lay), dev.from_host(atm.t_lay)
+    k._check(k.L.rrtmgp_api_kdist_interpolation(
+        k.h, ncol, nlay, play.ptr, tlay.ptr, col_gas.ptr, out["jtemp"].ptr, out["fmajor"].ptr,
+        out["fminor"].ptr, out["col_mix"].ptr, out["tropo"].ptr, out["jeta"].ptr, out["jpress"].ptr))
+    got = {n: d.to_host() for n, d in out.items()}
+    for n in ("jtemp", "jpress", "tropo", "jeta"):
+        assert np.array_equal(got[n], ref[n]), n
+    # Weights are produced without FMA contraction: identical to the contraction-free oracle.
+    for n in ("col_mix", "fmajor", "fminor"):
+        assert np.array_equal(got[n], ref[n]), n
+
+
+def _gas_optics_both(G, gkd, okd, atm, lw, precision, O):
+    col_dry = O.get_col_dry(atm.vmr["h2o"], atm.p_lev)
+    ref = O.gas_optics(okd, atm, col_dry=col_dry)
+    k = gkd[(lw, precision)]
+    got = G.gas_optics(k, atm, G.GasConcs(atm.vmr, precision), col_dry)
+    return ref, {n: _host(d) for n, d in got.items()}
+
+
+@pytest.mark.parametrize("lw", [True, False])
+def test_gas_optics_fp64(G, gkd, okd_lw, okd_sw, atm_c0, O, lw):
+    ref, got = _gas_optics_both(G, gkd, okd_lw if lw else okd_sw, atm_c0, lw, 8, O)
+    for n in ref:
+        assert rel_err(got[n], ref[n], floor=1e-300) < FP64_RTOL, n
+
+
+@pytest.mark.parametrize("lw", [True, False])
+def test_gas_optics_fp32(G, gkd, okd_lw, okd_sw, atm_c0, O, lw):
+    ref, got = _gas_optics_both(G, gkd, okd_lw if lw else okd_sw, atm_c0, lw, 4, O)
+    for n in ref:
+        scale = np.abs(ref[n]).max()
+        err = np.abs(got[n] - ref[n]) / np.maximum(np.abs(ref[n]), 1e-6 * scale)
+        assert err.max() < 2e-4, (n, err.max())
+
+
+@pytest.mark.parametrize("two_stream,delta", [(False, False), (True, False), (True, True)])
+def test_cloud_optics(G, O, lut_sw, atm_cloudy, two_stream, delta):
+    cl_o, cl_g = O.Cloud(lut_sw), G.Cloud(lut_sw)
+    a = atm_cloudy
+    ref = cl_o.optics(a.lwp, a.iwp, a.rel, a.rei, two_stream)
+    got = cl_g.optics(a.lwp, a.iwp, a.rel, a.rei, two_stream, delta_scale=delta)
+    if two_stream:
+        if delta:
+            ref = O.delta_scale(*ref)
+        for r, g in zip(ref, got):
+            assert rel_err(_host(g), r, floor=1e-300) < 1e-13
+    else:
+        assert rel_err(_host(got), ref, floor=1e-300) < 1e-13
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+@pytest.mark.parametrize("nmu", [1, 3])
+def test_rte_lw_fp64(G, gkd, okd_lw, syn, O, top_at_1, nmu):
+    atm = syn.make_atmosphere(31, 40, 60, top_at_1=top_at_1)
+    go = O.gas_optics(okd_lw, atm)
+    k = gkd[(True, 8)]
+    dev = G.DeviceArray.from_host
+    args = [dev(go[n]) for n in ("tau", "lay_src", "lev_src_inc", "lev_src_dec", "sfc_src")]
+    inc = np.random.default_rng(1).uniform(0, 2, (atm.ncol, k.ngpt))
+    up, dn = G.rte_lw(k, top_at_1, *args, atm.emis_sfc, inc_flux=inc, n_gauss_angles=nmu)
+    rup, rdn = O.rte_lw(go["tau"], go["lay_src"], go["lev_src_inc"], go["lev_src_dec"], go["sfc_src"],
+                        atm.emis_sfc, okd_lw.band_lims, top_at_1, inc_flux=inc, n_gauss_angles=nmu)
+    assert rel_err(_host(up), rup) < FP64_RTOL and rel_err(_host(dn), rdn, floor=1e-6) < FP64_RTOL
+    bup, bdn = G.rte_lw(k, top_at_1, *args, atm.emis_sfc, inc_flux=inc, n_gauss_angles=nmu, do_broadband=True)
+    assert rel_err(_host(bup), rup.sum(2)) < FP64_RTOL and rel_err(_host(bdn), rdn.sum(2), floor=1e-6) < FP64_RTOL
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_rte_sw_fp64(G, gkd, okd_sw, syn, O, lut_sw, top_at_1):
+    atm = syn.make_atmosphere(32, 40, 60, cloudy=True, top_at_1=top_at_1)
+    go = O.gas_optics(okd_sw, atm)
+    ct, cs, cg = O.delta_scale(*O.Cloud(lut_sw).optics(atm.lwp, atm.iwp, atm.rel, atm.rei, True))
+    tau, ssa, g = O.inc_bybnd(go["tau"], ct, okd_sw.band_lims, go["ssa"], go["g"], cs, cg)
+    k = gkd[(False, 8)]
+    dev = G.DeviceArray.from_host
+    up, dn, dr = G.rte_sw(k, top_at_1, dev(tau), dev(ssa), dev(g), atm.mu0, go["toa_src"], atm.sfc_alb,
+                          0.5 * atm.sfc_alb)
+    rup, rdn, rdr = O.rte_sw(tau, ssa, g, atm.mu0, go["toa_src"], atm.sfc_alb, 0.5 * atm.sfc_alb,
+                             okd_sw.band_lims, top_at_1)
+    assert rel_err(_host(up), rup, floor=1e-9) < FP64_RTOL
+    assert rel_err(_host(dn), rdn, floor=1e-9) < FP64_RTOL
+    assert rel_err(_host(dr), rdr, floor=1e-9) < FP64_RTOL
+    bu, bd, bdr = G.rte_sw(k, top_at_1, dev(tau), dev(ssa), dev(g), atm.mu0, go["toa_src"], atm.sfc_alb,
+                           0.5 * atm.sfc_alb, do_broadband=True)
+    assert rel_err(_host(bu), rup.sum(2)) < FP64_RTOL and rel_err(_host(bd), rdn.sum(2)) < FP64_RTOL
+
+
+def test_add_to_and_fluxes(G, gkd, okd_sw, O):
+    rng = np.random.default_rng(5)
+    k = gkd[(False, 8)]
+    ncol, nlay, ng, nb = 7, 9, k.ngpt, k.nbnd
+    t1, s1, g1 = (rng.uniform(0, 1, (ncol, nlay, ng)) for _ in range(3))
+    t2, s2, g2 = (rng.uniform(0, 1, (ncol, nlay, nb)) for _ in range(3))
+    dev = G.DeviceArray.from_host
+    dt, ds, dg = dev(t1), dev(s1), dev(g1)
+    G.add_to(k, dt, dev(t2), ds, dg, dev(s2), dev(g2))
+    rt, rs, rg = O.inc_bybnd(t1, t2, okd_sw.band_lims, s1, g1, s2, g2)
+    assert rel_err(_host(dt), rt) < 1e-14 and rel_err(_host(ds), rs) < 1e-14 and rel_err(_host(dg), rg) < 1e-13
+    up, dn, dr = (rng.uniform(0, 10, (ncol, nlay + 1, ng)) for _ in range(3))
+    o = G.fluxes_reduce(k, dev(up), dev(dn), dev(dr), byband=True)
+    assert rel_err(_host(o["flux_up"]), O.sum_broadband(up)) < 1e-14
+    assert rel_err(_host(o["flux_net"]), O.sum_broadband(dn) - O.sum_broadband(up), floor=1e-9) < 1e-13
+    assert rel_err(_host(o["bnd_flux_dn_dir"]), O.sum_byband(dr, okd_sw.band_lims)) < 1e-14
+    assert rel_err(_host(o["bnd_flux_net"]), O.sum_byband(dn, okd_sw.band_lims) - O.sum_byband(up, okd_sw.band_lims), floor=1e-9) < 1e-13
+
+
+def test_heating_rate(G, O, syn):
+    atm = syn.make_atmosphere(3, 6, 30)
+    rng = np.random.default_rng(2)
+    up, dn = rng.uniform(100, 400, (6, 31)), rng.uniform(100, 400, (6, 31))
+    for prec in (8, 4):
+        hr = _host(G.heating_rate(up, dn, atm.p_lev, precision=prec))
+        ref = O.heating_rate(dn - up, atm.p_lev)
+        # FP32 inputs are rounded once; the differences themselves are formed in FP64.
+        assert rel_err(hr, ref, floor=1e-3) < (1e-12 if prec == 8 else 1e-3)
+
+
+def _solve_both(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm, precision, bnd=False, block=None):
+    kl, ks = gkd[(True, precision)], gkd[(False, precision)]
+    cloudy = atm.lwp is not None
+    slw = G.SolverLW(kl, G.Cloud(lut_lw, precision) if cloudy else None)
+    ssw = G.SolverSW(ks, G.Cloud(lut_sw, precision) if cloudy else None)
+    if block:
+        slw.set_block(block)
+        ssw.set_block(block)
+    da = G.DeviceAtmosphere(atm, precision)
+    tsi = np.full(atm.ncol, atm.tsi / okd_sw.scalar("tsi"))
+    glw = {n: _host(v) for n, v in slw.solve(da, bnd=bnd).items() if v is not None}
+    gsw = {n: _host(v) for n, v in ssw.solve(da, tsi, bnd=bnd).items() if v is not None}
+    rlw = O.solve_lw(okd_lw, atm, O.Cloud(lut_lw) if cloudy else None, bnd=bnd)
+    rsw = O.solve_sw(okd_sw, atm, O.Cloud(lut_sw) if cloudy else None, bnd=bnd, tsi_scaling=tsi)
+    return glw, gsw, rlw, rsw
+
+
+@pytest.mark.parametrize("bnd", [False, True])
+def test_solvers_allsky_fp64(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy, bnd):
+    glw, gsw, rlw, rsw = _solve_both(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy, 8, bnd=bnd)
+    for n in rlw:
+        assert rel_err(glw[n], rlw[n], floor=1e-6) < FP64_RTOL, n
+    for n in rsw:
+        assert rel_err(gsw[n], rsw[n], floor=1e-6) < FP64_RTOL, n
+
+
+def test_solvers_column_blocks(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, syn):
+    atm = syn.make_atmosphere(44, 50, 40, cloudy=True)
+    a = _solve_both(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm, 8, block=16)   # 3 blocks + residual 2
+    for n in a[2]:
+        assert rel_err(a[0][n], a[2][n], floor=1e-6) < FP64_RTOL, n
+    for n in a[3]:
+        assert rel_err(a[1][n], a[3][n], floor=1e-6) < FP64_RTOL, n
+
+
+def test_solvers_allsky_fp32_flux_error(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy):
+    glw, gsw, rlw, rsw = _solve_both(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy, 4)
+    err_lw = max(np.abs(glw[n] - rlw[n]).max() for n in ("flux_up", "flux_dn"))
+    err_sw = max(np.abs(gsw[n] - rsw[n]).max() for n in ("flux_up", "flux_dn", "flux_dn_dir"))
+    print(f"FP32 max |flux error| LW {err_lw:.3e} SW {err_sw:.3e} W m-2")
+    assert err_lw < 0.05 and err_sw < 0.05
+
+
+def test_range_checks_raise(G, gkd, syn, O):
+    atm = syn.make_atmosphere(7, 4, 20)
+    atm.t_lay[0, 3] = 100.0
+    k = gkd[(True, 8)]
+    with pytest.raises(ValueError, match="tlay is out of range"):
+        G.gas_optics(k, atm, G.GasConcs(atm.vmr), O.get_col_dry(atm.vmr["h2o"], atm.p_lev))
+    with pytest.raises(ValueError, match="out of range"):
+        G.GasConcs({"h2o": np.full((4, 20), 1.5)})
