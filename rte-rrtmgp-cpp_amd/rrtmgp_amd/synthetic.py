@@ -17,14 +17,28 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
+from scipy.special import log_ndtr
 
 GENERATOR_NOTES = """
-* k-table magnitude: ln k ~ N(-8, 3^2) as specified, multiplied by a reference cross-section
-  K_SCALE = 1e-21 cm^2.  Taken literally (k ~ e^-8 cm^2 per molecule) every layer would have an
-  optical depth of order 1e18.  With the scale the median LW column optical depth is ~1.6
-  (5%/95%: 1e-3/13) and the clear-sky OLR of the nlay=60 profiles is ~243 W m-2.
+* k-table magnitude: ln k ~ N(-8, 3^2) at every table node as specified, multiplied by a
+  reference cross-section K_SCALE (2e-20 cm^2 longwave, 1e-21 cm^2 shortwave).  The nodes of
+  one g-point are not drawn independently: ln k = -8 + 3 (sqrt(1/2) a_g + sqrt(1/2) s_g) with
+  a_g ~ N(0, 1) per g-point and s_g(eta, p, T) a smooth Gaussian field with exactly unit
+  variance at every node (multilinear interpolation of an iid N(0, 1) coarse grid of 3 x 8 x 4
+  nodes over (eta, p, T), each interpolation row scaled to unit L2 norm).  Real tables vary
+  smoothly between neighbouring reference temperatures, pressures and eta values; independent
+  draws would make neighbours differ by up to ~1e9, far from any physical table, and would turn
+  the rounding of the interpolation weights into O(1) relative errors at any finite precision.
+  Minor tables use the same construction over (eta, T) with a 3 x 4 coarse grid.  Taken
+  literally (k ~ e^-8 cm^2 per molecule) every layer would have an optical depth of order 1e18;
+  with the scales the median clear-sky column optical depth of the nlay=60 profiles is ~2.9
+  (LW) / ~0.4 (SW), the mean OLR is ~230 W m-2 and ~2/3 of the TOA solar flux reaches the
+  surface.
 * gases known to the k-distribution: h2o, co2, o3, n2o, ch4.  ch4 only ever appears as a minor
   contributor, so dropping it from the available gases exercises the minor-table reduction.
+* Planck fractions: Dirichlet(1) over the g-points of a band at every (eta, p, T) node, made
+  smooth across nodes by a Gaussian copula: the Gamma(1) variates are -ln(1 - Phi(s)) of smooth
+  unit-variance fields s built as for the k-tables.
 * key species per band are fixed lists (LW_KEY_LOWER/UPPER, SW_KEY_LOWER/UPPER).
 * press_ref_trop = 9948.431564193395 Pa; vmr_ref = typical mixing ratios x exp(U(-0.3, 0.3))
   per (tropo half, reference temperature); "dry air" (gas 0) reference vmr = 1.
@@ -39,12 +53,14 @@ GENERATOR_NOTES = """
 * cloud LUTs: 20 sizes per band; liquid r_eff 2.5..21.5 um, ice D_eff 10..180 um; extinction
   1.5/r_eff (m2/g, liquid) or 3/D_eff (ice) times U(0.8, 1.2); ssa U(0.5, 1), g U(0.7, 0.9).
 * profiles: level z = 7 km * ln(1.013e5 / p); play = geometric mean of the bounding levels;
-  o3 Gaussian in ln p with sigma 1.0 around 1000 Pa; level order is surface first
+  o3 Gaussian in ln p with sigma 1.0 around 1000 Pa on a 3e-8 background (without it o3 falls
+  to ~1e-16 at 1 Pa, far below any observed value); level order is surface first
   (top_at_1 = False) unless top_at_1=True is asked for.
 * ice clouds: same 20% layer fraction as liquid.
 """
 
-K_SCALE = 1.0e-21
+O3_BACKGROUND = 3.0e-8
+K_SCALE = {True: 2.0e-20, False: 1.0e-21}   # longwave, shortwave
 GAS_NAMES = ["h2o", "co2", "o3", "n2o", "ch4"]
 
 # Real RRTMGP band limits (cm-1), a physical convention, not data of the reference.
@@ -145,7 +161,36 @@ class KDist:
         return int(self.band2gpt.max())
 
 
-def _minor_block(rng, nbnd, gpt_per_band, n_intervals, first_bands):
+def _interp_rows(n_fine, n_coarse):
+    """(n_fine, n_coarse) linear-interpolation matrix, each row scaled to unit L2 norm."""
+    x = np.linspace(0.0, n_coarse - 1.0, n_fine)
+    i0 = np.minimum(np.floor(x).astype(int), n_coarse - 2)
+    w = x - i0
+    a = np.zeros((n_fine, n_coarse))
+    a[np.arange(n_fine), i0] = 1.0 - w
+    a[np.arange(n_fine), i0 + 1] = w
+    return a / np.sqrt((a * a).sum(axis=1, keepdims=True))
+
+
+def _smooth_unit_field(rng, nlead, fine, coarse):
+    """nlead independent Gaussian fields on the node grid ``fine`` (one axis per entry), each
+    N(0, 1) at every node and smooth between nodes: multilinear interpolation of an iid N(0, 1)
+    grid of ``coarse`` nodes with row-normalised weights (the tensor product of unit-norm rows
+    has unit norm, so every node keeps unit variance).  Returns shape (nlead, *fine)."""
+    f = rng.normal(size=(nlead, *coarse))
+    for ax, (nf, nc) in enumerate(zip(fine, coarse)):
+        f = np.moveaxis(np.tensordot(f, _interp_rows(nf, nc), axes=([ax + 1], [1])), -1, ax + 1)
+    return f
+
+
+def _lognormal_table(rng, nlead, fine, coarse, scale):
+    """exp(-8 + 3 z) * scale with z = sqrt(1/2) a + sqrt(1/2) s: N(0, 1) at every node."""
+    a = rng.normal(size=(nlead,) + (1,) * len(fine))
+    z = math.sqrt(0.5) * a + math.sqrt(0.5) * _smooth_unit_field(rng, nlead, fine, coarse)
+    return np.exp(-8.0 + 3.0 * z) * scale
+
+
+def _minor_block(rng, nbnd, gpt_per_band, n_intervals, first_bands, scale):
     """Minor intervals: one per band in ``first_bands`` order, extra ones on random bands."""
     bands = list(first_bands)
     extra = n_intervals - len(bands)
@@ -182,7 +227,7 @@ def _minor_block(rng, nbnd, gpt_per_band, n_intervals, first_bands):
         start[i] = pos
         pos += gpt_per_band
     ncontrib = pos - 1
-    kminor = np.exp(rng.normal(-8.0, 3.0, size=(ncontrib, NETA, NTEMP))) * K_SCALE
+    kminor = _lognormal_table(rng, ncontrib, (NETA, NTEMP), (3, 4), scale)
     return (fa(kminor), idents, fa(limits, np.int32), np.array(dens, dtype=np.int8), scal,
             np.array(compl, dtype=np.int8), np.array(start, dtype=np.int32))
 
@@ -217,11 +262,11 @@ def make_kdist(is_lw: bool, seed: int) -> KDist:
         for itr in range(2):
             vmr_ref[itr, ig + 1, :] = typical[g][itr] * np.exp(rng.uniform(-0.3, 0.3, NTEMP))
 
-    kmajor = np.exp(rng.normal(-8.0, 3.0, size=(ngpt, NETA, NPRES + 1, NTEMP))) * K_SCALE
+    kmajor = _lognormal_table(rng, ngpt, (NETA, NPRES + 1, NTEMP), (3, 8, 4), K_SCALE[is_lw])
 
     n_low, n_upp = 21, 16
-    lo = _minor_block(rng, nbnd, gpb, n_low, range(min(nbnd, n_low)))
-    up = _minor_block(rng, nbnd, gpb, n_upp, range(min(nbnd, n_upp)))
+    lo = _minor_block(rng, nbnd, gpb, n_low, range(min(nbnd, n_low)), K_SCALE[is_lw])
+    up = _minor_block(rng, nbnd, gpb, n_upp, range(min(nbnd, n_upp)), K_SCALE[is_lw])
 
     kd = KDist(
         is_lw=is_lw, gas_names=list(GAS_NAMES), key_species=fa(key_species, np.int32),
@@ -242,7 +287,8 @@ def make_kdist(is_lw: bool, seed: int) -> KDist:
         for b in range(nbnd):
             totplnk[:, b] = planck_band_radiance(tplanck, edges[b], edges[b + 1])
         kd.totplnk = fa(totplnk)
-        pf = rng.gamma(1.0, size=(ngpt, NETA, NPRES + 1, NTEMP))
+        # Gamma(1) = Exp(1) variates -ln(1 - Phi(s)) = -ln Phi(-s) of smooth N(0, 1) fields.
+        pf = -log_ndtr(-_smooth_unit_field(rng, ngpt, (NETA, NPRES + 1, NTEMP), (3, 8, 4)))
         pf = pf.reshape(nbnd, gpb, NETA, NPRES + 1, NTEMP)
         pf /= pf.sum(axis=1, keepdims=True)
         kd.planck_frac = fa(pf.reshape(ngpt, NETA, NPRES + 1, NTEMP))
@@ -365,7 +411,7 @@ def make_atmosphere(seed: int, ncol: int, nlay: int, cloudy: bool = False,
     t_sfc_out = t_lev[:, 0].copy()
 
     h2o = np.maximum(1e-2 * np.exp(-np.maximum(z_lay, 0.0) / 2.0), 3e-6)
-    o3 = 8e-6 * np.exp(-0.5 * ((np.log(p_lay) - math.log(1000.0)) / 1.0) ** 2)
+    o3 = 8e-6 * np.exp(-0.5 * ((np.log(p_lay) - math.log(1000.0)) / 1.0) ** 2) + O3_BACKGROUND
     vmr = {"h2o": h2o, "o3": o3, "co2": np.array([[400e-6]]), "ch4": np.array([[1.8e-6]]),
            "n2o": np.array([[3.2e-7]])}
 

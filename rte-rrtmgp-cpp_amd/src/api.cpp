@@ -100,7 +100,8 @@ int rrtmgp_api_kdist_create(const rrtmgp_kdist_file* f, void** out)
     return guarded([&] {
         const int nb = f->nbnd, ng = f->ngpt, nt = f->ntemp, np = f->npres, ne = f->neta;
         Gas_concs_gpu avail;
-        for (const auto& g : names(f->available_gases, f->n_available).v()) avail.set_vmr(g, Float(0.));
+        const auto avail_names = names(f->available_gases, f->n_available);
+        for (const auto& g : avail_names.v()) avail.set_vmr(g, Float(0.));
         auto gas = names(f->gas_names, f->ngas);
         auto ks = int_arr<3>(f->key_species, {2, 2, nb});
         auto b2g = int_arr<2>(f->band2gpt, {2, nb});

@@ -75,7 +75,26 @@ def atm_cloudy(syn):
     return syn.make_atmosphere(syn.SEED_ATMOS[1], 24, 128, cloudy=True)
 
 
+class Err(float):
+    """A float error measure whose repr names the worst element (shown by failing asserts)."""
+
+    def __new__(cls, value, info=""):
+        obj = float.__new__(cls, value)
+        obj.info = info
+        return obj
+
+    def __repr__(self):
+        return f"{float(self):.3e} [{self.info}]"
+
+
 def rel_err(a, b, floor=0.0):
+    """max |a - b| / max(|b|, floor), carrying the index and both values of the worst element."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    return np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)) if a.size else 0.0
+    if not a.size:
+        return Err(0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        e = np.abs(a - b) / np.maximum(np.abs(b), floor)
+    e = np.where(a == b, 0.0, np.where(np.isnan(e), np.inf, e))
+    i = np.unravel_index(int(np.argmax(e)), e.shape)
+    return Err(e[i], f"at {tuple(int(x) for x in i)} of {e.shape}: got {a[i]!r} want {b[i]!r}")

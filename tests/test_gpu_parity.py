@@ -15,6 +15,7 @@ from conftest import AVAILABLE, rel_err
 pytestmark = pytest.mark.gpu
 
 FP64_RTOL = 1e-10
+SW_GPT_RTOL = 1e-8
 
 
 @pytest.fixture(scope="module")
@@ -81,9 +82,14 @@ def test_interpolation_indices_bit_exact(G, gkd, okd_lw, syn, O, top_at_1):
     got = {n: d.to_host() for n, d in out.items()}
     for n in ("jtemp", "jpress", "tropo", "jeta"):
         assert np.array_equal(got[n], ref[n]), n
-    # Weights are produced without FMA contraction: identical to the contraction-free oracle.
-    for n in ("col_mix", "fmajor", "fminor"):
-        assert np.array_equal(got[n], ref[n]), n
+    # Weights are produced without FMA contraction.  col_mix and fminor involve only +, *, / and
+    # fmod (correctly rounded on both sides): bit-exact.  fmajor also carries fpress, built from
+    # log(p); the device log and the C library's log are each within an ulp or so of the true
+    # value, not correctly rounded, so fpress may differ by |d log p| / press_ref_log_delta
+    # ~ 2 ulp(11.6) / 0.2 ~ 2e-14.  fmajor = (1 - fpress) f or fpress f, with f in [0, 1].
+    for n in ("col_mix", "fminor"):
+        assert np.array_equal(got[n], ref[n]), (n, rel_err(got[n], ref[n], floor=1e-300))
+    assert np.abs(got["fmajor"] - ref["fmajor"]).max() < 1e-13, rel_err(got["fmajor"], ref["fmajor"], floor=1.0)
 
 
 def _gas_optics_both(G, gkd, okd, atm, lw, precision, O):
@@ -101,13 +107,54 @@ def test_gas_optics_fp64(G, gkd, okd_lw, okd_sw, atm_c0, O, lw):
         assert rel_err(got[n], ref[n], floor=1e-300) < FP64_RTOL, n
 
 
+def _eta_edge_mask(O, okd, atm, neta):
+    """(ncol, nlay, ngpt) mask of g-points whose FP64 eta lies within FP32 rounding below 1.
+
+    The reference interpolation is discontinuous at eta = 1 (interpolation_kernel,
+    src_kernels_cuda/gas_optics_rrtmgp_kernels.cu:272-276): eta = 1 gives jeta = neta-1 with
+    feta = 0, i.e. the value at eta node neta-1, while eta -> 1 from below tends to the value at
+    node neta.  When ratio * col2 / col1 < ~6e-8 an FP32 build rounds eta to exactly 1 where FP64
+    does not, and picks the other side of the jump.  Such g-points are a property of the
+    algorithm at FP32, not a rounding error; they are listed here and checked separately.
+    """
+    ref = O.gas_optics(okd, atm, col_dry=O.get_col_dry(atm.vmr["h2o"], atm.p_lev), interp=True)
+    fm = ref["fminor"]                                   # (2 [1-feta, feta] x ftemp_term, 2, ncol, nlay, nflav)
+    tot = fm[0] + fm[1]
+    feta = np.divide(fm[1], tot, out=np.zeros_like(tot), where=tot > 0)
+    edge = ((ref["jeta"] == neta - 1) & (feta > 1.0 - 1e-5)).any(axis=0)   # (ncol, nlay, nflav)
+    ngpt = okd.ngpt
+    gf = okd.get_int("gpoint_flavor").reshape((2, ngpt), order="F") - 1      # (itropo, gpt)
+    itropo = 1 - ref["tropo"].astype(int)
+    flav = gf[itropo[:, :, None], np.arange(ngpt)[None, None, :]]            # (ncol, nlay, ngpt)
+    return np.take_along_axis(edge, flav, axis=2)
+
+
 @pytest.mark.parametrize("lw", [True, False])
-def test_gas_optics_fp32(G, gkd, okd_lw, okd_sw, atm_c0, O, lw):
-    ref, got = _gas_optics_both(G, gkd, okd_lw if lw else okd_sw, atm_c0, lw, 4, O)
+def test_gas_optics_fp32(G, gkd, okd_lw, okd_sw, atm_c0, O, syn, lw):
+    """FP32 build against the FP64 oracle: 2e-4 relative (floor 1e-6 of the field's maximum).
+
+    Two FP32 semantics of the reference are reproduced rather than FP64's: the CPU combine rule
+    sets ssa = 0 where tau <= 2 epsilon (src/Gas_optics_rrtmgp.cpp:365-385), with epsilon the
+    FP32 one; and the eta = 1 jump (see _eta_edge_mask).  Elements within 1e-3 of the ssa
+    threshold, where either side is right, are skipped."""
+    okd = okd_lw if lw else okd_sw
+    ref, got = _gas_optics_both(G, gkd, okd, atm_c0, lw, 4, O)
+    edge = _eta_edge_mask(O, okd, atm_c0, syn.NETA)
+    print(f"g-points at the eta = 1 jump: {int(edge.sum())} of {edge.size}")
+    assert edge.mean() < 0.01
+    keep3 = ~edge
+    if not lw:
+        thr = 2.0 * np.finfo(np.float32).eps
+        tau = ref["tau"]
+        ref = dict(ref)
+        ref["ssa"] = np.where(tau > thr, ref["ssa"], 0.0)
+        ref["g"] = np.where(tau > thr, ref["g"], 0.0)
+        keep3 &= np.abs(tau / thr - 1.0) > 1e-3
+        print(f"layers below the FP32 combine threshold: {int((tau <= thr).sum())}")
     for n in ref:
         scale = np.abs(ref[n]).max()
-        err = np.abs(got[n] - ref[n]) / np.maximum(np.abs(ref[n]), 1e-6 * scale)
-        assert err.max() < 2e-4, (n, err.max())
+        keep = keep3 if ref[n].ndim == 3 else np.ones(ref[n].shape, dtype=bool)
+        assert rel_err(got[n][keep], ref[n][keep], floor=1e-6 * scale) < 2e-4, n
 
 
 @pytest.mark.parametrize("two_stream,delta", [(False, False), (True, False), (True, True)])
@@ -154,9 +201,15 @@ def test_rte_sw_fp64(G, gkd, okd_sw, syn, O, lut_sw, top_at_1):
                           0.5 * atm.sfc_alb)
     rup, rdn, rdr = O.rte_sw(tau, ssa, g, atm.mu0, go["toa_src"], atm.sfc_alb, 0.5 * atm.sfc_alb,
                              okd_sw.band_lims, top_at_1)
-    assert rel_err(_host(up), rup, floor=1e-9) < FP64_RTOL
-    assert rel_err(_host(dn), rdn, floor=1e-9) < FP64_RTOL
-    assert rel_err(_host(dr), rdr, floor=1e-9) < FP64_RTOL
+    # Relative to max(|flux|, 1e-6 W m-2): in deep layers a g-point's flux falls to ~1e-19 of its
+    # TOA value and FP64 rounding of the TOA-scale terms (~1e-16 W m-2) dominates it there.
+    # The per-g-point SW problem is ill-conditioned in places (near-conservative cloud layers
+    # over reflective ones): toggling FMA contraction alone in the oracle moves the flux up at
+    # (col 35, level 44, g-point 90) of this case by 3.6e-10 relative.  Hence 1e-8 here; the
+    # summed fluxes below keep the 1e-10 bar.
+    assert rel_err(_host(up), rup, floor=1e-6) < SW_GPT_RTOL
+    assert rel_err(_host(dn), rdn, floor=1e-6) < SW_GPT_RTOL
+    assert rel_err(_host(dr), rdr, floor=1e-6) < SW_GPT_RTOL
     bu, bd, bdr = G.rte_sw(k, top_at_1, dev(tau), dev(ssa), dev(g), atm.mu0, go["toa_src"], atm.sfc_alb,
                            0.5 * atm.sfc_alb, do_broadband=True)
     assert rel_err(_host(bu), rup.sum(2)) < FP64_RTOL and rel_err(_host(bd), rdn.sum(2)) < FP64_RTOL
