@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Throughput of the all-sky LW+SW column-radiation forward on one MI355X.
+
+Workload: 65536 columns (256 x 256) x 128 layers, all-sky (liquid and ice clouds), FP32, plain
+RRTMGP gas optics, seeded synthetic k-distributions (LW 16 bands / 256 g-points, SW 14 bands /
+224 g-points), cloud LUTs and profiles.  One step = Radiation_solver_longwave_gpu::solve_gpu +
+Radiation_solver_shortwave_gpu::solve_gpu over all columns (gas optics, cloud optics, solvers,
+broadband fluxes), inputs already resident in HBM.
+
+metric: col-gpt-layers/s = ncol * nlay * (256 + 224) / (time of one step).
+
+Also reported, on the same JSON line:
+  roofline      the kernel with the largest share of GPU time: its algorithmic bytes (the
+                class-API arrays it must read or write, per launch) over its average launch
+                duration, measured with HIP events on its launch stream during the timed steps,
+                against the 8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE/WRITE_SIZE when a
+                summary of them for this workload is committed under profiles/.
+  cpu_baseline  the FP64 CPU oracle (a restatement of the reference CPU path) on the first
+                columns of the same workload, timed on this host.
+  parity        max |flux - FP64 oracle| over those columns (W m-2).
+  kernels       per-kernel launches, time and algorithmic GB/s of the timed steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "rte-rrtmgp-cpp_amd"), os.path.join(ROOT, "oracle")]
+
+import numpy as np  # noqa: E402
+
+METRIC = "LW+SW col-gpt-layers/s, plain vs NN gas optics at ≤0.1 W/m² flux err; adj/fwd"
+HBM_PEAK_GBS = 8000.0
+AVAILABLE = ["h2o", "co2", "o3", "n2o", "ch4"]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(ncol, nlay, E, kd_lw, kd_sw, n2d_gases, cloudy):
+    """Algorithmic bytes of each kernel over one LW+SW step, as {kernel: bytes}.
+
+    Per kernel, the arrays of the class API (include/*.h of the reference) it must read or
+    write once: e.g. LW gas optics writes tau, lay_src, lev_src_inc, lev_src_dec (4 E per
+    col-gpt-layer), the LW solver reads them back and writes broadband fluxes.  Per-(col, lay)
+    inputs are included; k-tables and LUTs (read through L2) are not.
+    """
+    nlev = nlay + 1
+    cl, cv = ncol * nlay, ncol * nlev
+    glw, gsw = kd_lw.ngpt, kd_sw.ngpt
+    blw, bsw = kd_lw.nbnd, kd_sw.nbnd
+    per_lay_in = (3 + n2d_gases) * E * cl            # play, tlay, col_dry, 2-D vmr
+    b = {}
+    b["gas_optics_lw_kernel"] = (4 * E * cl * glw + 2 * E * ncol * glw + per_lay_in + E * cv + E * ncol
+                                 + (E * cl * blw if cloudy else 0))
+    b["gas_optics_sw_kernel"] = 3 * E * cl * gsw + per_lay_in + (3 * E * cl * bsw if cloudy else 0)
+    b["lw_solver_kernel"] = 4 * E * cl * glw + E * ncol * glw + E * ncol * blw + 2 * E * cv
+    b["sw_solver_kernel"] = 3 * E * cl * gsw + E * ncol * (gsw + 2 * bsw + 1) + 3 * E * cv
+    # Reductions of g-point fluxes (2 LW, 3 SW launches per step): read ngpt, write one.
+    b["sum_gpt_kernel"] = 2 * E * cv * (glw + 1) + 3 * E * cv * (gsw + 1)
+    if cloudy:
+        b["cloud_optics_kernel"] = 4 * E * cl * 2 + E * cl * blw + 3 * E * cl * bsw
+    b["col_dry_kernel"] = 2 * (E * cv + 2 * E * cl)
+    b["net_kernel"] = 2 * 3 * E * cv
+    b["scale_by_col_kernel"] = 2 * E * ncol * gsw + E * ncol
+    return b
+
+
+def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
+    """FP64 oracle on the first ncols columns with `threads` OpenMP threads; returns the timing
+    record and the max flux error of the GPU fluxes over those columns."""
+    import oracle as O
+    from rrtmgp_amd import synthetic as syn
+    sub = syn.subset_columns(atm, slice(0, ncols))
+    okl, oks = O.Kdist(kl, AVAILABLE), O.Kdist(ks, AVAILABLE)
+    ocl, ocs = O.Cloud(ll), O.Cloud(ls)
+    tsi = np.full(sub.ncol, sub.tsi / oks.scalar("tsi"))
+    prev = O.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        rlw = O.solve_lw(okl, sub, ocl)
+        rsw = O.solve_sw(oks, sub, ocs, tsi_scaling=tsi)
+        dt = time.perf_counter() - t0
+    finally:
+        O.set_threads(prev)
+    units = sub.ncol * sub.nlay * (okl.ngpt + oks.ngpt)
+    err = {}
+    for name, ref, got in (("lw", rlw, glw), ("sw", rsw, gsw)):
+        for k, v in ref.items():
+            if k in got:
+                err[f"{name}_{k}"] = float(np.abs(got[k][:ncols] - v).max())
+    return units / dt, dt, err
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncol", type=int, default=65536)
+    ap.add_argument("--nlay", type=int, default=128)
+    ap.add_argument("--precision", type=int, choices=(32, 64), default=32)
+    ap.add_argument("--n-col-block", type=int, default=0, help="columns per solver block (0: all)")
+    ap.add_argument("--cpu-cols", type=int, default=4096, help="columns of the CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: OMP_NUM_THREADS, max 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): columns are independent, "
+              "so a multi-GPU run would shard columns with no exchange; only the one-GPU "
+              "throughput is measured here.", file=sys.stderr, flush=True)
+        sys.exit(2)
+
+    from rrtmgp_amd import api, device, profiling, synthetic as syn
+
+    prec = 4 if args.precision == 32 else 8
+    E = prec
+    ncol, nlay = args.ncol, args.nlay
+    log(f"workload: all-sky LW+SW, {ncol} columns x {nlay} layers, FP{args.precision}")
+    kl, ks = syn.make_kdist(True, syn.SEED_KDIST_LW), syn.make_kdist(False, syn.SEED_KDIST_SW)
+    ll, ls = syn.make_cloud_lut(True, syn.SEED_CLOUD_LW), syn.make_cloud_lut(False, syn.SEED_CLOUD_SW)
+    t0 = time.perf_counter()
+    atm = syn.make_atmosphere(syn.SEED_ATMOS[1], ncol, nlay, cloudy=True)
+    log(f"profiles generated in {time.perf_counter() - t0:.1f} s")
+
+    kd_lw, kd_sw = api.KDist(kl, AVAILABLE, prec), api.KDist(ks, AVAILABLE, prec)
+    solver_lw = api.SolverLW(kd_lw, api.Cloud(ll, prec))
+    solver_sw = api.SolverSW(kd_sw, api.Cloud(ls, prec))
+    blk = args.n_col_block if args.n_col_block > 0 else ncol
+    solver_lw.set_block(blk)
+    solver_sw.set_block(blk)
+    da = api.DeviceAtmosphere(atm, prec)
+    dt = api.real_dtype(prec)
+    tsi = device.DeviceArray.from_host(np.full(ncol, atm.tsi / kd_sw.scalar("tsi")), dt)
+    nlev = nlay + 1
+    out_lw = {k: device.DeviceArray((ncol, nlev), dt) for k in ("flux_up", "flux_dn", "flux_net")}
+    out_sw = {k: device.DeviceArray((ncol, nlev), dt) for k in ("flux_up", "flux_dn", "flux_dn_dir", "flux_net")}
+    device.synchronize()
+    log("inputs resident on the device")
+
+    def step():
+        solver_lw.solve(da, outputs=out_lw)
+        solver_sw.solve(da, tsi, outputs=out_sw)
+
+    for i in range(max(args.warmup, 1)):
+        step()
+        device.synchronize()
+        log(f"warmup step {i + 1} done")
+
+    start, stop = device.Event(), device.Event()
+    profiling.enable(True)
+    device.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        step()
+    stop.record()
+    device.synchronize()
+    wall = time.perf_counter() - t0
+    profiling.enable(False)
+    gpu_ms = start.elapsed_ms(stop)
+    kernels = profiling.collect()
+    log(f"{args.steps} timed steps: {wall * 1e3:.1f} ms wall, {gpu_ms:.1f} ms between events")
+
+    units = ncol * nlay * (kd_lw.ngpt + kd_sw.ngpt)
+    ms_per_step = wall * 1e3 / args.steps
+    value = units * args.steps / wall
+
+    n2d = sum(1 for v in atm.vmr.values() if v.shape[0] > 1)
+    abytes = algorithmic_bytes(ncol, nlay, E, kd_lw, kd_sw, n2d, True)
+    krows = []
+    for k in kernels:
+        total_b = abytes.get(k.name)
+        total_b = total_b * args.steps if total_b is not None else None
+        krows.append({"kernel": k.name, "launches": k.launches, "total_ms": round(k.total_ms, 4),
+                      "avg_ms": round(k.avg_ms, 5),
+                      "algorithmic_GBps": (round(total_b / (k.total_ms * 1e-3) / 1e9, 1)
+                                           if total_b is not None and k.total_ms > 0 else None)})
+    krows.sort(key=lambda r: -r["total_ms"])
+    dom = next((r for r in krows if r["algorithmic_GBps"] is not None), None)
+    roofline = None
+    if dom is None:
+        log("no kernel timings were recorded; roofline left empty")
+    else:
+        dom_bytes_per_launch = abytes[dom["kernel"]] * args.steps / dom["launches"]
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                tr = json.load(f)
+            rec = tr.get("kernels", {}).get(dom["kernel"])
+            if rec and tr.get("workload") == {"ncol": ncol, "nlay": nlay, "precision": args.precision}:
+                traffic = rec.get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["algorithmic_GBps"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(dom["algorithmic_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "algorithmic_bytes_per_launch": int(dom_bytes_per_launch),
+                    "avg_launch_ms": dom["avg_ms"]}
+
+    glw = {k: v.to_host() for k, v in out_lw.items()}
+    gsw = {k: v.to_host() for k, v in out_sw.items()}
+    finite = all(np.isfinite(a).all() for a in list(glw.values()) + list(gsw.values()))
+
+    cpu = None
+    parity = {"finite": bool(finite)}
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        ncpu = min(args.cpu_cols, ncol)
+        log(f"CPU baseline: FP64 oracle on {ncpu} columns, {threads} threads")
+        rate, secs, err = cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncpu, threads)
+        cpu = {"value": round(rate, 1), "unit": "col-gpt-layers/s", "cores": threads, "kind": "port",
+               "sample": f"first {ncpu} of the {ncol} columns x {nlay} layers, all-sky LW+SW, FP64 "
+                         f"oracle (restated reference CPU path, 12-column blocks), {secs:.2f} s"}
+        parity.update({"columns": ncpu, "flux_err_max_wm2": round(max(err.values()), 6),
+                       "per_output": {k: round(v, 6) for k, v in err.items()}})
+        log(f"CPU baseline {rate:.3e} U/s in {secs:.2f} s; max flux error {max(err.values()):.3e} W m-2")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "col-gpt-layers/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == 4 else "f64",
+        "data": "synthetic: seeded RRTMGP-layout k-distributions, cloud LUTs and profiles",
+        "config": {"workload": f"all-sky LW+SW forward, plain gas optics, {ncol} columns x {nlay} layers",
+                   "ncol": ncol, "nlay": nlay, "ngpt_lw": kd_lw.ngpt, "ngpt_sw": kd_sw.ngpt,
+                   "n_col_block": blk, "gpu_ms_per_step": round(gpu_ms / args.steps, 3)},
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "kernels": krows,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -63,6 +63,23 @@ int rrtmgp_hip_event_destroy(void* event);
 int rrtmgp_hip_event_record(void* event, void* stream);
 int rrtmgp_hip_event_elapsed_ms(void* start, void* stop, float* ms);
 
+/* Caching device allocator (the reference's counterparts: allocate_gpu/free_gpu with its own */
+/* pool, include/tools_gpu.h:71-98, Memory_pool_gpu include/mem_pool_gpu.h:12, src_cuda/mem_pool_gpu.cu). */
+/* A freed block is kept and handed out again only to a request on the same device and stream,*/
+/* so stream order covers reuse.  release() synchronizes and frees all unused blocks.         */
+int rrtmgp_hip_cache_malloc(void** ptr, size_t bytes, void* stream);
+int rrtmgp_hip_cache_free(void* ptr, void* stream);
+int rrtmgp_hip_cache_release(void);
+int rrtmgp_hip_cache_stats(size_t* live_bytes, size_t* cached_bytes);
+
+/* Per-kernel timing.  No counterpart in the reference (its timing is external profiling):   */
+/* with timing on, every kernel launch of this library is bracketed by two HIP events on its */
+/* launch stream.  enable(1) clears earlier records; collect() waits for the recorded events  */
+/* and totals them per kernel name; get(i) reads total i (kernel name, launches, total ms).   */
+int rrtmgp_hip_timing_enable(int on);
+int rrtmgp_hip_timing_collect(int* n_kernels);
+int rrtmgp_hip_timing_get(int i, char* name, int len, int64_t* launches, double* total_ms);
+
 /* ------------------------------------------------------------------------------------------ */
 /* Device-resident k-distribution, as prepared by the Gas_optics_rrtmgp_gpu constructor        */
 /* (reference: src_cuda/Gas_optics_rrtmgp.cu:779-802 uploads the same tables).  Real-valued    */
@@ -269,6 +286,13 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_EXPAND)
         T* broadband_up, T* broadband_dn, int do_jacobians, const T* sfc_src_jac,              \
         T* flux_up_jac, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_SOLVER)
+
+/* lw_secants_array (rte_solver_kernels_cuda.h:62-64): secants (ncol, ngpt, n_gauss_quad)    */
+/* filled with gauss_Ds(imu, n_gauss_quad) of the device table (max_gauss_pts, max_gauss_pts).*/
+#define RRTMGP_DECL_LW_SECANTS(S, T)                                                           \
+    int rrtmgp_hip_lw_secants_array_##S(int64_t ncol, int64_t ngpt, int64_t n_gauss_quad,    \
+        int64_t max_gauss_pts, const T* gauss_Ds, T* secants, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_SECANTS)
 
 /* sw_solver_2stream (rte_solver_kernels_cuda.h:36-44; CPU F90:512-605, 996-1227).  mu0      */
 /* (ncol); sfc_alb_dir/dif, inc_flux_dir, inc_flux_dif (ncol, ngpt).  flux_dn is total        */

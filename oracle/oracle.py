@@ -391,6 +391,12 @@ def heating_rate(flux_net, plev):
     return out
 
 
+def set_threads(n: int) -> int:
+    """Threads of solve_lw/solve_sw's column-block loop (1 = serial, as the reference driver);
+    returns the previous value."""
+    return int(lib().oracle_set_threads(int(n)))
+
+
 def solve_lw(kd: Kdist, atm, cloud: Cloud | None = None, bnd=False, col_dry=None, n_col_block=12):
     """Radiation_solver_longwave::solve restated; returns dict of (ncol, nlev[, nbnd]) arrays."""
     ncol, nlay = atm.p_lay.shape

@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <map>
 #include <memory>
@@ -1224,6 +1225,34 @@ int guarded(F&& f)
     catch (const std::range_error& e) { last_error = e.what(); return 2; }
     catch (const std::exception& e) { last_error = e.what(); return 1; }
 }
+
+// Threads of the column-block loops of oracle_solve_lw/sw.  1 (the default) runs the blocks in
+// order on the calling thread, as the reference's driver does; more spreads independent blocks
+// over OpenMP threads (a parallel variant of the CPU baseline, not a reference behaviour).
+int g_threads = 1;
+
+template<typename F>
+void for_col_blocks(int ncol, int n_col_block, F&& body)
+{
+    const int nblocks = (ncol + n_col_block - 1) / n_col_block;
+    if (g_threads <= 1)
+    {
+        for (int b = 0; b < nblocks; ++b) body(b * n_col_block);
+        return;
+    }
+    std::exception_ptr first;
+    #pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+    for (int b = 0; b < nblocks; ++b)
+    {
+        try { body(b * n_col_block); }
+        catch (...)
+        {
+            #pragma omp critical(oracle_first_error)
+            { if (!first) first = std::current_exception(); }
+        }
+    }
+    if (first) std::rethrow_exception(first);
+}
 }  // namespace
 
 
@@ -1233,6 +1262,14 @@ int guarded(F&& f)
 extern "C" {
 
 const char* oracle_last_error() { return last_error.c_str(); }
+
+// Sets the thread count of oracle_solve_lw/sw (values < 1 mean 1); returns the previous one.
+int oracle_set_threads(int n)
+{
+    const int prev = g_threads;
+    g_threads = n < 1 ? 1 : n;
+    return prev;
+}
 
 void* oracle_kdist_create(const OracleKdistFile* f)
 {
@@ -1499,7 +1536,7 @@ int oracle_solve_lw(void* kd_h, void* cloud_h, int ncol, int nlay, const double*
         const bool out_bnd = bnd_up != nullptr;
         const auto gnames = split_names(gc->names, gc->n);
         if (n_col_block <= 0) n_col_block = 12;
-        for (int cs = 0; cs < ncol; cs += n_col_block)
+        for_col_blocks(ncol, n_col_block, [&](const int cs)
         {
             const int n = std::min(n_col_block, ncol - cs);
             auto sub = [&](const double* a, int nz) {
@@ -1579,7 +1616,7 @@ int oracle_solve_lw(void* kd_h, void* cloud_h, int ncol, int nlay, const double*
                     const size_t o = cs + i + size_t(ncol)*k, s = i + size_t(n)*k;
                     flux_up[o] = fu[s]; flux_dn[o] = fd[s]; flux_net[o] = fd[s] - fu[s];
                 }
-        }
+        });
     });
 }
 
@@ -1603,7 +1640,7 @@ int oracle_solve_sw(void* kd_h, void* cloud_h, int delta_cloud, int ncol, int nl
         const bool out_bnd = bnd_up != nullptr;
         const auto gnames = split_names(gc->names, gc->n);
         if (n_col_block <= 0) n_col_block = 12;
-        for (int cs = 0; cs < ncol; cs += n_col_block)
+        for_col_blocks(ncol, n_col_block, [&](const int cs)
         {
             const int n = std::min(n_col_block, ncol - cs);
             auto sub = [&](const double* a, int nz) {
@@ -1692,7 +1729,7 @@ int oracle_solve_sw(void* kd_h, void* cloud_h, int delta_cloud, int ncol, int nl
                     const size_t o = cs + i + size_t(ncol)*k, s = i + size_t(n)*k;
                     flux_up[o] = fu[s]; flux_dn[o] = fd[s]; flux_dn_dir[o] = fdir[s]; flux_net[o] = fd[s] - fu[s];
                 }
-        }
+        });
     });
 }
 

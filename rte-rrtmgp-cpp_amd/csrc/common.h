@@ -27,6 +27,28 @@ int check_hip(hipError_t e, const char* what);
 // Launch-status check: launch errors surface as hipGetLastError.
 #define RRTMGP_LAUNCHED(name) RRTMGP_TRY(::rrtmgp_hip::check_hip(hipGetLastError(), name))
 
+// Optional per-launch timing (rrtmgp_hip_timing_* of rrtmgp_hip.h).  When timing is on, a
+// LaunchTimer records a HIP event on the launch stream at construction and another at
+// destruction, so one timer must wrap exactly one launch in its own scope.  Off: no HIP calls.
+class LaunchTimer
+{
+    public:
+        LaunchTimer(const char* name, hipStream_t stream);
+        ~LaunchTimer();
+        LaunchTimer(const LaunchTimer&) = delete;
+        LaunchTimer& operator=(const LaunchTimer&) = delete;
+
+    private:
+        const char* name_;
+        hipStream_t stream_;
+        hipEvent_t start_ = nullptr;
+};
+
+#define RRTMGP_CAT2(a, b) a##b
+#define RRTMGP_CAT(a, b) RRTMGP_CAT2(a, b)
+#define RRTMGP_TIMED(name, stream) \
+    ::rrtmgp_hip::LaunchTimer RRTMGP_CAT(launch_timer_, __LINE__)((name), (stream))
+
 inline int arg_error(const std::string& msg)
 {
     set_error(msg);
@@ -40,23 +62,27 @@ inline unsigned grid1(int64_t n, int block)
     return static_cast<unsigned>((n + block - 1) / block);
 }
 
-// Keeps memory freed by hipFreeAsync in the device's default pool instead of returning it to
-// the driver at every synchronization, so repeated solver calls do not re-map scratch.
+// Caching device allocator of the library (rrtmgp_hip_cache_* of rrtmgp_hip.h): blocks are
+// reused by later requests on the same device and stream; see runtime.hip.
+int cache_alloc(void** p, size_t bytes, hipStream_t stream);
+int cache_free(void* p, hipStream_t stream);
 int retain_pool_memory();
 
-// Device allocation of scratch memory on a stream (stream-ordered allocator).
+// Scratch of a launcher: taken from the cache, returned right after the launches that use it
+// are queued (stream order keeps a later reuse on the same stream behind them).
 template<typename T>
 int scratch_alloc(T** p, size_t n, hipStream_t s)
 {
-    if (n == 0) { *p = nullptr; return RRTMGP_OK; }
-    RRTMGP_TRY(retain_pool_memory());
-    return check_hip(hipMallocAsync(reinterpret_cast<void**>(p), n * sizeof(T), s), "hipMallocAsync");
+    void* v = nullptr;
+    RRTMGP_TRY(cache_alloc(&v, n * sizeof(T), s));
+    *p = static_cast<T*>(v);
+    return RRTMGP_OK;
 }
 
 template<typename T>
 void scratch_free(T* p, hipStream_t s)
 {
-    if (p) (void)hipFreeAsync(p, s);
+    if (p) (void)cache_free(p, s);
 }
 
 // Precision-dependent constants.

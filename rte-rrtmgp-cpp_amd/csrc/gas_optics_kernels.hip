@@ -698,6 +698,7 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
     const Kd<Real> k = make_kd<Real>(*kd);
     const Tile tile = make_tile(ncol, nlay, kd->nbnd);
+    RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(tile.nunits)), dim3(GO_BX, GO_BY), 0,
                        as_stream(stream), k, g, ncol, nlay, tile, play, tlay, tlev, tsfc, col_dry,
                        sfc_lay, cloud_tau, tau, lay_src, lev_inc, lev_dec, sfc_src, sfc_jac);
@@ -719,6 +720,7 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, gs));
     const Kd<Real> k = make_kd<Real>(*kd);
     const Tile tile = make_tile(ncol, nlay, kd->nbnd);
+    RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(tile.nunits)), dim3(GO_BX, GO_BY), 0,
                        as_stream(stream), k, gs, ncol, nlay, tile, play, tlay, col_dry, ctau, cssa, cg,
                        tau, ssa, g);
@@ -753,6 +755,7 @@ extern "C" int rrtmgp_hip_interpolation_##S(const rrtmgp_kdist_dev* kd, int64_t 
     RRTMGP_TRY(check_kd(kd, false, false));                                                       \
     const Kd<T> k = make_kd<T>(*kd);                                                               \
     const int64_t n = ncol * nlay;                                                                 \
+    RRTMGP_TIMED("interpolation_kernel", as_stream(stream));                                       \
     hipLaunchKernelGGL(interpolation_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                \
                        as_stream(stream), k, ncol, nlay, play, tlay, col_gas, jtemp, fmajor,       \
                        fminor, col_mix, tropo, jeta, jpress);                                      \
@@ -767,6 +770,7 @@ extern "C" int rrtmgp_hip_compute_tau_absorption_##S(const rrtmgp_kdist_dev* kd,
     RRTMGP_TRY(check_kd(kd, false, false));                                                       \
     const Kd<T> k = make_kd<T>(*kd);                                                               \
     const int64_t n = ncol * nlay * kd->nbnd;                                                      \
+    RRTMGP_TIMED("tau_absorption_kernel", as_stream(stream));                                      \
     hipLaunchKernelGGL(tau_absorption_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,               \
                        as_stream(stream), k, ncol, nlay, tropo, col_mix, fmajor, fminor, play,     \
                        tlay, col_gas, jeta, jtemp, jpress, tau);                                   \
@@ -780,6 +784,7 @@ extern "C" int rrtmgp_hip_compute_tau_rayleigh_##S(const rrtmgp_kdist_dev* kd, i
     RRTMGP_TRY(check_kd(kd, false, true));                                                        \
     const Kd<T> k = make_kd<T>(*kd);                                                               \
     const int64_t n = ncol * nlay * kd->ngpt;                                                      \
+    RRTMGP_TIMED("tau_rayleigh_kernel", as_stream(stream));                                        \
     hipLaunchKernelGGL(tau_rayleigh_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                 \
                        as_stream(stream), k, ncol, nlay, col_dry, col_gas, fminor, jeta, tropo,    \
                        jtemp, tau_rayleigh);                                                       \
@@ -790,6 +795,7 @@ extern "C" int rrtmgp_hip_combine_abs_and_rayleigh_##S(int64_t ncol, int64_t nla
     const T* tau_abs, const T* tau_rayleigh, T* tau, T* ssa, T* g, void* stream)                   \
 {                                                                                                  \
     const int64_t n = ncol * nlay * ngpt;                                                          \
+    RRTMGP_TIMED("combine_kernel", as_stream(stream));                                             \
     hipLaunchKernelGGL(combine_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
                        n, tau_abs, tau_rayleigh, tau, ssa, g);                                     \
     RRTMGP_LAUNCHED("combine_kernel");                                                            \
@@ -803,6 +809,7 @@ extern "C" int rrtmgp_hip_compute_planck_source_##S(const rrtmgp_kdist_dev* kd, 
     RRTMGP_TRY(check_kd(kd, true, false));                                                        \
     const Kd<T> k = make_kd<T>(*kd);                                                               \
     const int64_t n = ncol * nlay * kd->ngpt;                                                      \
+    RRTMGP_TIMED("planck_kernel", as_stream(stream));                                              \
     hipLaunchKernelGGL(planck_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),    \
                        k, ncol, nlay, tlay, tlev, tsfc, sfc_lay, fmajor, jeta, tropo, jtemp,       \
                        jpress, sfc_src, lay_src, lev_src_inc, lev_src_dec, sfc_src_jac);           \
@@ -814,6 +821,7 @@ extern "C" int rrtmgp_hip_get_col_dry_##S(int64_t ncol, int64_t nlay, const T* v
 {                                                                                                  \
     const int64_t n = ncol * nlay;                                                                 \
     const int64_t scol = h2o_dim1 == 1 ? 0 : 1, slay = h2o_dim2 == 1 ? 0 : h2o_dim1;              \
+    RRTMGP_TIMED("col_dry_kernel", as_stream(stream));                                             \
     hipLaunchKernelGGL(col_dry_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
                        ncol, nlay, vmr_h2o, scol, slay, plev, col_dry);                            \
     RRTMGP_LAUNCHED("col_dry_kernel");                                                            \
@@ -825,6 +833,7 @@ extern "C" int rrtmgp_hip_fill_col_gas_##S(int64_t ncol, int64_t nlay, const rrt
     Gases<T> g;                                                                                    \
     RRTMGP_TRY(make_gases(gases, gases ? gases->ngas : 0, ncol, nlay, g));                        \
     const int64_t n = ncol * nlay;                                                                 \
+    RRTMGP_TIMED("fill_col_gas_kernel", as_stream(stream));                                        \
     hipLaunchKernelGGL(fill_col_gas_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                 \
                        as_stream(stream), ncol, nlay, int(gases->ngas), g, col_dry, col_gas);      \
     RRTMGP_LAUNCHED("fill_col_gas_kernel");                                                       \

@@ -268,6 +268,7 @@ extern "C" int rrtmgp_hip_cloud_optics_##S(int64_t ncol, int64_t nlay, int64_t n
     if (liq_nsteps < 2 || ice_nsteps < 2) return arg_error("cloud_optics: LUT needs >= 2 sizes");  \
     if (two_stream && (!ssa || !g)) return arg_error("cloud_optics: ssa/g required for 2str");     \
     const int64_t n = ncol * nlay;                                                                   \
+    RRTMGP_TIMED("cloud_optics_kernel", as_stream(stream));                                          \
     hipLaunchKernelGGL(cloud_optics_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),\
                        ncol, nlay, nbnd, liq_nsteps, liq_step_size, radliq_lwr, lut_extliq,         \
                        lut_ssaliq, lut_asyliq, ice_nsteps, ice_step_size, radice_lwr, lut_extice,   \
@@ -280,6 +281,7 @@ extern "C" int rrtmgp_hip_increment_##S(int64_t ncol, int64_t nlay, int64_t ngpt
     T* ssa_inout, T* g_inout, const T* tau_in, const T* ssa_in, const T* g_in, void* stream)        \
 {                                                                                                    \
     const int64_t n = ncol * nlay * ngpt;                                                            \
+    RRTMGP_TIMED("increment_kernel", as_stream(stream));                                             \
     hipLaunchKernelGGL(increment_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
                        n, tau_inout, ssa_inout, g_inout, tau_in, ssa_in, g_in);                     \
     RRTMGP_LAUNCHED("increment_kernel");                                                            \
@@ -290,6 +292,7 @@ extern "C" int rrtmgp_hip_inc_bybnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt
     const int* band_lims_gpt, void* stream)                                                          \
 {                                                                                                    \
     const int64_t n = ncol * nlay * ngpt;                                                            \
+    RRTMGP_TIMED("inc_bybnd_kernel", as_stream(stream));                                             \
     hipLaunchKernelGGL(inc_bybnd_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
                        ncol, nlay, ngpt, tau_inout, ssa_inout, g_inout, tau_in, ssa_in, g_in, nbnd, \
                        band_lims_gpt);                                                               \
@@ -300,6 +303,7 @@ extern "C" int rrtmgp_hip_delta_scale_2str_##S(int64_t ncol, int64_t nlay, int64
     T* ssa, T* g, void* stream)                                                                      \
 {                                                                                                    \
     const int64_t n = ncol * nlay * ngpt;                                                            \
+    RRTMGP_TIMED("delta_scale_kernel", as_stream(stream));                                           \
     hipLaunchKernelGGL(delta_scale_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), \
                        n, tau, ssa, g);                                                              \
     RRTMGP_LAUNCHED("delta_scale_kernel");                                                          \
@@ -309,6 +313,7 @@ extern "C" int rrtmgp_hip_sum_broadband_##S(int64_t ncol, int64_t nlev, int64_t 
     const T* gpt_flux, T* flux, void* stream)                                                        \
 {                                                                                                    \
     const int64_t n = ncol * nlev;                                                                   \
+    RRTMGP_TIMED("sum_broadband_kernel", as_stream(stream));                                         \
     hipLaunchKernelGGL(sum_broadband_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                  \
                        as_stream(stream), n, ngpt, gpt_flux, flux);                                 \
     RRTMGP_LAUNCHED("sum_broadband_kernel");                                                        \
@@ -318,6 +323,7 @@ extern "C" int rrtmgp_hip_net_broadband_##S(int64_t ncol, int64_t nlev, const T*
     const T* flux_up, T* flux_net, void* stream)                                                     \
 {                                                                                                    \
     const int64_t n = ncol * nlev;                                                                   \
+    RRTMGP_TIMED("net_kernel", as_stream(stream));                                                   \
     hipLaunchKernelGGL(net_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), n,      \
                        flux_dn, flux_up, flux_net);                                                  \
     RRTMGP_LAUNCHED("net_kernel");                                                                  \
@@ -327,6 +333,7 @@ extern "C" int rrtmgp_hip_sum_byband_##S(int64_t ncol, int64_t nlev, int64_t ngp
     const int* band_lims, const T* gpt_flux, T* bnd_flux, void* stream)                             \
 {                                                                                                    \
     const int64_t n = ncol * nlev * nbnd;                                                            \
+    RRTMGP_TIMED("sum_byband_kernel", as_stream(stream));                                            \
     hipLaunchKernelGGL(sum_byband_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),  \
                        ncol * nlev, nbnd, band_lims, gpt_flux, bnd_flux);                           \
     RRTMGP_LAUNCHED("sum_byband_kernel");                                                           \
@@ -338,6 +345,7 @@ extern "C" int rrtmgp_hip_net_byband_full_##S(int64_t ncol, int64_t nlev, int64_
     T* bnd_flux_net, void* stream)                                                                   \
 {                                                                                                    \
     const int64_t n = ncol * nlev * nbnd;                                                            \
+    RRTMGP_TIMED("net_byband_full_kernel", as_stream(stream));                                       \
     hipLaunchKernelGGL(net_byband_full_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                \
                        as_stream(stream), ncol * nlev, nbnd, band_lims, gpt_flux_dn, gpt_flux_up,   \
                        bnd_flux_net);                                                                \
@@ -349,6 +357,7 @@ extern "C" int rrtmgp_hip_heating_rate_##S(int64_t ncol, int64_t nlay, const T* 
     const T* flux_dn, const T* plev, T* heating_rate, void* stream)                                 \
 {                                                                                                    \
     const int64_t n = ncol * nlay;                                                                   \
+    RRTMGP_TIMED("heating_rate_kernel", as_stream(stream));                                          \
     hipLaunchKernelGGL(heating_rate_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                   \
                        as_stream(stream), ncol, nlay, flux_up, flux_dn, plev, heating_rate);        \
     RRTMGP_LAUNCHED("heating_rate_kernel");                                                         \
@@ -358,6 +367,7 @@ extern "C" int rrtmgp_hip_expand_and_transpose_##S(int64_t ncol, int64_t nbnd, i
     const int* band_lims_gpt, const T* arr_in, T* arr_out, void* stream)                            \
 {                                                                                                    \
     const int64_t n = ncol * ngpt;                                                                   \
+    RRTMGP_TIMED("expand_kernel", as_stream(stream));                                                \
     hipLaunchKernelGGL(expand_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),      \
                        ncol, nbnd, ngpt, band_lims_gpt, arr_in, arr_out);                           \
     RRTMGP_LAUNCHED("expand_kernel");                                                               \
@@ -367,6 +377,7 @@ extern "C" int rrtmgp_hip_apply_bc_##S(int64_t ncol, int64_t nlay, int64_t ngpt,
     const T* inc_flux, const T* factor, T* flux_dn, void* stream)                                   \
 {                                                                                                    \
     const int64_t n = ncol * ngpt;                                                                   \
+    RRTMGP_TIMED("apply_bc_kernel", as_stream(stream));                                              \
     hipLaunchKernelGGL(apply_bc_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),    \
                        ncol, nlay, ngpt, top_at_1, inc_flux, factor, flux_dn);                      \
     RRTMGP_LAUNCHED("apply_bc_kernel");                                                             \
@@ -376,6 +387,7 @@ extern "C" int rrtmgp_hip_spread_col_##S(int64_t ncol, int64_t ngpt, const T* sr
     void* stream)                                                                                    \
 {                                                                                                    \
     const int64_t n = ncol * ngpt;                                                                   \
+    RRTMGP_TIMED("spread_col_kernel", as_stream(stream));                                            \
     hipLaunchKernelGGL(spread_col_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),  \
                        ncol, ngpt, src, dst);                                                        \
     RRTMGP_LAUNCHED("spread_col_kernel");                                                           \
@@ -385,6 +397,7 @@ extern "C" int rrtmgp_hip_scale_by_col_##S(int64_t ncol, int64_t n2, T* arr, con
     void* stream)                                                                                    \
 {                                                                                                    \
     const int64_t n = ncol * n2;                                                                     \
+    RRTMGP_TIMED("scale_by_col_kernel", as_stream(stream));                                          \
     hipLaunchKernelGGL(scale_by_col_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                   \
                        as_stream(stream), ncol, n2, arr, factor);                                   \
     RRTMGP_LAUNCHED("scale_by_col_kernel");                                                         \
@@ -395,6 +408,7 @@ extern "C" int rrtmgp_hip_copy_cols_##S(const T* src, int64_t src_ncol, int64_t 
 {                                                                                                    \
     const int64_t n = ncopy * nrows;                                                                 \
     if (n == 0) return RRTMGP_OK;                                                                    \
+    RRTMGP_TIMED("copy_cols_kernel", as_stream(stream));                                             \
     hipLaunchKernelGGL(copy_cols_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
                        src, src_ncol, src_col0, dst, dst_ncol, dst_col0, ncopy, nrows);             \
     RRTMGP_LAUNCHED("copy_cols_kernel");                                                            \

@@ -204,6 +204,31 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
 }
 
 template<typename Real>
+__global__ void lw_secants_kernel(int64_t n, int64_t ncg, int64_t n_quad, int64_t max_pts,
+                                  const Real* __restrict__ gauss_Ds, Real* __restrict__ secants)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t imu = i / ncg;
+    secants[i] = gauss_Ds[imu + (n_quad - 1) * max_pts];
+}
+
+template<typename Real>
+int lw_secants(int64_t ncol, int64_t ngpt, int64_t n_quad, int64_t max_pts, const Real* gauss_Ds,
+               Real* secants, void* stream)
+{
+    if (ncol <= 0 || ngpt <= 0) return arg_error("lw_secants_array: empty extents");
+    if (n_quad < 1 || n_quad > max_pts) return arg_error("lw_secants_array: n_gauss_quad out of range");
+    if (!gauss_Ds || !secants) return arg_error("lw_secants_array: NULL array");
+    const int64_t n = ncol * ngpt * n_quad;
+    RRTMGP_TIMED("lw_secants_kernel", as_stream(stream));
+    hipLaunchKernelGGL(lw_secants_kernel<Real>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),
+                       n, ncol * ngpt, n_quad, max_pts, gauss_Ds, secants);
+    RRTMGP_LAUNCHED("lw_secants_kernel");
+    return RRTMGP_OK;
+}
+
+template<typename Real>
 __global__ void sum_gpt_kernel(int64_t ncv, int64_t ngpt, const Real* __restrict__ f, Real* __restrict__ out)
 {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -236,18 +261,26 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
     }
     if (do_jac) RRTMGP_TRY(scratch_alloc(&jac, size_t(ncv * ngpt), s));
     const dim3 grid(grid1(ncol, SV_BX), grid1(ngpt, SV_BY));
-    hipLaunchKernelGGL(lw_solver_kernel<Real>, grid, dim3(SV_BX, SV_BY), 0, s, ncol, nlay, ngpt, top_at_1,
-                       int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux, up, dn,
-                       sfc_src_jac, jac);
+    {
+        RRTMGP_TIMED("lw_solver_kernel", s);
+        hipLaunchKernelGGL(lw_solver_kernel<Real>, grid, dim3(SV_BX, SV_BY), 0, s, ncol, nlay, ngpt, top_at_1,
+                           int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux, up, dn,
+                           sfc_src_jac, jac);
+    }
     int rc = check_hip(hipGetLastError(), "lw_solver_kernel");
     if (rc == RRTMGP_OK && do_broadband)
     {
-        hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, up, bb_up);
-        hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, dn, bb_dn);
+        for (int i = 0; i < 2; ++i)
+        {
+            RRTMGP_TIMED("sum_gpt_kernel", s);
+            hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt,
+                               i == 0 ? up : dn, i == 0 ? bb_up : bb_dn);
+        }
         rc = check_hip(hipGetLastError(), "sum_gpt_kernel");
     }
     if (rc == RRTMGP_OK && do_jac)
     {
+        RRTMGP_TIMED("sum_gpt_kernel", s);
         hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, jac, flux_up_jac);
         rc = check_hip(hipGetLastError(), "sum_gpt_kernel");
     }
@@ -282,15 +315,22 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     RRTMGP_TRY(scratch_alloc(&SU, size_t(ng), s));
     RRTMGP_TRY(scratch_alloc(&SD, size_t(ng), s));
     const dim3 grid(grid1(ncol, SV_BX), grid1(ngpt, SV_BY));
-    hipLaunchKernelGGL(sw_solver_kernel<Real>, grid, dim3(SV_BX, SV_BY), 0, s, ncol, nlay, ngpt, top_at_1,
-                       tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir, has_dif_bc ? inc_dif : nullptr,
-                       up, dn, dr, R, T, SU, SD);
+    {
+        RRTMGP_TIMED("sw_solver_kernel", s);
+        hipLaunchKernelGGL(sw_solver_kernel<Real>, grid, dim3(SV_BX, SV_BY), 0, s, ncol, nlay, ngpt, top_at_1,
+                           tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir, has_dif_bc ? inc_dif : nullptr,
+                           up, dn, dr, R, T, SU, SD);
+    }
     int rc = check_hip(hipGetLastError(), "sw_solver_kernel");
     if (rc == RRTMGP_OK && do_broadband)
     {
-        hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, up, bb_up);
-        hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, dn, bb_dn);
-        hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, dr, bb_dir);
+        const Real* src[3] = {up, dn, dr};
+        Real* dst[3] = {bb_up, bb_dn, bb_dir};
+        for (int i = 0; i < 3; ++i)
+        {
+            RRTMGP_TIMED("sum_gpt_kernel", s);
+            hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, src[i], dst[i]);
+        }
         rc = check_hip(hipGetLastError(), "sum_gpt_kernel");
     }
     scratch_free(R, s); scratch_free(T, s); scratch_free(SU, s); scratch_free(SD, s);
@@ -325,5 +365,14 @@ extern "C" int rrtmgp_hip_sw_solver_2stream_##S(int64_t ncol, int64_t nlay, int6
                         do_broadband, broadband_up, broadband_dn, broadband_dir, stream);            \
 }
 
+#define RRTMGP_DEF_SECANTS(S, T)                                                                    \
+extern "C" int rrtmgp_hip_lw_secants_array_##S(int64_t ncol, int64_t ngpt, int64_t n_gauss_quad,  \
+    int64_t max_gauss_pts, const T* gauss_Ds, T* secants, void* stream)                             \
+{                                                                                                    \
+    return lw_secants<T>(ncol, ngpt, n_gauss_quad, max_gauss_pts, gauss_Ds, secants, stream);       \
+}
+
+RRTMGP_DEF_SECANTS(f32, float)
+RRTMGP_DEF_SECANTS(f64, double)
 RRTMGP_DEF_SOLVERS(f32, float)
 RRTMGP_DEF_SOLVERS(f64, double)

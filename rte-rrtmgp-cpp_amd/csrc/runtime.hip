@@ -1,7 +1,10 @@
 // Runtime helpers and error state of the C ABI (include/rrtmgp_hip.h).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.h"
 
@@ -19,6 +22,145 @@ int check_hip(hipError_t e, const char* what)
     if (e == hipSuccess) return RRTMGP_OK;
     g_last_error = std::string(what) + ": " + hipGetErrorString(e);
     return RRTMGP_ERR_HIP;
+}
+
+namespace
+{
+struct TimedLaunch { const char* name; hipEvent_t start, stop; };
+std::mutex g_timing_mutex;
+std::atomic<bool> g_timing_on{false};
+std::vector<TimedLaunch> g_timed;       // launches recorded since timing was switched on
+std::vector<hipEvent_t> g_event_pool;   // events of cleared records, reused
+struct KernelTotal { std::string name; int64_t launches; double total_ms; };
+std::vector<KernelTotal> g_totals;      // built by rrtmgp_hip_timing_collect
+
+hipEvent_t pooled_event()
+{
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    if (!g_event_pool.empty())
+    {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void clear_records()
+{
+    for (auto& t : g_timed) { g_event_pool.push_back(t.start); g_event_pool.push_back(t.stop); }
+    g_timed.clear();
+    g_totals.clear();
+}
+}  // namespace
+
+LaunchTimer::LaunchTimer(const char* name, hipStream_t stream) : name_(name), stream_(stream)
+{
+    if (!g_timing_on.load(std::memory_order_relaxed)) return;
+    start_ = pooled_event();
+    if (start_ && hipEventRecord(start_, stream_) != hipSuccess) start_ = nullptr;
+}
+
+LaunchTimer::~LaunchTimer()
+{
+    if (!start_) return;
+    hipEvent_t stop = pooled_event();
+    if (!stop) return;
+    (void)hipEventRecord(stop, stream_);
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    g_timed.push_back({name_, start_, stop});
+}
+
+namespace
+{
+// Caching allocator: blocks freed by their user go to a free list tagged with the device and
+// the stream they were freed on, and are handed out again only to requests on that same
+// device and stream (so stream order protects any reuse).  Nothing is returned to the driver
+// until rrtmgp_hip_cache_release or an out-of-memory retry.
+struct CachedBlock { void* ptr; size_t bytes; int device; hipStream_t stream; };
+std::mutex g_cache_mutex;
+std::vector<CachedBlock> g_cache_free;
+std::vector<CachedBlock> g_cache_live;
+
+size_t round_block(size_t bytes)
+{
+    const size_t q = bytes > (size_t(1) << 20) ? (size_t(2) << 20) : size_t(512);
+    return (bytes + q - 1) / q * q;
+}
+
+int release_free_blocks()
+{
+    std::vector<CachedBlock> blocks;
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mutex);
+        blocks.swap(g_cache_free);
+    }
+    if (blocks.empty()) return RRTMGP_OK;
+    int cur = 0;
+    RRTMGP_HIP(hipGetDevice(&cur));
+    for (const auto& b : blocks)
+    {
+        RRTMGP_HIP(hipSetDevice(b.device));
+        RRTMGP_HIP(hipDeviceSynchronize());
+        RRTMGP_HIP(hipFree(b.ptr));
+    }
+    RRTMGP_HIP(hipSetDevice(cur));
+    return RRTMGP_OK;
+}
+}  // namespace
+
+int cache_alloc(void** p, size_t bytes, hipStream_t stream)
+{
+    *p = nullptr;
+    if (bytes == 0) return RRTMGP_OK;
+    bytes = round_block(bytes);
+    int dev = 0;
+    RRTMGP_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mutex);
+        const size_t limit = std::max(2 * bytes, bytes + (size_t(4) << 20));
+        auto best = g_cache_free.end();
+        for (auto it = g_cache_free.begin(); it != g_cache_free.end(); ++it)
+            if (it->device == dev && it->stream == stream && it->bytes >= bytes && it->bytes <= limit &&
+                (best == g_cache_free.end() || it->bytes < best->bytes))
+                best = it;
+        if (best != g_cache_free.end())
+        {
+            *p = best->ptr;
+            g_cache_live.push_back(*best);
+            g_cache_free.erase(best);
+            return RRTMGP_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory)
+    {
+        (void)hipGetLastError();
+        RRTMGP_TRY(release_free_blocks());
+        e = hipMalloc(p, bytes);
+    }
+    RRTMGP_TRY(check_hip(e, "hipMalloc (cached scratch)"));
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    g_cache_live.push_back({*p, bytes, dev, stream});
+    return RRTMGP_OK;
+}
+
+int cache_free(void* p, hipStream_t stream)
+{
+    if (!p) return RRTMGP_OK;
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    for (auto it = g_cache_live.begin(); it != g_cache_live.end(); ++it)
+        if (it->ptr == p)
+        {
+            CachedBlock b = *it;
+            b.stream = stream;
+            g_cache_live.erase(it);
+            g_cache_free.push_back(b);
+            return RRTMGP_OK;
+        }
+    return arg_error("cache_free: pointer was not allocated by the cache");
 }
 
 int retain_pool_memory()
@@ -41,6 +183,68 @@ using namespace rrtmgp_hip;
 extern "C" {
 
 const char* rrtmgp_hip_last_error(void) { return g_last_error.c_str(); }
+
+int rrtmgp_hip_cache_malloc(void** ptr, size_t bytes, void* stream)
+{
+    if (!ptr) return arg_error("cache_malloc: NULL output");
+    return cache_alloc(ptr, bytes, as_stream(stream));
+}
+
+int rrtmgp_hip_cache_free(void* ptr, void* stream) { return cache_free(ptr, as_stream(stream)); }
+
+int rrtmgp_hip_cache_release(void) { return release_free_blocks(); }
+
+int rrtmgp_hip_cache_stats(size_t* live_bytes, size_t* cached_bytes)
+{
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    size_t l = 0, c = 0;
+    for (const auto& b : g_cache_live) l += b.bytes;
+    for (const auto& b : g_cache_free) c += b.bytes;
+    if (live_bytes) *live_bytes = l;
+    if (cached_bytes) *cached_bytes = c;
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_timing_enable(int on)
+{
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    if (on) clear_records();
+    g_timing_on.store(on != 0);
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_timing_collect(int* n_kernels)
+{
+    if (!n_kernels) return arg_error("timing_collect: NULL output");
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    g_totals.clear();
+    for (const auto& t : g_timed)
+    {
+        RRTMGP_HIP(hipEventSynchronize(t.stop));
+        float ms = 0.f;
+        RRTMGP_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+        auto it = std::find_if(g_totals.begin(), g_totals.end(),
+                               [&](const KernelTotal& k) { return k.name == t.name; });
+        if (it == g_totals.end()) g_totals.push_back({t.name, 1, double(ms)});
+        else { it->launches += 1; it->total_ms += ms; }
+    }
+    *n_kernels = int(g_totals.size());
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_timing_get(int i, char* name, int len, int64_t* launches, double* total_ms)
+{
+    std::lock_guard<std::mutex> lock(g_timing_mutex);
+    if (i < 0 || i >= int(g_totals.size())) return arg_error("timing_get: index out of range");
+    if (name && len > 0)
+    {
+        std::strncpy(name, g_totals[size_t(i)].name.c_str(), size_t(len - 1));
+        name[len - 1] = '\0';
+    }
+    if (launches) *launches = g_totals[size_t(i)].launches;
+    if (total_ms) *total_ms = g_totals[size_t(i)].total_ms;
+    return RRTMGP_OK;
+}
 
 int rrtmgp_hip_device_count(int* n)
 {
@@ -182,16 +386,19 @@ int count_outside(int64_t n, const Real* arr, Real lo, Real hi, int64_t* count, 
 {
     hipStream_t s = as_stream(stream);
     unsigned long long* d = nullptr;
-    RRTMGP_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(unsigned long long), s));
+    RRTMGP_TRY(scratch_alloc(&d, 1, s));
     RRTMGP_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s));
     const int check_hi = (hi == hi) ? 1 : 0;
     const unsigned blocks = unsigned(std::min<int64_t>(4096, (n + 255) / 256 + 1));
-    hipLaunchKernelGGL(count_outside_kernel<Real>, dim3(blocks), dim3(256), 0, s, n, arr, lo, hi, check_hi, d);
-    RRTMGP_LAUNCHED("count_outside_kernel");
+    {
+        RRTMGP_TIMED("count_outside_kernel", s);
+        hipLaunchKernelGGL(count_outside_kernel<Real>, dim3(blocks), dim3(256), 0, s, n, arr, lo, hi, check_hi, d);
+        RRTMGP_LAUNCHED("count_outside_kernel");
+    }
     unsigned long long h = 0;
     RRTMGP_HIP(hipMemcpyAsync(&h, d, sizeof(h), hipMemcpyDeviceToHost, s));
-    RRTMGP_HIP(hipFreeAsync(d, s));
     RRTMGP_HIP(hipStreamSynchronize(s));
+    scratch_free(d, s);
     *count = int64_t(h);
     return RRTMGP_OK;
 }

@@ -243,14 +243,14 @@ class Array_gpu
         {
             if (ncells == 0) { data_ptr = nullptr; return; }
             void* p = nullptr;
-            rrtmgp_cpp::hip_check(rrtmgp_hip_malloc(&p, bytes()));
+            rrtmgp_cpp::hip_check(rrtmgp_hip_cache_malloc(&p, bytes(), rrtmgp_cpp::stream()));
             data_ptr = static_cast<T*>(p);
             if (zero) rrtmgp_cpp::hip_check(rrtmgp_hip_memset(data_ptr, 0, bytes(), rrtmgp_cpp::stream()));
         }
 
         void release()
         {
-            if (owning && data_ptr) (void)rrtmgp_hip_free(data_ptr);
+            if (owning && data_ptr) (void)rrtmgp_hip_cache_free(data_ptr, rrtmgp_cpp::stream());
             data_ptr = nullptr;
         }
 

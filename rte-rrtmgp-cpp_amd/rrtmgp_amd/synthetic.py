@@ -390,60 +390,59 @@ class Atmosphere:
 def make_atmosphere(seed: int, ncol: int, nlay: int, cloudy: bool = False,
                     top_at_1: bool = False, nbnd_lw: int = 16, nbnd_sw: int = 14,
                     conservative_frac: float = 0.0) -> Atmosphere:
+    # Fields are drawn and computed as (nz, ncol) C-order arrays, whose transposes are the
+    # (ncol, nz) Fortran-order arrays of the driver layout without a copy.
     rng = np.random.Generator(np.random.PCG64(seed))
     nlev = nlay + 1
     p0 = np.exp(np.linspace(math.log(1.013e5), math.log(1.0), nlev))
-    p_lev = p0[None, :] * rng.uniform(0.99, 1.01, size=(ncol, nlev))
-    p_lev = np.clip(p_lev, 1.0, 1.1e5)
-    p_lay = np.sqrt(p_lev[:, :-1] * p_lev[:, 1:])
+    p_lev = rng.uniform(0.99, 1.01, size=(nlev, ncol))
+    p_lev *= p0[:, None]
+    np.clip(p_lev, 1.0, 1.1e5, out=p_lev)
+    p_lay = np.sqrt(p_lev[:-1] * p_lev[1:])
 
     h = 7.0
-    z_lev = h * np.log(1.013e5 / p_lev)
-    z_lay = h * np.log(1.013e5 / p_lay)
     t_sfc = rng.uniform(280.0, 310.0, size=ncol)
     z_trop = rng.uniform(10.0, 16.0, size=ncol)
 
-    def temp(z):
-        return t_sfc[:, None] - 6.5 * np.minimum(z, z_trop[:, None])
+    def temp(p):
+        z = np.log(1.013e5 / p)
+        z *= h
+        np.clip(z, 0.0, z_trop[None, :], out=z)      # max(z, 0), isothermal above z_trop
+        return t_sfc[None, :] - 6.5 * z
 
-    t_lev = temp(np.maximum(z_lev, 0.0))
-    t_lay = temp(np.maximum(z_lay, 0.0))
-    t_sfc_out = t_lev[:, 0].copy()
+    t_lev = temp(p_lev)
+    t_lay = temp(p_lay)
+    t_sfc_out = t_lev[0].copy()
 
-    h2o = np.maximum(1e-2 * np.exp(-np.maximum(z_lay, 0.0) / 2.0), 3e-6)
-    o3 = 8e-6 * np.exp(-0.5 * ((np.log(p_lay) - math.log(1000.0)) / 1.0) ** 2) + O3_BACKGROUND
-    vmr = {"h2o": h2o, "o3": o3, "co2": np.array([[400e-6]]), "ch4": np.array([[1.8e-6]]),
-           "n2o": np.array([[3.2e-7]])}
+    z_lay = np.maximum(h * np.log(1.013e5 / p_lay), 0.0)
+    h2o = np.maximum(1e-2 * np.exp(-z_lay / 2.0), 3e-6)
+    lnp = np.log(p_lay) - math.log(1000.0)
+    o3 = 8e-6 * np.exp(-0.5 * lnp * lnp) + O3_BACKGROUND
+    del z_lay, lnp
+    vmr = {"h2o": h2o.T, "o3": o3.T, "co2": fa([[400e-6]]), "ch4": fa([[1.8e-6]]),
+           "n2o": fa([[3.2e-7]])}
 
-    emis = np.full((nbnd_lw, ncol), 0.98)
-    alb = np.repeat(rng.uniform(0.06, 0.3, size=ncol)[None, :], nbnd_sw, axis=0)
+    emis = np.full((nbnd_lw, ncol), 0.98, order="F")
+    alb = fa(np.repeat(rng.uniform(0.06, 0.3, size=ncol)[None, :], nbnd_sw, axis=0))
     mu0 = rng.uniform(0.1, 1.0, size=ncol)
 
-    atm = Atmosphere(p_lay=p_lay, p_lev=p_lev, t_lay=t_lay, t_lev=t_lev, t_sfc=t_sfc_out,
+    atm = Atmosphere(p_lay=p_lay.T, p_lev=p_lev.T, t_lay=t_lay.T, t_lev=t_lev.T, t_sfc=t_sfc_out,
                      vmr=vmr, emis_sfc=emis, sfc_alb=alb, mu0=mu0, tsi=1360.0)
     if cloudy:
         liq_zone = (p_lay <= 900e2) & (p_lay >= 400e2)
         ice_zone = (p_lay <= 400e2) & (p_lay >= 150e2)
-        liq = liq_zone & (rng.random((ncol, nlay)) < 0.2)
-        ice = ice_zone & (rng.random((ncol, nlay)) < 0.2)
-        lwp = np.where(liq, 50.0 * np.exp(rng.normal(0.0, 1.0, (ncol, nlay))), 0.0)
-        iwp = np.where(ice, 20.0 * np.exp(rng.normal(0.0, 1.0, (ncol, nlay))), 0.0)
-        rel = np.where(liq, rng.uniform(4.0, 20.0, (ncol, nlay)), 0.0)
-        rei = np.where(ice, rng.uniform(20.0, 120.0, (ncol, nlay)), 0.0)
-        atm.lwp, atm.iwp, atm.rel, atm.rei = lwp, iwp, rel, rei
+        liq = liq_zone & (rng.random((nlay, ncol)) < 0.2)
+        ice = ice_zone & (rng.random((nlay, ncol)) < 0.2)
+        lwp = np.where(liq, 50.0 * np.exp(rng.normal(0.0, 1.0, (nlay, ncol))), 0.0)
+        iwp = np.where(ice, 20.0 * np.exp(rng.normal(0.0, 1.0, (nlay, ncol))), 0.0)
+        rel = np.where(liq, rng.uniform(4.0, 20.0, (nlay, ncol)), 0.0)
+        rei = np.where(ice, rng.uniform(20.0, 120.0, (nlay, ncol)), 0.0)
+        atm.lwp, atm.iwp, atm.rel, atm.rei = lwp.T, iwp.T, rel.T, rei.T
         if conservative_frac > 0.0:
             cl = liq | ice
-            atm.extra["conservative"] = cl & (rng.random((ncol, nlay)) < conservative_frac)
+            atm.extra["conservative"] = fa((cl & (rng.random((nlay, ncol)) < conservative_frac)).T)
     if top_at_1:
         atm = flip_vertical(atm)
-    # Fortran order everywhere.
-    for name in ("p_lay", "p_lev", "t_lay", "t_lev", "lwp", "iwp", "rel", "rei"):
-        v = getattr(atm, name)
-        if v is not None:
-            setattr(atm, name, fa(v))
-    atm.vmr = {k: fa(v) for k, v in atm.vmr.items()}
-    atm.emis_sfc, atm.sfc_alb = fa(atm.emis_sfc), fa(atm.sfc_alb)
-    atm.t_sfc, atm.mu0 = fa(atm.t_sfc), fa(atm.mu0)
     return atm
 
 
@@ -460,6 +459,21 @@ def flip_vertical(atm: Atmosphere) -> Atmosphere:
                       iwp=flip(atm.iwp), rel=flip(atm.rel), rei=flip(atm.rei),
                       top_at_1=not atm.top_at_1, extra=extra)
 
+
+def subset_columns(atm: Atmosphere, cols) -> Atmosphere:
+    """The columns ``cols`` (an index array or slice) of an atmosphere."""
+    def sub(a, axis=0):
+        if a is None:
+            return None
+        return fa(np.take(a, np.arange(atm.ncol)[cols], axis=axis))
+    vmr = {k: (v if v.shape[0] == 1 else sub(v)) for k, v in atm.vmr.items()}
+    extra = {k: (sub(v) if isinstance(v, np.ndarray) and v.ndim >= 1 and v.shape[0] == atm.ncol else v)
+             for k, v in atm.extra.items()}
+    return Atmosphere(p_lay=sub(atm.p_lay), p_lev=sub(atm.p_lev), t_lay=sub(atm.t_lay),
+                      t_lev=sub(atm.t_lev), t_sfc=sub(atm.t_sfc), vmr=vmr,
+                      emis_sfc=sub(atm.emis_sfc, 1), sfc_alb=sub(atm.sfc_alb, 1), mu0=sub(atm.mu0),
+                      tsi=atm.tsi, lwp=sub(atm.lwp), iwp=sub(atm.iwp), rel=sub(atm.rel),
+                      rei=sub(atm.rei), top_at_1=atm.top_at_1, extra=extra)
 
 # Seeds of the named workloads.
 SEED_KDIST_LW = 20260101

@@ -40,17 +40,20 @@ void Rte_lw_gpu::rte_lw(
     Array_gpu<Float,2> sfc_emis_gpt({ncol, ngpt});
     expand_and_transpose(optical_props, sfc_emis, sfc_emis_gpt);
 
+    // Gauss tables (max_gauss_pts = 4) as the reference keeps them (src_cuda/Rte_lw.cu:91-124):
+    // column nmus holds the secants / weights of an nmus-point quadrature.
     const int nmus = n_gauss_angles;
-    Array<Float,3> secants_h({ncol, ngpt, nmus});
+    Array<Float,2> ds_h({4, 4});
     Array<Float,1> weights_h({nmus});
+    for (int n = 0; n < 4; ++n)
+        for (int imu = 0; imu < 4; ++imu)
+            ds_h({imu + 1, n + 1}) = gauss_Ds[n][imu];
     for (int imu = 0; imu < nmus; ++imu)
-    {
         weights_h({imu + 1}) = gauss_wts[nmus - 1][imu];
-        std::fill(secants_h.v().begin() + size_t(imu) * ncol * ngpt,
-                  secants_h.v().begin() + size_t(imu + 1) * ncol * ngpt, gauss_Ds[nmus - 1][imu]);
-    }
-    Array_gpu<Float,3> secants(secants_h);
+    Array_gpu<Float,2> ds(ds_h);
     Array_gpu<Float,1> weights(weights_h);
+    Array_gpu<Float,3> secants({ncol, ngpt, nmus});
+    hip_check(K<Float>::lw_secants_array(ncol, ngpt, nmus, 4, ds.ptr(), secants.ptr(), stream()));
 
     // CPU semantics: a (ncol, nlev, 1) output array selects broadband fluxes (src/Rte_lw.cpp:137).
     const bool do_broadband = (gpt_flux_up.dim(3) == 1) && (ngpt > 1);

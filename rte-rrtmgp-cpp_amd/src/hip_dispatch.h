@@ -22,6 +22,7 @@ template<typename T> struct K;
     static constexpr auto delta_scale_2str = rrtmgp_hip_delta_scale_2str_##S; \
     static constexpr auto apply_bc = rrtmgp_hip_apply_bc_##S;            \
     static constexpr auto expand_and_transpose = rrtmgp_hip_expand_and_transpose_##S; \
+    static constexpr auto lw_secants_array = rrtmgp_hip_lw_secants_array_##S; \
     static constexpr auto lw_solver_noscat = rrtmgp_hip_lw_solver_noscat_##S; \
     static constexpr auto sw_solver_2stream = rrtmgp_hip_sw_solver_2stream_##S; \
     static constexpr auto sum_broadband = rrtmgp_hip_sum_broadband_##S;  \

@@ -354,3 +354,23 @@ def test_solve_sw_composed(O, okd_sw, syn, lut_sw):
     assert rel_err(dn.sum(2), a["flux_dn"]) < 1e-12
     assert rel_err(dr.sum(2), a["flux_dn_dir"], floor=1e-9) < 1e-12
     np.testing.assert_allclose(a["flux_dn"][:, -1], atm.mu0 * atm.tsi, rtol=1e-12)
+
+
+def test_solve_threads_identical(O, okd_lw, okd_sw, lut_lw, lut_sw, syn):
+    """The OpenMP block loop gives the serial loop's results bit for bit, errors included."""
+    atm = syn.make_atmosphere(41, 30, 40, cloudy=True)
+    tsi = np.full(atm.ncol, atm.tsi / okd_sw.scalar("tsi"))
+    prev = O.set_threads(1)
+    try:
+        a = O.solve_lw(okd_lw, atm, O.Cloud(lut_lw)), O.solve_sw(okd_sw, atm, O.Cloud(lut_sw), tsi_scaling=tsi)
+        O.set_threads(4)
+        b = O.solve_lw(okd_lw, atm, O.Cloud(lut_lw)), O.solve_sw(okd_sw, atm, O.Cloud(lut_sw), tsi_scaling=tsi)
+        for x, y in zip(a, b):
+            for n in x:
+                assert np.array_equal(x[n], y[n]), n
+        bad = syn.make_atmosphere(41, 30, 40)
+        bad.t_lay[17, 5] = 400.0
+        with pytest.raises(ValueError):
+            O.solve_lw(okd_lw, bad)
+    finally:
+        O.set_threads(prev)
