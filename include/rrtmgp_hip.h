@@ -305,6 +305,26 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_SECANTS)
         T* broadband_up, T* broadband_dn, T* broadband_dir, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
 
+/* Band-summed variants (no counterpart in the reference, whose drivers reduce materialised   */
+/* g-point fluxes with Fluxes_byband::reduce, src/Fluxes.cpp:142-197): the solvers sum the    */
+/* fluxes over the g-points of each band (band_lims_gpt (2, nbnd), 1-based, host memory) and  */
+/* write band fluxes (ncol, nlev, nbnd) and/or broadband fluxes (ncol, nlev); either set may  */
+/* be NULL.  The SW flux_dn outputs are total (diffuse + direct).                             */
+#define RRTMGP_DECL_SOLVERS_BND(S, T)                                                          \
+    int rrtmgp_hip_lw_solver_noscat_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,          \
+        int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,           \
+        const T* lay_source, const T* lev_source_inc, const T* lev_source_dec,                 \
+        const T* sfc_emis, const T* sfc_src, const T* inc_flux, int64_t nbnd,                  \
+        const int* band_lims_gpt, T* bnd_flux_up, T* bnd_flux_dn, T* broadband_up,             \
+        T* broadband_dn, void* stream);                                                         \
+    int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,         \
+        int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,                    \
+        const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,     \
+        const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,         \
+        T* bnd_flux_dn, T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir,   \
+        void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
+
 /* ------------------------------------------------------------------------------------------ */
 /* Flux reductions (Fluxes_kernels_cuda, include_kernels_cuda/fluxes_kernels_cuda.h:31-50;    */
 /* Fortran rte_sum_broadband / rte_net_broadband_precalc / sum_byband / net_byband_precalc,   */

@@ -1,21 +1,124 @@
 // Longwave (no-scattering, Gaussian quadrature) and shortwave (two-stream + adding) solvers.
 //
-// One thread owns one (column, g-point) and runs the layer recurrences of that profile
-// sequentially; lanes of a wavefront are consecutive columns, so each layer step of the
-// wavefront reads and writes one contiguous 256-byte row of the (ncol, nlay|nlev, ngpt)
-// arrays.  Arithmetic follows src_kernels/mo_rte_solver_kernels.F90 (lw_solver_noscat :58-260,
+// Arithmetic follows src_kernels/mo_rte_solver_kernels.F90 (lw_solver_noscat :58-260,
 // lw_source_noscat :616-653, transport :659-723, sw_solver_2stream :512-605,
 // sw_dif_and_source :996-1109, adding :1117-1227) with the Rdir/Tdir clamp of
 // src_kernels_cuda/rte_solver_kernels.cu:581-583.
+//
+// Layout of the work.  A work-group is 64 columns (the lanes of each wavefront) x up to 16
+// g-points of one band (one wavefront each).  Each thread runs the layer recurrences of one
+// (column, g-point); every layer step of a wavefront reads one contiguous 256-byte row of the
+// (ncol, nlay, ngpt) inputs.  Fluxes are summed over the g-points of the work-group in LDS
+// every K levels (a fixed order, so results are reproducible), giving per-band partial fluxes
+// without materialising (ncol, nlev, ngpt) g-point fluxes; a small kernel then forms band and
+// broadband fluxes.  Callers that ask for g-point fluxes get them written directly instead.
+//
+// Shortwave adding.  The reference runs the adding method as an albedo/source sweep up from
+// the surface followed by a flux sweep down, keeping per-layer R, T and sources in memory.
+// Here the same layer equations are solved two-sided: a sweep down from the top carries the
+// reflectance R_up and diffuse source D of the column above each level (and the direct
+// beam), storing them only every K levels; the sweep up from the surface carries the albedo
+// A and source S of the column below, replaying each K-layer segment from its checkpoint in
+// registers, and at every level the fluxes follow from both sides:
+//     dn = (D + R_up S) / (1 - R_up A),   up = A dn + S.
+// Per (column, g-point) the memory traffic is the inputs read twice plus a checkpoint per K
+// levels, instead of four layer arrays written and read back.
+//
+// Precision islands.  The two-stream layer coefficients (gamma, k, exponentials, R/T, the
+// 1/(1 - k^2 mu0^2) guard) and the adding denominators 1/(1 - R A) are evaluated in FP64 in
+// every build (with the FP64 k_min = 1e-12 and epsilon), as are the carried adding states;
+// the inputs, outputs and checkpoints keep the build's precision.
+#include <algorithm>
+#include <vector>
+
 #include "common.h"
 
 namespace rrtmgp_hip
 {
-constexpr int SV_BX = 64;   // columns per work-group
-constexpr int SV_BY = 4;    // g-points per work-group
+constexpr int SB_COLS = 64;     // columns per work-group: the lanes of each wavefront
+constexpr int SB_GPTS = 16;     // g-points per work-group: one wavefront each, within one band
+constexpr int MAX_GROUPS = 256;
 
-template<typename Real>
-__global__ __launch_bounds__(SV_BX * SV_BY)
+// Segment lengths: levels per LDS reduction round (LW, SW) and the SW checkpoint spacing, sized
+// so that the SW segment registers fit the 128 VGPRs of a 1024-thread work-group without spills.
+template<typename Real> struct LwSeg;
+template<> struct LwSeg<float> { static constexpr int K = 16; };
+template<> struct LwSeg<double> { static constexpr int K = 8; };
+template<typename Real> struct SwSeg;
+template<> struct SwSeg<float> { static constexpr int K = 7; };
+template<> struct SwSeg<double> { static constexpr int K = 4; };
+
+// G-point groups: consecutive g-points of one band, at most SB_GPTS each.
+struct Groups
+{
+    int n;
+    int start[MAX_GROUPS];
+    int count[MAX_GROUPS];
+    int band[MAX_GROUPS];
+};
+
+// Groups from band limits (2, nbnd), 1-based inclusive; without band limits the g-points are cut
+// into consecutive chunks (enough for broadband sums).
+int make_groups(int64_t ngpt, int64_t nbnd, const int* band_lims, Groups& g)
+{
+    g.n = 0;
+    auto add = [&](int s, int e, int b) {
+        for (int x = s; x < e; x += SB_GPTS)
+        {
+            if (g.n == MAX_GROUPS) return false;
+            g.start[g.n] = x;
+            g.count[g.n] = std::min(SB_GPTS, e - x);
+            g.band[g.n] = b;
+            ++g.n;
+        }
+        return true;
+    };
+    if (band_lims)
+    {
+        int expect = 1;
+        for (int64_t b = 0; b < nbnd; ++b)
+        {
+            const int s = band_lims[2 * b], e = band_lims[2 * b + 1];
+            if (s != expect || e < s || e > ngpt) return arg_error("solver: band limits do not tile the g-points");
+            if (!add(s - 1, e, int(b))) return arg_error("solver: too many g-point groups");
+            expect = e + 1;
+        }
+        if (expect != ngpt + 1) return arg_error("solver: band limits do not cover all g-points");
+    }
+    else if (!add(0, int(ngpt), 0))
+        return arg_error("solver: too many g-point groups");
+    return RRTMGP_OK;
+}
+
+// Writes the LDS slots kk < nslots, components c < NC, summed over the group's g-points, to
+// out[c] at level lev_of(kk) (skipped when negative).
+// red layout: ((kk * NC + c) * SB_GPTS + w) * SB_COLS + lane.
+template<typename Real, int NC, typename LevOf>
+__device__ __forceinline__ void write_group_sums(const Real* red, int nslots, int gcount, int64_t col0,
+                                                 int64_t ncol, int64_t nlev, int gi, Real* const (&out)[NC],
+                                                 LevOf lev_of)
+{
+    const int tid = threadIdx.y * SB_COLS + threadIdx.x;
+    const int items = nslots * NC * SB_COLS;
+    for (int it = tid; it < items; it += SB_COLS * SB_GPTS)
+    {
+        const int lane = it % SB_COLS;
+        const int rest = it / SB_COLS;
+        const int c = rest % NC, kk = rest / NC;
+        const int64_t icol = col0 + lane;
+        const int64_t lev = lev_of(kk);
+        if (icol >= ncol || lev < 0) continue;
+        const Real* r = red + (int64_t(kk) * NC + c) * SB_GPTS * SB_COLS + lane;
+        Real s = r[0];
+        for (int w = 1; w < gcount; ++w) s += r[w * SB_COLS];
+        out[c][icol + ncol * (lev + nlev * gi)] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Longwave: no scattering, linear-in-tau source, nmus Gaussian angles.
+template<typename Real, bool JAC>
+__global__ __launch_bounds__(SB_COLS * SB_GPTS)
 void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int nmus,
                       const Real* __restrict__ secants, const Real* __restrict__ weights,
                       const Real* __restrict__ tau, const Real* __restrict__ lay_src,
@@ -23,184 +126,385 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
                       const Real* __restrict__ emis, const Real* __restrict__ sfc_src,
                       const Real* __restrict__ inc_flux, Real* __restrict__ flux_up,
                       Real* __restrict__ flux_dn, const Real* __restrict__ sfc_src_jac,
-                      Real* __restrict__ jac)
+                      Groups grp, Real* __restrict__ grp_up, Real* __restrict__ grp_dn,
+                      Real* __restrict__ grp_jac)
 {
-    const int64_t icol = int64_t(blockIdx.x) * SV_BX + threadIdx.x;
-    const int64_t igpt = int64_t(blockIdx.y) * SV_BY + threadIdx.y;
-    if (icol >= ncol || igpt >= ngpt) return;
-    const int64_t nlev = nlay + 1;
-    const int64_t c2 = icol + ncol * igpt;
-    const int64_t lb = icol + ncol * nlay * igpt;     // layer (ilay) offset: + ncol*ilay
-    const int64_t vb = icol + ncol * nlev * igpt;     // level offset
-    const Real* lev_dn = top_at_1 ? lev_inc : lev_dec;
-    const Real* lev_up = top_at_1 ? lev_dec : lev_inc;
-    const Real tau_thresh = sqrt(Limits<Real>::eps);
-    const Real e = emis[c2];
-    const Real ssrc = sfc_src[c2];
-    const Real pi = Real(3.14159265358979323846);
-    const int64_t top_lev = top_at_1 ? 0 : nlay;
-    const int64_t sfc_lev = top_at_1 ? nlay : 0;
+    constexpr int K = LwSeg<Real>::K;
+    constexpr int NC = JAC ? 2 : 1;
+    __shared__ Real red[K * NC * SB_GPTS * SB_COLS];
 
-    for (int imu = 0; imu < nmus; ++imu)
-    {
-        const Real D = secants[c2 + ncol * ngpt * imu];
-        const Real fac = Real(2) * pi * weights[imu];
-        auto put = [&](Real* arr, int64_t lev, Real v) {
-            const int64_t o = vb + ncol * lev;
-            arr[o] = (imu == 0) ? v : arr[o] + v;
-        };
-        Real I = (inc_flux ? inc_flux[c2] : Real(0)) / fac;
-        put(flux_dn, top_lev, fac * I);
-        for (int64_t j = 0; j < nlay; ++j)
-        {
-            const int64_t ilay = top_at_1 ? j : nlay - 1 - j;
-            const int64_t l = lb + ncol * ilay;
-            const Real t = tau[l] * D;
-            const Real T = exp(-t);
-            const Real fact = (t > tau_thresh) ? (Real(1) - T) / t - T : t * (Real(0.5) - Real(1. / 3.) * t);
-            const Real ld = lev_dn[l];
-            const Real src = (Real(1) - T) * ld + Real(2) * fact * (lay_src[l] - ld);
-            I = T * I + src;
-            put(flux_dn, top_at_1 ? ilay + 1 : ilay, fac * I);
-        }
-        Real Iu = I * (Real(1) - e) + e * ssrc;
-        put(flux_up, sfc_lev, fac * Iu);
-        Real Ij = Real(0);
-        if (jac)
-        {
-            Ij = e * sfc_src_jac[c2];
-            put(jac, sfc_lev, fac * Ij);
-        }
-        for (int64_t j = 0; j < nlay; ++j)
-        {
-            const int64_t ilay = top_at_1 ? nlay - 1 - j : j;
-            const int64_t l = lb + ncol * ilay;
-            const Real t = tau[l] * D;
-            const Real T = exp(-t);
-            const Real fact = (t > tau_thresh) ? (Real(1) - T) / t - T : t * (Real(0.5) - Real(1. / 3.) * t);
-            const Real lu = lev_up[l];
-            const Real src = (Real(1) - T) * lu + Real(2) * fact * (lay_src[l] - lu);
-            Iu = T * Iu + src;
-            const int64_t lev = top_at_1 ? ilay : ilay + 1;
-            put(flux_up, lev, fac * Iu);
-            if (jac)
-            {
-                Ij = T * Ij;
-                put(jac, lev, fac * Ij);
-            }
-        }
-    }
-}
-
-// Two-stream layer properties of sw_dif_and_source (F90:1043-1104) for one layer.
-template<typename Real>
-__device__ __forceinline__ void sw_two_stream(Real tau, Real w0, Real g, Real mu0, Real& Rdif, Real& Tdif,
-                                              Real& Rdir, Real& Tdir, Real& Tnoscat)
-{
-    const Real eps = Limits<Real>::eps;
-    const Real gamma1 = (Real(8) - w0 * (Real(5) + Real(3) * g)) * Real(.25);
-    const Real gamma2 = Real(3) * (w0 * (Real(1) - g)) * Real(.25);
-    const Real gamma3 = (Real(2) - Real(3) * mu0 * g) * Real(.25);
-    const Real gamma4 = Real(1) - gamma3;
-    const Real alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
-    const Real alpha2 = gamma1 * gamma3 + gamma2 * gamma4;
-    const Real k = sqrt(max((gamma1 - gamma2) * (gamma1 + gamma2), Limits<Real>::k_min));
-    const Real k_mu = k * mu0, k_gamma3 = k * gamma3, k_gamma4 = k * gamma4;
-    const Real e1 = exp(-tau * k);
-    const Real e2 = e1 * e1;
-    const Real RT = Real(1) / (k * (Real(1) + e2) + gamma1 * (Real(1) - e2));
-    Rdif = RT * gamma2 * (Real(1) - e2);
-    Tdif = RT * Real(2) * k * e1;
-    const Real d = Real(1) - k_mu * k_mu;
-    const Real denom = (fabs(d) >= eps) ? d : eps;
-    const Real RT2 = w0 * RT / denom;
-    Tnoscat = exp(-tau / mu0);
-    Real rd = RT2 * ((Real(1) - k_mu) * (alpha2 + k_gamma3) - (Real(1) + k_mu) * (alpha2 - k_gamma3) * e2 -
-                     Real(2) * (k_gamma3 - alpha2 * k_mu) * e1 * Tnoscat);
-    Real td = -RT2 * ((Real(1) + k_mu) * (alpha1 + k_gamma4) * Tnoscat -
-                      (Real(1) - k_mu) * (alpha1 - k_gamma4) * e2 * Tnoscat -
-                      Real(2) * (k_gamma4 + alpha1 * k_mu) * e1);
-    rd = max(eps, min(rd, Real(1) - Tnoscat));
-    td = max(eps, min(td, Real(1) - Tnoscat - rd));
-    Rdir = rd;
-    Tdir = td;
-}
-
-// Shortwave: direct beam + layer properties (top -> surface), adding up-sweep (surface -> top),
-// flux down-sweep (top -> surface).  R, T, SU, SD are per-layer scratch arrays; the level
-// albedo and source of the up-sweep are parked in flux_up / flux_dn until the down-sweep
-// overwrites them with the fluxes.
-template<typename Real>
-__global__ __launch_bounds__(SV_BX * SV_BY)
-void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
-                      const Real* __restrict__ tau, const Real* __restrict__ ssa,
-                      const Real* __restrict__ g, const Real* __restrict__ mu0,
-                      const Real* __restrict__ alb_dir, const Real* __restrict__ alb_dif,
-                      const Real* __restrict__ inc_dir, const Real* __restrict__ inc_dif,
-                      Real* __restrict__ flux_up, Real* __restrict__ flux_dn, Real* __restrict__ flux_dir,
-                      Real* __restrict__ R, Real* __restrict__ T, Real* __restrict__ SU, Real* __restrict__ SD)
-{
-    const int64_t icol = int64_t(blockIdx.x) * SV_BX + threadIdx.x;
-    const int64_t igpt = int64_t(blockIdx.y) * SV_BY + threadIdx.y;
-    if (icol >= ncol || igpt >= ngpt) return;
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int gi = blockIdx.y;
+    const int gcount = grp.count[gi];
+    const int64_t col0 = int64_t(blockIdx.x) * SB_COLS;
+    const int64_t icol = col0 + lane;
+    const int64_t igpt = grp.start[gi] + w;
+    const bool active = icol < ncol && w < gcount;
+    const bool reduce = grp_dn != nullptr;
     const int64_t nlev = nlay + 1;
     const int64_t c2 = icol + ncol * igpt;
     const int64_t lb = icol + ncol * nlay * igpt;
     const int64_t vb = icol + ncol * nlev * igpt;
-    const Real m0 = mu0[icol];
-    const int64_t top_lev = top_at_1 ? 0 : nlay;
-    const int64_t sfc_lev = top_at_1 ? nlay : 0;
+    const Real* lev_dn = top_at_1 ? lev_inc : lev_dec;
+    const Real* lev_up = top_at_1 ? lev_dec : lev_inc;
+    const Real tau_thresh = sqrt(Limits<Real>::eps);
+    const Real pi = Real(3.14159265358979323846);
 
-    Real dir = inc_dir[c2] * m0;
-    flux_dir[vb + ncol * top_lev] = dir;
-    for (int64_t j = 0; j < nlay; ++j)
+    Real D[4], fac[4], I[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
     {
-        const int64_t ilay = top_at_1 ? j : nlay - 1 - j;
-        const int64_t l = lb + ncol * ilay;
-        Real rd, td, rb, tb, tn;
-        sw_two_stream(tau[l], ssa[l], g[l], m0, rd, td, rb, tb, tn);
-        R[l] = rd; T[l] = td;
-        SU[l] = rb * dir;
-        SD[l] = tb * dir;
-        dir = tn * dir;
-        flux_dir[vb + ncol * (top_at_1 ? ilay + 1 : ilay)] = dir;
+        D[mu] = Real(0); fac[mu] = Real(0); I[mu] = Real(0);
+        if (active && mu < nmus)
+        {
+            D[mu] = secants[c2 + ncol * ngpt * mu];
+            fac[mu] = Real(2) * pi * weights[mu];
+            I[mu] = (inc_flux ? inc_flux[c2] : Real(0)) / fac[mu];
+        }
     }
-    // adding (SH08): albedo and source of the column below each level
-    Real alb = alb_dif[c2];
-    Real src = dir * alb_dir[c2];
-    flux_up[vb + ncol * sfc_lev] = alb;
-    flux_dn[vb + ncol * sfc_lev] = src;
-    for (int64_t j = 0; j < nlay; ++j)
+
+    // Down sweep: level j counted from the top (0 = top boundary).
+    for (int64_t s0 = 0; s0 <= nlay; s0 += K)
     {
-        const int64_t ilay = top_at_1 ? nlay - 1 - j : j;
-        const int64_t l = lb + ncol * ilay;
-        const Real rd = R[l], td = T[l];
-        const Real denom = Real(1) / (Real(1) - rd * alb);
-        const Real alb_new = rd + td * td * alb * denom;
-        src = SU[l] + td * denom * (src + alb * SD[l]);
-        alb = alb_new;
-        SU[l] = denom;
-        const int64_t lev = top_at_1 ? ilay : ilay + 1;
-        flux_up[vb + ncol * lev] = alb;
-        flux_dn[vb + ncol * lev] = src;
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+        {
+            const int64_t j = s0 + kk;
+            Real f = Real(0);
+            if (active && j <= nlay)
+            {
+                if (j > 0)
+                {
+                    const int64_t ilay = top_at_1 ? j - 1 : nlay - j;
+                    const int64_t l = lb + ncol * ilay;
+                    const Real tv = tau[l], ls = lay_src[l], ld = lev_dn[l];
+#pragma unroll
+                    for (int mu = 0; mu < 4; ++mu)
+                        if (mu < nmus)
+                        {
+                            const Real t = tv * D[mu];
+                            const Real T = exp(-t);
+                            const Real fact = (t > tau_thresh) ? (Real(1) - T) / t - T
+                                                               : t * (Real(0.5) - Real(1. / 3.) * t);
+                            I[mu] = T * I[mu] + ((Real(1) - T) * ld + Real(2) * fact * (ls - ld));
+                        }
+                }
+#pragma unroll
+                for (int mu = 0; mu < 4; ++mu)
+                    if (mu < nmus) f = (mu == 0) ? fac[0] * I[0] : f + fac[mu] * I[mu];
+                if (flux_dn) flux_dn[vb + ncol * (top_at_1 ? j : nlay - j)] = f;
+            }
+            if (reduce) red[(kk * NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
+        }
+        if (reduce)
+        {
+            __syncthreads();
+            Real* const outs[1] = {grp_dn};
+            const int nsl = int(std::min<int64_t>(K, nlay + 1 - s0));
+            write_group_sums<Real, 1>(red, nsl, gcount, col0, ncol, nlev, gi, outs, [&](int kk) -> int64_t {
+                const int64_t j = s0 + kk;
+                return top_at_1 ? j : nlay - j;
+            });
+            __syncthreads();
+        }
     }
-    Real dn = inc_dif ? inc_dif[c2] : Real(0);
+
+    // Surface: reflection of the downward intensity plus emission; then the up sweep with
+    // level j counted from the surface (0 = surface).
+    Real Ij[4];
     {
-        const int64_t o = vb + ncol * top_lev;
-        const Real up = dn * flux_up[o] + flux_dn[o];
-        flux_up[o] = up;
-        flux_dn[o] = dn + flux_dir[o];
+        const Real e = active ? emis[c2] : Real(0);
+        const Real ss = active ? sfc_src[c2] : Real(0);
+        const Real sj = (active && JAC) ? sfc_src_jac[c2] : Real(0);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu)
+        {
+            I[mu] = I[mu] * (Real(1) - e) + e * ss;
+            Ij[mu] = e * sj;
+        }
     }
-    for (int64_t j = 0; j < nlay; ++j)
+    for (int64_t s0 = 0; s0 <= nlay; s0 += K)
     {
-        const int64_t ilay = top_at_1 ? j : nlay - 1 - j;
-        const int64_t l = lb + ncol * ilay;
-        const int64_t o = vb + ncol * (top_at_1 ? ilay + 1 : ilay);
-        const Real a_b = flux_up[o], s_b = flux_dn[o];
-        dn = (T[l] * dn + R[l] * s_b + SD[l]) * SU[l];
-        flux_up[o] = dn * a_b + s_b;
-        flux_dn[o] = dn + flux_dir[o];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+        {
+            const int64_t j = s0 + kk;
+            Real f = Real(0), fj = Real(0);
+            if (active && j <= nlay)
+            {
+                if (j > 0)
+                {
+                    const int64_t ilay = top_at_1 ? nlay - j : j - 1;
+                    const int64_t l = lb + ncol * ilay;
+                    const Real tv = tau[l], ls = lay_src[l], lu = lev_up[l];
+#pragma unroll
+                    for (int mu = 0; mu < 4; ++mu)
+                        if (mu < nmus)
+                        {
+                            const Real t = tv * D[mu];
+                            const Real T = exp(-t);
+                            const Real fact = (t > tau_thresh) ? (Real(1) - T) / t - T
+                                                               : t * (Real(0.5) - Real(1. / 3.) * t);
+                            I[mu] = T * I[mu] + ((Real(1) - T) * lu + Real(2) * fact * (ls - lu));
+                            if (JAC) Ij[mu] = T * Ij[mu];
+                        }
+                }
+#pragma unroll
+                for (int mu = 0; mu < 4; ++mu)
+                    if (mu < nmus)
+                    {
+                        f = (mu == 0) ? fac[0] * I[0] : f + fac[mu] * I[mu];
+                        if (JAC) fj = (mu == 0) ? fac[0] * Ij[0] : fj + fac[mu] * Ij[mu];
+                    }
+                if (flux_up) flux_up[vb + ncol * (top_at_1 ? nlay - j : j)] = f;
+            }
+            if (reduce)
+            {
+                red[(kk * NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
+                if (JAC) red[(kk * NC + 1) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = fj;
+            }
+        }
+        if (reduce)
+        {
+            __syncthreads();
+            const int nsl = int(std::min<int64_t>(K, nlay + 1 - s0));
+            auto lev_of = [&](int kk) -> int64_t {
+                const int64_t j = s0 + kk;
+                return top_at_1 ? nlay - j : j;
+            };
+            if constexpr (JAC)
+            {
+                Real* const outs[2] = {grp_up, grp_jac};
+                write_group_sums<Real, 2>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+            }
+            else
+            {
+                Real* const outs[1] = {grp_up};
+                write_group_sums<Real, 1>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+            }
+            __syncthreads();
+        }
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shortwave two-stream layer properties (sw_dif_and_source, F90:1043-1104; clamp .cu:581-583),
+// evaluated in FP64.
+__device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, double mu0, double& Rdif,
+                                              double& Tdif, double& Rdir, double& Tdir, double& Tnoscat)
+{
+    const double eps = Limits<double>::eps;
+    const double gamma1 = (8. - w0 * (5. + 3. * g)) * .25;
+    const double gamma2 = 3. * (w0 * (1. - g)) * .25;
+    const double gamma3 = (2. - 3. * mu0 * g) * .25;
+    const double gamma4 = 1. - gamma3;
+    const double alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
+    const double alpha2 = gamma1 * gamma3 + gamma2 * gamma4;
+    const double k = sqrt(fmax((gamma1 - gamma2) * (gamma1 + gamma2), Limits<double>::k_min));
+    const double k_mu = k * mu0, k_gamma3 = k * gamma3, k_gamma4 = k * gamma4;
+    const double e1 = exp(-tau * k);
+    const double e2 = e1 * e1;
+    const double RT = 1. / (k * (1. + e2) + gamma1 * (1. - e2));
+    Rdif = RT * gamma2 * (1. - e2);
+    Tdif = RT * 2. * k * e1;
+    const double d = 1. - k_mu * k_mu;
+    const double denom = (fabs(d) >= eps) ? d : eps;
+    const double RT2 = w0 * RT / denom;
+    Tnoscat = exp(-tau / mu0);
+    double rd = RT2 * ((1. - k_mu) * (alpha2 + k_gamma3) - (1. + k_mu) * (alpha2 - k_gamma3) * e2 -
+                       2. * (k_gamma3 - alpha2 * k_mu) * e1 * Tnoscat);
+    double td = -RT2 * ((1. + k_mu) * (alpha1 + k_gamma4) * Tnoscat -
+                        (1. - k_mu) * (alpha1 - k_gamma4) * e2 * Tnoscat -
+                        2. * (k_gamma4 + alpha1 * k_mu) * e1);
+    rd = fmax(eps, fmin(rd, 1. - Tnoscat));
+    td = fmax(eps, fmin(td, 1. - Tnoscat - rd));
+    Rdir = rd;
+    Tdir = td;
+}
+
+// One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
+struct LayerSW { double R, T, SU, SD, Tn; };
+
+__device__ __forceinline__ LayerSW sw_layer(double tau, double w0, double g, double mu0, double dir)
+{
+    double Rdif, Tdif, Rdir, Tdir, Tn;
+    sw_two_stream(tau, w0, g, mu0, Rdif, Tdif, Rdir, Tdir, Tn);
+    return {Rdif, Tdif, Rdir * dir, Tdir * dir, Tn};
+}
+
+// Column above the next level from the column above this one and the layer between.
+__device__ __forceinline__ void above_update(const LayerSW& L, double& R_up, double& D)
+{
+    const double den = 1. / (1. - L.R * R_up);
+    D = L.SD + L.T * (D + R_up * L.SU) * den;
+    R_up = L.R + L.T * L.T * R_up * den;
+}
+
+template<typename Real>
+__global__ __launch_bounds__(SB_COLS * SB_GPTS)
+void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
+                      const Real* __restrict__ tau, const Real* __restrict__ ssa,
+                      const Real* __restrict__ gasy, const Real* __restrict__ mu0,
+                      const Real* __restrict__ alb_dir, const Real* __restrict__ alb_dif,
+                      const Real* __restrict__ inc_dir, const Real* __restrict__ inc_dif,
+                      Real* __restrict__ flux_up, Real* __restrict__ flux_dn, Real* __restrict__ flux_dir,
+                      Real* __restrict__ ckpt, Groups grp, Real* __restrict__ grp_up,
+                      Real* __restrict__ grp_dn, Real* __restrict__ grp_dir)
+{
+    constexpr int K = SwSeg<Real>::K;
+    constexpr int NC = 3;
+    // Slots 0..K-1: the levels of a segment; slot K: the surface level.
+    __shared__ Real red[(K + 1) * NC * SB_GPTS * SB_COLS];
+
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int gi = blockIdx.y;
+    const int gcount = grp.count[gi];
+    const int64_t col0 = int64_t(blockIdx.x) * SB_COLS;
+    const int64_t icol = col0 + lane;
+    const int64_t igpt = grp.start[gi] + w;
+    const bool active = icol < ncol && w < gcount;
+    const bool reduce = grp_dn != nullptr;
+    const int64_t nlev = nlay + 1;
+    const int64_t nseg = (nlay + K - 1) / K;
+    const int64_t c2 = icol + ncol * igpt;
+    const int64_t lb = icol + ncol * nlay * igpt;
+    const int64_t vb = icol + ncol * nlev * igpt;
+    // Checkpoint q of segment s at ckpt[((s * 3 + q) * ngpt + igpt) * ncol + icol].
+    auto ck = [&](int64_t s, int q) -> Real& { return ckpt[((s * 3 + q) * ngpt + igpt) * ncol + icol]; };
+    auto lay_of = [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - 1 - j; };   // layer j from the top
+    auto lev_of = [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - j; };       // level j from the top
+
+    const double m0 = active ? double(mu0[icol]) : 1.;
+    double R_up = 0., D = 0., dir = 0.;
+    if (active)
+    {
+        dir = double(inc_dir[c2]) * m0;
+        D = inc_dif ? double(inc_dif[c2]) : 0.;
+        // Sweep down: the column above every level; checkpoints at the top of each segment.
+        for (int64_t j = 0; j < nlay; ++j)
+        {
+            if (j % K == 0)
+            {
+                const int64_t s = j / K;
+                ck(s, 0) = Real(R_up); ck(s, 1) = Real(D); ck(s, 2) = Real(dir);
+            }
+            const int64_t l = lb + ncol * lay_of(j);
+            const LayerSW L = sw_layer(tau[l], ssa[l], gasy[l], m0, dir);
+            above_update(L, R_up, D);
+            dir *= L.Tn;
+        }
+    }
+
+    // Surface.
+    double A = active ? double(alb_dif[c2]) : 0.;
+    double S = dir * (active ? double(alb_dir[c2]) : 0.);
+    {
+        const double dn = (D + R_up * S) / (1. - R_up * A);
+        const double up = A * dn + S;
+        if (active && flux_up)
+        {
+            const int64_t o = vb + ncol * lev_of(nlay);
+            flux_up[o] = Real(up); flux_dn[o] = Real(dn + dir); flux_dir[o] = Real(dir);
+        }
+        if (reduce)
+        {
+            red[((K * NC + 0) * SB_GPTS + w) * SB_COLS + lane] = Real(up);
+            red[((K * NC + 1) * SB_GPTS + w) * SB_COLS + lane] = Real(dn + dir);
+            red[((K * NC + 2) * SB_GPTS + w) * SB_COLS + lane] = Real(dir);
+        }
+    }
+
+    // Sweep up, one segment at a time from the bottom.
+    for (int64_t s = nseg - 1; s >= 0; --s)
+    {
+        const int64_t j0 = s * K;
+        const int nk = int(std::min<int64_t>(K, nlay - j0));
+        if (active)
+        {
+            Real sR[K], sT[K], sSU[K], sSD[K], sRup[K], sD[K], sDir[K];
+            double r_up = double(ck(s, 0)), d = double(ck(s, 1)), dr = double(ck(s, 2));
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+                if (kk < nk)
+                {
+                    sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(dr);
+                    const int64_t l = lb + ncol * lay_of(j0 + kk);
+                    const LayerSW L = sw_layer(tau[l], ssa[l], gasy[l], m0, dr);
+                    sR[kk] = Real(L.R); sT[kk] = Real(L.T); sSU[kk] = Real(L.SU); sSD[kk] = Real(L.SD);
+                    above_update(L, r_up, d);
+                    dr *= L.Tn;
+                }
+#pragma unroll
+            for (int kk = K - 1; kk >= 0; --kk)
+                if (kk < nk)
+                {
+                    const double R = sR[kk], T = sT[kk];
+                    const double den = 1. / (1. - R * A);
+                    S = double(sSU[kk]) + T * (S + A * double(sSD[kk])) * den;
+                    A = R + T * T * A * den;
+                    const double ru = sRup[kk];
+                    const double dn = (double(sD[kk]) + ru * S) / (1. - ru * A);
+                    const double up = A * dn + S;
+                    const double dd = sDir[kk];
+                    if (flux_up)
+                    {
+                        const int64_t o = vb + ncol * lev_of(j0 + kk);
+                        flux_up[o] = Real(up); flux_dn[o] = Real(dn + dd); flux_dir[o] = Real(dd);
+                    }
+                    if (reduce)
+                    {
+                        red[((kk * NC + 0) * SB_GPTS + w) * SB_COLS + lane] = Real(up);
+                        red[((kk * NC + 1) * SB_GPTS + w) * SB_COLS + lane] = Real(dn + dd);
+                        red[((kk * NC + 2) * SB_GPTS + w) * SB_COLS + lane] = Real(dd);
+                    }
+                }
+        }
+        if (reduce)
+        {
+            __syncthreads();
+            Real* const outs[3] = {grp_up, grp_dn, grp_dir};
+            // The surface slot K goes out with the first (bottom) segment.
+            const int nsl = (s == nseg - 1) ? K + 1 : nk;
+            write_group_sums<Real, 3>(red, nsl, gcount, col0, ncol, nlev, gi, outs, [&](int kk) -> int64_t {
+                if (kk == K) return lev_of(nlay);
+                return kk < nk ? lev_of(j0 + kk) : int64_t(-1);
+            });
+            __syncthreads();
+        }
+    }
+}
+
+// Band sums (when a band spans several groups) and broadband sums of group fluxes
+// (ncol, nlev, ngroups): per (column, level), groups in order, then bands in order.
+template<typename Real>
+__global__ void group_sums_kernel(int64_t ncv, Groups grp, const Real* __restrict__ gf,
+                                  Real* __restrict__ bnd, Real* __restrict__ bb)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncv) return;
+    Real total = Real(0), band = Real(0);
+    for (int g = 0; g < grp.n; ++g)
+    {
+        const Real v = gf[i + ncv * g];
+        band = (g > 0 && grp.band[g] == grp.band[g - 1]) ? band + v : v;
+        const bool last = (g + 1 == grp.n) || grp.band[g + 1] != grp.band[g];
+        if (last)
+        {
+            if (bnd) bnd[i + ncv * grp.band[g]] = band;
+            total = (grp.band[g] == grp.band[0]) ? band : total + band;
+        }
+    }
+    if (bb) bb[i] = total;
+}
+
+template<typename Real>
+int group_sums(int64_t ncv, const Groups& grp, const Real* gf, Real* bnd, Real* bb, hipStream_t s)
+{
+    if (!bnd && !bb) return RRTMGP_OK;
+    RRTMGP_TIMED("group_sums_kernel", s);
+    hipLaunchKernelGGL(group_sums_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, grp, gf, bnd, bb);
+    RRTMGP_LAUNCHED("group_sums_kernel");
+    return RRTMGP_OK;
 }
 
 template<typename Real>
@@ -228,64 +532,63 @@ int lw_secants(int64_t ncol, int64_t ngpt, int64_t n_quad, int64_t max_pts, cons
     return RRTMGP_OK;
 }
 
-template<typename Real>
-__global__ void sum_gpt_kernel(int64_t ncv, int64_t ngpt, const Real* __restrict__ f, Real* __restrict__ out)
-{
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= ncv) return;
-    Real s = Real(0);
-    for (int64_t g = 0; g < ngpt; ++g) s += f[i + ncv * g];
-    out[i] = s;
-}
-
+// Outputs are either g-point fluxes (flux_*) or band (bnd_*, (ncol, nlev, nbnd)) and/or broadband
+// (bb_*, (ncol, nlev)) sums.  Group sums go straight into the band outputs when every band is a
+// single group, else through scratch.
 template<typename Real>
 int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nmus, const Real* secants,
               const Real* weights, const Real* tau, const Real* lay, const Real* linc, const Real* ldec,
               const Real* emis, const Real* sfc_src, const Real* inc_flux, Real* flux_up, Real* flux_dn,
-              int do_broadband, Real* bb_up, Real* bb_dn, int do_jac, const Real* sfc_src_jac, Real* flux_up_jac,
-              void* stream)
+              int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bb_up, Real* bb_dn,
+              int do_jac, const Real* sfc_src_jac, Real* flux_up_jac, void* stream)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("lw_solver: empty extents");
     if (nmus < 1 || nmus > 4) return arg_error("lw_solver: n_gauss_angles must be 1..4");
     if (!secants || !weights || !tau || !lay || !linc || !ldec || !emis || !sfc_src)
         return arg_error("lw_solver: NULL input");
-    if (do_broadband ? (!bb_up || !bb_dn) : (!flux_up || !flux_dn)) return arg_error("lw_solver: NULL output");
-    if (do_jac && (!sfc_src_jac || !flux_up_jac)) return arg_error("lw_solver: NULL Jacobian array");
+    const bool spectral = flux_up || flux_dn;
+    if (spectral && (!flux_up || !flux_dn)) return arg_error("lw_solver: flux_up and flux_dn go together");
+    if (!spectral && !((bnd_up && bnd_dn) || (bb_up && bb_dn))) return arg_error("lw_solver: no output");
+    if (spectral && (bnd_up || bnd_dn || bb_up || bb_dn)) return arg_error("lw_solver: g-point and summed outputs are exclusive");
+    if ((bnd_up || bnd_dn) && (!bnd_up || !bnd_dn || !band_lims)) return arg_error("lw_solver: band fluxes need both arrays and band limits");
+    if (do_jac && (!sfc_src_jac || !flux_up_jac || spectral)) return arg_error("lw_solver: the Jacobian needs broadband output");
     hipStream_t s = as_stream(stream);
+    Groups grp;
+    RRTMGP_TRY(make_groups(ngpt, band_lims ? nbnd : 1, band_lims, grp));
     const int64_t ncv = ncol * (nlay + 1);
-    Real *up = flux_up, *dn = flux_dn, *jac = nullptr;
-    if (do_broadband)
+    const bool direct = bnd_up != nullptr && grp.n == int(nbnd);
+    const dim3 grid(grid1(ncol, SB_COLS), unsigned(grp.n));
+    const dim3 block(SB_COLS, SB_GPTS);
+
+    Real *g_up = nullptr, *g_dn = nullptr, *g_jac = nullptr, *scratch = nullptr;
+    if (!spectral)
     {
-        RRTMGP_TRY(scratch_alloc(&up, size_t(ncv * ngpt), s));
-        RRTMGP_TRY(scratch_alloc(&dn, size_t(ncv * ngpt), s));
+        const int64_t nbuf = (direct ? 0 : 2) + (do_jac ? 1 : 0);
+        RRTMGP_TRY(scratch_alloc(&scratch, size_t(ncv * grp.n * nbuf), s));
+        g_up = direct ? bnd_up : scratch;
+        g_dn = direct ? bnd_dn : scratch + ncv * grp.n;
+        if (do_jac) g_jac = scratch + ncv * grp.n * (direct ? 0 : 2);
     }
-    if (do_jac) RRTMGP_TRY(scratch_alloc(&jac, size_t(ncv * ngpt), s));
-    const dim3 grid(grid1(ncol, SV_BX), grid1(ngpt, SV_BY));
+    int rc = RRTMGP_OK;
     {
         RRTMGP_TIMED("lw_solver_kernel", s);
-        hipLaunchKernelGGL(lw_solver_kernel<Real>, grid, dim3(SV_BX, SV_BY), 0, s, ncol, nlay, ngpt, top_at_1,
-                           int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux, up, dn,
-                           sfc_src_jac, jac);
+        if (do_jac)
+            hipLaunchKernelGGL((lw_solver_kernel<Real, true>), grid, block, 0, s, ncol, nlay, ngpt, top_at_1,
+                               int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux,
+                               flux_up, flux_dn, sfc_src_jac, grp, g_up, g_dn, g_jac);
+        else
+            hipLaunchKernelGGL((lw_solver_kernel<Real, false>), grid, block, 0, s, ncol, nlay, ngpt, top_at_1,
+                               int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux,
+                               flux_up, flux_dn, nullptr, grp, g_up, g_dn, nullptr);
+        rc = check_hip(hipGetLastError(), "lw_solver_kernel");
     }
-    int rc = check_hip(hipGetLastError(), "lw_solver_kernel");
-    if (rc == RRTMGP_OK && do_broadband)
+    if (rc == RRTMGP_OK && !spectral)
     {
-        for (int i = 0; i < 2; ++i)
-        {
-            RRTMGP_TIMED("sum_gpt_kernel", s);
-            hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt,
-                               i == 0 ? up : dn, i == 0 ? bb_up : bb_dn);
-        }
-        rc = check_hip(hipGetLastError(), "sum_gpt_kernel");
+        rc = group_sums<Real>(ncv, grp, g_up, direct ? nullptr : bnd_up, bb_up, s);
+        if (rc == RRTMGP_OK) rc = group_sums<Real>(ncv, grp, g_dn, direct ? nullptr : bnd_dn, bb_dn, s);
+        if (rc == RRTMGP_OK && do_jac) rc = group_sums<Real>(ncv, grp, g_jac, nullptr, flux_up_jac, s);
     }
-    if (rc == RRTMGP_OK && do_jac)
-    {
-        RRTMGP_TIMED("sum_gpt_kernel", s);
-        hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, jac, flux_up_jac);
-        rc = check_hip(hipGetLastError(), "sum_gpt_kernel");
-    }
-    if (do_broadband) { scratch_free(up, s); scratch_free(dn, s); }
-    scratch_free(jac, s);
+    scratch_free(scratch, s);
     return rc;
 }
 
@@ -293,48 +596,52 @@ template<typename Real>
 int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real* tau, const Real* ssa,
               const Real* g, const Real* mu0, const Real* alb_dir, const Real* alb_dif, const Real* inc_dir,
               Real* flux_up, Real* flux_dn, Real* flux_dir, int has_dif_bc, const Real* inc_dif,
-              int do_broadband, Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream)
+              int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bnd_dir,
+              Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("sw_solver: empty extents");
     if (!tau || !ssa || !g || !mu0 || !alb_dir || !alb_dif || !inc_dir) return arg_error("sw_solver: NULL input");
-    if (do_broadband ? (!bb_up || !bb_dn || !bb_dir) : (!flux_up || !flux_dn || !flux_dir))
-        return arg_error("sw_solver: NULL output");
+    const bool spectral = flux_up || flux_dn || flux_dir;
+    const bool want_bnd = bnd_up || bnd_dn || bnd_dir, want_bb = bb_up || bb_dn || bb_dir;
+    if (spectral && (!flux_up || !flux_dn || !flux_dir)) return arg_error("sw_solver: flux_up, flux_dn and flux_dir go together");
+    if (spectral && (want_bnd || want_bb)) return arg_error("sw_solver: g-point and summed outputs are exclusive");
+    if (want_bnd && (!bnd_up || !bnd_dn || !bnd_dir || !band_lims)) return arg_error("sw_solver: band fluxes need all three arrays and band limits");
+    if (want_bb && (!bb_up || !bb_dn || !bb_dir)) return arg_error("sw_solver: broadband fluxes need all three arrays");
+    if (!spectral && !want_bnd && !want_bb) return arg_error("sw_solver: no output");
     if (has_dif_bc && !inc_dif) return arg_error("sw_solver: has_dif_bc without inc_flux_dif");
     hipStream_t s = as_stream(stream);
-    const int64_t ncv = ncol * (nlay + 1), ng = ncol * nlay * ngpt;
-    Real *up = flux_up, *dn = flux_dn, *dr = flux_dir;
-    Real *R = nullptr, *T = nullptr, *SU = nullptr, *SD = nullptr;
-    if (do_broadband)
+    Groups grp;
+    RRTMGP_TRY(make_groups(ngpt, band_lims ? nbnd : 1, band_lims, grp));
+    constexpr int K = SwSeg<Real>::K;
+    const int64_t ncv = ncol * (nlay + 1), nseg = (nlay + K - 1) / K;
+    const bool direct = want_bnd && grp.n == int(nbnd);
+    const dim3 grid(grid1(ncol, SB_COLS), unsigned(grp.n));
+    const dim3 block(SB_COLS, SB_GPTS);
+
+    Real *ckpt = nullptr, *scratch = nullptr;
+    RRTMGP_TRY(scratch_alloc(&ckpt, size_t(nseg * 3 * ngpt * ncol), s));
+    if (!spectral && !direct)
     {
-        RRTMGP_TRY(scratch_alloc(&up, size_t(ncv * ngpt), s));
-        RRTMGP_TRY(scratch_alloc(&dn, size_t(ncv * ngpt), s));
-        RRTMGP_TRY(scratch_alloc(&dr, size_t(ncv * ngpt), s));
+        const int rc0 = scratch_alloc(&scratch, size_t(ncv * grp.n * 3), s);
+        if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); return rc0; }
     }
-    RRTMGP_TRY(scratch_alloc(&R, size_t(ng), s));
-    RRTMGP_TRY(scratch_alloc(&T, size_t(ng), s));
-    RRTMGP_TRY(scratch_alloc(&SU, size_t(ng), s));
-    RRTMGP_TRY(scratch_alloc(&SD, size_t(ng), s));
-    const dim3 grid(grid1(ncol, SV_BX), grid1(ngpt, SV_BY));
+    Real* gf[3] = {nullptr, nullptr, nullptr};
+    Real* const bo[3] = {bnd_up, bnd_dn, bnd_dir};
+    Real* const bbo[3] = {bb_up, bb_dn, bb_dir};
+    if (!spectral)
+        for (int c = 0; c < 3; ++c) gf[c] = direct ? bo[c] : scratch + ncv * grp.n * c;
+    int rc = RRTMGP_OK;
     {
         RRTMGP_TIMED("sw_solver_kernel", s);
-        hipLaunchKernelGGL(sw_solver_kernel<Real>, grid, dim3(SV_BX, SV_BY), 0, s, ncol, nlay, ngpt, top_at_1,
-                           tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir, has_dif_bc ? inc_dif : nullptr,
-                           up, dn, dr, R, T, SU, SD);
+        hipLaunchKernelGGL(sw_solver_kernel<Real>, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, tau, ssa, g,
+                           mu0, alb_dir, alb_dif, inc_dir, has_dif_bc ? inc_dif : nullptr, flux_up, flux_dn,
+                           flux_dir, ckpt, grp, gf[0], gf[1], gf[2]);
+        rc = check_hip(hipGetLastError(), "sw_solver_kernel");
     }
-    int rc = check_hip(hipGetLastError(), "sw_solver_kernel");
-    if (rc == RRTMGP_OK && do_broadband)
-    {
-        const Real* src[3] = {up, dn, dr};
-        Real* dst[3] = {bb_up, bb_dn, bb_dir};
-        for (int i = 0; i < 3; ++i)
-        {
-            RRTMGP_TIMED("sum_gpt_kernel", s);
-            hipLaunchKernelGGL(sum_gpt_kernel<Real>, dim3(grid1(ncv, 256)), dim3(256), 0, s, ncv, ngpt, src[i], dst[i]);
-        }
-        rc = check_hip(hipGetLastError(), "sum_gpt_kernel");
-    }
-    scratch_free(R, s); scratch_free(T, s); scratch_free(SU, s); scratch_free(SD, s);
-    if (do_broadband) { scratch_free(up, s); scratch_free(dn, s); scratch_free(dr, s); }
+    for (int c = 0; c < 3 && rc == RRTMGP_OK && !spectral; ++c)
+        rc = group_sums<Real>(ncv, grp, gf[c], direct ? nullptr : bo[c], bbo[c], s);
+    scratch_free(ckpt, s);
+    scratch_free(scratch, s);
     return rc;
 }
 }  // namespace rrtmgp_hip
@@ -350,9 +657,22 @@ extern "C" int rrtmgp_hip_lw_solver_noscat_##S(int64_t ncol, int64_t nlay, int64
     void* stream)                                                                                    \
 {                                                                                                    \
     return lw_solver<T>(ncol, nlay, ngpt, top_at_1, nmus, secants, weights, tau, lay_source,        \
-                        lev_source_inc, lev_source_dec, sfc_emis, sfc_src, inc_flux, flux_up,        \
-                        flux_dn, do_broadband, broadband_up, broadband_dn, do_jacobians,             \
-                        sfc_src_jac, flux_up_jac, stream);                                           \
+                        lev_source_inc, lev_source_dec, sfc_emis, sfc_src, inc_flux,                 \
+                        do_broadband ? nullptr : flux_up, do_broadband ? nullptr : flux_dn,          \
+                        0, nullptr, nullptr, nullptr, do_broadband ? broadband_up : nullptr,         \
+                        do_broadband ? broadband_dn : nullptr, do_jacobians, sfc_src_jac,            \
+                        flux_up_jac, stream);                                                        \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_lw_solver_noscat_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,      \
+    int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,                   \
+    const T* lay_source, const T* lev_source_inc, const T* lev_source_dec, const T* sfc_emis,       \
+    const T* sfc_src, const T* inc_flux, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,    \
+    T* bnd_flux_dn, T* broadband_up, T* broadband_dn, void* stream)                                 \
+{                                                                                                    \
+    return lw_solver<T>(ncol, nlay, ngpt, top_at_1, nmus, secants, weights, tau, lay_source,        \
+                        lev_source_inc, lev_source_dec, sfc_emis, sfc_src, inc_flux, nullptr,        \
+                        nullptr, nbnd, band_lims_gpt, bnd_flux_up, bnd_flux_dn, broadband_up,        \
+                        broadband_dn, 0, nullptr, nullptr, stream);                                  \
 }                                                                                                    \
 extern "C" int rrtmgp_hip_sw_solver_2stream_##S(int64_t ncol, int64_t nlay, int64_t ngpt,         \
     int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0, const T* sfc_alb_dir,       \
@@ -361,8 +681,22 @@ extern "C" int rrtmgp_hip_sw_solver_2stream_##S(int64_t ncol, int64_t nlay, int6
     T* broadband_dir, void* stream)                                                                  \
 {                                                                                                    \
     return sw_solver<T>(ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, sfc_alb_dir, sfc_alb_dif,     \
-                        inc_flux_dir, flux_up, flux_dn, flux_dir, has_dif_bc, inc_flux_dif,          \
-                        do_broadband, broadband_up, broadband_dn, broadband_dir, stream);            \
+                        inc_flux_dir, do_broadband ? nullptr : flux_up,                              \
+                        do_broadband ? nullptr : flux_dn, do_broadband ? nullptr : flux_dir,         \
+                        has_dif_bc, inc_flux_dif, 0, nullptr, nullptr, nullptr, nullptr,             \
+                        do_broadband ? broadband_up : nullptr, do_broadband ? broadband_dn : nullptr,\
+                        do_broadband ? broadband_dir : nullptr, stream);                             \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,     \
+    int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0, const T* sfc_alb_dir,       \
+    const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc, const T* inc_flux_dif,             \
+    int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up, T* bnd_flux_dn, T* bnd_flux_dir,        \
+    T* broadband_up, T* broadband_dn, T* broadband_dir, void* stream)                               \
+{                                                                                                    \
+    return sw_solver<T>(ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, sfc_alb_dir, sfc_alb_dif,     \
+                        inc_flux_dir, nullptr, nullptr, nullptr, has_dif_bc, inc_flux_dif, nbnd,     \
+                        band_lims_gpt, bnd_flux_up, bnd_flux_dn, bnd_flux_dir, broadband_up,         \
+                        broadband_dn, broadband_dir, stream);                                        \
 }
 
 #define RRTMGP_DEF_SECANTS(S, T)                                                                    \

@@ -1,9 +1,10 @@
 // Column-radiation drivers on the GPU (reference include_test/Radiation_solver.h:33-233,
 // src_test/Radiation_solver.cu:419-950).  The call sequence is the reference's
 // (gas optics -> cloud optics -> add_to -> rte -> flux reduction); the cloud add_to is folded
-// into the gas-optics pass, the SW cloud delta-scaling into the cloud-optics pass, and
-// broadband fluxes are reduced inside the solver (CPU semantics) unless band fluxes are asked
-// for.  Workspaces persist across calls; columns are processed in blocks of n_col_block.
+// into the gas-optics pass, the SW cloud delta-scaling into the cloud-optics pass, and band and
+// broadband fluxes are summed inside the solvers (the CPU semantics of broadband output), so no
+// g-point flux arrays exist.  Workspaces persist across calls; columns are processed in blocks
+// of n_col_block.
 #ifndef RRTMGP_AMD_RADIATION_SOLVER_H
 #define RRTMGP_AMD_RADIATION_SOLVER_H
 
@@ -54,8 +55,8 @@ class Radiation_solver_longwave_gpu
         std::unique_ptr<Optical_props_arry_gpu> optical_props;
         std::unique_ptr<Optical_props_1scl_gpu> cloud_props;
         std::unique_ptr<Source_func_lw_gpu> sources;
-        Array_gpu<Float,3> gpt_flux_up, gpt_flux_dn;
-        std::unique_ptr<Fluxes_byband_gpu> bnd_fluxes;
+        Array_gpu<Float,2> blk_flux_up, blk_flux_dn;
+        Array_gpu<Float,3> blk_bnd_up, blk_bnd_dn;
         Array_gpu<Float,2> col_dry_ws;
 };
 
@@ -96,8 +97,8 @@ class Radiation_solver_shortwave_gpu
         std::unique_ptr<Optical_props_arry_gpu> optical_props;
         std::unique_ptr<Optical_props_2str_gpu> cloud_props;
         Array_gpu<Float,2> toa_src;
-        Array_gpu<Float,3> gpt_flux_up, gpt_flux_dn, gpt_flux_dir;
-        std::unique_ptr<Fluxes_byband_gpu> bnd_fluxes;
+        Array_gpu<Float,2> blk_flux_up, blk_flux_dn, blk_flux_dir;
+        Array_gpu<Float,3> blk_bnd_up, blk_bnd_dn, blk_bnd_dir;
         Array_gpu<Float,2> col_dry_ws;
 };
 

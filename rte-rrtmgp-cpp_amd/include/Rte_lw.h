@@ -25,6 +25,19 @@ class Rte_lw_gpu
                 Array_gpu<Float,3>& gpt_flux_dn,
                 const int n_gauss_angles);
 
+        // Band and broadband fluxes summed inside the solver (no g-point flux arrays); either
+        // output pair may be empty.  Not in the reference, whose drivers reduce g-point fluxes
+        // with Fluxes_byband (src/Fluxes.cpp:142-197).
+        void rte_lw_fluxes(
+                const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
+                const Bool top_at_1,
+                const Source_func_lw_gpu& sources,
+                const Array_gpu<Float,2>& sfc_emis,
+                const Array_gpu<Float,2>& inc_flux,
+                Array_gpu<Float,3>& bnd_flux_up, Array_gpu<Float,3>& bnd_flux_dn,
+                Array_gpu<Float,2>& flux_up, Array_gpu<Float,2>& flux_dn,
+                const int n_gauss_angles);
+
         void expand_and_transpose(
                 const std::unique_ptr<Optical_props_arry_gpu>& ops,
                 const Array_gpu<Float,2> arr_in,

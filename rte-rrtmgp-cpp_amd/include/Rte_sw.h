@@ -24,6 +24,20 @@ class Rte_sw_gpu
                 Array_gpu<Float,3>& gpt_flux_dn,
                 Array_gpu<Float,3>& gpt_flux_dir);
 
+        // Band and broadband fluxes summed inside the solver (no g-point flux arrays); either
+        // output set may be empty.  flux_dn is the total (diffuse + direct) downward flux.
+        void rte_sw_fluxes(
+                const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
+                const Bool top_at_1,
+                const Array_gpu<Float,1>& mu0,
+                const Array_gpu<Float,2>& inc_flux_dir,
+                const Array_gpu<Float,2>& sfc_alb_dir,
+                const Array_gpu<Float,2>& sfc_alb_dif,
+                const Array_gpu<Float,2>& inc_flux_dif,
+                Array_gpu<Float,3>& bnd_flux_up, Array_gpu<Float,3>& bnd_flux_dn,
+                Array_gpu<Float,3>& bnd_flux_dir,
+                Array_gpu<Float,2>& flux_up, Array_gpu<Float,2>& flux_dn, Array_gpu<Float,2>& flux_dir);
+
         void expand_and_transpose(
                 const std::unique_ptr<Optical_props_arry_gpu>& ops,
                 const Array_gpu<Float,2> arr_in,

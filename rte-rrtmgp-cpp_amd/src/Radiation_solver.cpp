@@ -74,11 +74,13 @@ void Radiation_solver_longwave_gpu::solve_gpu(
         sources = std::make_unique<Source_func_lw_gpu>(nblk, nlay, *kdist);
         cloud_props.reset();
         if (cloud_optics) cloud_props = std::make_unique<Optical_props_1scl_gpu>(nblk, nlay, *cloud_optics);
-        const int ng_out = switch_output_bnd_fluxes ? ngpt : 1;
-        gpt_flux_up = Array_gpu<Float,3>({nblk, nlev, ng_out});
-        gpt_flux_dn = Array_gpu<Float,3>({nblk, nlev, ng_out});
-        bnd_fluxes.reset();
-        if (switch_output_bnd_fluxes) bnd_fluxes = std::make_unique<Fluxes_byband_gpu>(nblk, nlev, nbnd);
+        // Block flux workspaces, used when a block does not span all columns.
+        const bool partial = nblk < ncol;
+        blk_flux_up = partial ? Array_gpu<Float,2>({nblk, nlev}) : Array_gpu<Float,2>();
+        blk_flux_dn = partial ? Array_gpu<Float,2>({nblk, nlev}) : Array_gpu<Float,2>();
+        const bool pb = partial && switch_output_bnd_fluxes;
+        blk_bnd_up = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
+        blk_bnd_dn = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
         ws_ncol = nblk; ws_nlay = nlay; ws_bnd = int(switch_output_bnd_fluxes);
     }
 
@@ -110,12 +112,10 @@ void Radiation_solver_longwave_gpu::solve_gpu(
                            a, b, c, d, fu, fd, fn, bu, bd, bn);
             scatter(fu.ptr(), n, nlev, lw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, lw_flux_dn.ptr(), ncol, c0);
-            scatter(fn.ptr(), n, nlev, lw_flux_net.ptr(), ncol, c0);
             if (switch_output_bnd_fluxes)
             {
                 scatter(bu.ptr(), n, int64_t(nlev) * nbnd, lw_bnd_flux_up.ptr(), ncol, c0);
                 scatter(bd.ptr(), n, int64_t(nlev) * nbnd, lw_bnd_flux_dn.ptr(), ncol, c0);
-                scatter(bn.ptr(), n, int64_t(nlev) * nbnd, lw_bnd_flux_net.ptr(), ncol, c0);
             }
             continue;
         }
@@ -147,29 +147,30 @@ void Radiation_solver_longwave_gpu::solve_gpu(
         }
         kdist->gas_optics_with_clouds(pl, pv, tl, ts, gc, *optical_props, *sources, cd, tv, cloud);
 
-        rte.rte_lw(optical_props, top_at_1, *sources, em, Array_gpu<Float,2>(), gpt_flux_up, gpt_flux_dn, 1);
-
-        Float* fu = whole ? lw_flux_up.ptr() : nullptr;
-        if (switch_output_bnd_fluxes)
+        // Band and broadband fluxes summed inside the solver, straight into the outputs when the
+        // block spans all columns.
+        Array_gpu<Float,2>& fu = whole ? lw_flux_up : blk_flux_up;
+        Array_gpu<Float,2>& fd = whole ? lw_flux_dn : blk_flux_dn;
+        Array_gpu<Float,3> no_bnd;
+        Array_gpu<Float,3>& bu = switch_output_bnd_fluxes ? (whole ? lw_bnd_flux_up : blk_bnd_up) : no_bnd;
+        Array_gpu<Float,3>& bd = switch_output_bnd_fluxes ? (whole ? lw_bnd_flux_dn : blk_bnd_dn) : no_bnd;
+        rte.rte_lw_fluxes(optical_props, top_at_1, *sources, em, Array_gpu<Float,2>(), bu, bd, fu, fd, 1);
+        if (!whole)
         {
-            bnd_fluxes->reduce(gpt_flux_up, gpt_flux_dn, optical_props, top_at_1);
-            scatter(bnd_fluxes->get_flux_up().ptr(), n, nlev, lw_flux_up.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_flux_dn().ptr(), n, nlev, lw_flux_dn.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_flux_net().ptr(), n, nlev, lw_flux_net.ptr(), ncol, c0);
-            const int64_t rows = int64_t(nlev) * nbnd;
-            scatter(bnd_fluxes->get_bnd_flux_up().ptr(), n, rows, lw_bnd_flux_up.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_bnd_flux_dn().ptr(), n, rows, lw_bnd_flux_dn.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_bnd_flux_net().ptr(), n, rows, lw_bnd_flux_net.ptr(), ncol, c0);
+            scatter(fu.ptr(), n, nlev, lw_flux_up.ptr(), ncol, c0);
+            scatter(fd.ptr(), n, nlev, lw_flux_dn.ptr(), ncol, c0);
+            if (switch_output_bnd_fluxes)
+            {
+                const int64_t rows = int64_t(nlev) * nbnd;
+                scatter(bu.ptr(), n, rows, lw_bnd_flux_up.ptr(), ncol, c0);
+                scatter(bd.ptr(), n, rows, lw_bnd_flux_dn.ptr(), ncol, c0);
+            }
         }
-        else
-        {
-            scatter(gpt_flux_up.ptr(), n, nlev, lw_flux_up.ptr(), ncol, c0);
-            scatter(gpt_flux_dn.ptr(), n, nlev, lw_flux_dn.ptr(), ncol, c0);
-        }
-        (void)fu;
     }
-    if (!switch_output_bnd_fluxes)
-        hip_check(K<Float>::net_broadband(ncol, nlev, lw_flux_dn.ptr(), lw_flux_up.ptr(), lw_flux_net.ptr(), stream()));
+    hip_check(K<Float>::net_broadband(ncol, nlev, lw_flux_dn.ptr(), lw_flux_up.ptr(), lw_flux_net.ptr(), stream()));
+    if (switch_output_bnd_fluxes)
+        hip_check(K<Float>::net_broadband(ncol, int64_t(nlev) * nbnd, lw_bnd_flux_dn.ptr(), lw_bnd_flux_up.ptr(),
+                                          lw_bnd_flux_net.ptr(), stream()));
 }
 
 
@@ -205,12 +206,14 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         cloud_props.reset();
         if (cloud_optics) cloud_props = std::make_unique<Optical_props_2str_gpu>(nblk, nlay, *cloud_optics);
         toa_src = Array_gpu<Float,2>({nblk, ngpt});
-        const int ng_out = switch_output_bnd_fluxes ? ngpt : 1;
-        gpt_flux_up = Array_gpu<Float,3>({nblk, nlev, ng_out});
-        gpt_flux_dn = Array_gpu<Float,3>({nblk, nlev, ng_out});
-        gpt_flux_dir = Array_gpu<Float,3>({nblk, nlev, ng_out});
-        bnd_fluxes.reset();
-        if (switch_output_bnd_fluxes) bnd_fluxes = std::make_unique<Fluxes_byband_gpu>(nblk, nlev, nbnd);
+        const bool partial = nblk < ncol;
+        blk_flux_up = partial ? Array_gpu<Float,2>({nblk, nlev}) : Array_gpu<Float,2>();
+        blk_flux_dn = partial ? Array_gpu<Float,2>({nblk, nlev}) : Array_gpu<Float,2>();
+        blk_flux_dir = partial ? Array_gpu<Float,2>({nblk, nlev}) : Array_gpu<Float,2>();
+        const bool pb = partial && switch_output_bnd_fluxes;
+        blk_bnd_up = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
+        blk_bnd_dn = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
+        blk_bnd_dir = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
         ws_ncol = nblk; ws_nlay = nlay; ws_bnd = int(switch_output_bnd_fluxes);
     }
 
@@ -244,14 +247,12 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
             scatter(fu.ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
             scatter(fdir.ptr(), n, nlev, sw_flux_dn_dir.ptr(), ncol, c0);
-            scatter(fn.ptr(), n, nlev, sw_flux_net.ptr(), ncol, c0);
             if (switch_output_bnd_fluxes)
             {
                 const int64_t rows = int64_t(nlev) * nbnd;
                 scatter(bu.ptr(), n, rows, sw_bnd_flux_up.ptr(), ncol, c0);
                 scatter(bd.ptr(), n, rows, sw_bnd_flux_dn.ptr(), ncol, c0);
                 scatter(bdir.ptr(), n, rows, sw_bnd_flux_dn_dir.ptr(), ncol, c0);
-                scatter(bn.ptr(), n, rows, sw_bnd_flux_net.ptr(), ncol, c0);
             }
             continue;
         }
@@ -289,29 +290,30 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         // TSI scaling of the incoming flux (src_test/Radiation_solver.cpp:739-743).
         hip_check(K<Float>::scale_by_col(n, ngpt, toa_src.ptr(), tsi.ptr(), stream()));
 
-        rte.rte_sw(optical_props, top_at_1, m0, toa_src, ad, af, Array_gpu<Float,2>(),
-                   gpt_flux_up, gpt_flux_dn, gpt_flux_dir);
-
-        if (switch_output_bnd_fluxes)
+        Array_gpu<Float,2>& fu = whole ? sw_flux_up : blk_flux_up;
+        Array_gpu<Float,2>& fd = whole ? sw_flux_dn : blk_flux_dn;
+        Array_gpu<Float,2>& fr = whole ? sw_flux_dn_dir : blk_flux_dir;
+        Array_gpu<Float,3> no_bnd;
+        Array_gpu<Float,3>& bu = switch_output_bnd_fluxes ? (whole ? sw_bnd_flux_up : blk_bnd_up) : no_bnd;
+        Array_gpu<Float,3>& bd = switch_output_bnd_fluxes ? (whole ? sw_bnd_flux_dn : blk_bnd_dn) : no_bnd;
+        Array_gpu<Float,3>& br = switch_output_bnd_fluxes ? (whole ? sw_bnd_flux_dn_dir : blk_bnd_dir) : no_bnd;
+        rte.rte_sw_fluxes(optical_props, top_at_1, m0, toa_src, ad, af, Array_gpu<Float,2>(), bu, bd, br, fu, fd, fr);
+        if (!whole)
         {
-            bnd_fluxes->reduce(gpt_flux_up, gpt_flux_dn, gpt_flux_dir, optical_props, top_at_1);
-            scatter(bnd_fluxes->get_flux_up().ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_flux_dn().ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_flux_dn_dir().ptr(), n, nlev, sw_flux_dn_dir.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_flux_net().ptr(), n, nlev, sw_flux_net.ptr(), ncol, c0);
-            const int64_t rows = int64_t(nlev) * nbnd;
-            scatter(bnd_fluxes->get_bnd_flux_up().ptr(), n, rows, sw_bnd_flux_up.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_bnd_flux_dn().ptr(), n, rows, sw_bnd_flux_dn.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_bnd_flux_dn_dir().ptr(), n, rows, sw_bnd_flux_dn_dir.ptr(), ncol, c0);
-            scatter(bnd_fluxes->get_bnd_flux_net().ptr(), n, rows, sw_bnd_flux_net.ptr(), ncol, c0);
-        }
-        else
-        {
-            scatter(gpt_flux_up.ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
-            scatter(gpt_flux_dn.ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
-            scatter(gpt_flux_dir.ptr(), n, nlev, sw_flux_dn_dir.ptr(), ncol, c0);
+            scatter(fu.ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
+            scatter(fd.ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
+            scatter(fr.ptr(), n, nlev, sw_flux_dn_dir.ptr(), ncol, c0);
+            if (switch_output_bnd_fluxes)
+            {
+                const int64_t rows = int64_t(nlev) * nbnd;
+                scatter(bu.ptr(), n, rows, sw_bnd_flux_up.ptr(), ncol, c0);
+                scatter(bd.ptr(), n, rows, sw_bnd_flux_dn.ptr(), ncol, c0);
+                scatter(br.ptr(), n, rows, sw_bnd_flux_dn_dir.ptr(), ncol, c0);
+            }
         }
     }
-    if (!switch_output_bnd_fluxes)
-        hip_check(K<Float>::net_broadband(ncol, nlev, sw_flux_dn.ptr(), sw_flux_up.ptr(), sw_flux_net.ptr(), stream()));
+    hip_check(K<Float>::net_broadband(ncol, nlev, sw_flux_dn.ptr(), sw_flux_up.ptr(), sw_flux_net.ptr(), stream()));
+    if (switch_output_bnd_fluxes)
+        hip_check(K<Float>::net_broadband(ncol, int64_t(nlev) * nbnd, sw_bnd_flux_dn.ptr(), sw_bnd_flux_up.ptr(),
+                                          sw_bnd_flux_net.ptr(), stream()));
 }

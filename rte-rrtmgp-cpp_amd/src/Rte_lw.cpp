@@ -21,6 +21,45 @@ const Float gauss_wts[4][4] = {{0.5, 0., 0., 0.},
                                {0.1355069134, 0.2034645680, 0.1298475476, 0.0311809710}};
 }
 
+namespace
+{
+// Per-call solver inputs: surface emissivity by g-point, secants (ncol, ngpt, nmus) and weights.
+struct LwPrep
+{
+    Array_gpu<Float,2> sfc_emis_gpt;
+    Array_gpu<Float,1> weights;
+    Array_gpu<Float,3> secants;
+};
+
+LwPrep lw_prep(Rte_lw_gpu& rte, const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
+               const Array_gpu<Float,2>& sfc_emis, const int n_gauss_angles)
+{
+    const int ncol = optical_props->get_ncol();
+    const int ngpt = optical_props->get_ngpt();
+    if (n_gauss_angles < 1 || n_gauss_angles > 4)
+        throw std::runtime_error("n_gauss_angles must be between 1 and 4");
+    LwPrep p;
+    p.sfc_emis_gpt = Array_gpu<Float,2>({ncol, ngpt});
+    rte.expand_and_transpose(optical_props, sfc_emis, p.sfc_emis_gpt);
+
+    // Gauss tables (max_gauss_pts = 4) as the reference keeps them (src_cuda/Rte_lw.cu:91-124):
+    // column nmus holds the secants / weights of an nmus-point quadrature.
+    const int nmus = n_gauss_angles;
+    Array<Float,2> ds_h({4, 4});
+    Array<Float,1> weights_h({nmus});
+    for (int n = 0; n < 4; ++n)
+        for (int imu = 0; imu < 4; ++imu)
+            ds_h({imu + 1, n + 1}) = gauss_Ds[n][imu];
+    for (int imu = 0; imu < nmus; ++imu)
+        weights_h({imu + 1}) = gauss_wts[nmus - 1][imu];
+    Array_gpu<Float,2> ds(ds_h);
+    p.weights = Array_gpu<Float,1>(weights_h);
+    p.secants = Array_gpu<Float,3>({ncol, ngpt, nmus});
+    hip_check(K<Float>::lw_secants_array(ncol, ngpt, nmus, 4, ds.ptr(), p.secants.ptr(), stream()));
+    return p;
+}
+}  // namespace
+
 void Rte_lw_gpu::rte_lw(
         const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
         const Bool top_at_1,
@@ -34,38 +73,47 @@ void Rte_lw_gpu::rte_lw(
     const int ncol = optical_props->get_ncol();
     const int nlay = optical_props->get_nlay();
     const int ngpt = optical_props->get_ngpt();
-    if (n_gauss_angles < 1 || n_gauss_angles > 4)
-        throw std::runtime_error("n_gauss_angles must be between 1 and 4");
-
-    Array_gpu<Float,2> sfc_emis_gpt({ncol, ngpt});
-    expand_and_transpose(optical_props, sfc_emis, sfc_emis_gpt);
-
-    // Gauss tables (max_gauss_pts = 4) as the reference keeps them (src_cuda/Rte_lw.cu:91-124):
-    // column nmus holds the secants / weights of an nmus-point quadrature.
-    const int nmus = n_gauss_angles;
-    Array<Float,2> ds_h({4, 4});
-    Array<Float,1> weights_h({nmus});
-    for (int n = 0; n < 4; ++n)
-        for (int imu = 0; imu < 4; ++imu)
-            ds_h({imu + 1, n + 1}) = gauss_Ds[n][imu];
-    for (int imu = 0; imu < nmus; ++imu)
-        weights_h({imu + 1}) = gauss_wts[nmus - 1][imu];
-    Array_gpu<Float,2> ds(ds_h);
-    Array_gpu<Float,1> weights(weights_h);
-    Array_gpu<Float,3> secants({ncol, ngpt, nmus});
-    hip_check(K<Float>::lw_secants_array(ncol, ngpt, nmus, 4, ds.ptr(), secants.ptr(), stream()));
+    const LwPrep p = lw_prep(*this, optical_props, sfc_emis, n_gauss_angles);
 
     // CPU semantics: a (ncol, nlev, 1) output array selects broadband fluxes (src/Rte_lw.cpp:137).
     const bool do_broadband = (gpt_flux_up.dim(3) == 1) && (ngpt > 1);
     const Float* inc = (inc_flux.size() == 0) ? nullptr : inc_flux.ptr();
     hip_check(K<Float>::lw_solver_noscat(
-            ncol, nlay, ngpt, top_at_1, nmus, secants.ptr(), weights.ptr(),
+            ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
             optical_props->get_tau().ptr(), sources.get_lay_source().ptr(),
             sources.get_lev_source_inc().ptr(), sources.get_lev_source_dec().ptr(),
-            sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc,
+            p.sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc,
             do_broadband ? nullptr : gpt_flux_up.ptr(), do_broadband ? nullptr : gpt_flux_dn.ptr(),
             do_broadband, do_broadband ? gpt_flux_up.ptr() : nullptr,
             do_broadband ? gpt_flux_dn.ptr() : nullptr, 0, nullptr, nullptr, stream()));
+}
+
+void Rte_lw_gpu::rte_lw_fluxes(
+        const std::unique_ptr<Optical_props_arry_gpu>& optical_props,
+        const Bool top_at_1,
+        const Source_func_lw_gpu& sources,
+        const Array_gpu<Float,2>& sfc_emis,
+        const Array_gpu<Float,2>& inc_flux,
+        Array_gpu<Float,3>& bnd_flux_up, Array_gpu<Float,3>& bnd_flux_dn,
+        Array_gpu<Float,2>& flux_up, Array_gpu<Float,2>& flux_dn,
+        const int n_gauss_angles)
+{
+    const int ncol = optical_props->get_ncol();
+    const int nlay = optical_props->get_nlay();
+    const int ngpt = optical_props->get_ngpt();
+    const int nbnd = optical_props->get_nband();
+    const LwPrep p = lw_prep(*this, optical_props, sfc_emis, n_gauss_angles);
+    const Float* inc = (inc_flux.size() == 0) ? nullptr : inc_flux.ptr();
+    const bool bnd = bnd_flux_up.size() > 0;
+    const bool bb = flux_up.size() > 0;
+    hip_check(K<Float>::lw_solver_noscat_bnd(
+            ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
+            optical_props->get_tau().ptr(), sources.get_lay_source().ptr(),
+            sources.get_lev_source_inc().ptr(), sources.get_lev_source_dec().ptr(),
+            p.sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
+            optical_props->get_band_lims_gpoint().ptr(), bnd ? bnd_flux_up.ptr() : nullptr,
+            bnd ? bnd_flux_dn.ptr() : nullptr, bb ? flux_up.ptr() : nullptr,
+            bb ? flux_dn.ptr() : nullptr, stream()));
 }
 
 void Rte_lw_gpu::expand_and_transpose(

@@ -25,6 +25,8 @@ template<typename T> struct K;
     static constexpr auto lw_secants_array = rrtmgp_hip_lw_secants_array_##S; \
     static constexpr auto lw_solver_noscat = rrtmgp_hip_lw_solver_noscat_##S; \
     static constexpr auto sw_solver_2stream = rrtmgp_hip_sw_solver_2stream_##S; \
+    static constexpr auto lw_solver_noscat_bnd = rrtmgp_hip_lw_solver_noscat_bnd_##S; \
+    static constexpr auto sw_solver_2stream_bnd = rrtmgp_hip_sw_solver_2stream_bnd_##S; \
     static constexpr auto sum_broadband = rrtmgp_hip_sum_broadband_##S;  \
     static constexpr auto net_broadband = rrtmgp_hip_net_broadband_##S;  \
     static constexpr auto sum_byband = rrtmgp_hip_sum_byband_##S;        \
