@@ -24,11 +24,13 @@
 // Per (column, g-point) the memory traffic is the inputs read twice plus a checkpoint per K
 // levels, instead of four layer arrays written and read back.
 //
-// Precision islands.  The two-stream layer coefficients (gamma, k, exponentials, R/T, the
-// 1/(1 - k^2 mu0^2) guard) and the adding denominators 1/(1 - R A) are evaluated in FP64 in
-// every build (with the FP64 k_min = 1e-12 and epsilon), as are the carried adding states;
-// the inputs, outputs and checkpoints keep the build's precision.
+// Precision.  The adding denominators 1/(1 - R A) and the carried adding states (A, S, R_up, D,
+// the direct beam) are FP64 in every build.  The two-stream layer coefficients are FP64 in the
+// FP64 build; the FP32 build evaluates them in a rearranged FP32 form (sw_layer_fp32) and keeps
+// FP64 for near-conservative layers, with the FP64 k_min = 1e-12 and epsilon.  Inputs, outputs
+// and checkpoints keep the build's precision.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -45,7 +47,7 @@ template<typename Real> struct LwSeg;
 template<> struct LwSeg<float> { static constexpr int K = 16; };
 template<> struct LwSeg<double> { static constexpr int K = 8; };
 template<typename Real> struct SwSeg;
-template<> struct SwSeg<float> { static constexpr int K = 7; };
+template<> struct SwSeg<float> { static constexpr int K = 4; };
 template<> struct SwSeg<double> { static constexpr int K = 4; };
 
 // G-point groups: consecutive g-points of one band, at most SB_GPTS each.
@@ -325,17 +327,105 @@ __device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, d
 // One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
 struct LayerSW { double R, T, SU, SD, Tn; };
 
-__device__ __forceinline__ LayerSW sw_layer(double tau, double w0, double g, double mu0, double dir)
+// FP32 building blocks on the hardware approximations (v_exp_f32, v_rcp_f32, v_sqrt_f32, each
+// within about 1 ulp), which cost one or two instructions instead of the IEEE sequences.
+__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// e^{-y} for y >= 0 through 2^x.
+__device__ __forceinline__ float exp_neg_f32(float y) { return __builtin_amdgcn_exp2f(-y * 1.44269504088896341f); }
+
+// 1 - e^{-y} for y >= 0: series below 0.25 (truncation < y^8/8! relative), else 1 - e^{-y}.
+__device__ __forceinline__ float one_minus_exp_neg_f32(float y)
 {
+    float t = 1.f / 720.f - y * (1.f / 5040.f);
+    t = 1.f / 120.f - y * t;
+    t = 1.f / 24.f - y * t;
+    t = 1.f / 6.f - y * t;
+    t = .5f - y * t;
+    t = 1.f - y * t;
+    return (y < .25f) ? y * t : 1.f - exp_neg_f32(y);
+}
+
+// 1/x in FP64: v_rcp_f64 and two Newton steps (for the adding denominators, x in (0, 1]).
+__device__ __forceinline__ double rcp_f64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.), r);
+    return fma(r, fma(-x, r, 1.), r);
+}
+
+// The layer properties in FP32, rearranged so that FP32 keeps its accuracy:
+//   k^2 = 2 (1 - w0) (gamma1 + gamma2)          (gamma1 - gamma2 = 2 (1 - w0) exactly),
+//   1 - e^{-2k tau} = a (2 - a) with a = 1 - e^{-k tau}, b = 1 - e^{-tau / mu0},
+// and the direct-beam solution written without the removable singularity at k mu0 = 1.  With
+// G = (e^{-k tau} - e^{-tau/mu0}) / (1 - k mu0) the reference's expressions (F90:1079-1104)
+// become, identically,
+//   Rdir = w0 RT [(alpha2 + k gamma3)(2a - a^2) + 2 (k gamma3 - alpha2 k mu0)(1 - a) G] / (1 + k mu0)
+//   Tdir = -w0 RT [(alpha1 - k gamma4)(2a - a^2)(1 - b) - 2k (gamma4 + mu0 alpha1) G] / (1 + k mu0)
+// where every O(1) term that cancels in the reference form has been cancelled analytically.  G is
+// e^{-tau/mu0} (tau/mu0) (e^x - 1)/x with x = tau (1 - k mu0) / mu0, from a series when |x| <= 1/2
+// and as the difference quotient otherwise.  The direct-beam transmittance, whose rounding
+// compounds down the column, keeps the accurate expf.  Returns false where FP64 is kept: near
+// conservative scattering (1 - w0 < 1e-6).
+__device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, float mu0, float inv_mu0, double dir,
+                                              LayerSW& L)
+{
+    const float one_m_w0 = 1.f - w0;
+    if (one_m_w0 < 1e-6f) return false;
+    const float gamma1 = (8.f - w0 * (5.f + 3.f * g)) * .25f;
+    const float gamma2 = 3.f * (w0 * (1.f - g)) * .25f;
+    const float k = __builtin_amdgcn_sqrtf(fmaxf(2.f * one_m_w0 * (gamma1 + gamma2), 1e-12f));
+    const float k_mu = k * mu0;
+    const float gamma3 = (2.f - 3.f * mu0 * g) * .25f;
+    const float gamma4 = 1.f - gamma3;
+    const float alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
+    const float alpha2 = gamma1 * gamma3 + gamma2 * gamma4;
+    const float kt = k * tau;
+    const float a = one_minus_exp_neg_f32(kt), e1 = exp_neg_f32(kt);
+    const float c2 = a * (2.f - a);
+    const float RT = rcp_f32(k * (2.f - c2) + gamma1 * c2);
+    const float y = tau * inv_mu0;
+    const float Tn = expf(-y);
+    const float b = (y < .25f) ? one_minus_exp_neg_f32(y) : 1.f - Tn;
+    const float ek = 1.f - k_mu;
+    const float x = y * ek;
+    float phi = 1.f / 40320.f;
+    phi = 1.f / 5040.f + x * phi;
+    phi = 1.f / 720.f + x * phi;
+    phi = 1.f / 120.f + x * phi;
+    phi = 1.f / 24.f + x * phi;
+    phi = 1.f / 6.f + x * phi;
+    phi = .5f + x * phi;
+    phi = 1.f + x * phi;
+    const float G = (fabsf(x) <= .5f) ? Tn * y * phi : (e1 - Tn) * rcp_f32(ek);
+    const float s = w0 * RT * rcp_f32(1.f + k_mu);
+    const float rdf = s * ((alpha2 + k * gamma3) * c2 + 2.f * (k * gamma3 - alpha2 * k_mu) * (1.f - a) * G);
+    const float tdf = -s * ((alpha1 - k * gamma4) * c2 * (1.f - b) - 2.f * k * (gamma4 + mu0 * alpha1) * G);
+    // Clamps of rte_solver_kernels.cu:581-583 with the FP64 epsilon of the island they replace.
+    const double eps = Limits<double>::eps;
+    const double rd = fmax(eps, fmin(double(rdf), double(b)));
+    const double td = fmax(eps, fmin(double(tdf), double(b) - rd));
+    L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), rd * dir, td * dir, double(Tn)};
+    return true;
+}
+
+// FP32 builds take the FP32 form where it is accurate and the FP64 island elsewhere; FP64
+// builds evaluate the reference expressions in FP64.
+template<typename Real>
+__device__ __forceinline__ LayerSW sw_layer(Real tau, Real w0, Real g, Real mu0, Real inv_mu0, double dir)
+{
+    LayerSW L;
+    if constexpr (std::is_same<Real, float>::value)
+        if (sw_layer_fp32(tau, w0, g, mu0, inv_mu0, dir, L)) return L;
     double Rdif, Tdif, Rdir, Tdir, Tn;
-    sw_two_stream(tau, w0, g, mu0, Rdif, Tdif, Rdir, Tdir, Tn);
+    sw_two_stream(double(tau), double(w0), double(g), double(mu0), Rdif, Tdif, Rdir, Tdir, Tn);
     return {Rdif, Tdif, Rdir * dir, Tdir * dir, Tn};
 }
 
 // Column above the next level from the column above this one and the layer between.
 __device__ __forceinline__ void above_update(const LayerSW& L, double& R_up, double& D)
 {
-    const double den = 1. / (1. - L.R * R_up);
+    const double den = rcp_f64(1. - L.R * R_up);
     D = L.SD + L.T * (D + R_up * L.SU) * den;
     R_up = L.R + L.T * L.T * R_up * den;
 }
@@ -374,24 +464,37 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     auto lay_of = [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - 1 - j; };   // layer j from the top
     auto lev_of = [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - j; };       // level j from the top
 
-    const double m0 = active ? double(mu0[icol]) : 1.;
+    const Real m0r = active ? mu0[icol] : Real(1);
+    const Real inv_m0r = Real(1) / m0r;
+    const double m0 = double(m0r);
     double R_up = 0., D = 0., dir = 0.;
     if (active)
     {
         dir = double(inc_dir[c2]) * m0;
         D = inc_dif ? double(inc_dif[c2]) : 0.;
-        // Sweep down: the column above every level; checkpoints at the top of each segment.
-        for (int64_t j = 0; j < nlay; ++j)
+        // Sweep down: the column above every level; checkpoints at the top of each segment.  The
+        // inputs of a segment are all requested before its dependent chain starts.
+        for (int64_t s = 0; s < nseg; ++s)
         {
-            if (j % K == 0)
-            {
-                const int64_t s = j / K;
-                ck(s, 0) = Real(R_up); ck(s, 1) = Real(D); ck(s, 2) = Real(dir);
-            }
-            const int64_t l = lb + ncol * lay_of(j);
-            const LayerSW L = sw_layer(tau[l], ssa[l], gasy[l], m0, dir);
-            above_update(L, R_up, D);
-            dir *= L.Tn;
+            const int64_t j0 = s * K;
+            const int nk = int(std::min<int64_t>(K, nlay - j0));
+            ck(s, 0) = Real(R_up); ck(s, 1) = Real(D); ck(s, 2) = Real(dir);
+            Real tv[K], wv[K], gv[K];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+                if (kk < nk)
+                {
+                    const int64_t l = lb + ncol * lay_of(j0 + kk);
+                    tv[kk] = tau[l]; wv[kk] = ssa[l]; gv[kk] = gasy[l];
+                }
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+                if (kk < nk)
+                {
+                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, dir);
+                    above_update(L, R_up, D);
+                    dir *= L.Tn;
+                }
         }
     }
 
@@ -399,7 +502,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     double A = active ? double(alb_dif[c2]) : 0.;
     double S = dir * (active ? double(alb_dir[c2]) : 0.);
     {
-        const double dn = (D + R_up * S) / (1. - R_up * A);
+        const double dn = (D + R_up * S) * rcp_f64(1. - R_up * A);
         const double up = A * dn + S;
         if (active && flux_up)
         {
@@ -423,13 +526,20 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         {
             Real sR[K], sT[K], sSU[K], sSD[K], sRup[K], sD[K], sDir[K];
             double r_up = double(ck(s, 0)), d = double(ck(s, 1)), dr = double(ck(s, 2));
+            Real tv[K], wv[K], gv[K];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+                if (kk < nk)
+                {
+                    const int64_t l = lb + ncol * lay_of(j0 + kk);
+                    tv[kk] = tau[l]; wv[kk] = ssa[l]; gv[kk] = gasy[l];
+                }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
                     sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(dr);
-                    const int64_t l = lb + ncol * lay_of(j0 + kk);
-                    const LayerSW L = sw_layer(tau[l], ssa[l], gasy[l], m0, dr);
+                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, dr);
                     sR[kk] = Real(L.R); sT[kk] = Real(L.T); sSU[kk] = Real(L.SU); sSD[kk] = Real(L.SD);
                     above_update(L, r_up, d);
                     dr *= L.Tn;
@@ -439,11 +549,11 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                 if (kk < nk)
                 {
                     const double R = sR[kk], T = sT[kk];
-                    const double den = 1. / (1. - R * A);
+                    const double den = rcp_f64(1. - R * A);
                     S = double(sSU[kk]) + T * (S + A * double(sSD[kk])) * den;
                     A = R + T * T * A * den;
                     const double ru = sRup[kk];
-                    const double dn = (double(sD[kk]) + ru * S) / (1. - ru * A);
+                    const double dn = (double(sD[kk]) + ru * S) * rcp_f64(1. - ru * A);
                     const double up = A * dn + S;
                     const double dd = sDir[kk];
                     if (flux_up)

@@ -215,6 +215,39 @@ def test_rte_sw_fp64(G, gkd, okd_sw, syn, O, lut_sw, top_at_1):
     assert rel_err(_host(bu), rup.sum(2)) < FP64_RTOL and rel_err(_host(bd), rdn.sum(2)) < FP64_RTOL
 
 
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_sw_solver_fp32_conservative_clouds(G, gkd, okd_sw, syn, O, lut_sw, top_at_1):
+    """Precision study of the SW solver: FP32 and FP64 builds on the FP64 oracle's optical
+    properties, with 10% of the cloudy layers forced to ssa = 1 - 1e-7, g = 0.85 (applied to the
+    cloud two-stream properties before delta-scaling and add_to).  FP32 broadband fluxes within
+    0.05 W m-2 of the FP64 oracle, FP64 within 1e-10 relative (floor 1e-6 W m-2)."""
+    atm = syn.make_atmosphere(35, 64, 128, cloudy=True, top_at_1=top_at_1, conservative_frac=0.1)
+    go = O.gas_optics(okd_sw, atm)
+    ct, cs, cg = O.Cloud(lut_sw).optics(atm.lwp, atm.iwp, atm.rel, atm.rei, True)
+    cons = atm.extra["conservative"]
+    assert cons.sum() > 0
+    cs = np.where(cons[:, :, None], 1.0 - 1e-7, cs)
+    cg = np.where(cons[:, :, None], 0.85, cg)
+    ct, cs, cg = O.delta_scale(ct, cs, cg)
+    tau, ssa, g = O.inc_bybnd(go["tau"], ct, okd_sw.band_lims, go["ssa"], go["g"], cs, cg)
+    rup, rdn, rdr = O.rte_sw(tau, ssa, g, atm.mu0, go["toa_src"], atm.sfc_alb, 0.5 * atm.sfc_alb,
+                             okd_sw.band_lims, top_at_1)
+    ref = {"up": rup.sum(2), "dn": rdn.sum(2), "dir": rdr.sum(2)}
+    errs = {}
+    for prec, dt in ((8, np.float64), (4, np.float32)):
+        dev = lambda a: G.DeviceArray.from_host(a, dt)
+        up, dn, dr = G.rte_sw(gkd[(False, prec)], top_at_1, dev(tau), dev(ssa), dev(g), atm.mu0, go["toa_src"],
+                              atm.sfc_alb, 0.5 * atm.sfc_alb, do_broadband=True)
+        got = {"up": _host(up), "dn": _host(dn), "dir": _host(dr)}
+        for n in ref:
+            if prec == 8:
+                assert rel_err(got[n], ref[n], floor=1e-6) < FP64_RTOL, n
+            else:
+                errs[n] = float(np.abs(got[n] - ref[n]).max())
+    print("FP32 SW solver, conservative cloud layers: max |flux error| W m-2", errs)
+    assert max(errs.values()) < 0.05, errs
+
+
 def test_add_to_and_fluxes(G, gkd, okd_sw, O):
     rng = np.random.default_rng(5)
     k = gkd[(False, 8)]
