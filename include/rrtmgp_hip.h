@@ -120,6 +120,18 @@ typedef struct
     const void* totplnk;                   /* (nplanck, nbnd)          LW only              */
     const void* planck_frac;               /* (ntemp, neta, npres+1, ngpt)  LW only         */
     const void* krayl;                     /* (ntemp, neta, ngpt, 2)   SW only              */
+    /* The gathered tables again with the g-point (contributor) index fastest, for the fused   */
+    /* gas-optics kernels, whose lanes run over the g-points of a band.                        */
+    const void* kmajor_g;                  /* (ngpt, npres+1, neta, ntemp)                  */
+    const void* planck_frac_g;             /* (ngpt, npres+1, neta, ntemp)  LW only         */
+    const void* kminor_lower_g;            /* (ncontrib_lower, neta, ntemp)                 */
+    const void* kminor_upper_g;            /* (ncontrib_upper, neta, ntemp)                 */
+    const void* krayl_g;                   /* (ngpt, neta, ntemp, 2)   SW only              */
+    int64_t ncontrib_lower, ncontrib_upper;
+    int64_t max_band_gpts;                 /* largest number of g-points in one band        */
+    int64_t max_band_minors;               /* most minor intervals touching one band        */
+    int64_t minor_flavor_uniform;          /* 1 if every minor interval touching a band has */
+                                           /* the band's flavor (intervals within bands)    */
 } rrtmgp_kdist_dev;
 
 /* Volume mixing ratios in the k-distribution's gas order; dim1 is 1 or ncol, dim2 is 1 or   */

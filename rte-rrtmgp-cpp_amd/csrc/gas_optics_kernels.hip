@@ -4,13 +4,16 @@
 // (interpolation :215-293, major :296-341, minor :356-476, Rayleigh :572-616, Planck :129-212,
 // interpolate1D :1-12) with the CPU combine rule of src/Gas_optics_rrtmgp.cpp:365-385.
 //
-// Design.  The fused kernels give one thread a (column, layer, band): the interpolation state
-// (temperature/pressure/eta indices and weights) is computed once for the band's flavor and
-// reused for all of the band's g-points; nothing of it is written to memory.  Lanes run over
-// columns, so every g-point store of tau / sources is one contiguous 256-byte row of the
-// reference layout (ncol, nlay, ngpt).  k-table gathers of a wavefront hit a few cache lines
-// (neighbouring columns share pressure and nearly share temperature), and work-groups of one
-// band are kept on one XCD so its L2 holds that band's k-table slice.
+// Design.  In the fused kernels a work-group (4 wavefronts) computes one layer of 64 columns
+// for one chunk of at most 16 g-points of a band.  Lane l of a wavefront takes g-point l % 16 of
+// pair l / 16 of its current four (column, layer) pairs.  The k-tables are gathered from copies
+// with the g-point index fastest, so the 16 lanes of a pair read one contiguous 64-byte segment
+// per table corner (4 segments per wavefront gather).  A pair's interpolation state (indices and
+// weights of the band's flavor) is computed by its 16 lanes together, one wavefront instruction
+// per four pairs, and never written to memory.  Results pass through LDS (rows padded to 68
+// floats: conflict-free for this lane layout) and leave as 256-byte column rows of the reference
+// layout (ncol, nlay, ngpt).  Work-groups of one chunk are consecutive and kept on one XCD, so
+// that XCD's L2 holds the chunk's table slice.
 #include <algorithm>
 #include <vector>
 
@@ -18,9 +21,10 @@
 
 namespace rrtmgp_hip
 {
-constexpr int GO_BX = 64;   // columns per work-group (one wavefront)
-constexpr int GO_BY = 4;    // layers per work-group
-constexpr int GCHUNK = 16;  // g-points held in registers at once
+constexpr int GO_THREADS = 256;   // work-group: 4 wavefronts
+constexpr int GO_COLS = 64;       // columns of a work-group's layer
+constexpr int GO_GPTS = 16;       // g-points per chunk: the lanes of one pair
+constexpr int GO_ROW = 68;        // LDS row length (64 columns + padding)
 
 template<typename Real>
 struct Kd
@@ -47,6 +51,13 @@ struct Kd
     const Real* totplnk;
     const Real* planck_frac;
     const Real* krayl;
+    // g-point-fastest copies and their strides
+    const Real* kmajor_g;          // (ngpt, npres+1, neta, ntemp)
+    const Real* planck_frac_g;
+    const Real* kminor_g[2];       // (ncontrib, neta, ntemp)
+    const Real* krayl_g;           // (ngpt, neta, ntemp, 2)
+    int64_t ncontrib[2];
+    int max_band_gpts;
 };
 
 template<typename Real>
@@ -80,6 +91,13 @@ Kd<Real> make_kd(const rrtmgp_kdist_dev& d)
     k.totplnk = static_cast<const Real*>(d.totplnk);
     k.planck_frac = static_cast<const Real*>(d.planck_frac);
     k.krayl = static_cast<const Real*>(d.krayl);
+    k.kmajor_g = static_cast<const Real*>(d.kmajor_g);
+    k.planck_frac_g = static_cast<const Real*>(d.planck_frac_g);
+    k.kminor_g[0] = static_cast<const Real*>(d.kminor_lower_g);
+    k.kminor_g[1] = static_cast<const Real*>(d.kminor_upper_g);
+    k.krayl_g = static_cast<const Real*>(d.krayl_g);
+    k.ncontrib[0] = d.ncontrib_lower; k.ncontrib[1] = d.ncontrib_upper;
+    k.max_band_gpts = int(d.max_band_gpts);
     return k;
 }
 
@@ -241,69 +259,195 @@ __device__ __forceinline__ Real minor_scaling(const Kd<Real>& k, const Gases<Rea
     return scaling;
 }
 
-// Adds the minor contributions of set s (itropo) to the g-points [g0, g0+GCHUNK) of one band.
+// ---------------------------------------------------------------------------------------------
+// Gathers from the g-fastest tables for g-point gg (same arithmetic order as major_tau,
+// planck_fraction and the minor / Rayleigh sums of the reference-layout kernels).
+// The g-fastest tables hold ntemp * neta * (npres+1) * ngpt < 2^31 entries (checked at launch),
+// so table offsets are 32-bit.
 template<typename Real>
-__device__ __forceinline__ void add_minor_chunk(const Kd<Real>& k, const Gases<Real>& g, const CL<Real>& c,
-                                                const FL<Real>& fb, int band_flav, int ibnd, int g0,
-                                                int64_t icol, int64_t ilay, Real p, Real t, Real col_dry,
-                                                Real (&tau)[GCHUNK])
+__device__ __forceinline__ Real major_tau_g(const Kd<Real>& k, const CL<Real>& c, const FL<Real>& f, int gg)
 {
+    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt, sT = sE * k.neta;
+    const int jp = c.jpress + c.itropo - 1;
+    const Real* km = k.kmajor_g;
+    Real tau = Real(0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        const int o = (c.jtemp - 1 + i) * sT + (f.jeta[i] - 1) * sE + jp * sP + gg;
+        tau += f.col_mix[i] * (f.fmajor[i][0] * km[o] + f.fmajor[i][1] * km[o + sE] +
+                               f.fmajor[i][2] * km[o + sP] + f.fmajor[i][3] * km[o + sP + sE]);
+    }
+    return tau;
+}
+
+template<typename Real>
+__device__ __forceinline__ Real planck_fraction_g(const Kd<Real>& k, const CL<Real>& c, const FL<Real>& f, int gg)
+{
+    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt, sT = sE * k.neta;
+    const int jp = c.jpress + c.itropo - 1;
+    const Real* pf = k.planck_frac_g;
+    const int o0 = (c.jtemp - 1) * sT + (f.jeta[0] - 1) * sE + jp * sP + gg;
+    const int o1 = (c.jtemp) * sT + (f.jeta[1] - 1) * sE + jp * sP + gg;
+    return (f.fmajor[0][0] * pf[o0] + f.fmajor[0][1] * pf[o0 + sE] + f.fmajor[0][2] * pf[o0 + sP] + f.fmajor[0][3] * pf[o0 + sP + sE])
+         + (f.fmajor[1][0] * pf[o1] + f.fmajor[1][1] * pf[o1 + sE] + f.fmajor[1][2] * pf[o1 + sP] + f.fmajor[1][3] * pf[o1 + sP + sE]);
+}
+
+// Grid of the fused kernels: units (chunk, layer, 64-column tile), chunk = (band, 16-g-point
+// part of the band), ordered so that consecutive units share a chunk.
+struct GoGrid
+{
+    int64_t ncoltiles, nlay, cpb, nunits;
+};
+
+inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t nbnd, int64_t max_band_gpts)
+{
+    GoGrid t;
+    t.ncoltiles = (ncol + GO_COLS - 1) / GO_COLS;
+    t.nlay = nlay;
+    t.cpb = std::max<int64_t>(1, (max_band_gpts + GO_GPTS - 1) / GO_GPTS);
+    t.nunits = t.ncoltiles * nlay * nbnd * t.cpb;
+    return t;
+}
+
+struct GoUnit
+{
+    int ibnd, gs, ge;
+    int64_t ilay, col0;
+};
+
+template<typename Real>
+__device__ __forceinline__ GoUnit go_unit(const Kd<Real>& k, const GoGrid& t, int64_t u)
+{
+    GoUnit r;
+    const int64_t rest = u / t.ncoltiles;
+    r.col0 = (u % t.ncoltiles) * GO_COLS;
+    r.ilay = rest % t.nlay;
+    const int64_t chunk = rest / t.nlay;
+    r.ibnd = int(chunk / t.cpb);
+    const int gs_b = k.band_lims_gpt[2 * r.ibnd] - 1, ge_b = k.band_lims_gpt[2 * r.ibnd + 1];
+    r.gs = gs_b + GO_GPTS * int(chunk % t.cpb);
+    r.ge = min(ge_b, r.gs + GO_GPTS);
+    return r;
+}
+
+constexpr int GO_MAXM = 8;   // minor intervals of one band per pair held in LDS
+
+// State of one (column, layer) pair for one band, computed once per pair (phase 1 of the fused
+// kernels) and read by the 16 g-point lanes of the pair from LDS (phase 2).
+template<typename Real>
+struct PairState
+{
+    int om[2];           // kmajor_g / planck_frac_g offsets for itemp 0/1, without the g-point
+    int mn[2];           // kminor_g offsets for itemp 0/1 (table of half itropo), without contributor
+    int ry[2];           // krayl_g offsets for itemp 0/1, without the g-point
+    int itropo;
+    int flags;           // bit 0: valid column
+    Real col_mix[2];
+    Real fmajor[2][4];
+    Real fminor[2][2];
+    Real scaling[GO_MAXM];
+    Real aux[5];         // LW: B(T_lay), B(T_lev above), B(T_lev below), B(T_sfc), B(T_sfc + 1); SW: col_ray
+};
+
+// Phase 1 for one pair.
+template<typename Real, bool LW>
+__device__ __forceinline__ void pair_state(const Kd<Real>& k, const Gases<Real>& g, int ibnd, int band_g0,
+                                           int64_t icol, int64_t ilay, int64_t ncol, const Real* play,
+                                           const Real* tlay, const Real* tlev, const Real* tsfc,
+                                           const Real* col_dry_a, bool at_sfc, PairState<Real>& st)
+{
+    const int64_t idx = icol + ncol * ilay;
+    const Real pv = play[idx], tv = tlay[idx], cd = col_dry_a[idx];
+    const CL<Real> c = interp_cl(k, pv, tv);
+    const int bflav = k.gpoint_flavor[2 * band_g0 + c.itropo] - 1;
+    const int gas1 = k.flavor[2 * bflav], gas2 = k.flavor[2 * bflav + 1];
+    FL<Real> f;
+    interp_flav(k, c, gas1, gas2, col_of(g, gas1, icol, ilay, cd), col_of(g, gas2, icol, ilay, cd), f);
+    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt, sT = sE * k.neta;
+    const int jp = c.jpress + c.itropo - 1;
     const int s = c.itropo;
+    const int mE = int(k.ncontrib[s]), mT = mE * k.neta;
+    const int rE = k.ngpt, rT = k.neta * k.ngpt, rS = rT * k.ntemp;
+    int flags = 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        st.om[i] = (c.jtemp - 1 + i) * sT + (f.jeta[i] - 1) * sE + jp * sP;
+        st.mn[i] = (c.jtemp - 1 + i) * mT + (f.jeta[i] - 1) * mE;
+        st.ry[i] = s * rS + (c.jtemp - 1 + i) * rT + (f.jeta[i] - 1) * rE;
+        st.col_mix[i] = f.col_mix[i];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) st.fmajor[i][q] = f.fmajor[i][q];
+        st.fminor[i][0] = f.fminor[i][0];
+        st.fminor[i][1] = f.fminor[i][1];
+    }
+    st.itropo = s;
+    // The launch checks that a band has at most GO_MAXM minor intervals, all of its flavor.
     const int q0 = k.band_start[s][ibnd], q1 = k.band_start[s][ibnd + 1];
+    for (int q = q0; q < q1; ++q)
+        st.scaling[q - q0] = minor_scaling(k, g, s, k.band_list[s][q], icol, ilay, pv, tv, cd);
+    st.flags = flags;
+    if constexpr (LW)
+    {
+        const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
+        st.aux[0] = interpolate1D(tv, k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        st.aux[1] = interpolate1D(tlev[icol + ncol * (ilay + 1)], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        st.aux[2] = interpolate1D(tlev[idx], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        if (at_sfc)
+        {
+            st.aux[3] = interpolate1D(tsfc[icol], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+            st.aux[4] = interpolate1D(tsfc[icol] + Real(1), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        }
+    }
+    else
+        st.aux[0] = col_of(g, k.idx_h2o, icol, ilay, cd) + cd;
+}
+
+// Phase 2: major + minor absorption of g-point gg from a pair's state (same arithmetic order as
+// major_tau and the minor sum of tau_absorption_kernel).
+template<typename Real>
+__device__ __forceinline__ Real absorption_from_state(const Kd<Real>& k, const PairState<Real>& st, int ibnd, int gg)
+{
+    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt;
+    const Real* km = k.kmajor_g;
+    Real tau = Real(0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        const int o = st.om[i] + gg;
+        tau += st.col_mix[i] * (st.fmajor[i][0] * km[o] + st.fmajor[i][1] * km[o + sE] +
+                                st.fmajor[i][2] * km[o + sP] + st.fmajor[i][3] * km[o + sP + sE]);
+    }
+    const int s = st.itropo;
+    const int q0 = k.band_start[s][ibnd], q1 = k.band_start[s][ibnd + 1];
+    const int mE = int(k.ncontrib[s]);
+    const Real* kn = k.kminor_g[s];
+    Real minor = Real(0);
     for (int q = q0; q < q1; ++q)
     {
         const int m = k.band_list[s][q];
         const int gs = k.minor_limits[s][2 * m] - 1, ge = k.minor_limits[s][2 * m + 1];
-        if (ge <= g0 || gs >= g0 + GCHUNK) continue;
-        const Real scaling = minor_scaling(k, g, s, m, icol, ilay, p, t, col_dry);
-        const int iflav = k.gpoint_flavor[2 * gs + s] - 1;
-        FL<Real> fo;
-        const FL<Real>* f = &fb;
-        if (iflav != band_flav)
+        if (gg >= gs && gg < ge)
         {
-            const int g1 = k.flavor[2 * iflav], g2 = k.flavor[2 * iflav + 1];
-            interp_flav(k, c, g1, g2, col_of(g, g1, icol, ilay, col_dry), col_of(g, g2, icol, ilay, col_dry), fo);
-            f = &fo;
-        }
-        const int j0 = f->jeta[0], j1 = f->jeta[1];
-        const int goff = k.kstart[s][m] - 1;
-        const int64_t se = k.ntemp, sc = int64_t(k.ntemp) * k.neta;
-        const Real* kb0 = k.kminor[s] + (c.jtemp - 1) + se * (j0 - 1);
-        const Real* kb1 = k.kminor[s] + (c.jtemp) + se * (j1 - 1);
-#pragma unroll
-        for (int j = 0; j < GCHUNK; ++j)
-        {
-            const int gg = g0 + j;
-            if (gg >= gs && gg < ge)
-            {
-                const int64_t off = sc * (gg - gs + goff);
-                const Real lt = f->fminor[0][0] * kb0[off] + f->fminor[0][1] * kb0[off + se] +
-                                f->fminor[1][0] * kb1[off] + f->fminor[1][1] * kb1[off + se];
-                tau[j] += lt * scaling;
-            }
+            const int ic = k.kstart[s][m] - 1 + (gg - gs);
+            const int o0 = st.mn[0] + ic, o1 = st.mn[1] + ic;
+            const Real lt = st.fminor[0][0] * kn[o0] + st.fminor[0][1] * kn[o0 + mE] +
+                            st.fminor[1][0] * kn[o1] + st.fminor[1][1] * kn[o1 + mE];
+            minor += lt * st.scaling[q - q0];
         }
     }
-}
-
-struct Tile
-{
-    int64_t ncoltiles, nlaytiles, nunits;
-};
-
-inline Tile make_tile(int64_t ncol, int64_t nlay, int64_t nbnd)
-{
-    Tile t;
-    t.ncoltiles = (ncol + GO_BX - 1) / GO_BX;
-    t.nlaytiles = (nlay + GO_BY - 1) / GO_BY;
-    t.nunits = t.ncoltiles * t.nlaytiles * nbnd;
-    return t;
+    return tau + minor;
 }
 
 // ---------------------------------------------------------------------------------------------
 // Fused longwave gas optics: tau (+ cloud absorption per band) and Planck sources.
+//   phase 1 (64 threads, lane = column): pair states -> LDS;
+//   phase 2 (all lanes, lane = (pair, g-point)): absorption and Planck fraction -> LDS;
+//   phase 3 (lane = column, wavefront = g-point row): sources, cloud add_to, coalesced stores.
 template<typename Real>
-__global__ __launch_bounds__(GO_BX * GO_BY)
-void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, Tile tile,
+__global__ __launch_bounds__(GO_THREADS)
+void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, GoGrid grid,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ tlev, const Real* __restrict__ tsfc,
                           const Real* __restrict__ col_dry_a, int sfc_lay,
@@ -312,138 +456,161 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                           Real* __restrict__ lev_dec, Real* __restrict__ sfc_src,
                           Real* __restrict__ sfc_jac)
 {
-    const int64_t u = xcd_remap(blockIdx.x, tile.nunits);
-    if (u >= tile.nunits) return;
-    const int64_t per_band = tile.ncoltiles * tile.nlaytiles;
-    const int ibnd = int(u / per_band);
-    const int64_t r = u % per_band;
-    const int64_t icol = (r % tile.ncoltiles) * GO_BX + threadIdx.x;
-    const int64_t ilay = (r / tile.ncoltiles) * GO_BY + threadIdx.y;
-    if (icol >= ncol || ilay >= nlay) return;
-
-    const int64_t idx = icol + ncol * ilay;
-    const int64_t ncl = ncol * nlay;
-    const Real p = play[idx], t = tlay[idx], cd = col_dry_a[idx];
-    const CL<Real> c = interp_cl(k, p, t);
-
-    const int gs_b = k.band_lims_gpt[2 * ibnd] - 1, ge_b = k.band_lims_gpt[2 * ibnd + 1];
-    const int bflav = k.gpoint_flavor[2 * gs_b + c.itropo] - 1;
-    const int gas1 = k.flavor[2 * bflav], gas2 = k.flavor[2 * bflav + 1];
-    FL<Real> f;
-    interp_flav(k, c, gas1, gas2, col_of(g, gas1, icol, ilay, cd), col_of(g, gas2, icol, ilay, cd), f);
-
-    const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
-    const Real b_lay = interpolate1D(t, k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-    const Real b_inc = interpolate1D(tlev[icol + ncol * (ilay + 1)], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-    const Real b_dec = interpolate1D(tlev[idx], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+    __shared__ PairState<Real> sst[GO_COLS];
+    __shared__ Real s_tau[GO_GPTS][GO_ROW];
+    __shared__ Real s_pf[GO_GPTS][GO_ROW];
+    const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
+    if (u >= grid.nunits) return;                       // uniform over the work-group
+    const GoUnit un = go_unit(k, grid, u);
+    const int ng = un.ge - un.gs;
+    if (ng <= 0) return;                                // uniform: an empty chunk of a short band
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int64_t ilay = un.ilay, ncl = ncol * nlay;
     const bool at_sfc = (ilay == sfc_lay - 1);
-    Real b_sfc = Real(0), b_sfc1 = Real(0);
-    if (at_sfc)
-    {
-        b_sfc = interpolate1D(tsfc[icol], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-        b_sfc1 = interpolate1D(tsfc[icol] + Real(1), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-    }
-    const Real ctau = cloud_tau ? cloud_tau[idx + ncl * ibnd] : Real(0);
 
-    for (int g0 = gs_b; g0 < ge_b; g0 += GCHUNK)
+    if (w == 0)
     {
-        Real tg[GCHUNK];
-#pragma unroll
-        for (int j = 0; j < GCHUNK; ++j)
-            tg[j] = (g0 + j < ge_b) ? major_tau(k, c, f, g0 + j) : Real(0);
-        add_minor_chunk(k, g, c, f, bflav, ibnd, g0, icol, ilay, p, t, cd, tg);
-#pragma unroll
-        for (int j = 0; j < GCHUNK; ++j)
+        const int64_t icol = un.col0 + lane;
+        if (icol < ncol)
+            pair_state<Real, true>(k, g, un.ibnd, un.gs, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a,
+                                   at_sfc, sst[lane]);
+        else
+            sst[lane].flags = 0;
+    }
+    __syncthreads();
+
+    const int p = lane / GO_GPTS, gl = lane % GO_GPTS;
+    const int gg = un.gs + gl;
+    for (int it = 0; it < GO_COLS / 16; ++it)
+    {
+        const int pi = it * 16 + w * 4 + p;
+        const PairState<Real>& st = sst[pi];
+        Real t = Real(0), pf = Real(0);
+        if ((st.flags & 1) && gl < ng)
         {
-            const int gg = g0 + j;
-            if (gg >= ge_b) break;
-            const int64_t o = idx + ncl * gg;
-            tau[o] = cloud_tau ? tg[j] + ctau : tg[j];
-            const Real pf = planck_fraction(k, c, f, gg);
-            lay_src[o] = pf * b_lay;
-            lev_inc[o] = pf * b_inc;
-            lev_dec[o] = pf * b_dec;
-            if (at_sfc)
-            {
-                sfc_src[icol + ncol * gg] = pf * b_sfc;
-                sfc_jac[icol + ncol * gg] = pf * (b_sfc1 - b_sfc);
-            }
+            t = absorption_from_state(k, st, un.ibnd, gg);
+            const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt;
+            const Real* pfr = k.planck_frac_g;
+            const int o0 = st.om[0] + gg, o1 = st.om[1] + gg;
+            pf = (st.fmajor[0][0] * pfr[o0] + st.fmajor[0][1] * pfr[o0 + sE] + st.fmajor[0][2] * pfr[o0 + sP] +
+                  st.fmajor[0][3] * pfr[o0 + sP + sE])
+               + (st.fmajor[1][0] * pfr[o1] + st.fmajor[1][1] * pfr[o1 + sE] + st.fmajor[1][2] * pfr[o1 + sP] +
+                  st.fmajor[1][3] * pfr[o1 + sP + sE]);
+        }
+        s_tau[gl][pi] = t;
+        s_pf[gl][pi] = pf;
+    }
+    __syncthreads();
+
+    const int64_t icol = un.col0 + lane;
+    if (icol >= ncol) return;
+    const PairState<Real>& st = sst[lane];
+    const int64_t idx = icol + ncol * ilay;
+    const Real ctau = cloud_tau ? cloud_tau[idx + ncl * un.ibnd] : Real(0);
+    for (int row = w; row < ng; row += GO_THREADS / 64)
+    {
+        const int gr = un.gs + row;
+        const int64_t o = idx + ncl * gr;
+        const Real pf = s_pf[row][lane];
+        tau[o] = cloud_tau ? s_tau[row][lane] + ctau : s_tau[row][lane];
+        lay_src[o] = pf * st.aux[0];
+        lev_inc[o] = pf * st.aux[1];
+        lev_dec[o] = pf * st.aux[2];
+        if (at_sfc)
+        {
+            sfc_src[icol + ncol * gr] = pf * st.aux[3];
+            sfc_jac[icol + ncol * gr] = pf * (st.aux[4] - st.aux[3]);
         }
     }
 }
 
-// Fused shortwave gas optics: absorption + Rayleigh + combine (+ band cloud add_to).
+// Fused shortwave gas optics: absorption + Rayleigh + combine (+ band cloud add_to), in the
+// same three phases.
 template<typename Real>
-__global__ __launch_bounds__(GO_BX * GO_BY)
-void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, Tile tile,
+__global__ __launch_bounds__(GO_THREADS)
+void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, GoGrid grid,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ col_dry_a, const Real* __restrict__ ctau_a,
                           const Real* __restrict__ cssa_a, const Real* __restrict__ cg_a,
                           Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ gout)
 {
-    const int64_t u = xcd_remap(blockIdx.x, tile.nunits);
-    if (u >= tile.nunits) return;
-    const int64_t per_band = tile.ncoltiles * tile.nlaytiles;
-    const int ibnd = int(u / per_band);
-    const int64_t r = u % per_band;
-    const int64_t icol = (r % tile.ncoltiles) * GO_BX + threadIdx.x;
-    const int64_t ilay = (r / tile.ncoltiles) * GO_BY + threadIdx.y;
-    if (icol >= ncol || ilay >= nlay) return;
+    __shared__ PairState<Real> sst[GO_COLS];
+    __shared__ Real s_abs[GO_GPTS][GO_ROW];
+    __shared__ Real s_ray[GO_GPTS][GO_ROW];
+    const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
+    if (u >= grid.nunits) return;
+    const GoUnit un = go_unit(k, grid, u);
+    const int ng = un.ge - un.gs;
+    if (ng <= 0) return;
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int64_t ilay = un.ilay, ncl = ncol * nlay;
 
+    if (w == 0)
+    {
+        const int64_t icol = un.col0 + lane;
+        if (icol < ncol)
+            pair_state<Real, false>(k, g, un.ibnd, un.gs, icol, ilay, ncol, play, tlay, nullptr, nullptr,
+                                    col_dry_a, false, sst[lane]);
+        else
+            sst[lane].flags = 0;
+    }
+    __syncthreads();
+
+    const int p = lane / GO_GPTS, gl = lane % GO_GPTS;
+    const int gg = un.gs + gl;
+    const int rE = k.ngpt;
+    for (int it = 0; it < GO_COLS / 16; ++it)
+    {
+        const int pi = it * 16 + w * 4 + p;
+        const PairState<Real>& st = sst[pi];
+        Real tg = Real(0), tray = Real(0);
+        if ((st.flags & 1) && gl < ng)
+        {
+            tg = absorption_from_state(k, st, un.ibnd, gg);
+            // Rayleigh (.cu:572-616).
+            const Real* kr = k.krayl_g;
+            const int o0 = st.ry[0] + gg, o1 = st.ry[1] + gg;
+            const Real kloc = st.fminor[0][0] * kr[o0] + st.fminor[0][1] * kr[o0 + rE] +
+                              st.fminor[1][0] * kr[o1] + st.fminor[1][1] * kr[o1 + rE];
+            tray = kloc * st.aux[0];
+        }
+        s_abs[gl][pi] = tg;
+        s_ray[gl][pi] = tray;
+    }
+    __syncthreads();
+
+    const int64_t icol = un.col0 + lane;
+    if (icol >= ncol) return;
     const int64_t idx = icol + ncol * ilay;
-    const int64_t ncl = ncol * nlay;
-    const Real p = play[idx], t = tlay[idx], cd = col_dry_a[idx];
-    const CL<Real> c = interp_cl(k, p, t);
-    const int gs_b = k.band_lims_gpt[2 * ibnd] - 1, ge_b = k.band_lims_gpt[2 * ibnd + 1];
-    const int bflav = k.gpoint_flavor[2 * gs_b + c.itropo] - 1;
-    const int gas1 = k.flavor[2 * bflav], gas2 = k.flavor[2 * bflav + 1];
-    FL<Real> f;
-    interp_flav(k, c, gas1, gas2, col_of(g, gas1, icol, ilay, cd), col_of(g, gas2, icol, ilay, cd), f);
-
-    const Real col_ray = col_of(g, k.idx_h2o, icol, ilay, cd) + cd;
-    const int64_t se = k.ntemp;
-    const Real* kr = k.krayl + int64_t(c.itropo) * k.ntemp * k.neta * k.ngpt;
     const Real eps2 = Real(2) * Limits<Real>::eps;
     const Real op_eps = Real(3) * Limits<Real>::tiny;
-    Real t2 = 0, s2 = 0, g2 = 0;
-    if (ctau_a) { t2 = ctau_a[idx + ncl * ibnd]; s2 = cssa_a[idx + ncl * ibnd]; g2 = cg_a[idx + ncl * ibnd]; }
-
-    for (int g0 = gs_b; g0 < ge_b; g0 += GCHUNK)
+    Real t2 = Real(0), s2 = Real(0), g2 = Real(0);
+    if (ctau_a)
     {
-        Real tg[GCHUNK];
-#pragma unroll
-        for (int j = 0; j < GCHUNK; ++j)
-            tg[j] = (g0 + j < ge_b) ? major_tau(k, c, f, g0 + j) : Real(0);
-        add_minor_chunk(k, g, c, f, bflav, ibnd, g0, icol, ilay, p, t, cd, tg);
-#pragma unroll
-        for (int j = 0; j < GCHUNK; ++j)
+        const int64_t ib = idx + ncl * un.ibnd;
+        t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
+    }
+    for (int row = w; row < ng; row += GO_THREADS / 64)
+    {
+        const Real tray = s_ray[row][lane];
+        const Real tt = s_abs[row][lane] + tray;
+        // The CPU combine rule (Gas_optics_rrtmgp.cpp:365-385).
+        Real tt_out = tt;
+        Real ss = (tt > eps2) ? tray / tt : Real(0);
+        Real gg_out = Real(0);
+        if (ctau_a)
         {
-            const int gg = g0 + j;
-            if (gg >= ge_b) break;
-            const int64_t sg = int64_t(k.ntemp) * k.neta * gg;
-            const Real* k0 = kr + sg + (c.jtemp - 1) + se * (f.jeta[0] - 1);
-            const Real* k1 = kr + sg + (c.jtemp) + se * (f.jeta[1] - 1);
-            const Real kloc = f.fminor[0][0] * k0[0] + f.fminor[0][1] * k0[se] +
-                              f.fminor[1][0] * k1[0] + f.fminor[1][1] * k1[se];
-            const Real tray = kloc * col_ray;
-            const Real tt = tg[j] + tray;
-            Real tt_out = tt;
-            Real ss = (tt > eps2) ? tray / tt : Real(0);
-            Real gg_out = Real(0);
-            if (ctau_a)
-            {
-                // inc_2stream_by_2stream_bybnd (optical_props_kernels.cu:99-136).
-                const Real tau12 = tt + t2;
-                const Real tauscat12 = (tt * ss) + (t2 * s2);
-                gg_out = ((tt * ss * Real(0)) + (t2 * s2 * g2)) / max(tauscat12, op_eps);
-                ss = tauscat12 / max(op_eps, tau12);
-                tt_out = tau12;
-            }
-            const int64_t o = idx + ncl * gg;
-            tau[o] = tt_out;
-            ssa[o] = ss;
-            gout[o] = gg_out;
+            // inc_2stream_by_2stream_bybnd (optical_props_kernels.cu:99-136).
+            const Real tau12 = tt + t2;
+            const Real tauscat12 = (tt * ss) + (t2 * s2);
+            gg_out = ((tt * ss * Real(0)) + (t2 * s2 * g2)) / max(tauscat12, op_eps);
+            ss = tauscat12 / max(op_eps, tau12);
+            tt_out = tau12;
         }
+        const int64_t o = idx + ncl * (un.gs + row);
+        tau[o] = tt_out;
+        ssa[o] = ss;
+        gout[o] = gg_out;
     }
 }
 
@@ -682,6 +849,24 @@ int check_kd(const rrtmgp_kdist_dev* kd, bool need_planck, bool need_rayleigh)
     return RRTMGP_OK;
 }
 
+// The fused kernels also read the g-fastest table copies.
+int check_kd_fused(const rrtmgp_kdist_dev* kd, bool lw)
+{
+    RRTMGP_TRY(check_kd(kd, lw, !lw));
+    if (!kd->kmajor_g || (lw && !kd->planck_frac_g) || (!lw && !kd->krayl_g))
+        return arg_error("k-distribution has no g-point-fastest tables");
+    if ((kd->nminor_lower > 0 && !kd->kminor_lower_g) || (kd->nminor_upper > 0 && !kd->kminor_upper_g))
+        return arg_error("k-distribution has no g-point-fastest minor tables");
+    if (kd->max_band_gpts < 1 || kd->max_band_gpts > kd->ngpt) return arg_error("max_band_gpts out of range");
+    if (kd->max_band_minors > GO_MAXM || !kd->minor_flavor_uniform)
+        return arg_error("fused gas optics: a band has more than " + std::to_string(GO_MAXM) +
+                         " minor intervals or an interval crossing into another band's flavor");
+    const int64_t nmaj = kd->ntemp * kd->neta * (kd->npres + 1) * kd->ngpt;
+    const int64_t nmin = kd->ntemp * kd->neta * std::max(kd->ncontrib_lower, kd->ncontrib_upper);
+    if (nmaj >= (int64_t(1) << 31) || nmin >= (int64_t(1) << 31)) return arg_error("k-tables too large for 32-bit offsets");
+    return RRTMGP_OK;
+}
+
 template<typename Real>
 int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const Real* play,
                   const Real* tlay, const Real* tlev, const Real* tsfc, const Real* col_dry,
@@ -689,7 +874,7 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
                   Real* lay_src, Real* lev_inc, Real* lev_dec, Real* sfc_src, Real* sfc_jac,
                   void* stream)
 {
-    RRTMGP_TRY(check_kd(kd, true, false));
+    RRTMGP_TRY(check_kd_fused(kd, true));
     if (ncol <= 0 || nlay <= 0) return arg_error("ncol and nlay must be positive");
     if (sfc_lay != 1 && sfc_lay != nlay) return arg_error("sfc_lay must be 1 or nlay");
     if (!play || !tlay || !tlev || !tsfc || !col_dry || !tau || !lay_src || !lev_inc || !lev_dec || !sfc_src || !sfc_jac)
@@ -697,10 +882,10 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     Gases<Real> g;
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
     const Kd<Real> k = make_kd<Real>(*kd);
-    const Tile tile = make_tile(ncol, nlay, kd->nbnd);
+    const GoGrid grid = make_go_grid(ncol, nlay, kd->nbnd, kd->max_band_gpts);
     RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
-    hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(tile.nunits)), dim3(GO_BX, GO_BY), 0,
-                       as_stream(stream), k, g, ncol, nlay, tile, play, tlay, tlev, tsfc, col_dry,
+    hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
+                       as_stream(stream), k, g, ncol, nlay, grid, play, tlay, tlev, tsfc, col_dry,
                        sfc_lay, cloud_tau, tau, lay_src, lev_inc, lev_dec, sfc_src, sfc_jac);
     RRTMGP_LAUNCHED("gas_optics_lw_kernel");
     return RRTMGP_OK;
@@ -712,17 +897,17 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
                   const Real* ctau, const Real* cssa, const Real* cg, Real* tau, Real* ssa, Real* g,
                   void* stream)
 {
-    RRTMGP_TRY(check_kd(kd, false, true));
+    RRTMGP_TRY(check_kd_fused(kd, false));
     if (ncol <= 0 || nlay <= 0) return arg_error("ncol and nlay must be positive");
     if (!play || !tlay || !col_dry || !tau || !ssa || !g) return arg_error("gas_optics_sw: NULL array");
     if (ctau && (!cssa || !cg)) return arg_error("cloud ssa/g missing");
     Gases<Real> gs;
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, gs));
     const Kd<Real> k = make_kd<Real>(*kd);
-    const Tile tile = make_tile(ncol, nlay, kd->nbnd);
+    const GoGrid grid = make_go_grid(ncol, nlay, kd->nbnd, kd->max_band_gpts);
     RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
-    hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(tile.nunits)), dim3(GO_BX, GO_BY), 0,
-                       as_stream(stream), k, gs, ncol, nlay, tile, play, tlay, col_dry, ctau, cssa, cg,
+    hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
+                       as_stream(stream), k, gs, ncol, nlay, grid, play, tlay, col_dry, ctau, cssa, cg,
                        tau, ssa, g);
     RRTMGP_LAUNCHED("gas_optics_sw_kernel");
     return RRTMGP_OK;
