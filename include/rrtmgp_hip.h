@@ -113,26 +113,34 @@ typedef struct
     const int* idx_minor_scaling_upper;
     const int* kminor_start_lower;         /* 1-based contributor offset                    */
     const int* kminor_start_upper;
-    const int* minor_band_start_lower;     /* (nbnd+1) offsets into minor_band_list_lower   */
-    const int* minor_band_list_lower;      /* 0-based minor intervals touching each band     */
-    const int* minor_band_start_upper;
-    const int* minor_band_list_upper;
-    const void* totplnk;                   /* (nplanck, nbnd)          LW only              */
+    const void* totplnk;                  /* (nplanck, nbnd)          LW only              */
     const void* planck_frac;               /* (ntemp, neta, npres+1, ngpt)  LW only         */
     const void* krayl;                     /* (ntemp, neta, ngpt, 2)   SW only              */
-    /* The gathered tables again with the g-point (contributor) index fastest, for the fused   */
-    /* gas-optics kernels, whose lanes run over the g-points of a band.                        */
-    const void* kmajor_g;                  /* (ngpt, npres+1, neta, ntemp)                  */
-    const void* planck_frac_g;             /* (ngpt, npres+1, neta, ntemp)  LW only         */
-    const void* kminor_lower_g;            /* (ncontrib_lower, neta, ntemp)                 */
-    const void* kminor_upper_g;            /* (ncontrib_upper, neta, ntemp)                 */
-    const void* krayl_g;                   /* (ngpt, neta, ntemp, 2)   SW only              */
-    int64_t ncontrib_lower, ncontrib_upper;
+    int64_t ncontrib_lower, ncontrib_upper;  /* last dims of kminor_lower / kminor_upper     */
+    /* Band-packed tables of the fused gas-optics entry points, filled in by               */
+    /* rrtmgp_hip_kdist_pack_* from the tables above (NULL until then).  Per band, the       */
+    /* band's g-points (padded with zeros to a multiple of 4) are the fastest index.         */
+    const void* go_bands;                  /* per-band descriptors (library-internal)       */
+    const void* kmajor_p;                  /* per band (gpad, npres+1, neta, ntemp)         */
+    const void* planck_frac_p;             /* as kmajor_p              LW only              */
+    const void* kminor_p;                  /* per band (gpad, slot, neta, ntemp, half)      */
+    const void* krayl_p;                   /* per band (gpad, neta, ntemp, half)  SW only   */
     int64_t max_band_gpts;                 /* largest number of g-points in one band        */
-    int64_t max_band_minors;               /* most minor intervals touching one band        */
-    int64_t minor_flavor_uniform;          /* 1 if every minor interval touching a band has */
-                                           /* the band's flavor (intervals within bands)    */
 } rrtmgp_kdist_dev;
+
+#define RRTMGP_DECLARE_BOTH(DECL) DECL(f32, float) DECL(f64, double)
+
+/* Builds the band-packed tables of a k-distribution whose reference-layout tables (above) */
+/* are filled in, and stores their device pointers in *kd.  No counterpart in the           */
+/* reference: its kernels gather from the reference layout                                  */
+/* (src_cuda/Gas_optics_rrtmgp.cu:779-802 uploads only that).  Requires every g-point of a  */
+/* band to share the band's flavor in each half of the atmosphere, at most 8 minor          */
+/* intervals per band and half, and each interval to have its band's flavor.  *pack owns the */
+/* device memory until rrtmgp_hip_kdist_pack_free.  Synchronous (one-off set-up).            */
+#define RRTMGP_DECL_KDIST_PACK(S, T)                                                           \
+    int rrtmgp_hip_kdist_pack_##S(rrtmgp_kdist_dev* kd, void** pack);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_KDIST_PACK)
+int rrtmgp_hip_kdist_pack_free(void* pack);
 
 /* Volume mixing ratios in the k-distribution's gas order; dim1 is 1 or ncol, dim2 is 1 or   */
 /* nlay (the broadcast rules of Gas_optics_rrtmgp::compute_gas_taus,                          */
@@ -145,7 +153,6 @@ typedef struct
     int64_t dim2[RRTMGP_MAX_GAS];
 } rrtmgp_gas_vmr;
 
-#define RRTMGP_DECLARE_BOTH(DECL) DECL(f32, float) DECL(f64, double)
 
 /* ------------------------------------------------------------------------------------------ */
 /* Gas optics                                                                                  */

@@ -5,16 +5,18 @@
 // interpolate1D :1-12) with the CPU combine rule of src/Gas_optics_rrtmgp.cpp:365-385.
 //
 // Design.  In the fused kernels a work-group (4 wavefronts) computes one layer of 64 columns
-// for one chunk of at most 16 g-points of a band.  Lane l of a wavefront takes g-point l % 16 of
-// pair l / 16 of its current four (column, layer) pairs.  The k-tables are gathered from copies
-// with the g-point index fastest, so the 16 lanes of a pair read one contiguous 64-byte segment
-// per table corner (4 segments per wavefront gather).  A pair's interpolation state (indices and
-// weights of the band's flavor) is computed by its 16 lanes together, one wavefront instruction
-// per four pairs, and never written to memory.  Results pass through LDS (rows padded to 68
-// floats: conflict-free for this lane layout) and leave as 256-byte column rows of the reference
-// layout (ncol, nlay, ngpt).  Work-groups of one chunk are consecutive and kept on one XCD, so
-// that XCD's L2 holds the chunk's table slice.
+// for one chunk of at most 16 g-points of a band, in three phases:
+//   1. wavefront 0, lane = column: the pair's interpolation state for the band's flavor
+//      (indices, weights, minor scalings, Planck values) -> LDS;
+//   2. lane = (pair, quad of 4 g-points): every lane reads one table corner of its four
+//      g-points with one 16-byte load from band-packed tables (g-points of a band fastest), so
+//      the 4 lanes of a pair read one contiguous 64-byte segment; results -> LDS;
+//   3. lane = column, wavefront = g-point row: finishing arithmetic and 256-byte coalesced
+//      stores in the reference layout (ncol, nlay, ngpt).
+// LDS rows are padded to 68 floats (conflict-free for both lane layouts).  Work-groups of one
+// chunk are consecutive and kept on one XCD, so that XCD's L2 holds the chunk's table slice.
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "common.h"
@@ -25,6 +27,29 @@ constexpr int GO_THREADS = 256;   // work-group: 4 wavefronts
 constexpr int GO_COLS = 64;       // columns of a work-group's layer
 constexpr int GO_GPTS = 16;       // g-points per chunk: the lanes of one pair
 constexpr int GO_ROW = 68;        // LDS row length (64 columns + padding)
+constexpr int GO_MAXM = 8;        // minor intervals per band and half of the atmosphere
+
+// One minor interval of a band (one half of the atmosphere), as gas indices of the col_gas
+// numbering (0 = dry air).
+struct GoMinor
+{
+    int gas;          // idx_minor
+    int dens;         // minor_scales_with_density
+    int scal_gas;     // idx_minor_scaling (<= 0: none)
+    int compl_;       // scale_by_complement
+};
+
+// Descriptor of one band for the fused kernels; half 0 is the lower atmosphere (tropo), 1 the
+// upper.  Offsets are in elements of the band-packed tables.
+struct GoBand
+{
+    int g0, ng, gp;          // first g-point (0-based), g-points, g-points padded to a multiple of 4
+    int koff, roff, moff;    // offsets in kmajor_p / planck_frac_p, krayl_p, kminor_p
+    int nslot;               // minor slots per half: max(nq[0], nq[1])
+    int nq[2];               // minor intervals of the band per half
+    int gas[2][2];           // the band's flavor gases per half
+    GoMinor minor[2][GO_MAXM];
+};
 
 template<typename Real>
 struct Kd
@@ -46,17 +71,15 @@ struct Kd
     const int* idx_minor[2];
     const int* idx_scal[2];
     const int* kstart[2];
-    const int* band_start[2];
-    const int* band_list[2];
     const Real* totplnk;
     const Real* planck_frac;
     const Real* krayl;
-    // g-point-fastest copies and their strides
-    const Real* kmajor_g;          // (ngpt, npres+1, neta, ntemp)
-    const Real* planck_frac_g;
-    const Real* kminor_g[2];       // (ncontrib, neta, ntemp)
-    const Real* krayl_g;           // (ngpt, neta, ntemp, 2)
-    int64_t ncontrib[2];
+    // Band-packed tables of the fused kernels (kdist_pack)
+    const struct GoBand* bands;
+    const Real* kmajor_p;
+    const Real* planck_frac_p;
+    const Real* kminor_p;
+    const Real* krayl_p;
     int max_band_gpts;
 };
 
@@ -86,17 +109,14 @@ Kd<Real> make_kd(const rrtmgp_kdist_dev& d)
     k.idx_minor[0] = d.idx_minor_lower; k.idx_minor[1] = d.idx_minor_upper;
     k.idx_scal[0] = d.idx_minor_scaling_lower; k.idx_scal[1] = d.idx_minor_scaling_upper;
     k.kstart[0] = d.kminor_start_lower; k.kstart[1] = d.kminor_start_upper;
-    k.band_start[0] = d.minor_band_start_lower; k.band_start[1] = d.minor_band_start_upper;
-    k.band_list[0] = d.minor_band_list_lower; k.band_list[1] = d.minor_band_list_upper;
     k.totplnk = static_cast<const Real*>(d.totplnk);
     k.planck_frac = static_cast<const Real*>(d.planck_frac);
     k.krayl = static_cast<const Real*>(d.krayl);
-    k.kmajor_g = static_cast<const Real*>(d.kmajor_g);
-    k.planck_frac_g = static_cast<const Real*>(d.planck_frac_g);
-    k.kminor_g[0] = static_cast<const Real*>(d.kminor_lower_g);
-    k.kminor_g[1] = static_cast<const Real*>(d.kminor_upper_g);
-    k.krayl_g = static_cast<const Real*>(d.krayl_g);
-    k.ncontrib[0] = d.ncontrib_lower; k.ncontrib[1] = d.ncontrib_upper;
+    k.bands = static_cast<const GoBand*>(d.go_bands);
+    k.kmajor_p = static_cast<const Real*>(d.kmajor_p);
+    k.planck_frac_p = static_cast<const Real*>(d.planck_frac_p);
+    k.kminor_p = static_cast<const Real*>(d.kminor_p);
+    k.krayl_p = static_cast<const Real*>(d.krayl_p);
     k.max_band_gpts = int(d.max_band_gpts);
     return k;
 }
@@ -234,23 +254,23 @@ __device__ __forceinline__ Real interpolate1D(Real val, Real offset, Real delta,
     return table[idx - 1] + frac * (table[idx] - table[idx - 1]);
 }
 
-// Scaling of minor interval m of set s at one (col, lay) (.cu:388-420).
-template<typename Real>
-__device__ __forceinline__ Real minor_scaling(const Kd<Real>& k, const Gases<Real>& g, int s, int m,
-                                              int64_t icol, int64_t ilay, Real p, Real t, Real col_dry)
+// Scaling of one minor interval at one (col, lay) (.cu:388-420); col(gas) gives the column
+// amount of a gas index (0 = dry air).
+template<typename Real, typename ColOf>
+__device__ __forceinline__ Real minor_scaling_of(const ColOf& col, int gas, bool dens, int scal_gas, bool compl_,
+                                                 int idx_h2o, Real p, Real t, Real col_dry)
 {
-    Real scaling = col_of(g, k.idx_minor[s][m], icol, ilay, col_dry);
-    if (k.dens[s][m])
+    Real scaling = col(gas);
+    if (dens)
     {
         const Real PaTohPa = Real(0.01);
         scaling *= PaTohPa * p / t;
-        const int is = k.idx_scal[s][m];
-        if (is > 0)
+        if (scal_gas > 0)
         {
             const Real vmr_fact = Real(1) / col_dry;
-            const Real dry_fact = Real(1) / (Real(1) + col_of(g, k.idx_h2o, icol, ilay, col_dry) * vmr_fact);
-            const Real cs = col_of(g, is, icol, ilay, col_dry);
-            if (k.compl_[s][m])
+            const Real dry_fact = Real(1) / (Real(1) + col(idx_h2o) * vmr_fact);
+            const Real cs = col(scal_gas);
+            if (compl_)
                 scaling *= (Real(1) - cs * vmr_fact * dry_fact);
             else
                 scaling *= cs * vmr_fact * dry_fact;
@@ -260,38 +280,18 @@ __device__ __forceinline__ Real minor_scaling(const Kd<Real>& k, const Gases<Rea
 }
 
 // ---------------------------------------------------------------------------------------------
-// Gathers from the g-fastest tables for g-point gg (same arithmetic order as major_tau,
-// planck_fraction and the minor / Rayleigh sums of the reference-layout kernels).
-// The g-fastest tables hold ntemp * neta * (npres+1) * ngpt < 2^31 entries (checked at launch),
-// so table offsets are 32-bit.
-template<typename Real>
-__device__ __forceinline__ Real major_tau_g(const Kd<Real>& k, const CL<Real>& c, const FL<Real>& f, int gg)
-{
-    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt, sT = sE * k.neta;
-    const int jp = c.jpress + c.itropo - 1;
-    const Real* km = k.kmajor_g;
-    Real tau = Real(0);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-    {
-        const int o = (c.jtemp - 1 + i) * sT + (f.jeta[i] - 1) * sE + jp * sP + gg;
-        tau += f.col_mix[i] * (f.fmajor[i][0] * km[o] + f.fmajor[i][1] * km[o + sE] +
-                               f.fmajor[i][2] * km[o + sP] + f.fmajor[i][3] * km[o + sP + sE]);
-    }
-    return tau;
-}
-
-template<typename Real>
-__device__ __forceinline__ Real planck_fraction_g(const Kd<Real>& k, const CL<Real>& c, const FL<Real>& f, int gg)
-{
-    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt, sT = sE * k.neta;
-    const int jp = c.jpress + c.itropo - 1;
-    const Real* pf = k.planck_frac_g;
-    const int o0 = (c.jtemp - 1) * sT + (f.jeta[0] - 1) * sE + jp * sP + gg;
-    const int o1 = (c.jtemp) * sT + (f.jeta[1] - 1) * sE + jp * sP + gg;
-    return (f.fmajor[0][0] * pf[o0] + f.fmajor[0][1] * pf[o0 + sE] + f.fmajor[0][2] * pf[o0 + sP] + f.fmajor[0][3] * pf[o0 + sP + sE])
-         + (f.fmajor[1][0] * pf[o1] + f.fmajor[1][1] * pf[o1 + sE] + f.fmajor[1][2] * pf[o1 + sP] + f.fmajor[1][3] * pf[o1 + sP + sE]);
-}
+// Fused kernels.
+//
+// They read the band-packed tables of kdist_pack.  Per band the band's g-points (padded with
+// zeros to gp, a multiple of 4) are the fastest index, so one 16-byte read yields one table
+// corner for four consecutive g-points:
+//   kmajor_p, planck_frac_p  (gp, npres+1, neta, ntemp)        at GoBand::koff
+//   krayl_p                  (gp, neta, ntemp, half)            at GoBand::roff
+//   kminor_p                 (gp, nslot, neta, ntemp, half)     at GoBand::moff
+// Slot q of half s of kminor_p holds the band's q-th minor interval of that half (ascending
+// interval order) and is zero at the g-points outside the interval: the minor sum runs over the
+// band's slots with no per-g-point interval test, and a g-point outside an interval adds an
+// exact zero, so the sum equals the reference's sum over the intervals containing the g-point.
 
 // Grid of the fused kernels: units (chunk, layer, 64-column tile), chunk = (band, 16-g-point
 // part of the band), ordered so that consecutive units share a chunk.
@@ -312,36 +312,58 @@ inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t nbnd, int64_t max
 
 struct GoUnit
 {
-    int ibnd, gs, ge;
+    int ibnd, c0, ncg;   // band, first g-point of the chunk within the band, g-points in the chunk
     int64_t ilay, col0;
 };
 
-template<typename Real>
-__device__ __forceinline__ GoUnit go_unit(const Kd<Real>& k, const GoGrid& t, int64_t u)
+// The launchers keep nunits below 2^31, so the decomposition runs in 32-bit arithmetic.
+__device__ __forceinline__ GoUnit go_unit(const GoBand* bands, const GoGrid& t, int64_t u64)
 {
     GoUnit r;
-    const int64_t rest = u / t.ncoltiles;
-    r.col0 = (u % t.ncoltiles) * GO_COLS;
-    r.ilay = rest % t.nlay;
-    const int64_t chunk = rest / t.nlay;
-    r.ibnd = int(chunk / t.cpb);
-    const int gs_b = k.band_lims_gpt[2 * r.ibnd] - 1, ge_b = k.band_lims_gpt[2 * r.ibnd + 1];
-    r.gs = gs_b + GO_GPTS * int(chunk % t.cpb);
-    r.ge = min(ge_b, r.gs + GO_GPTS);
+    const uint32_t u = uint32_t(u64), nct = uint32_t(t.ncoltiles), nl = uint32_t(t.nlay);
+    const uint32_t cpb = uint32_t(t.cpb);
+    const uint32_t rest = u / nct;
+    r.col0 = int64_t(u - rest * nct) * GO_COLS;
+    const uint32_t chunk = rest / nl;
+    r.ilay = rest - chunk * nl;
+    r.ibnd = int(chunk / cpb);
+    r.c0 = GO_GPTS * int(chunk - uint32_t(r.ibnd) * cpb);
+    r.ncg = min(bands[r.ibnd].ng - r.c0, GO_GPTS);
     return r;
 }
 
-constexpr int GO_MAXM = 8;   // minor intervals of one band per pair held in LDS
+// Volume-mixing-ratio pointers and strides, staged in LDS so that a per-lane gas index reads
+// them with an ordinary LDS load.
+template<typename Real>
+struct GasRef
+{
+    const Real* p;
+    int64_t scol, slay;
+};
+
+template<typename Real>
+__device__ __forceinline__ void stage_gases(const Gases<Real>& g, int ngas, GasRef<Real>* sg)
+{
+    for (int i = 0; i < ngas; ++i)
+        sg[i] = GasRef<Real>{g.p[i], g.scol[i], g.slay[i]};
+}
+
+template<typename Real>
+__device__ __forceinline__ Real col_lds(const GasRef<Real>* sg, int gas, int64_t icol, int64_t ilay, Real col_dry)
+{
+    if (gas == 0) return col_dry;
+    const GasRef<Real>& r = sg[gas - 1];
+    return r.p[icol * r.scol + ilay * r.slay] * col_dry;
+}
 
 // State of one (column, layer) pair for one band, computed once per pair (phase 1 of the fused
-// kernels) and read by the 16 g-point lanes of the pair from LDS (phase 2).
+// kernels) and read from LDS by the four lanes that share the pair (phase 2).
 template<typename Real>
 struct PairState
 {
-    int om[2];           // kmajor_g / planck_frac_g offsets for itemp 0/1, without the g-point
-    int mn[2];           // kminor_g offsets for itemp 0/1 (table of half itropo), without contributor
-    int ry[2];           // krayl_g offsets for itemp 0/1, without the g-point
-    int itropo;
+    int om[2];           // kmajor_p / planck_frac_p offsets in the band for itemp 0/1, no g-point
+    int mn[2];           // kminor_p offsets in the band (half included) for itemp 0/1
+    int ry[2];           // krayl_p offsets in the band (half included) for itemp 0/1
     int flags;           // bit 0: valid column
     Real col_mix[2];
     Real fmajor[2][4];
@@ -350,9 +372,9 @@ struct PairState
     Real aux[5];         // LW: B(T_lay), B(T_lev above), B(T_lev below), B(T_sfc), B(T_sfc + 1); SW: col_ray
 };
 
-// Phase 1 for one pair.
+// Phase 1 for one pair (lane = column).
 template<typename Real, bool LW>
-__device__ __forceinline__ void pair_state(const Kd<Real>& k, const Gases<Real>& g, int ibnd, int band_g0,
+__device__ __forceinline__ void pair_state(const Kd<Real>& k, const GasRef<Real>* sg, const GoBand& bd, int ibnd,
                                            int64_t icol, int64_t ilay, int64_t ncol, const Real* play,
                                            const Real* tlay, const Real* tlev, const Real* tsfc,
                                            const Real* col_dry_a, bool at_sfc, PairState<Real>& st)
@@ -360,34 +382,39 @@ __device__ __forceinline__ void pair_state(const Kd<Real>& k, const Gases<Real>&
     const int64_t idx = icol + ncol * ilay;
     const Real pv = play[idx], tv = tlay[idx], cd = col_dry_a[idx];
     const CL<Real> c = interp_cl(k, pv, tv);
-    const int bflav = k.gpoint_flavor[2 * band_g0 + c.itropo] - 1;
-    const int gas1 = k.flavor[2 * bflav], gas2 = k.flavor[2 * bflav + 1];
-    FL<Real> f;
-    interp_flav(k, c, gas1, gas2, col_of(g, gas1, icol, ilay, cd), col_of(g, gas2, icol, ilay, cd), f);
-    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt, sT = sE * k.neta;
-    const int jp = c.jpress + c.itropo - 1;
     const int s = c.itropo;
-    const int mE = int(k.ncontrib[s]), mT = mE * k.neta;
-    const int rE = k.ngpt, rT = k.neta * k.ngpt, rS = rT * k.ntemp;
-    int flags = 1;
+    const int gas1 = bd.gas[s][0], gas2 = bd.gas[s][1];
+    FL<Real> f;
+    interp_flav(k, c, gas1, gas2, col_lds(sg, gas1, icol, ilay, cd), col_lds(sg, gas2, icol, ilay, cd), f);
+    const int gp = bd.gp, np1 = k.npres + 1, nte = k.ntemp * k.neta;
+    const int jp = c.jpress + c.itropo - 1;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
     {
-        st.om[i] = (c.jtemp - 1 + i) * sT + (f.jeta[i] - 1) * sE + jp * sP;
-        st.mn[i] = (c.jtemp - 1 + i) * mT + (f.jeta[i] - 1) * mE;
-        st.ry[i] = s * rS + (c.jtemp - 1 + i) * rT + (f.jeta[i] - 1) * rE;
+        const int te = (c.jtemp - 1 + i) * k.neta + (f.jeta[i] - 1);     // (temperature, eta) node
+        st.om[i] = (te * np1 + jp) * gp;
+        st.mn[i] = (s * nte + te) * bd.nslot * gp;
+        st.ry[i] = (s * nte + te) * gp;
         st.col_mix[i] = f.col_mix[i];
 #pragma unroll
         for (int q = 0; q < 4; ++q) st.fmajor[i][q] = f.fmajor[i][q];
         st.fminor[i][0] = f.fminor[i][0];
         st.fminor[i][1] = f.fminor[i][1];
     }
-    st.itropo = s;
-    // The launch checks that a band has at most GO_MAXM minor intervals, all of its flavor.
-    const int q0 = k.band_start[s][ibnd], q1 = k.band_start[s][ibnd + 1];
-    for (int q = q0; q < q1; ++q)
-        st.scaling[q - q0] = minor_scaling(k, g, s, k.band_list[s][q], icol, ilay, pv, tv, cd);
-    st.flags = flags;
+    const int nq = bd.nq[s];
+    for (int q = 0; q < bd.nslot; ++q)
+    {
+        Real sc = Real(0);
+        if (q < nq)
+        {
+            const GoMinor md = bd.minor[s][q];
+            sc = minor_scaling_of(
+                    [&](int gas) { return col_lds(sg, gas, icol, ilay, cd); },
+                    md.gas, md.dens != 0, md.scal_gas, md.compl_ != 0, k.idx_h2o, pv, tv, cd);
+        }
+        st.scaling[q] = sc;
+    }
+    st.flags = 1;
     if constexpr (LW)
     {
         const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
@@ -401,52 +428,128 @@ __device__ __forceinline__ void pair_state(const Kd<Real>& k, const Gases<Real>&
         }
     }
     else
-        st.aux[0] = col_of(g, k.idx_h2o, icol, ilay, cd) + cd;
+        st.aux[0] = col_lds(sg, k.idx_h2o, icol, ilay, cd) + cd;
 }
 
-// Phase 2: major + minor absorption of g-point gg from a pair's state (same arithmetic order as
-// major_tau and the minor sum of tau_absorption_kernel).
+// Entries i..i+3 of a band-packed table (quads are 4-element aligned).  The table base is
+// uniform and the byte offset fits 32 bits (kdist_pack checks it), so the read is addressed as
+// scalar base + 32-bit lane offset.
 template<typename Real>
-__device__ __forceinline__ Real absorption_from_state(const Kd<Real>& k, const PairState<Real>& st, int ibnd, int gg)
+__device__ __forceinline__ void load4(const Real* __restrict__ base, uint32_t i, Real (&v)[4])
 {
-    const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt;
-    const Real* km = k.kmajor_g;
-    Real tau = Real(0);
+    const char* p = reinterpret_cast<const char*>(base) + (i * uint32_t(sizeof(Real)));
+    if constexpr (sizeof(Real) == 4)
+    {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    }
+    else
+    {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 b = *reinterpret_cast<const double2*>(p + 16);
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    }
+}
+
+// Phase 2: major + minor absorption of g-points gl..gl+3 of the band (same arithmetic order
+// per g-point as major_tau and the minor sum of tau_absorption_kernel).
+template<typename Real>
+__device__ __forceinline__ void absorption4(const Kd<Real>& k, const GoBand& bd, const PairState<Real>& st,
+                                            uint32_t gl, Real (&tau)[4])
+{
+    const uint32_t gp = bd.gp;
+    // Minor sum first: its loop then holds no major-gather data live.
+    Real minor[4] = {Real(0), Real(0), Real(0), Real(0)};
+    const int nslot = bd.nslot;
+    const uint32_t sQ = uint32_t(nslot) * gp;          // eta stride
+    const uint32_t mb = uint32_t(bd.moff) + gl;
+    for (int q = 0; q < nslot; ++q)
+    {
+        Real a[4], b[4], c[4], d[4];
+        const uint32_t o0 = mb + uint32_t(st.mn[0]) + uint32_t(q) * gp;
+        const uint32_t o1 = mb + uint32_t(st.mn[1]) + uint32_t(q) * gp;
+        load4(k.kminor_p, o0, a); load4(k.kminor_p, o0 + sQ, b);
+        load4(k.kminor_p, o1, c); load4(k.kminor_p, o1 + sQ, d);
+        const Real sc = st.scaling[q];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            const Real lt = st.fminor[0][0] * a[e] + st.fminor[0][1] * b[e] +
+                            st.fminor[1][0] * c[e] + st.fminor[1][1] * d[e];
+            minor[e] += lt * sc;
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const uint32_t sE = uint32_t(k.npres + 1) * gp, sP = gp;
+    const uint32_t kb = uint32_t(bd.koff) + gl;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) tau[e] = Real(0);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
     {
-        const int o = st.om[i] + gg;
-        tau += st.col_mix[i] * (st.fmajor[i][0] * km[o] + st.fmajor[i][1] * km[o + sE] +
-                                st.fmajor[i][2] * km[o + sP] + st.fmajor[i][3] * km[o + sP + sE]);
+        Real a[4], b[4], c[4], d[4];
+        const uint32_t o = kb + uint32_t(st.om[i]);
+        load4(k.kmajor_p, o, a); load4(k.kmajor_p, o + sE, b);
+        load4(k.kmajor_p, o + sP, c); load4(k.kmajor_p, o + sP + sE, d);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            tau[e] += st.col_mix[i] * (st.fmajor[i][0] * a[e] + st.fmajor[i][1] * b[e] +
+                                       st.fmajor[i][2] * c[e] + st.fmajor[i][3] * d[e]);
     }
-    const int s = st.itropo;
-    const int q0 = k.band_start[s][ibnd], q1 = k.band_start[s][ibnd + 1];
-    const int mE = int(k.ncontrib[s]);
-    const Real* kn = k.kminor_g[s];
-    Real minor = Real(0);
-    for (int q = q0; q < q1; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) tau[e] += minor[e];
+}
+
+// Planck fraction of g-points gl..gl+3 (.cu:170-180): the corners and weights of the major gather.
+template<typename Real>
+__device__ __forceinline__ void planck_fraction4(const Kd<Real>& k, const GoBand& bd, const PairState<Real>& st,
+                                                 uint32_t gl, Real (&pf)[4])
+{
+    const uint32_t gp = bd.gp, sE = uint32_t(k.npres + 1) * gp, sP = gp;
+    const uint32_t kb = uint32_t(bd.koff) + gl;
+    Real v[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
     {
-        const int m = k.band_list[s][q];
-        const int gs = k.minor_limits[s][2 * m] - 1, ge = k.minor_limits[s][2 * m + 1];
-        if (gg >= gs && gg < ge)
-        {
-            const int ic = k.kstart[s][m] - 1 + (gg - gs);
-            const int o0 = st.mn[0] + ic, o1 = st.mn[1] + ic;
-            const Real lt = st.fminor[0][0] * kn[o0] + st.fminor[0][1] * kn[o0 + mE] +
-                            st.fminor[1][0] * kn[o1] + st.fminor[1][1] * kn[o1 + mE];
-            minor += lt * st.scaling[q - q0];
-        }
+        Real a[4], b[4], c[4], d[4];
+        const uint32_t o = kb + uint32_t(st.om[i]);
+        load4(k.planck_frac_p, o, a); load4(k.planck_frac_p, o + sE, b);
+        load4(k.planck_frac_p, o + sP, c); load4(k.planck_frac_p, o + sP + sE, d);
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            v[i][e] = st.fmajor[i][0] * a[e] + st.fmajor[i][1] * b[e] + st.fmajor[i][2] * c[e] +
+                      st.fmajor[i][3] * d[e];
     }
-    return tau + minor;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) pf[e] = v[0][e] + v[1][e];
+}
+
+// Rayleigh optical depth of g-points gl..gl+3 (.cu:572-616).
+template<typename Real>
+__device__ __forceinline__ void rayleigh4(const Kd<Real>& k, const GoBand& bd, const PairState<Real>& st,
+                                          uint32_t gl, Real (&tr)[4])
+{
+    const uint32_t gp = bd.gp;
+    const uint32_t rb = uint32_t(bd.roff) + gl;
+    Real a[4], b[4], c[4], d[4];
+    load4(k.krayl_p, rb + uint32_t(st.ry[0]), a); load4(k.krayl_p, rb + uint32_t(st.ry[0]) + gp, b);
+    load4(k.krayl_p, rb + uint32_t(st.ry[1]), c); load4(k.krayl_p, rb + uint32_t(st.ry[1]) + gp, d);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+    {
+        const Real kloc = st.fminor[0][0] * a[e] + st.fminor[0][1] * b[e] +
+                          st.fminor[1][0] * c[e] + st.fminor[1][1] * d[e];
+        tr[e] = kloc * st.aux[0];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
 // Fused longwave gas optics: tau (+ cloud absorption per band) and Planck sources.
-//   phase 1 (64 threads, lane = column): pair states -> LDS;
-//   phase 2 (all lanes, lane = (pair, g-point)): absorption and Planck fraction -> LDS;
+//   phase 1 (wavefront 0, lane = column): pair states -> LDS;
+//   phase 2 (lane = (pair, quad of g-points)): absorption and Planck fraction -> LDS;
 //   phase 3 (lane = column, wavefront = g-point row): sources, cloud add_to, coalesced stores.
 template<typename Real>
-__global__ __launch_bounds__(GO_THREADS)
+__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, GoGrid grid,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ tlev, const Real* __restrict__ tsfc,
@@ -456,49 +559,53 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                           Real* __restrict__ lev_dec, Real* __restrict__ sfc_src,
                           Real* __restrict__ sfc_jac)
 {
+    static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
     __shared__ Real s_tau[GO_GPTS][GO_ROW];
     __shared__ Real s_pf[GO_GPTS][GO_ROW];
+    __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
     const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
     if (u >= grid.nunits) return;                       // uniform over the work-group
-    const GoUnit un = go_unit(k, grid, u);
-    const int ng = un.ge - un.gs;
-    if (ng <= 0) return;                                // uniform: an empty chunk of a short band
+    const GoUnit un = go_unit(k.bands, grid, u);
+    if (un.ncg <= 0) return;                            // uniform: an empty chunk of a short band
+    const GoBand& bd = k.bands[un.ibnd];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int64_t ilay = un.ilay, ncl = ncol * nlay;
     const bool at_sfc = (ilay == sfc_lay - 1);
 
+    if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
+    __syncthreads();
     if (w == 0)
     {
         const int64_t icol = un.col0 + lane;
         if (icol < ncol)
-            pair_state<Real, true>(k, g, un.ibnd, un.gs, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a,
+            pair_state<Real, true>(k, sg, bd, un.ibnd, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a,
                                    at_sfc, sst[lane]);
         else
             sst[lane].flags = 0;
     }
     __syncthreads();
 
-    const int p = lane / GO_GPTS, gl = lane % GO_GPTS;
-    const int gg = un.gs + gl;
-    for (int it = 0; it < GO_COLS / 16; ++it)
     {
-        const int pi = it * 16 + w * 4 + p;
+        const int pi = w * 16 + lane / 4;               // pair
+        const int gq = 4 * (lane % 4);                  // first g-point of the quad in the chunk
         const PairState<Real>& st = sst[pi];
-        Real t = Real(0), pf = Real(0);
-        if ((st.flags & 1) && gl < ng)
+        Real t4[4] = {Real(0), Real(0), Real(0), Real(0)};
+        Real p4[4] = {Real(0), Real(0), Real(0), Real(0)};
+        if ((st.flags & 1) && gq < un.ncg)
         {
-            t = absorption_from_state(k, st, un.ibnd, gg);
-            const int sP = k.ngpt, sE = (k.npres + 1) * k.ngpt;
-            const Real* pfr = k.planck_frac_g;
-            const int o0 = st.om[0] + gg, o1 = st.om[1] + gg;
-            pf = (st.fmajor[0][0] * pfr[o0] + st.fmajor[0][1] * pfr[o0 + sE] + st.fmajor[0][2] * pfr[o0 + sP] +
-                  st.fmajor[0][3] * pfr[o0 + sP + sE])
-               + (st.fmajor[1][0] * pfr[o1] + st.fmajor[1][1] * pfr[o1 + sE] + st.fmajor[1][2] * pfr[o1 + sP] +
-                  st.fmajor[1][3] * pfr[o1 + sP + sE]);
+            const uint32_t gl = uint32_t(un.c0 + gq);
+            absorption4(k, bd, st, gl, t4);
+            // Bounds the table reads in flight (and so the registers) to one gather group.
+            __builtin_amdgcn_sched_barrier(0);
+            planck_fraction4(k, bd, st, gl, p4);
         }
-        s_tau[gl][pi] = t;
-        s_pf[gl][pi] = pf;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            s_tau[gq + e][pi] = t4[e];
+            s_pf[gq + e][pi] = p4[e];
+        }
     }
     __syncthreads();
 
@@ -507,9 +614,10 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const PairState<Real>& st = sst[lane];
     const int64_t idx = icol + ncol * ilay;
     const Real ctau = cloud_tau ? cloud_tau[idx + ncl * un.ibnd] : Real(0);
-    for (int row = w; row < ng; row += GO_THREADS / 64)
+    const int gbase = bd.g0 + un.c0;
+    for (int row = w; row < un.ncg; row += GO_THREADS / 64)
     {
-        const int gr = un.gs + row;
+        const int gr = gbase + row;
         const int64_t o = idx + ncl * gr;
         const Real pf = s_pf[row][lane];
         tau[o] = cloud_tau ? s_tau[row][lane] + ctau : s_tau[row][lane];
@@ -527,55 +635,58 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
 // Fused shortwave gas optics: absorption + Rayleigh + combine (+ band cloud add_to), in the
 // same three phases.
 template<typename Real>
-__global__ __launch_bounds__(GO_THREADS)
+__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
 void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, GoGrid grid,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ col_dry_a, const Real* __restrict__ ctau_a,
                           const Real* __restrict__ cssa_a, const Real* __restrict__ cg_a,
                           Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ gout)
 {
+    static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
     __shared__ Real s_abs[GO_GPTS][GO_ROW];
     __shared__ Real s_ray[GO_GPTS][GO_ROW];
+    __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
     const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
     if (u >= grid.nunits) return;
-    const GoUnit un = go_unit(k, grid, u);
-    const int ng = un.ge - un.gs;
-    if (ng <= 0) return;
+    const GoUnit un = go_unit(k.bands, grid, u);
+    if (un.ncg <= 0) return;
+    const GoBand& bd = k.bands[un.ibnd];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int64_t ilay = un.ilay, ncl = ncol * nlay;
 
+    if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
+    __syncthreads();
     if (w == 0)
     {
         const int64_t icol = un.col0 + lane;
         if (icol < ncol)
-            pair_state<Real, false>(k, g, un.ibnd, un.gs, icol, ilay, ncol, play, tlay, nullptr, nullptr,
+            pair_state<Real, false>(k, sg, bd, un.ibnd, icol, ilay, ncol, play, tlay, nullptr, nullptr,
                                     col_dry_a, false, sst[lane]);
         else
             sst[lane].flags = 0;
     }
     __syncthreads();
 
-    const int p = lane / GO_GPTS, gl = lane % GO_GPTS;
-    const int gg = un.gs + gl;
-    const int rE = k.ngpt;
-    for (int it = 0; it < GO_COLS / 16; ++it)
     {
-        const int pi = it * 16 + w * 4 + p;
+        const int pi = w * 16 + lane / 4;
+        const int gq = 4 * (lane % 4);
         const PairState<Real>& st = sst[pi];
-        Real tg = Real(0), tray = Real(0);
-        if ((st.flags & 1) && gl < ng)
+        Real t4[4] = {Real(0), Real(0), Real(0), Real(0)};
+        Real r4[4] = {Real(0), Real(0), Real(0), Real(0)};
+        if ((st.flags & 1) && gq < un.ncg)
         {
-            tg = absorption_from_state(k, st, un.ibnd, gg);
-            // Rayleigh (.cu:572-616).
-            const Real* kr = k.krayl_g;
-            const int o0 = st.ry[0] + gg, o1 = st.ry[1] + gg;
-            const Real kloc = st.fminor[0][0] * kr[o0] + st.fminor[0][1] * kr[o0 + rE] +
-                              st.fminor[1][0] * kr[o1] + st.fminor[1][1] * kr[o1 + rE];
-            tray = kloc * st.aux[0];
+            const uint32_t gl = uint32_t(un.c0 + gq);
+            absorption4(k, bd, st, gl, t4);
+            __builtin_amdgcn_sched_barrier(0);
+            rayleigh4(k, bd, st, gl, r4);
         }
-        s_abs[gl][pi] = tg;
-        s_ray[gl][pi] = tray;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            s_abs[gq + e][pi] = t4[e];
+            s_ray[gq + e][pi] = r4[e];
+        }
     }
     __syncthreads();
 
@@ -590,7 +701,8 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         const int64_t ib = idx + ncl * un.ibnd;
         t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
     }
-    for (int row = w; row < ng; row += GO_THREADS / 64)
+    const int gbase = bd.g0 + un.c0;
+    for (int row = w; row < un.ncg; row += GO_THREADS / 64)
     {
         const Real tray = s_ray[row][lane];
         const Real tt = s_abs[row][lane] + tray;
@@ -607,7 +719,7 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
             ss = tauscat12 / max(op_eps, tau12);
             tt_out = tau12;
         }
-        const int64_t o = idx + ncl * (un.gs + row);
+        const int64_t o = idx + ncl * (gbase + row);
         tau[o] = tt_out;
         ssa[o] = ss;
         gout[o] = gg_out;
@@ -845,25 +957,204 @@ int check_kd(const rrtmgp_kdist_dev* kd, bool need_planck, bool need_rayleigh)
         return arg_error("k-distribution has no Planck tables (longwave entry on a shortwave kdist)");
     if (need_rayleigh && !kd->krayl)
         return arg_error("k-distribution has no Rayleigh tables (shortwave entry on a longwave kdist)");
-    if (kd->idx_h2o < 1) return arg_error("idx_h2o cannot be found");
+    if (kd->idx_h2o < 1 || kd->idx_h2o > kd->ngas) return arg_error("idx_h2o cannot be found");
     return RRTMGP_OK;
 }
 
-// The fused kernels also read the g-fastest table copies.
+// The fused kernels read the band-packed tables of kdist_pack.
 int check_kd_fused(const rrtmgp_kdist_dev* kd, bool lw)
 {
     RRTMGP_TRY(check_kd(kd, lw, !lw));
-    if (!kd->kmajor_g || (lw && !kd->planck_frac_g) || (!lw && !kd->krayl_g))
-        return arg_error("k-distribution has no g-point-fastest tables");
-    if ((kd->nminor_lower > 0 && !kd->kminor_lower_g) || (kd->nminor_upper > 0 && !kd->kminor_upper_g))
-        return arg_error("k-distribution has no g-point-fastest minor tables");
+    if (!kd->go_bands || !kd->kmajor_p || !kd->kminor_p || (lw && !kd->planck_frac_p) || (!lw && !kd->krayl_p))
+        return arg_error("k-distribution has no band-packed tables (call rrtmgp_hip_kdist_pack first)");
     if (kd->max_band_gpts < 1 || kd->max_band_gpts > kd->ngpt) return arg_error("max_band_gpts out of range");
-    if (kd->max_band_minors > GO_MAXM || !kd->minor_flavor_uniform)
-        return arg_error("fused gas optics: a band has more than " + std::to_string(GO_MAXM) +
-                         " minor intervals or an interval crossing into another band's flavor");
-    const int64_t nmaj = kd->ntemp * kd->neta * (kd->npres + 1) * kd->ngpt;
-    const int64_t nmin = kd->ntemp * kd->neta * std::max(kd->ncontrib_lower, kd->ncontrib_upper);
-    if (nmaj >= (int64_t(1) << 31) || nmin >= (int64_t(1) << 31)) return arg_error("k-tables too large for 32-bit offsets");
+    return RRTMGP_OK;
+}
+
+int check_go_grid(const GoGrid& g)
+{
+    if (g.nunits >= (int64_t(1) << 31))
+        return arg_error("fused gas optics: too many (column tile, layer, chunk) units for one launch; "
+                         "split the columns into blocks");
+    return RRTMGP_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Band-packed tables (layout described above the fused kernels), built on the host from the
+// reference-layout tables of the k-distribution.
+
+struct GoPack
+{
+    void* mem[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    ~GoPack()
+    {
+        for (void* p : mem)
+            if (p) (void)hipFree(p);
+    }
+};
+
+template<typename T>
+int pack_d2h(std::vector<T>& out, const void* src, int64_t n, const char* what)
+{
+    out.assign(size_t(std::max<int64_t>(n, 0)), T());
+    if (n <= 0) return RRTMGP_OK;
+    if (!src) return arg_error(std::string("kdist_pack: ") + what + " is NULL");
+    RRTMGP_HIP(hipMemcpy(out.data(), src, size_t(n) * sizeof(T), hipMemcpyDeviceToHost));
+    return RRTMGP_OK;
+}
+
+template<typename T>
+int pack_h2d(void** dst, const std::vector<T>& v)
+{
+    RRTMGP_HIP(hipMalloc(dst, std::max<size_t>(v.size(), 1) * sizeof(T)));
+    if (!v.empty()) RRTMGP_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return RRTMGP_OK;
+}
+
+template<typename Real>
+int kdist_pack(rrtmgp_kdist_dev* kd, void** pack_out)
+{
+    if (!kd || !pack_out) return arg_error("kdist_pack: NULL argument");
+    *pack_out = nullptr;
+    const bool lw = kd->planck_frac != nullptr;
+    RRTMGP_TRY(check_kd(kd, lw, !lw));
+    const int64_t nt = kd->ntemp, ne = kd->neta, np1 = kd->npres + 1, ngpt = kd->ngpt, nb = kd->nbnd;
+    if (nt < 2 || ne < 2 || np1 < 3 || ngpt < 1 || nb < 1) return arg_error("kdist_pack: bad table dimensions");
+    const int64_t nmin[2] = {kd->nminor_lower, kd->nminor_upper};
+    const int64_t ncon[2] = {kd->ncontrib_lower, kd->ncontrib_upper};
+    const void* kmin_src[2] = {kd->kminor_lower, kd->kminor_upper};
+    const void* lim_src[2] = {kd->minor_limits_gpt_lower, kd->minor_limits_gpt_upper};
+    const void* idx_src[2] = {kd->idx_minor_lower, kd->idx_minor_upper};
+    const void* scal_src[2] = {kd->idx_minor_scaling_lower, kd->idx_minor_scaling_upper};
+    const void* kst_src[2] = {kd->kminor_start_lower, kd->kminor_start_upper};
+    const void* dens_src[2] = {kd->minor_scales_with_density_lower, kd->minor_scales_with_density_upper};
+    const void* cmpl_src[2] = {kd->scale_by_complement_lower, kd->scale_by_complement_upper};
+
+    std::vector<Real> kmaj, pfr, kray, kmin[2];
+    std::vector<int> lims, gflav, flav, mlim[2], idxm[2], idxs[2], kst[2];
+    std::vector<Bool> dens[2], cmpl[2];
+    const int64_t nmaj = nt * ne * np1 * ngpt;
+    RRTMGP_TRY(pack_d2h(kmaj, kd->kmajor, nmaj, "kmajor"));
+    if (lw) RRTMGP_TRY(pack_d2h(pfr, kd->planck_frac, nmaj, "planck_frac"));
+    else RRTMGP_TRY(pack_d2h(kray, kd->krayl, nt * ne * ngpt * 2, "krayl"));
+    RRTMGP_TRY(pack_d2h(lims, kd->band_lims_gpt, 2 * nb, "band_lims_gpt"));
+    RRTMGP_TRY(pack_d2h(gflav, kd->gpoint_flavor, 2 * ngpt, "gpoint_flavor"));
+    RRTMGP_TRY(pack_d2h(flav, kd->flavor, 2 * kd->nflav, "flavor"));
+    for (int s = 0; s < 2; ++s)
+    {
+        if (nmin[s] < 0 || ncon[s] < 0) return arg_error("kdist_pack: negative minor dimensions");
+        RRTMGP_TRY(pack_d2h(kmin[s], kmin_src[s], nt * ne * ncon[s], "kminor"));
+        RRTMGP_TRY(pack_d2h(mlim[s], lim_src[s], 2 * nmin[s], "minor_limits_gpt"));
+        RRTMGP_TRY(pack_d2h(idxm[s], idx_src[s], nmin[s], "idx_minor"));
+        RRTMGP_TRY(pack_d2h(idxs[s], scal_src[s], nmin[s], "idx_minor_scaling"));
+        RRTMGP_TRY(pack_d2h(kst[s], kst_src[s], nmin[s], "kminor_start"));
+        RRTMGP_TRY(pack_d2h(dens[s], dens_src[s], nmin[s], "minor_scales_with_density"));
+        RRTMGP_TRY(pack_d2h(cmpl[s], cmpl_src[s], nmin[s], "scale_by_complement"));
+    }
+
+    auto gas_ok = [&](int gas) { return gas >= 0 && gas <= kd->ngas; };
+    std::vector<GoBand> bands(static_cast<size_t>(nb));
+    std::vector<int> minor_of(size_t(nb) * 2 * GO_MAXM, -1);
+    int64_t koff = 0, roff = 0, moff = 0;
+    int max_gpts = 0;
+    for (int64_t b = 0; b < nb; ++b)
+    {
+        GoBand& B = bands[b];
+        B = GoBand{};
+        const int g0 = lims[2 * b] - 1, g1 = lims[2 * b + 1];        // g-points [g0, g1)
+        if (g0 < 0 || g1 > ngpt || g1 <= g0) return arg_error("kdist_pack: bad band_lims_gpt");
+        B.g0 = g0; B.ng = g1 - g0; B.gp = (B.ng + 3) / 4 * 4;
+        max_gpts = std::max(max_gpts, B.ng);
+        for (int s = 0; s < 2; ++s)
+        {
+            const int fl = gflav[2 * g0 + s];
+            if (fl < 1 || fl > kd->nflav) return arg_error("kdist_pack: gpoint_flavor out of range");
+            for (int gg = g0; gg < g1; ++gg)
+                if (gflav[2 * gg + s] != fl) return arg_error("kdist_pack: the g-points of a band must share one flavor");
+            B.gas[s][0] = flav[2 * (fl - 1)];
+            B.gas[s][1] = flav[2 * (fl - 1) + 1];
+            if (!gas_ok(B.gas[s][0]) || !gas_ok(B.gas[s][1])) return arg_error("kdist_pack: flavor gas out of range");
+            int q = 0;
+            for (int64_t m = 0; m < nmin[s]; ++m)
+            {
+                const int ms = mlim[s][2 * m] - 1, me = mlim[s][2 * m + 1] - 1;   // inclusive
+                if (!(ms <= g1 - 1 && me >= g0)) continue;
+                if (ms < 0 || me >= ngpt || me < ms) return arg_error("kdist_pack: bad minor_limits_gpt");
+                if (q == GO_MAXM)
+                    return arg_error("kdist_pack: more than " + std::to_string(GO_MAXM) + " minor intervals in a band");
+                if (gflav[2 * ms + s] != fl)
+                    return arg_error("kdist_pack: a minor interval does not have its band's flavor");
+                if (!gas_ok(idxm[s][m]) || idxs[s][m] > kd->ngas) return arg_error("kdist_pack: minor gas out of range");
+                if (kst[s][m] < 1 || kst[s][m] - 1 + (me - ms) >= ncon[s])
+                    return arg_error("kdist_pack: kminor_start out of range");
+                B.minor[s][q] = GoMinor{idxm[s][m], dens[s][m] ? 1 : 0, idxs[s][m], cmpl[s][m] ? 1 : 0};
+                minor_of[(b * 2 + s) * GO_MAXM + q] = int(m);
+                ++q;
+            }
+            B.nq[s] = q;
+        }
+        B.nslot = std::max(B.nq[0], B.nq[1]);
+        B.koff = int(koff); B.roff = int(roff); B.moff = int(moff);
+        koff += nt * ne * np1 * B.gp;
+        roff += 2 * nt * ne * B.gp;
+        moff += 2 * nt * ne * B.nslot * B.gp;
+        const int64_t lim = (int64_t(1) << 32) / int64_t(sizeof(Real));    // 32-bit byte offsets
+        if (koff >= lim || moff >= lim || roff >= lim)
+            return arg_error("kdist_pack: tables too large for 32-bit offsets");
+    }
+
+    std::vector<Real> kmaj_p(size_t(koff), Real(0)), pfr_p(lw ? size_t(koff) : 0, Real(0));
+    std::vector<Real> kray_p(lw ? 0 : size_t(roff), Real(0)), kmin_p(size_t(moff), Real(0));
+    for (int64_t b = 0; b < nb; ++b)
+    {
+        const GoBand& B = bands[b];
+        for (int64_t t = 0; t < nt; ++t)
+            for (int64_t e = 0; e < ne; ++e)
+            {
+                for (int64_t p = 0; p < np1; ++p)
+                    for (int gl = 0; gl < B.ng; ++gl)
+                    {
+                        const size_t dst = size_t(B.koff + ((t * ne + e) * np1 + p) * B.gp + gl);
+                        const size_t src = size_t(t + nt * (e + ne * (p + np1 * (B.g0 + gl))));
+                        kmaj_p[dst] = kmaj[src];
+                        if (lw) pfr_p[dst] = pfr[src];
+                    }
+                for (int s = 0; s < 2; ++s)
+                {
+                    if (!lw)
+                        for (int gl = 0; gl < B.ng; ++gl)
+                            kray_p[size_t(B.roff + ((s * nt + t) * ne + e) * B.gp + gl)] =
+                                    kray[size_t(t + nt * (e + ne * ((B.g0 + gl) + ngpt * s)))];
+                    for (int q = 0; q < B.nq[s]; ++q)
+                    {
+                        const int m = minor_of[(b * 2 + s) * GO_MAXM + q];
+                        const int ms = mlim[s][2 * m] - 1, me = mlim[s][2 * m + 1] - 1;
+                        for (int gl = 0; gl < B.ng; ++gl)
+                        {
+                            const int gg = B.g0 + gl;
+                            if (gg < ms || gg > me) continue;
+                            const int64_t ic = kst[s][m] - 1 + (gg - ms);
+                            kmin_p[size_t(B.moff + (((s * nt + t) * ne + e) * B.nslot + q) * B.gp + gl)] =
+                                    kmin[s][size_t(t + nt * (e + ne * ic))];
+                        }
+                    }
+                }
+            }
+    }
+
+    auto pk = std::make_unique<GoPack>();
+    RRTMGP_TRY(pack_h2d(&pk->mem[0], bands));
+    RRTMGP_TRY(pack_h2d(&pk->mem[1], kmaj_p));
+    RRTMGP_TRY(pack_h2d(&pk->mem[2], kmin_p));
+    if (lw) RRTMGP_TRY(pack_h2d(&pk->mem[3], pfr_p));
+    else RRTMGP_TRY(pack_h2d(&pk->mem[4], kray_p));
+    kd->go_bands = pk->mem[0];
+    kd->kmajor_p = pk->mem[1];
+    kd->kminor_p = pk->mem[2];
+    kd->planck_frac_p = pk->mem[3];
+    kd->krayl_p = pk->mem[4];
+    kd->max_band_gpts = max_gpts;
+    *pack_out = pk.release();
     return RRTMGP_OK;
 }
 
@@ -883,6 +1174,7 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
     const Kd<Real> k = make_kd<Real>(*kd);
     const GoGrid grid = make_go_grid(ncol, nlay, kd->nbnd, kd->max_band_gpts);
+    RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
                        as_stream(stream), k, g, ncol, nlay, grid, play, tlay, tlev, tsfc, col_dry,
@@ -905,6 +1197,7 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, gs));
     const Kd<Real> k = make_kd<Real>(*kd);
     const GoGrid grid = make_go_grid(ncol, nlay, kd->nbnd, kd->max_band_gpts);
+    RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
                        as_stream(stream), k, gs, ncol, nlay, grid, play, tlay, col_dry, ctau, cssa, cg,
@@ -1027,3 +1320,11 @@ extern "C" int rrtmgp_hip_fill_col_gas_##S(int64_t ncol, int64_t nlay, const rrt
 
 RRTMGP_DEF_GAS_OPTICS(f32, float)
 RRTMGP_DEF_GAS_OPTICS(f64, double)
+
+extern "C" int rrtmgp_hip_kdist_pack_f32(rrtmgp_kdist_dev* kd, void** pack) { return kdist_pack<float>(kd, pack); }
+extern "C" int rrtmgp_hip_kdist_pack_f64(rrtmgp_kdist_dev* kd, void** pack) { return kdist_pack<double>(kd, pack); }
+extern "C" int rrtmgp_hip_kdist_pack_free(void* pack)
+{
+    delete static_cast<GoPack*>(pack);
+    return RRTMGP_OK;
+}

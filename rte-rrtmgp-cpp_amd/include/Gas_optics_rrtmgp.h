@@ -202,16 +202,13 @@ class Gas_optics_rrtmgp_gpu : public Gas_optics_gpu
         Array_gpu<Float,2> totplnk_gpu;
         Array_gpu<Float,4> planck_frac_gpu, kmajor_gpu, krayl_gpu;
         Array_gpu<Float,3> vmr_ref_gpu, kminor_lower_gpu, kminor_upper_gpu;
-        Array_gpu<Float,4> kmajor_g_gpu, planck_frac_g_gpu, krayl_g_gpu;   // g-point index fastest
-        Array_gpu<Float,3> kminor_lower_g_gpu, kminor_upper_g_gpu;
         Array_gpu<int,2> flavor_gpu, gpoint_flavor_gpu, minor_limits_gpt_lower_gpu, minor_limits_gpt_upper_gpu;
         Array_gpu<Bool,1> minor_scales_with_density_lower_gpu, minor_scales_with_density_upper_gpu;
         Array_gpu<Bool,1> scale_by_complement_lower_gpu, scale_by_complement_upper_gpu;
         Array_gpu<int,1> kminor_start_lower_gpu, kminor_start_upper_gpu;
         Array_gpu<int,1> idx_minor_lower_gpu, idx_minor_upper_gpu;
         Array_gpu<int,1> idx_minor_scaling_lower_gpu, idx_minor_scaling_upper_gpu;
-        Array_gpu<int,1> minor_band_start_lower_gpu, minor_band_list_lower_gpu;
-        Array_gpu<int,1> minor_band_start_upper_gpu, minor_band_list_upper_gpu;
+        std::shared_ptr<void> go_pack;    // band-packed tables (rrtmgp_hip_kdist_pack)
         rrtmgp_kdist_dev dev;
 
         void init_abs_coeffs(

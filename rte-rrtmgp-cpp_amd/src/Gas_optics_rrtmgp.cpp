@@ -376,14 +376,6 @@ Float Gas_optics_rrtmgp_gpu::get_tsi() const
 
 void Gas_optics_rrtmgp_gpu::upload()
 {
-    // All g-points of a band share the band's flavor (gpoint_flavor derives from gpt2band).
-    const Array<int,2>& bl = get_band_lims_gpoint();
-    for (int b = 1; b <= get_nband(); ++b)
-        for (int g = bl({1, b}); g <= bl({2, b}); ++g)
-            for (int a = 1; a <= 2; ++a)
-                if (gpoint_flavor({a, g}) != gpoint_flavor({a, bl({1, b})}) || gpoint_flavor({a, g}) < 1)
-                    throw std::runtime_error("Gas optics: g-points of a band must share one flavor");
-
     up(press_ref_log_gpu, press_ref_log); up(temp_ref_gpu, temp_ref); up(vmr_ref_gpu, vmr_ref);
     up(kmajor_gpu, kmajor); up(kminor_lower_gpu, kminor_lower); up(kminor_upper_gpu, kminor_upper);
     up(flavor_gpu, flavor); up(gpoint_flavor_gpu, gpoint_flavor);
@@ -398,30 +390,6 @@ void Gas_optics_rrtmgp_gpu::upload()
     up(idx_minor_scaling_upper_gpu, idx_minor_scaling_upper);
     up(totplnk_gpu, totplnk); up(planck_frac_gpu, planck_frac); up(krayl_gpu, krayl);
     up(solar_source_gpu, solar_source);
-
-    // Minor intervals touching each band, ascending, for both halves of the atmosphere.
-    int max_minors = 0;
-    bool flavor_uniform = true;
-    auto band_csr = [&](const Array<int,2>& lim, Array_gpu<int,1>& start_d, Array_gpu<int,1>& list_d, int half) {
-        const int nb = get_nband();
-        std::vector<int> start(1, 0), list;
-        for (int b = 1; b <= nb; ++b)
-        {
-            for (int m = 1; m <= lim.dim(2); ++m)
-                if (lim({1, m}) <= bl({2, b}) && lim({2, m}) >= bl({1, b}))
-                {
-                    list.push_back(m - 1);
-                    if (gpoint_flavor({half, lim({1, m})}) != gpoint_flavor({half, bl({1, b})})) flavor_uniform = false;
-                }
-            start.push_back(int(list.size()));
-            max_minors = std::max(max_minors, start.back() - start[start.size() - 2]);
-        }
-        if (list.empty()) list.push_back(0);
-        start_d = Array_gpu<int,1>(Array<int,1>(start, {int(start.size())}));
-        list_d = Array_gpu<int,1>(Array<int,1>(list, {int(list.size())}));
-    };
-    band_csr(minor_limits_gpt_lower, minor_band_start_lower_gpu, minor_band_list_lower_gpu, 1);
-    band_csr(minor_limits_gpt_upper, minor_band_start_upper_gpu, minor_band_list_upper_gpu, 2);
 
     dev = rrtmgp_kdist_dev{};
     dev.ngas = get_ngas(); dev.nflav = get_nflav(); dev.neta = kmajor_neta; dev.npres = kmajor_npres;
@@ -447,60 +415,15 @@ void Gas_optics_rrtmgp_gpu::upload()
     dev.idx_minor_scaling_lower = idx_minor_scaling_lower_gpu.ptr();
     dev.idx_minor_scaling_upper = idx_minor_scaling_upper_gpu.ptr();
     dev.kminor_start_lower = kminor_start_lower_gpu.ptr(); dev.kminor_start_upper = kminor_start_upper_gpu.ptr();
-    dev.minor_band_start_lower = minor_band_start_lower_gpu.ptr();
-    dev.minor_band_list_lower = minor_band_list_lower_gpu.ptr();
-    dev.minor_band_start_upper = minor_band_start_upper_gpu.ptr();
-    dev.minor_band_list_upper = minor_band_list_upper_gpu.ptr();
     dev.totplnk = totplnk_gpu.ptr(); dev.planck_frac = planck_frac_gpu.ptr(); dev.krayl = krayl_gpu.ptr();
 
-    // g-fastest copies: reversing all dims of a Fortran array puts its last index first.
-    auto reversed = [](const Array<Float,4>& a) {
-        const int n1 = a.dim(1), n2 = a.dim(2), n3 = a.dim(3), n4 = a.dim(4);
-        std::vector<Float> o(size_t(a.size()));
-        for (int i4 = 0; i4 < n4; ++i4)
-            for (int i3 = 0; i3 < n3; ++i3)
-                for (int i2 = 0; i2 < n2; ++i2)
-                    for (int i1 = 0; i1 < n1; ++i1)
-                        o[i4 + size_t(n4) * (i3 + size_t(n3) * (i2 + size_t(n2) * i1))] =
-                                a.v()[i1 + size_t(n1) * (i2 + size_t(n2) * (i3 + size_t(n3) * i4))];
-        return Array<Float,4>(o, {n4, n3, n2, n1});
-    };
-    auto reversed3 = [](const Array<Float,3>& a) {
-        const int n1 = a.dim(1), n2 = a.dim(2), n3 = a.dim(3);
-        std::vector<Float> o(size_t(a.size()));
-        for (int i3 = 0; i3 < n3; ++i3)
-            for (int i2 = 0; i2 < n2; ++i2)
-                for (int i1 = 0; i1 < n1; ++i1)
-                    o[i3 + size_t(n3) * (i2 + size_t(n2) * i1)] = a.v()[i1 + size_t(n1) * (i2 + size_t(n2) * i3)];
-        return Array<Float,3>(o, {n3, n2, n1});
-    };
-    up(kmajor_g_gpu, reversed(kmajor));
-    if (planck_frac.size()) up(planck_frac_g_gpu, reversed(planck_frac));
-    if (kminor_lower.size()) up(kminor_lower_g_gpu, reversed3(kminor_lower));
-    if (kminor_upper.size()) up(kminor_upper_g_gpu, reversed3(kminor_upper));
-    if (krayl.size())
-    {
-        // (ntemp, neta, ngpt, 2) -> (ngpt, neta, ntemp, 2)
-        const int nt = krayl.dim(1), ne = krayl.dim(2), ng = krayl.dim(3);
-        std::vector<Float> o(size_t(krayl.size()));
-        for (int s2 = 0; s2 < 2; ++s2)
-            for (int g = 0; g < ng; ++g)
-                for (int e = 0; e < ne; ++e)
-                    for (int t = 0; t < nt; ++t)
-                        o[g + size_t(ng) * (e + size_t(ne) * (t + size_t(nt) * s2))] =
-                                krayl.v()[t + size_t(nt) * (e + size_t(ne) * (g + size_t(ng) * s2))];
-        up(krayl_g_gpu, Array<Float,4>(o, {ng, ne, nt, 2}));
-    }
-    dev.kmajor_g = kmajor_g_gpu.ptr(); dev.planck_frac_g = planck_frac_g_gpu.ptr();
-    dev.kminor_lower_g = kminor_lower_g_gpu.ptr(); dev.kminor_upper_g = kminor_upper_g_gpu.ptr();
-    dev.krayl_g = krayl_g_gpu.ptr();
     dev.ncontrib_lower = kminor_lower.size() ? kminor_lower.dim(3) : 0;
     dev.ncontrib_upper = kminor_upper.size() ? kminor_upper.dim(3) : 0;
-    int max_gpts = 0;
-    for (int b = 1; b <= get_nband(); ++b) max_gpts = std::max(max_gpts, bl({2, b}) - bl({1, b}) + 1);
-    dev.max_band_gpts = max_gpts;
-    dev.max_band_minors = max_minors;
-    dev.minor_flavor_uniform = flavor_uniform ? 1 : 0;
+
+    // Band-packed tables of the fused gas-optics kernels.
+    void* pk = nullptr;
+    hip_check(K<Float>::kdist_pack(&dev, &pk));
+    go_pack.reset(pk, [](void* q) { rrtmgp_hip_kdist_pack_free(q); });
 }
 
 

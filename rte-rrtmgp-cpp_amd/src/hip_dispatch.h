@@ -8,6 +8,7 @@ template<typename T> struct K;
 
 #define RRTMGP_K_ENTRIES(S)                                              \
     static constexpr auto gas_optics_lw = rrtmgp_hip_gas_optics_lw_##S;  \
+    static constexpr auto kdist_pack = rrtmgp_hip_kdist_pack_##S;  \
     static constexpr auto gas_optics_sw = rrtmgp_hip_gas_optics_sw_##S;  \
     static constexpr auto interpolation = rrtmgp_hip_interpolation_##S;  \
     static constexpr auto compute_tau_absorption = rrtmgp_hip_compute_tau_absorption_##S; \
