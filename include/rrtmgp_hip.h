@@ -113,7 +113,7 @@ typedef struct
     const int* idx_minor_scaling_upper;
     const int* kminor_start_lower;         /* 1-based contributor offset                    */
     const int* kminor_start_upper;
-    const void* totplnk;                  /* (nplanck, nbnd)          LW only              */
+    const void* totplnk;                   /* (nplanck, nbnd)          LW only              */
     const void* planck_frac;               /* (ntemp, neta, npres+1, ngpt)  LW only         */
     const void* krayl;                     /* (ntemp, neta, ngpt, 2)   SW only              */
     int64_t ncontrib_lower, ncontrib_upper;  /* last dims of kminor_lower / kminor_upper     */
@@ -343,6 +343,39 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
         T* bnd_flux_dn, T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir,   \
         void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
+
+/* ------------------------------------------------------------------------------------------ */
+/* Adjoints.  No counterpart in the reference (its LW path has only the surface-temperature   */
+/* Jacobian, src/Rte_lw.cpp:140).                                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Reverse mode of rrtmgp_hip_lw_solver_noscat_* with broadband output: for cotangents         */
+/* bb_up_cot, bb_dn_cot (ncol, nlev) of the broadband fluxes, writes the gradients of          */
+/* L = sum(bb_up_cot * broadband_up + bb_dn_cot * broadband_dn) with respect to tau, lay_source,*/
+/* lev_source_inc, lev_source_dec (ncol, nlay, ngpt) and sfc_src, sfc_emis (ncol, ngpt), for   */
+/* the same forward inputs (inc_flux may be NULL; it receives no gradient).                    */
+#define RRTMGP_DECL_LW_ADJOINT(S, T)                                                           \
+    int rrtmgp_hip_lw_solver_noscat_adjoint_##S(int64_t ncol, int64_t nlay, int64_t ngpt,      \
+        int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,           \
+        const T* lay_source, const T* lev_source_inc, const T* lev_source_dec,                 \
+        const T* sfc_emis, const T* sfc_src, const T* inc_flux, const T* bb_up_cot,            \
+        const T* bb_dn_cot, T* d_tau, T* d_lay_source, T* d_lev_source_inc,                    \
+        T* d_lev_source_dec, T* d_sfc_src, T* d_sfc_emis, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_ADJOINT)
+
+/* Reverse mode of rrtmgp_hip_sw_solver_2stream_* with broadband output: for cotangents        */
+/* bb_up_cot, bb_dn_cot, bb_dir_cot (ncol, nlev) of the broadband up, total down and direct    */
+/* fluxes, writes the gradients of L = sum(cot * flux) with respect to tau, ssa, g             */
+/* (ncol, nlay, ngpt) and sfc_alb_dir, sfc_alb_dif (ncol, ngpt).  The layer properties are     */
+/* differentiated in FP64 in both precisions.  Uses FP64 scratch of 14 (nlay+1) ngpt ncol      */
+/* values; callers bound it by calling per band (or g-point range).                            */
+#define RRTMGP_DECL_SW_ADJOINT(S, T)                                                           \
+    int rrtmgp_hip_sw_solver_2stream_adjoint_##S(int64_t ncol, int64_t nlay, int64_t ngpt,     \
+        int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,                    \
+        const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir,                     \
+        const T* inc_flux_dif, const T* bb_up_cot, const T* bb_dn_cot, const T* bb_dir_cot,    \
+        T* d_tau, T* d_ssa, T* d_g, T* d_sfc_alb_dir, T* d_sfc_alb_dif, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_ADJOINT)
 
 /* ------------------------------------------------------------------------------------------ */
 /* Flux reductions (Fluxes_kernels_cuda, include_kernels_cuda/fluxes_kernels_cuda.h:31-50;    */

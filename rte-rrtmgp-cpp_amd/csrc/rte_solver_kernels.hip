@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "common.h"
+#include "two_stream.h"
 
 namespace rrtmgp_hip
 {
@@ -41,11 +42,13 @@ constexpr int SB_COLS = 64;     // columns per work-group: the lanes of each wav
 constexpr int SB_GPTS = 16;     // g-points per work-group: one wavefront each, within one band
 constexpr int MAX_GROUPS = 256;
 
-// Segment lengths: levels per LDS reduction round (LW, SW) and the SW checkpoint spacing, sized
-// so that the SW segment registers fit the 128 VGPRs of a 1024-thread work-group without spills.
+// Segment lengths: levels per LDS reduction round (LW, SW) and the SW checkpoint spacing.  The
+// LW segment inputs are double-buffered in registers (K1 for one angle, K4 for up to four), sized
+// for 64 VGPRs in FP32 (two 1024-thread work-groups per CU) without spills; the SW segment
+// registers fit the 128 VGPRs of a 1024-thread work-group.
 template<typename Real> struct LwSeg;
-template<> struct LwSeg<float> { static constexpr int K = 16; };
-template<> struct LwSeg<double> { static constexpr int K = 8; };
+template<> struct LwSeg<float> { static constexpr int K1 = 6, K4 = 4; };
+template<> struct LwSeg<double> { static constexpr int K1 = 4, K4 = 4; };
 template<typename Real> struct SwSeg;
 template<> struct SwSeg<float> { static constexpr int K = 4; };
 template<> struct SwSeg<double> { static constexpr int K = 4; };
@@ -117,215 +120,27 @@ __device__ __forceinline__ void write_group_sums(const Real* red, int nslots, in
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Longwave: no scattering, linear-in-tau source, nmus Gaussian angles.
-template<typename Real, bool JAC>
-__global__ __launch_bounds__(SB_COLS * SB_GPTS)
-void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int nmus,
-                      const Real* __restrict__ secants, const Real* __restrict__ weights,
-                      const Real* __restrict__ tau, const Real* __restrict__ lay_src,
-                      const Real* __restrict__ lev_inc, const Real* __restrict__ lev_dec,
-                      const Real* __restrict__ emis, const Real* __restrict__ sfc_src,
-                      const Real* __restrict__ inc_flux, Real* __restrict__ flux_up,
-                      Real* __restrict__ flux_dn, const Real* __restrict__ sfc_src_jac,
-                      Groups grp, Real* __restrict__ grp_up, Real* __restrict__ grp_dn,
-                      Real* __restrict__ grp_jac)
+// write_group_sums for component 0 only of an LDS layout with NC components per slot.
+template<typename Real, int NC, typename LevOf>
+__device__ __forceinline__ void write_group_sums_stride(const Real* red, int nslots, int gcount, int64_t col0,
+                                                        int64_t ncol, int64_t nlev, int gi,
+                                                        Real* const (&out)[1], LevOf lev_of)
 {
-    constexpr int K = LwSeg<Real>::K;
-    constexpr int NC = JAC ? 2 : 1;
-    __shared__ Real red[K * NC * SB_GPTS * SB_COLS];
-
-    const int lane = threadIdx.x, w = threadIdx.y;
-    const int gi = blockIdx.y;
-    const int gcount = grp.count[gi];
-    const int64_t col0 = int64_t(blockIdx.x) * SB_COLS;
-    const int64_t icol = col0 + lane;
-    const int64_t igpt = grp.start[gi] + w;
-    const bool active = icol < ncol && w < gcount;
-    const bool reduce = grp_dn != nullptr;
-    const int64_t nlev = nlay + 1;
-    const int64_t c2 = icol + ncol * igpt;
-    const int64_t lb = icol + ncol * nlay * igpt;
-    const int64_t vb = icol + ncol * nlev * igpt;
-    const Real* lev_dn = top_at_1 ? lev_inc : lev_dec;
-    const Real* lev_up = top_at_1 ? lev_dec : lev_inc;
-    const Real tau_thresh = sqrt(Limits<Real>::eps);
-    const Real pi = Real(3.14159265358979323846);
-
-    Real D[4], fac[4], I[4];
-#pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
+    const int tid = threadIdx.y * SB_COLS + threadIdx.x;
+    const int items = nslots * SB_COLS;
+    for (int it = tid; it < items; it += SB_COLS * SB_GPTS)
     {
-        D[mu] = Real(0); fac[mu] = Real(0); I[mu] = Real(0);
-        if (active && mu < nmus)
-        {
-            D[mu] = secants[c2 + ncol * ngpt * mu];
-            fac[mu] = Real(2) * pi * weights[mu];
-            I[mu] = (inc_flux ? inc_flux[c2] : Real(0)) / fac[mu];
-        }
-    }
-
-    // Down sweep: level j counted from the top (0 = top boundary).
-    for (int64_t s0 = 0; s0 <= nlay; s0 += K)
-    {
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk)
-        {
-            const int64_t j = s0 + kk;
-            Real f = Real(0);
-            if (active && j <= nlay)
-            {
-                if (j > 0)
-                {
-                    const int64_t ilay = top_at_1 ? j - 1 : nlay - j;
-                    const int64_t l = lb + ncol * ilay;
-                    const Real tv = tau[l], ls = lay_src[l], ld = lev_dn[l];
-#pragma unroll
-                    for (int mu = 0; mu < 4; ++mu)
-                        if (mu < nmus)
-                        {
-                            const Real t = tv * D[mu];
-                            const Real T = exp(-t);
-                            const Real fact = (t > tau_thresh) ? (Real(1) - T) / t - T
-                                                               : t * (Real(0.5) - Real(1. / 3.) * t);
-                            I[mu] = T * I[mu] + ((Real(1) - T) * ld + Real(2) * fact * (ls - ld));
-                        }
-                }
-#pragma unroll
-                for (int mu = 0; mu < 4; ++mu)
-                    if (mu < nmus) f = (mu == 0) ? fac[0] * I[0] : f + fac[mu] * I[mu];
-                if (flux_dn) flux_dn[vb + ncol * (top_at_1 ? j : nlay - j)] = f;
-            }
-            if (reduce) red[(kk * NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
-        }
-        if (reduce)
-        {
-            __syncthreads();
-            Real* const outs[1] = {grp_dn};
-            const int nsl = int(std::min<int64_t>(K, nlay + 1 - s0));
-            write_group_sums<Real, 1>(red, nsl, gcount, col0, ncol, nlev, gi, outs, [&](int kk) -> int64_t {
-                const int64_t j = s0 + kk;
-                return top_at_1 ? j : nlay - j;
-            });
-            __syncthreads();
-        }
-    }
-
-    // Surface: reflection of the downward intensity plus emission; then the up sweep with
-    // level j counted from the surface (0 = surface).
-    Real Ij[4];
-    {
-        const Real e = active ? emis[c2] : Real(0);
-        const Real ss = active ? sfc_src[c2] : Real(0);
-        const Real sj = (active && JAC) ? sfc_src_jac[c2] : Real(0);
-#pragma unroll
-        for (int mu = 0; mu < 4; ++mu)
-        {
-            I[mu] = I[mu] * (Real(1) - e) + e * ss;
-            Ij[mu] = e * sj;
-        }
-    }
-    for (int64_t s0 = 0; s0 <= nlay; s0 += K)
-    {
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk)
-        {
-            const int64_t j = s0 + kk;
-            Real f = Real(0), fj = Real(0);
-            if (active && j <= nlay)
-            {
-                if (j > 0)
-                {
-                    const int64_t ilay = top_at_1 ? nlay - j : j - 1;
-                    const int64_t l = lb + ncol * ilay;
-                    const Real tv = tau[l], ls = lay_src[l], lu = lev_up[l];
-#pragma unroll
-                    for (int mu = 0; mu < 4; ++mu)
-                        if (mu < nmus)
-                        {
-                            const Real t = tv * D[mu];
-                            const Real T = exp(-t);
-                            const Real fact = (t > tau_thresh) ? (Real(1) - T) / t - T
-                                                               : t * (Real(0.5) - Real(1. / 3.) * t);
-                            I[mu] = T * I[mu] + ((Real(1) - T) * lu + Real(2) * fact * (ls - lu));
-                            if (JAC) Ij[mu] = T * Ij[mu];
-                        }
-                }
-#pragma unroll
-                for (int mu = 0; mu < 4; ++mu)
-                    if (mu < nmus)
-                    {
-                        f = (mu == 0) ? fac[0] * I[0] : f + fac[mu] * I[mu];
-                        if (JAC) fj = (mu == 0) ? fac[0] * Ij[0] : fj + fac[mu] * Ij[mu];
-                    }
-                if (flux_up) flux_up[vb + ncol * (top_at_1 ? nlay - j : j)] = f;
-            }
-            if (reduce)
-            {
-                red[(kk * NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
-                if (JAC) red[(kk * NC + 1) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = fj;
-            }
-        }
-        if (reduce)
-        {
-            __syncthreads();
-            const int nsl = int(std::min<int64_t>(K, nlay + 1 - s0));
-            auto lev_of = [&](int kk) -> int64_t {
-                const int64_t j = s0 + kk;
-                return top_at_1 ? nlay - j : j;
-            };
-            if constexpr (JAC)
-            {
-                Real* const outs[2] = {grp_up, grp_jac};
-                write_group_sums<Real, 2>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
-            }
-            else
-            {
-                Real* const outs[1] = {grp_up};
-                write_group_sums<Real, 1>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
-            }
-            __syncthreads();
-        }
+        const int lane = it % SB_COLS;
+        const int kk = it / SB_COLS;
+        const int64_t icol = col0 + lane;
+        const int64_t lev = lev_of(kk);
+        if (icol >= ncol || lev < 0) continue;
+        const Real* r = red + int64_t(kk) * NC * SB_GPTS * SB_COLS + lane;
+        Real s = r[0];
+        for (int w = 1; w < gcount; ++w) s += r[w * SB_COLS];
+        out[0][icol + ncol * (lev + nlev * gi)] = s;
     }
 }
-
-// ---------------------------------------------------------------------------------------------
-// Shortwave two-stream layer properties (sw_dif_and_source, F90:1043-1104; clamp .cu:581-583),
-// evaluated in FP64.
-__device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, double mu0, double& Rdif,
-                                              double& Tdif, double& Rdir, double& Tdir, double& Tnoscat)
-{
-    const double eps = Limits<double>::eps;
-    const double gamma1 = (8. - w0 * (5. + 3. * g)) * .25;
-    const double gamma2 = 3. * (w0 * (1. - g)) * .25;
-    const double gamma3 = (2. - 3. * mu0 * g) * .25;
-    const double gamma4 = 1. - gamma3;
-    const double alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
-    const double alpha2 = gamma1 * gamma3 + gamma2 * gamma4;
-    const double k = sqrt(fmax((gamma1 - gamma2) * (gamma1 + gamma2), Limits<double>::k_min));
-    const double k_mu = k * mu0, k_gamma3 = k * gamma3, k_gamma4 = k * gamma4;
-    const double e1 = exp(-tau * k);
-    const double e2 = e1 * e1;
-    const double RT = 1. / (k * (1. + e2) + gamma1 * (1. - e2));
-    Rdif = RT * gamma2 * (1. - e2);
-    Tdif = RT * 2. * k * e1;
-    const double d = 1. - k_mu * k_mu;
-    const double denom = (fabs(d) >= eps) ? d : eps;
-    const double RT2 = w0 * RT / denom;
-    Tnoscat = exp(-tau / mu0);
-    double rd = RT2 * ((1. - k_mu) * (alpha2 + k_gamma3) - (1. + k_mu) * (alpha2 - k_gamma3) * e2 -
-                       2. * (k_gamma3 - alpha2 * k_mu) * e1 * Tnoscat);
-    double td = -RT2 * ((1. + k_mu) * (alpha1 + k_gamma4) * Tnoscat -
-                        (1. - k_mu) * (alpha1 - k_gamma4) * e2 * Tnoscat -
-                        2. * (k_gamma4 + alpha1 * k_mu) * e1);
-    rd = fmax(eps, fmin(rd, 1. - Tnoscat));
-    td = fmax(eps, fmin(td, 1. - Tnoscat - rd));
-    Rdir = rd;
-    Tdir = td;
-}
-
-// One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
-struct LayerSW { double R, T, SU, SD, Tn; };
 
 // FP32 building blocks on the hardware approximations (v_exp_f32, v_rcp_f32, v_sqrt_f32, each
 // within about 1 ulp), which cost one or two instructions instead of the IEEE sequences.
@@ -345,6 +160,210 @@ __device__ __forceinline__ float one_minus_exp_neg_f32(float y)
     t = 1.f - y * t;
     return (y < .25f) ? y * t : 1.f - exp_neg_f32(y);
 }
+
+// Array element i at a uniform base with a 32-bit lane offset: addressed as scalar base +
+// 32-bit VGPR offset (the launchers keep the offsets below 2^32 bytes).
+template<typename Real>
+__device__ __forceinline__ const Real& at32(const Real* base, uint32_t i)
+{
+    return *reinterpret_cast<const Real*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(Real)));
+}
+
+template<typename Real>
+__device__ __forceinline__ Real& at32(Real* base, uint32_t i)
+{
+    return *reinterpret_cast<Real*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(Real)));
+}
+
+// Layer transmittance T = e^{-t} and the linear-in-tau source factor of lw_source_noscat
+// (F90:616-653): fact = (1 - T)/t - T, with t (1/2 - t/3) below sqrt(epsilon).  The FP32 build
+// evaluates fact for t < 1/4 from its series t/2 - t^2/3 + t^3/8 - t^4/30 + t^5/144 - t^6/840
+// + t^7/5760 (the reference's small-t form is its first two terms; truncation < 3e-9 relative),
+// which avoids the cancellation of (1 - T)/t - T, and uses the hardware exp2 and reciprocal.
+__device__ __forceinline__ void lw_layer(double t, double& T, double& fact)
+{
+    T = exp(-t);
+    fact = (t > sqrt(Limits<double>::eps)) ? (1. - T) / t - T : t * (.5 - (1. / 3.) * t);
+}
+
+__device__ __forceinline__ void lw_layer(float t, float& T, float& fact)
+{
+    T = exp_neg_f32(t);
+    float q = 1.f / 840.f - t * (1.f / 5760.f);
+    q = 1.f / 144.f - t * q;
+    q = 1.f / 30.f - t * q;
+    q = 1.f / 8.f - t * q;
+    q = 1.f / 3.f - t * q;
+    q = .5f - t * q;
+    fact = (t < .25f) ? t * q : (1.f - T) * rcp_f32(t) - T;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Longwave: no scattering, linear-in-tau source, nmus <= NMU Gaussian angles.  The layer inputs
+// of the next K-level segment are read while the current segment is computed.
+template<typename Real, bool JAC, int NMU>
+__global__ __launch_bounds__(SB_COLS * SB_GPTS) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 8 : 4)))
+void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int nmus,
+                      const Real* __restrict__ secants, const Real* __restrict__ weights,
+                      const Real* __restrict__ tau, const Real* __restrict__ lay_src,
+                      const Real* __restrict__ lev_inc, const Real* __restrict__ lev_dec,
+                      const Real* __restrict__ emis, const Real* __restrict__ sfc_src,
+                      const Real* __restrict__ inc_flux, Real* __restrict__ flux_up,
+                      Real* __restrict__ flux_dn, const Real* __restrict__ sfc_src_jac,
+                      Groups grp, Real* __restrict__ grp_up, Real* __restrict__ grp_dn,
+                      Real* __restrict__ grp_jac)
+{
+    constexpr int K = NMU == 1 ? LwSeg<Real>::K1 : LwSeg<Real>::K4;
+    constexpr int NC = JAC ? 2 : 1;
+    __shared__ Real red[K * NC * SB_GPTS * SB_COLS];
+
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.y));   // wavefront = g-point
+    const int gi = blockIdx.y;
+    const int gcount = grp.count[gi];
+    const int64_t col0 = int64_t(blockIdx.x) * SB_COLS;
+    const int64_t icol = col0 + lane;
+    const int64_t igpt = grp.start[gi] + w;
+    const bool active = icol < ncol && w < gcount;
+    const bool reduce = grp_dn != nullptr;
+    const int64_t nlev = nlay + 1;
+    const int64_t c2 = icol + ncol * igpt;
+    // Per-g-point slabs at wavefront-uniform bases; lane offsets icol + ncol * level (32-bit).
+    const int64_t lslab = ncol * nlay * igpt, vslab = ncol * nlev * igpt;
+    const Real* tau_g = tau + lslab;
+    const Real* lay_g = lay_src + lslab;
+    const Real* lev_dn_g = (top_at_1 ? lev_inc : lev_dec) + lslab;
+    const Real* lev_up_g = (top_at_1 ? lev_dec : lev_inc) + lslab;
+    Real* flux_up_g = flux_up ? flux_up + vslab : nullptr;
+    Real* flux_dn_g = flux_dn ? flux_dn + vslab : nullptr;
+    const uint32_t ucol = uint32_t(lane) + uint32_t(col0), uncol = uint32_t(ncol);
+    const Real pi = Real(3.14159265358979323846);
+
+    Real D[NMU], fac[NMU], I[NMU], Ij[NMU];
+#pragma unroll
+    for (int mu = 0; mu < NMU; ++mu)
+    {
+        D[mu] = Real(0); fac[mu] = Real(0); I[mu] = Real(0); Ij[mu] = Real(0);
+        if (active && mu < nmus)
+        {
+            D[mu] = secants[c2 + ncol * ngpt * mu];
+            fac[mu] = Real(2) * pi * weights[mu];
+            I[mu] = (inc_flux ? inc_flux[c2] : Real(0)) / fac[mu];
+        }
+    }
+
+    // Inputs of levels s0 .. s0+K-1 of a sweep (level j > 0 ends layer layer_of(j)).
+    Real bt[K], bl[K], bx[K];
+    auto fetch = [&](int64_t s0, const Real* lev_g, auto layer_of) {
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk)
+        {
+            const int64_t j = s0 + kk;
+            bt[kk] = Real(0); bl[kk] = Real(0); bx[kk] = Real(0);
+            if (active && j >= 1 && j <= nlay)
+            {
+                const uint32_t l = ucol + uncol * uint32_t(layer_of(j));
+                bt[kk] = at32(tau_g, l); bl[kk] = at32(lay_g, l); bx[kk] = at32(lev_g, l);
+            }
+        }
+    };
+    // One sweep over levels 0..nlay; lev_out(j) is the output level of sweep level j.
+    auto sweep = [&](const Real* lev_g, auto layer_of, auto lev_out, Real* flux_g, Real* grp_out, bool up) {
+        fetch(0, lev_g, layer_of);
+        for (int64_t s0 = 0; s0 <= nlay; s0 += K)
+        {
+            Real ct[K], cl[K], cx[K];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) { ct[kk] = bt[kk]; cl[kk] = bl[kk]; cx[kk] = bx[kk]; }
+            if (s0 + K <= nlay) fetch(s0 + K, lev_g, layer_of);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+            {
+                const int64_t j = s0 + kk;
+                Real f = Real(0), fj = Real(0);
+                if (active && j <= nlay)
+                {
+                    if (j > 0)
+                    {
+#pragma unroll
+                        for (int mu = 0; mu < NMU; ++mu)
+                            if (mu < nmus)
+                            {
+                                Real T, fact;
+                                lw_layer(ct[kk] * D[mu], T, fact);
+                                I[mu] = T * I[mu] + ((Real(1) - T) * cx[kk] + Real(2) * fact * (cl[kk] - cx[kk]));
+                                if (JAC && up) Ij[mu] = T * Ij[mu];
+                            }
+                    }
+#pragma unroll
+                    for (int mu = 0; mu < NMU; ++mu)
+                        if (mu < nmus)
+                        {
+                            f = (mu == 0) ? fac[0] * I[0] : f + fac[mu] * I[mu];
+                            if (JAC && up) fj = (mu == 0) ? fac[0] * Ij[0] : fj + fac[mu] * Ij[mu];
+                        }
+                    if (flux_g) at32(flux_g, ucol + uncol * uint32_t(lev_out(j))) = f;
+                }
+                if (reduce)
+                {
+                    red[(kk * NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
+                    if (JAC && up) red[(kk * NC + 1) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = fj;
+                }
+            }
+            if (reduce)
+            {
+                __syncthreads();
+                const int nsl = int(std::min<int64_t>(K, nlay + 1 - s0));
+                auto lev_of = [&](int kk) -> int64_t { return lev_out(s0 + kk); };
+                if (JAC && up)
+                {
+                    Real* const outs[2] = {grp_out, grp_jac};
+                    write_group_sums<Real, 2>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+                }
+                else
+                {
+                    // NC == 2 keeps the component stride of the Jacobian layout.
+                    Real* const outs[1] = {grp_out};
+                    write_group_sums_stride<Real, NC>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+                }
+                __syncthreads();
+            }
+        }
+    };
+
+    // Down sweep: level j counted from the top (0 = top boundary).
+    sweep(lev_dn_g,
+          [&](int64_t j) -> int64_t { return top_at_1 ? j - 1 : nlay - j; },
+          [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - j; }, flux_dn_g, grp_dn, false);
+
+    // Surface: reflection of the downward intensity plus emission; then the up sweep with
+    // level j counted from the surface (0 = surface).
+    {
+        const Real e = active ? emis[c2] : Real(0);
+        const Real ss = active ? sfc_src[c2] : Real(0);
+        const Real sj = (active && JAC) ? sfc_src_jac[c2] : Real(0);
+#pragma unroll
+        for (int mu = 0; mu < NMU; ++mu)
+        {
+            I[mu] = I[mu] * (Real(1) - e) + e * ss;
+            Ij[mu] = e * sj;
+        }
+    }
+    sweep(lev_up_g,
+          [&](int64_t j) -> int64_t { return top_at_1 ? nlay - j : j - 1; },
+          [&](int64_t j) -> int64_t { return top_at_1 ? nlay - j : j; }, flux_up_g, grp_up, true);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shortwave two-stream layer properties in FP64 (sw_two_stream_t of two_stream.h).
+__device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, double mu0, double& Rdif,
+                                              double& Tdif, double& Rdir, double& Tdir, double& Tnoscat)
+{
+    sw_two_stream_t<double>(tau, w0, g, mu0, Rdif, Tdif, Rdir, Tdir, Tnoscat);
+}
+
+// One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
+struct LayerSW { double R, T, SU, SD, Tn; };
 
 // 1/x in FP64: v_rcp_f64 and two Newton steps (for the adding denominators, x in (0, 1]).
 __device__ __forceinline__ double rcp_f64(double x)
@@ -654,6 +673,8 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("lw_solver: empty extents");
     if (nmus < 1 || nmus > 4) return arg_error("lw_solver: n_gauss_angles must be 1..4");
+    if (ncol * (nlay + 1) * int64_t(sizeof(Real)) >= (int64_t(1) << 32))
+        return arg_error("lw_solver: ncol * (nlay + 1) values of one g-point must stay below 4 GiB");
     if (!secants || !weights || !tau || !lay || !linc || !ldec || !emis || !sfc_src)
         return arg_error("lw_solver: NULL input");
     const bool spectral = flux_up || flux_dn;
@@ -682,14 +703,15 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
     int rc = RRTMGP_OK;
     {
         RRTMGP_TIMED("lw_solver_kernel", s);
+        auto launch = [&](auto kernel, const Real* jac_src, Real* jac_out) {
+            hipLaunchKernelGGL(kernel, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, int(nmus), secants,
+                               weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux, flux_up, flux_dn,
+                               jac_src, grp, g_up, g_dn, jac_out);
+        };
         if (do_jac)
-            hipLaunchKernelGGL((lw_solver_kernel<Real, true>), grid, block, 0, s, ncol, nlay, ngpt, top_at_1,
-                               int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux,
-                               flux_up, flux_dn, sfc_src_jac, grp, g_up, g_dn, g_jac);
+            launch(nmus == 1 ? lw_solver_kernel<Real, true, 1> : lw_solver_kernel<Real, true, 4>, sfc_src_jac, g_jac);
         else
-            hipLaunchKernelGGL((lw_solver_kernel<Real, false>), grid, block, 0, s, ncol, nlay, ngpt, top_at_1,
-                               int(nmus), secants, weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux,
-                               flux_up, flux_dn, nullptr, grp, g_up, g_dn, nullptr);
+            launch(nmus == 1 ? lw_solver_kernel<Real, false, 1> : lw_solver_kernel<Real, false, 4>, nullptr, nullptr);
         rc = check_hip(hipGetLastError(), "lw_solver_kernel");
     }
     if (rc == RRTMGP_OK && !spectral)

@@ -1,0 +1,157 @@
+"""GPU adjoints against central finite differences of the FP64 CPU oracle.
+
+The solver adjoints return the gradient of L = <a_up, broadband_up> + <a_dn, broadband_dn> for
+random cotangents a_up, a_dn; each check compares <gradient, v> with the central difference
+(L(x + h v) - L(x - h v)) / 2h of the oracle's FP64 forward along random directions v.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+# Gaussian secants and weights of Rte_lw::rte_lw (src/Rte_lw.cpp:100-113).
+GAUSS = {1: ([1.66], [0.5]), 2: ([1.18350343, 2.81649655], [0.3180413817, 0.1819586183])}
+
+
+@pytest.fixture(scope="module")
+def G():
+    from rrtmgp_amd import api, device
+    if device.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return api
+
+
+def _host(d):
+    return d.to_host().astype(np.float64)
+
+
+def _lw_case(O, okd_lw, syn, ncol, nlay, top_at_1, seed):
+    atm = syn.make_atmosphere(seed, ncol, nlay, top_at_1=top_at_1)
+    go = O.gas_optics(okd_lw, atm)
+    bl = np.asarray(okd_lw.band_lims)
+    band_of = np.concatenate([np.full(bl[1, b] - bl[0, b] + 1, b) for b in range(bl.shape[1])])
+    return atm, go, bl, band_of
+
+
+def _lw_adjoint(G, go, emis_gpt, a_up, a_dn, top_at_1, nmu, prec):
+    from rrtmgp_amd import _lib
+    dt = np.float64 if prec == 8 else np.float32
+    ncol, nlay, ngpt = go["tau"].shape
+    ds, ws = GAUSS[nmu]
+    sec = np.empty((ncol, ngpt, nmu))
+    for mu in range(nmu):
+        sec[:, :, mu] = ds[mu]
+    dev = lambda a: G.DeviceArray.from_host(np.asarray(a, dt))
+    names = ("tau", "lay_src", "lev_src_inc", "lev_src_dec")
+    d = {n: dev(go[n]) for n in names}
+    sfc, emis, au, ad, dsec, dw = (dev(a) for a in (go["sfc_src"], emis_gpt, a_up, a_dn, sec, ws))
+    out = {n: G.DeviceArray((ncol, nlay, ngpt), dt) for n in names}
+    out["sfc_src"], out["emis"] = G.DeviceArray((ncol, ngpt), dt), G.DeviceArray((ncol, ngpt), dt)
+    fn = getattr(_lib.hip(), "rrtmgp_hip_lw_solver_noscat_adjoint_" + ("f64" if prec == 8 else "f32"))
+    _lib.check_hip(fn(C.c_int64(ncol), C.c_int64(nlay), C.c_int64(ngpt), C.c_int(int(top_at_1)), C.c_int64(nmu),
+                      dsec.ptr, dw.ptr, d["tau"].ptr, d["lay_src"].ptr, d["lev_src_inc"].ptr, d["lev_src_dec"].ptr,
+                      emis.ptr, sfc.ptr, None, au.ptr, ad.ptr, out["tau"].ptr, out["lay_src"].ptr,
+                      out["lev_src_inc"].ptr, out["lev_src_dec"].ptr, out["sfc_src"].ptr, out["emis"].ptr, None))
+    return {k: _host(v) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+@pytest.mark.parametrize("nmu", [1, 2])
+def test_lw_solver_adjoint_fd(G, O, okd_lw, syn, top_at_1, nmu):
+    atm, go, bl, band_of = _lw_case(O, okd_lw, syn, 6, 24, top_at_1, 41)
+    ncol, nlay, ngpt = go["tau"].shape
+    rng = np.random.default_rng(3)
+    a_up, a_dn = rng.normal(size=(ncol, nlay + 1)), rng.normal(size=(ncol, nlay + 1))
+    emis_b = np.asarray(atm.emis_sfc, np.float64) * rng.uniform(0.9, 1.0, np.shape(atm.emis_sfc))
+    emis_gpt = emis_b[band_of, :].T
+    g = _lw_adjoint(G, go, emis_gpt, a_up, a_dn, top_at_1, nmu, 8)
+
+    def loss(x):
+        up, dn = O.rte_lw(x["tau"], x["lay_src"], x["lev_src_inc"], x["lev_src_dec"], x["sfc_src"], x["emis"],
+                          bl, top_at_1, n_gauss_angles=nmu, do_broadband=True)
+        return float(np.sum(a_up * up) + np.sum(a_dn * dn))
+
+    base = {k: go[k] for k in ("tau", "lay_src", "lev_src_inc", "lev_src_dec", "sfc_src")}
+    base["emis"] = emis_b
+    for trial in range(3):
+        v = {k: rng.normal(size=np.shape(a)) * np.abs(a) for k, a in base.items()}
+        v["emis"] = rng.normal(size=emis_b.shape) * 0.01
+        h = 1e-5
+        plus = {k: base[k] + h * v[k] for k in base}
+        minus = {k: base[k] - h * v[k] for k in base}
+        fd = (loss(plus) - loss(minus)) / (2 * h)
+        ad = sum(np.sum(g[k] * v[k]) for k in ("tau", "lay_src", "lev_src_inc", "lev_src_dec", "sfc_src"))
+        ad += np.sum(g["emis"] * v["emis"][band_of, :].T)
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (trial, ad, fd)
+
+
+def test_lw_solver_adjoint_fp32_vs_fp64(G, O, okd_lw, syn):
+    atm, go, bl, band_of = _lw_case(O, okd_lw, syn, 64, 60, True, 42)
+    ncol, nlay, ngpt = go["tau"].shape
+    rng = np.random.default_rng(5)
+    a_up, a_dn = rng.normal(size=(ncol, nlay + 1)), rng.normal(size=(ncol, nlay + 1))
+    emis_gpt = np.asarray(atm.emis_sfc)[band_of, :].T
+    g8 = _lw_adjoint(G, go, emis_gpt, a_up, a_dn, True, 1, 8)
+    g4 = _lw_adjoint(G, go, emis_gpt, a_up, a_dn, True, 1, 4)
+    for k in g8:
+        scale = np.abs(g8[k]).max()
+        assert np.abs(g4[k] - g8[k]).max() <= 1e-4 * scale, k
+
+
+def _sw_inputs(O, okd_sw, lut_sw, syn, ncol, nlay, top_at_1, seed):
+    atm = syn.make_atmosphere(seed, ncol, nlay, cloudy=True, top_at_1=top_at_1)
+    go = O.gas_optics(okd_sw, atm)
+    ct, cs, cg = O.delta_scale(*O.Cloud(lut_sw).optics(atm.lwp, atm.iwp, atm.rel, atm.rei, True))
+    tau, ssa, g = O.inc_bybnd(go["tau"], ct, okd_sw.band_lims, go["ssa"], go["g"], cs, cg)
+    bl = np.asarray(okd_sw.band_lims)
+    band_of = np.concatenate([np.full(bl[1, b] - bl[0, b] + 1, b) for b in range(bl.shape[1])])
+    return atm, go, tau, ssa, g, bl, band_of
+
+
+def _sw_adjoint(G, tau, ssa, g, mu0, inc, ad_gpt, af_gpt, a_up, a_dn, a_dir, top_at_1, prec):
+    from rrtmgp_amd import _lib
+    dt = np.float64 if prec == 8 else np.float32
+    ncol, nlay, ngpt = tau.shape
+    dev = lambda a: G.DeviceArray.from_host(np.asarray(a, dt))
+    ins = [dev(a) for a in (tau, ssa, g, mu0, ad_gpt, af_gpt, inc)]
+    cots = [dev(a) for a in (a_up, a_dn, a_dir)]
+    out = [G.DeviceArray((ncol, nlay, ngpt), dt) for _ in range(3)] + [G.DeviceArray((ncol, ngpt), dt) for _ in range(2)]
+    fn = getattr(_lib.hip(), "rrtmgp_hip_sw_solver_2stream_adjoint_" + ("f64" if prec == 8 else "f32"))
+    _lib.check_hip(fn(C.c_int64(ncol), C.c_int64(nlay), C.c_int64(ngpt), C.c_int(int(top_at_1)),
+                      *[a.ptr for a in ins[:6]], ins[6].ptr, None, *[a.ptr for a in cots],
+                      *[a.ptr for a in out], None))
+    return dict(zip(("tau", "ssa", "g", "alb_dir", "alb_dif"), (_host(a) for a in out)))
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_sw_solver_adjoint_fd(G, O, okd_sw, lut_sw, syn, top_at_1):
+    atm, go, tau, ssa, g, bl, band_of = _sw_inputs(O, okd_sw, lut_sw, syn, 6, 24, top_at_1, 43)
+    ncol, nlay, ngpt = tau.shape
+    rng = np.random.default_rng(9)
+    a_up, a_dn, a_dir = (rng.normal(size=(ncol, nlay + 1)) for _ in range(3))
+    alb_dir = np.asarray(atm.sfc_alb, np.float64)
+    alb_dif = 0.5 * alb_dir
+    grad = _sw_adjoint(G, tau, ssa, g, atm.mu0, go["toa_src"], alb_dir[band_of, :].T, alb_dif[band_of, :].T,
+                       a_up, a_dn, a_dir, top_at_1, 8)
+
+    def loss(x):
+        up, dn, dr = O.rte_sw(x["tau"], x["ssa"], x["g"], atm.mu0, go["toa_src"], x["alb_dir"], x["alb_dif"],
+                              bl, top_at_1, do_broadband=True)
+        return float(np.sum(a_up * up) + np.sum(a_dn * dn) + np.sum(a_dir * dr))
+
+    base = {"tau": tau, "ssa": ssa, "g": g, "alb_dir": alb_dir, "alb_dif": alb_dif}
+    for trial in range(3):
+        v = {"tau": rng.normal(size=tau.shape) * tau,
+             "ssa": rng.normal(size=ssa.shape) * np.minimum(ssa, 1 - ssa),
+             "g": rng.normal(size=g.shape) * np.minimum(np.abs(g), 1 - np.abs(g)),
+             "alb_dir": rng.normal(size=alb_dir.shape) * 0.01,
+             "alb_dif": rng.normal(size=alb_dif.shape) * 0.01}
+        h = 1e-5
+        fd = (loss({k: base[k] + h * v[k] for k in base}) - loss({k: base[k] - h * v[k] for k in base})) / (2 * h)
+        ad = sum(np.sum(grad[k] * v[k]) for k in ("tau", "ssa", "g"))
+        ad += np.sum(grad["alb_dir"] * v["alb_dir"][band_of, :].T) + np.sum(grad["alb_dif"] * v["alb_dif"][band_of, :].T)
+        assert abs(ad - fd) <= 1e-6 * abs(fd), (trial, ad, fd)

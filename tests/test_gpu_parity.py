@@ -189,6 +189,44 @@ def test_rte_lw_fp64(G, gkd, okd_lw, syn, O, top_at_1, nmu):
     assert rel_err(_host(bup), rup.sum(2)) < FP64_RTOL and rel_err(_host(bdn), rdn.sum(2), floor=1e-6) < FP64_RTOL
 
 
+# Gaussian secants and weights of Rte_lw::rte_lw (src/Rte_lw.cpp:100-113), 1 and 3 angles.
+_GAUSS = {1: ([1.66], [0.5]), 3: ([1.09719858, 1.69338507, 4.70941630], [0.2009319137, 0.2292411064, 0.0698269799])}
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+@pytest.mark.parametrize("nmu", [1, 3])
+def test_lw_solver_jacobian_fp64(G, okd_lw, syn, O, top_at_1, nmu):
+    """The thin C ABI's LW solver with broadband output and the surface-temperature Jacobian."""
+    from rrtmgp_amd import _lib
+    atm = syn.make_atmosphere(34, 40, 60, top_at_1=top_at_1)
+    go = O.gas_optics(okd_lw, atm)
+    ncol, nlay, ngpt = go["tau"].shape
+    rng = np.random.default_rng(7)
+    jac_src = rng.uniform(0.01, 0.1, (ncol, ngpt))
+    bl = np.asarray(okd_lw.band_lims)
+    band_of = np.concatenate([np.full(bl[1, b] - bl[0, b] + 1, b) for b in range(bl.shape[1])])
+    emis_gpt = np.asarray(atm.emis_sfc)[band_of, :].T
+    ds, ws = _GAUSS[nmu]
+    sec = np.empty((ncol, ngpt, nmu))
+    for mu in range(nmu):
+        sec[:, :, mu] = ds[mu]
+    dev = lambda a: G.DeviceArray.from_host(np.asarray(a, np.float64))
+    d = {n: dev(go[n]) for n in ("tau", "lay_src", "lev_src_inc", "lev_src_dec", "sfc_src")}
+    d_sec, d_w, d_emis, d_jsrc = dev(sec), dev(ws), dev(emis_gpt), dev(jac_src)
+    out = {n: G.DeviceArray((ncol, nlay + 1), np.float64) for n in ("up", "dn", "jac")}
+    _lib.check_hip(_lib.hip().rrtmgp_hip_lw_solver_noscat_f64(
+        C.c_int64(ncol), C.c_int64(nlay), C.c_int64(ngpt), C.c_int(int(top_at_1)), C.c_int64(nmu),
+        d_sec.ptr, d_w.ptr, d["tau"].ptr, d["lay_src"].ptr, d["lev_src_inc"].ptr, d["lev_src_dec"].ptr,
+        d_emis.ptr, d["sfc_src"].ptr, None, None, None, C.c_int(1), out["up"].ptr, out["dn"].ptr,
+        C.c_int(1), d_jsrc.ptr, out["jac"].ptr, None))
+    rup, rdn, rjac = O.rte_lw(go["tau"], go["lay_src"], go["lev_src_inc"], go["lev_src_dec"], go["sfc_src"],
+                              atm.emis_sfc, okd_lw.band_lims, top_at_1, n_gauss_angles=nmu,
+                              do_broadband=True, sfc_src_jac=jac_src)
+    assert rel_err(_host(out["up"]), rup) < FP64_RTOL
+    assert rel_err(_host(out["dn"]), rdn, floor=1e-6) < FP64_RTOL
+    assert rel_err(_host(out["jac"]), rjac) < FP64_RTOL
+
+
 @pytest.mark.parametrize("top_at_1", [False, True])
 def test_rte_sw_fp64(G, gkd, okd_sw, syn, O, lut_sw, top_at_1):
     atm = syn.make_atmosphere(32, 40, 60, cloudy=True, top_at_1=top_at_1)
