@@ -198,6 +198,31 @@ int rrtmgp_api_solve_sw(void* solver, int ncol, int nlay, int cloudy, int bnd_fl
                         rrtmgp_real* bnd_up, rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir,
                         rrtmgp_real* bnd_net);
 
+/* Adjoints of the two drivers above (no reference counterpart): for cotangents g_flux_* of   */
+/* the broadband fluxes (ncol, nlev), the gradients of L = sum(g_flux * flux) with respect to */
+/* tlay, tlev, tsfc (LW), ln play, plev (through col_dry, which is computed from h2o), the    */
+/* vmr of every k-distribution gas (d_vmr (ncol, nlay, ngas), k-distribution gas order) and   */
+/* the surface emissivity / albedos ((nbnd, ncol)).  Clouds (cloudy != 0) are held fixed.     */
+/* Gas optics is recomputed band by band.  All arrays are device arrays; outputs overwritten. */
+int rrtmgp_api_adjoint_lw(void* solver, int ncol, int nlay, int cloudy, const rrtmgp_real* play,
+                          const rrtmgp_real* plev, const rrtmgp_real* tlay, const rrtmgp_real* tlev,
+                          const rrtmgp_real* tsfc, void* gas_concs, const rrtmgp_real* emis_sfc,
+                          const rrtmgp_real* lwp, const rrtmgp_real* iwp, const rrtmgp_real* rel,
+                          const rrtmgp_real* rei, const rrtmgp_real* g_flux_up,
+                          const rrtmgp_real* g_flux_dn, rrtmgp_real* d_tlay, rrtmgp_real* d_tlev,
+                          rrtmgp_real* d_tsfc, rrtmgp_real* d_lnp_lay, rrtmgp_real* d_plev,
+                          rrtmgp_real* d_vmr, rrtmgp_real* d_emis_sfc);
+int rrtmgp_api_adjoint_sw(void* solver, int ncol, int nlay, int cloudy, int delta_cloud,
+                          const rrtmgp_real* play, const rrtmgp_real* plev, const rrtmgp_real* tlay,
+                          void* gas_concs, const rrtmgp_real* sfc_alb_dir,
+                          const rrtmgp_real* sfc_alb_dif, const rrtmgp_real* tsi_scaling,
+                          const rrtmgp_real* mu0, const rrtmgp_real* lwp, const rrtmgp_real* iwp,
+                          const rrtmgp_real* rel, const rrtmgp_real* rei,
+                          const rrtmgp_real* g_flux_up, const rrtmgp_real* g_flux_dn,
+                          const rrtmgp_real* g_flux_dir, rrtmgp_real* d_tlay,
+                          rrtmgp_real* d_lnp_lay, rrtmgp_real* d_plev, rrtmgp_real* d_vmr,
+                          rrtmgp_real* d_sfc_alb_dir, rrtmgp_real* d_sfc_alb_dif);
+
 #ifdef __cplusplus
 }
 #endif

@@ -183,6 +183,23 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_LW)
         T* tau, T* ssa, T* g, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_SW)
 
+/* The fused entries above restricted to bands band_start .. band_start+band_count-1 (0-based):*/
+/* the output g-point dimension is that range's g-points only (ngpt_range, counted from the    */
+/* first g-point of band_start); cloud arrays stay (ncol, nlay, nbnd).  No counterpart in the  */
+/* reference (whose gas optics always covers every band); used by the adjoint driver to        */
+/* recompute gas optics band by band.                                                           */
+#define RRTMGP_DECL_GAS_OPTICS_BANDS(S, T)                                                     \
+    int rrtmgp_hip_gas_optics_lw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,     \
+        int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay,          \
+        const T* tlev, const T* tsfc, const T* col_dry, const rrtmgp_gas_vmr* gases,           \
+        int sfc_lay, const T* cloud_tau, T* tau, T* lay_src, T* lev_src_inc, T* lev_src_dec,   \
+        T* sfc_src, T* sfc_src_jac, void* stream);                                              \
+    int rrtmgp_hip_gas_optics_sw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,     \
+        int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay,          \
+        const T* col_dry, const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_ssa,  \
+        const T* cloud_g, T* tau, T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_BANDS)
+
 /* Gas_optics_rrtmgp_kernels_cuda::interpolation (include_kernels_cuda/                       */
 /* gas_optics_rrtmgp_kernels_cuda.h:42-61; Fortran rrtmgp_interpolation, rrtmgp_kernels.h:60-79)*/
 /* Materializes jtemp, jpress, tropo (ncol,nlay), jeta (2,ncol,nlay,nflav), col_mix (2,...),  */
@@ -376,6 +393,47 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_ADJOINT)
         const T* inc_flux_dif, const T* bb_up_cot, const T* bb_dn_cot, const T* bb_dir_cot,    \
         T* d_tau, T* d_ssa, T* d_g, T* d_sfc_alb_dir, T* d_sfc_alb_dif, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_ADJOINT)
+
+/* Reverse mode of the fused gas optics (rrtmgp_hip_gas_optics_{lw,sw}_bands_* for one band):  */
+/* given the cotangents of the band's outputs ((ncol, nlay, ngb) with ngb = the band's         */
+/* g-points; g_sfc_src (ncol, ngb)), ADDS the gradients with respect to the column amounts     */
+/* d_col_gas (ncol, nlay, ngas+1; gas 0 = dry air, i.e. col_dry), T_lay and ln p_lay           */
+/* (ncol, nlay), and in the longwave T_lev at each layer's level ilay (d_tlev_lo) and          */
+/* ilay+1 (d_tlev_hi) (ncol, nlay) and T_sfc (ncol).  The SW cotangents are those of the       */
+/* outputs after the optional band cloud add_to.  Call band by band on one stream.             */
+/* get_col_dry_adjoint: reverse mode of rrtmgp_hip_get_col_dry_* (vmr_h2o with its broadcast  */
+/* dims); ADDS into d_vmr_h2o (ncol, nlay, full) and d_plev (ncol, nlev).                       */
+#define RRTMGP_DECL_GO_ADJOINT(S, T)                                                           \
+    int rrtmgp_hip_gas_optics_lw_adjoint_##S(const rrtmgp_kdist_dev* kd, int64_t band,         \
+        int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* tlev, const T* tsfc, \
+        const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* g_tau,           \
+        const T* g_lay_src, const T* g_lev_src_inc, const T* g_lev_src_dec,                    \
+        const T* g_sfc_src, T* d_col_gas, T* d_tlay, T* d_lnp_lay, T* d_tlev_lo,               \
+        T* d_tlev_hi, T* d_tsfc, void* stream);                                                 \
+    int rrtmgp_hip_gas_optics_sw_adjoint_##S(const rrtmgp_kdist_dev* kd, int64_t band,         \
+        int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* col_dry,            \
+        const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_ssa,                   \
+        const T* cloud_g, const T* g_tau, const T* g_ssa, const T* g_g, T* d_col_gas,          \
+        T* d_tlay, T* d_lnp_lay, void* stream);                                                 \
+    int rrtmgp_hip_get_col_dry_adjoint_##S(int64_t ncol, int64_t nlay, const T* vmr_h2o,      \
+        int64_t h2o_dim1, int64_t h2o_dim2, const T* plev, const T* g_col_dry, T* d_vmr_h2o,   \
+        T* d_plev, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GO_ADJOINT)
+
+/* Assembly steps of the adjoint driver.  col_chain: from d_col_gas (ncol, nlay, ngas+1) to    */
+/* d_vmr (ncol, nlay, ngas, overwritten; gases in the descriptor's order) and d_col_dry        */
+/* (ncol, nlay, overwritten).  levels: d_tlev (ncol, nlev) from the per-layer contributions    */
+/* of rrtmgp_hip_gas_optics_lw_adjoint_*.  gpt_to_band: band sums of (ncol, ngpt) values into  */
+/* (nbnd, ncol) (band_lims_gpt device, 1-based).                                               */
+#define RRTMGP_DECL_ADJ_HELPERS(S, T)                                                          \
+    int rrtmgp_hip_adjoint_col_chain_##S(int64_t ncol, int64_t nlay,                           \
+        const rrtmgp_gas_vmr* gases, const T* col_dry, const T* d_col_gas, T* d_vmr,           \
+        T* d_col_dry, void* stream);                                                            \
+    int rrtmgp_hip_adjoint_levels_##S(int64_t ncol, int64_t nlay, const T* d_lo, const T* d_hi,\
+        T* d_tlev, void* stream);                                                               \
+    int rrtmgp_hip_gpt_to_band_##S(int64_t ncol, int64_t nbnd, const int* band_lims_gpt,      \
+        const T* in, T* out, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_ADJ_HELPERS)
 
 /* ------------------------------------------------------------------------------------------ */
 /* Flux reductions (Fluxes_kernels_cuda, include_kernels_cuda/fluxes_kernels_cuda.h:31-50;    */

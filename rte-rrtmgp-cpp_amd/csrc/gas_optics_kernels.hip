@@ -295,18 +295,22 @@ __device__ __forceinline__ Real minor_scaling_of(const ColOf& col, int gas, bool
 
 // Grid of the fused kernels: units (chunk, layer, 64-column tile), chunk = (band, 16-g-point
 // part of the band), ordered so that consecutive units share a chunk.
+// A launch covers bands b0 .. b0+nbnd-1; its outputs hold their g-points only, counted from the
+// first g-point of band b0.
 struct GoGrid
 {
     int64_t ncoltiles, nlay, cpb, nunits;
+    int b0;
 };
 
-inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t nbnd, int64_t max_band_gpts)
+inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t b0, int64_t nbnd, int64_t max_band_gpts)
 {
     GoGrid t;
     t.ncoltiles = (ncol + GO_COLS - 1) / GO_COLS;
     t.nlay = nlay;
     t.cpb = std::max<int64_t>(1, (max_band_gpts + GO_GPTS - 1) / GO_GPTS);
     t.nunits = t.ncoltiles * nlay * nbnd * t.cpb;
+    t.b0 = int(b0);
     return t;
 }
 
@@ -326,8 +330,9 @@ __device__ __forceinline__ GoUnit go_unit(const GoBand* bands, const GoGrid& t, 
     r.col0 = int64_t(u - rest * nct) * GO_COLS;
     const uint32_t chunk = rest / nl;
     r.ilay = rest - chunk * nl;
-    r.ibnd = int(chunk / cpb);
-    r.c0 = GO_GPTS * int(chunk - uint32_t(r.ibnd) * cpb);
+    const int cb = int(chunk / cpb);
+    r.c0 = GO_GPTS * int(chunk - uint32_t(cb) * cpb);
+    r.ibnd = t.b0 + cb;
     r.ncg = min(bands[r.ibnd].ng - r.c0, GO_GPTS);
     return r;
 }
@@ -614,7 +619,7 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const PairState<Real>& st = sst[lane];
     const int64_t idx = icol + ncol * ilay;
     const Real ctau = cloud_tau ? cloud_tau[idx + ncl * un.ibnd] : Real(0);
-    const int gbase = bd.g0 + un.c0;
+    const int gbase = bd.g0 - k.bands[grid.b0].g0 + un.c0;
     for (int row = w; row < un.ncg; row += GO_THREADS / 64)
     {
         const int gr = gbase + row;
@@ -701,7 +706,7 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         const int64_t ib = idx + ncl * un.ibnd;
         t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
     }
-    const int gbase = bd.g0 + un.c0;
+    const int gbase = bd.g0 - k.bands[grid.b0].g0 + un.c0;
     for (int row = w; row < un.ncg; row += GO_THREADS / 64)
     {
         const Real tray = s_ray[row][lane];
@@ -723,6 +728,377 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         tau[o] = tt_out;
         ssa[o] = ss;
         gout[o] = gg_out;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Gas-optics adjoint.  For one band, given the cotangents of the band's fused outputs, adds the
+// gradients with respect to the per-(column, layer) inputs: the column amounts col_gas
+// (gas 0 = dry air), T_lay, ln p_lay and, in the longwave, T_lev at the layer's two levels and
+// T_sfc.  Per g-point the outputs are linear in the table values with the pair's weights:
+//   tau_abs = sum_i col_mix_i sum_c fmajor_ic K_icg + sum_q scaling_q sum_ie fminor_ie Km_qieg
+//   pfrac   = sum_ic fmajor_ic P_icg,    tau_ray = col_ray sum_ie fminor_ie Kr_ieg,
+// so phase 2 accumulates, per pair, the gradients of those weights over the band's g-points
+// (one quad of g-points per lane, reduced over the pair's four lanes); phase 3 carries them
+// through the interpolation (ftemp, fpress, eta -> fmajor/fminor, col_mix), the minor scalings and
+// the Planck interpolation by hand.  The integer indices are piecewise constant.
+
+constexpr int GA_CM = 0, GA_FM = 2, GA_SC = 10, GA_FN = 18, GA_X = 22, GA_N = 26;
+
+// Derivative of interpolate1D with respect to val.
+template<typename Real>
+__device__ __forceinline__ Real interpolate1D_slope(Real val, Real offset, Real delta, int len, const Real* table)
+{
+    const Real val0 = (val - offset) / delta;
+    const int idx = min(len - 1, max(1, int(val0) + 1));
+    return (table[idx] - table[idx - 1]) / delta;
+}
+
+// Phase 3: the gradients of one pair's weights (acc) carried to the inputs.
+template<typename Real, bool LW>
+__device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const GasRef<Real>* sg, const GoBand& bd,
+                                                    int ibnd, int64_t icol, int64_t ilay, int64_t ncol, int64_t ncl,
+                                                    const Real* play, const Real* tlay, const Real* tlev,
+                                                    const Real* tsfc, const Real* col_dry_a, bool at_sfc,
+                                                    const Real* acc, Real* d_col, Real* d_tlay, Real* d_lnp,
+                                                    Real* d_tlev_lo, Real* d_tlev_hi, Real* d_tsfc)
+{
+    const int64_t idx = icol + ncol * ilay;
+    const Real pv = play[idx], tv = tlay[idx], cd = col_dry_a[idx];
+    const CL<Real> c = interp_cl(k, pv, tv);
+    const int s = c.itropo;
+    const int gas1 = bd.gas[s][0], gas2 = bd.gas[s][1];
+    const Real c1 = col_lds(sg, gas1, icol, ilay, cd), c2 = col_lds(sg, gas2, icol, ilay, cd);
+    auto dcol = [&](int gas, Real v) { d_col[idx + ncl * gas] += v; };
+    Real g_ftemp = Real(0), g_fp = Real(0), g_c1 = Real(0), g_c2 = Real(0);
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+    {
+        // interp_flav (.cu:254-291) for itemp = i, with its intermediates.
+        const int64_t vb = c.itropo + int64_t(c.jtemp + i - 1) * (k.ngas + 1) * 2;
+        const Real r = k.vmr_ref[vb + 2 * gas1] / k.vmr_ref[vb + 2 * gas2];
+        const Real cm = c1 + r * c2;
+        const bool mix = cm > Real(2) * Limits<Real>::tiny;
+        const Real eta = mix ? c1 / cm : Real(0.5);
+        const Real feta = fmod(eta * Real(k.neta - 1), Real(1));
+        const Real ftt = i == 0 ? Real(1) - c.ftemp : c.ftemp;
+        const Real m0 = (Real(1) - feta) * ftt, m1 = feta * ftt;
+        const Real* gfm = acc + GA_FM + 4 * i;
+        const Real* gfn = acc + GA_FN + 2 * i;
+        const Real g_m0 = (Real(1) - c.fpress) * gfm[0] + c.fpress * gfm[2] + gfn[0];
+        const Real g_m1 = (Real(1) - c.fpress) * gfm[1] + c.fpress * gfm[3] + gfn[1];
+        g_fp += -m0 * gfm[0] - m1 * gfm[1] + m0 * gfm[2] + m1 * gfm[3];
+        const Real g_feta = ftt * (g_m1 - g_m0);
+        const Real g_ftt = (Real(1) - feta) * g_m0 + feta * g_m1;
+        g_ftemp += i == 0 ? -g_ftt : g_ftt;
+        const Real g_cm = acc[GA_CM + i];
+        g_c1 += g_cm;
+        g_c2 += g_cm * r;
+        if (mix)
+        {
+            const Real g_eta = g_feta * Real(k.neta - 1);
+            const Real icm2 = Real(1) / (cm * cm);
+            g_c1 += g_eta * (cm - c1) * icm2;
+            g_c2 -= g_eta * c1 * r * icm2;
+        }
+    }
+    dcol(gas1, g_c1);
+    dcol(gas2, g_c2);
+    Real g_t = g_ftemp / k.temp_ref_delta;
+    Real g_lnp = g_fp / k.press_ref_log_delta;
+    // Minor scalings (.cu:388-420): sc = col(gas) [P F], P = 0.01 p / T, F = X or 1 - X with
+    // X = col(scal) / (col_dry + col(h2o)).
+    const int nq = bd.nq[s];
+    for (int q = 0; q < nq; ++q)
+    {
+        const GoMinor md = bd.minor[s][q];
+        const Real g_sc = acc[GA_SC + q];
+        const Real base = col_lds(sg, md.gas, icol, ilay, cd);
+        if (!md.dens)
+        {
+            dcol(md.gas, g_sc);
+            continue;
+        }
+        const Real P = Real(0.01) * pv / tv;
+        Real F = Real(1), X = Real(0), den = Real(1);
+        if (md.scal_gas > 0)
+        {
+            den = cd + col_lds(sg, k.idx_h2o, icol, ilay, cd);
+            X = col_lds(sg, md.scal_gas, icol, ilay, cd) / den;
+            F = md.compl_ ? Real(1) - X : X;
+        }
+        dcol(md.gas, g_sc * P * F);
+        const Real g_P = g_sc * base * F;
+        g_lnp += g_P * P;
+        g_t -= g_P * P / tv;
+        if (md.scal_gas > 0)
+        {
+            const Real g_X = (md.compl_ ? -g_sc : g_sc) * base * P;
+            dcol(md.scal_gas, g_X / den);
+            dcol(0, -g_X * X / den);
+            dcol(k.idx_h2o, -g_X * X / den);
+        }
+    }
+    if constexpr (LW)
+    {
+        const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
+        g_t += acc[GA_X + 0] * interpolate1D_slope(tv, k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        d_tlev_hi[idx] += acc[GA_X + 1] * interpolate1D_slope(tlev[icol + ncol * (ilay + 1)], k.temp_ref_min,
+                                                              k.totplnk_delta, k.nplanck, tab);
+        d_tlev_lo[idx] += acc[GA_X + 2] * interpolate1D_slope(tlev[idx], k.temp_ref_min, k.totplnk_delta,
+                                                              k.nplanck, tab);
+        if (at_sfc)
+            d_tsfc[icol] += acc[GA_X + 3] * interpolate1D_slope(tsfc[icol], k.temp_ref_min, k.totplnk_delta,
+                                                                k.nplanck, tab);
+    }
+    else
+    {
+        dcol(k.idx_h2o, acc[GA_X]);      // col_ray = col(h2o) + col_dry
+        dcol(0, acc[GA_X]);
+    }
+    d_tlay[idx] += g_t;
+    d_lnp[idx] += g_lnp;
+}
+
+// Sum over the four lanes of a pair (lanes 4p .. 4p+3 of a wavefront).
+__device__ __forceinline__ float quad_sum(float v)
+{
+    v += __shfl_xor(v, 1);
+    return v + __shfl_xor(v, 2);
+}
+__device__ __forceinline__ double quad_sum(double v)
+{
+    v += __shfl_xor(v, 1);
+    return v + __shfl_xor(v, 2);
+}
+
+// One band; a work-group takes 64 columns of one layer and loops over the band's 16-g-point
+// chunks.  LW cotangents: ga (tau), gb (lay_src), gc (lev_src_inc), gd (lev_src_dec) of
+// (ncol, nlay, ngb) and gs (sfc_src, (ncol, ngb)); SW: ga (tau), gb (ssa), gc (g).
+template<typename Real, bool LW>
+__global__ __launch_bounds__(GO_THREADS)
+void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, int ibnd,
+                               const Real* __restrict__ play, const Real* __restrict__ tlay,
+                               const Real* __restrict__ tlev, const Real* __restrict__ tsfc,
+                               const Real* __restrict__ col_dry_a, int sfc_lay,
+                               const Real* __restrict__ ctau_a, const Real* __restrict__ cssa_a,
+                               const Real* __restrict__ cg_a,
+                               const Real* __restrict__ ga, const Real* __restrict__ gb,
+                               const Real* __restrict__ gc, const Real* __restrict__ gd,
+                               const Real* __restrict__ gs,
+                               Real* __restrict__ d_col, Real* __restrict__ d_tlay, Real* __restrict__ d_lnp,
+                               Real* __restrict__ d_tlev_lo, Real* __restrict__ d_tlev_hi, Real* __restrict__ d_tsfc)
+{
+    static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
+    __shared__ PairState<Real> sst[GO_COLS];
+    __shared__ Real s_g[5][GO_GPTS][GO_ROW];
+    __shared__ Real s_acc[GO_COLS][GA_N + 1];
+    __shared__ Real s_cl[3][GO_COLS];
+    __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
+    const int64_t ncoltiles = (ncol + GO_COLS - 1) / GO_COLS;
+    const int64_t u = xcd_remap(blockIdx.x, ncoltiles * nlay);
+    if (u >= ncoltiles * nlay) return;                     // uniform over the work-group
+    const int64_t ilay = u / ncoltiles, col0 = (u % ncoltiles) * GO_COLS;
+    const GoBand& bd = k.bands[ibnd];
+    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int64_t ncl = ncol * nlay;
+    const bool at_sfc = LW && (ilay == sfc_lay - 1);
+
+    if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
+    __syncthreads();
+    if (w == 0)
+    {
+        const int64_t icol = col0 + lane;
+        if (icol < ncol)
+        {
+            pair_state<Real, LW>(k, sg, bd, ibnd, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a, at_sfc,
+                                 sst[lane]);
+            if (!LW)
+            {
+                const int64_t ib = icol + ncol * ilay + ncl * ibnd;
+                s_cl[0][lane] = ctau_a ? ctau_a[ib] : Real(0);
+                s_cl[1][lane] = ctau_a ? cssa_a[ib] : Real(0);
+                s_cl[2][lane] = ctau_a ? cg_a[ib] : Real(0);
+            }
+        }
+        else
+            sst[lane].flags = 0;
+    }
+
+    const int pi = w * 16 + lane / 4;                  // pair of phase 2
+    const int gq = 4 * (lane % 4);                     // first g-point of the lane's quad in a chunk
+    Real acc[GA_N];
+#pragma unroll
+    for (int a = 0; a < GA_N; ++a) acc[a] = Real(0);
+    const Real eps2 = Real(2) * Limits<Real>::eps;
+    const Real op_eps = Real(3) * Limits<Real>::tiny;
+
+    for (int c0 = 0; c0 < bd.ng; c0 += GO_GPTS)
+    {
+        const int ncg = min(bd.ng - c0, GO_GPTS);
+        __syncthreads();                               // previous chunk's LDS reads are done
+        {
+            const int64_t icol = col0 + lane;
+            for (int row = w; row < ncg; row += GO_THREADS / 64)
+            {
+                Real v0 = Real(0), v1 = Real(0), v2 = Real(0), v3 = Real(0), v4 = Real(0);
+                if (icol < ncol)
+                {
+                    const int64_t o = icol + ncol * ilay + ncl * (c0 + row);
+                    v0 = ga[o]; v1 = gb[o]; v2 = gc[o];
+                    if (LW)
+                    {
+                        v3 = gd[o];
+                        if (at_sfc) v4 = gs[icol + ncol * (c0 + row)];
+                    }
+                }
+                s_g[0][row][lane] = v0; s_g[1][row][lane] = v1; s_g[2][row][lane] = v2;
+                s_g[3][row][lane] = v3; s_g[4][row][lane] = v4;
+            }
+        }
+        __syncthreads();
+        const PairState<Real>& st = sst[pi];
+        if (!((st.flags & 1) && gq < ncg)) continue;
+        const uint32_t gl = uint32_t(c0 + gq);
+        const uint32_t gp = bd.gp, sE = uint32_t(k.npres + 1) * gp, sP = gp;
+        const uint32_t kb = uint32_t(bd.koff) + gl;
+        Real K[2][4][4], P[2][4][4];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+        {
+            const uint32_t o = kb + uint32_t(st.om[i]);
+            load4(k.kmajor_p, o, K[i][0]); load4(k.kmajor_p, o + sE, K[i][1]);
+            load4(k.kmajor_p, o + sP, K[i][2]); load4(k.kmajor_p, o + sP + sE, K[i][3]);
+            if constexpr (LW)
+            {
+                load4(k.planck_frac_p, o, P[i][0]); load4(k.planck_frac_p, o + sE, P[i][1]);
+                load4(k.planck_frac_p, o + sP, P[i][2]); load4(k.planck_frac_p, o + sP + sE, P[i][3]);
+            }
+        }
+        // Forward values per g-point that the cotangents need.
+        Real gabs[4], gray[4];
+        Real kr[2][2][4];
+        if (!LW)
+        {
+            const uint32_t rb = uint32_t(bd.roff) + gl;
+            load4(k.krayl_p, rb + uint32_t(st.ry[0]), kr[0][0]); load4(k.krayl_p, rb + uint32_t(st.ry[0]) + gp, kr[0][1]);
+            load4(k.krayl_p, rb + uint32_t(st.ry[1]), kr[1][0]); load4(k.krayl_p, rb + uint32_t(st.ry[1]) + gp, kr[1][1]);
+            Real tabs[4];
+            absorption4(k, bd, st, gl, tabs);
+            const Real t2 = s_cl[0][pi], s2 = s_cl[1][pi], g2 = s_cl[2][pi];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+            {
+                const Real tray = (st.fminor[0][0] * kr[0][0][e] + st.fminor[0][1] * kr[0][1][e] +
+                                   st.fminor[1][0] * kr[1][0][e] + st.fminor[1][1] * kr[1][1][e]) * st.aux[0];
+                const Real tt = tabs[e] + tray;
+                const Real ss = (tt > eps2) ? tray / tt : Real(0);
+                const Real tau12 = tt + t2;
+                const Real tauscat = tt * ss + t2 * s2;
+                const Real gout = (t2 * s2 * g2) / max(tauscat, op_eps);
+                const Real ssa = tauscat / max(op_eps, tau12);
+                const Real gto = s_g[0][gq + e][pi], gwo = s_g[1][gq + e][pi], ggo = s_g[2][gq + e][pi];
+                const Real g_ts = (tau12 > op_eps ? gwo / tau12 : Real(0)) -
+                                  (tauscat > op_eps ? ggo * gout / tauscat : Real(0));
+                const Real g_t12 = gto - (tau12 > op_eps ? gwo * ssa / tau12 : Real(0));
+                gabs[e] = g_t12;
+                gray[e] = g_t12 + (tt > eps2 ? g_ts : Real(0));
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) { gabs[e] = s_g[0][gq + e][pi]; gray[e] = Real(0); }
+        }
+        // Major and Planck terms.
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+        {
+            Real gpf = Real(0);
+            if (LW)
+            {
+                Real pf = Real(0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) pf += st.fmajor[i][c] * P[i][c][e];
+                const Real gl_ = s_g[1][gq + e][pi], gi_ = s_g[2][gq + e][pi], gd_ = s_g[3][gq + e][pi];
+                const Real gs_ = at_sfc ? s_g[4][gq + e][pi] : Real(0);
+                gpf = gl_ * st.aux[0] + gi_ * st.aux[1] + gd_ * st.aux[2] + (at_sfc ? gs_ * st.aux[3] : Real(0));
+                acc[GA_X + 0] += gl_ * pf; acc[GA_X + 1] += gi_ * pf;
+                acc[GA_X + 2] += gd_ * pf; acc[GA_X + 3] += gs_ * pf;
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+            {
+                Real sk = Real(0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                {
+                    sk += st.fmajor[i][c] * K[i][c][e];
+                    Real gw = gabs[e] * st.col_mix[i] * K[i][c][e];
+                    if constexpr (LW) gw += gpf * P[i][c][e];
+                    acc[GA_FM + 4 * i + c] += gw;
+                }
+                acc[GA_CM + i] += gabs[e] * sk;
+            }
+            if (!LW)
+            {
+                Real sr = Real(0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        sr += st.fminor[i][c] * kr[i][c][e];
+                        acc[GA_FN + 2 * i + c] += gray[e] * st.aux[0] * kr[i][c][e];
+                    }
+                acc[GA_X] += gray[e] * sr;
+            }
+        }
+        // Minor terms.
+        const int nslot = bd.nslot;
+        const uint32_t sQ = uint32_t(nslot) * gp;
+        const uint32_t mb = uint32_t(bd.moff) + gl;
+#pragma unroll
+        for (int q = 0; q < GO_MAXM; ++q)
+        {
+            if (q >= nslot) break;
+            Real km[2][2][4];
+            const uint32_t o0 = mb + uint32_t(st.mn[0]) + uint32_t(q) * gp;
+            const uint32_t o1 = mb + uint32_t(st.mn[1]) + uint32_t(q) * gp;
+            load4(k.kminor_p, o0, km[0][0]); load4(k.kminor_p, o0 + sQ, km[0][1]);
+            load4(k.kminor_p, o1, km[1][0]); load4(k.kminor_p, o1 + sQ, km[1][1]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+            {
+                Real sm = Real(0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int c = 0; c < 2; ++c)
+                    {
+                        sm += st.fminor[i][c] * km[i][c][e];
+                        acc[GA_FN + 2 * i + c] += gabs[e] * st.scaling[q] * km[i][c][e];
+                    }
+                acc[GA_SC + q] += gabs[e] * sm;
+            }
+        }
+    }
+    // Sum over the pair's four lanes; the first lane of each pair stores the pair's totals.
+#pragma unroll
+    for (int a = 0; a < GA_N; ++a)
+    {
+        const Real v = quad_sum(acc[a]);
+        if (lane % 4 == 0) s_acc[pi][a] = v;
+    }
+    __syncthreads();
+    if (w == 0)
+    {
+        const int64_t icol = col0 + lane;
+        if (icol < ncol)
+            pair_state_backward<Real, LW>(k, sg, bd, ibnd, icol, ilay, ncol, ncl, play, tlay, tlev, tsfc, col_dry_a,
+                                          at_sfc, s_acc[lane], d_col, d_tlay, d_lnp, d_tlev_lo, d_tlev_hi, d_tsfc);
     }
 }
 
@@ -1159,7 +1535,7 @@ int kdist_pack(rrtmgp_kdist_dev* kd, void** pack_out)
 }
 
 template<typename Real>
-int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const Real* play,
+int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t ncol, int64_t nlay, const Real* play,
                   const Real* tlay, const Real* tlev, const Real* tsfc, const Real* col_dry,
                   const rrtmgp_gas_vmr* gases, int sfc_lay, const Real* cloud_tau, Real* tau,
                   Real* lay_src, Real* lev_inc, Real* lev_dec, Real* sfc_src, Real* sfc_jac,
@@ -1173,7 +1549,8 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     Gases<Real> g;
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
     const Kd<Real> k = make_kd<Real>(*kd);
-    const GoGrid grid = make_go_grid(ncol, nlay, kd->nbnd, kd->max_band_gpts);
+    if (b0 < 0 || nb < 1 || b0 + nb > kd->nbnd) return arg_error("gas_optics_lw: band range outside the k-distribution");
+    const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts);
     RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
@@ -1184,7 +1561,7 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
 }
 
 template<typename Real>
-int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const Real* play,
+int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t ncol, int64_t nlay, const Real* play,
                   const Real* tlay, const Real* col_dry, const rrtmgp_gas_vmr* gases,
                   const Real* ctau, const Real* cssa, const Real* cg, Real* tau, Real* ssa, Real* g,
                   void* stream)
@@ -1196,13 +1573,91 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, const 
     Gases<Real> gs;
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, gs));
     const Kd<Real> k = make_kd<Real>(*kd);
-    const GoGrid grid = make_go_grid(ncol, nlay, kd->nbnd, kd->max_band_gpts);
+    if (b0 < 0 || nb < 1 || b0 + nb > kd->nbnd) return arg_error("gas_optics_sw: band range outside the k-distribution");
+    const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts);
     RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
                        as_stream(stream), k, gs, ncol, nlay, grid, play, tlay, col_dry, ctau, cssa, cg,
                        tau, ssa, g);
     RRTMGP_LAUNCHED("gas_optics_sw_kernel");
+    return RRTMGP_OK;
+}
+
+template<typename Real>
+int gas_optics_adjoint(const rrtmgp_kdist_dev* kd, bool lw, int64_t band, int64_t ncol, int64_t nlay,
+                       const Real* play, const Real* tlay, const Real* tlev, const Real* tsfc, const Real* col_dry,
+                       const rrtmgp_gas_vmr* gases, int sfc_lay, const Real* ctau, const Real* cssa, const Real* cg,
+                       const Real* ga, const Real* gb, const Real* gc, const Real* gd, const Real* gs, Real* d_col,
+                       Real* d_tlay, Real* d_lnp, Real* d_tlev_lo, Real* d_tlev_hi, Real* d_tsfc, void* stream)
+{
+    RRTMGP_TRY(check_kd_fused(kd, lw));
+    if (ncol <= 0 || nlay <= 0) return arg_error("gas_optics_adjoint: ncol and nlay must be positive");
+    if (band < 0 || band >= kd->nbnd) return arg_error("gas_optics_adjoint: band outside the k-distribution");
+    if (!play || !tlay || !col_dry || !ga || !gb || !gc || !d_col || !d_tlay || !d_lnp)
+        return arg_error("gas_optics_adjoint: NULL array");
+    if (lw && (!tlev || !tsfc || !gd || !gs || !d_tlev_lo || !d_tlev_hi || !d_tsfc || (sfc_lay != 1 && sfc_lay != nlay)))
+        return arg_error("gas_optics_adjoint: longwave arrays missing or sfc_lay not 1 or nlay");
+    if (!lw && ctau && (!cssa || !cg)) return arg_error("gas_optics_adjoint: cloud ssa/g missing");
+    Gases<Real> g;
+    RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
+    const Kd<Real> k = make_kd<Real>(*kd);
+    const int64_t nunits = ((ncol + GO_COLS - 1) / GO_COLS) * nlay;
+    if (nunits >= (int64_t(1) << 31)) return arg_error("gas_optics_adjoint: too many column tiles x layers");
+    RRTMGP_TIMED("gas_optics_adjoint_kernel", as_stream(stream));
+    if (lw)
+        hipLaunchKernelGGL((gas_optics_adjoint_kernel<Real, true>), dim3(xcd_grid(nunits)), dim3(GO_THREADS), 0,
+                           as_stream(stream), k, g, ncol, nlay, int(band), play, tlay, tlev, tsfc, col_dry, sfc_lay,
+                           nullptr, nullptr, nullptr, ga, gb, gc, gd, gs, d_col, d_tlay, d_lnp, d_tlev_lo,
+                           d_tlev_hi, d_tsfc);
+    else
+        hipLaunchKernelGGL((gas_optics_adjoint_kernel<Real, false>), dim3(xcd_grid(nunits)), dim3(GO_THREADS), 0,
+                           as_stream(stream), k, g, ncol, nlay, int(band), play, tlay, nullptr, nullptr, col_dry, 1,
+                           ctau, cssa, cg, ga, gb, gc, nullptr, nullptr, d_col, d_tlay, d_lnp, nullptr, nullptr,
+                           nullptr);
+    RRTMGP_LAUNCHED("gas_optics_adjoint_kernel");
+    return RRTMGP_OK;
+}
+
+// Adjoint of get_col_dry (col_dry_kernel): col_dry = K |dp| / (m_dry + m_h2o v) exactly, so
+// d/dv = -col_dry m_h2o / (m_dry + m_h2o v) and d/dp_lev = +-K / (m_dry + m_h2o v).  Adds into
+// d_h2o (ncol, nlay) and d_plev (ncol, nlev), one thread per column.
+template<typename Real>
+__global__ void col_dry_adjoint_kernel(int64_t ncol, int64_t nlay, const Real* __restrict__ h2o, int64_t scol,
+                                       int64_t slay, const Real* __restrict__ plev,
+                                       const Real* __restrict__ g_col_dry, Real* __restrict__ d_h2o,
+                                       Real* __restrict__ d_plev)
+{
+    const int64_t icol = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (icol >= ncol) return;
+    const Real g0 = Real(9.80665), avogad = Real(6.02214076e23), m_dry = Real(0.028964), m_h2o = Real(0.018016);
+    const Real Kc = Real(10) * avogad / (Real(1000) * Real(100) * g0);
+    for (int64_t ilay = 0; ilay < nlay; ++ilay)
+    {
+        const int64_t i = icol + ncol * ilay;
+        const Real v = h2o[icol * scol + ilay * slay];
+        const Real m = m_dry + m_h2o * v;
+        const Real dp = plev[i] - plev[i + ncol];
+        const Real gc = g_col_dry[i];
+        d_h2o[i] += -gc * Kc * fabs(dp) * m_h2o / (m * m);
+        const Real sgn = dp >= Real(0) ? Real(1) : Real(-1);
+        d_plev[i] += gc * Kc * sgn / m;
+        d_plev[i + ncol] -= gc * Kc * sgn / m;
+    }
+}
+
+template<typename Real>
+int col_dry_adjoint(int64_t ncol, int64_t nlay, const Real* h2o, int64_t d1, int64_t d2, const Real* plev,
+                    const Real* g_col_dry, Real* d_h2o, Real* d_plev, void* stream)
+{
+    if (ncol <= 0 || nlay <= 0 || !h2o || !plev || !g_col_dry || !d_h2o || !d_plev)
+        return arg_error("col_dry_adjoint: bad arguments");
+    if (!((d1 == 1 || d1 == ncol) && (d2 == 1 || d2 == nlay))) return arg_error("col_dry_adjoint: bad h2o dims");
+    const int64_t scol = d1 == 1 ? 0 : 1, slay = d2 == 1 ? 0 : d1;
+    RRTMGP_TIMED("col_dry_adjoint_kernel", as_stream(stream));
+    hipLaunchKernelGGL(col_dry_adjoint_kernel<Real>, dim3(grid1(ncol, 256)), dim3(256), 0, as_stream(stream),
+                       ncol, nlay, h2o, scol, slay, plev, g_col_dry, d_h2o, d_plev);
+    RRTMGP_LAUNCHED("col_dry_adjoint_kernel");
     return RRTMGP_OK;
 }
 }  // namespace rrtmgp_hip
@@ -1215,16 +1670,33 @@ extern "C" int rrtmgp_hip_gas_optics_lw_##S(const rrtmgp_kdist_dev* kd, int64_t 
     const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau, T* tau, T* lay_src,              \
     T* lev_src_inc, T* lev_src_dec, T* sfc_src, T* sfc_src_jac, void* stream)                      \
 {                                                                                                  \
-    return gas_optics_lw<T>(kd, ncol, nlay, play, tlay, tlev, tsfc, col_dry, gases, sfc_lay,      \
-                            cloud_tau, tau, lay_src, lev_src_inc, lev_src_dec, sfc_src,            \
-                            sfc_src_jac, stream);                                                  \
+    return gas_optics_lw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, tlev, tsfc, col_dry, \
+                            gases, sfc_lay, cloud_tau, tau, lay_src, lev_src_inc, lev_src_dec,     \
+                            sfc_src, sfc_src_jac, stream);                                         \
+}                                                                                                  \
+extern "C" int rrtmgp_hip_gas_optics_lw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,  \
+    int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* tlev,    \
+    const T* tsfc, const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau,  \
+    T* tau, T* lay_src, T* lev_src_inc, T* lev_src_dec, T* sfc_src, T* sfc_src_jac, void* stream)   \
+{                                                                                                  \
+    return gas_optics_lw<T>(kd, band_start, band_count, ncol, nlay, play, tlay, tlev, tsfc,       \
+                            col_dry, gases, sfc_lay, cloud_tau, tau, lay_src, lev_src_inc,         \
+                            lev_src_dec, sfc_src, sfc_src_jac, stream);                            \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_gas_optics_sw_##S(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, \
     const T* play, const T* tlay, const T* col_dry, const rrtmgp_gas_vmr* gases,                   \
     const T* cloud_tau, const T* cloud_ssa, const T* cloud_g, T* tau, T* ssa, T* g, void* stream)  \
 {                                                                                                  \
-    return gas_optics_sw<T>(kd, ncol, nlay, play, tlay, col_dry, gases, cloud_tau, cloud_ssa,     \
-                            cloud_g, tau, ssa, g, stream);                                         \
+    return gas_optics_sw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, col_dry, gases,      \
+                            cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, stream);                   \
+}                                                                                                  \
+extern "C" int rrtmgp_hip_gas_optics_sw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,  \
+    int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* col_dry, \
+    const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_ssa, const T* cloud_g, T* tau,  \
+    T* ssa, T* g, void* stream)                                                                     \
+{                                                                                                  \
+    return gas_optics_sw<T>(kd, band_start, band_count, ncol, nlay, play, tlay, col_dry, gases,    \
+                            cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, stream);                   \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_interpolation_##S(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, \
     const T* play, const T* tlay, const T* col_gas, int* jtemp, T* fmajor, T* fminor, T* col_mix,  \
@@ -1323,6 +1795,38 @@ RRTMGP_DEF_GAS_OPTICS(f64, double)
 
 extern "C" int rrtmgp_hip_kdist_pack_f32(rrtmgp_kdist_dev* kd, void** pack) { return kdist_pack<float>(kd, pack); }
 extern "C" int rrtmgp_hip_kdist_pack_f64(rrtmgp_kdist_dev* kd, void** pack) { return kdist_pack<double>(kd, pack); }
+#define RRTMGP_DEF_GO_ADJOINT(S, T)                                                                 \
+extern "C" int rrtmgp_hip_gas_optics_lw_adjoint_##S(const rrtmgp_kdist_dev* kd, int64_t band,       \
+    int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* tlev, const T* tsfc,          \
+    const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* g_tau, const T* g_lay_src,  \
+    const T* g_lev_src_inc, const T* g_lev_src_dec, const T* g_sfc_src, T* d_col_gas, T* d_tlay,     \
+    T* d_lnp_lay, T* d_tlev_lo, T* d_tlev_hi, T* d_tsfc, void* stream)                               \
+{                                                                                                    \
+    return gas_optics_adjoint<T>(kd, true, band, ncol, nlay, play, tlay, tlev, tsfc, col_dry, gases, \
+                                 sfc_lay, nullptr, nullptr, nullptr, g_tau, g_lay_src, g_lev_src_inc,\
+                                 g_lev_src_dec, g_sfc_src, d_col_gas, d_tlay, d_lnp_lay, d_tlev_lo,  \
+                                 d_tlev_hi, d_tsfc, stream);                                         \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_gas_optics_sw_adjoint_##S(const rrtmgp_kdist_dev* kd, int64_t band,       \
+    int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* col_dry,                      \
+    const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_ssa, const T* cloud_g,           \
+    const T* g_tau, const T* g_ssa, const T* g_g, T* d_col_gas, T* d_tlay, T* d_lnp_lay, void* stream)\
+{                                                                                                    \
+    return gas_optics_adjoint<T>(kd, false, band, ncol, nlay, play, tlay, nullptr, nullptr, col_dry, \
+                                 gases, 1, cloud_tau, cloud_ssa, cloud_g, g_tau, g_ssa, g_g, nullptr, \
+                                 nullptr, d_col_gas, d_tlay, d_lnp_lay, nullptr, nullptr, nullptr,   \
+                                 stream);                                                            \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_get_col_dry_adjoint_##S(int64_t ncol, int64_t nlay, const T* vmr_h2o,     \
+    int64_t h2o_dim1, int64_t h2o_dim2, const T* plev, const T* g_col_dry, T* d_vmr_h2o, T* d_plev,  \
+    void* stream)                                                                                    \
+{                                                                                                    \
+    return col_dry_adjoint<T>(ncol, nlay, vmr_h2o, h2o_dim1, h2o_dim2, plev, g_col_dry, d_vmr_h2o,   \
+                              d_plev, stream);                                                       \
+}
+RRTMGP_DEF_GO_ADJOINT(f32, float)
+RRTMGP_DEF_GO_ADJOINT(f64, double)
+
 extern "C" int rrtmgp_hip_kdist_pack_free(void* pack)
 {
     delete static_cast<GoPack*>(pack);

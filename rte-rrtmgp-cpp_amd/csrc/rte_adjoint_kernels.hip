@@ -448,6 +448,62 @@ int sw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Rea
     return rc;
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// Assembly of the gas-optics gradients.
+
+// col_gas(gas) = vmr(gas) col_dry for gas >= 1 and col_gas(0) = col_dry, so
+//   d vmr(gas) = d col_gas(gas) col_dry,   d col_dry = d col_gas(0) + sum_gas d col_gas(gas) vmr(gas).
+// d_col (ncol, nlay, ngas+1) -> d_vmr (ncol, nlay, ngas) (overwritten), d_col_dry (ncol, nlay).
+template<typename Real>
+__global__ void col_chain_kernel(int64_t ncol, int64_t nlay, int ngas, const rrtmgp_gas_vmr gases,
+                                 const Real* __restrict__ col_dry, const Real* __restrict__ d_col,
+                                 Real* __restrict__ d_vmr, Real* __restrict__ d_col_dry)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t ncl = ncol * nlay;
+    if (i >= ncl) return;
+    const int64_t icol = i % ncol, ilay = i / ncol;
+    const Real cd = col_dry[i];
+    Real g = d_col[i];
+    for (int gas = 0; gas < ngas; ++gas)
+    {
+        const int64_t d1 = gases.dim1[gas];
+        const int64_t vi = (d1 == 1 ? 0 : icol) + (gases.dim2[gas] == 1 ? 0 : d1 * ilay);
+        const Real v = static_cast<const Real*>(gases.vmr[gas])[vi];
+        const Real dc = d_col[i + ncl * (gas + 1)];
+        d_vmr[i + ncl * gas] = dc * cd;
+        g += dc * v;
+    }
+    d_col_dry[i] = g;
+}
+
+// d_tlev (ncol, nlev): level i receives d_lo of layer i and d_hi of layer i-1.
+template<typename Real>
+__global__ void levels_kernel(int64_t ncol, int64_t nlay, const Real* __restrict__ lo, const Real* __restrict__ hi,
+                              Real* __restrict__ d_tlev)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * (nlay + 1)) return;
+    const int64_t ilev = i / ncol;
+    Real v = Real(0);
+    if (ilev < nlay) v += lo[i];
+    if (ilev > 0) v += hi[i - ncol];
+    d_tlev[i] = v;
+}
+
+// out (nbnd, ncol) = sums over each band's g-points of in (ncol, ngpt); band_lims (2, nbnd) 1-based.
+template<typename Real>
+__global__ void gpt_to_band_kernel(int64_t ncol, int64_t nbnd, const int* __restrict__ band_lims,
+                                   const Real* __restrict__ in, Real* __restrict__ out)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * nbnd) return;
+    const int64_t b = i % nbnd, icol = i / nbnd;
+    Real s = Real(0);
+    for (int g = band_lims[2 * b] - 1; g < band_lims[2 * b + 1]; ++g) s += in[icol + ncol * g];
+    out[b + nbnd * icol] = s;
+}
 }  // namespace rrtmgp_hip
 
 using namespace rrtmgp_hip;
@@ -478,3 +534,44 @@ extern "C" int rrtmgp_hip_sw_solver_2stream_adjoint_##S(int64_t ncol, int64_t nl
 
 RRTMGP_DEF_ADJOINT(f32, float)
 RRTMGP_DEF_ADJOINT(f64, double)
+
+#define RRTMGP_DEF_ADJ_HELPERS(S, T)                                                               \
+extern "C" int rrtmgp_hip_adjoint_col_chain_##S(int64_t ncol, int64_t nlay,                        \
+    const rrtmgp_gas_vmr* gases, const T* col_dry, const T* d_col_gas, T* d_vmr, T* d_col_dry,       \
+    void* stream)                                                                                   \
+{                                                                                                   \
+    if (!gases || gases->ngas < 1 || gases->ngas > RRTMGP_MAX_GAS || !col_dry || !d_col_gas ||     \
+        !d_vmr || !d_col_dry || ncol <= 0 || nlay <= 0)                                             \
+        return arg_error("adjoint_col_chain: bad arguments");                                      \
+    const int64_t n = ncol * nlay;                                                                  \
+    RRTMGP_TIMED("col_chain_kernel", as_stream(stream));                                            \
+    hipLaunchKernelGGL(col_chain_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),  \
+                       ncol, nlay, int(gases->ngas), *gases, col_dry, d_col_gas, d_vmr, d_col_dry); \
+    RRTMGP_LAUNCHED("col_chain_kernel");                                                           \
+    return RRTMGP_OK;                                                                               \
+}                                                                                                   \
+extern "C" int rrtmgp_hip_adjoint_levels_##S(int64_t ncol, int64_t nlay, const T* d_lo,            \
+    const T* d_hi, T* d_tlev, void* stream)                                                         \
+{                                                                                                   \
+    if (!d_lo || !d_hi || !d_tlev || ncol <= 0 || nlay <= 0) return arg_error("adjoint_levels: bad arguments"); \
+    const int64_t n = ncol * (nlay + 1);                                                            \
+    RRTMGP_TIMED("levels_kernel", as_stream(stream));                                               \
+    hipLaunchKernelGGL(levels_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),     \
+                       ncol, nlay, d_lo, d_hi, d_tlev);                                             \
+    RRTMGP_LAUNCHED("levels_kernel");                                                              \
+    return RRTMGP_OK;                                                                               \
+}                                                                                                   \
+extern "C" int rrtmgp_hip_gpt_to_band_##S(int64_t ncol, int64_t nbnd, const int* band_lims_gpt,    \
+    const T* in, T* out, void* stream)                                                              \
+{                                                                                                   \
+    if (!band_lims_gpt || !in || !out || ncol <= 0 || nbnd <= 0) return arg_error("gpt_to_band: bad arguments"); \
+    const int64_t n = ncol * nbnd;                                                                  \
+    RRTMGP_TIMED("gpt_to_band_kernel", as_stream(stream));                                          \
+    hipLaunchKernelGGL(gpt_to_band_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),\
+                       ncol, nbnd, band_lims_gpt, in, out);                                         \
+    RRTMGP_LAUNCHED("gpt_to_band_kernel");                                                         \
+    return RRTMGP_OK;                                                                               \
+}
+
+RRTMGP_DEF_ADJ_HELPERS(f32, float)
+RRTMGP_DEF_ADJ_HELPERS(f64, double)

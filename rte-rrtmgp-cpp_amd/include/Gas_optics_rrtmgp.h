@@ -160,6 +160,7 @@ class Gas_optics_rrtmgp_gpu : public Gas_optics_gpu
         const Array<int,2>& get_flavor() const { return flavor; }
         const Array<int,2>& get_gpoint_flavor() const { return gpoint_flavor; }
         const rrtmgp_kdist_dev& device_tables() const { return dev; }
+        const Array_gpu<Float,1>& solar_source_device() const { return solar_source_gpu; }
         const Array<int,2>& get_minor_limits_gpt(bool lower) const { return lower ? minor_limits_gpt_lower : minor_limits_gpt_upper; }
         const Array<int,1>& get_kminor_start(bool lower) const { return lower ? kminor_start_lower : kminor_start_upper; }
         const Array<int,1>& get_idx_minor(bool lower) const { return lower ? idx_minor_lower : idx_minor_upper; }

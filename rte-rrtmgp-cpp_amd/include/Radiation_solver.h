@@ -46,6 +46,7 @@ class Radiation_solver_longwave_gpu
         int get_n_gpt() const { return kdist->get_ngpt(); }
         int get_n_bnd() const { return kdist->get_nband(); }
         Gas_optics_rrtmgp_gpu& get_kdist() { return *kdist; }
+        Cloud_optics_gpu* get_cloud_optics() { return cloud_optics.get(); }
 
     private:
         std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist;
@@ -88,6 +89,7 @@ class Radiation_solver_shortwave_gpu
         int get_n_bnd() const { return kdist->get_nband(); }
         Float get_tsi() const { return kdist->get_tsi(); }
         Gas_optics_rrtmgp_gpu& get_kdist() { return *kdist; }
+        Cloud_optics_gpu* get_cloud_optics() { return cloud_optics.get(); }
 
     private:
         std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist;

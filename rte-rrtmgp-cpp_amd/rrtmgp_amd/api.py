@@ -412,6 +412,29 @@ class SolverLW(_Solver):
         return o
 
 
+    def adjoint(self, da: DeviceAtmosphere, g_flux_up, g_flux_dn):
+        """Gradients of sum(g_flux_up * flux_up + g_flux_dn * flux_dn) over the broadband fluxes
+        (ncol, nlev) of solve(da) (rrtmgp_api_adjoint_lw).  Returns device arrays: t_lay, t_lev,
+        t_sfc, lnp_lay (with respect to ln p_lay), p_lev, vmr (ncol, nlay, ngas; the k-distribution's
+        gas order) and emis_sfc (nbnd, ncol)."""
+        dt = real_dtype(self.precision)
+        ncol, nlay = da.ncol, da.nlay
+        nlev = nlay + 1
+        gu, gd = _dev(g_flux_up, dt), _dev(g_flux_dn, dt)
+        o = {"t_lay": DeviceArray((ncol, nlay), dt), "t_lev": DeviceArray((ncol, nlev), dt),
+             "t_sfc": DeviceArray((ncol,), dt), "lnp_lay": DeviceArray((ncol, nlay), dt),
+             "p_lev": DeviceArray((ncol, nlev), dt), "vmr": DeviceArray((ncol, nlay, self.kd.ngas), dt),
+             "emis_sfc": DeviceArray((self.kd.nbnd, ncol), dt)}
+        cloudy = da.cloudy and self.cloud is not None
+        self._check(self.L.rrtmgp_api_adjoint_lw(
+            self.h, ncol, nlay, int(cloudy), da.p_lay.ptr, da.p_lev.ptr, da.t_lay.ptr, da.t_lev.ptr, da.t_sfc.ptr,
+            da.gas_concs.h, da.emis_sfc.ptr, _p(da.lwp) if cloudy else None, _p(da.iwp) if cloudy else None,
+            _p(da.rel) if cloudy else None, _p(da.rei) if cloudy else None, gu.ptr, gd.ptr,
+            o["t_lay"].ptr, o["t_lev"].ptr, o["t_sfc"].ptr, o["lnp_lay"].ptr, o["p_lev"].ptr, o["vmr"].ptr,
+            o["emis_sfc"].ptr))
+        return o
+
+
 class SolverSW(_Solver):
     """Radiation_solver_shortwave_gpu."""
 
@@ -441,4 +464,30 @@ class SolverSW(_Solver):
             o["flux_up"].ptr, o["flux_dn"].ptr, o["flux_dn_dir"].ptr, o["flux_net"].ptr,
             _p(o.get("bnd_flux_up")), _p(o.get("bnd_flux_dn")), _p(o.get("bnd_flux_dn_dir")),
             _p(o.get("bnd_flux_net"))))
+        return o
+
+    def adjoint(self, da: DeviceAtmosphere, tsi_scaling, g_flux_up, g_flux_dn, g_flux_dir, delta_cloud=True,
+                sfc_alb_dif=None):
+        """Gradients of sum(g_flux_up * flux_up + g_flux_dn * flux_dn + g_flux_dir * flux_dn_dir) over
+        the broadband fluxes of solve(da, tsi_scaling) (rrtmgp_api_adjoint_sw).  The diffuse albedo
+        defaults to the direct one, as in solve().  Returns device arrays: t_lay, lnp_lay, p_lev,
+        vmr (ncol, nlay, ngas), sfc_alb_dir and sfc_alb_dif (nbnd, ncol)."""
+        dt = real_dtype(self.precision)
+        ncol, nlay = da.ncol, da.nlay
+        nlev = nlay + 1
+        tsi = _dev(tsi_scaling, dt)
+        af = da.sfc_alb if sfc_alb_dif is None else _dev(sfc_alb_dif, dt)
+        gu, gd, gr = _dev(g_flux_up, dt), _dev(g_flux_dn, dt), _dev(g_flux_dir, dt)
+        o = {"t_lay": DeviceArray((ncol, nlay), dt), "lnp_lay": DeviceArray((ncol, nlay), dt),
+             "p_lev": DeviceArray((ncol, nlev), dt), "vmr": DeviceArray((ncol, nlay, self.kd.ngas), dt),
+             "sfc_alb_dir": DeviceArray((self.kd.nbnd, ncol), dt),
+             "sfc_alb_dif": DeviceArray((self.kd.nbnd, ncol), dt)}
+        cloudy = da.cloudy and self.cloud is not None
+        self._check(self.L.rrtmgp_api_adjoint_sw(
+            self.h, ncol, nlay, int(cloudy), int(delta_cloud), da.p_lay.ptr, da.p_lev.ptr, da.t_lay.ptr,
+            da.gas_concs.h, da.sfc_alb.ptr, af.ptr, tsi.ptr, da.mu0.ptr,
+            _p(da.lwp) if cloudy else None, _p(da.iwp) if cloudy else None,
+            _p(da.rel) if cloudy else None, _p(da.rei) if cloudy else None, gu.ptr, gd.ptr, gr.ptr,
+            o["t_lay"].ptr, o["lnp_lay"].ptr, o["p_lev"].ptr, o["vmr"].ptr, o["sfc_alb_dir"].ptr,
+            o["sfc_alb_dif"].ptr))
         return o

@@ -11,6 +11,7 @@
 #include "Gas_concs.h"
 #include "Gas_optics_rrtmgp.h"
 #include "Optical_props.h"
+#include "Radiation_adjoint.h"
 #include "Radiation_solver.h"
 #include "Rte_lw.h"
 #include "Rte_sw.h"
@@ -620,6 +621,65 @@ int rrtmgp_api_solve_sw(void* h, int ncol, int nlay, int cloudy, int bnd, int de
                          cloudy ? view<2>(rel, {ncol, nlay}) : Array_gpu<Float,2>(),
                          cloudy ? view<2>(rei, {ncol, nlay}) : Array_gpu<Float,2>(),
                          fu, fd, fdir, fn, bu, bd, bdir, bn);
+    });
+}
+
+
+int rrtmgp_api_adjoint_lw(void* h, int ncol, int nlay, int cloudy, const rrtmgp_real* play,
+                          const rrtmgp_real* plev, const rrtmgp_real* tlay, const rrtmgp_real* tlev,
+                          const rrtmgp_real* tsfc, void* gas_concs, const rrtmgp_real* emis_sfc,
+                          const rrtmgp_real* lwp, const rrtmgp_real* iwp, const rrtmgp_real* rel,
+                          const rrtmgp_real* rei, const rrtmgp_real* g_flux_up,
+                          const rrtmgp_real* g_flux_dn, rrtmgp_real* d_tlay, rrtmgp_real* d_tlev,
+                          rrtmgp_real* d_tsfc, rrtmgp_real* d_lnp_lay, rrtmgp_real* d_plev,
+                          rrtmgp_real* d_vmr, rrtmgp_real* d_emis_sfc)
+{
+    return guarded([&] {
+        auto* s = static_cast<Solver*>(h);
+        if (!s || !s->lw) throw std::invalid_argument("not a longwave solver");
+        const int nlev = nlay + 1, nbnd = s->lw->get_n_bnd();
+        Gas_optics_rrtmgp_gpu& kd = s->lw->get_kdist();
+        Lw_adjoint_out o;
+        o.d_t_lay = view<2>(d_tlay, {ncol, nlay}); o.d_t_lev = view<2>(d_tlev, {ncol, nlev});
+        o.d_t_sfc = view<1>(d_tsfc, {ncol}); o.d_lnp_lay = view<2>(d_lnp_lay, {ncol, nlay});
+        o.d_p_lev = view<2>(d_plev, {ncol, nlev}); o.d_vmr = view<3>(d_vmr, {ncol, nlay, kd.get_ngas()});
+        o.d_emis_sfc = view<2>(d_emis_sfc, {nbnd, ncol});
+        auto c = [&](const rrtmgp_real* p) { return cloudy ? view<2>(p, {ncol, nlay}) : Array_gpu<Float,2>(); };
+        lw_adjoint_gpu(kd, cloudy ? s->lw->get_cloud_optics() : nullptr, *static_cast<Gas_concs_gpu*>(gas_concs),
+                       view<2>(play, {ncol, nlay}), view<2>(plev, {ncol, nlev}), view<2>(tlay, {ncol, nlay}),
+                       view<2>(tlev, {ncol, nlev}), view<1>(tsfc, {ncol}), view<2>(emis_sfc, {nbnd, ncol}),
+                       c(lwp), c(iwp), c(rel), c(rei), view<2>(g_flux_up, {ncol, nlev}),
+                       view<2>(g_flux_dn, {ncol, nlev}), o);
+    });
+}
+
+int rrtmgp_api_adjoint_sw(void* h, int ncol, int nlay, int cloudy, int delta_cloud,
+                          const rrtmgp_real* play, const rrtmgp_real* plev, const rrtmgp_real* tlay,
+                          void* gas_concs, const rrtmgp_real* sfc_alb_dir,
+                          const rrtmgp_real* sfc_alb_dif, const rrtmgp_real* tsi_scaling,
+                          const rrtmgp_real* mu0, const rrtmgp_real* lwp, const rrtmgp_real* iwp,
+                          const rrtmgp_real* rel, const rrtmgp_real* rei,
+                          const rrtmgp_real* g_flux_up, const rrtmgp_real* g_flux_dn,
+                          const rrtmgp_real* g_flux_dir, rrtmgp_real* d_tlay,
+                          rrtmgp_real* d_lnp_lay, rrtmgp_real* d_plev, rrtmgp_real* d_vmr,
+                          rrtmgp_real* d_sfc_alb_dir, rrtmgp_real* d_sfc_alb_dif)
+{
+    return guarded([&] {
+        auto* s = static_cast<Solver*>(h);
+        if (!s || !s->sw) throw std::invalid_argument("not a shortwave solver");
+        const int nlev = nlay + 1, nbnd = s->sw->get_n_bnd();
+        Gas_optics_rrtmgp_gpu& kd = s->sw->get_kdist();
+        Sw_adjoint_out o;
+        o.d_t_lay = view<2>(d_tlay, {ncol, nlay}); o.d_lnp_lay = view<2>(d_lnp_lay, {ncol, nlay});
+        o.d_p_lev = view<2>(d_plev, {ncol, nlev}); o.d_vmr = view<3>(d_vmr, {ncol, nlay, kd.get_ngas()});
+        o.d_sfc_alb_dir = view<2>(d_sfc_alb_dir, {nbnd, ncol}); o.d_sfc_alb_dif = view<2>(d_sfc_alb_dif, {nbnd, ncol});
+        auto c = [&](const rrtmgp_real* p) { return cloudy ? view<2>(p, {ncol, nlay}) : Array_gpu<Float,2>(); };
+        sw_adjoint_gpu(kd, cloudy ? s->sw->get_cloud_optics() : nullptr, delta_cloud != 0,
+                       *static_cast<Gas_concs_gpu*>(gas_concs), view<2>(play, {ncol, nlay}),
+                       view<2>(plev, {ncol, nlev}), view<2>(tlay, {ncol, nlay}), view<2>(sfc_alb_dir, {nbnd, ncol}),
+                       view<2>(sfc_alb_dif, {nbnd, ncol}), view<1>(tsi_scaling, {ncol}), view<1>(mu0, {ncol}),
+                       c(lwp), c(iwp), c(rel), c(rei), view<2>(g_flux_up, {ncol, nlev}),
+                       view<2>(g_flux_dn, {ncol, nlev}), view<2>(g_flux_dir, {ncol, nlev}), o);
     });
 }
 

@@ -9,6 +9,16 @@ template<typename T> struct K;
 #define RRTMGP_K_ENTRIES(S)                                              \
     static constexpr auto gas_optics_lw = rrtmgp_hip_gas_optics_lw_##S;  \
     static constexpr auto kdist_pack = rrtmgp_hip_kdist_pack_##S;  \
+    static constexpr auto gas_optics_lw_bands = rrtmgp_hip_gas_optics_lw_bands_##S; \
+    static constexpr auto gas_optics_sw_bands = rrtmgp_hip_gas_optics_sw_bands_##S; \
+    static constexpr auto gas_optics_lw_adjoint = rrtmgp_hip_gas_optics_lw_adjoint_##S; \
+    static constexpr auto gas_optics_sw_adjoint = rrtmgp_hip_gas_optics_sw_adjoint_##S; \
+    static constexpr auto get_col_dry_adjoint = rrtmgp_hip_get_col_dry_adjoint_##S; \
+    static constexpr auto lw_solver_noscat_adjoint = rrtmgp_hip_lw_solver_noscat_adjoint_##S; \
+    static constexpr auto sw_solver_2stream_adjoint = rrtmgp_hip_sw_solver_2stream_adjoint_##S; \
+    static constexpr auto adjoint_col_chain = rrtmgp_hip_adjoint_col_chain_##S; \
+    static constexpr auto adjoint_levels = rrtmgp_hip_adjoint_levels_##S; \
+    static constexpr auto gpt_to_band = rrtmgp_hip_gpt_to_band_##S; \
     static constexpr auto gas_optics_sw = rrtmgp_hip_gas_optics_sw_##S;  \
     static constexpr auto interpolation = rrtmgp_hip_interpolation_##S;  \
     static constexpr auto compute_tau_absorption = rrtmgp_hip_compute_tau_absorption_##S; \

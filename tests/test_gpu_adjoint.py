@@ -155,3 +155,104 @@ def test_sw_solver_adjoint_fd(G, O, okd_sw, lut_sw, syn, top_at_1):
         ad = sum(np.sum(grad[k] * v[k]) for k in ("tau", "ssa", "g"))
         ad += np.sum(grad["alb_dir"] * v["alb_dir"][band_of, :].T) + np.sum(grad["alb_dif"] * v["alb_dif"][band_of, :].T)
         assert abs(ad - fd) <= 1e-6 * abs(fd), (trial, ad, fd)
+
+
+# ---------------------------------------------------------------------------------------------
+# End-to-end adjoints (rrtmgp_api_adjoint_{lw,sw}): band-by-band gas-optics recomputation, solver
+# adjoint, gas-optics adjoint and the col_dry chain, against central differences of the FP64
+# oracle's full all-sky solve along random directions in every input group.
+
+def _perturbed(atm, v, h):
+    import copy
+    a = copy.deepcopy(atm)
+    a.t_lay = atm.t_lay + h * v["t_lay"]
+    a.p_lay = atm.p_lay * np.exp(h * v["lnp_lay"])
+    a.p_lev = atm.p_lev + h * v["p_lev"]
+    a.vmr = dict(atm.vmr)
+    for gas in ("h2o", "o3"):
+        a.vmr[gas] = atm.vmr[gas] + h * v[gas]
+    if "t_lev" in v:
+        a.t_lev = atm.t_lev + h * v["t_lev"]
+        a.t_sfc = atm.t_sfc + h * v["t_sfc"]
+        a.emis_sfc = atm.emis_sfc + h * v["emis_sfc"]
+    if "sfc_alb" in v:
+        a.sfc_alb = atm.sfc_alb + h * v["sfc_alb"]
+    return a
+
+
+def _directions(rng, atm, lw):
+    v = {"t_lay": rng.normal(size=atm.t_lay.shape),
+         # Pressures at the table's lower edge (1 Pa) stay put: the oracle's range check rejects
+         # any step below it.
+         "lnp_lay": rng.normal(size=atm.p_lay.shape) * 1e-3 * (atm.p_lay > 2.),
+         "p_lev": rng.normal(size=atm.p_lev.shape) * 1e-3 * atm.p_lev * (atm.p_lev > 2.),
+         "h2o": rng.normal(size=atm.vmr["h2o"].shape) * 0.01 * atm.vmr["h2o"],
+         "o3": rng.normal(size=atm.vmr["o3"].shape) * 0.01 * atm.vmr["o3"]}
+    if lw:
+        v.update({"t_lev": rng.normal(size=atm.t_lev.shape), "t_sfc": rng.normal(size=atm.t_sfc.shape),
+                  "emis_sfc": rng.normal(size=atm.emis_sfc.shape) * 0.01})
+    else:
+        v["sfc_alb"] = rng.normal(size=atm.sfc_alb.shape) * 0.01
+    return v
+
+
+def _dot(G, grad, v, names, lw):
+    ih, io = names.index("h2o"), names.index("o3")
+    vm = grad["vmr"]
+    ad = (np.sum(grad["t_lay"] * v["t_lay"]) + np.sum(grad["lnp_lay"] * v["lnp_lay"]) +
+          np.sum(grad["p_lev"] * v["p_lev"]) + np.sum(vm[:, :, ih] * v["h2o"]) + np.sum(vm[:, :, io] * v["o3"]))
+    if lw:
+        ad += (np.sum(grad["t_lev"] * v["t_lev"]) + np.sum(grad["t_sfc"] * v["t_sfc"]) +
+               np.sum(grad["emis_sfc"] * v["emis_sfc"]))
+    else:
+        ad += np.sum((grad["sfc_alb_dir"] + grad["sfc_alb_dif"]) * v["sfc_alb"])
+    return ad
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_lw_adjoint_end_to_end_fd(G, O, syn, kd_lw, okd_lw, lut_lw, top_at_1):
+    from conftest import AVAILABLE
+    atm = syn.make_atmosphere(51, 8, 30, cloudy=True, top_at_1=top_at_1)
+    kd = G.KDist(kd_lw, AVAILABLE, precision=8)
+    solver = G.SolverLW(kd, G.Cloud(lut_lw, 8))
+    da = G.DeviceAtmosphere(atm, 8)
+    rng = np.random.default_rng(11)
+    a_up, a_dn = rng.normal(size=(atm.ncol, atm.nlay + 1)), rng.normal(size=(atm.ncol, atm.nlay + 1))
+    grad = {k: _host(d) for k, d in solver.adjoint(da, a_up, a_dn).items()}
+    ocl = O.Cloud(lut_lw)
+
+    def loss(a):
+        o = O.solve_lw(okd_lw, a, ocl)
+        return float(np.sum(a_up * o["flux_up"]) + np.sum(a_dn * o["flux_dn"]))
+
+    for trial in range(3):
+        v = _directions(rng, atm, True)
+        h = 1e-4
+        fd = (loss(_perturbed(atm, v, h)) - loss(_perturbed(atm, v, -h))) / (2 * h)
+        ad = _dot(G, grad, v, kd.gas_names, True)
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_sw_adjoint_end_to_end_fd(G, O, syn, kd_sw, okd_sw, lut_sw, top_at_1):
+    from conftest import AVAILABLE
+    atm = syn.make_atmosphere(52, 8, 30, cloudy=True, top_at_1=top_at_1)
+    kd = G.KDist(kd_sw, AVAILABLE, precision=8)
+    solver = G.SolverSW(kd, G.Cloud(lut_sw, 8))
+    da = G.DeviceAtmosphere(atm, 8)
+    tsi = np.full(atm.ncol, atm.tsi / kd.scalar("tsi"))
+    rng = np.random.default_rng(12)
+    a_up, a_dn, a_dir = (rng.normal(size=(atm.ncol, atm.nlay + 1)) for _ in range(3))
+    grad = {k: _host(d) for k, d in solver.adjoint(da, tsi, a_up, a_dn, a_dir).items()}
+    ocl = O.Cloud(lut_sw)
+
+    def loss(a):
+        o = O.solve_sw(okd_sw, a, ocl, tsi_scaling=tsi)
+        return float(np.sum(a_up * o["flux_up"]) + np.sum(a_dn * o["flux_dn"]) + np.sum(a_dir * o["flux_dn_dir"]))
+
+    for trial in range(3):
+        v = _directions(rng, atm, False)
+        h = 1e-4
+        fd = (loss(_perturbed(atm, v, h)) - loss(_perturbed(atm, v, -h))) / (2 * h)
+        ad = _dot(G, grad, v, kd.gas_names, False)
+        assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
