@@ -83,6 +83,48 @@ def algorithmic_bytes(ncol, nlay, E, kd_lw, kd_sw, n2d_gases, cloudy):
     return b
 
 
+def adjoint_measure(kl, ks, ll, ls, prec, ncol=16384, nlay=128, reps=3):
+    """Forward and adjoint (LW + SW, all-sky) of the 16384-column workload: mean times, their ratio
+    and the device memory pool after the adjoint (the library's caching allocator grows only, so
+    its size is the peak of the run)."""
+    import ctypes as C
+    from rrtmgp_amd import _lib, api, device
+    from rrtmgp_amd import synthetic as syn
+    atm = syn.make_atmosphere(syn.SEED_ATMOS[1], ncol, nlay, cloudy=True)
+    kd_lw, kd_sw = api.KDist(kl, AVAILABLE, prec), api.KDist(ks, AVAILABLE, prec)
+    slw, ssw = api.SolverLW(kd_lw, api.Cloud(ll, prec)), api.SolverSW(kd_sw, api.Cloud(ls, prec))
+    dt = api.real_dtype(prec)
+    da = api.DeviceAtmosphere(atm, prec)
+    tsi = device.DeviceArray.from_host(np.full(ncol, atm.tsi / kd_sw.scalar("tsi")), dt)
+    rng = np.random.default_rng(0)
+    cot = [device.DeviceArray.from_host(rng.normal(size=(ncol, nlay + 1)), dt) for _ in range(5)]
+
+    def fwd():
+        slw.solve(da)
+        ssw.solve(da, tsi)
+
+    def adj():
+        slw.adjoint(da, cot[0], cot[1])
+        ssw.adjoint(da, tsi, cot[2], cot[3], cot[4])
+
+    out = {}
+    for name, fn in (("forward", fwd), ("adjoint", adj)):
+        fn()
+        device.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        device.synchronize()
+        out[name] = (time.perf_counter() - t0) * 1e3 / reps
+    live, cached = C.c_size_t(0), C.c_size_t(0)
+    _lib.check_hip(_lib.hip().rrtmgp_hip_cache_stats(C.byref(live), C.byref(cached)))
+    return {"workload": f"all-sky LW+SW, {ncol} columns x {nlay} layers, FP{8 * prec}",
+            "forward_ms": round(out["forward"], 3), "adjoint_ms": round(out["adjoint"], 3),
+            "adjoint_over_forward": round(out["adjoint"] / out["forward"], 3),
+            "device_pool_bytes": int(live.value + cached.value),
+            "gradients": "T_lay, T_lev, T_sfc, ln p_lay, p_lev, vmr of every gas, emissivity, albedos"}
+
+
 def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
     """FP64 oracle on the first ncols columns with `threads` OpenMP threads; returns the timing
     record and the max flux error of the GPU fluxes over those columns."""
@@ -136,6 +178,7 @@ def main():
     ap.add_argument("--cpu-cols", type=int, default=4096, help="columns of the CPU baseline sample")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: OMP_NUM_THREADS, max 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="launch one untimed 1 GiB device copy first (reference dispatch for PMC counters)")
     args = ap.parse_args()
@@ -257,6 +300,12 @@ def main():
                        "per_output": {k: round(v, 6) for k, v in err.items()}})
         log(f"CPU baseline {rate:.3e} U/s in {secs:.2f} s; max flux error {max(err.values()):.3e} W m-2")
 
+    adjoint = None
+    if not args.no_adjoint:
+        log("adjoint: LW + SW forward and adjoint at 16384 columns")
+        adjoint = adjoint_measure(kl, ks, ll, ls, prec)
+        log(f"adjoint {adjoint['adjoint_ms']:.1f} ms vs forward {adjoint['forward_ms']:.1f} ms")
+
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "col-gpt-layers/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
@@ -266,7 +315,7 @@ def main():
         "config": {"workload": f"all-sky LW+SW forward, plain gas optics, {ncol} columns x {nlay} layers",
                    "ncol": ncol, "nlay": nlay, "ngpt_lw": kd_lw.ngpt, "ngpt_sw": kd_sw.ngpt,
                    "n_col_block": blk, "gpu_ms_per_step": round(gpu_ms / args.steps, 3)},
-        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "kernels": krows,
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "adjoint": adjoint, "kernels": krows,
     }
     print(json.dumps(line), flush=True)
 

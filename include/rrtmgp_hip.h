@@ -461,6 +461,95 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_FLUXES)
         const T* flux_dn, const T* plev, T* heating_rate, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_HEATING)
 
+/* Reverse mode of rrtmgp_hip_heating_rate_*: for the cotangent g_heating_rate (ncol, nlay),   */
+/* ADDS the gradients into g_flux_up, g_flux_dn and d_plev (ncol, nlev).  No reference         */
+/* counterpart.                                                                                 */
+#define RRTMGP_DECL_HEATING_ADJ(S, T)                                                          \
+    int rrtmgp_hip_heating_rate_adjoint_##S(int64_t ncol, int64_t nlay, const T* flux_up,     \
+        const T* flux_dn, const T* plev, const T* g_heating_rate, T* g_flux_up, T* g_flux_dn,  \
+        T* d_plev, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_HEATING_ADJ)
+
+/* ------------------------------------------------------------------------------------------ */
+/* MLP gas-optics surrogate (FP32 path; BF16 matrix cores with FP32 accumulation).  No         */
+/* counterpart in the reference.  Per (column, layer) the network maps                         */
+/* x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o] (normalised with x_mean/x_std) through  */
+/* two softplus layers of 128 to n_out outputs (denormalised with y_mean/y_std), interleaved  */
+/* per g-point: LW (ln(tau/col_dry), Planck fraction), SW (ln(tau_abs/col_dry),               */
+/* ln(tau_ray/col_dry)).  Weights are row-major (out, in) BF16; W1 is (128, 16) with the 7    */
+/* inputs first.  Inputs outside [x_lo, x_hi] flag their 64-column tile in tile_flags          */
+/* ((ncol/64 rounded up) x nlay, layer-major) for the plain gas optics to recompute.            */
+/* ------------------------------------------------------------------------------------------ */
+typedef struct
+{
+    const void* W1;                        /* (128, 16) BF16                                */
+    const void* W2;                        /* (128, 128) BF16                               */
+    const void* W3;                        /* (n_out, 128) BF16                             */
+    const float* b1;
+    const float* b2;
+    const float* b3;
+    const float* x_mean;                   /* (7)                                           */
+    const float* x_std;
+    const float* x_lo;                     /* training envelope of the raw inputs           */
+    const float* x_hi;
+    const float* y_mean;                   /* (n_out)                                       */
+    const float* y_std;
+    int64_t n_out;                         /* 2 ngpt, a multiple of 32                      */
+} rrtmgp_mlp_net;
+
+/* Inference, fused with the gas-optics epilogue (outputs as rrtmgp_hip_gas_optics_lw/sw_f32, */
+/* including the band cloud add_to).  gases: exactly h2o, o3, co2, ch4, n2o in this order;    */
+/* gpt2band (ngpt) 0-based band of each g-point; totplnk (nplanck, nbnd) FP32.                 */
+int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* net, int64_t ncol, int64_t nlay, int64_t ngpt,
+                                 int64_t nbnd, const int* gpt2band, const float* play, const float* tlay,
+                                 const float* tlev, const float* tsfc, const float* col_dry,
+                                 const rrtmgp_gas_vmr* gases, const float* totplnk, int64_t nplanck,
+                                 double temp_min, double dplanck, int sfc_lay, const float* cloud_tau,
+                                 float* tau, float* lay_src, float* lev_src_inc, float* lev_src_dec,
+                                 float* sfc_src, float* sfc_src_jac, int* tile_flags, void* stream);
+int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* net, int64_t ncol, int64_t nlay, int64_t ngpt,
+                                 int64_t nbnd, const int* gpt2band, const float* play, const float* tlay,
+                                 const float* col_dry, const rrtmgp_gas_vmr* gases, const float* cloud_tau,
+                                 const float* cloud_ssa, const float* cloud_g, float* tau, float* ssa,
+                                 float* g, int* tile_flags, void* stream);
+
+/* Training.  gemm_bf16: C[m sCm + n sCn] (+)= sum_k A[m sAm + k sAk] B[k sBk + n sBn], BF16      */
+/* operands, FP32 result (accumulate != 0 adds to C).  Row-major (rows, n) arrays below.        */
+int rrtmgp_hip_mlp_gemm_bf16(int64_t M, int64_t N, int64_t K, const void* A, int64_t sAm, int64_t sAk,
+                             const void* B, int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn,
+                             int accumulate, void* stream);
+/* Z += bias (per column); H = softplus(Z) in BF16.                                             */
+int rrtmgp_hip_mlp_bias_softplus(int64_t rows, int n, float* Z, const float* bias, void* H, void* stream);
+/* Y += bias; dY = 2 (Y - T)/(rows n) in BF16 and FP32; loss_parts[block] = partial sums of      */
+/* (Y - T)^2 over blocks of 256 elements.                                                        */
+int rrtmgp_hip_mlp_mse_grad(int64_t rows, int n, float* Y, const float* bias, const float* T, void* dY,
+                            float* dYf, float* loss_parts, void* stream);
+/* dZ = dH * sigmoid(Z) (softplus'), BF16 and FP32.                                              */
+int rrtmgp_hip_mlp_softplus_back(int64_t n, const float* dH, const float* Z, void* dZ, float* dZf,
+                                 void* stream);
+/* db[j] = sum over rows of dZ[r, j].                                                            */
+int rrtmgp_hip_mlp_bias_grad(int64_t rows, int n, const float* dZ, float* db, void* stream);
+/* Adam step `step` (1-based) on FP32 master weights p with moments m, v; writes the BF16 copy   */
+/* p_bf16 when it is not NULL.                                                                   */
+int rrtmgp_hip_mlp_adam(int64_t n, float* p, const float* g, float* m, float* v, void* p_bf16, float lr,
+                        float beta1, float beta2, float eps, int64_t step, void* stream);
+int rrtmgp_hip_mlp_to_bf16(int64_t n, const float* x, void* y, void* stream);
+/* Training data from the plain gas optics: raw inputs X (ncol nlay, 16) with row = icol +      */
+/* ncol ilay; targets Y (ncol nlay, 2 ngpt) from LW (a = tau, b = lay_src) or SW (a = tau,       */
+/* b = ssa) outputs of a clear-sky run.                                                          */
+int rrtmgp_hip_mlp_inputs(int64_t ncol, int64_t nlay, const float* play, const float* tlay,
+                          const rrtmgp_gas_vmr* gases, float* X, void* stream);
+int rrtmgp_hip_mlp_targets(int lw, int64_t ncol, int64_t nlay, int64_t ngpt, const int* gpt2band,
+                           const float* tlay, const float* col_dry, const float* a, const float* b,
+                           const float* totplnk, int64_t nplanck, double temp_min, double dplanck,
+                           float* Y, void* stream);
+/* dst[i, :] = src[idx[i], :] (rows of n); y = (x - mean)/std over the first n_norm columns     */
+/* (others 0), BF16 and/or FP32.                                                                 */
+int rrtmgp_hip_mlp_gather_rows(int64_t rows, int n, const int64_t* idx, const float* src, float* dst,
+                               void* stream);
+int rrtmgp_hip_mlp_normalize_rows(int64_t rows, int n, int n_norm, const float* x, const float* mean,
+                                  const float* std_, void* y_bf16, float* y_f32, void* stream);
+
 /* spread_col_kernel (src_cuda/Gas_optics_rrtmgp.cu:40-68): dst(icol, igpt) = src(igpt), and  */
 /* the per-column scaling of the driver (toa_src * tsi_scaling, src_test/Radiation_solver.cpp: */
 /* 739-743): arr(icol, j) *= factor(icol) for j < n2.                                          */

@@ -1,0 +1,693 @@
+// MLP gas-optics surrogate: training and inference kernels (BF16 MFMA, FP32 accumulation).
+// No counterpart in the reference; the network and its targets come from the plain gas optics.
+//
+// Network, per (column, layer): x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o], normalised
+// with the training set's mean and standard deviation, then
+//     h1 = softplus(W1 x + b1),  h2 = softplus(W2 h1 + b2),  y = W3 h2 + b3      (hidden 128)
+// and outputs denormalised with per-output mean/std.  Outputs are interleaved per g-point:
+//     LW row 2g = ln(tau_g / col_dry), row 2g+1 = Planck fraction_g           (no = 2 ngpt)
+//     SW row 2g = ln(tau_abs_g / col_dry), row 2g+1 = ln(tau_ray_g / col_dry)
+// Weights are row-major (out, in) in BF16; FP32 master copies live with the optimiser.
+//
+// Matrix cores: v_mfma_f32_32x32x16_bf16.  For D(32x32) += A(32x16) B(16x32), lane l holds
+// A[l%32][8(l/32) .. 8(l/32)+7] and B[8(l/32) .. +7][l%32] (8 BF16 each) and D rows
+// 8(r/4) + 4(l/32) + r%4 (register r = 0..15) of column l%32.
+#include <algorithm>
+#include <cmath>
+
+#include "common.h"
+
+namespace rrtmgp_hip
+{
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int MLP_H = 128;     // hidden width
+constexpr int MLP_IN = 7;      // inputs
+constexpr int MLP_INP = 16;    // inputs padded to one MFMA K step
+
+__device__ __forceinline__ float softplus(float z) { return z > 20.f ? z : log1pf(expf(z)); }
+__device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + expf(-z)); }
+
+__device__ __forceinline__ int mfma_row(int r, int lane) { return 8 * (r / 4) + 4 * (lane / 32) + (r % 4); }
+
+// ---------------------------------------------------------------------------------------------
+// Training GEMM: C[m][n] = sum_k A(m, k) B(k, n) (+ C if accumulate), with strided BF16 operands
+// (A(m, k) = A[m sAm + k sAk], B(k, n) = B[k sBk + n sBn]) and an FP32 result C[m sCm + n sCn].
+// Work-group: 64x64 tile of C, 4 wavefronts of 32x32; K in steps of 16 through LDS.  Bounds are
+// checked (out-of-range operands read as zero).
+constexpr int GT = 64, GK = 16;
+
+__global__ __launch_bounds__(256)
+void gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, const __bf16* __restrict__ A, int64_t sAm, int64_t sAk,
+                      const __bf16* __restrict__ B, int64_t sBk, int64_t sBn, float* __restrict__ C,
+                      int64_t sCm, int64_t sCn, int accumulate)
+{
+    __shared__ __bf16 sA[GT][GK + 8];     // [m][k]
+    __shared__ __bf16 sB[GT][GK + 8];     // [n][k]
+    const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+    const int64_t m0 = int64_t(blockIdx.y) * GT, n0 = int64_t(blockIdx.x) * GT;
+    const int wm = (w / 2) * 32, wn = (w % 2) * 32;
+    f32x16 acc = {};
+    for (int64_t k0 = 0; k0 < K; k0 += GK)
+    {
+        // 64 x 16 elements of each operand, 4 per thread.
+        for (int e = tid; e < GT * GK; e += 256)
+        {
+            const int r = e / GK, c = e % GK;
+            const int64_t gm = m0 + r, gn = n0 + r, gk = k0 + c;
+            sA[r][c] = (gm < M && gk < K) ? A[gm * sAm + gk * sAk] : __bf16(0.f);
+            sB[r][c] = (gn < N && gk < K) ? B[gk * sBk + gn * sBn] : __bf16(0.f);
+        }
+        __syncthreads();
+        bf16x8 a, b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+        {
+            a[e] = sA[wm + lane % 32][8 * (lane / 32) + e];
+            b[e] = sB[wn + lane % 32][8 * (lane / 32) + e];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+    {
+        const int64_t gm = m0 + wm + mfma_row(r, lane), gn = n0 + wn + lane % 32;
+        if (gm < M && gn < N)
+        {
+            float* c = C + gm * sCm + gn * sCn;
+            *c = accumulate ? *c + acc[r] : acc[r];
+        }
+    }
+}
+
+// Z (rows x n, FP32) += bias[n]; H = softplus(Z) as BF16 (both row-major, ld = n).
+__global__ void bias_softplus_kernel(int64_t rows, int n, float* __restrict__ Z, const float* __restrict__ bias,
+                                     __bf16* __restrict__ H)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows * n) return;
+    const float z = Z[i] + bias[i % n];
+    Z[i] = z;
+    H[i] = __bf16(softplus(z));
+}
+
+// Y += bias; dY = 2 (Y - T) / (rows n) (mean squared error) as BF16; loss partial sums per block.
+__global__ void mse_grad_kernel(int64_t rows, int n, float* __restrict__ Y, const float* __restrict__ bias,
+                                const float* __restrict__ T, __bf16* __restrict__ dY, float* __restrict__ dYf,
+                                float* __restrict__ loss_parts)
+{
+    __shared__ float red[256];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < rows * n)
+    {
+        const float y = Y[i] + bias[i % n];
+        Y[i] = y;
+        const float d = y - T[i];
+        l = d * d;
+        const float g = 2.f * d / float(rows * n);
+        dY[i] = __bf16(g);
+        dYf[i] = g;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss_parts[blockIdx.x] = red[0];
+}
+
+// dZ = dH * softplus'(Z) = dH * sigmoid(Z); FP32 dH in, BF16 and FP32 dZ out.
+__global__ void softplus_back_kernel(int64_t n, const float* __restrict__ dH, const float* __restrict__ Z,
+                                     __bf16* __restrict__ dZ, float* __restrict__ dZf)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = dH[i] * sigmoid(Z[i]);
+    dZ[i] = __bf16(g);
+    dZf[i] = g;
+}
+
+// db[j] = sum over rows of dZ[r][j] (row-major, ld = n); one thread per column j.
+__global__ void bias_grad_kernel(int64_t rows, int n, const float* __restrict__ dZ, float* __restrict__ db)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float s = 0.f;
+    for (int64_t r = 0; r < rows; ++r) s += dZ[r * n + j];
+    db[j] = s;
+}
+
+// Adam on FP32 master weights; writes the BF16 copy (Kingma & Ba 2015, bias-corrected).
+__global__ void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, __bf16* __restrict__ p16, float lr, float b1, float b2, float eps,
+                            float c1, float c2)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (p16) p16[i] = __bf16(pi);
+}
+
+// FP32 -> BF16.
+__global__ void to_bf16_kernel(int64_t n, const float* __restrict__ x, __bf16* __restrict__ y)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = __bf16(x[i]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fused inference.  A work-group takes 64 consecutive columns of one layer:
+//   1. lane = pair: normalised inputs and layer 1 (FP32 VALU, 7 inputs), softplus -> LDS h1;
+//   2. layer 2 on MFMA (D = W2 h1^T: 4 row tiles x 2 pair tiles), softplus -> LDS h2;
+//   3. layer 3 on MFMA (D = W3 h2^T: rows = interleaved outputs, columns = pairs), whose
+//      epilogue turns each (g-point, pair) couple of rows into the gas-optics outputs and stores
+//      them with 32 consecutive columns per half-wavefront.
+// A pair outside the training envelope (any raw input beyond its range) marks its 64-column
+// tile in tile_flags; the plain gas optics then recomputes those tiles.
+struct MlpNet
+{
+    const __bf16* W1;        // (128, 16): 7 inputs, zero padding
+    const __bf16* W2;        // (128, 128)
+    const __bf16* W3;        // (no, 128)
+    const float* b1;
+    const float* b2;
+    const float* b3;
+    const float* x_mean;     // (7)
+    const float* x_std;
+    const float* x_lo;       // training envelope (raw inputs)
+    const float* x_hi;
+    const float* y_mean;     // (no)
+    const float* y_std;
+    int no;
+};
+
+// The inputs' gases in network order: h2o, o3, co2, ch4, n2o (vmr with broadcast strides).
+struct MlpGases
+{
+    const float* p[5];
+    int64_t scol[5], slay[5];
+};
+
+template<bool LW>
+__global__ __launch_bounds__(256)
+void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int nbnd, const int* __restrict__ gpt2band,
+                           const float* __restrict__ play, const float* __restrict__ tlay,
+                           const float* __restrict__ tlev, const float* __restrict__ tsfc,
+                           const float* __restrict__ col_dry, MlpGases gases,
+                           const float* __restrict__ totplnk, int nplanck, float temp_min, float dplanck,
+                           int sfc_lay, const float* __restrict__ ctau, const float* __restrict__ cssa,
+                           const float* __restrict__ cg, float* __restrict__ out0, float* __restrict__ out1,
+                           float* __restrict__ out2, float* __restrict__ out3, float* __restrict__ sfc_src,
+                           float* __restrict__ sfc_jac, int* __restrict__ tile_flags)
+{
+    constexpr int P = 64;
+    __shared__ __bf16 h1[P][MLP_H + 8];
+    __shared__ __bf16 h2[P][MLP_H + 8];
+    __shared__ float sB[LW ? 4 : 1][16][P];        // LW: Planck values per (band, pair)
+    __shared__ float s_cd[P];
+    __shared__ int s_out;
+    const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+    const int64_t ntile = (ncol + P - 1) / P;
+    const int64_t u = blockIdx.x;
+    if (u >= ntile * nlay) return;
+    const int64_t ilay = u / ntile, col0 = (u % ntile) * P;
+    const int64_t ncl = ncol * nlay;
+    const bool at_sfc = LW && (ilay == sfc_lay - 1);
+    if (tid == 0) s_out = 0;
+    __syncthreads();
+
+    // 1. Inputs and layer 1 (wavefront 0 computes the inputs; all waves the hidden units).
+    __shared__ float xs[P][MLP_IN];
+    if (w == 0)
+    {
+        const int64_t icol = col0 + lane;
+        const int64_t idx = icol + ncol * ilay;
+        float x[MLP_IN] = {};
+        bool out = false;
+        if (icol < ncol)
+        {
+            x[0] = tlay[idx];
+            x[1] = logf(play[idx]);
+#pragma unroll
+            for (int gsel = 0; gsel < 5; ++gsel)
+                x[2 + gsel] = logf(gases.p[gsel][icol * gases.scol[gsel] + ilay * gases.slay[gsel]]);
+#pragma unroll
+            for (int i = 0; i < MLP_IN; ++i) out |= !(x[i] >= net.x_lo[i] && x[i] <= net.x_hi[i]);
+            s_cd[lane] = col_dry[idx];
+        }
+        else
+            s_cd[lane] = 0.f;
+        // BF16 inputs, as the training GEMM sees them.
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) xs[lane][i] = float(__bf16((x[i] - net.x_mean[i]) / net.x_std[i]));
+        if (out) atomicOr(&s_out, 1);
+        if constexpr (LW)
+        {
+            // Planck values of the pair for every band (interpolate1D over totplnk).
+            auto B = [&](float t, int b) {
+                const float v0 = (t - temp_min) / dplanck;
+                const float fr = v0 - float(int(v0));
+                const int id = min(nplanck - 1, max(1, int(v0) + 1));
+                const float* tab = totplnk + int64_t(b) * nplanck;
+                return tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+            };
+            for (int b = 0; b < nbnd; ++b)
+            {
+                float bl = 0.f, bh = 0.f, bd = 0.f, bs = 0.f;
+                if (icol < ncol)
+                {
+                    bl = B(tlay[idx], b);
+                    bh = B(tlev[icol + ncol * (ilay + 1)], b);
+                    bd = B(tlev[idx], b);
+                    if (at_sfc) bs = B(tsfc[icol], b);
+                }
+                sB[0][b][lane] = bl; sB[1][b][lane] = bh; sB[2][b][lane] = bd;
+                sB[3][b][lane] = bs;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < P * MLP_H; e += 256)
+    {
+        const int p = e / MLP_H, j = e % MLP_H;
+        float z = net.b1[j];
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) z += float(net.W1[j * MLP_INP + i]) * xs[p][i];
+        h1[p][j] = __bf16(softplus(z));
+    }
+    __syncthreads();
+
+    // 2. Layer 2: D (128 x 64) = W2 (128 x 128) h1^T; wave w takes row tile w, both pair tiles.
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+    {
+        f32x16 acc = {};
+#pragma unroll
+        for (int k0 = 0; k0 < MLP_H; k0 += 16)
+        {
+            bf16x8 a, b;
+            const __bf16* wr = net.W2 + (w * 32 + lane % 32) * MLP_H + k0 + 8 * (lane / 32);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+            {
+                a[e] = wr[e];
+                b[e] = h1[pt * 32 + lane % 32][k0 + 8 * (lane / 32) + e];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+        {
+            const int j = w * 32 + mfma_row(r, lane);
+            h2[pt * 32 + lane % 32][j] = __bf16(softplus(acc[r] + net.b2[j]));
+        }
+    }
+    __syncthreads();
+
+    // 3. Layer 3 and the epilogue: row tiles of 32 outputs (16 g-points) spread over the waves.
+    const int ntiles = net.no / 32;
+    const int64_t icol_base = col0;
+    for (int t = w; t < ntiles * 2; t += 4)
+    {
+        const int rt = t / 2, pt = t % 2;
+        f32x16 acc = {};
+#pragma unroll
+        for (int k0 = 0; k0 < MLP_H; k0 += 16)
+        {
+            bf16x8 a, b;
+            const __bf16* wr = net.W3 + int64_t(rt * 32 + lane % 32) * MLP_H + k0 + 8 * (lane / 32);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+            {
+                a[e] = wr[e];
+                b[e] = h2[pt * 32 + lane % 32][k0 + 8 * (lane / 32) + e];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        const int pl = pt * 32 + lane % 32;
+        const int64_t icol = icol_base + pl;
+        if (icol >= ncol) continue;
+        const int64_t idx = icol + ncol * ilay;
+        const float cd = s_cd[pl];
+#pragma unroll
+        for (int r = 0; r < 16; r += 2)
+        {
+            const int row = rt * 32 + mfma_row(r, lane);          // even: first output of g-point
+            const int g = row / 2;
+            const float y0 = (acc[r] + net.b3[row]) * net.y_std[row] + net.y_mean[row];
+            const float y1 = (acc[r + 1] + net.b3[row + 1]) * net.y_std[row + 1] + net.y_mean[row + 1];
+            const int b = gpt2band[g];
+            const int64_t o = idx + ncl * g;
+            if constexpr (LW)
+            {
+                const float tau = expf(y0) * cd;
+                const float pf = y1;
+                out0[o] = ctau ? tau + ctau[idx + ncl * b] : tau;
+                out1[o] = pf * sB[0][b][pl];
+                out2[o] = pf * sB[1][b][pl];
+                out3[o] = pf * sB[2][b][pl];
+                if (at_sfc)
+                {
+                    // sfc_jac: pf (B(T_sfc + 1) - B(T_sfc)), as the plain gas optics.
+                    const float ts = tsfc[icol];
+                    const float v0 = (ts + 1.f - temp_min) / dplanck;
+                    const float fr = v0 - float(int(v0));
+                    const int id = min(nplanck - 1, max(1, int(v0) + 1));
+                    const float* tab = totplnk + int64_t(b) * nplanck;
+                    const float b1s = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+                    sfc_src[icol + ncol * g] = pf * sB[3][b][pl];
+                    sfc_jac[icol + ncol * g] = pf * (b1s - sB[3][b][pl]);
+                }
+            }
+            else
+            {
+                const float tabs = expf(y0) * cd, tray = expf(y1) * cd;
+                const float tt = tabs + tray;
+                const float eps2 = 2.f * FLT_EPSILON, op_eps = 3.f * FLT_MIN;
+                float ss = tt > eps2 ? tray / tt : 0.f, tt_out = tt, gg = 0.f;
+                if (ctau)
+                {
+                    const int64_t ib = idx + ncl * b;
+                    const float t2 = ctau[ib], s2 = cssa[ib], g2 = cg[ib];
+                    const float tau12 = tt + t2, tauscat = tt * ss + t2 * s2;
+                    gg = (t2 * s2 * g2) / fmaxf(tauscat, op_eps);
+                    ss = tauscat / fmaxf(op_eps, tau12);
+                    tt_out = tau12;
+                }
+                out0[o] = tt_out;
+                out1[o] = ss;
+                out2[o] = gg;
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) tile_flags[u] = s_out;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Training data from the plain gas optics (clear sky).  Row r = (icol, ilay) pair, icol fastest.
+// X (rows, 16): raw inputs, zero padded.  Y (rows, no): interleaved targets.
+__global__ void mlp_inputs_kernel(int64_t ncol, int64_t nlay, const float* __restrict__ play,
+                                  const float* __restrict__ tlay, MlpGases gases, float* __restrict__ X)
+{
+    const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= ncol * nlay) return;
+    const int64_t icol = r % ncol, ilay = r / ncol;
+    float* x = X + r * MLP_INP;
+    x[0] = tlay[r];
+    x[1] = logf(play[r]);
+#pragma unroll
+    for (int g = 0; g < 5; ++g) x[2 + g] = logf(gases.p[g][icol * gases.scol[g] + ilay * gases.slay[g]]);
+#pragma unroll
+    for (int i = MLP_IN; i < MLP_INP; ++i) x[i] = 0.f;
+}
+
+template<bool LW>
+__global__ void mlp_targets_kernel(int64_t ncol, int64_t nlay, int ngpt, const int* __restrict__ gpt2band,
+                                   const float* __restrict__ tlay, const float* __restrict__ col_dry,
+                                   const float* __restrict__ a, const float* __restrict__ b,
+                                   const float* __restrict__ totplnk, int nplanck, float temp_min, float dplanck,
+                                   float* __restrict__ Y)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t ncl = ncol * nlay;
+    if (i >= ncl * ngpt) return;
+    const int64_t r = i % ncl;
+    const int g = int(i / ncl);
+    const float cd = col_dry[r];
+    const float tiny = 1e-30f;
+    float y0, y1;
+    if (LW)
+    {
+        // a = tau, b = lay_src: pf = lay_src / B(T_lay) of the g-point's band.
+        const float v0 = (tlay[r] - temp_min) / dplanck;
+        const float fr = v0 - float(int(v0));
+        const int id = min(nplanck - 1, max(1, int(v0) + 1));
+        const float* tab = totplnk + int64_t(gpt2band[g]) * nplanck;
+        const float B = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+        y0 = logf(fmaxf(a[i], tiny) / cd);
+        y1 = b[i] / B;
+    }
+    else
+    {
+        // a = tau, b = ssa: tau_abs = tau (1 - ssa), tau_ray = tau ssa.
+        y0 = logf(fmaxf(a[i] * (1.f - b[i]), tiny) / cd);
+        y1 = logf(fmaxf(a[i] * b[i], tiny) / cd);
+    }
+    Y[r * (2 * ngpt) + 2 * g] = y0;
+    Y[r * (2 * ngpt) + 2 * g + 1] = y1;
+}
+
+// dst[i] = src[idx[i]] for rows of n FP32 values.
+__global__ void gather_rows_kernel(int64_t rows, int n, const int64_t* __restrict__ idx, const float* __restrict__ src,
+                                   float* __restrict__ dst)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows * n) return;
+    const int64_t r = i / n, c = i % n;
+    dst[i] = src[idx[r] * n + c];
+}
+
+// Row-wise (x - mean) / std over n columns (the first n_norm of each row), FP32 -> BF16.
+__global__ void normalize_rows_kernel(int64_t rows, int n, int n_norm, const float* __restrict__ x,
+                                      const float* __restrict__ mean, const float* __restrict__ std_,
+                                      __bf16* __restrict__ y16, float* __restrict__ y32)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows * n) return;
+    const int c = int(i % n);
+    const float v = c < n_norm ? (x[i] - mean[c]) / std_[c] : 0.f;
+    if (y16) y16[i] = __bf16(v);
+    if (y32) y32[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers.
+int gemm_bf16(int64_t M, int64_t N, int64_t K, const void* A, int64_t sAm, int64_t sAk, const void* B, int64_t sBk,
+              int64_t sBn, float* C, int64_t sCm, int64_t sCn, int accumulate, void* stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return arg_error("mlp_gemm: bad arguments");
+    const dim3 grid(grid1(N, GT), grid1(M, GT));
+    RRTMGP_TIMED("gemm_bf16_kernel", as_stream(stream));
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, as_stream(stream), M, N, K,
+                       static_cast<const __bf16*>(A), sAm, sAk, static_cast<const __bf16*>(B), sBk, sBn, C, sCm, sCn,
+                       accumulate);
+    RRTMGP_LAUNCHED("gemm_bf16_kernel");
+    return RRTMGP_OK;
+}
+}  // namespace rrtmgp_hip
+
+using namespace rrtmgp_hip;
+
+extern "C" {
+
+int rrtmgp_hip_mlp_gemm_bf16(int64_t M, int64_t N, int64_t K, const void* A, int64_t sAm, int64_t sAk, const void* B,
+                             int64_t sBk, int64_t sBn, float* C, int64_t sCm, int64_t sCn, int accumulate, void* stream)
+{
+    return gemm_bf16(M, N, K, A, sAm, sAk, B, sBk, sBn, C, sCm, sCn, accumulate, stream);
+}
+
+int rrtmgp_hip_mlp_bias_softplus(int64_t rows, int n, float* Z, const float* bias, void* H, void* stream)
+{
+    if (rows <= 0 || n <= 0 || !Z || !bias || !H) return arg_error("mlp_bias_softplus: bad arguments");
+    RRTMGP_TIMED("bias_softplus_kernel", as_stream(stream));
+    hipLaunchKernelGGL(bias_softplus_kernel, dim3(grid1(rows * n, 256)), dim3(256), 0, as_stream(stream), rows, n, Z,
+                       bias, static_cast<__bf16*>(H));
+    RRTMGP_LAUNCHED("bias_softplus_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_mse_grad(int64_t rows, int n, float* Y, const float* bias, const float* T, void* dY, float* dYf,
+                            float* loss_parts, void* stream)
+{
+    if (rows <= 0 || n <= 0 || !Y || !bias || !T || !dY || !dYf || !loss_parts)
+        return arg_error("mlp_mse_grad: bad arguments");
+    RRTMGP_TIMED("mse_grad_kernel", as_stream(stream));
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(grid1(rows * n, 256)), dim3(256), 0, as_stream(stream), rows, n, Y, bias,
+                       T, static_cast<__bf16*>(dY), dYf, loss_parts);
+    RRTMGP_LAUNCHED("mse_grad_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_softplus_back(int64_t n, const float* dH, const float* Z, void* dZ, float* dZf, void* stream)
+{
+    if (n <= 0 || !dH || !Z || !dZ || !dZf) return arg_error("mlp_softplus_back: bad arguments");
+    RRTMGP_TIMED("softplus_back_kernel", as_stream(stream));
+    hipLaunchKernelGGL(softplus_back_kernel, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), n, dH, Z,
+                       static_cast<__bf16*>(dZ), dZf);
+    RRTMGP_LAUNCHED("softplus_back_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_bias_grad(int64_t rows, int n, const float* dZ, float* db, void* stream)
+{
+    if (rows <= 0 || n <= 0 || !dZ || !db) return arg_error("mlp_bias_grad: bad arguments");
+    RRTMGP_TIMED("bias_grad_kernel", as_stream(stream));
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(grid1(n, 64)), dim3(64), 0, as_stream(stream), rows, n, dZ, db);
+    RRTMGP_LAUNCHED("bias_grad_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_adam(int64_t n, float* p, const float* g, float* m, float* v, void* p_bf16, float lr, float beta1,
+                        float beta2, float eps, int64_t step, void* stream)
+{
+    if (n <= 0 || !p || !g || !m || !v || step < 1) return arg_error("mlp_adam: bad arguments");
+    const float c1 = 1.f - std::pow(beta1, float(step)), c2 = 1.f - std::pow(beta2, float(step));
+    RRTMGP_TIMED("adam_kernel", as_stream(stream));
+    hipLaunchKernelGGL(adam_kernel, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), n, p, g, m, v,
+                       static_cast<__bf16*>(p_bf16), lr, beta1, beta2, eps, c1, c2);
+    RRTMGP_LAUNCHED("adam_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_to_bf16(int64_t n, const float* x, void* y, void* stream)
+{
+    if (n <= 0 || !x || !y) return arg_error("mlp_to_bf16: bad arguments");
+    RRTMGP_TIMED("to_bf16_kernel", as_stream(stream));
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), n, x,
+                       static_cast<__bf16*>(y));
+    RRTMGP_LAUNCHED("to_bf16_kernel");
+    return RRTMGP_OK;
+}
+
+
+static int make_mlp_net(const rrtmgp_mlp_net* n, MlpNet& net)
+{
+    if (!n || !n->W1 || !n->W2 || !n->W3 || !n->b1 || !n->b2 || !n->b3 || !n->x_mean || !n->x_std || !n->x_lo ||
+        !n->x_hi || !n->y_mean || !n->y_std || n->n_out <= 0 || n->n_out % 32 != 0)
+        return arg_error("mlp: incomplete network (the output count must be a multiple of 32)");
+    net.W1 = static_cast<const __bf16*>(n->W1); net.W2 = static_cast<const __bf16*>(n->W2);
+    net.W3 = static_cast<const __bf16*>(n->W3);
+    net.b1 = n->b1; net.b2 = n->b2; net.b3 = n->b3; net.x_mean = n->x_mean; net.x_std = n->x_std;
+    net.x_lo = n->x_lo; net.x_hi = n->x_hi; net.y_mean = n->y_mean; net.y_std = n->y_std; net.no = int(n->n_out);
+    return RRTMGP_OK;
+}
+
+static int make_mlp_gases(const rrtmgp_gas_vmr* g, int64_t ncol, int64_t nlay, MlpGases& out)
+{
+    if (!g || g->ngas != 5) return arg_error("mlp: expects the 5 input gases h2o, o3, co2, ch4, n2o");
+    for (int i = 0; i < 5; ++i)
+    {
+        const int64_t d1 = g->dim1[i], d2 = g->dim2[i];
+        if (!g->vmr[i] || !((d1 == 1 || d1 == ncol) && (d2 == 1 || d2 == nlay))) return arg_error("mlp: bad gas vmr");
+        out.p[i] = static_cast<const float*>(g->vmr[i]);
+        out.scol[i] = d1 == 1 ? 0 : 1;
+        out.slay[i] = d2 == 1 ? 0 : d1;
+    }
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd,
+                                 const int* gpt2band, const float* play, const float* tlay, const float* tlev,
+                                 const float* tsfc, const float* col_dry, const rrtmgp_gas_vmr* gases,
+                                 const float* totplnk, int64_t nplanck, double temp_min, double dplanck, int sfc_lay,
+                                 const float* cloud_tau, float* tau, float* lay_src, float* lev_src_inc,
+                                 float* lev_src_dec, float* sfc_src, float* sfc_src_jac, int* tile_flags, void* stream)
+{
+    MlpNet net;
+    RRTMGP_TRY(make_mlp_net(n, net));
+    MlpGases g;
+    RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, g));
+    if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || nbnd > 16 || nplanck < 2 || !gpt2band || !play ||
+        !tlay || !tlev || !tsfc || !col_dry || !totplnk || !tau || !lay_src || !lev_src_inc || !lev_src_dec ||
+        !sfc_src || !sfc_src_jac || !tile_flags || (sfc_lay != 1 && sfc_lay != nlay))
+        return arg_error("mlp_gas_optics_lw: bad arguments");
+    const int64_t nunits = ((ncol + 63) / 64) * nlay;
+    RRTMGP_TIMED("mlp_gas_optics_lw_kernel", as_stream(stream));
+    hipLaunchKernelGGL(mlp_gas_optics_kernel<true>, dim3(unsigned(nunits)), dim3(256), 0, as_stream(stream), net,
+                       ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, tlev, tsfc, col_dry, g, totplnk,
+                       int(nplanck), float(temp_min), float(dplanck), sfc_lay, cloud_tau, nullptr, nullptr, tau,
+                       lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac, tile_flags);
+    RRTMGP_LAUNCHED("mlp_gas_optics_lw_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd,
+                                 const int* gpt2band, const float* play, const float* tlay, const float* col_dry,
+                                 const rrtmgp_gas_vmr* gases, const float* cloud_tau, const float* cloud_ssa,
+                                 const float* cloud_g, float* tau, float* ssa, float* g, int* tile_flags, void* stream)
+{
+    MlpNet net;
+    RRTMGP_TRY(make_mlp_net(n, net));
+    MlpGases gs;
+    RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, gs));
+    if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || !gpt2band || !play || !tlay || !col_dry ||
+        !tau || !ssa || !g || !tile_flags || (cloud_tau && (!cloud_ssa || !cloud_g)))
+        return arg_error("mlp_gas_optics_sw: bad arguments");
+    const int64_t nunits = ((ncol + 63) / 64) * nlay;
+    RRTMGP_TIMED("mlp_gas_optics_sw_kernel", as_stream(stream));
+    hipLaunchKernelGGL(mlp_gas_optics_kernel<false>, dim3(unsigned(nunits)), dim3(256), 0, as_stream(stream), net,
+                       ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, nullptr, nullptr, col_dry, gs,
+                       nullptr, 2, 0.f, 1.f, 1, cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, nullptr, nullptr,
+                       nullptr, tile_flags);
+    RRTMGP_LAUNCHED("mlp_gas_optics_sw_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_inputs(int64_t ncol, int64_t nlay, const float* play, const float* tlay,
+                          const rrtmgp_gas_vmr* gases, float* X, void* stream)
+{
+    MlpGases g;
+    RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, g));
+    if (!play || !tlay || !X) return arg_error("mlp_inputs: bad arguments");
+    RRTMGP_TIMED("mlp_inputs_kernel", as_stream(stream));
+    hipLaunchKernelGGL(mlp_inputs_kernel, dim3(grid1(ncol * nlay, 256)), dim3(256), 0, as_stream(stream), ncol,
+                       nlay, play, tlay, g, X);
+    RRTMGP_LAUNCHED("mlp_inputs_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_targets(int lw, int64_t ncol, int64_t nlay, int64_t ngpt, const int* gpt2band, const float* tlay,
+                           const float* col_dry, const float* a, const float* b, const float* totplnk,
+                           int64_t nplanck, double temp_min, double dplanck, float* Y, void* stream)
+{
+    if (ncol <= 0 || nlay <= 0 || ngpt <= 0 || !gpt2band || !tlay || !col_dry || !a || !b || !Y ||
+        (lw && (!totplnk || nplanck < 2)))
+        return arg_error("mlp_targets: bad arguments");
+    const int64_t n = ncol * nlay * ngpt;
+    RRTMGP_TIMED("mlp_targets_kernel", as_stream(stream));
+    if (lw)
+        hipLaunchKernelGGL(mlp_targets_kernel<true>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), ncol, nlay,
+                           int(ngpt), gpt2band, tlay, col_dry, a, b, totplnk, int(nplanck), float(temp_min),
+                           float(dplanck), Y);
+    else
+        hipLaunchKernelGGL(mlp_targets_kernel<false>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), ncol,
+                           nlay, int(ngpt), gpt2band, tlay, col_dry, a, b, nullptr, 2, 0.f, 1.f, Y);
+    RRTMGP_LAUNCHED("mlp_targets_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_gather_rows(int64_t rows, int n, const int64_t* idx, const float* src, float* dst, void* stream)
+{
+    if (rows <= 0 || n <= 0 || !idx || !src || !dst) return arg_error("mlp_gather_rows: bad arguments");
+    RRTMGP_TIMED("gather_rows_kernel", as_stream(stream));
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(grid1(rows * n, 256)), dim3(256), 0, as_stream(stream), rows, n, idx,
+                       src, dst);
+    RRTMGP_LAUNCHED("gather_rows_kernel");
+    return RRTMGP_OK;
+}
+
+int rrtmgp_hip_mlp_normalize_rows(int64_t rows, int n, int n_norm, const float* x, const float* mean,
+                                  const float* std_, void* y_bf16, float* y_f32, void* stream)
+{
+    if (rows <= 0 || n <= 0 || n_norm < 0 || n_norm > n || !x || !mean || !std_ || (!y_bf16 && !y_f32))
+        return arg_error("mlp_normalize_rows: bad arguments");
+    RRTMGP_TIMED("normalize_rows_kernel", as_stream(stream));
+    hipLaunchKernelGGL(normalize_rows_kernel, dim3(grid1(rows * n, 256)), dim3(256), 0, as_stream(stream), rows, n,
+                       n_norm, x, mean, std_, static_cast<__bf16*>(y_bf16), y_f32);
+    RRTMGP_LAUNCHED("normalize_rows_kernel");
+    return RRTMGP_OK;
+}
+
+}  // extern "C"

@@ -200,6 +200,34 @@ __global__ void heating_rate_kernel(int64_t ncol, int64_t nlay, const Real* __re
     hr[i] = Real(g0 / cp * (net_a - net_b) / (double(plev[j]) - double(plev[i])) * sec_per_day);
 }
 
+// Adjoint of heating_rate_kernel: hr_i = c (net_i - net_{i+1}) / (p_{i+1} - p_i), net = dn - up.
+// Adds into g_up, g_dn (ncol, nlev) and d_plev (ncol, nlev); one thread per column walks its
+// layers, so neighbouring layers' updates of a shared level do not race.
+template<typename Real>
+__global__ void heating_rate_adjoint_kernel(int64_t ncol, int64_t nlay, const Real* __restrict__ up,
+                                            const Real* __restrict__ dn, const Real* __restrict__ plev,
+                                            const Real* __restrict__ g_hr, Real* __restrict__ g_up,
+                                            Real* __restrict__ g_dn, Real* __restrict__ d_plev)
+{
+    const int64_t icol = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (icol >= ncol) return;
+    const double c = 9.80665 / 1004.64 * 86400.0;
+    for (int64_t ilay = 0; ilay < nlay; ++ilay)
+    {
+        const int64_t i = icol + ncol * ilay, j = i + ncol;
+        const double net_a = double(dn[i]) - double(up[i]);
+        const double net_b = double(dn[j]) - double(up[j]);
+        const double dp = double(plev[j]) - double(plev[i]);
+        const double g = double(g_hr[i]);
+        const double gn = c * g / dp;                               // d hr / d net_a = -d hr / d net_b
+        const double gp = c * g * (net_a - net_b) / (dp * dp);      // -d hr / d p_j = d hr / d p_i
+        g_dn[i] += Real(gn); g_up[i] -= Real(gn);
+        g_dn[j] -= Real(gn); g_up[j] += Real(gn);
+        d_plev[i] += Real(gp);
+        d_plev[j] -= Real(gp);
+    }
+}
+
 // expand_and_transpose (src_cuda/Rte_lw.cu:36-56): (nbnd, ncol) -> (ncol, ngpt).
 template<typename Real>
 __global__ void expand_kernel(int64_t ncol, int64_t nbnd, int64_t ngpt, const int* __restrict__ bl,
@@ -361,6 +389,20 @@ extern "C" int rrtmgp_hip_heating_rate_##S(int64_t ncol, int64_t nlay, const T* 
     hipLaunchKernelGGL(heating_rate_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0,                   \
                        as_stream(stream), ncol, nlay, flux_up, flux_dn, plev, heating_rate);        \
     RRTMGP_LAUNCHED("heating_rate_kernel");                                                         \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_heating_rate_adjoint_##S(int64_t ncol, int64_t nlay, const T* flux_up,  \
+    const T* flux_dn, const T* plev, const T* g_heating_rate, T* g_flux_up, T* g_flux_dn, T* d_plev, \
+    void* stream)                                                                                    \
+{                                                                                                    \
+    if (ncol <= 0 || nlay <= 0 || !flux_up || !flux_dn || !plev || !g_heating_rate || !g_flux_up ||  \
+        !g_flux_dn || !d_plev)                                                                       \
+        return arg_error("heating_rate_adjoint: bad arguments");                                    \
+    RRTMGP_TIMED("heating_rate_adjoint_kernel", as_stream(stream));                                  \
+    hipLaunchKernelGGL(heating_rate_adjoint_kernel<T>, dim3(grid1(ncol, 256)), dim3(256), 0,        \
+                       as_stream(stream), ncol, nlay, flux_up, flux_dn, plev, g_heating_rate,       \
+                       g_flux_up, g_flux_dn, d_plev);                                               \
+    RRTMGP_LAUNCHED("heating_rate_adjoint_kernel");                                                 \
     return RRTMGP_OK;                                                                                \
 }                                                                                                    \
 extern "C" int rrtmgp_hip_expand_and_transpose_##S(int64_t ncol, int64_t nbnd, int64_t ngpt,      \
