@@ -100,6 +100,13 @@ int rrtmgp_api_kdist_get_int(void* kdist, const char* name, int* out, int64_t ca
 int rrtmgp_api_kdist_get_real(void* kdist, const char* name, double* out, int64_t cap, int64_t* n);
 int rrtmgp_api_kdist_gas_names(void* kdist, char* buf, int len);
 int rrtmgp_api_kdist_set_check_ranges(void* kdist, int on);
+/* MLP gas-optics surrogate (no reference counterpart; FP32 library only): with a network set,  */
+/* every gas optics of this k-distribution, including the solvers', evaluates it and          */
+/* recomputes the units outside its training envelope with the plain kernels.  The network's   */
+/* device arrays stay the caller's; net = NULL restores the plain gas optics.                 */
+/* last_fallback: (64-column tile, layer) units the last surrogate call recomputed.            */
+int rrtmgp_api_kdist_set_surrogate(void* kdist, const rrtmgp_mlp_net* net);
+int rrtmgp_api_kdist_last_fallback(void* kdist, int64_t* n_units);
 
 /* Gas_concs_gpu (include/Gas_concs.h:74-91). */
 int rrtmgp_api_gas_concs_create(const rrtmgp_api_gases* gases, void** gas_concs);

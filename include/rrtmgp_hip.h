@@ -183,6 +183,23 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_LW)
         T* tau, T* ssa, T* g, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_SW)
 
+/* The fused gas optics restricted to listed (64-column tile, layer) pairs: tiles[i] =         */
+/* ilay * ((ncol + 63)/64) + tile, 0-based, for i < ntiles (ntiles = 0 computes nothing);      */
+/* outputs outside the listed pairs are left as they are.  No reference counterpart: the       */
+/* fallback of the MLP surrogate (rrtmgp_hip_mlp_gas_optics_*) for pairs outside its training  */
+/* envelope, whose tile list it writes.                                                        */
+#define RRTMGP_DECL_GAS_OPTICS_TILES(S, T)                                                     \
+    int rrtmgp_hip_gas_optics_lw_tiles_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,          \
+        int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay,          \
+        const T* tlev, const T* tsfc, const T* col_dry, const rrtmgp_gas_vmr* gases,           \
+        int sfc_lay, const T* cloud_tau, T* tau, T* lay_src, T* lev_src_inc, T* lev_src_dec,   \
+        T* sfc_src, T* sfc_src_jac, void* stream);                                             \
+    int rrtmgp_hip_gas_optics_sw_tiles_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,          \
+        int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay,          \
+        const T* col_dry, const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_ssa, \
+        const T* cloud_g, T* tau, T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_TILES)
+
 /* The fused entries above restricted to bands band_start .. band_start+band_count-1 (0-based):*/
 /* the output g-point dimension is that range's g-points only (ngpt_range, counted from the    */
 /* first g-point of band_start); cloud arrays stay (ncol, nlay, nbnd).  No counterpart in the  */
@@ -477,8 +494,10 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_HEATING_ADJ)
 /* two softplus layers of 128 to n_out outputs (denormalised with y_mean/y_std), interleaved  */
 /* per g-point: LW (ln(tau/col_dry), Planck fraction), SW (ln(tau_abs/col_dry),               */
 /* ln(tau_ray/col_dry)).  Weights are row-major (out, in) BF16; W1 is (128, 16) with the 7    */
-/* inputs first.  Inputs outside [x_lo, x_hi] flag their 64-column tile in tile_flags          */
-/* ((ncol/64 rounded up) x nlay, layer-major) for the plain gas optics to recompute.            */
+/* inputs first.  A (64-column tile, layer) unit with any input outside [x_lo, x_hi] goes on   */
+/* the fallback list (1 + ntile nlay ints, ntile = (ncol + 63)/64): fallback[0] = number of     */
+/* listed units, fallback[1 ..] = ilay * ntile + tile, the tile list of                         */
+/* rrtmgp_hip_gas_optics_{lw,sw}_tiles_f32, which recomputes them with the plain gas optics.    */
 /* ------------------------------------------------------------------------------------------ */
 typedef struct
 {
@@ -506,12 +525,12 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* net, int64_t ncol, int64_
                                  const rrtmgp_gas_vmr* gases, const float* totplnk, int64_t nplanck,
                                  double temp_min, double dplanck, int sfc_lay, const float* cloud_tau,
                                  float* tau, float* lay_src, float* lev_src_inc, float* lev_src_dec,
-                                 float* sfc_src, float* sfc_src_jac, int* tile_flags, void* stream);
+                                 float* sfc_src, float* sfc_src_jac, int* fallback, void* stream);
 int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* net, int64_t ncol, int64_t nlay, int64_t ngpt,
                                  int64_t nbnd, const int* gpt2band, const float* play, const float* tlay,
                                  const float* col_dry, const rrtmgp_gas_vmr* gases, const float* cloud_tau,
                                  const float* cloud_ssa, const float* cloud_g, float* tau, float* ssa,
-                                 float* g, int* tile_flags, void* stream);
+                                 float* g, int* fallback, void* stream);
 
 /* Training.  gemm_bf16: C[m sCm + n sCn] (+)= sum_k A[m sAm + k sAk] B[k sBk + n sBn], BF16      */
 /* operands, FP32 result (accumulate != 0 adds to C).  Row-major (rows, n) arrays below.        */

@@ -297,19 +297,26 @@ __device__ __forceinline__ Real minor_scaling_of(const ColOf& col, int gas, bool
 // part of the band), ordered so that consecutive units share a chunk.
 // A launch covers bands b0 .. b0+nbnd-1; its outputs hold their g-points only, counted from the
 // first g-point of band b0.
+// Units are (64-column tile, layer, band chunk) triples, tile fastest.  With a tile list the
+// (tile, layer) pairs are tiles[0 .. nlist-1] (each ilay * ncoltiles + tile) instead of all of them.
 struct GoGrid
 {
     int64_t ncoltiles, nlay, cpb, nunits;
     int b0;
+    const int* tiles;
+    int64_t nlist;
 };
 
-inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t b0, int64_t nbnd, int64_t max_band_gpts)
+inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t b0, int64_t nbnd, int64_t max_band_gpts,
+                           const int* tiles = nullptr, int64_t nlist = 0)
 {
     GoGrid t;
     t.ncoltiles = (ncol + GO_COLS - 1) / GO_COLS;
     t.nlay = nlay;
     t.cpb = std::max<int64_t>(1, (max_band_gpts + GO_GPTS - 1) / GO_GPTS);
-    t.nunits = t.ncoltiles * nlay * nbnd * t.cpb;
+    t.tiles = tiles;
+    t.nlist = tiles ? nlist : 0;
+    t.nunits = (tiles ? nlist : t.ncoltiles * nlay) * nbnd * t.cpb;
     t.b0 = int(b0);
     return t;
 }
@@ -326,10 +333,28 @@ __device__ __forceinline__ GoUnit go_unit(const GoBand* bands, const GoGrid& t, 
     GoUnit r;
     const uint32_t u = uint32_t(u64), nct = uint32_t(t.ncoltiles), nl = uint32_t(t.nlay);
     const uint32_t cpb = uint32_t(t.cpb);
-    const uint32_t rest = u / nct;
-    r.col0 = int64_t(u - rest * nct) * GO_COLS;
-    const uint32_t chunk = rest / nl;
-    r.ilay = rest - chunk * nl;
+    uint32_t chunk;
+    if (t.tiles)
+    {
+        const uint32_t nls = uint32_t(t.nlist);
+        chunk = u / nls;
+        const uint32_t id = uint32_t(t.tiles[u - chunk * nls]);
+        const uint32_t ilay = id / nct;
+        r.col0 = int64_t(id - ilay * nct) * GO_COLS;
+        r.ilay = ilay;
+        if (ilay >= nl)                                  // not a tile of this grid: no work
+        {
+            r.ibnd = t.b0; r.c0 = 0; r.ncg = 0; r.ilay = 0;
+            return r;
+        }
+    }
+    else
+    {
+        const uint32_t rest = u / nct;
+        r.col0 = int64_t(u - rest * nct) * GO_COLS;
+        chunk = rest / nl;
+        r.ilay = rest - chunk * nl;
+    }
     const int cb = int(chunk / cpb);
     r.c0 = GO_GPTS * int(chunk - uint32_t(cb) * cpb);
     r.ibnd = t.b0 + cb;
@@ -1539,7 +1564,7 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
                   const Real* tlay, const Real* tlev, const Real* tsfc, const Real* col_dry,
                   const rrtmgp_gas_vmr* gases, int sfc_lay, const Real* cloud_tau, Real* tau,
                   Real* lay_src, Real* lev_inc, Real* lev_dec, Real* sfc_src, Real* sfc_jac,
-                  void* stream)
+                  void* stream, const int* tiles = nullptr, int64_t ntiles = 0)
 {
     RRTMGP_TRY(check_kd_fused(kd, true));
     if (ncol <= 0 || nlay <= 0) return arg_error("ncol and nlay must be positive");
@@ -1550,7 +1575,10 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
     const Kd<Real> k = make_kd<Real>(*kd);
     if (b0 < 0 || nb < 1 || b0 + nb > kd->nbnd) return arg_error("gas_optics_lw: band range outside the k-distribution");
-    const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts);
+    if (ntiles < 0 || ntiles > ((ncol + GO_COLS - 1) / GO_COLS) * nlay || (ntiles > 0 && !tiles))
+        return arg_error("fused gas optics: bad tile list");
+    if (tiles && ntiles == 0) return RRTMGP_OK;                 // nothing listed
+    const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts, tiles, ntiles);
     RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
@@ -1564,7 +1592,7 @@ template<typename Real>
 int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t ncol, int64_t nlay, const Real* play,
                   const Real* tlay, const Real* col_dry, const rrtmgp_gas_vmr* gases,
                   const Real* ctau, const Real* cssa, const Real* cg, Real* tau, Real* ssa, Real* g,
-                  void* stream)
+                  void* stream, const int* tiles = nullptr, int64_t ntiles = 0)
 {
     RRTMGP_TRY(check_kd_fused(kd, false));
     if (ncol <= 0 || nlay <= 0) return arg_error("ncol and nlay must be positive");
@@ -1574,7 +1602,10 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, gs));
     const Kd<Real> k = make_kd<Real>(*kd);
     if (b0 < 0 || nb < 1 || b0 + nb > kd->nbnd) return arg_error("gas_optics_sw: band range outside the k-distribution");
-    const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts);
+    if (ntiles < 0 || ntiles > ((ncol + GO_COLS - 1) / GO_COLS) * nlay || (ntiles > 0 && !tiles))
+        return arg_error("fused gas optics: bad tile list");
+    if (tiles && ntiles == 0) return RRTMGP_OK;                 // nothing listed
+    const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts, tiles, ntiles);
     RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
@@ -1697,6 +1728,25 @@ extern "C" int rrtmgp_hip_gas_optics_sw_bands_##S(const rrtmgp_kdist_dev* kd, in
 {                                                                                                  \
     return gas_optics_sw<T>(kd, band_start, band_count, ncol, nlay, play, tlay, col_dry, gases,    \
                             cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, stream);                   \
+}                                                                                                  \
+extern "C" int rrtmgp_hip_gas_optics_lw_tiles_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,      \
+    int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay, const T* tlev,    \
+    const T* tsfc, const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau,  \
+    T* tau, T* lay_src, T* lev_src_inc, T* lev_src_dec, T* sfc_src, T* sfc_src_jac, void* stream)   \
+{                                                                                                  \
+    if (!tiles) return arg_error("gas_optics_lw_tiles: NULL tile list");                          \
+    return gas_optics_lw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, tlev, tsfc, col_dry, \
+                            gases, sfc_lay, cloud_tau, tau, lay_src, lev_src_inc, lev_src_dec,     \
+                            sfc_src, sfc_src_jac, stream, tiles, ntiles);                          \
+}                                                                                                  \
+extern "C" int rrtmgp_hip_gas_optics_sw_tiles_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,      \
+    int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay, const T* col_dry, \
+    const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_ssa, const T* cloud_g, T* tau,  \
+    T* ssa, T* g, void* stream)                                                                     \
+{                                                                                                  \
+    if (!tiles) return arg_error("gas_optics_sw_tiles: NULL tile list");                          \
+    return gas_optics_sw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, col_dry, gases,      \
+                            cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, stream, tiles, ntiles);    \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_interpolation_##S(const rrtmgp_kdist_dev* kd, int64_t ncol, int64_t nlay, \
     const T* play, const T* tlay, const T* col_gas, int* jtemp, T* fmajor, T* fminor, T* col_mix,  \

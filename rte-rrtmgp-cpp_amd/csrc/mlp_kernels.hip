@@ -173,8 +173,10 @@ __global__ void to_bf16_kernel(int64_t n, const float* __restrict__ x, __bf16* _
 //   3. layer 3 on MFMA (D = W3 h2^T: rows = interleaved outputs, columns = pairs), whose
 //      epilogue turns each (g-point, pair) couple of rows into the gas-optics outputs and stores
 //      them with 32 consecutive columns per half-wavefront.
-// A pair outside the training envelope (any raw input beyond its range) marks its 64-column
-// tile in tile_flags; the plain gas optics then recomputes those tiles.
+// A pair outside the training envelope (any raw input beyond its range) puts its (64-column
+// tile, layer) unit on the fallback list: fallback[0] counts the listed units (zeroed by the
+// launcher) and fallback[1 ..] holds them as ilay * ntile + tile, in no particular order, for
+// the plain gas optics to recompute.
 struct MlpNet
 {
     const __bf16* W1;        // (128, 16): 7 inputs, zero padding
@@ -209,7 +211,7 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                            int sfc_lay, const float* __restrict__ ctau, const float* __restrict__ cssa,
                            const float* __restrict__ cg, float* __restrict__ out0, float* __restrict__ out1,
                            float* __restrict__ out2, float* __restrict__ out3, float* __restrict__ sfc_src,
-                           float* __restrict__ sfc_jac, int* __restrict__ tile_flags)
+                           float* __restrict__ sfc_jac, int* __restrict__ fallback)
 {
     constexpr int P = 64;
     __shared__ __bf16 h1[P][MLP_H + 8];
@@ -391,9 +393,14 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
             }
         }
     }
-    __syncthreads();
-    if (tid == 0) tile_flags[u] = s_out;
+    if (tid == 0 && s_out)
+    {
+        const int pos = atomicAdd(fallback, 1);
+        fallback[1 + pos] = int(u);
+    }
 }
+
+__global__ void zero_int_kernel(int* p) { *p = 0; }
 
 // ---------------------------------------------------------------------------------------------
 // Training data from the plain gas optics (clear sky).  Row r = (icol, ilay) pair, icol fastest.
@@ -593,7 +600,7 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
                                  const float* tsfc, const float* col_dry, const rrtmgp_gas_vmr* gases,
                                  const float* totplnk, int64_t nplanck, double temp_min, double dplanck, int sfc_lay,
                                  const float* cloud_tau, float* tau, float* lay_src, float* lev_src_inc,
-                                 float* lev_src_dec, float* sfc_src, float* sfc_src_jac, int* tile_flags, void* stream)
+                                 float* lev_src_dec, float* sfc_src, float* sfc_src_jac, int* fallback, void* stream)
 {
     MlpNet net;
     RRTMGP_TRY(make_mlp_net(n, net));
@@ -601,14 +608,16 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
     RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, g));
     if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || nbnd > 16 || nplanck < 2 || !gpt2band || !play ||
         !tlay || !tlev || !tsfc || !col_dry || !totplnk || !tau || !lay_src || !lev_src_inc || !lev_src_dec ||
-        !sfc_src || !sfc_src_jac || !tile_flags || (sfc_lay != 1 && sfc_lay != nlay))
+        !sfc_src || !sfc_src_jac || !fallback || (sfc_lay != 1 && sfc_lay != nlay))
         return arg_error("mlp_gas_optics_lw: bad arguments");
     const int64_t nunits = ((ncol + 63) / 64) * nlay;
+    if (nunits >= (int64_t(1) << 31)) return arg_error("mlp_gas_optics_lw: too many column tiles");
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, as_stream(stream), fallback);
     RRTMGP_TIMED("mlp_gas_optics_lw_kernel", as_stream(stream));
     hipLaunchKernelGGL(mlp_gas_optics_kernel<true>, dim3(unsigned(nunits)), dim3(256), 0, as_stream(stream), net,
                        ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, tlev, tsfc, col_dry, g, totplnk,
                        int(nplanck), float(temp_min), float(dplanck), sfc_lay, cloud_tau, nullptr, nullptr, tau,
-                       lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac, tile_flags);
+                       lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac, fallback);
     RRTMGP_LAUNCHED("mlp_gas_optics_lw_kernel");
     return RRTMGP_OK;
 }
@@ -616,21 +625,23 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
 int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd,
                                  const int* gpt2band, const float* play, const float* tlay, const float* col_dry,
                                  const rrtmgp_gas_vmr* gases, const float* cloud_tau, const float* cloud_ssa,
-                                 const float* cloud_g, float* tau, float* ssa, float* g, int* tile_flags, void* stream)
+                                 const float* cloud_g, float* tau, float* ssa, float* g, int* fallback, void* stream)
 {
     MlpNet net;
     RRTMGP_TRY(make_mlp_net(n, net));
     MlpGases gs;
     RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, gs));
     if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || !gpt2band || !play || !tlay || !col_dry ||
-        !tau || !ssa || !g || !tile_flags || (cloud_tau && (!cloud_ssa || !cloud_g)))
+        !tau || !ssa || !g || !fallback || (cloud_tau && (!cloud_ssa || !cloud_g)))
         return arg_error("mlp_gas_optics_sw: bad arguments");
     const int64_t nunits = ((ncol + 63) / 64) * nlay;
+    if (nunits >= (int64_t(1) << 31)) return arg_error("mlp_gas_optics_sw: too many column tiles");
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, as_stream(stream), fallback);
     RRTMGP_TIMED("mlp_gas_optics_sw_kernel", as_stream(stream));
     hipLaunchKernelGGL(mlp_gas_optics_kernel<false>, dim3(unsigned(nunits)), dim3(256), 0, as_stream(stream), net,
                        ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, nullptr, nullptr, col_dry, gs,
                        nullptr, 2, 0.f, 1.f, 1, cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, nullptr, nullptr,
-                       nullptr, tile_flags);
+                       nullptr, fallback);
     RRTMGP_LAUNCHED("mlp_gas_optics_sw_kernel");
     return RRTMGP_OK;
 }

@@ -168,7 +168,25 @@ class Gas_optics_rrtmgp_gpu : public Gas_optics_gpu
         const Array<Float,3>& get_kminor(bool lower) const { return lower ? kminor_lower : kminor_upper; }
         const Array<Float,4>& get_kmajor() const { return kmajor; }
 
+        // MLP gas-optics surrogate (FP32 builds; no reference counterpart).  With a network set,
+        // gas_optics / gas_optics_with_clouds evaluate it (rrtmgp_hip_mlp_gas_optics_*) and
+        // recompute the (64-column tile, layer) units outside its training envelope with the plain
+        // kernels.  The network's device arrays belong to the caller and must outlive its use;
+        // nullptr restores the plain gas optics.  Throws std::invalid_argument in FP64 builds or
+        // when the output count is not 2 ngpt.
+        void set_surrogate(const rrtmgp_mlp_net* net);
+        bool has_surrogate() const { return use_mlp; }
+        // (tile, layer) units the last surrogate call recomputed with the plain kernels.
+        int64_t last_fallback_units() const { return n_fallback; }
+
     private:
+        bool use_mlp = false;
+        rrtmgp_mlp_net mlp{};
+        Array_gpu<int,1> gpt2band_gpu;      // 0-based band of each g-point
+        Array_gpu<int,1> fallback_gpu;      // count + list of (tile, layer) units
+        int64_t n_fallback = 0;
+        int64_t run_fallback_list(int ncol, int nlay);
+
         Array<Float,2> totplnk;
         Array<Float,4> planck_frac;
         Float totplnk_delta = 0;

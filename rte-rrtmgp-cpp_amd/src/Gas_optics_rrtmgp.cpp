@@ -7,11 +7,48 @@
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
+#include <string>
+#include <type_traits>
 
 #include "Gas_concs.h"
 #include "Optical_props.h"
 #include "Source_functions.h"
 #include "hip_dispatch.h"
+
+namespace
+{
+// The surrogate's entry points are FP32 only; set_surrogate rejects FP64 builds before any call.
+template<typename T> struct Mlp
+{
+    template<typename... A> static int lw(A...) { throw std::invalid_argument("MLP surrogate: FP32 only"); }
+    template<typename... A> static int sw(A...) { throw std::invalid_argument("MLP surrogate: FP32 only"); }
+};
+template<> struct Mlp<float>
+{
+    static constexpr auto lw = rrtmgp_hip_mlp_gas_optics_lw;
+    static constexpr auto sw = rrtmgp_hip_mlp_gas_optics_sw;
+};
+
+// The surrogate's input gases, in its order (h2o, o3, co2, ch4, n2o), from the gas concentrations.
+rrtmgp_gas_vmr mlp_vmr_desc(const Gas_concs_gpu& gas_desc, int ncol, int nlay)
+{
+    static const char* names[] = {"h2o", "o3", "co2", "ch4", "n2o"};
+    rrtmgp_gas_vmr out{};
+    out.ngas = 5;
+    for (int i = 0; i < 5; ++i)
+    {
+        if (!gas_desc.exists(names[i]))
+            throw std::runtime_error(std::string("MLP surrogate: gas ") + names[i] + " missing from gas_desc");
+        const Array_gpu<Float,2>& v = gas_desc.get_vmr(names[i]);
+        if ((v.dim(1) != 1 && v.dim(1) != ncol) || (v.dim(2) != 1 && v.dim(2) != nlay))
+            throw std::runtime_error(std::string("gas ") + names[i] + " has dims incompatible with (ncol, nlay)");
+        out.vmr[i] = v.ptr();
+        out.dim1[i] = v.dim(1);
+        out.dim2[i] = v.dim(2);
+    }
+    return out;
+}
+}  // namespace
 
 using rrtmgp_cpp::hip_check;
 using rrtmgp_cpp::stream;
@@ -509,6 +546,27 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
     const int sfc_lay = play({1, 1}) > play({1, nlay}) ? 1 : nlay;
     if (cloud && cloud->get_ngpt() != get_nband())
         throw std::runtime_error("Cannot add optical properties with incompatible band - gpoint combination");
+    if (use_mlp)
+    {
+        const int need = 1 + (ncol + 63) / 64 * nlay;
+        if (fallback_gpu.size() < need) fallback_gpu = Array_gpu<int,1>({need});
+        const rrtmgp_gas_vmr mv = mlp_vmr_desc(gas_desc, ncol, nlay);
+        hip_check(Mlp<Float>::lw(
+                &mlp, ncol, nlay, get_ngpt(), get_nband(), gpt2band_gpu.ptr(), play.ptr(), tlay.ptr(),
+                tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &mv, static_cast<const Float*>(dev.totplnk), dev.nplanck,
+                dev.temp_ref_min, dev.totplnk_delta, sfc_lay, cloud ? cloud->get_tau().ptr() : nullptr,
+                props.get_tau().ptr(), sources.get_lay_source().ptr(), sources.get_lev_source_inc().ptr(),
+                sources.get_lev_source_dec().ptr(), sources.get_sfc_source().ptr(),
+                sources.get_sfc_source_jac().ptr(), fallback_gpu.ptr(), stream()));
+        if (run_fallback_list(ncol, nlay) > 0)
+            hip_check(K<Float>::gas_optics_lw_tiles(
+                    &dev, ncol, nlay, fallback_gpu.ptr() + 1, n_fallback, play.ptr(), tlay.ptr(), tlev.ptr(),
+                    tsfc.ptr(), col_dry.ptr(), &v, sfc_lay, cloud ? cloud->get_tau().ptr() : nullptr,
+                    props.get_tau().ptr(), sources.get_lay_source().ptr(), sources.get_lev_source_inc().ptr(),
+                    sources.get_lev_source_dec().ptr(), sources.get_sfc_source().ptr(),
+                    sources.get_sfc_source_jac().ptr(), stream()));
+        return;
+    }
     hip_check(K<Float>::gas_optics_lw(&dev, ncol, nlay, play.ptr(), tlay.ptr(), tlev.ptr(), tsfc.ptr(),
                                       col_dry.ptr(), &v, sfc_lay, cloud ? cloud->get_tau().ptr() : nullptr,
                                       props.get_tau().ptr(), sources.get_lay_source().ptr(),
@@ -530,10 +588,64 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
     make_vmr_desc(gas_desc, ncol, nlay, v);
     if (cloud && cloud->get_ngpt() != get_nband())
         throw std::runtime_error("Cannot add optical properties with incompatible band - gpoint combination");
-    hip_check(K<Float>::gas_optics_sw(&dev, ncol, nlay, play.ptr(), tlay.ptr(), col_dry.ptr(), &v,
-                                      cloud ? cloud->get_tau().ptr() : nullptr,
-                                      cloud ? cloud->get_ssa().ptr() : nullptr,
-                                      cloud ? cloud->get_g().ptr() : nullptr,
-                                      props.get_tau().ptr(), props.get_ssa().ptr(), props.get_g().ptr(), stream()));
+    const Float* ct = cloud ? cloud->get_tau().ptr() : nullptr;
+    const Float* cs = cloud ? cloud->get_ssa().ptr() : nullptr;
+    const Float* cg = cloud ? cloud->get_g().ptr() : nullptr;
+    if (use_mlp)
+    {
+        const int need = 1 + (ncol + 63) / 64 * nlay;
+        if (fallback_gpu.size() < need) fallback_gpu = Array_gpu<int,1>({need});
+        const rrtmgp_gas_vmr mv = mlp_vmr_desc(gas_desc, ncol, nlay);
+        hip_check(Mlp<Float>::sw(&mlp, ncol, nlay, get_ngpt(), get_nband(), gpt2band_gpu.ptr(), play.ptr(),
+                                 tlay.ptr(), col_dry.ptr(), &mv, ct, cs, cg, props.get_tau().ptr(),
+                                 props.get_ssa().ptr(), props.get_g().ptr(), fallback_gpu.ptr(), stream()));
+        if (run_fallback_list(ncol, nlay) > 0)
+            hip_check(K<Float>::gas_optics_sw_tiles(&dev, ncol, nlay, fallback_gpu.ptr() + 1, n_fallback,
+                                                    play.ptr(), tlay.ptr(), col_dry.ptr(), &v, ct, cs, cg,
+                                                    props.get_tau().ptr(), props.get_ssa().ptr(),
+                                                    props.get_g().ptr(), stream()));
+    }
+    else
+        hip_check(K<Float>::gas_optics_sw(&dev, ncol, nlay, play.ptr(), tlay.ptr(), col_dry.ptr(), &v, ct, cs, cg,
+                                          props.get_tau().ptr(), props.get_ssa().ptr(), props.get_g().ptr(),
+                                          stream()));
     hip_check(K<Float>::spread_col(ncol, get_ngpt(), solar_source_gpu.ptr(), toa_src.ptr(), stream()));
+}
+
+
+void Gas_optics_rrtmgp_gpu::set_surrogate(const rrtmgp_mlp_net* net)
+{
+    if (!net)
+    {
+        use_mlp = false;
+        return;
+    }
+    if (!std::is_same<Float, float>::value)
+        throw std::invalid_argument("the MLP gas-optics surrogate runs in FP32 builds only");
+    if (net->n_out != 2 * int64_t(get_ngpt()))
+        throw std::invalid_argument("MLP surrogate: the output count must be 2 ngpt");
+    if (source_is_internal() && dev.nbnd > 16)
+        throw std::invalid_argument("MLP surrogate: at most 16 longwave bands");
+    // 0-based band of each g-point, from band_lims_gpt (1-based, inclusive).
+    const Array<int,2>& lims = get_band_lims_gpoint();
+    Array<int,1> g2b({get_ngpt()});
+    for (int b = 1; b <= get_nband(); ++b)
+        for (int g = lims({1, b}); g <= lims({2, b}); ++g) g2b({g}) = b - 1;
+    gpt2band_gpu = Array_gpu<int,1>(g2b);
+    mlp = *net;
+    use_mlp = true;
+}
+
+
+// Reads the number of (tile, layer) units the surrogate listed for the plain kernels (a 4-byte
+// device-to-host copy on the library stream) and sizes the list for the next call.
+int64_t Gas_optics_rrtmgp_gpu::run_fallback_list(const int ncol, const int nlay)
+{
+    int count = 0;
+    hip_check(rrtmgp_hip_memcpy_d2h(&count, fallback_gpu.ptr(), sizeof(int), stream()));
+    hip_check(rrtmgp_hip_stream_synchronize(stream()));
+    const int64_t units = int64_t((ncol + 63) / 64) * nlay;
+    if (count < 0 || count > units) throw std::runtime_error("MLP surrogate: corrupt fallback list");
+    n_fallback = count;
+    return n_fallback;
 }

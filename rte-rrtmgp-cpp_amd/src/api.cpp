@@ -221,6 +221,16 @@ int rrtmgp_api_kdist_set_check_ranges(void* h, int on)
     return guarded([&] { K_(h).k->set_check_ranges(on != 0); });
 }
 
+int rrtmgp_api_kdist_set_surrogate(void* h, const rrtmgp_mlp_net* net)
+{
+    return guarded([&] { K_(h).k->set_surrogate(net); });
+}
+
+int rrtmgp_api_kdist_last_fallback(void* h, int64_t* n)
+{
+    return guarded([&] { *n = K_(h).k->last_fallback_units(); });
+}
+
 int rrtmgp_api_gas_concs_create(const rrtmgp_api_gases* g, void** out)
 {
     return guarded([&] {

@@ -1,0 +1,334 @@
+"""MLP gas-optics surrogate: matrix-core GEMM, training gradients, fused inference, fallback and the
+flux error of a trained network.
+
+The checks: the BF16 GEMM against a float64 product of the same BF16-rounded operands; one training
+step's gradients against a float64 numpy MLP that rounds to BF16 where the kernels do; the fused
+inference kernel against the same numpy network on real atmosphere inputs; the fallback list
+against the pairs outside the envelope, with those units equal to the plain gas optics bit for bit;
+training lowering the loss; and the fluxes of a trained surrogate against the plain path's.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from conftest import AVAILABLE  # noqa: E402
+
+
+def bf16_bits(x):
+    """float32 -> BF16 bit patterns (round to nearest even), as uint16."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return r.astype(np.uint16)
+
+
+def bf16(x):
+    """float32 values rounded to BF16 (returned as float64)."""
+    b = bf16_bits(x).astype(np.uint32) << 16
+    return b.view(np.float32).astype(np.float64)
+
+
+def softplus(z):
+    return np.where(z > 20.0, z, np.log1p(np.exp(np.minimum(z, 20.0))))
+
+
+@pytest.fixture(scope="module")
+def S():
+    from rrtmgp_amd import surrogate
+    return surrogate
+
+
+@pytest.fixture(scope="module")
+def L():
+    from rrtmgp_amd import _lib
+    return _lib.hip()
+
+
+@pytest.fixture(scope="module")
+def D():
+    from rrtmgp_amd.device import DeviceArray
+    return DeviceArray
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(64, 64, 16, False, False), (100, 70, 40, True, False),
+                                         (33, 129, 300, False, True), (128, 512, 128, True, True)])
+def test_gemm_bf16(L, D, M, N, K, ta, tb):
+    rng = np.random.default_rng(M * 7 + N)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    # Stored transposed when ta/tb: element (m, k) at [k M + m].
+    Ah = A.T.copy() if ta else A
+    Bh = B.T.copy() if tb else B
+    dA = D.from_host(bf16_bits(Ah).ravel(), np.uint16)
+    dB = D.from_host(bf16_bits(Bh).ravel(), np.uint16)
+    dC = D((M * N,), np.float32)
+    sAm, sAk = (1, M) if ta else (K, 1)
+    sBk, sBn = (1, K) if tb else (N, 1)
+    i = C.c_int64
+    rc = L.rrtmgp_hip_mlp_gemm_bf16(i(M), i(N), i(K), dA.ptr, i(sAm), i(sAk), dB.ptr, i(sBk), i(sBn), dC.ptr,
+                                    i(N), i(1), C.c_int(0), None)
+    assert rc == 0
+    got = dC.to_host().reshape(M, N)
+    ref = bf16(A) @ bf16(B)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+    # accumulate = 1 adds to C
+    rc = L.rrtmgp_hip_mlp_gemm_bf16(i(M), i(N), i(K), dA.ptr, i(sAm), i(sAk), dB.ptr, i(sBk), i(sBn), dC.ptr,
+                                    i(N), i(1), C.c_int(1), None)
+    assert rc == 0
+    np.testing.assert_allclose(dC.to_host().reshape(M, N), 2 * ref, rtol=1e-5, atol=2e-5 * np.abs(ref).max())
+
+
+@pytest.fixture(scope="module")
+def kd32_lw(kd_lw):
+    from rrtmgp_amd import api
+    return api.KDist(kd_lw, AVAILABLE, precision=4)
+
+
+@pytest.fixture(scope="module")
+def kd32_sw(kd_sw):
+    from rrtmgp_amd import api
+    return api.KDist(kd_sw, AVAILABLE, precision=4)
+
+
+@pytest.fixture(scope="module")
+def atm_train(syn):
+    return [syn.make_atmosphere(500 + s, 512, 60) for s in range(4)]
+
+
+@pytest.fixture(scope="module")
+def ts_lw(S, kd32_lw, atm_train):
+    return S.build_training_set(kd32_lw, atm_train, seed=3)
+
+
+@pytest.fixture(scope="module")
+def ts_sw(S, kd32_sw, atm_train):
+    return S.build_training_set(kd32_sw, atm_train, seed=4)
+
+
+def _np_forward(w, xn16):
+    """float64 network with the kernels' BF16 rounding points; xn16 are BF16-rounded inputs (B, 16)."""
+    W1, W2, W3 = (bf16(w[k]) for k in ("W1", "W2", "W3"))
+    z1 = xn16 @ W1.T + w["b1"]
+    h1 = bf16(softplus(z1).astype(np.float32))
+    z2 = h1 @ W2.T + w["b2"]
+    h2 = bf16(softplus(z2).astype(np.float32))
+    y = h2 @ W3.T + w["b3"]
+    return z1, h1, z2, h2, y
+
+
+def test_training_targets(S, kd32_lw, ts_lw, atm_train):
+    """Targets of the first atmosphere against the plain gas optics outputs they come from."""
+    X = ts_lw.X.to_host().T
+    Y = ts_lw.Y.to_host().T
+    atm = atm_train[0]
+    ncol, nlay = atm.p_lay.shape
+    # Inputs: row r = icol + ncol ilay.
+    t = np.asarray(atm.t_lay, np.float32).ravel(order="F")
+    p = np.asarray(atm.p_lay, np.float32).ravel(order="F")
+    np.testing.assert_array_equal(X[: ncol * nlay, 0], t)
+    np.testing.assert_allclose(X[: ncol * nlay, 1], np.log(p), rtol=1e-6)
+    assert np.all(X[:, 7:] == 0)
+    # Planck fractions of a band sum to one over its g-points.
+    pf = Y[: ncol * nlay, 1::2]
+    lims = kd32_lw.src.band2gpt
+    for b in range(kd32_lw.nbnd):
+        s = pf[:, lims[0, b] - 1:lims[1, b]].sum(axis=1)
+        np.testing.assert_allclose(s, 1.0, rtol=2e-5)
+    assert np.all(np.isfinite(Y))
+
+
+def test_training_gradients(S, kd32_lw, ts_lw):
+    """One forward/backward pass against the numpy MLP (same BF16 rounding points)."""
+    net = S.MlpGasOptics(kd32_lw, seed=5)
+    net.fit_normalisation(ts_lw)
+    rng = np.random.default_rng(6)
+    # Non-zero biases so that their gradients and uses are exercised.
+    for k, n in (("b1", 128), ("b2", 128), ("b3", net.n_out)):
+        getattr(net, k).p.free()
+        from rrtmgp_amd.device import DeviceArray
+        getattr(net, k).p = DeviceArray.from_host(rng.normal(0, 0.1, n).astype(np.float32), np.float32)
+    idx = rng.integers(0, ts_lw.rows, 192)
+    w = net.host_weights()
+    loss = net.forward_backward(ts_lw, idx)
+    got = {k: getattr(net, k).g.to_host().reshape(getattr(net, k).shape) for k in ("W1", "b1", "W2", "b2", "W3", "b3")}
+    hs = net.host_stats
+    X = ts_lw.X.to_host().T[idx]
+    Y = ts_lw.Y.to_host().T[idx]
+    xn = np.zeros((len(idx), 16), np.float32)
+    xn[:, :7] = ((X[:, :7] - hs["x_mean"].astype(np.float32)) / hs["x_std"].astype(np.float32))
+    xn = bf16(xn)
+    yn = (Y - hs["y_mean"].astype(np.float32)) / hs["y_std"].astype(np.float32)
+    z1, h1, z2, h2, y = _np_forward(w, xn)
+    n = y.size
+    dy = 2.0 * (y - yn) / n
+    np.testing.assert_allclose(float(loss.to_host().sum()) / n, np.mean((y - yn) ** 2), rtol=2e-3)
+    dy16 = bf16(dy.astype(np.float32))
+    ref = {"W3": dy16.T @ h2, "b3": dy.sum(0)}
+    dh2 = dy16 @ bf16(w["W3"])
+    dz2 = dh2 / (1.0 + np.exp(-z2))
+    dz2_16 = bf16(dz2.astype(np.float32))
+    ref["W2"] = dz2_16.T @ h1
+    ref["b2"] = dz2.sum(0)
+    dh1 = dz2_16 @ bf16(w["W2"])
+    dz1 = dh1 / (1.0 + np.exp(-z1))
+    dz1_16 = bf16(dz1.astype(np.float32))
+    ref["W1"] = dz1_16.T @ xn
+    ref["b1"] = dz1.sum(0)
+    for k in ref:
+        scale = np.abs(ref[k]).max()
+        err = np.abs(got[k] - ref[k]).max() / scale
+        assert err < 2e-2, (k, err)
+        # Direction: the gradients agree to better than 1e-3 in the aggregate.
+        rel = np.linalg.norm(got[k] - ref[k]) / np.linalg.norm(ref[k])
+        assert rel < 5e-3, (k, rel)
+    assert np.all(got["W1"][:, 7:] == 0)
+
+
+def _eval_plain(api, kd, atm, vmr):
+    from rrtmgp_amd.device import DeviceArray
+    gc = api.GasConcs(vmr, 4)
+    cd = api.get_col_dry(vmr["h2o"], atm.p_lev, 4)
+    return api.gas_optics(kd, atm, gc, cd), cd
+
+
+def _dev_vmr(atm, S, scale=None):
+    from rrtmgp_amd.device import DeviceArray
+    ncol, nlay = atm.p_lay.shape
+    out = {}
+    for g in S.GASES:
+        v = np.broadcast_to(np.asarray(atm.vmr[g], np.float64), (ncol, nlay)).copy(order="F")
+        if scale is not None and g in scale:
+            v *= scale[g]
+        out[g] = DeviceArray.from_host(v, np.float32)
+    return out
+
+
+@pytest.mark.parametrize("lw", [True, False])
+def test_inference_matches_numpy(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw, lw):
+    """The fused kernel (through the k-distribution) against the numpy network on one atmosphere."""
+    from rrtmgp_amd import api
+    kd, ts = (kd32_lw, ts_lw) if lw else (kd32_sw, ts_sw)
+    net = S.MlpGasOptics(kd, seed=7)
+    net.fit_normalisation(ts)
+    net.train(ts, steps=50, batch=1024)               # weights away from their initial draw
+    atm = syn.make_atmosphere(777, 200, 60)
+    ncol, nlay = atm.p_lay.shape
+    vmr = _dev_vmr(atm, S)
+    plain, cd = _eval_plain(api, kd, atm, vmr)
+    S.set_surrogate(kd, net)
+    try:
+        o, _ = _eval_plain(api, kd, atm, vmr)
+        nfb = S.last_fallback_units(kd)
+    finally:
+        S.set_surrogate(kd, None)
+    assert nfb == 0                                   # inside the training envelope
+    w = net.host_weights()
+    hs = net.host_stats
+    x = np.stack([atm.t_lay.ravel(order="F"), np.log(atm.p_lay.ravel(order="F"))] +
+                 [np.log(np.broadcast_to(atm.vmr[g], (ncol, nlay)).ravel(order="F")) for g in S.GASES],
+                 axis=1).astype(np.float32)
+    xn = np.zeros((x.shape[0], 16), np.float32)
+    xn[:, :7] = (x - hs["x_mean"].astype(np.float32)) / hs["x_std"].astype(np.float32)
+    y = _np_forward(w, bf16(xn))[4] * hs["y_std"] + hs["y_mean"]       # (pairs, 2 ngpt)
+    cdh = cd.to_host().ravel(order="F")[:, None]
+    tau = o["tau"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
+    if lw:
+        np.testing.assert_allclose(np.log(tau / cdh), y[:, 0::2], atol=2e-3 * np.abs(y[:, 0::2]).max())
+        pf_plain = plain["lay_src"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
+        src = o["lay_src"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
+        # lay_src = pf B_band(T_lay): the ratio to the plain source is pf_mlp / pf_plain.
+        np.testing.assert_allclose(src, y[:, 1::2] * _planck(kd, atm.t_lay.ravel(order="F")),
+                                   rtol=2e-3, atol=2e-3 * np.abs(pf_plain).max())
+    else:
+        ssa = o["ssa"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
+        tabs = np.exp(y[:, 0::2]) * cdh
+        tray = np.exp(y[:, 1::2]) * cdh
+        np.testing.assert_allclose(tau, tabs + tray, rtol=5e-3)
+        np.testing.assert_allclose(ssa, tray / (tabs + tray), rtol=5e-3, atol=1e-6)
+        assert np.all(o["g"].to_host() == 0)
+
+
+def _planck(kd, t):
+    """B_band(T) of each g-point's band for temperatures t (pairs), float64."""
+    tp = np.asarray(kd.src.totplnk, np.float32).astype(np.float64)
+    tmin, tmax = kd.scalar("temp_ref_min"), kd.scalar("temp_ref_max")
+    n = tp.shape[0]
+    d = (tmax - tmin) / (n - 1)
+    v0 = (t - tmin) / d
+    fr = v0 - np.floor(v0)
+    i = np.clip(np.floor(v0).astype(int) + 1, 1, n - 1)
+    B = tp[i - 1] + fr[:, None] * (tp[i] - tp[i - 1])               # (pairs, nbnd)
+    lims = kd.src.band2gpt
+    g2b = np.zeros(kd.ngpt, int)
+    for b in range(kd.nbnd):
+        g2b[lims[0, b] - 1:lims[1, b]] = b
+    return B[:, g2b]
+
+
+@pytest.mark.parametrize("lw", [True, False])
+def test_fallback_tiles(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw, lw):
+    """Pairs outside the envelope: their (tile, layer) units equal the plain gas optics exactly."""
+    from rrtmgp_amd import api
+    kd, ts = (kd32_lw, ts_lw) if lw else (kd32_sw, ts_sw)
+    net = S.MlpGasOptics(kd, seed=8)
+    net.fit_normalisation(ts)
+    atm = syn.make_atmosphere(778, 300, 60)           # 5 column tiles, the last one partial
+    ncol, nlay = atm.p_lay.shape
+    # Ozone far above the envelope in a few pairs.
+    scale = np.ones((ncol, nlay))
+    hot = [(5, 3), (70, 3), (299, 59), (130, 20), (131, 20)]
+    for c, l in hot:
+        scale[c, l] = 1e3
+    vmr = _dev_vmr(atm, S, {"o3": scale})
+    plain, _ = _eval_plain(api, kd, atm, vmr)
+    S.set_surrogate(kd, net)
+    try:
+        o, _ = _eval_plain(api, kd, atm, vmr)
+        nfb = S.last_fallback_units(kd)
+    finally:
+        S.set_surrogate(kd, None)
+    units = {(c // 64, l) for c, l in hot}
+    assert nfb == len(units)
+    names = ("tau", "lay_src", "lev_src_inc", "lev_src_dec") if lw else ("tau", "ssa", "g")
+    for k in names:
+        a = o[k].to_host()
+        b = plain[k].to_host()
+        for (tile, l) in units:
+            cs = slice(64 * tile, min(ncol, 64 * tile + 64))
+            np.testing.assert_array_equal(a[cs, l], b[cs, l], err_msg=f"{k} tile {tile} layer {l}")
+        # Elsewhere the surrogate's own values (not the plain ones).
+        assert not np.array_equal(a[:, 10], b[:, 10])
+
+
+def test_training_and_flux_error(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw):
+    """Training lowers the loss; the trained surrogate's fluxes stay close to the plain path's."""
+    from rrtmgp_amd import api
+    res = {}
+    for lw, kd, ts in ((True, kd32_lw, ts_lw), (False, kd32_sw, ts_sw)):
+        net = S.MlpGasOptics(kd, seed=9)
+        net.fit_normalisation(ts)
+        hold = np.random.default_rng(10).integers(0, ts.rows, 8192)
+        l0 = net.loss(ts, hold)
+        net.train(ts, steps=1500, batch=4096, lr=2e-3, lr_final=1e-4)
+        l1 = net.loss(ts, hold)
+        assert l1 < 0.05 * l0, (lw, l0, l1)
+        atm = syn.make_atmosphere(901, 256, 60)
+        da = api.DeviceAtmosphere(atm, precision=4, cloudy=False)
+        solver = api.SolverLW(kd) if lw else api.SolverSW(kd)
+        tsi = np.ones(atm.ncol)
+        ref = solver.solve(da) if lw else solver.solve(da, tsi)
+        ref = {k: ref[k].to_host() for k in ("flux_up", "flux_dn")}
+        S.set_surrogate(kd, net)
+        try:
+            got = solver.solve(da) if lw else solver.solve(da, tsi)
+            got = {k: got[k].to_host() for k in ("flux_up", "flux_dn")}
+        finally:
+            S.set_surrogate(kd, None)
+        for k in ("flux_up", "flux_dn"):
+            rmse = float(np.sqrt(np.mean((got[k] - ref[k]) ** 2)))
+            res[(lw, k)] = (rmse, float(np.abs(ref[k]).max()))
+    print("surrogate flux RMSE [W/m2] (rmse, max |flux|):", res)
+    for (lw, k), (rmse, fmax) in res.items():
+        assert rmse < 0.02 * fmax, ((lw, k), rmse, fmax)
