@@ -4,7 +4,7 @@
  * This is the thin layer the C++ class API (Gas_optics_rrtmgp_gpu, Cloud_optics_gpu,
  * Rte_lw_gpu, Rte_sw_gpu, Fluxes_*_gpu) calls, and the surface an FFI binds.  It replaces the
  * reference's two internal kernel boundaries:
- *   - the CUDA launcher namespaces of include_kernels_cuda/*.h (device pointers), and
+ *   - the CUDA launcher namespaces of the include_kernels_cuda headers (device pointers), and
  *   - the Fortran C-ABI of include/rrtmgp_kernels.h:31-200 (CPU path).
  * Conventions (all entry points):
  *   - arrays are device pointers in the reference's Fortran layouts (first index fastest,

@@ -25,8 +25,21 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int MLP_H = 128;     // hidden width
 constexpr int MLP_IN = 7;      // inputs
 constexpr int MLP_INP = 16;    // inputs padded to one MFMA K step
+constexpr float MLP_LN_FLOOR = -100.f;   // ln(tau / col_dry) target where tau = 0
 
 __device__ __forceinline__ float softplus(float z) { return z > 20.f ? z : log1pf(expf(z)); }
+// max(z, 0) + ln(1 + e^-|z|) on the hardware exp2/log2 (absolute error ~1e-7, far below the BF16
+// rounding of the activations it feeds).
+__device__ __forceinline__ float softplus_fast(float z)
+{
+    return fmaxf(z, 0.f) + __logf(1.f + __expf(-fabsf(z)));
+}
+
+// Store at a wave-uniform base plus a 32-bit lane offset (scalar base, one VGPR of offset).
+__device__ __forceinline__ void store_at(float* base, uint32_t i, float v)
+{
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(float))) = v;
+}
 __device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + expf(-z)); }
 
 __device__ __forceinline__ int mfma_row(int r, int lane) { return 8 * (r / 4) + 4 * (lane / 32) + (r % 4); }
@@ -38,10 +51,13 @@ __device__ __forceinline__ int mfma_row(int r, int lane) { return 8 * (r / 4) + 
 // checked (out-of-range operands read as zero).
 constexpr int GT = 64, GK = 16;
 
+// With split-K (gridDim.z > 1) work-group z covers K range [z kchunk, (z + 1) kchunk) and writes its
+// partial tile to C + z M N (row-major, no accumulate); sum_split_kernel then adds the partials in
+// order of z, so the result does not depend on scheduling.
 __global__ __launch_bounds__(256)
 void gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, const __bf16* __restrict__ A, int64_t sAm, int64_t sAk,
                       const __bf16* __restrict__ B, int64_t sBk, int64_t sBn, float* __restrict__ C,
-                      int64_t sCm, int64_t sCn, int accumulate)
+                      int64_t sCm, int64_t sCn, int accumulate, int64_t kchunk)
 {
     __shared__ __bf16 sA[GT][GK + 8];     // [m][k]
     __shared__ __bf16 sB[GT][GK + 8];     // [n][k]
@@ -49,15 +65,23 @@ void gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, const __bf16* __restrict_
     const int64_t m0 = int64_t(blockIdx.y) * GT, n0 = int64_t(blockIdx.x) * GT;
     const int wm = (w / 2) * 32, wn = (w % 2) * 32;
     f32x16 acc = {};
-    for (int64_t k0 = 0; k0 < K; k0 += GK)
+    const int64_t kb = int64_t(blockIdx.z) * kchunk, ke = std::min<int64_t>(K, kb + kchunk);
+    if (gridDim.z > 1)
+    {
+        C += int64_t(blockIdx.z) * M * N;
+        sCm = N;
+        sCn = 1;
+        accumulate = 0;
+    }
+    for (int64_t k0 = kb; k0 < ke; k0 += GK)
     {
         // 64 x 16 elements of each operand, 4 per thread.
         for (int e = tid; e < GT * GK; e += 256)
         {
             const int r = e / GK, c = e % GK;
             const int64_t gm = m0 + r, gn = n0 + r, gk = k0 + c;
-            sA[r][c] = (gm < M && gk < K) ? A[gm * sAm + gk * sAk] : __bf16(0.f);
-            sB[r][c] = (gn < N && gk < K) ? B[gk * sBk + gn * sBn] : __bf16(0.f);
+            sA[r][c] = (gm < M && gk < ke) ? A[gm * sAm + gk * sAk] : __bf16(0.f);
+            sB[r][c] = (gn < N && gk < ke) ? B[gk * sBk + gn * sBn] : __bf16(0.f);
         }
         __syncthreads();
         bf16x8 a, b;
@@ -80,6 +104,18 @@ void gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, const __bf16* __restrict_
             *c = accumulate ? *c + acc[r] : acc[r];
         }
     }
+}
+
+// C[m sCm + n sCn] (+)= sum over z of P[z][m][n], z in order.
+__global__ void sum_split_kernel(int64_t M, int64_t N, int nsplit, const float* __restrict__ P, float* __restrict__ C,
+                                 int64_t sCm, int64_t sCn, int accumulate)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= M * N) return;
+    float s = 0.f;
+    for (int z = 0; z < nsplit; ++z) s += P[int64_t(z) * M * N + i];
+    float* c = C + (i / N) * sCm + (i % N) * sCn;
+    *c = accumulate ? *c + s : s;
 }
 
 // Z (rows x n, FP32) += bias[n]; H = softplus(Z) as BF16 (both row-major, ld = n).
@@ -132,14 +168,22 @@ __global__ void softplus_back_kernel(int64_t n, const float* __restrict__ dH, co
     dZf[i] = g;
 }
 
-// db[j] = sum over rows of dZ[r][j] (row-major, ld = n); one thread per column j.
-__global__ void bias_grad_kernel(int64_t rows, int n, const float* __restrict__ dZ, float* __restrict__ db)
+// Partial column sums of dZ (rows x n, row-major): work-group (x, z) sums rows [z chunk, (z + 1)
+// chunk) of columns [64 x, 64 x + 64), four row lanes per column combined in LDS in fixed order;
+// P[z][j] receives the partial (summed over z by sum_split_kernel).
+__global__ __launch_bounds__(256)
+void bias_partial_kernel(int64_t rows, int n, int64_t chunk, const float* __restrict__ dZ, float* __restrict__ P)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+    __shared__ float red[4][64];
+    const int c = threadIdx.x % 64, q = threadIdx.x / 64;
+    const int j = blockIdx.x * 64 + c;
+    const int64_t r0 = int64_t(blockIdx.z) * chunk, r1 = std::min<int64_t>(rows, r0 + chunk);
     float s = 0.f;
-    for (int64_t r = 0; r < rows; ++r) s += dZ[r * n + j];
-    db[j] = s;
+    if (j < n)
+        for (int64_t r = r0 + q; r < r1; r += 4) s += dZ[r * n + j];
+    red[q][c] = s;
+    __syncthreads();
+    if (q == 0 && j < n) P[int64_t(blockIdx.z) * n + j] = ((red[0][c] + red[1][c]) + red[2][c]) + red[3][c];
 }
 
 // Adam on FP32 master weights; writes the BF16 copy (Kingma & Ba 2015, bias-corrected).
@@ -167,12 +211,21 @@ __global__ void to_bf16_kernel(int64_t n, const float* __restrict__ x, __bf16* _
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fused inference.  A work-group takes 64 consecutive columns of one layer:
-//   1. lane = pair: normalised inputs and layer 1 (FP32 VALU, 7 inputs), softplus -> LDS h1;
-//   2. layer 2 on MFMA (D = W2 h1^T: 4 row tiles x 2 pair tiles), softplus -> LDS h2;
-//   3. layer 3 on MFMA (D = W3 h2^T: rows = interleaved outputs, columns = pairs), whose
-//      epilogue turns each (g-point, pair) couple of rows into the gas-optics outputs and stores
-//      them with 32 consecutive columns per half-wavefront.
+// Fused inference, weight-stationary.  A persistent work-group of 8 wavefronts loops over units
+// of 64 consecutive columns of one layer.  Each wavefront keeps its share of the weights in
+// registers for the whole launch, as matrix-core A operands: row tile w % 4 of W1 (one K step)
+// and row tiles w and w + 8 of W3 (8 K steps each); W2 is staged in LDS.  Per unit:
+//   0. wavefront 0 turns the unit's inputs (loaded during the previous unit) into normalised BF16
+//      inputs in LDS;
+//   1. h1 = softplus(W1 x + b1): D (128 x 64) = W1 (128 x 16) x^T, wavefront w taking row tile
+//      w % 4 and column tile w / 4; meanwhile (LW) the band Planck values of the layer, its levels
+//      and the surface, bands spread over the wavefronts;
+//   2. h2 = softplus(W2 h1 + b2) with the same split, K = 128;
+//   3. y = W3 h2 (+ b3, denormalised) and the epilogue, which turns each (g-point, column) pair of
+//      rows into the gas-optics outputs, adds the band cloud properties as the plain kernels do,
+//      and stores 32 consecutive columns per half-wavefront; wavefront 0 first issues the next
+//      unit's input loads.
+// Only inputs and outputs touch HBM; the weights are read once per work-group.
 // A pair outside the training envelope (any raw input beyond its range) puts its (64-column
 // tile, layer) unit on the fallback list: fallback[0] counts the listed units (zeroed by the
 // launcher) and fallback[1 ..] holds them as ilay * ntile + tile, in no particular order, for
@@ -201,8 +254,46 @@ struct MlpGases
     int64_t scol[5], slay[5];
 };
 
+constexpr int MLP_P = 64;             // columns per unit
+constexpr int MLP_MAXNO = 512;        // outputs the register-resident W3 can hold
+constexpr int MLP_WAVES = 8;
+constexpr int MLP_RT = MLP_MAXNO / 32 / MLP_WAVES;   // W3 row tiles per wavefront
+
+__device__ __forceinline__ bf16x8 load_bf16x8(const __bf16* p)
+{
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+
+// Raw inputs of one column of a unit (wavefront 0, lane = column).
+struct MlpIn
+{
+    float t, p, v[5], cd, th, td, ts;
+};
+
 template<bool LW>
-__global__ __launch_bounds__(256)
+__device__ __forceinline__ void mlp_load_inputs(MlpIn& in, int64_t u, int64_t ntile, int64_t ncol, int lane,
+                                                const float* __restrict__ play, const float* __restrict__ tlay,
+                                                const float* __restrict__ tlev, const float* __restrict__ tsfc,
+                                                const float* __restrict__ col_dry, const MlpGases& gases)
+{
+    const int64_t ilay = u / ntile, icol = (u % ntile) * MLP_P + lane;
+    if (icol >= ncol) return;
+    const int64_t idx = icol + ncol * ilay;
+    in.t = tlay[idx];
+    in.p = play[idx];
+#pragma unroll
+    for (int g = 0; g < 5; ++g) in.v[g] = gases.p[g][icol * gases.scol[g] + ilay * gases.slay[g]];
+    in.cd = col_dry[idx];
+    if constexpr (LW)
+    {
+        in.th = tlev[icol + ncol * (ilay + 1)];
+        in.td = tlev[idx];
+        in.ts = tsfc[icol];
+    }
+}
+
+template<bool LW>
+__global__ __launch_bounds__(64 * MLP_WAVES) __attribute__((amdgpu_waves_per_eu(2)))
 void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int nbnd, const int* __restrict__ gpt2band,
                            const float* __restrict__ play, const float* __restrict__ tlay,
                            const float* __restrict__ tlev, const float* __restrict__ tsfc,
@@ -213,190 +304,232 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                            float* __restrict__ out2, float* __restrict__ out3, float* __restrict__ sfc_src,
                            float* __restrict__ sfc_jac, int* __restrict__ fallback)
 {
-    constexpr int P = 64;
-    __shared__ __bf16 h1[P][MLP_H + 8];
-    __shared__ __bf16 h2[P][MLP_H + 8];
-    __shared__ float sB[LW ? 4 : 1][16][P];        // LW: Planck values per (band, pair)
+    constexpr int P = MLP_P;
+    __shared__ __align__(16) __bf16 xs[P][MLP_INP + 8];
+    __shared__ __align__(16) __bf16 h1[P][MLP_H + 8];
+    __shared__ __align__(16) __bf16 h2[P][MLP_H + 8];
+    __shared__ __align__(16) __bf16 sW2[MLP_H][MLP_H + 8];
+    __shared__ float sB[LW ? 4 : 1][16][P];         // LW: Planck values per (band, column)
+    __shared__ float sT[LW ? 4 : 1][P];             // LW: T of the layer, upper and lower level, surface
     __shared__ float s_cd[P];
+    __shared__ float s_b1[MLP_H], s_b2[MLP_H];
+    __shared__ float4 s_so[MLP_MAXNO / 2];        // per g-point: (scale, offset) of rows 2g, 2g+1
+    __shared__ int s_g2b[MLP_MAXNO / 2];
     __shared__ int s_out;
-    const int tid = threadIdx.x, w = tid / 64, lane = tid % 64;
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int w = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int no = net.no, nrt = no / 32;
     const int64_t ntile = (ncol + P - 1) / P;
-    const int64_t u = blockIdx.x;
-    if (u >= ntile * nlay) return;
-    const int64_t ilay = u / ntile, col0 = (u % ntile) * P;
+    const int64_t nunits = ntile * nlay;
     const int64_t ncl = ncol * nlay;
-    const bool at_sfc = LW && (ilay == sfc_lay - 1);
-    if (tid == 0) s_out = 0;
-    __syncthreads();
 
-    // 1. Inputs and layer 1 (wavefront 0 computes the inputs; all waves the hidden units).
-    __shared__ float xs[P][MLP_IN];
-    if (w == 0)
+    // Register-resident weights (A operands: lane l holds row l % 32, K half l / 32).
+    const int ar = lane % 32, ak = 8 * (lane / 32);
+    const int rt12 = w % 4, pt12 = w / 4;           // layers 1 and 2: row tile, column tile
+    const bf16x8 A1 = load_bf16x8(net.W1 + (rt12 * 32 + ar) * MLP_INP + ak);
+    for (int e = tid; e < MLP_H * MLP_H / 8; e += 64 * MLP_WAVES)
     {
-        const int64_t icol = col0 + lane;
-        const int64_t idx = icol + ncol * ilay;
-        float x[MLP_IN] = {};
-        bool out = false;
-        if (icol < ncol)
-        {
-            x[0] = tlay[idx];
-            x[1] = logf(play[idx]);
-#pragma unroll
-            for (int gsel = 0; gsel < 5; ++gsel)
-                x[2 + gsel] = logf(gases.p[gsel][icol * gases.scol[gsel] + ilay * gases.slay[gsel]]);
-#pragma unroll
-            for (int i = 0; i < MLP_IN; ++i) out |= !(x[i] >= net.x_lo[i] && x[i] <= net.x_hi[i]);
-            s_cd[lane] = col_dry[idx];
-        }
-        else
-            s_cd[lane] = 0.f;
-        // BF16 inputs, as the training GEMM sees them.
-#pragma unroll
-        for (int i = 0; i < MLP_IN; ++i) xs[lane][i] = float(__bf16((x[i] - net.x_mean[i]) / net.x_std[i]));
-        if (out) atomicOr(&s_out, 1);
-        if constexpr (LW)
-        {
-            // Planck values of the pair for every band (interpolate1D over totplnk).
-            auto B = [&](float t, int b) {
-                const float v0 = (t - temp_min) / dplanck;
-                const float fr = v0 - float(int(v0));
-                const int id = min(nplanck - 1, max(1, int(v0) + 1));
-                const float* tab = totplnk + int64_t(b) * nplanck;
-                return tab[id - 1] + fr * (tab[id] - tab[id - 1]);
-            };
-            for (int b = 0; b < nbnd; ++b)
-            {
-                float bl = 0.f, bh = 0.f, bd = 0.f, bs = 0.f;
-                if (icol < ncol)
-                {
-                    bl = B(tlay[idx], b);
-                    bh = B(tlev[icol + ncol * (ilay + 1)], b);
-                    bd = B(tlev[idx], b);
-                    if (at_sfc) bs = B(tsfc[icol], b);
-                }
-                sB[0][b][lane] = bl; sB[1][b][lane] = bh; sB[2][b][lane] = bd;
-                sB[3][b][lane] = bs;
-            }
-        }
+        const int r = e / (MLP_H / 8), c = 8 * (e % (MLP_H / 8));
+        *reinterpret_cast<bf16x8*>(&sW2[r][c]) = load_bf16x8(net.W2 + r * MLP_H + c);
     }
-    __syncthreads();
-    for (int e = tid; e < P * MLP_H; e += 256)
+    bf16x8 A3[MLP_RT][MLP_H / 16];
+#pragma unroll
+    for (int i = 0; i < MLP_RT; ++i)
     {
-        const int p = e / MLP_H, j = e % MLP_H;
-        float z = net.b1[j];
+        const int rt = w + MLP_WAVES * i;
 #pragma unroll
-        for (int i = 0; i < MLP_IN; ++i) z += float(net.W1[j * MLP_INP + i]) * xs[p][i];
-        h1[p][j] = __bf16(softplus(z));
+        for (int ks = 0; ks < MLP_H / 16; ++ks)
+            A3[i][ks] = rt < nrt ? load_bf16x8(net.W3 + int64_t(rt * 32 + ar) * MLP_H + ks * 16 + ak) : bf16x8{};
     }
-    __syncthreads();
+    for (int e = tid; e < MLP_H; e += 64 * MLP_WAVES) { s_b1[e] = net.b1[e]; s_b2[e] = net.b2[e]; }
+    for (int e = tid; e < no / 2; e += 64 * MLP_WAVES)
+    {
+        const int a = 2 * e, b = 2 * e + 1;
+        s_so[e] = make_float4(net.y_std[a], net.b3[a] * net.y_std[a] + net.y_mean[a],
+                              net.y_std[b], net.b3[b] * net.y_std[b] + net.y_mean[b]);
+    }
+    for (int e = tid; e < no / 2; e += 64 * MLP_WAVES) s_g2b[e] = gpt2band[e];
 
-    // 2. Layer 2: D (128 x 64) = W2 (128 x 128) h1^T; wave w takes row tile w, both pair tiles.
-#pragma unroll
-    for (int pt = 0; pt < 2; ++pt)
-    {
-        f32x16 acc = {};
-#pragma unroll
-        for (int k0 = 0; k0 < MLP_H; k0 += 16)
-        {
-            bf16x8 a, b;
-            const __bf16* wr = net.W2 + (w * 32 + lane % 32) * MLP_H + k0 + 8 * (lane / 32);
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-            {
-                a[e] = wr[e];
-                b[e] = h1[pt * 32 + lane % 32][k0 + 8 * (lane / 32) + e];
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r)
-        {
-            const int j = w * 32 + mfma_row(r, lane);
-            h2[pt * 32 + lane % 32][j] = __bf16(softplus(acc[r] + net.b2[j]));
-        }
-    }
-    __syncthreads();
+    MlpIn in{};
+    if (w == 0 && int64_t(blockIdx.x) < nunits)
+        mlp_load_inputs<LW>(in, blockIdx.x, ntile, ncol, lane, play, tlay, tlev, tsfc, col_dry, gases);
 
-    // 3. Layer 3 and the epilogue: row tiles of 32 outputs (16 g-points) spread over the waves.
-    const int ntiles = net.no / 32;
-    const int64_t icol_base = col0;
-    for (int t = w; t < ntiles * 2; t += 4)
+    for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x)
     {
-        const int rt = t / 2, pt = t % 2;
-        f32x16 acc = {};
-#pragma unroll
-        for (int k0 = 0; k0 < MLP_H; k0 += 16)
+        const int64_t ilay = u / ntile, col0 = (u % ntile) * P;
+        const bool at_sfc = LW && (ilay == sfc_lay - 1);
+        if (tid == 0) s_out = 0;
+        __syncthreads();     // the previous unit is done with the LDS buffers; constants staged
+
+        // 0. Normalised inputs (wavefront 0, lane = column).
+        if (w == 0)
         {
-            bf16x8 a, b;
-            const __bf16* wr = net.W3 + int64_t(rt * 32 + lane % 32) * MLP_H + k0 + 8 * (lane / 32);
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
+            const bool valid = col0 + lane < ncol;
+            float x[MLP_IN] = {};
+            bool out = false;
+            if (valid)
             {
-                a[e] = wr[e];
-                b[e] = h2[pt * 32 + lane % 32][k0 + 8 * (lane / 32) + e];
-            }
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-        }
-        const int pl = pt * 32 + lane % 32;
-        const int64_t icol = icol_base + pl;
-        if (icol >= ncol) continue;
-        const int64_t idx = icol + ncol * ilay;
-        const float cd = s_cd[pl];
+                x[0] = in.t;
+                x[1] = __logf(in.p);
 #pragma unroll
-        for (int r = 0; r < 16; r += 2)
-        {
-            const int row = rt * 32 + mfma_row(r, lane);          // even: first output of g-point
-            const int g = row / 2;
-            const float y0 = (acc[r] + net.b3[row]) * net.y_std[row] + net.y_mean[row];
-            const float y1 = (acc[r + 1] + net.b3[row + 1]) * net.y_std[row + 1] + net.y_mean[row + 1];
-            const int b = gpt2band[g];
-            const int64_t o = idx + ncl * g;
+                for (int g = 0; g < 5; ++g) x[2 + g] = __logf(in.v[g]);
+#pragma unroll
+                for (int i = 0; i < MLP_IN; ++i) out |= !(x[i] >= net.x_lo[i] && x[i] <= net.x_hi[i]);
+            }
+            s_cd[lane] = valid ? in.cd : 0.f;
+#pragma unroll
+            for (int i = 0; i < MLP_INP; ++i)
+                xs[lane][i] = i < MLP_IN ? __bf16((x[i] - net.x_mean[i]) / net.x_std[i]) : __bf16(0.f);
             if constexpr (LW)
             {
-                const float tau = expf(y0) * cd;
-                const float pf = y1;
-                out0[o] = ctau ? tau + ctau[idx + ncl * b] : tau;
-                out1[o] = pf * sB[0][b][pl];
-                out2[o] = pf * sB[1][b][pl];
-                out3[o] = pf * sB[2][b][pl];
-                if (at_sfc)
-                {
-                    // sfc_jac: pf (B(T_sfc + 1) - B(T_sfc)), as the plain gas optics.
-                    const float ts = tsfc[icol];
-                    const float v0 = (ts + 1.f - temp_min) / dplanck;
-                    const float fr = v0 - float(int(v0));
-                    const int id = min(nplanck - 1, max(1, int(v0) + 1));
-                    const float* tab = totplnk + int64_t(b) * nplanck;
-                    const float b1s = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
-                    sfc_src[icol + ncol * g] = pf * sB[3][b][pl];
-                    sfc_jac[icol + ncol * g] = pf * (b1s - sB[3][b][pl]);
-                }
+                sT[0][lane] = valid ? in.t : temp_min;
+                sT[1][lane] = valid ? in.th : temp_min;
+                sT[2][lane] = valid ? in.td : temp_min;
+                sT[3][lane] = valid ? in.ts : temp_min;
             }
-            else
+            if (out) atomicOr(&s_out, 1);
+        }
+        __syncthreads();
+
+        // 1. Layer 1 on the matrix cores (K = 16: the 7 inputs and padding); LW Planck values.
+        {
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&xs[pt12 * 32 + lane % 32][ak]);
+            f32x16 acc = {};
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, b, acc, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
             {
-                const float tabs = expf(y0) * cd, tray = expf(y1) * cd;
-                const float tt = tabs + tray;
-                const float eps2 = 2.f * FLT_EPSILON, op_eps = 3.f * FLT_MIN;
-                float ss = tt > eps2 ? tray / tt : 0.f, tt_out = tt, gg = 0.f;
-                if (ctau)
-                {
-                    const int64_t ib = idx + ncl * b;
-                    const float t2 = ctau[ib], s2 = cssa[ib], g2 = cg[ib];
-                    const float tau12 = tt + t2, tauscat = tt * ss + t2 * s2;
-                    gg = (t2 * s2 * g2) / fmaxf(tauscat, op_eps);
-                    ss = tauscat / fmaxf(op_eps, tau12);
-                    tt_out = tau12;
-                }
-                out0[o] = tt_out;
-                out1[o] = ss;
-                out2[o] = gg;
+                const int j = rt12 * 32 + mfma_row(r, lane);
+                h1[pt12 * 32 + lane % 32][j] = __bf16(softplus_fast(acc[r] + s_b1[j]));
             }
         }
-    }
-    if (tid == 0 && s_out)
-    {
-        const int pos = atomicAdd(fallback, 1);
-        fallback[1 + pos] = int(u);
+        if constexpr (LW)
+        {
+            // interpolate1D over totplnk for the band's Planck values of this column.
+            for (int b = w; b < nbnd; b += MLP_WAVES)
+            {
+                const float* tab = totplnk + int64_t(b) * nplanck;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                {
+                    const float v0 = (sT[q][lane] - temp_min) / dplanck;
+                    const float fr = v0 - float(int(v0));
+                    const int id = min(nplanck - 1, max(1, int(v0) + 1));
+                    sB[q][b][lane] = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+                }
+            }
+        }
+        __syncthreads();
+
+        // 2. Layer 2.
+        {
+            f32x16 acc = {};
+#pragma unroll
+            for (int ks = 0; ks < MLP_H / 16; ++ks)
+            {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sW2[rt12 * 32 + ar][ks * 16 + ak]);
+                const bf16x8 b = *reinterpret_cast<const bf16x8*>(&h1[pt12 * 32 + lane % 32][ks * 16 + ak]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+            {
+                const int j = rt12 * 32 + mfma_row(r, lane);
+                h2[pt12 * 32 + lane % 32][j] = __bf16(softplus_fast(acc[r] + s_b2[j]));
+            }
+        }
+        __syncthreads();
+
+        // Next unit's inputs, in flight during layer 3.
+        if (w == 0 && u + gridDim.x < nunits)
+            mlp_load_inputs<LW>(in, u + gridDim.x, ntile, ncol, lane, play, tlay, tlev, tsfc, col_dry, gases);
+
+        // 3. Layer 3 and the epilogue.  Rows 2g, 2g+1 of D are g-point g; for register r the
+        // g-point is gbase + c(r), gbase = 16 rt + 2 (lane / 32), c(r) = 4 (r / 4) + (r % 4) / 2.
+#pragma unroll 1
+        for (int pt = 0; pt < 2; ++pt)
+        {
+            bf16x8 bop[MLP_H / 16];
+#pragma unroll
+            for (int ks = 0; ks < MLP_H / 16; ++ks)
+                bop[ks] = *reinterpret_cast<const bf16x8*>(&h2[pt * 32 + lane % 32][ks * 16 + ak]);
+            const int pl = pt * 32 + lane % 32;
+            const int64_t icol = col0 + pl;
+            const int64_t idx = icol + ncol * ilay;
+            const float cd = s_cd[pl];
+#pragma unroll
+            for (int i = 0; i < MLP_RT; ++i)
+            {
+                const int rt = w + MLP_WAVES * i;
+                if (rt >= nrt) break;                           // uniform over the wavefront
+                f32x16 acc = {};
+#pragma unroll
+                for (int ks = 0; ks < MLP_H / 16; ++ks)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A3[i][ks], bop[ks], acc, 0, 0, 0);
+                if (icol >= ncol) continue;
+                const int gbase = 16 * rt + 2 * (lane / 32);
+                // Output (column icol, g-point gbase + c): uniform part col0 + ncol ilay + ncl 16 rt
+                // (+ ncl c), lane part pl + 2 ncl (lane / 32), below 2^32 bytes by the host check.
+                const int64_t ubase = col0 + ncol * ilay + ncl * (16 * rt);
+                const uint32_t vofs = uint32_t(pl) + uint32_t(2 * ncl) * uint32_t(lane / 32);
+#pragma unroll
+                for (int r = 0; r < 16; r += 2)
+                {
+                    const int c = 4 * (r / 4) + (r % 4) / 2;
+                    const int g = gbase + c;
+                    const int64_t uo = ubase + ncl * c;                     // uniform
+                    const float4 so = s_so[g];
+                    const float y0 = acc[r] * so.x + so.y;
+                    const float y1 = acc[r + 1] * so.z + so.w;
+                    const int b = s_g2b[g];
+                    if constexpr (LW)
+                    {
+                        const float tau = __expf(y0) * cd;
+                        const float pf = y1;
+                        store_at(out0 + uo, vofs, ctau ? tau + ctau[idx + ncl * b] : tau);
+                        store_at(out1 + uo, vofs, pf * sB[0][b][pl]);
+                        store_at(out2 + uo, vofs, pf * sB[1][b][pl]);
+                        store_at(out3 + uo, vofs, pf * sB[2][b][pl]);
+                        if (at_sfc)
+                        {
+                            // sfc_jac: pf (B(T_sfc + 1) - B(T_sfc)), as the plain gas optics.
+                            const float v0 = (sT[3][pl] + 1.f - temp_min) / dplanck;
+                            const float fr = v0 - float(int(v0));
+                            const int id = min(nplanck - 1, max(1, int(v0) + 1));
+                            const float* tab = totplnk + int64_t(b) * nplanck;
+                            const float b1s = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+                            const int64_t os = icol + ncol * g;
+                            sfc_src[os] = pf * sB[3][b][pl];
+                            sfc_jac[os] = pf * (b1s - sB[3][b][pl]);
+                        }
+                    }
+                    else
+                    {
+                        const float tabs = __expf(y0) * cd, tray = __expf(y1) * cd;
+                        const float tt = tabs + tray;
+                        const float eps2 = 2.f * FLT_EPSILON, op_eps = 3.f * FLT_MIN;
+                        float ss = tt > eps2 ? tray / tt : 0.f, tt_out = tt, gg = 0.f;
+                        if (ctau)
+                        {
+                            const int64_t ib = idx + ncl * b;
+                            const float t2 = ctau[ib], s2 = cssa[ib], g2 = cg[ib];
+                            const float tau12 = tt + t2, tauscat = tt * ss + t2 * s2;
+                            gg = (t2 * s2 * g2) / fmaxf(tauscat, op_eps);
+                            ss = tauscat / fmaxf(op_eps, tau12);
+                            tt_out = tau12;
+                        }
+                        store_at(out0 + uo, vofs, tt_out);
+                        store_at(out1 + uo, vofs, ss);
+                        store_at(out2 + uo, vofs, gg);
+                    }
+                }
+            }
+        }
+        if (tid == 0 && s_out)
+        {
+            const int pos = atomicAdd(fallback, 1);
+            fallback[1 + pos] = int(u);
+        }
     }
 }
 
@@ -432,8 +565,10 @@ __global__ void mlp_targets_kernel(int64_t ncol, int64_t nlay, int ngpt, const i
     if (i >= ncl * ngpt) return;
     const int64_t r = i % ncl;
     const int g = int(i / ncl);
-    const float cd = col_dry[r];
-    const float tiny = 1e-30f;
+    // ln(tau / col_dry) as ln tau - ln col_dry (tau / col_dry can be below the FP32 range), floored
+    // at MLP_LN_FLOOR where tau = 0: a constant the network can learn, independent of col_dry.
+    const float lcd = logf(col_dry[r]);
+    auto lnr = [&](float t) { return t > 0.f ? fmaxf(logf(t) - lcd, MLP_LN_FLOOR) : MLP_LN_FLOOR; };
     float y0, y1;
     if (LW)
     {
@@ -443,14 +578,14 @@ __global__ void mlp_targets_kernel(int64_t ncol, int64_t nlay, int ngpt, const i
         const int id = min(nplanck - 1, max(1, int(v0) + 1));
         const float* tab = totplnk + int64_t(gpt2band[g]) * nplanck;
         const float B = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
-        y0 = logf(fmaxf(a[i], tiny) / cd);
+        y0 = lnr(a[i]);
         y1 = b[i] / B;
     }
     else
     {
         // a = tau, b = ssa: tau_abs = tau (1 - ssa), tau_ray = tau ssa.
-        y0 = logf(fmaxf(a[i] * (1.f - b[i]), tiny) / cd);
-        y1 = logf(fmaxf(a[i] * b[i], tiny) / cd);
+        y0 = lnr(a[i] * (1.f - b[i]));
+        y1 = lnr(a[i] * b[i]);
     }
     Y[r * (2 * ngpt) + 2 * g] = y0;
     Y[r * (2 * ngpt) + 2 * g + 1] = y1;
@@ -485,12 +620,27 @@ int gemm_bf16(int64_t M, int64_t N, int64_t K, const void* A, int64_t sAm, int64
               int64_t sBn, float* C, int64_t sCm, int64_t sCn, int accumulate, void* stream)
 {
     if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return arg_error("mlp_gemm: bad arguments");
-    const dim3 grid(grid1(N, GT), grid1(M, GT));
-    RRTMGP_TIMED("gemm_bf16_kernel", as_stream(stream));
-    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, as_stream(stream), M, N, K,
-                       static_cast<const __bf16*>(A), sAm, sAk, static_cast<const __bf16*>(B), sBk, sBn, C, sCm, sCn,
-                       accumulate);
+    const int64_t tiles = grid1(N, GT) * grid1(M, GT);
+    // Split K when there are few output tiles and a long K (the weight gradients): at least 256 of
+    // K per split, about 256 work-groups in all.
+    int64_t nsplit = 1;
+    if (tiles < 128 && K >= 512) nsplit = std::max<int64_t>(1, std::min<int64_t>(K / 256, 256 / tiles));
+    const int64_t kchunk = ((K + nsplit - 1) / nsplit + GK - 1) / GK * GK;
+    nsplit = (K + kchunk - 1) / kchunk;
+    const hipStream_t s = as_stream(stream);
+    float* part = nullptr;
+    if (nsplit > 1) RRTMGP_TRY(scratch_alloc(&part, size_t(nsplit * M * N), s));
+    const dim3 grid(grid1(N, GT), grid1(M, GT), unsigned(nsplit));
+    RRTMGP_TIMED("gemm_bf16_kernel", s);
+    hipLaunchKernelGGL(gemm_bf16_kernel, grid, dim3(256), 0, s, M, N, K, static_cast<const __bf16*>(A), sAm, sAk,
+                       static_cast<const __bf16*>(B), sBk, sBn, nsplit > 1 ? part : C, sCm, sCn, accumulate, kchunk);
     RRTMGP_LAUNCHED("gemm_bf16_kernel");
+    if (nsplit > 1)
+    {
+        hipLaunchKernelGGL(sum_split_kernel, dim3(grid1(M * N, 256)), dim3(256), 0, s, M, N, int(nsplit), part, C,
+                           sCm, sCn, accumulate);
+        scratch_free(part, s);
+    }
     return RRTMGP_OK;
 }
 }  // namespace rrtmgp_hip
@@ -540,9 +690,18 @@ int rrtmgp_hip_mlp_softplus_back(int64_t n, const float* dH, const float* Z, voi
 int rrtmgp_hip_mlp_bias_grad(int64_t rows, int n, const float* dZ, float* db, void* stream)
 {
     if (rows <= 0 || n <= 0 || !dZ || !db) return arg_error("mlp_bias_grad: bad arguments");
-    RRTMGP_TIMED("bias_grad_kernel", as_stream(stream));
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(grid1(n, 64)), dim3(64), 0, as_stream(stream), rows, n, dZ, db);
-    RRTMGP_LAUNCHED("bias_grad_kernel");
+    const hipStream_t s = as_stream(stream);
+    const int64_t nz = std::max<int64_t>(1, std::min<int64_t>(64, rows / 64));
+    const int64_t chunk = (rows + nz - 1) / nz;
+    float* part = nullptr;
+    RRTMGP_TRY(scratch_alloc(&part, size_t(nz * n), s));
+    RRTMGP_TIMED("bias_partial_kernel", s);
+    hipLaunchKernelGGL(bias_partial_kernel, dim3(grid1(n, 64), 1, unsigned(nz)), dim3(256), 0, s, rows, n, chunk, dZ,
+                       part);
+    RRTMGP_LAUNCHED("bias_partial_kernel");
+    hipLaunchKernelGGL(sum_split_kernel, dim3(grid1(n, 256)), dim3(256), 0, s, int64_t(1), int64_t(n), int(nz), part,
+                       db, int64_t(n), int64_t(1), 0);
+    scratch_free(part, s);
     return RRTMGP_OK;
 }
 
@@ -568,6 +727,40 @@ int rrtmgp_hip_mlp_to_bf16(int64_t n, const float* x, void* y, void* stream)
     return RRTMGP_OK;
 }
 
+
+// Persistent grid of the inference kernel: one 8-wavefront work-group per compute unit (two
+// wavefronts per SIMD: its register budget), no more than there are units.
+static unsigned mlp_grid(int64_t nunits)
+{
+    static int cus = 0;
+    if (cus == 0)
+    {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 256;
+    }
+    return unsigned(std::min<int64_t>(nunits, int64_t(cus)));
+}
+
+// The register-resident weights are read as 16-byte vectors.
+static int check_mlp_weights(const MlpNet& net)
+{
+    auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+    if (!al(net.W1) || !al(net.W2) || !al(net.W3)) return arg_error("mlp: weights must be 16-byte aligned");
+    if (net.no > MLP_MAXNO) return arg_error("mlp: at most 512 outputs (256 g-points)");
+    return RRTMGP_OK;
+}
+
+// The epilogue's lane offsets (2 ncol nlay + 63 elements) must fit 32 bits of bytes.
+static int check_mlp_offsets(int64_t ncol, int64_t nlay)
+{
+    if ((2 * ncol * nlay + 64) * int64_t(sizeof(float)) >= (int64_t(1) << 32))
+        return arg_error("mlp gas optics: ncol * nlay too large for one call; split the columns into blocks");
+    return RRTMGP_OK;
+}
 
 static int make_mlp_net(const rrtmgp_mlp_net* n, MlpNet& net)
 {
@@ -612,9 +805,11 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
         return arg_error("mlp_gas_optics_lw: bad arguments");
     const int64_t nunits = ((ncol + 63) / 64) * nlay;
     if (nunits >= (int64_t(1) << 31)) return arg_error("mlp_gas_optics_lw: too many column tiles");
+    RRTMGP_TRY(check_mlp_weights(net));
+    RRTMGP_TRY(check_mlp_offsets(ncol, nlay));
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, as_stream(stream), fallback);
     RRTMGP_TIMED("mlp_gas_optics_lw_kernel", as_stream(stream));
-    hipLaunchKernelGGL(mlp_gas_optics_kernel<true>, dim3(unsigned(nunits)), dim3(256), 0, as_stream(stream), net,
+    hipLaunchKernelGGL(mlp_gas_optics_kernel<true>, dim3(mlp_grid(nunits)), dim3(64 * MLP_WAVES), 0, as_stream(stream), net,
                        ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, tlev, tsfc, col_dry, g, totplnk,
                        int(nplanck), float(temp_min), float(dplanck), sfc_lay, cloud_tau, nullptr, nullptr, tau,
                        lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac, fallback);
@@ -636,9 +831,11 @@ int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
         return arg_error("mlp_gas_optics_sw: bad arguments");
     const int64_t nunits = ((ncol + 63) / 64) * nlay;
     if (nunits >= (int64_t(1) << 31)) return arg_error("mlp_gas_optics_sw: too many column tiles");
+    RRTMGP_TRY(check_mlp_weights(net));
+    RRTMGP_TRY(check_mlp_offsets(ncol, nlay));
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, as_stream(stream), fallback);
     RRTMGP_TIMED("mlp_gas_optics_sw_kernel", as_stream(stream));
-    hipLaunchKernelGGL(mlp_gas_optics_kernel<false>, dim3(unsigned(nunits)), dim3(256), 0, as_stream(stream), net,
+    hipLaunchKernelGGL(mlp_gas_optics_kernel<false>, dim3(mlp_grid(nunits)), dim3(64 * MLP_WAVES), 0, as_stream(stream), net,
                        ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, nullptr, nullptr, col_dry, gs,
                        nullptr, 2, 0.f, 1.f, 1, cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, nullptr, nullptr,
                        nullptr, fallback);

@@ -292,17 +292,20 @@ class MlpGasOptics:
                                           _f(eps), _i(self.step), None))
         self._cnet = None
 
-    def train(self, ts: TrainingSet, steps=2000, batch=4096, lr=1e-3, lr_final=1e-4, seed=2, log_every=0):
-        """Adam over random mini-batches with a geometric learning-rate decay; returns the loss
-        history (mean squared normalised error) at the logged steps."""
+    def train(self, ts: TrainingSet, steps=2000, batch=4096, lr=1e-3, lr_final=1e-5, decay_from=0.5, seed=2,
+              log_every=0):
+        """Adam over random mini-batches: learning rate lr for the first decay_from of the steps, then
+        a geometric decay to lr_final.  Returns the loss history (mean squared normalised error) at
+        the logged steps."""
         if self.stats is None:
             self.fit_normalisation(ts)
         rng = np.random.Generator(np.random.PCG64(seed))
         hist = []
+        s0 = int(decay_from * steps)
         for s in range(steps):
             idx = rng.integers(0, ts.rows, batch)
             loss = self.forward_backward(ts, idx)
-            rate = lr * (lr_final / lr) ** (s / max(1, steps - 1))
+            rate = lr if s < s0 else lr * (lr_final / lr) ** ((s - s0) / max(1, steps - 1 - s0))
             self.adam_step(rate)
             if log_every and (s % log_every == 0 or s == steps - 1):
                 hist.append((s, float(loss.to_host().sum()) / (batch * self.n_out)))

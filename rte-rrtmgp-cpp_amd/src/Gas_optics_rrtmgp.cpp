@@ -405,6 +405,8 @@ void Gas_optics_rrtmgp_gpu::set_solar_variability(const Float mg_index, const Fl
 
 Float Gas_optics_rrtmgp_gpu::get_tsi() const
 {
+    if (solar_source.size() != get_ngpt())
+        throw std::runtime_error("get_tsi: the k-distribution has no solar source (longwave tables)");
     Float tsi = 0.;
     for (int igpt = 1; igpt <= get_ngpt(); ++igpt) tsi += solar_source({igpt});
     return tsi;

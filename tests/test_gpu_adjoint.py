@@ -256,3 +256,46 @@ def test_sw_adjoint_end_to_end_fd(G, O, syn, kd_sw, okd_sw, lut_sw, top_at_1):
         fd = (loss(_perturbed(atm, v, h)) - loss(_perturbed(atm, v, -h))) / (2 * h)
         ad = _dot(G, grad, v, kd.gas_names, False)
         assert abs(ad - fd) <= 1e-5 * abs(fd), (trial, ad, fd)
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+def test_heating_rate_adjoint(G, prec):
+    """rrtmgp_hip_heating_rate_adjoint_*: exact derivatives of hr = c (net_i - net_{i+1}) /
+    (p_{i+1} - p_i), net = dn - up, against central differences of that formula in float64."""
+    import ctypes as C
+    from rrtmgp_amd import _lib
+    from rrtmgp_amd.device import DeviceArray
+    rng = np.random.default_rng(31)
+    ncol, nlay = 5, 9
+    dt = np.float64 if prec == 8 else np.float32
+    up = rng.uniform(100, 400, (ncol, nlay + 1)).astype(dt)
+    dn = rng.uniform(100, 400, (ncol, nlay + 1)).astype(dt)
+    p = np.cumsum(rng.uniform(500, 5000, (ncol, nlay + 1)), axis=1).astype(dt)
+    g = rng.normal(size=(ncol, nlay)).astype(dt)
+    c = 9.80665 / 1004.64 * 86400.0
+
+    def hr(u, d, pl):
+        net = d.astype(np.float64) - u.astype(np.float64)
+        return c * (net[:, :-1] - net[:, 1:]) / (pl[:, 1:].astype(np.float64) - pl[:, :-1])
+
+    dev = [DeviceArray.from_host(x, dt) for x in (up, dn, p, g)]
+    outs = [DeviceArray.from_host(np.full((ncol, nlay + 1), 0.5), dt) for _ in range(3)]   # the entry adds
+    fn = getattr(_lib.hip(), "rrtmgp_hip_heating_rate_adjoint_" + ("f64" if prec == 8 else "f32"))
+    assert fn(C.c_int64(ncol), C.c_int64(nlay), dev[0].ptr, dev[1].ptr, dev[2].ptr, dev[3].ptr, outs[0].ptr,
+              outs[1].ptr, outs[2].ptr, None) == 0
+    gu, gd, gp = (o.to_host().astype(np.float64) - 0.5 for o in outs)
+    up64, dn64, p64, g64 = (x.astype(np.float64) for x in (up, dn, p, g))
+    tol = 1e-7 if prec == 8 else 2e-5          # FP64: the O(h^2) error of differencing 1/dp
+    for name, base, got in (("up", up64, gu), ("dn", dn64, gd), ("p", p64, gp)):
+        v = rng.normal(size=base.shape) * (10.0 if name == "p" else 1.0)
+        h = 1e-3
+
+        def f(s):
+            args = {"up": up64, "dn": dn64, "p": p64}
+            args[name] = base + s * v
+            return float(np.sum(g64 * hr(args["up"], args["dn"], args["p"])))
+
+        fd = (f(h) - f(-h)) / (2 * h)
+        ad = float(np.sum(got * v))
+        assert abs(ad - fd) <= tol * max(abs(fd), 1e-3), (name, ad, fd)
+

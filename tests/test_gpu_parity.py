@@ -48,6 +48,13 @@ def test_init_tables_match_oracle(gkd, okd_lw, okd_sw):
         assert np.array_equal(k.get_real("kminor_lower"), okd.get_real("kminor_lower"))
 
 
+def test_tsi_of_tables(gkd, okd_sw):
+    """get_tsi sums the solar source (SW); longwave tables have none and raise."""
+    assert gkd[(False, 8)].scalar("tsi") == pytest.approx(okd_sw.scalar("tsi"), rel=1e-14)
+    with pytest.raises(Exception, match="no solar source"):
+        gkd[(True, 8)].scalar("tsi")
+
+
 def test_init_minor_reduction_matches_oracle(G, O, kd_lw):
     avail = [g for g in AVAILABLE if g != "ch4"]
     k, o = G.KDist(kd_lw, avail), O.Kdist(kd_lw, avail)

@@ -31,8 +31,11 @@ def test_torch_fluxes_and_heating_rate_gradients(G, O, syn, kd_lw, kd_sw, okd_lw
     rng = np.random.default_rng(21)
     nlev = atm.nlay + 1
     a = {k: rng.normal(size=(atm.ncol, nlev)) for k in ("up", "dn", "dir")}
-    a_hr = rng.normal(size=(atm.ncol, atm.nlay)) * 10.
-    tsi = np.full(atm.ncol, atm.tsi / kd.scalar("tsi"))
+    # Heating-rate cotangents only below 1000 Pa: above, layers of ~1 Pa make the heating rate a
+    # quotient of near-cancelling flux differences, and the oracle's central differences of it
+    # wander by ~1e-3 relative as the step shrinks (the adjoint agrees with its large-step limit).
+    a_hr = rng.normal(size=(atm.ncol, atm.nlay)) * 10. * (atm.p_lay > 1000.)
+    tsi = None if lw else np.full(atm.ncol, atm.tsi / kd.scalar("tsi"))
     if lw:
         up, dn = torch_ops.lw_fluxes(solver, x["t_lay"], x["t_lev"], x["t_sfc"], x["p_lay"], x["p_lev"], x["h2o"],
                                      x["o3"], x["emis_sfc"], other, clouds)
@@ -61,7 +64,9 @@ def test_torch_fluxes_and_heating_rate_gradients(G, O, syn, kd_lw, kd_sw, okd_lw
 
     for trial in range(2):
         v = _directions(rng, atm, lw)
-        h = 1e-4
+        # LW: the Planck table has a node every 1 K, so steps much above 1e-4 K cross kinks of its
+        # linear interpolation; SW: at 1e-4 the FP64 two-stream's rounding at tiny tau shows.
+        h = 1e-4 if lw else 1e-3
         fd = (loss(_perturbed(atm, v, h)) - loss(_perturbed(atm, v, -h))) / (2 * h)
         ad = (np.sum(grad["t_lay"] * v["t_lay"]) + np.sum(grad["p_lay"] * atm.p_lay * v["lnp_lay"]) +
               np.sum(grad["p_lev"] * v["p_lev"]) + np.sum(grad["h2o"] * v["h2o"]) + np.sum(grad["o3"] * v["o3"]))
