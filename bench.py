@@ -18,6 +18,10 @@ Also reported, on the same JSON line:
   cpu_baseline  the FP64 CPU oracle (a restatement of the reference CPU path) on the first
                 columns of the same workload, timed on this host.
   parity        max |flux - FP64 oracle| over those columns (W m-2).
+  nn            the NN gas-optics path: LW and SW MLP surrogates trained on the GPU on the plain
+                gas optics, then the same step timed with them, its speed-up over the plain path
+                and its flux RMSE / max error against the plain path (--no-nn skips it).
+  adjoint       adjoint (reverse-mode) time over forward time at 16384 columns (--no-adjoint).
   kernels       per-kernel launches, time and algorithmic GB/s of the timed steps.
 """
 from __future__ import annotations
@@ -125,6 +129,66 @@ def adjoint_measure(kl, ks, ll, ls, prec, ncol=16384, nlay=128, reps=3):
             "gradients": "T_lay, T_lev, T_sfc, ln p_lay, p_lev, vmr of every gas, emissivity, albedos"}
 
 
+def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain_kernels, train_steps=20000,
+               n_train_atm=32):
+    """The NN gas-optics path: one surrogate per spectral range trained on the GPU on the plain gas
+    optics of n_train_atm x 1024 x 128 perturbed clear-sky pairs (about 4M samples), set on the
+    k-distributions, then the same all-sky step timed and its fluxes compared with the plain path's.
+    Training and the comparison are outside the timed steps."""
+    from rrtmgp_amd import device, profiling, surrogate as S, synthetic as syn
+    t0 = time.perf_counter()
+    atms = [syn.make_atmosphere(500 + s, 1024, 128) for s in range(n_train_atm)]
+    nets, losses = {}, {}
+    for lw, kd in ((True, kd_lw), (False, kd_sw)):
+        ts = S.build_training_set(kd, atms, seed=3 if lw else 4)
+        net = S.MlpGasOptics(kd, seed=9)
+        net.fit_normalisation(ts)
+        hold = np.random.default_rng(10).integers(0, ts.rows, 16384)
+        net.train(ts, steps=train_steps, batch=4096, lr=1e-3, lr_final=1e-5)
+        losses["lw" if lw else "sw"] = round(net.loss(ts, hold), 5)
+        nets[lw] = net
+        rows = ts.rows
+        del ts
+    device.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"NN: trained LW and SW surrogates in {train_s:.1f} s; normalised losses {losses}")
+    S.set_surrogate(kd_lw, nets[True])
+    S.set_surrogate(kd_sw, nets[False])
+    try:
+        step()
+        device.synchronize()
+        profiling.enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        device.synchronize()
+        wall = time.perf_counter() - t0
+        profiling.enable(False)
+        kt = {k.name: k.avg_ms for k in profiling.collect()}
+        fb = {"lw": S.last_fallback_units(kd_lw), "sw": S.last_fallback_units(kd_sw)}
+        nlw = {k: v.to_host() for k, v in out_lw.items()}
+        nsw = {k: v.to_host() for k, v in out_sw.items()}
+    finally:
+        S.set_surrogate(kd_lw, None)
+        S.set_surrogate(kd_sw, None)
+    rmse, emax = {}, {}
+    for tag, got, ref in (("lw", nlw, glw), ("sw", nsw, gsw)):
+        for k in ("flux_up", "flux_dn", "flux_net"):
+            d = got[k].astype(np.float64) - ref[k]
+            rmse[f"{tag}_{k}"] = round(float(np.sqrt(np.mean(d * d))), 4)
+            emax[f"{tag}_{k}"] = round(float(np.abs(d).max()), 4)
+    ms = wall * 1e3 / steps
+    return {"ms_per_step": round(ms, 3), "value": round(units * steps / wall, 1),
+            "gas_optics_ms": {"plain_lw": round(plain_kernels.get("gas_optics_lw_kernel", 0.0), 3),
+                              "nn_lw": round(kt.get("mlp_gas_optics_lw_kernel", 0.0), 3),
+                              "plain_sw": round(plain_kernels.get("gas_optics_sw_kernel", 0.0), 3),
+                              "nn_sw": round(kt.get("mlp_gas_optics_sw_kernel", 0.0), 3)},
+            "flux_rmse_wm2": rmse, "flux_err_max_wm2": emax, "fallback_units_last_call": fb,
+            "training": {"samples_per_range": rows, "steps": train_steps, "batch": 4096, "optimizer": "Adam lr 1e-3",
+                         "seconds_lw_sw": round(train_s, 1), "holdout_loss": losses},
+            "network": "7 -> 128 -> 128 -> 2 ngpt, softplus, BF16 weights/activations, FP32 accumulation"}
+
+
 def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
     """FP64 oracle on the first ncols columns with `threads` OpenMP threads; returns the timing
     record and the max flux error of the GPU fluxes over those columns."""
@@ -179,6 +243,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: OMP_NUM_THREADS, max 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--no-nn", action="store_true", help="skip the NN gas-optics (surrogate) measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="launch one untimed 1 GiB device copy first (reference dispatch for PMC counters)")
     args = ap.parse_args()
@@ -300,6 +365,14 @@ def main():
                        "per_output": {k: round(v, 6) for k, v in err.items()}})
         log(f"CPU baseline {rate:.3e} U/s in {secs:.2f} s; max flux error {max(err.values()):.3e} W m-2")
 
+    nn = None
+    if not args.no_nn and prec == 4:
+        log("NN gas optics: training surrogates and timing the same step with them")
+        nn = nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, args.steps, units,
+                        {k.name: k.avg_ms for k in kernels})
+        nn["speedup_vs_plain"] = round(ms_per_step / nn["ms_per_step"], 3)
+        log(f"NN path {nn['ms_per_step']:.2f} ms/step ({nn['speedup_vs_plain']:.2f}x); flux RMSE {nn['flux_rmse_wm2']}")
+
     adjoint = None
     if not args.no_adjoint:
         log("adjoint: LW + SW forward and adjoint at 16384 columns")
@@ -315,7 +388,8 @@ def main():
         "config": {"workload": f"all-sky LW+SW forward, plain gas optics, {ncol} columns x {nlay} layers",
                    "ncol": ncol, "nlay": nlay, "ngpt_lw": kd_lw.ngpt, "ngpt_sw": kd_sw.ngpt,
                    "n_col_block": blk, "gpu_ms_per_step": round(gpu_ms / args.steps, 3)},
-        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "adjoint": adjoint, "kernels": krows,
+        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "nn": nn, "adjoint": adjoint,
+        "kernels": krows,
     }
     print(json.dumps(line), flush=True)
 

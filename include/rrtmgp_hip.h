@@ -489,11 +489,12 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_HEATING_ADJ)
 
 /* ------------------------------------------------------------------------------------------ */
 /* MLP gas-optics surrogate (FP32 path; BF16 matrix cores with FP32 accumulation).  No         */
-/* counterpart in the reference.  Per (column, layer) the network maps                         */
-/* x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o] (normalised with x_mean/x_std) through  */
+/* counterpart in the reference.  Per (column, layer) the network maps x = [T, ln p, ln h2o,     */
+/* ln o3, ln co2, ln ch4, ln n2o, f] (f = 1 where p > p_trop, the lower-atmosphere tables;      */
+/* normalised with x_mean/x_std) through                                                        */
 /* two softplus layers of 128 to n_out outputs (denormalised with y_mean/y_std), interleaved  */
 /* per g-point: LW (ln(tau/col_dry), Planck fraction), SW (ln(tau_abs/col_dry),               */
-/* ln(tau_ray/col_dry)).  Weights are row-major (out, in) BF16; W1 is (128, 16) with the 7    */
+/* ln(tau_ray/col_dry)).  Weights are row-major (out, in) BF16; W1 is (128, 16) with the 8    */
 /* inputs first.  A (64-column tile, layer) unit with any input outside [x_lo, x_hi] goes on   */
 /* the fallback list (1 + ntile nlay ints, ntile = (ncol + 63)/64): fallback[0] = number of     */
 /* listed units, fallback[1 ..] = ilay * ntile + tile, the tile list of                         */
@@ -507,13 +508,14 @@ typedef struct
     const float* b1;
     const float* b2;
     const float* b3;
-    const float* x_mean;                   /* (7)                                           */
+    const float* x_mean;                   /* (8)                                           */
     const float* x_std;
     const float* x_lo;                     /* training envelope of the raw inputs           */
     const float* x_hi;
     const float* y_mean;                   /* (n_out)                                       */
     const float* y_std;
     int64_t n_out;                         /* 2 ngpt, a multiple of 32                      */
+    double p_trop;                         /* tropopause pressure of the k-distribution [Pa] */
 } rrtmgp_mlp_net;
 
 /* Inference, fused with the gas-optics epilogue (outputs as rrtmgp_hip_gas_optics_lw/sw_f32, */
@@ -557,7 +559,7 @@ int rrtmgp_hip_mlp_to_bf16(int64_t n, const float* x, void* y, void* stream);
 /* ncol ilay; targets Y (ncol nlay, 2 ngpt) from LW (a = tau, b = lay_src) or SW (a = tau,       */
 /* b = ssa) outputs of a clear-sky run.                                                          */
 int rrtmgp_hip_mlp_inputs(int64_t ncol, int64_t nlay, const float* play, const float* tlay,
-                          const rrtmgp_gas_vmr* gases, float* X, void* stream);
+                          const rrtmgp_gas_vmr* gases, double p_trop, float* X, void* stream);
 int rrtmgp_hip_mlp_targets(int lw, int64_t ncol, int64_t nlay, int64_t ngpt, const int* gpt2band,
                            const float* tlay, const float* col_dry, const float* a, const float* b,
                            const float* totplnk, int64_t nplanck, double temp_min, double dplanck,

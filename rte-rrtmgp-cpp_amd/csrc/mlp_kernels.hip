@@ -1,8 +1,9 @@
 // MLP gas-optics surrogate: training and inference kernels (BF16 MFMA, FP32 accumulation).
 // No counterpart in the reference; the network and its targets come from the plain gas optics.
 //
-// Network, per (column, layer): x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o], normalised
-// with the training set's mean and standard deviation, then
+// Network, per (column, layer): x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o, f], with f = 1
+// in the lower atmosphere (p > p_trop, where the k-distribution switches tables) and 0 above,
+// normalised with the training set's mean and standard deviation, then
 //     h1 = softplus(W1 x + b1),  h2 = softplus(W2 h1 + b2),  y = W3 h2 + b3      (hidden 128)
 // and outputs denormalised with per-output mean/std.  Outputs are interleaved per g-point:
 //     LW row 2g = ln(tau_g / col_dry), row 2g+1 = Planck fraction_g           (no = 2 ngpt)
@@ -23,7 +24,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int MLP_H = 128;     // hidden width
-constexpr int MLP_IN = 7;      // inputs
+constexpr int MLP_IN = 8;      // inputs: T, ln p, 5 ln vmr, lower-atmosphere flag
 constexpr int MLP_INP = 16;    // inputs padded to one MFMA K step
 constexpr float MLP_LN_FLOOR = -100.f;   // ln(tau / col_dry) target where tau = 0
 
@@ -245,6 +246,7 @@ struct MlpNet
     const float* y_mean;     // (no)
     const float* y_std;
     int no;
+    float ln_p_trop;         // ln of the k-distribution's tropopause pressure
 };
 
 // The inputs' gases in network order: h2o, o3, co2, ch4, n2o (vmr with broadcast strides).
@@ -255,6 +257,7 @@ struct MlpGases
 };
 
 constexpr int MLP_P = 64;             // columns per unit
+constexpr int MLP_MAXPLANCK = 4096;   // totplnk entries (nplanck x nbnd) staged in LDS
 constexpr int MLP_MAXNO = 512;        // outputs the register-resident W3 can hold
 constexpr int MLP_WAVES = 8;
 constexpr int MLP_RT = MLP_MAXNO / 32 / MLP_WAVES;   // W3 row tiles per wavefront
@@ -311,6 +314,8 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
     __shared__ __align__(16) __bf16 sW2[MLP_H][MLP_H + 8];
     __shared__ float sB[LW ? 4 : 1][16][P];         // LW: Planck values per (band, column)
     __shared__ float sT[LW ? 4 : 1][P];             // LW: T of the layer, upper and lower level, surface
+    __shared__ float sC[LW ? 1 : 3][16][P];         // band cloud tau (LW); tau, ssa, g (SW)
+    __shared__ float sPl[LW ? MLP_MAXPLANCK : 1];   // LW: totplnk (nplanck, nbnd)
     __shared__ float s_cd[P];
     __shared__ float s_b1[MLP_H], s_b2[MLP_H];
     __shared__ float4 s_so[MLP_MAXNO / 2];        // per g-point: (scale, offset) of rows 2g, 2g+1
@@ -349,10 +354,32 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                               net.y_std[b], net.b3[b] * net.y_std[b] + net.y_mean[b]);
     }
     for (int e = tid; e < no / 2; e += 64 * MLP_WAVES) s_g2b[e] = gpt2band[e];
+    if constexpr (LW)
+        for (int e = tid; e < nplanck * nbnd; e += 64 * MLP_WAVES) sPl[e] = totplnk[e];
 
     MlpIn in{};
     if (w == 0 && int64_t(blockIdx.x) < nunits)
         mlp_load_inputs<LW>(in, blockIdx.x, ntile, ncol, lane, play, tlay, tlev, tsfc, col_dry, gases);
+    // Cloud properties of bands w and w + 8 for the next unit's columns, one unit ahead.
+    float cl[LW ? 1 : 3][2] = {};
+    auto load_cloud = [&](int64_t v) {
+        if (!ctau) return;
+        const int64_t vt = v % ntile, vl = v / ntile;
+        const int64_t icol = vt * P + lane;
+        if (icol >= ncol) return;
+        const int64_t idx = icol + ncol * vl;
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+        {
+            const int b = w + MLP_WAVES * k;
+            if (b < nbnd)
+            {
+                cl[0][k] = ctau[idx + ncl * b];
+                if constexpr (!LW) { cl[1][k] = cssa[idx + ncl * b]; cl[2][k] = cg[idx + ncl * b]; }
+            }
+        }
+    };
+    if (int64_t(blockIdx.x) < nunits) load_cloud(blockIdx.x);
 
     for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x)
     {
@@ -373,6 +400,7 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                 x[1] = __logf(in.p);
 #pragma unroll
                 for (int g = 0; g < 5; ++g) x[2 + g] = __logf(in.v[g]);
+                x[7] = x[1] > net.ln_p_trop ? 1.f : 0.f;
 #pragma unroll
                 for (int i = 0; i < MLP_IN; ++i) out |= !(x[i] >= net.x_lo[i] && x[i] <= net.x_hi[i]);
             }
@@ -403,12 +431,27 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                 h1[pt12 * 32 + lane % 32][j] = __bf16(softplus_fast(acc[r] + s_b1[j]));
             }
         }
+        if (ctau)
+        {
+            // Band cloud properties of the unit's columns (loaded during the previous unit),
+            // bands w and w + 8.
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+            {
+                const int b = w + MLP_WAVES * k;
+                if (b < nbnd)
+                {
+                    sC[0][b][lane] = cl[0][k];
+                    if constexpr (!LW) { sC[1][b][lane] = cl[1][k]; sC[2][b][lane] = cl[2][k]; }
+                }
+            }
+        }
         if constexpr (LW)
         {
             // interpolate1D over totplnk for the band's Planck values of this column.
             for (int b = w; b < nbnd; b += MLP_WAVES)
             {
-                const float* tab = totplnk + int64_t(b) * nplanck;
+                const float* tab = sPl + b * nplanck;
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                 {
@@ -440,9 +483,10 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
         }
         __syncthreads();
 
-        // Next unit's inputs, in flight during layer 3.
+        // Next unit's inputs and cloud properties, in flight during layer 3.
         if (w == 0 && u + gridDim.x < nunits)
             mlp_load_inputs<LW>(in, u + gridDim.x, ntile, ncol, lane, play, tlay, tlev, tsfc, col_dry, gases);
+        if (u + gridDim.x < nunits) load_cloud(u + gridDim.x);
 
         // 3. Layer 3 and the epilogue.  Rows 2g, 2g+1 of D are g-point g; for register r the
         // g-point is gbase + c(r), gbase = 16 rt + 2 (lane / 32), c(r) = 4 (r / 4) + (r % 4) / 2.
@@ -486,7 +530,7 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                     {
                         const float tau = __expf(y0) * cd;
                         const float pf = y1;
-                        store_at(out0 + uo, vofs, ctau ? tau + ctau[idx + ncl * b] : tau);
+                        store_at(out0 + uo, vofs, ctau ? tau + sC[0][b][pl] : tau);
                         store_at(out1 + uo, vofs, pf * sB[0][b][pl]);
                         store_at(out2 + uo, vofs, pf * sB[1][b][pl]);
                         store_at(out3 + uo, vofs, pf * sB[2][b][pl]);
@@ -496,7 +540,7 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                             const float v0 = (sT[3][pl] + 1.f - temp_min) / dplanck;
                             const float fr = v0 - float(int(v0));
                             const int id = min(nplanck - 1, max(1, int(v0) + 1));
-                            const float* tab = totplnk + int64_t(b) * nplanck;
+                            const float* tab = sPl + b * nplanck;
                             const float b1s = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
                             const int64_t os = icol + ncol * g;
                             sfc_src[os] = pf * sB[3][b][pl];
@@ -508,14 +552,13 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                         const float tabs = __expf(y0) * cd, tray = __expf(y1) * cd;
                         const float tt = tabs + tray;
                         const float eps2 = 2.f * FLT_EPSILON, op_eps = 3.f * FLT_MIN;
-                        float ss = tt > eps2 ? tray / tt : 0.f, tt_out = tt, gg = 0.f;
+                        float ss = tt > eps2 ? __fdividef(tray, tt) : 0.f, tt_out = tt, gg = 0.f;
                         if (ctau)
                         {
-                            const int64_t ib = idx + ncl * b;
-                            const float t2 = ctau[ib], s2 = cssa[ib], g2 = cg[ib];
+                            const float t2 = sC[0][b][pl], s2 = sC[1][b][pl], g2 = sC[2][b][pl];
                             const float tau12 = tt + t2, tauscat = tt * ss + t2 * s2;
-                            gg = (t2 * s2 * g2) / fmaxf(tauscat, op_eps);
-                            ss = tauscat / fmaxf(op_eps, tau12);
+                            gg = __fdividef(t2 * s2 * g2, fmaxf(tauscat, op_eps));
+                            ss = __fdividef(tauscat, fmaxf(op_eps, tau12));
                             tt_out = tau12;
                         }
                         store_at(out0 + uo, vofs, tt_out);
@@ -539,7 +582,8 @@ __global__ void zero_int_kernel(int* p) { *p = 0; }
 // Training data from the plain gas optics (clear sky).  Row r = (icol, ilay) pair, icol fastest.
 // X (rows, 16): raw inputs, zero padded.  Y (rows, no): interleaved targets.
 __global__ void mlp_inputs_kernel(int64_t ncol, int64_t nlay, const float* __restrict__ play,
-                                  const float* __restrict__ tlay, MlpGases gases, float* __restrict__ X)
+                                  const float* __restrict__ tlay, MlpGases gases, float ln_p_trop,
+                                  float* __restrict__ X)
 {
     const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= ncol * nlay) return;
@@ -549,6 +593,7 @@ __global__ void mlp_inputs_kernel(int64_t ncol, int64_t nlay, const float* __res
     x[1] = logf(play[r]);
 #pragma unroll
     for (int g = 0; g < 5; ++g) x[2 + g] = logf(gases.p[g][icol * gases.scol[g] + ilay * gases.slay[g]]);
+    x[7] = x[1] > ln_p_trop ? 1.f : 0.f;
 #pragma unroll
     for (int i = MLP_IN; i < MLP_INP; ++i) x[i] = 0.f;
 }
@@ -771,6 +816,8 @@ static int make_mlp_net(const rrtmgp_mlp_net* n, MlpNet& net)
     net.W3 = static_cast<const __bf16*>(n->W3);
     net.b1 = n->b1; net.b2 = n->b2; net.b3 = n->b3; net.x_mean = n->x_mean; net.x_std = n->x_std;
     net.x_lo = n->x_lo; net.x_hi = n->x_hi; net.y_mean = n->y_mean; net.y_std = n->y_std; net.no = int(n->n_out);
+    if (!(n->p_trop > 0.0)) return arg_error("mlp: the network needs the tropopause pressure of its training");
+    net.ln_p_trop = float(std::log(n->p_trop));
     return RRTMGP_OK;
 }
 
@@ -799,7 +846,8 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
     RRTMGP_TRY(make_mlp_net(n, net));
     MlpGases g;
     RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, g));
-    if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || nbnd > 16 || nplanck < 2 || !gpt2band || !play ||
+    if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || nbnd > 16 || nplanck < 2 ||
+        nplanck * nbnd > MLP_MAXPLANCK || !gpt2band || !play ||
         !tlay || !tlev || !tsfc || !col_dry || !totplnk || !tau || !lay_src || !lev_src_inc || !lev_src_dec ||
         !sfc_src || !sfc_src_jac || !fallback || (sfc_lay != 1 && sfc_lay != nlay))
         return arg_error("mlp_gas_optics_lw: bad arguments");
@@ -826,7 +874,7 @@ int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
     RRTMGP_TRY(make_mlp_net(n, net));
     MlpGases gs;
     RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, gs));
-    if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || !gpt2band || !play || !tlay || !col_dry ||
+    if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || nbnd > 16 || !gpt2band || !play || !tlay || !col_dry ||
         !tau || !ssa || !g || !fallback || (cloud_tau && (!cloud_ssa || !cloud_g)))
         return arg_error("mlp_gas_optics_sw: bad arguments");
     const int64_t nunits = ((ncol + 63) / 64) * nlay;
@@ -844,14 +892,14 @@ int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
 }
 
 int rrtmgp_hip_mlp_inputs(int64_t ncol, int64_t nlay, const float* play, const float* tlay,
-                          const rrtmgp_gas_vmr* gases, float* X, void* stream)
+                          const rrtmgp_gas_vmr* gases, double p_trop, float* X, void* stream)
 {
     MlpGases g;
     RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, g));
-    if (!play || !tlay || !X) return arg_error("mlp_inputs: bad arguments");
+    if (!play || !tlay || !X || !(p_trop > 0.0)) return arg_error("mlp_inputs: bad arguments");
     RRTMGP_TIMED("mlp_inputs_kernel", as_stream(stream));
     hipLaunchKernelGGL(mlp_inputs_kernel, dim3(grid1(ncol * nlay, 256)), dim3(256), 0, as_stream(stream), ncol,
-                       nlay, play, tlay, g, X);
+                       nlay, play, tlay, g, float(std::log(p_trop)), X);
     RRTMGP_LAUNCHED("mlp_inputs_kernel");
     return RRTMGP_OK;
 }

@@ -2,8 +2,10 @@
 fused inference behind a k-distribution.
 
 No reference counterpart.  Per (column, layer) pair the network maps
-    x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o]
-(normalised with the training set's mean and standard deviation) through two softplus layers of
+    x = [T, ln p, ln h2o, ln o3, ln co2, ln ch4, ln n2o, f],
+f = 1 in the lower atmosphere (p > p_trop, where the k-distribution switches to its lower tables,
+a jump in ln p that a smooth network otherwise fits poorly) and 0 above, normalised with the
+training set's mean and standard deviation, through two softplus layers of
 128 units to 2 ngpt outputs, interleaved per g-point:
     longwave   (ln(tau / col_dry), Planck fraction)
     shortwave  (ln(tau_abs / col_dry), ln(tau_ray / col_dry))
@@ -25,7 +27,7 @@ from .device import DeviceArray, synchronize
 
 GASES = ("h2o", "o3", "co2", "ch4", "n2o")
 HIDDEN = 128
-N_IN = 7
+N_IN = 8            # T, ln p, 5 ln vmr, lower-atmosphere flag
 N_INP = 16          # inputs padded to one matrix-core K step
 MAX_GAS = 32        # RRTMGP_MAX_GAS
 
@@ -37,7 +39,7 @@ class GasVmrC(C.Structure):
 
 class MlpNetC(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("W1", "W2", "W3", "b1", "b2", "b3", "x_mean", "x_std", "x_lo",
-                                          "x_hi", "y_mean", "y_std")] + [("n_out", C.c_int64)]
+                                          "x_hi", "y_mean", "y_std")] + [("n_out", C.c_int64), ("p_trop", C.c_double)]
 
 
 def _L():
@@ -112,7 +114,8 @@ def build_training_set(kd: api.KDist, atmospheres, seed=0, lo=0.5, hi=2.0) -> Tr
         desc = gas_desc(vmr, ncol, nlay)
         xp = C.c_void_p(X.ptr.value + 4 * N_INP * r0)
         yp = C.c_void_p(Y.ptr.value + 4 * no * r0)
-        _chk(_L().rrtmgp_hip_mlp_inputs(_i(ncol), _i(nlay), play.ptr, tlay.ptr, C.byref(desc), xp, None))
+        _chk(_L().rrtmgp_hip_mlp_inputs(_i(ncol), _i(nlay), play.ptr, tlay.ptr, C.byref(desc),
+                                        C.c_double(kd.src.press_ref_trop), xp, None))
         b = o["lay_src"] if kd.is_lw else o["ssa"]
         _chk(_L().rrtmgp_hip_mlp_targets(
             C.c_int(int(kd.is_lw)), _i(ncol), _i(nlay), _i(kd.ngpt), g2b.ptr, tlay.ptr, cd.ptr, o["tau"].ptr, b.ptr,
@@ -183,6 +186,7 @@ class MlpGasOptics:
         self.kd = kd
         self.lw = kd.is_lw
         self.n_out = 2 * kd.ngpt
+        self.p_trop = float(kd.src.press_ref_trop)
         rng = np.random.Generator(np.random.PCG64(seed))
 
         def glorot(o, i, fan_in):
@@ -204,7 +208,7 @@ class MlpGasOptics:
     # -- normalisation -------------------------------------------------------------------------
     def fit_normalisation(self, ts: TrainingSet, sample=65536, seed=1):
         """Mean/std of inputs and targets over a random sample of rows; input envelope over all rows."""
-        X = ts.X.to_host().T[:, :N_IN].astype(np.float64)                 # (rows, 7)
+        X = ts.X.to_host().T[:, :N_IN].astype(np.float64)                 # (rows, N_IN)
         rng = np.random.Generator(np.random.PCG64(seed))
         idx = np.sort(rng.choice(ts.rows, size=min(sample, ts.rows), replace=False))
         Ys = self._rows(ts.Y, ts.n_out, idx).astype(np.float64)
@@ -292,7 +296,7 @@ class MlpGasOptics:
                                           _f(eps), _i(self.step), None))
         self._cnet = None
 
-    def train(self, ts: TrainingSet, steps=2000, batch=4096, lr=1e-3, lr_final=1e-5, decay_from=0.5, seed=2,
+    def train(self, ts: TrainingSet, steps=20000, batch=4096, lr=1e-3, lr_final=1e-5, decay_from=0.5, seed=2,
               log_every=0):
         """Adam over random mini-batches: learning rate lr for the first decay_from of the steps, then
         a geometric decay to lr_final.  Returns the loss history (mean squared normalised error) at
@@ -326,12 +330,14 @@ class MlpGasOptics:
             self._cnet = MlpNetC(self.W1.b16.ptr.value, self.W2.b16.ptr.value, self.W3.b16.ptr.value,
                                  self.b1.p.ptr.value, self.b2.p.ptr.value, self.b3.p.ptr.value,
                                  st["x_mean"].ptr.value, st["x_std"].ptr.value, st["x_lo"].ptr.value,
-                                 st["x_hi"].ptr.value, st["y_mean"].ptr.value, st["y_std"].ptr.value, self.n_out)
+                                 st["x_hi"].ptr.value, st["y_mean"].ptr.value, st["y_std"].ptr.value, self.n_out,
+                                 self.p_trop)
         return self._cnet
 
     def host_weights(self) -> dict:
         w = {k: getattr(self, k).host() for k in ("W1", "b1", "W2", "b2", "W3", "b3")}
         w.update(self.host_stats)
+        w["p_trop"] = np.float64(self.p_trop)
         return w
 
     def save(self, path):
@@ -349,4 +355,5 @@ class MlpGasOptics:
             setattr(net, k, _Param(a, p.b16 is not None))
         net.params = [net.W1, net.b1, net.W2, net.b2, net.W3, net.b3]
         net.set_normalisation(*(d[k] for k in ("x_mean", "x_std", "x_lo", "x_hi", "y_mean", "y_std")))
+        net.p_trop = float(d["p_trop"])
         return net
