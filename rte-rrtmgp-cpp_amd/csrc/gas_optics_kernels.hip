@@ -901,7 +901,7 @@ __device__ __forceinline__ double quad_sum(double v)
 // chunks.  LW cotangents: ga (tau), gb (lay_src), gc (lev_src_inc), gd (lev_src_dec) of
 // (ncol, nlay, ngb) and gs (sfc_src, (ncol, ngb)); SW: ga (tau), gb (ssa), gc (g).
 template<typename Real, bool LW>
-__global__ __launch_bounds__(GO_THREADS)
+__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1)))
 void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, int ibnd,
                                const Real* __restrict__ play, const Real* __restrict__ tlay,
                                const Real* __restrict__ tlev, const Real* __restrict__ tsfc,
@@ -919,6 +919,7 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
     __shared__ Real s_g[5][GO_GPTS][GO_ROW];
     __shared__ Real s_acc[GO_COLS][GA_N + 1];
     __shared__ Real s_cl[3][GO_COLS];
+    __shared__ Real s_sc[GO_MAXM][GO_THREADS];      // per lane: cotangent sums of the minor scalings
     __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
     const int64_t ncoltiles = (ncol + GO_COLS - 1) / GO_COLS;
     const int64_t u = xcd_remap(blockIdx.x, ncoltiles * nlay);
@@ -955,6 +956,8 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
     Real acc[GA_N];
 #pragma unroll
     for (int a = 0; a < GA_N; ++a) acc[a] = Real(0);
+#pragma unroll
+    for (int q = 0; q < GO_MAXM; ++q) s_sc[q][threadIdx.x] = Real(0);
     const Real eps2 = Real(2) * Limits<Real>::eps;
     const Real op_eps = Real(3) * Limits<Real>::tiny;
 
@@ -987,18 +990,13 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
         const uint32_t gl = uint32_t(c0 + gq);
         const uint32_t gp = bd.gp, sE = uint32_t(k.npres + 1) * gp, sP = gp;
         const uint32_t kb = uint32_t(bd.koff) + gl;
-        Real K[2][4][4], P[2][4][4];
+        Real K[2][4][4];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
         {
             const uint32_t o = kb + uint32_t(st.om[i]);
             load4(k.kmajor_p, o, K[i][0]); load4(k.kmajor_p, o + sE, K[i][1]);
             load4(k.kmajor_p, o + sP, K[i][2]); load4(k.kmajor_p, o + sP + sE, K[i][3]);
-            if constexpr (LW)
-            {
-                load4(k.planck_frac_p, o, P[i][0]); load4(k.planck_frac_p, o + sE, P[i][1]);
-                load4(k.planck_frac_p, o + sP, P[i][2]); load4(k.planck_frac_p, o + sP + sE, P[i][3]);
-            }
         }
         // Forward values per g-point that the cotangents need.
         Real gabs[4], gray[4];
@@ -1035,24 +1033,10 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
 #pragma unroll
             for (int e = 0; e < 4; ++e) { gabs[e] = s_g[0][gq + e][pi]; gray[e] = Real(0); }
         }
-        // Major and Planck terms.
+        // Major terms.
 #pragma unroll
         for (int e = 0; e < 4; ++e)
         {
-            Real gpf = Real(0);
-            if (LW)
-            {
-                Real pf = Real(0);
-#pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) pf += st.fmajor[i][c] * P[i][c][e];
-                const Real gl_ = s_g[1][gq + e][pi], gi_ = s_g[2][gq + e][pi], gd_ = s_g[3][gq + e][pi];
-                const Real gs_ = at_sfc ? s_g[4][gq + e][pi] : Real(0);
-                gpf = gl_ * st.aux[0] + gi_ * st.aux[1] + gd_ * st.aux[2] + (at_sfc ? gs_ * st.aux[3] : Real(0));
-                acc[GA_X + 0] += gl_ * pf; acc[GA_X + 1] += gi_ * pf;
-                acc[GA_X + 2] += gd_ * pf; acc[GA_X + 3] += gs_ * pf;
-            }
 #pragma unroll
             for (int i = 0; i < 2; ++i)
             {
@@ -1061,9 +1045,7 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
                 for (int c = 0; c < 4; ++c)
                 {
                     sk += st.fmajor[i][c] * K[i][c][e];
-                    Real gw = gabs[e] * st.col_mix[i] * K[i][c][e];
-                    if constexpr (LW) gw += gpf * P[i][c][e];
-                    acc[GA_FM + 4 * i + c] += gw;
+                    acc[GA_FM + 4 * i + c] += gabs[e] * st.col_mix[i] * K[i][c][e];
                 }
                 acc[GA_CM + i] += gabs[e] * sk;
             }
@@ -1081,19 +1063,55 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
                 acc[GA_X] += gray[e] * sr;
             }
         }
-        // Minor terms.
+        // Planck-fraction terms (LW), after the major table values are dead: the two table slabs
+        // are never live together.
+        if constexpr (LW)
+        {
+            __builtin_amdgcn_sched_barrier(0);
+            Real P[2][4][4];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+            {
+                const uint32_t o = kb + uint32_t(st.om[i]);
+                load4(k.planck_frac_p, o, P[i][0]); load4(k.planck_frac_p, o + sE, P[i][1]);
+                load4(k.planck_frac_p, o + sP, P[i][2]); load4(k.planck_frac_p, o + sP + sE, P[i][3]);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+            {
+                Real pf = Real(0);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) pf += st.fmajor[i][c] * P[i][c][e];
+                const Real gl_ = s_g[1][gq + e][pi], gi_ = s_g[2][gq + e][pi], gd_ = s_g[3][gq + e][pi];
+                const Real gs_ = at_sfc ? s_g[4][gq + e][pi] : Real(0);
+                const Real gpf = gl_ * st.aux[0] + gi_ * st.aux[1] + gd_ * st.aux[2] +
+                                 (at_sfc ? gs_ * st.aux[3] : Real(0));
+                acc[GA_X + 0] += gl_ * pf; acc[GA_X + 1] += gi_ * pf;
+                acc[GA_X + 2] += gd_ * pf; acc[GA_X + 3] += gs_ * pf;
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[GA_FM + 4 * i + c] += gpf * P[i][c][e];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // Minor terms, one slot at a time (a rolled loop keeps one slot's table values live); the
+        // slot's scaling cotangent goes to the lane's LDS slot.
         const int nslot = bd.nslot;
         const uint32_t sQ = uint32_t(nslot) * gp;
         const uint32_t mb = uint32_t(bd.moff) + gl;
-#pragma unroll
-        for (int q = 0; q < GO_MAXM; ++q)
+#pragma unroll 1
+        for (int q = 0; q < nslot; ++q)
         {
-            if (q >= nslot) break;
             Real km[2][2][4];
             const uint32_t o0 = mb + uint32_t(st.mn[0]) + uint32_t(q) * gp;
             const uint32_t o1 = mb + uint32_t(st.mn[1]) + uint32_t(q) * gp;
             load4(k.kminor_p, o0, km[0][0]); load4(k.kminor_p, o0 + sQ, km[0][1]);
             load4(k.kminor_p, o1, km[1][0]); load4(k.kminor_p, o1 + sQ, km[1][1]);
+            const Real scq = st.scaling[q];
+            Real tq = Real(0);
 #pragma unroll
             for (int e = 0; e < 4; ++e)
             {
@@ -1104,17 +1122,18 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
                     for (int c = 0; c < 2; ++c)
                     {
                         sm += st.fminor[i][c] * km[i][c][e];
-                        acc[GA_FN + 2 * i + c] += gabs[e] * st.scaling[q] * km[i][c][e];
+                        acc[GA_FN + 2 * i + c] += gabs[e] * scq * km[i][c][e];
                     }
-                acc[GA_SC + q] += gabs[e] * sm;
+                tq += gabs[e] * sm;
             }
+            s_sc[q][threadIdx.x] += tq;
         }
     }
     // Sum over the pair's four lanes; the first lane of each pair stores the pair's totals.
 #pragma unroll
     for (int a = 0; a < GA_N; ++a)
     {
-        const Real v = quad_sum(acc[a]);
+        const Real v = quad_sum(a >= GA_SC && a < GA_SC + GO_MAXM ? s_sc[a - GA_SC][threadIdx.x] : acc[a]);
         if (lane % 4 == 0) s_acc[pi][a] = v;
     }
     __syncthreads();
