@@ -186,7 +186,8 @@ def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain
             "flux_rmse_wm2": rmse, "flux_err_max_wm2": emax, "fallback_units_last_call": fb,
             "training": {"samples_per_range": rows, "steps": train_steps, "batch": 4096, "optimizer": "Adam lr 1e-3",
                          "seconds_lw_sw": round(train_s, 1), "holdout_loss": losses},
-            "network": "7 -> 128 -> 128 -> 2 ngpt, softplus, BF16 weights/activations, FP32 accumulation"}
+            "network": "8 inputs (T, ln p, 5 ln vmr, lower-atmosphere flag) -> 128 -> 128 -> 2 ngpt, softplus, "
+                       "BF16 weights/activations, FP32 accumulation"}
 
 
 def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
