@@ -284,8 +284,12 @@ int lw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t n
 }
 // ---------------------------------------------------------------------------------------------
 // Shortwave adjoint: one (column, g-point) per thread, 64 columns x 4 g-points per work-group.
+// State store per (column, g-point): the layer properties R, T, Rdir, Tdir, Tn in the build's
+// precision (the FP32 build's forward solver also works with FP32 layer properties) and the
+// level states D, Ru, dir, A, S and the reverse sweep's adjoints lA, lS in FP64.
 constexpr int SWA_GPTS = 4;
-constexpr int SWA_NQ = 14;   // stored quantities per level
+constexpr int SWA_NL = 5;    // stored layer properties
+constexpr int SWA_NV = 7;    // stored level states
 
 template<typename Real>
 __global__ __launch_bounds__(ADJ_COLS * SWA_GPTS)
@@ -294,9 +298,9 @@ void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                        const Real* __restrict__ mu0, const Real* __restrict__ alb_dir,
                        const Real* __restrict__ alb_dif, const Real* __restrict__ inc_dir,
                        const Real* __restrict__ inc_dif, const Real* __restrict__ a_up,
-                       const Real* __restrict__ a_dn, const Real* __restrict__ a_dir, double* __restrict__ st,
-                       Real* __restrict__ d_tau, Real* __restrict__ d_ssa, Real* __restrict__ d_g,
-                       Real* __restrict__ d_alb_dir, Real* __restrict__ d_alb_dif)
+                       const Real* __restrict__ a_dn, const Real* __restrict__ a_dir, Real* __restrict__ sl,
+                       double* __restrict__ sv, Real* __restrict__ d_tau, Real* __restrict__ d_ssa,
+                       Real* __restrict__ d_g, Real* __restrict__ d_alb_dir, Real* __restrict__ d_alb_dif)
 {
     const int64_t icol = int64_t(blockIdx.x) * ADJ_COLS + threadIdx.x;
     const int64_t igpt = int64_t(blockIdx.y) * SWA_GPTS + threadIdx.y;
@@ -306,48 +310,52 @@ void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     const int64_t lb = icol + ncol * nlay * igpt;
     auto lay = [&](int64_t j) -> int64_t { return lb + ncol * (top_at_1 ? j : n - 1 - j); };
     auto lev = [&](int64_t j) -> int64_t { return icol + ncol * (top_at_1 ? j : n - j); };
-    // Stored states: q = 0 R, 1 T, 2 Rdir, 3 Tdir, 4 Tn (layers); 5 D, 6 Ru, 7 dir, 8 A, 9 S (levels);
-    // 10..13 the layer gradients of the top-down reverse sweep (R, T, SU, SD).
-    auto Q = [&](int q, int64_t j) -> double& { return st[((int64_t(q) * nlev + j) * ngpt + igpt) * ncol + icol]; };
+    // Layer properties: 0 R, 1 T, 2 Rdir, 3 Tdir, 4 Tn.  Level states: 0 D, 1 Ru, 2 dir, 3 A, 4 S,
+    // 5 lA, 6 lS (adjoints of A and S at the level, from the top-down reverse sweep).
+    auto L = [&](int q, int64_t jj) -> Real& { return sl[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
+    auto V = [&](int q, int64_t jj) -> double& { return sv[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
     const double m0 = double(mu0[icol]);
 
     // Forward, top-down: layer properties, direct beam, column above each level.
     {
         double dir = double(inc_dir[c2]) * m0, D = inc_dif ? double(inc_dif[c2]) : 0., Ru = 0.;
-        for (int64_t j = 0; j < n; ++j)
+        for (int64_t jj = 0; jj < n; ++jj)
         {
-            Q(5, j) = D; Q(6, j) = Ru; Q(7, j) = dir;
-            const int64_t l = lay(j);
+            V(0, jj) = D; V(1, jj) = Ru; V(2, jj) = dir;
+            const int64_t l = lay(jj);
             double R, T, Rd, Td, Tn;
             sw_two_stream_t<double>(double(tau[l]), double(ssa[l]), double(gasy[l]), m0, R, T, Rd, Td, Tn);
-            Q(0, j) = R; Q(1, j) = T; Q(2, j) = Rd; Q(3, j) = Td; Q(4, j) = Tn;
+            // The stored (build-precision) values are the ones every later sweep uses.
+            L(0, jj) = Real(R); L(1, jj) = Real(T); L(2, jj) = Real(Rd); L(3, jj) = Real(Td); L(4, jj) = Real(Tn);
+            R = double(Real(R)); T = double(Real(T)); Rd = double(Real(Rd)); Td = double(Real(Td));
+            Tn = double(Real(Tn));
             const double SU = Rd * dir, SD = Td * dir;
             const double e1 = 1. / (1. - R * Ru);
             D = SD + T * (D + Ru * SU) * e1;
             Ru = R + T * T * Ru * e1;
             dir *= Tn;
         }
-        Q(5, n) = D; Q(6, n) = Ru; Q(7, n) = dir;
+        V(0, n) = D; V(1, n) = Ru; V(2, n) = dir;
     }
     // Forward, bottom-up: column below each level.
     const double ad = double(alb_dir[c2]), af = double(alb_dif[c2]);
     {
-        double A = af, S = Q(7, n) * ad;
-        Q(8, n) = A; Q(9, n) = S;
-        for (int64_t j = n - 1; j >= 0; --j)
+        double A = af, S = V(2, n) * ad;
+        V(3, n) = A; V(4, n) = S;
+        for (int64_t jj = n - 1; jj >= 0; --jj)
         {
-            const double R = Q(0, j), T = Q(1, j), dir = Q(7, j);
-            const double SU = Q(2, j) * dir, SD = Q(3, j) * dir;
+            const double R = double(L(0, jj)), T = double(L(1, jj)), dir = V(2, jj);
+            const double SU = double(L(2, jj)) * dir, SD = double(L(3, jj)) * dir;
             const double e2 = 1. / (1. - R * A);
             S = SU + T * (S + A * SD) * e2;
             A = R + T * T * A * e2;
-            Q(8, j) = A; Q(9, j) = S;
+            V(3, jj) = A; V(4, jj) = S;
         }
     }
     // Flux cotangents at level j: gradients with respect to D, Ru, A, S.
-    auto level_partials = [&](int64_t j, double& gD, double& gRu, double& gA, double& gS) {
-        const double D = Q(5, j), Ru = Q(6, j), A = Q(8, j), S = Q(9, j);
-        const double au = double(a_up[lev(j)]), adn = double(a_dn[lev(j)]);
+    auto level_partials = [&](int64_t jj, double D, double Ru, double A, double S, double& gD, double& gRu,
+                              double& gA, double& gS) {
+        const double au = double(a_up[lev(jj)]), adn = double(a_dn[lev(jj)]);
         const double q = D + Ru * S, den = 1. / (1. - Ru * A), dn = q * den;
         const double gdn = adn + au * A;
         gD = gdn * den;
@@ -355,69 +363,91 @@ void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         gA = gdn * q * Ru * den * den + au * dn;
         gS = gdn * Ru * den + au;
     };
+    // Contributions of the bottom-up recursion's adjoint at layer j to the layer's R, T, SU, SD
+    // cotangents, from the adjoints lA, lS of A_j, S_j.
+    auto bottom_up_partials = [](double lA, double lS, double R, double T, double SD, double A1, double S1,
+                                 double& gR, double& gT, double& gSU, double& gSD) {
+        const double e2 = 1. / (1. - R * A1);
+        const double w = S1 + A1 * SD;
+        gR = lA * (1. + T * T * A1 * A1 * e2 * e2) + lS * T * w * A1 * e2 * e2;
+        gT = lA * 2. * T * A1 * e2 + lS * w * e2;
+        gSU = lS;
+        gSD = lS * T * A1 * e2;
+    };
 
     // Reverse of the bottom-up recursion, top-down.
     double lA = 0., lS = 0.;
-    for (int64_t j = 0; j <= n; ++j)
     {
-        double gD, gRu, gA, gS;
-        level_partials(j, gD, gRu, gA, gS);
-        lA += gA; lS += gS;
-        if (j == n) break;
-        const double R = Q(0, j), T = Q(1, j), dir = Q(7, j);
-        const double SD = Q(3, j) * dir;
-        const double A1 = Q(8, j + 1), S1 = Q(9, j + 1);
-        const double e2 = 1. / (1. - R * A1);
-        const double w = S1 + A1 * SD;
-        Q(10, j) = lA * (1. + T * T * A1 * A1 * e2 * e2) + lS * T * w * A1 * e2 * e2;
-        Q(11, j) = lA * 2. * T * A1 * e2 + lS * w * e2;
-        Q(12, j) = lS;
-        Q(13, j) = lS * T * A1 * e2;
-        const double nA = lA * T * T * e2 * e2 + lS * T * e2 * (SD + w * R * e2);
-        lS = lS * T * e2;
-        lA = nA;
+        double A = V(3, 0), S = V(4, 0);
+        for (int64_t jj = 0; jj <= n; ++jj)
+        {
+            double gD, gRu, gA, gS;
+            level_partials(jj, V(0, jj), V(1, jj), A, S, gD, gRu, gA, gS);
+            lA += gA; lS += gS;
+            if (jj == n) break;
+            V(5, jj) = lA; V(6, jj) = lS;
+            const double R = double(L(0, jj)), T = double(L(1, jj)), dir = V(2, jj);
+            const double SD = double(L(3, jj)) * dir;
+            const double A1 = V(3, jj + 1), S1 = V(4, jj + 1);
+            const double e2 = 1. / (1. - R * A1);
+            const double w = S1 + A1 * SD;
+            const double nA = lA * T * T * e2 * e2 + lS * T * e2 * (SD + w * R * e2);
+            lS = lS * T * e2;
+            lA = nA;
+            A = A1; S = S1;
+        }
     }
     d_alb_dif[c2] = Real(lA);
-    d_alb_dir[c2] = Real(lS * Q(7, n));
+    d_alb_dir[c2] = Real(lS * V(2, n));
 
     // Reverse of the top-down recursion and the direct beam, bottom-up.
     double lD, lRu, ldir;
+    double A1 = V(3, n), S1 = V(4, n);
     {
         double gD, gRu, gA, gS;
-        level_partials(n, gD, gRu, gA, gS);
+        level_partials(n, V(0, n), V(1, n), A1, S1, gD, gRu, gA, gS);
         lD = gD; lRu = gRu;
         ldir = double(a_dn[lev(n)]) + double(a_dir[lev(n)]) + lS * ad;
     }
-    for (int64_t j = n - 1; j >= 0; --j)
+    for (int64_t jj = n - 1; jj >= 0; --jj)
     {
-        const double R = Q(0, j), T = Q(1, j), Rd = Q(2, j), Td = Q(3, j), Tn = Q(4, j);
-        const double D = Q(5, j), Ru = Q(6, j), dir = Q(7, j);
-        const double SU = Rd * dir;
+        // Layer properties and their Jacobian by forward mode on (tau, w0, g); the values are
+        // rounded to the stored precision, as the earlier sweeps used them.
+        const int64_t l = lay(jj);
+        Dual3 oR, oT, oRd, oTd, oTn;
+        sw_two_stream_t<Dual3>(mk(double(tau[l]), 1., 0., 0.), mk(double(ssa[l]), 0., 1., 0.),
+                               mk(double(gasy[l]), 0., 0., 1.), m0, oR, oT, oRd, oTd, oTn);
+        const double R = double(Real(oR.v)), T = double(Real(oT.v)), Rd = double(Real(oRd.v));
+        const double Td = double(Real(oTd.v)), Tn = double(Real(oTn.v));
+        const double D = V(0, jj), Ru = V(1, jj), dir = V(2, jj);
+        const double SU = Rd * dir, SD = Td * dir;
+        // A_j, S_j rebuilt from A_{j+1}, S_{j+1} exactly as the bottom-up forward sweep formed them.
+        const double e2f = 1. / (1. - R * A1);
+        const double S = SU + T * (S1 + A1 * SD) * e2f;
+        const double A = R + T * T * A1 * e2f;
+        double bR, bT, bSU, bSD;
+        bottom_up_partials(V(5, jj), V(6, jj), R, T, SD, A1, S1, bR, bT, bSU, bSD);
         const double e1 = 1. / (1. - R * Ru);
         const double v = D + Ru * SU;
-        const double gSD = Q(13, j) + lD;
-        const double gSU = Q(12, j) + lD * T * Ru * e1;
-        const double gT = Q(11, j) + lD * v * e1 + lRu * 2. * T * Ru * e1;
-        const double gR = Q(10, j) + lD * T * v * Ru * e1 * e1 + lRu * (1. + T * T * Ru * Ru * e1 * e1);
+        const double gSD = bSD + lD;
+        const double gSU = bSU + lD * T * Ru * e1;
+        const double gT = bT + lD * v * e1 + lRu * 2. * T * Ru * e1;
+        const double gR = bR + lD * T * v * Ru * e1 * e1 + lRu * (1. + T * T * Ru * Ru * e1 * e1);
         const double gRd = gSU * dir, gTd = gSD * dir, gTn = ldir * dir;
         const double nD = lD * T * e1;
         const double nRu = lD * (T * SU * e1 + T * v * R * e1 * e1) + lRu * T * T * e1 * e1;
         const double ndir = ldir * Tn + gSU * Rd + gSD * Td;
-        // Layer Jacobian by forward mode on (tau, w0, g).
-        const int64_t l = lay(j);
-        Dual3 oR, oT, oRd, oTd, oTn;
-        sw_two_stream_t<Dual3>(mk(double(tau[l]), 1., 0., 0.), mk(double(ssa[l]), 0., 1., 0.),
-                               mk(double(gasy[l]), 0., 0., 1.), m0, oR, oT, oRd, oTd, oTn);
         double g3[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             g3[c] = gR * oR.d[c] + gT * oT.d[c] + gRd * oRd.d[c] + gTd * oTd.d[c] + gTn * oTn.d[c];
         d_tau[l] = Real(g3[0]); d_ssa[l] = Real(g3[1]); d_g[l] = Real(g3[2]);
         double gD, gRu, gA, gS;
-        level_partials(j, gD, gRu, gA, gS);
+        level_partials(jj, D, Ru, A, S, gD, gRu, gA, gS);
         lD = nD + gD;
         lRu = nRu + gRu;
-        ldir = ndir + double(a_dn[lev(j)]) + double(a_dir[lev(j)]);
+        ldir = ndir + double(a_dn[lev(jj)]) + double(a_dir[lev(jj)]);
+        A1 = A; S1 = S;
     }
 }
 
@@ -432,19 +462,27 @@ int sw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Rea
         !d_ssa || !d_g || !d_alb_dir || !d_alb_dif)
         return arg_error("sw_adjoint: NULL array");
     hipStream_t s = as_stream(stream);
-    double* st = nullptr;
-    RRTMGP_TRY(scratch_alloc(&st, size_t(SWA_NQ * (nlay + 1) * ngpt * ncol), s));
+    const size_t per = size_t((nlay + 1) * ngpt * ncol);
+    Real* sl = nullptr;
+    double* sv = nullptr;
+    RRTMGP_TRY(scratch_alloc(&sl, SWA_NL * per, s));
+    if (int rc0 = scratch_alloc(&sv, SWA_NV * per, s); rc0 != RRTMGP_OK)
+    {
+        scratch_free(sl, s);
+        return rc0;
+    }
     const dim3 grid(grid1(ncol, ADJ_COLS), grid1(ngpt, SWA_GPTS));
     const dim3 block(ADJ_COLS, SWA_GPTS);
     int rc;
     {
         RRTMGP_TIMED("sw_adjoint_kernel", s);
         hipLaunchKernelGGL(sw_adjoint_kernel<Real>, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0,
-                           alb_dir, alb_dif, inc_dir, inc_dif, a_up, a_dn, a_dir, st, d_tau, d_ssa, d_g, d_alb_dir,
-                           d_alb_dif);
+                           alb_dir, alb_dif, inc_dir, inc_dif, a_up, a_dn, a_dir, sl, sv, d_tau, d_ssa, d_g,
+                           d_alb_dir, d_alb_dif);
         rc = check_hip(hipGetLastError(), "sw_adjoint_kernel");
     }
-    scratch_free(st, s);
+    scratch_free(sv, s);
+    scratch_free(sl, s);
     return rc;
 }
 
