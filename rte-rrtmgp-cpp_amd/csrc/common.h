@@ -114,4 +114,8 @@ inline unsigned xcd_grid(int64_t n_units)
 {
     return static_cast<unsigned>(((n_units + n_xcd - 1) / n_xcd) * n_xcd);
 }
+
+
+// Compute units of the current device (cached after the first call; 256 if the query fails).
+int device_cu_count();
 }  // namespace rrtmgp_hip

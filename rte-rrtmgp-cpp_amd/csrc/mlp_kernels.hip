@@ -777,17 +777,7 @@ int rrtmgp_hip_mlp_to_bf16(int64_t n, const float* x, void* y, void* stream)
 // wavefronts per SIMD: its register budget), no more than there are units.
 static unsigned mlp_grid(int64_t nunits)
 {
-    static int cus = 0;
-    if (cus == 0)
-    {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
-            cus = n;
-        else
-            cus = 256;
-    }
-    return unsigned(std::min<int64_t>(nunits, int64_t(cus)));
+    return unsigned(std::min<int64_t>(nunits, int64_t(device_cu_count())));
 }
 
 // The register-resident weights are read as 16-byte vectors.

@@ -24,6 +24,21 @@ int check_hip(hipError_t e, const char* what)
     return RRTMGP_ERR_HIP;
 }
 
+int device_cu_count()
+{
+    constexpr int max_dev = 64;
+    static std::atomic<int> cus[max_dev] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= max_dev) return 256;
+    int n = cus[dev].load();
+    if (n == 0)
+    {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev].store(n);
+    }
+    return n;
+}
+
 namespace
 {
 struct TimedLaunch { const char* name; hipEvent_t start, stop; };
