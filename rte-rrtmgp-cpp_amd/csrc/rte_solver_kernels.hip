@@ -149,8 +149,8 @@ __device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf
 // e^{-y} for y >= 0 through 2^x.
 __device__ __forceinline__ float exp_neg_f32(float y) { return __builtin_amdgcn_exp2f(-y * 1.44269504088896341f); }
 
-// 1 - e^{-y} for y >= 0: series below 0.25 (truncation < y^8/8! relative), else 1 - e^{-y}.
-__device__ __forceinline__ float one_minus_exp_neg_f32(float y)
+// 1 - e^{-y} for 0 <= y < 0.25 from its series (truncation < y^8/8! relative).
+__device__ __forceinline__ float one_minus_exp_neg_series_f32(float y)
 {
     float t = 1.f / 720.f - y * (1.f / 5040.f);
     t = 1.f / 120.f - y * t;
@@ -158,7 +158,13 @@ __device__ __forceinline__ float one_minus_exp_neg_f32(float y)
     t = 1.f / 6.f - y * t;
     t = .5f - y * t;
     t = 1.f - y * t;
-    return (y < .25f) ? y * t : 1.f - exp_neg_f32(y);
+    return y * t;
+}
+
+// 1 - e^{-y} for y >= 0 given e = e^{-y}: the series below 0.25, else 1 - e.
+__device__ __forceinline__ float one_minus_exp_neg_f32(float y, float e)
+{
+    return (y < .25f) ? one_minus_exp_neg_series_f32(y) : 1.f - e;
 }
 
 // Array element i at a uniform base with a 32-bit lane offset: addressed as scalar base +
@@ -400,12 +406,13 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     const float alpha1 = gamma1 * gamma4 + gamma2 * gamma3;
     const float alpha2 = gamma1 * gamma3 + gamma2 * gamma4;
     const float kt = k * tau;
-    const float a = one_minus_exp_neg_f32(kt), e1 = exp_neg_f32(kt);
+    const float e1 = exp_neg_f32(kt);
+    const float a = one_minus_exp_neg_f32(kt, e1);
     const float c2 = a * (2.f - a);
     const float RT = rcp_f32(k * (2.f - c2) + gamma1 * c2);
     const float y = tau * inv_mu0;
     const float Tn = expf(-y);
-    const float b = (y < .25f) ? one_minus_exp_neg_f32(y) : 1.f - Tn;
+    const float b = one_minus_exp_neg_f32(y, Tn);
     const float ek = 1.f - k_mu;
     const float x = y * ek;
     float phi = 1.f / 40320.f;
@@ -420,11 +427,12 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     const float s = w0 * RT * rcp_f32(1.f + k_mu);
     const float rdf = s * ((alpha2 + k * gamma3) * c2 + 2.f * (k * gamma3 - alpha2 * k_mu) * (1.f - a) * G);
     const float tdf = -s * ((alpha1 - k * gamma4) * c2 * (1.f - b) - 2.f * k * (gamma4 + mu0 * alpha1) * G);
-    // Clamps of rte_solver_kernels.cu:581-583 with the FP64 epsilon of the island they replace.
-    const double eps = Limits<double>::eps;
-    const double rd = fmax(eps, fmin(double(rdf), double(b)));
-    const double td = fmax(eps, fmin(double(tdf), double(b) - rd));
-    L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), rd * dir, td * dir, double(Tn)};
+    // Clamps of rte_solver_kernels.cu:581-583 with the FP64 epsilon of the island they replace
+    // (representable in FP32).
+    const float eps = float(Limits<double>::eps);
+    const float rd = fmaxf(eps, fminf(rdf, b));
+    const float td = fmaxf(eps, fminf(tdf, b - rd));
+    L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), double(rd) * dir, double(td) * dir, double(Tn)};
     return true;
 }
 
