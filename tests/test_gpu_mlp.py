@@ -362,3 +362,104 @@ def test_training_and_flux_error(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw):
     # a production surrogate needs: it guards against regressions of training and inference.
     for (lw, k), (rmse, fmax) in res.items():
         assert rmse < 0.025 * fmax, ((lw, k), rmse, fmax)
+
+
+@pytest.mark.parametrize("lw", [True, False])
+def test_inference_cloudy_top_at_1(S, syn, L, D, kd32_lw, kd32_sw, ts_lw, ts_sw, lw):
+    """rrtmgp_hip_mlp_gas_optics_* through the C ABI on a cloudy, top-at-1 atmosphere with a ragged
+    column count: every output (LW sources at both levels, surface source and Jacobian at
+    sfc_lay = nlay; SW combine with the band cloud properties) against the numpy network."""
+    kd, ts = (kd32_lw, ts_lw) if lw else (kd32_sw, ts_sw)
+    net = S.MlpGasOptics(kd, seed=11)
+    net.fit_normalisation(ts)
+    net.train(ts, steps=50, batch=1024)
+    atm = syn.flip_vertical(syn.subset_columns(syn.make_atmosphere(502, 512, 60), np.arange(131)))
+    ncol, nlay = atm.p_lay.shape
+    nb, ng = kd.nbnd, kd.ngpt
+    rng = np.random.default_rng(12)
+    f32 = lambda a: D.from_host(np.asarray(a, np.float32), np.float32)          # noqa: E731
+    vmr = {g: f32(np.broadcast_to(np.asarray(atm.vmr[g], np.float64), (ncol, nlay)).copy(order="F"))
+           for g in S.GASES}
+    desc = S.gas_desc(vmr, ncol, nlay)
+    cd = rng.uniform(1e22, 1e24, (ncol, nlay))
+    ctau = rng.uniform(0, 2, (ncol, nlay, nb)) * (rng.random((ncol, nlay, 1)) < 0.5)
+    cssa = rng.uniform(0.5, 1, (ncol, nlay, nb))
+    cgg = rng.uniform(0.7, 0.9, (ncol, nlay, nb))
+    g2b = S.gpt2band(kd)
+    fb = D((1 + ((ncol + 63) // 64) * nlay,), np.int32)
+    cnet = net.c_net()
+    i = C.c_int64
+    outs = [D((ncol, nlay, ng), np.float32) for _ in range(4 if lw else 3)]
+    # Every device array the call reads stays referenced until the kernel has finished.
+    d_play, d_tlay, d_cd, d_ctau = f32(atm.p_lay), f32(atm.t_lay), f32(cd), f32(ctau)
+    if lw:
+        tp, nplanck, tmin, dpl = S.planck_table(kd)
+        d_tlev, d_tsfc = f32(atm.t_lev), f32(atm.t_sfc)
+        sfc = [D((ncol, ng), np.float32) for _ in range(2)]
+        rc = L.rrtmgp_hip_mlp_gas_optics_lw(
+            C.byref(cnet), i(ncol), i(nlay), i(ng), i(nb), g2b.ptr, d_play.ptr, d_tlay.ptr, d_tlev.ptr,
+            d_tsfc.ptr, d_cd.ptr, C.byref(desc), tp.ptr, i(nplanck), C.c_double(tmin), C.c_double(dpl),
+            C.c_int(nlay), d_ctau.ptr, outs[0].ptr, outs[1].ptr, outs[2].ptr, outs[3].ptr, sfc[0].ptr,
+            sfc[1].ptr, fb.ptr, None)
+    else:
+        d_cssa, d_cg = f32(cssa), f32(cgg)
+        rc = L.rrtmgp_hip_mlp_gas_optics_sw(
+            C.byref(cnet), i(ncol), i(nlay), i(ng), i(nb), g2b.ptr, d_play.ptr, d_tlay.ptr, d_cd.ptr,
+            C.byref(desc), d_ctau.ptr, d_cssa.ptr, d_cg.ptr, outs[0].ptr, outs[1].ptr, outs[2].ptr, fb.ptr,
+            None)
+    assert rc == 0
+    from rrtmgp_amd.device import synchronize
+    synchronize()
+    assert fb.to_host()[0] == 0                              # training columns: inside the envelope
+    w, hs = net.host_weights(), net.host_stats
+    f64 = lambda a: np.asarray(a, np.float32).astype(np.float64)                  # noqa: E731
+    lnp = np.log(f64(atm.p_lay)).ravel(order="F")
+    x = np.stack([f64(atm.t_lay).ravel(order="F"), lnp] +
+                 [np.log(f64(np.broadcast_to(atm.vmr[g], (ncol, nlay)))).ravel(order="F") for g in S.GASES] +
+                 [(lnp > np.log(net.p_trop)).astype(np.float64)], axis=1).astype(np.float32)
+    xn = np.zeros((x.shape[0], 16), np.float32)
+    xn[:, :8] = (x - hs["x_mean"].astype(np.float32)) / hs["x_std"].astype(np.float32)
+    y = _np_forward(w, bf16(xn))[4] * hs["y_std"] + hs["y_mean"]            # (pairs, 2 ngpt)
+    cdf = f64(cd).ravel(order="F")[:, None]
+    lims = kd.src.band2gpt
+    g2bh = np.zeros(ng, int)
+    for b in range(nb):
+        g2bh[lims[0, b] - 1:lims[1, b]] = b
+    cl = lambda a: f64(a).reshape(ncol * nlay, nb, order="F")[:, g2bh]          # noqa: E731
+    got = [o.to_host().reshape(ncol * nlay, ng, order="F").astype(np.float64) for o in outs]
+    if lw:
+        tau_gas = np.exp(y[:, 0::2]) * cdf
+        pf = y[:, 1::2]
+        ref_tau = tau_gas + cl(ctau)
+        r = np.abs(got[0] - ref_tau) / ref_tau
+        assert r.max() < 2e-2 and np.count_nonzero(r > 5e-3 + 3e-4) <= r.size // 10000, float(r.max())
+        tl = f64(atm.t_lev)
+        b_lay = _planck(kd, f64(atm.t_lay).ravel(order="F"))
+        b_hi = _planck(kd, tl[:, 1:].ravel(order="F"))
+        b_lo = _planck(kd, tl[:, :-1].ravel(order="F"))
+        for k, bref in ((1, b_lay), (2, b_hi), (3, b_lo)):
+            ref = pf * bref
+            np.testing.assert_allclose(got[k], ref, rtol=2e-3, atol=2e-3 * np.abs(ref).max(), err_msg=f"source {k}")
+        # Surface: layer nlay (top-at-1), Planck of T_sfc and T_sfc + 1.
+        s_src = sfc[0].to_host().astype(np.float64)
+        s_jac = sfc[1].to_host().astype(np.float64)
+        pf_s = pf.reshape(ncol, nlay, ng, order="F")[:, nlay - 1, :]
+        b_s = _planck(kd, f64(atm.t_sfc))
+        b_s1 = _planck(kd, f64(atm.t_sfc) + 1.0)
+        np.testing.assert_allclose(s_src, pf_s * b_s, rtol=2e-3, atol=2e-3 * np.abs(s_src).max())
+        np.testing.assert_allclose(s_jac, pf_s * (b_s1 - b_s), rtol=3e-3, atol=3e-3 * np.abs(s_jac).max())
+    else:
+        tabs = np.exp(y[:, 0::2]) * cdf
+        tray = np.exp(y[:, 1::2]) * cdf
+        tt = tabs + tray
+        eps2 = 2 * np.finfo(np.float32).eps
+        ss = np.where(tt > eps2, tray / np.where(tt > 0, tt, 1.0), 0.0)
+        t2, s2, g2 = cl(ctau), cl(cssa), cl(cgg)
+        tau12 = tt + t2
+        tsc = tt * ss + t2 * s2
+        ref_g = t2 * s2 * g2 / np.maximum(tsc, 3 * np.finfo(np.float32).tiny)
+        ref_ssa = tsc / np.maximum(tau12, 3 * np.finfo(np.float32).tiny)
+        assert_close_log(np.log(got[0]), np.log(tau12), "ln tau")
+        d = np.abs(got[1] - ref_ssa)
+        assert d.max() < 2e-2 and np.count_nonzero(d > 1e-4 + 5e-3 * ref_ssa) <= d.size // 10000
+        np.testing.assert_allclose(got[2], ref_g, rtol=2e-3, atol=2e-4)
