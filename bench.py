@@ -129,7 +129,7 @@ def adjoint_measure(kl, ks, ll, ls, prec, ncol=16384, nlay=128, reps=3):
             "gradients": "T_lay, T_lev, T_sfc, ln p_lay, p_lev, vmr of every gas, emissivity, albedos"}
 
 
-def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain_kernels, train_steps=20000,
+def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain_kernels, train_steps=60000,
                n_train_atm=32):
     """The NN gas-optics path: one surrogate per spectral range trained on the GPU on the plain gas
     optics of n_train_atm x 1024 x 128 perturbed clear-sky pairs (about 4M samples), set on the

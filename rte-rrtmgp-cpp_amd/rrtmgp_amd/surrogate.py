@@ -296,7 +296,7 @@ class MlpGasOptics:
                                           _f(eps), _i(self.step), None))
         self._cnet = None
 
-    def train(self, ts: TrainingSet, steps=20000, batch=4096, lr=1e-3, lr_final=1e-5, decay_from=0.5, seed=2,
+    def train(self, ts: TrainingSet, steps=60000, batch=4096, lr=1e-3, lr_final=1e-5, decay_from=0.5, seed=2,
               log_every=0):
         """Adam over random mini-batches: learning rate lr for the first decay_from of the steps, then
         a geometric decay to lr_final.  Returns the loss history (mean squared normalised error) at
