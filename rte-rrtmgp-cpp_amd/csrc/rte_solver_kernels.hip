@@ -381,17 +381,19 @@ __device__ __forceinline__ double rcp_f64(double x)
 
 // The layer properties in FP32, rearranged so that FP32 keeps its accuracy:
 //   k^2 = 2 (1 - w0) (gamma1 + gamma2)          (gamma1 - gamma2 = 2 (1 - w0) exactly),
-//   1 - e^{-2k tau} = a (2 - a) with a = 1 - e^{-k tau}, b = 1 - e^{-tau / mu0},
+//   1 - e^{-2k tau} = a (2 - a) with a = 1 - e^{-k tau},
 // and the direct-beam solution written without the removable singularity at k mu0 = 1.  With
 // G = (e^{-k tau} - e^{-tau/mu0}) / (1 - k mu0) the reference's expressions (F90:1079-1104)
 // become, identically,
-//   Rdir = w0 RT [(alpha2 + k gamma3)(2a - a^2) + 2 (k gamma3 - alpha2 k mu0)(1 - a) G] / (1 + k mu0)
-//   Tdir = -w0 RT [(alpha1 - k gamma4)(2a - a^2)(1 - b) - 2k (gamma4 + mu0 alpha1) G] / (1 + k mu0)
+//   Rdir = w0 RT [(alpha2 + k gamma3)(2a - a^2) + 2 (k gamma3 - alpha2 k mu0) e^{-k tau} G] / (1 + k mu0)
+//   Tdir = -w0 RT [(alpha1 - k gamma4)(2a - a^2) e^{-tau/mu0} - 2k (gamma4 + mu0 alpha1) G] / (1 + k mu0)
 // where every O(1) term that cancels in the reference form has been cancelled analytically.  G is
 // e^{-tau/mu0} (tau/mu0) (e^x - 1)/x with x = tau (1 - k mu0) / mu0, from a series when |x| <= 1/2
-// and as the difference quotient otherwise.  The direct-beam transmittance, whose rounding
-// compounds down the column, keeps the accurate expf.  Returns false where FP64 is kept: near
-// conservative scattering (1 - w0 < 1e-6).
+// and as the difference quotient otherwise.  a keeps its series for small k tau: where k is small
+// RT ~ 1/(2k) amplifies the absolute error of 1 - e^{-k tau}.  1 - e^{-tau/mu0} only bounds
+// Rdir + Tdir and is taken as it comes.  The transmittance e^{-tau/mu0} of this layer enters G
+// and Tdir only; the direct beam itself is carried by the caller (Beam<float>).  Returns false
+// where FP64 is kept: near conservative scattering (1 - w0 < 1e-6).
 __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, float mu0, float inv_mu0, double dir,
                                               LayerSW& L)
 {
@@ -409,10 +411,8 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     const float e1 = exp_neg_f32(kt);
     const float a = one_minus_exp_neg_f32(kt, e1);
     const float c2 = a * (2.f - a);
-    const float RT = rcp_f32(k * (2.f - c2) + gamma1 * c2);
     const float y = tau * inv_mu0;
-    const float Tn = expf(-y);
-    const float b = one_minus_exp_neg_f32(y, Tn);
+    const float Tn = exp_neg_f32(y);
     const float ek = 1.f - k_mu;
     const float x = y * ek;
     float phi = 1.f / 40320.f;
@@ -424,11 +424,16 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     phi = .5f + x * phi;
     phi = 1.f + x * phi;
     const float G = (fabsf(x) <= .5f) ? Tn * y * phi : (e1 - Tn) * rcp_f32(ek);
-    const float s = w0 * RT * rcp_f32(1.f + k_mu);
-    const float rdf = s * ((alpha2 + k * gamma3) * c2 + 2.f * (k * gamma3 - alpha2 * k_mu) * (1.f - a) * G);
-    const float tdf = -s * ((alpha1 - k * gamma4) * c2 * (1.f - b) - 2.f * k * (gamma4 + mu0 * alpha1) * G);
+    // One reciprocal for RT = 1 / (k (2 - c2) + gamma1 c2) and 1 / (1 + k mu0).
+    const float opk = 1.f + k_mu;
+    const float q = rcp_f32((k * (2.f - c2) + gamma1 * c2) * opk);
+    const float RT = q * opk;
+    const float s = w0 * q;
+    const float rdf = s * ((alpha2 + k * gamma3) * c2 + 2.f * (k * gamma3 - alpha2 * k_mu) * e1 * G);
+    const float tdf = -s * ((alpha1 - k * gamma4) * c2 * Tn - 2.f * k * (gamma4 + mu0 * alpha1) * G);
     // Clamps of rte_solver_kernels.cu:581-583 with the FP64 epsilon of the island they replace
     // (representable in FP32).
+    const float b = 1.f - Tn;
     const float eps = float(Limits<double>::eps);
     const float rd = fmaxf(eps, fminf(rdf, b));
     const float td = fmaxf(eps, fminf(tdf, b - rd));
@@ -457,6 +462,37 @@ __device__ __forceinline__ void above_update(const LayerSW& L, double& R_up, dou
     R_up = L.R + L.T * L.T * R_up * den;
 }
 
+// The direct beam below each layer of the top-down sweeps.  FP64 builds multiply the layer
+// transmittances as the reference does (F90 sw_solver_2stream).  FP32 builds evaluate
+// dir0 e^{-sum tau/mu0} from the optical depth summed in FP64, so the rounding of the hardware
+// exponential stays that of one evaluation instead of compounding down the column.  The
+// checkpoint value is the beam (FP64 builds) or the summed optical depth (FP32 builds).
+template<typename Real> struct Beam;
+template<> struct Beam<double>
+{
+    double dir;
+    __device__ void init(double dir0) { dir = dir0; }
+    __device__ void step(const LayerSW& L, double, double) { dir *= L.Tn; }
+    __device__ double saved() const { return dir; }
+    __device__ void restore(double v) { dir = v; }
+};
+template<> struct Beam<float>
+{
+    double dir0, ycum, dir;
+    __device__ void init(double d0) { dir0 = d0; ycum = 0.; dir = d0; }
+    __device__ void step(const LayerSW&, float tau, float inv_mu0)
+    {
+        ycum += double(tau * inv_mu0);
+        dir = dir0 * double(exp_neg_f32(float(ycum)));
+    }
+    __device__ float saved() const { return float(ycum); }
+    __device__ void restore(float v)
+    {
+        ycum = double(v);
+        dir = dir0 * double(exp_neg_f32(v));
+    }
+};
+
 template<typename Real>
 __global__ __launch_bounds__(SB_COLS * SB_GPTS)
 void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
@@ -473,7 +509,8 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     // Slots 0..K-1: the levels of a segment; slot K: the surface level.
     __shared__ Real red[(K + 1) * NC * SB_GPTS * SB_COLS];
 
-    const int lane = threadIdx.x, w = threadIdx.y;
+    const int lane = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.y));   // wavefront = g-point
     const int gi = blockIdx.y;
     const int gcount = grp.count[gi];
     const int64_t col0 = int64_t(blockIdx.x) * SB_COLS;
@@ -484,20 +521,29 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     const int64_t nlev = nlay + 1;
     const int64_t nseg = (nlay + K - 1) / K;
     const int64_t c2 = icol + ncol * igpt;
-    const int64_t lb = icol + ncol * nlay * igpt;
-    const int64_t vb = icol + ncol * nlev * igpt;
-    // Checkpoint q of segment s at ckpt[((s * 3 + q) * ngpt + igpt) * ncol + icol].
-    auto ck = [&](int64_t s, int q) -> Real& { return ckpt[((s * 3 + q) * ngpt + igpt) * ncol + icol]; };
-    auto lay_of = [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - 1 - j; };   // layer j from the top
-    auto lev_of = [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - j; };       // level j from the top
+    // Per-g-point slabs at wavefront-uniform bases; lane offsets icol + ncol * level (32-bit).
+    const int64_t lslab = ncol * nlay * igpt, vslab = ncol * nlev * igpt;
+    const Real* tau_g = tau + lslab;
+    const Real* ssa_g = ssa + lslab;
+    const Real* gasy_g = gasy + lslab;
+    Real* up_g = flux_up ? flux_up + vslab : nullptr;
+    Real* dn_g = flux_up ? flux_dn + vslab : nullptr;
+    Real* dir_g = flux_up ? flux_dir + vslab : nullptr;
+    const uint32_t ucol = uint32_t(lane) + uint32_t(col0), uncol = uint32_t(ncol);
+    // Checkpoint q of segment s at ck_g[(s * 3 + q) * ncol + icol], one slab per g-point.
+    Real* ck_g = ckpt + ncol * nseg * 3 * igpt;
+    auto ck = [&](int64_t s, int q) -> Real& { return at32(ck_g, ucol + uncol * uint32_t(s * 3 + q)); };
+    auto lay_of = [&](int64_t j) -> uint32_t { return uint32_t(top_at_1 ? j : nlay - 1 - j); };   // layer j from the top
+    auto lev_of = [&](int64_t j) -> uint32_t { return uint32_t(top_at_1 ? j : nlay - j); };       // level j from the top
 
     const Real m0r = active ? mu0[icol] : Real(1);
     const Real inv_m0r = Real(1) / m0r;
     const double m0 = double(m0r);
-    double R_up = 0., D = 0., dir = 0.;
+    double R_up = 0., D = 0.;
+    Beam<Real> beam;
+    beam.init(active ? double(inc_dir[c2]) * m0 : 0.);
     if (active)
     {
-        dir = double(inc_dir[c2]) * m0;
         D = inc_dif ? double(inc_dif[c2]) : 0.;
         // Sweep down: the column above every level; checkpoints at the top of each segment.  The
         // inputs of a segment are all requested before its dependent chain starts.
@@ -505,36 +551,37 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         {
             const int64_t j0 = s * K;
             const int nk = int(std::min<int64_t>(K, nlay - j0));
-            ck(s, 0) = Real(R_up); ck(s, 1) = Real(D); ck(s, 2) = Real(dir);
+            ck(s, 0) = Real(R_up); ck(s, 1) = Real(D); ck(s, 2) = Real(beam.saved());
             Real tv[K], wv[K], gv[K];
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
-                    const int64_t l = lb + ncol * lay_of(j0 + kk);
-                    tv[kk] = tau[l]; wv[kk] = ssa[l]; gv[kk] = gasy[l];
+                    const uint32_t l = ucol + uncol * lay_of(j0 + kk);
+                    tv[kk] = at32(tau_g, l); wv[kk] = at32(ssa_g, l); gv[kk] = at32(gasy_g, l);
                 }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
-                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, dir);
+                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, beam.dir);
                     above_update(L, R_up, D);
-                    dir *= L.Tn;
+                    beam.step(L, tv[kk], inv_m0r);
                 }
         }
     }
 
     // Surface.
     double A = active ? double(alb_dif[c2]) : 0.;
-    double S = dir * (active ? double(alb_dir[c2]) : 0.);
+    double S = beam.dir * (active ? double(alb_dir[c2]) : 0.);
     {
+        const double dir = beam.dir;
         const double dn = (D + R_up * S) * rcp_f64(1. - R_up * A);
         const double up = A * dn + S;
-        if (active && flux_up)
+        if (active && up_g)
         {
-            const int64_t o = vb + ncol * lev_of(nlay);
-            flux_up[o] = Real(up); flux_dn[o] = Real(dn + dir); flux_dir[o] = Real(dir);
+            const uint32_t o = ucol + uncol * lev_of(nlay);
+            at32(up_g, o) = Real(up); at32(dn_g, o) = Real(dn + dir); at32(dir_g, o) = Real(dir);
         }
         if (reduce)
         {
@@ -552,24 +599,26 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         if (active)
         {
             Real sR[K], sT[K], sSU[K], sSD[K], sRup[K], sD[K], sDir[K];
-            double r_up = double(ck(s, 0)), d = double(ck(s, 1)), dr = double(ck(s, 2));
+            double r_up = double(ck(s, 0)), d = double(ck(s, 1));
+            Beam<Real> bm = beam;
+            bm.restore(ck(s, 2));
             Real tv[K], wv[K], gv[K];
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
-                    const int64_t l = lb + ncol * lay_of(j0 + kk);
-                    tv[kk] = tau[l]; wv[kk] = ssa[l]; gv[kk] = gasy[l];
+                    const uint32_t l = ucol + uncol * lay_of(j0 + kk);
+                    tv[kk] = at32(tau_g, l); wv[kk] = at32(ssa_g, l); gv[kk] = at32(gasy_g, l);
                 }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
-                    sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(dr);
-                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, dr);
+                    sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(bm.dir);
+                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, bm.dir);
                     sR[kk] = Real(L.R); sT[kk] = Real(L.T); sSU[kk] = Real(L.SU); sSD[kk] = Real(L.SD);
                     above_update(L, r_up, d);
-                    dr *= L.Tn;
+                    bm.step(L, tv[kk], inv_m0r);
                 }
 #pragma unroll
             for (int kk = K - 1; kk >= 0; --kk)
@@ -583,10 +632,10 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                     const double dn = (double(sD[kk]) + ru * S) * rcp_f64(1. - ru * A);
                     const double up = A * dn + S;
                     const double dd = sDir[kk];
-                    if (flux_up)
+                    if (up_g)
                     {
-                        const int64_t o = vb + ncol * lev_of(j0 + kk);
-                        flux_up[o] = Real(up); flux_dn[o] = Real(dn + dd); flux_dir[o] = Real(dd);
+                        const uint32_t o = ucol + uncol * lev_of(j0 + kk);
+                        at32(up_g, o) = Real(up); at32(dn_g, o) = Real(dn + dd); at32(dir_g, o) = Real(dd);
                     }
                     if (reduce)
                     {
@@ -604,7 +653,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
             const int nsl = (s == nseg - 1) ? K + 1 : nk;
             write_group_sums<Real, 3>(red, nsl, gcount, col0, ncol, nlev, gi, outs, [&](int kk) -> int64_t {
                 if (kk == K) return lev_of(nlay);
-                return kk < nk ? lev_of(j0 + kk) : int64_t(-1);
+                return kk < nk ? int64_t(lev_of(j0 + kk)) : int64_t(-1);
             });
             __syncthreads();
         }
@@ -740,6 +789,10 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
               Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("sw_solver: empty extents");
+    // 32-bit lane offsets within one g-point's slab of levels or of checkpoints.
+    if (ncol * std::max<int64_t>(nlay + 1, 3 * ((nlay + SwSeg<Real>::K - 1) / SwSeg<Real>::K)) *
+            int64_t(sizeof(Real)) >= (int64_t(1) << 32))
+        return arg_error("sw_solver: ncol * (nlay + 1) values of one g-point must stay below 4 GiB");
     if (!tau || !ssa || !g || !mu0 || !alb_dir || !alb_dif || !inc_dir) return arg_error("sw_solver: NULL input");
     const bool spectral = flux_up || flux_dn || flux_dir;
     const bool want_bnd = bnd_up || bnd_dn || bnd_dir, want_bb = bb_up || bb_dn || bb_dir;
