@@ -371,11 +371,12 @@ __device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, d
 // One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
 struct LayerSW { double R, T, SU, SD, Tn; };
 
-// 1/x in FP64: v_rcp_f64 and two Newton steps (for the adding denominators, x in (0, 1]).
+// 1/x in FP64 for the adding denominators, x in (0, 1]: v_rcp_f64 (relative error up to 4.6e-8
+// on gfx950) and one Newton step, which leaves at most 2.3e-15, far below the FP32 rounding of
+// the fluxes and of the carried states' inputs.
 __device__ __forceinline__ double rcp_f64(double x)
 {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.), r);
+    const double r = __builtin_amdgcn_rcp(x);
     return fma(r, fma(-x, r, 1.), r);
 }
 
