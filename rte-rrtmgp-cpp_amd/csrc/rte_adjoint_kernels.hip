@@ -22,7 +22,7 @@
 // sources SU_j = Rdir_j dir_j, SD_j = Tdir_j dir_j; top-down D_{j+1} = SD + T (D + Ru SU) e1,
 // Ru_{j+1} = R + T^2 Ru e1, e1 = 1/(1 - R Ru); bottom-up A_j = R + T^2 A' e2,
 // S_j = SU + T (S' + A' SD) e2, e2 = 1/(1 - R A') (A' = A_{j+1}); fluxes dn = (D + Ru S)/(1 - Ru A),
-// up = A dn + S, total down dn + dir.  The adjoint stores these states (FP64, per column,
+// up = A dn + S, total down dn + dir.  The adjoint stores these states (in the build's precision, per column,
 // g-point and level, in a scratch buffer sized by the caller's g-point range), then runs the
 // reverse of the bottom-up recursion top-down and the reverse of the top-down recursion and the
 // direct beam bottom-up.  The layer Jacobian d(R, T, Rdir, Tdir, Tn)/d(tau, w0, g) comes from the
@@ -286,7 +286,9 @@ int lw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t n
 // Shortwave adjoint: one (column, g-point) per thread, 64 columns x 4 g-points per work-group.
 // State store per (column, g-point): the layer properties R, T, Rdir, Tdir, Tn in the build's
 // precision (the FP32 build's forward solver also works with FP32 layer properties) and the
-// level states D, Ru, dir, A, S and the reverse sweep's adjoints lA, lS in FP64.
+// level states D, Ru, dir, A, S and the reverse sweep's adjoints lA, lS, also in the build's
+// precision.  The recursions carry them in FP64; the stored values are the FP64 ones rounded,
+// and every sweep computes in FP64 from what it reads.
 constexpr int SWA_GPTS = 4;
 constexpr int SWA_NL = 5;    // stored layer properties
 constexpr int SWA_NV = 7;    // stored level states
@@ -299,7 +301,7 @@ void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                        const Real* __restrict__ alb_dif, const Real* __restrict__ inc_dir,
                        const Real* __restrict__ inc_dif, const Real* __restrict__ a_up,
                        const Real* __restrict__ a_dn, const Real* __restrict__ a_dir, Real* __restrict__ sl,
-                       double* __restrict__ sv, Real* __restrict__ d_tau, Real* __restrict__ d_ssa,
+                       Real* __restrict__ sv, Real* __restrict__ d_tau, Real* __restrict__ d_ssa,
                        Real* __restrict__ d_g, Real* __restrict__ d_alb_dir, Real* __restrict__ d_alb_dif)
 {
     const int64_t icol = int64_t(blockIdx.x) * ADJ_COLS + threadIdx.x;
@@ -313,7 +315,7 @@ void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     // Layer properties: 0 R, 1 T, 2 Rdir, 3 Tdir, 4 Tn.  Level states: 0 D, 1 Ru, 2 dir, 3 A, 4 S,
     // 5 lA, 6 lS (adjoints of A and S at the level, from the top-down reverse sweep).
     auto L = [&](int q, int64_t jj) -> Real& { return sl[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
-    auto V = [&](int q, int64_t jj) -> double& { return sv[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
+    auto V = [&](int q, int64_t jj) -> Real& { return sv[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
     const double m0 = double(mu0[icol]);
 
     // Forward, top-down: layer properties, direct beam, column above each level.
@@ -464,7 +466,7 @@ int sw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Rea
     hipStream_t s = as_stream(stream);
     const size_t per = size_t((nlay + 1) * ngpt * ncol);
     Real* sl = nullptr;
-    double* sv = nullptr;
+    Real* sv = nullptr;
     RRTMGP_TRY(scratch_alloc(&sl, SWA_NL * per, s));
     if (int rc0 = scratch_alloc(&sv, SWA_NV * per, s); rc0 != RRTMGP_OK)
     {

@@ -157,6 +157,38 @@ def test_sw_solver_adjoint_fd(G, O, okd_sw, lut_sw, syn, top_at_1):
         assert abs(ad - fd) <= 1e-6 * abs(fd), (trial, ad, fd)
 
 
+@pytest.mark.parametrize("conservative", [0.0, 0.1])
+def test_sw_solver_adjoint_fp32_vs_fp64(G, O, okd_sw, lut_sw, syn, conservative):
+    """The FP32 build's SW adjoint (FP32 inputs and state store, FP64 recursions) against the FP64
+    build's on the same FP32-representable inputs, with and without near-conservative cloud
+    layers (10% of the cloudy layers at ssa = 1 - 1e-7, g = 0.85 before delta-scaling)."""
+    atm = syn.make_atmosphere(44, 48, 60, cloudy=True, conservative_frac=conservative)
+    go = O.gas_optics(okd_sw, atm)
+    ct, cs, cg = O.Cloud(lut_sw).optics(atm.lwp, atm.iwp, atm.rel, atm.rei, True)
+    if conservative:
+        cons = atm.extra["conservative"]
+        assert cons.sum() > 0
+        cs = np.where(cons[:, :, None], 1.0 - 1e-7, cs)
+        cg = np.where(cons[:, :, None], 0.85, cg)
+    ct, cs, cg = O.delta_scale(ct, cs, cg)
+    tau, ssa, g = O.inc_bybnd(go["tau"], ct, okd_sw.band_lims, go["ssa"], go["g"], cs, cg)
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)          # noqa: E731
+    tau, ssa, g = r32(tau), r32(ssa), r32(g)
+    bl = np.asarray(okd_sw.band_lims)
+    band_of = np.concatenate([np.full(bl[1, b] - bl[0, b] + 1, b) for b in range(bl.shape[1])])
+    ncol, nlay, ngpt = tau.shape
+    rng = np.random.default_rng(6)
+    a_up, a_dn, a_dir = (r32(rng.normal(size=(ncol, nlay + 1))) for _ in range(3))
+    alb = r32(np.asarray(atm.sfc_alb, np.float64)[band_of, :].T)
+    args = (tau, ssa, g, r32(atm.mu0), r32(go["toa_src"]), alb, r32(0.5 * alb), a_up, a_dn, a_dir, False)
+    g8 = _sw_adjoint(G, *args, 8)
+    g4 = _sw_adjoint(G, *args, 4)
+    errs = {k: float(np.abs(g4[k] - g8[k]).max() / np.abs(g8[k]).max()) for k in g8}
+    print("SW adjoint FP32 vs FP64, max |diff| / max |FP64|:", errs)
+    for k, e in errs.items():
+        assert e <= 5e-6, (k, e)
+
+
 # ---------------------------------------------------------------------------------------------
 # End-to-end adjoints (rrtmgp_api_adjoint_{lw,sw}): band-by-band gas-optics recomputation, solver
 # adjoint, gas-optics adjoint and the col_dry chain, against central differences of the FP64
