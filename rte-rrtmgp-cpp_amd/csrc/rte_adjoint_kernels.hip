@@ -284,151 +284,196 @@ int lw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t n
 }
 // ---------------------------------------------------------------------------------------------
 // Shortwave adjoint: one (column, g-point) per thread, 64 columns x 4 g-points per work-group.
-// State store per (column, g-point): the layer properties R, T, Rdir, Tdir, Tn in the build's
-// precision (the FP32 build's forward solver also works with FP32 layer properties) and the
-// level states D, Ru, dir, A, S and the reverse sweep's adjoints lA, lS, also in the build's
-// precision.  The recursions carry them in FP64; the stored values are the FP64 ones rounded,
-// and every sweep computes in FP64 from what it reads.
+// State store per (column, g-point): the layer properties R, T, Rdir, Tdir, Tn and the level
+// states D, Ru, dir, A, S and the reverse sweep's adjoints lA, lS, all in the build's precision.
+// The recursions carry them in FP64; the stored values are the FP64 ones rounded, and every
+// sweep computes in FP64 from what it reads.  The four sweeps are four kernels: they communicate
+// through the store only, and the three light ones then run at the occupancy their own registers
+// allow instead of that of the dual-number sweep.
 constexpr int SWA_GPTS = 4;
 constexpr int SWA_NL = 5;    // stored layer properties
 constexpr int SWA_NV = 7;    // stored level states
 
 template<typename Real>
-__global__ __launch_bounds__(ADJ_COLS * SWA_GPTS)
-void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
-                       const Real* __restrict__ tau, const Real* __restrict__ ssa, const Real* __restrict__ gasy,
-                       const Real* __restrict__ mu0, const Real* __restrict__ alb_dir,
-                       const Real* __restrict__ alb_dif, const Real* __restrict__ inc_dir,
-                       const Real* __restrict__ inc_dif, const Real* __restrict__ a_up,
-                       const Real* __restrict__ a_dn, const Real* __restrict__ a_dir, Real* __restrict__ sl,
-                       Real* __restrict__ sv, Real* __restrict__ d_tau, Real* __restrict__ d_ssa,
-                       Real* __restrict__ d_g, Real* __restrict__ d_alb_dir, Real* __restrict__ d_alb_dif)
+struct SwAdjArgs
 {
-    const int64_t icol = int64_t(blockIdx.x) * ADJ_COLS + threadIdx.x;
-    const int64_t igpt = int64_t(blockIdx.y) * SWA_GPTS + threadIdx.y;
-    if (icol >= ncol || igpt >= ngpt) return;     // no barriers in this kernel
-    const int64_t n = nlay, nlev = nlay + 1;
-    const int64_t c2 = icol + ncol * igpt;
-    const int64_t lb = icol + ncol * nlay * igpt;
-    auto lay = [&](int64_t j) -> int64_t { return lb + ncol * (top_at_1 ? j : n - 1 - j); };
-    auto lev = [&](int64_t j) -> int64_t { return icol + ncol * (top_at_1 ? j : n - j); };
-    // Layer properties: 0 R, 1 T, 2 Rdir, 3 Tdir, 4 Tn.  Level states: 0 D, 1 Ru, 2 dir, 3 A, 4 S,
-    // 5 lA, 6 lS (adjoints of A and S at the level, from the top-down reverse sweep).
-    auto L = [&](int q, int64_t jj) -> Real& { return sl[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
-    auto V = [&](int q, int64_t jj) -> Real& { return sv[((int64_t(q) * nlev + jj) * ngpt + igpt) * ncol + icol]; };
-    const double m0 = double(mu0[icol]);
+    int64_t ncol, nlay, ngpt;
+    int top_at_1;
+    const Real *tau, *ssa, *gasy, *mu0, *alb_dir, *alb_dif, *inc_dir, *inc_dif, *a_up, *a_dn, *a_dir;
+    Real *sl, *sv, *d_tau, *d_ssa, *d_g, *d_alb_dir, *d_alb_dif;
+};
 
-    // Forward, top-down: layer properties, direct beam, column above each level.
-    {
-        double dir = double(inc_dir[c2]) * m0, D = inc_dif ? double(inc_dif[c2]) : 0., Ru = 0.;
-        for (int64_t jj = 0; jj < n; ++jj)
-        {
-            V(0, jj) = D; V(1, jj) = Ru; V(2, jj) = dir;
-            const int64_t l = lay(jj);
-            double R, T, Rd, Td, Tn;
-            sw_two_stream_t<double>(double(tau[l]), double(ssa[l]), double(gasy[l]), m0, R, T, Rd, Td, Tn);
-            // The stored (build-precision) values are the ones every later sweep uses.
-            L(0, jj) = Real(R); L(1, jj) = Real(T); L(2, jj) = Real(Rd); L(3, jj) = Real(Td); L(4, jj) = Real(Tn);
-            R = double(Real(R)); T = double(Real(T)); Rd = double(Real(Rd)); Td = double(Real(Td));
-            Tn = double(Real(Tn));
-            const double SU = Rd * dir, SD = Td * dir;
-            const double e1 = 1. / (1. - R * Ru);
-            D = SD + T * (D + Ru * SU) * e1;
-            Ru = R + T * T * Ru * e1;
-            dir *= Tn;
-        }
-        V(0, n) = D; V(1, n) = Ru; V(2, n) = dir;
-    }
-    // Forward, bottom-up: column below each level.
-    const double ad = double(alb_dir[c2]), af = double(alb_dif[c2]);
-    {
-        double A = af, S = V(2, n) * ad;
-        V(3, n) = A; V(4, n) = S;
-        for (int64_t jj = n - 1; jj >= 0; --jj)
-        {
-            const double R = double(L(0, jj)), T = double(L(1, jj)), dir = V(2, jj);
-            const double SU = double(L(2, jj)) * dir, SD = double(L(3, jj)) * dir;
-            const double e2 = 1. / (1. - R * A);
-            S = SU + T * (S + A * SD) * e2;
-            A = R + T * T * A * e2;
-            V(3, jj) = A; V(4, jj) = S;
-        }
-    }
+// Indexing of one thread's (column, g-point).  Layer properties: 0 R, 1 T, 2 Rdir, 3 Tdir, 4 Tn.
+// Level states: 0 D, 1 Ru, 2 dir, 3 A, 4 S, 5 lA, 6 lS (adjoints of A and S at the level, from
+// the top-down reverse sweep; slot n of 5 and 6 holds the sweep's final values).
+template<typename Real>
+struct SwAdjAt
+{
+    const SwAdjArgs<Real>& a;
+    int64_t icol, igpt, n, nlev, c2, lb;
+    __device__ SwAdjAt(const SwAdjArgs<Real>& args, int64_t ic, int64_t ig)
+        : a(args), icol(ic), igpt(ig), n(args.nlay), nlev(args.nlay + 1), c2(ic + args.ncol * ig),
+          lb(ic + args.ncol * args.nlay * ig) {}
+    __device__ int64_t lay(int64_t j) const { return lb + a.ncol * (a.top_at_1 ? j : n - 1 - j); }
+    __device__ int64_t lev(int64_t j) const { return icol + a.ncol * (a.top_at_1 ? j : n - j); }
+    __device__ Real& L(int q, int64_t jj) const { return a.sl[((int64_t(q) * nlev + jj) * a.ngpt + igpt) * a.ncol + icol]; }
+    __device__ Real& V(int q, int64_t jj) const { return a.sv[((int64_t(q) * nlev + jj) * a.ngpt + igpt) * a.ncol + icol]; }
+
     // Flux cotangents at level j: gradients with respect to D, Ru, A, S.
-    auto level_partials = [&](int64_t jj, double D, double Ru, double A, double S, double& gD, double& gRu,
-                              double& gA, double& gS) {
-        const double au = double(a_up[lev(jj)]), adn = double(a_dn[lev(jj)]);
+    __device__ void level_partials(int64_t jj, double D, double Ru, double A, double S, double& gD, double& gRu,
+                                   double& gA, double& gS) const
+    {
+        const double au = double(a.a_up[lev(jj)]), adn = double(a.a_dn[lev(jj)]);
         const double q = D + Ru * S, den = 1. / (1. - Ru * A), dn = q * den;
         const double gdn = adn + au * A;
         gD = gdn * den;
         gRu = gdn * (S * den + q * A * den * den);
         gA = gdn * q * Ru * den * den + au * dn;
         gS = gdn * Ru * den + au;
-    };
-    // Contributions of the bottom-up recursion's adjoint at layer j to the layer's R, T, SU, SD
-    // cotangents, from the adjoints lA, lS of A_j, S_j.
-    auto bottom_up_partials = [](double lA, double lS, double R, double T, double SD, double A1, double S1,
-                                 double& gR, double& gT, double& gSU, double& gSD) {
-        const double e2 = 1. / (1. - R * A1);
-        const double w = S1 + A1 * SD;
-        gR = lA * (1. + T * T * A1 * A1 * e2 * e2) + lS * T * w * A1 * e2 * e2;
-        gT = lA * 2. * T * A1 * e2 + lS * w * e2;
-        gSU = lS;
-        gSD = lS * T * A1 * e2;
-    };
-
-    // Reverse of the bottom-up recursion, top-down.
-    double lA = 0., lS = 0.;
-    {
-        double A = V(3, 0), S = V(4, 0);
-        for (int64_t jj = 0; jj <= n; ++jj)
-        {
-            double gD, gRu, gA, gS;
-            level_partials(jj, V(0, jj), V(1, jj), A, S, gD, gRu, gA, gS);
-            lA += gA; lS += gS;
-            if (jj == n) break;
-            V(5, jj) = lA; V(6, jj) = lS;
-            const double R = double(L(0, jj)), T = double(L(1, jj)), dir = V(2, jj);
-            const double SD = double(L(3, jj)) * dir;
-            const double A1 = V(3, jj + 1), S1 = V(4, jj + 1);
-            const double e2 = 1. / (1. - R * A1);
-            const double w = S1 + A1 * SD;
-            const double nA = lA * T * T * e2 * e2 + lS * T * e2 * (SD + w * R * e2);
-            lS = lS * T * e2;
-            lA = nA;
-            A = A1; S = S1;
-        }
     }
-    d_alb_dif[c2] = Real(lA);
-    d_alb_dir[c2] = Real(lS * V(2, n));
+};
 
-    // Reverse of the top-down recursion and the direct beam, bottom-up.
-    double lD, lRu, ldir;
-    double A1 = V(3, n), S1 = V(4, n);
+// Contributions of the bottom-up recursion's adjoint at layer j to the layer's R, T, SU, SD
+// cotangents, from the adjoints lA, lS of A_j, S_j.
+__device__ __forceinline__ void bottom_up_partials(double lA, double lS, double R, double T, double SD, double A1,
+                                                   double S1, double& gR, double& gT, double& gSU, double& gSD)
+{
+    const double e2 = 1. / (1. - R * A1);
+    const double w = S1 + A1 * SD;
+    gR = lA * (1. + T * T * A1 * A1 * e2 * e2) + lS * T * w * A1 * e2 * e2;
+    gT = lA * 2. * T * A1 * e2 + lS * w * e2;
+    gSU = lS;
+    gSD = lS * T * A1 * e2;
+}
+
+template<typename Real>
+__device__ __forceinline__ bool sw_adj_thread(const SwAdjArgs<Real>& a, int64_t& icol, int64_t& igpt)
+{
+    icol = int64_t(blockIdx.x) * ADJ_COLS + threadIdx.x;
+    igpt = int64_t(blockIdx.y) * SWA_GPTS + threadIdx.y;
+    return icol < a.ncol && igpt < a.ngpt;
+}
+
+// Sweep 1, forward top-down: layer properties, direct beam, column above each level.
+template<typename Real>
+__global__ __launch_bounds__(ADJ_COLS * SWA_GPTS)
+void sw_adjoint_down_kernel(SwAdjArgs<Real> a)
+{
+    int64_t icol, igpt;
+    if (!sw_adj_thread(a, icol, igpt)) return;
+    const SwAdjAt<Real> t(a, icol, igpt);
+    const double m0 = double(a.mu0[icol]);
+    double dir = double(a.inc_dir[t.c2]) * m0, D = a.inc_dif ? double(a.inc_dif[t.c2]) : 0., Ru = 0.;
+    for (int64_t jj = 0; jj < t.n; ++jj)
+    {
+        t.V(0, jj) = Real(D); t.V(1, jj) = Real(Ru); t.V(2, jj) = Real(dir);
+        const int64_t l = t.lay(jj);
+        double R, T, Rd, Td, Tn;
+        sw_two_stream_t<double>(double(a.tau[l]), double(a.ssa[l]), double(a.gasy[l]), m0, R, T, Rd, Td, Tn);
+        // The stored (build-precision) values are the ones every later sweep uses.
+        t.L(0, jj) = Real(R); t.L(1, jj) = Real(T); t.L(2, jj) = Real(Rd); t.L(3, jj) = Real(Td); t.L(4, jj) = Real(Tn);
+        R = double(Real(R)); T = double(Real(T)); Rd = double(Real(Rd)); Td = double(Real(Td));
+        Tn = double(Real(Tn));
+        const double SU = Rd * dir, SD = Td * dir;
+        const double e1 = 1. / (1. - R * Ru);
+        D = SD + T * (D + Ru * SU) * e1;
+        Ru = R + T * T * Ru * e1;
+        dir *= Tn;
+    }
+    t.V(0, t.n) = Real(D); t.V(1, t.n) = Real(Ru); t.V(2, t.n) = Real(dir);
+}
+
+// Sweep 2, forward bottom-up: column below each level.
+template<typename Real>
+__global__ __launch_bounds__(ADJ_COLS * SWA_GPTS)
+void sw_adjoint_up_kernel(SwAdjArgs<Real> a)
+{
+    int64_t icol, igpt;
+    if (!sw_adj_thread(a, icol, igpt)) return;
+    const SwAdjAt<Real> t(a, icol, igpt);
+    double A = double(a.alb_dif[t.c2]), S = double(t.V(2, t.n)) * double(a.alb_dir[t.c2]);
+    t.V(3, t.n) = Real(A); t.V(4, t.n) = Real(S);
+    for (int64_t jj = t.n - 1; jj >= 0; --jj)
+    {
+        const double R = double(t.L(0, jj)), T = double(t.L(1, jj)), dir = double(t.V(2, jj));
+        const double SU = double(t.L(2, jj)) * dir, SD = double(t.L(3, jj)) * dir;
+        const double e2 = 1. / (1. - R * A);
+        S = SU + T * (S + A * SD) * e2;
+        A = R + T * T * A * e2;
+        t.V(3, jj) = Real(A); t.V(4, jj) = Real(S);
+    }
+}
+
+// Sweep 3: reverse of the bottom-up recursion, top-down; albedo gradients.
+template<typename Real>
+__global__ __launch_bounds__(ADJ_COLS * SWA_GPTS)
+void sw_adjoint_rev_down_kernel(SwAdjArgs<Real> a)
+{
+    int64_t icol, igpt;
+    if (!sw_adj_thread(a, icol, igpt)) return;
+    const SwAdjAt<Real> t(a, icol, igpt);
+    const int64_t n = t.n;
+    double lA = 0., lS = 0.;
+    double A = double(t.V(3, 0)), S = double(t.V(4, 0));
+    for (int64_t jj = 0; jj <= n; ++jj)
     {
         double gD, gRu, gA, gS;
-        level_partials(n, V(0, n), V(1, n), A1, S1, gD, gRu, gA, gS);
+        t.level_partials(jj, double(t.V(0, jj)), double(t.V(1, jj)), A, S, gD, gRu, gA, gS);
+        lA += gA; lS += gS;
+        if (jj == n) break;
+        t.V(5, jj) = Real(lA); t.V(6, jj) = Real(lS);
+        const double R = double(t.L(0, jj)), T = double(t.L(1, jj)), dir = double(t.V(2, jj));
+        const double SD = double(t.L(3, jj)) * dir;
+        const double A1 = double(t.V(3, jj + 1)), S1 = double(t.V(4, jj + 1));
+        const double e2 = 1. / (1. - R * A1);
+        const double w = S1 + A1 * SD;
+        const double nA = lA * T * T * e2 * e2 + lS * T * e2 * (SD + w * R * e2);
+        lS = lS * T * e2;
+        lA = nA;
+        A = A1; S = S1;
+    }
+    t.V(5, n) = Real(lA); t.V(6, n) = Real(lS);
+    a.d_alb_dif[t.c2] = Real(lA);
+    a.d_alb_dir[t.c2] = Real(lS * double(t.V(2, n)));
+}
+
+// Sweep 4: reverse of the top-down recursion and the direct beam, bottom-up; layer gradients.
+template<typename Real>
+__global__ __launch_bounds__(ADJ_COLS * SWA_GPTS)
+void sw_adjoint_rev_up_kernel(SwAdjArgs<Real> a)
+{
+    int64_t icol, igpt;
+    if (!sw_adj_thread(a, icol, igpt)) return;
+    const SwAdjAt<Real> t(a, icol, igpt);
+    const int64_t n = t.n;
+    const double m0 = double(a.mu0[icol]);
+    const double ad = double(a.alb_dir[t.c2]);
+    double lD, lRu, ldir;
+    double A1 = double(t.V(3, n)), S1 = double(t.V(4, n));
+    {
+        double gD, gRu, gA, gS;
+        t.level_partials(n, double(t.V(0, n)), double(t.V(1, n)), A1, S1, gD, gRu, gA, gS);
         lD = gD; lRu = gRu;
-        ldir = double(a_dn[lev(n)]) + double(a_dir[lev(n)]) + lS * ad;
+        ldir = double(a.a_dn[t.lev(n)]) + double(a.a_dir[t.lev(n)]) + double(t.V(6, n)) * ad;
     }
     for (int64_t jj = n - 1; jj >= 0; --jj)
     {
         // Layer properties and their Jacobian by forward mode on (tau, w0, g); the values are
         // rounded to the stored precision, as the earlier sweeps used them.
-        const int64_t l = lay(jj);
+        const int64_t l = t.lay(jj);
         Dual3 oR, oT, oRd, oTd, oTn;
-        sw_two_stream_t<Dual3>(mk(double(tau[l]), 1., 0., 0.), mk(double(ssa[l]), 0., 1., 0.),
-                               mk(double(gasy[l]), 0., 0., 1.), m0, oR, oT, oRd, oTd, oTn);
+        sw_two_stream_t<Dual3>(mk(double(a.tau[l]), 1., 0., 0.), mk(double(a.ssa[l]), 0., 1., 0.),
+                               mk(double(a.gasy[l]), 0., 0., 1.), m0, oR, oT, oRd, oTd, oTn);
         const double R = double(Real(oR.v)), T = double(Real(oT.v)), Rd = double(Real(oRd.v));
         const double Td = double(Real(oTd.v)), Tn = double(Real(oTn.v));
-        const double D = V(0, jj), Ru = V(1, jj), dir = V(2, jj);
+        const double D = double(t.V(0, jj)), Ru = double(t.V(1, jj)), dir = double(t.V(2, jj));
         const double SU = Rd * dir, SD = Td * dir;
         // A_j, S_j rebuilt from A_{j+1}, S_{j+1} exactly as the bottom-up forward sweep formed them.
         const double e2f = 1. / (1. - R * A1);
         const double S = SU + T * (S1 + A1 * SD) * e2f;
         const double A = R + T * T * A1 * e2f;
         double bR, bT, bSU, bSD;
-        bottom_up_partials(V(5, jj), V(6, jj), R, T, SD, A1, S1, bR, bT, bSU, bSD);
+        bottom_up_partials(double(t.V(5, jj)), double(t.V(6, jj)), R, T, SD, A1, S1, bR, bT, bSU, bSD);
         const double e1 = 1. / (1. - R * Ru);
         const double v = D + Ru * SU;
         const double gSD = bSD + lD;
@@ -443,12 +488,12 @@ void sw_adjoint_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             g3[c] = gR * oR.d[c] + gT * oT.d[c] + gRd * oRd.d[c] + gTd * oTd.d[c] + gTn * oTn.d[c];
-        d_tau[l] = Real(g3[0]); d_ssa[l] = Real(g3[1]); d_g[l] = Real(g3[2]);
+        a.d_tau[l] = Real(g3[0]); a.d_ssa[l] = Real(g3[1]); a.d_g[l] = Real(g3[2]);
         double gD, gRu, gA, gS;
-        level_partials(jj, D, Ru, A, S, gD, gRu, gA, gS);
+        t.level_partials(jj, D, Ru, A, S, gD, gRu, gA, gS);
         lD = nD + gD;
         lRu = nRu + gRu;
-        ldir = ndir + double(a_dn[lev(jj)]) + double(a_dir[lev(jj)]);
+        ldir = ndir + double(a.a_dn[t.lev(jj)]) + double(a.a_dir[t.lev(jj)]);
         A1 = A; S1 = S;
     }
 }
@@ -473,15 +518,33 @@ int sw_adjoint(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Rea
         scratch_free(sl, s);
         return rc0;
     }
+    const SwAdjArgs<Real> args{ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir, inc_dif,
+                               a_up, a_dn, a_dir, sl, sv, d_tau, d_ssa, d_g, d_alb_dir, d_alb_dif};
     const dim3 grid(grid1(ncol, ADJ_COLS), grid1(ngpt, SWA_GPTS));
     const dim3 block(ADJ_COLS, SWA_GPTS);
-    int rc;
+    int rc = RRTMGP_OK;
     {
-        RRTMGP_TIMED("sw_adjoint_kernel", s);
-        hipLaunchKernelGGL(sw_adjoint_kernel<Real>, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0,
-                           alb_dir, alb_dif, inc_dir, inc_dif, a_up, a_dn, a_dir, sl, sv, d_tau, d_ssa, d_g,
-                           d_alb_dir, d_alb_dif);
-        rc = check_hip(hipGetLastError(), "sw_adjoint_kernel");
+        RRTMGP_TIMED("sw_adjoint_down_kernel", s);
+        hipLaunchKernelGGL(sw_adjoint_down_kernel<Real>, grid, block, 0, s, args);
+        rc = check_hip(hipGetLastError(), "sw_adjoint_down_kernel");
+    }
+    if (rc == RRTMGP_OK)
+    {
+        RRTMGP_TIMED("sw_adjoint_up_kernel", s);
+        hipLaunchKernelGGL(sw_adjoint_up_kernel<Real>, grid, block, 0, s, args);
+        rc = check_hip(hipGetLastError(), "sw_adjoint_up_kernel");
+    }
+    if (rc == RRTMGP_OK)
+    {
+        RRTMGP_TIMED("sw_adjoint_rev_down_kernel", s);
+        hipLaunchKernelGGL(sw_adjoint_rev_down_kernel<Real>, grid, block, 0, s, args);
+        rc = check_hip(hipGetLastError(), "sw_adjoint_rev_down_kernel");
+    }
+    if (rc == RRTMGP_OK)
+    {
+        RRTMGP_TIMED("sw_adjoint_rev_up_kernel", s);
+        hipLaunchKernelGGL(sw_adjoint_rev_up_kernel<Real>, grid, block, 0, s, args);
+        rc = check_hip(hipGetLastError(), "sw_adjoint_rev_up_kernel");
     }
     scratch_free(sv, s);
     scratch_free(sl, s);
