@@ -30,11 +30,15 @@ def bf16(x):
     return b.view(np.float32).astype(np.float64)
 
 
-def assert_close_log(got, ref, name):
+def assert_close_log(got, ref, name, spread=None):
     """Log-space outputs of the kernel against the numpy network: within 5e-3 (+3e-4 relative)
     everywhere but isolated elements, where a BF16 rounding flip of one hidden unit moves the output
-    by up to a few 1e-3 more; none beyond 2e-2."""
+    by up to a few 1e-3 more; none beyond 2e-2.  spread (per output column) scales the bounds for
+    outputs whose normalisation is wider than the plain standard deviation (loss weights below 1
+    widen it by 1/sqrt(w), and a flip moves them by as much more)."""
     d = np.abs(got - ref)
+    if spread is not None:
+        d = d / np.maximum(np.asarray(spread)[None, :], 1.0)
     tol = 5e-3 + 3e-4 * np.abs(ref)
     assert d.max() < 2e-2, (name, float(d.max()))
     assert np.count_nonzero(d > tol) <= max(1, d.size // 100000), (name, int(np.count_nonzero(d > tol)))
@@ -250,7 +254,8 @@ def test_inference_matches_numpy(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw, lw):
     cdh = cd.to_host().ravel(order="F")[:, None]
     tau = o["tau"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
     if lw:
-        assert_close_log(np.log(tau / cdh), y[:, 0::2], "ln tau/col_dry")
+        w = net.loss_weights()
+        assert_close_log(np.log(tau / cdh), y[:, 0::2], "ln tau/col_dry", spread=1.0 / np.sqrt(w[0::2]))
         pf_plain = plain["lay_src"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
         src = o["lay_src"].to_host().reshape(ncol * nlay, kd.ngpt, order="F")
         # lay_src = pf B_band(T_lay): the ratio to the plain source is pf_mlp / pf_plain.
