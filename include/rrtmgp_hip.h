@@ -378,6 +378,34 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
         void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
 
+/* Longwave with factored Planck sources.  No counterpart in the reference, whose gas optics */
+/* writes lay_source, lev_source_inc and lev_source_dec per g-point (Source_func_lw,          */
+/* include/Source_functions.h:73-99) for Rte_lw to read.  Every one of them is a g-point's     */
+/* Planck fraction times its band's Planck function at the layer or level.  The factored pair */
+/* keeps that product unevaluated:                                                              */
+/*   gas_optics_lw_factored   writes tau, planck_frac (ncol, nlay, ngpt), band_lay_src         */
+/*                            (ncol, nlay, nbnd) = B_band(T_lay), band_lev_src (ncol, nlev,    */
+/*                            nbnd) = B_band(T_lev), and sfc_src / sfc_src_jac as the plain    */
+/*                            entry;                                                            */
+/*   lw_solver_noscat_bnd_factored  reads them and forms lay_source = planck_frac band_lay_src,*/
+/*                            lev_source_inc (layer i) = planck_frac band_lev_src (level i+1)  */
+/*                            and lev_source_dec = planck_frac band_lev_src (level i), the same*/
+/*                            FP products, so fluxes equal the plain pair's bit for bit.       */
+/* Two arrays of (ncol, nlay, ngpt) are neither written nor read.  Used by the longwave driver */
+/* (Radiation_solver_longwave_gpu) when no surrogate is set.                                    */
+#define RRTMGP_DECL_LW_FACTORED(S, T)                                                          \
+    int rrtmgp_hip_gas_optics_lw_factored_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,       \
+        int64_t nlay, const T* play, const T* tlay, const T* tlev, const T* tsfc,              \
+        const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau,        \
+        T* tau, T* planck_frac, T* band_lay_src, T* band_lev_src, T* sfc_src, T* sfc_src_jac,  \
+        void* stream);                                                                          \
+    int rrtmgp_hip_lw_solver_noscat_bnd_factored_##S(int64_t ncol, int64_t nlay, int64_t ngpt, \
+        int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,           \
+        const T* planck_frac, const T* band_lay_src, const T* band_lev_src, const T* sfc_emis,  \
+        const T* sfc_src, const T* inc_flux, int64_t nbnd, const int* band_lims_gpt,           \
+        T* bnd_flux_up, T* bnd_flux_dn, T* broadband_up, T* broadband_dn, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_FACTORED)
+
 /* ------------------------------------------------------------------------------------------ */
 /* Adjoints.  No counterpart in the reference (its LW path has only the surface-temperature   */
 /* Jacobian, src/Rte_lw.cpp:140).                                                              */

@@ -587,7 +587,8 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                           const Real* __restrict__ cloud_tau, Real* __restrict__ tau,
                           Real* __restrict__ lay_src, Real* __restrict__ lev_inc,
                           Real* __restrict__ lev_dec, Real* __restrict__ sfc_src,
-                          Real* __restrict__ sfc_jac)
+                          Real* __restrict__ sfc_jac, Real* __restrict__ bnd_lay,
+                          Real* __restrict__ bnd_lev)
 {
     static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
@@ -645,6 +646,32 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const int64_t idx = icol + ncol * ilay;
     const Real ctau = cloud_tau ? cloud_tau[idx + ncl * un.ibnd] : Real(0);
     const int gbase = bd.g0 - k.bands[grid.b0].g0 + un.c0;
+    if (bnd_lay)
+    {
+        // Factored sources: Planck fractions per g-point, band Planck values per layer and level
+        // (written once, by the band's first chunk; level nlay by the top layer).
+        if (w == 0 && un.c0 == 0)
+        {
+            const int64_t nlev = nlay + 1, bb = un.ibnd - grid.b0;
+            bnd_lay[idx + ncl * bb] = st.aux[0];
+            bnd_lev[icol + ncol * ilay + ncol * nlev * bb] = st.aux[2];
+            if (ilay == nlay - 1) bnd_lev[icol + ncol * nlay + ncol * nlev * bb] = st.aux[1];
+        }
+        for (int row = w; row < un.ncg; row += GO_THREADS / 64)
+        {
+            const int gr = gbase + row;
+            const int64_t o = idx + ncl * gr;
+            const Real pf = s_pf[row][lane];
+            tau[o] = cloud_tau ? s_tau[row][lane] + ctau : s_tau[row][lane];
+            lay_src[o] = pf;
+            if (at_sfc)
+            {
+                sfc_src[icol + ncol * gr] = pf * st.aux[3];
+                sfc_jac[icol + ncol * gr] = pf * (st.aux[4] - st.aux[3]);
+            }
+        }
+        return;
+    }
     for (int row = w; row < un.ncg; row += GO_THREADS / 64)
     {
         const int gr = gbase + row;
@@ -1583,12 +1610,16 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
                   const Real* tlay, const Real* tlev, const Real* tsfc, const Real* col_dry,
                   const rrtmgp_gas_vmr* gases, int sfc_lay, const Real* cloud_tau, Real* tau,
                   Real* lay_src, Real* lev_inc, Real* lev_dec, Real* sfc_src, Real* sfc_jac,
-                  void* stream, const int* tiles = nullptr, int64_t ntiles = 0)
+                  void* stream, const int* tiles = nullptr, int64_t ntiles = 0, Real* bnd_lay = nullptr,
+                  Real* bnd_lev = nullptr)
 {
     RRTMGP_TRY(check_kd_fused(kd, true));
     if (ncol <= 0 || nlay <= 0) return arg_error("ncol and nlay must be positive");
     if (sfc_lay != 1 && sfc_lay != nlay) return arg_error("sfc_lay must be 1 or nlay");
-    if (!play || !tlay || !tlev || !tsfc || !col_dry || !tau || !lay_src || !lev_inc || !lev_dec || !sfc_src || !sfc_jac)
+    const bool factored = bnd_lay != nullptr;
+    if (factored != (bnd_lev != nullptr)) return arg_error("gas_optics_lw: band Planck arrays go together");
+    if (!play || !tlay || !tlev || !tsfc || !col_dry || !tau || !lay_src || !sfc_src || !sfc_jac ||
+        (!factored && (!lev_inc || !lev_dec)))
         return arg_error("gas_optics_lw: NULL array");
     Gases<Real> g;
     RRTMGP_TRY(make_gases(gases, kd->ngas, ncol, nlay, g));
@@ -1602,7 +1633,7 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
     RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
                        as_stream(stream), k, g, ncol, nlay, grid, play, tlay, tlev, tsfc, col_dry,
-                       sfc_lay, cloud_tau, tau, lay_src, lev_inc, lev_dec, sfc_src, sfc_jac);
+                       sfc_lay, cloud_tau, tau, lay_src, lev_inc, lev_dec, sfc_src, sfc_jac, bnd_lay, bnd_lev);
     RRTMGP_LAUNCHED("gas_optics_lw_kernel");
     return RRTMGP_OK;
 }
@@ -1723,6 +1754,16 @@ extern "C" int rrtmgp_hip_gas_optics_lw_##S(const rrtmgp_kdist_dev* kd, int64_t 
     return gas_optics_lw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, tlev, tsfc, col_dry, \
                             gases, sfc_lay, cloud_tau, tau, lay_src, lev_src_inc, lev_src_dec,     \
                             sfc_src, sfc_src_jac, stream);                                         \
+}                                                                                                  \
+extern "C" int rrtmgp_hip_gas_optics_lw_factored_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,     \
+    int64_t nlay, const T* play, const T* tlay, const T* tlev, const T* tsfc, const T* col_dry,     \
+    const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau, T* tau, T* planck_frac,          \
+    T* band_lay_src, T* band_lev_src, T* sfc_src, T* sfc_src_jac, void* stream)                    \
+{                                                                                                  \
+    if (!band_lay_src || !band_lev_src) return arg_error("gas_optics_lw_factored: NULL band array"); \
+    return gas_optics_lw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, tlev, tsfc, col_dry, \
+                            gases, sfc_lay, cloud_tau, tau, planck_frac, nullptr, nullptr,         \
+                            sfc_src, sfc_src_jac, stream, nullptr, 0, band_lay_src, band_lev_src); \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_gas_optics_lw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,  \
     int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* tlev,    \

@@ -207,7 +207,7 @@ __device__ __forceinline__ void lw_layer(float t, float& T, float& fact)
 // ---------------------------------------------------------------------------------------------
 // Longwave: no scattering, linear-in-tau source, nmus <= NMU Gaussian angles.  The layer inputs
 // of the next K-level segment are read while the current segment is computed.
-template<typename Real, bool JAC, int NMU>
+template<typename Real, bool JAC, int NMU, bool FACT>
 __global__ __launch_bounds__(SB_COLS * SB_GPTS) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 8 : 4)))
 void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int nmus,
                       const Real* __restrict__ secants, const Real* __restrict__ weights,
@@ -217,7 +217,8 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
                       const Real* __restrict__ inc_flux, Real* __restrict__ flux_up,
                       Real* __restrict__ flux_dn, const Real* __restrict__ sfc_src_jac,
                       Groups grp, Real* __restrict__ grp_up, Real* __restrict__ grp_dn,
-                      Real* __restrict__ grp_jac)
+                      Real* __restrict__ grp_jac, const Real* __restrict__ bnd_lay,
+                      const Real* __restrict__ bnd_lev)
 {
     constexpr int K = NMU == 1 ? LwSeg<Real>::K1 : LwSeg<Real>::K4;
     constexpr int NC = JAC ? 2 : 1;
@@ -238,8 +239,12 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
     const int64_t lslab = ncol * nlay * igpt, vslab = ncol * nlev * igpt;
     const Real* tau_g = tau + lslab;
     const Real* lay_g = lay_src + lslab;
-    const Real* lev_dn_g = (top_at_1 ? lev_inc : lev_dec) + lslab;
-    const Real* lev_up_g = (top_at_1 ? lev_dec : lev_inc) + lslab;
+    const Real* lev_dn_g = FACT ? nullptr : (top_at_1 ? lev_inc : lev_dec) + lslab;
+    const Real* lev_up_g = FACT ? nullptr : (top_at_1 ? lev_dec : lev_inc) + lslab;
+    // Factored sources (FACT): lay_src holds Planck fractions; the group's band Planck values per
+    // layer and level come from bnd_lay (ncol, nlay, nbnd) and bnd_lev (ncol, nlev, nbnd).
+    const Real* blay_g = FACT ? bnd_lay + ncol * nlay * grp.band[gi] : nullptr;
+    const Real* blev_g = FACT ? bnd_lev + ncol * nlev * grp.band[gi] : nullptr;
     Real* flux_up_g = flux_up ? flux_up + vslab : nullptr;
     Real* flux_dn_g = flux_dn ? flux_dn + vslab : nullptr;
     const uint32_t ucol = uint32_t(lane) + uint32_t(col0), uncol = uint32_t(ncol);
@@ -258,9 +263,11 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
         }
     }
 
-    // Inputs of levels s0 .. s0+K-1 of a sweep (level j > 0 ends layer layer_of(j)).
+    // Inputs of levels s0 .. s0+K-1 of a sweep (level j > 0 ends layer layer_of(j)).  With factored
+    // sources the level source of layer L is at level L + lsh (lev_src_inc: L + 1, lev_src_dec: L),
+    // and the products are the gas optics' own (no contraction), so the sources are bit-identical.
     Real bt[K], bl[K], bx[K];
-    auto fetch = [&](int64_t s0, const Real* lev_g, auto layer_of) {
+    auto fetch = [&](int64_t s0, const Real* lev_g, int lsh, auto layer_of) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk)
         {
@@ -268,20 +275,32 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
             bt[kk] = Real(0); bl[kk] = Real(0); bx[kk] = Real(0);
             if (active && j >= 1 && j <= nlay)
             {
-                const uint32_t l = ucol + uncol * uint32_t(layer_of(j));
-                bt[kk] = at32(tau_g, l); bl[kk] = at32(lay_g, l); bx[kk] = at32(lev_g, l);
+                const uint32_t L = uint32_t(layer_of(j));
+                const uint32_t l = ucol + uncol * L;
+                bt[kk] = at32(tau_g, l);
+                if constexpr (FACT)
+                {
+#pragma clang fp contract(off)
+                    const Real pf = at32(lay_g, l);
+                    bl[kk] = pf * at32(blay_g, l);
+                    bx[kk] = pf * at32(blev_g, ucol + uncol * (L + uint32_t(lsh)));
+                }
+                else
+                {
+                    bl[kk] = at32(lay_g, l); bx[kk] = at32(lev_g, l);
+                }
             }
         }
     };
     // One sweep over levels 0..nlay; lev_out(j) is the output level of sweep level j.
-    auto sweep = [&](const Real* lev_g, auto layer_of, auto lev_out, Real* flux_g, Real* grp_out, bool up) {
-        fetch(0, lev_g, layer_of);
+    auto sweep = [&](const Real* lev_g, int lsh, auto layer_of, auto lev_out, Real* flux_g, Real* grp_out, bool up) {
+        fetch(0, lev_g, lsh, layer_of);
         for (int64_t s0 = 0; s0 <= nlay; s0 += K)
         {
             Real ct[K], cl[K], cx[K];
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) { ct[kk] = bt[kk]; cl[kk] = bl[kk]; cx[kk] = bx[kk]; }
-            if (s0 + K <= nlay) fetch(s0 + K, lev_g, layer_of);
+            if (s0 + K <= nlay) fetch(s0 + K, lev_g, lsh, layer_of);
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
             {
@@ -338,7 +357,7 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
     };
 
     // Down sweep: level j counted from the top (0 = top boundary).
-    sweep(lev_dn_g,
+    sweep(lev_dn_g, top_at_1 ? 1 : 0,
           [&](int64_t j) -> int64_t { return top_at_1 ? j - 1 : nlay - j; },
           [&](int64_t j) -> int64_t { return top_at_1 ? j : nlay - j; }, flux_dn_g, grp_dn, false);
 
@@ -355,7 +374,7 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
             Ij[mu] = e * sj;
         }
     }
-    sweep(lev_up_g,
+    sweep(lev_up_g, top_at_1 ? 0 : 1,
           [&](int64_t j) -> int64_t { return top_at_1 ? nlay - j : j - 1; },
           [&](int64_t j) -> int64_t { return top_at_1 ? nlay - j : j; }, flux_up_g, grp_up, true);
 }
@@ -727,13 +746,17 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
               const Real* weights, const Real* tau, const Real* lay, const Real* linc, const Real* ldec,
               const Real* emis, const Real* sfc_src, const Real* inc_flux, Real* flux_up, Real* flux_dn,
               int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bb_up, Real* bb_dn,
-              int do_jac, const Real* sfc_src_jac, Real* flux_up_jac, void* stream)
+              int do_jac, const Real* sfc_src_jac, Real* flux_up_jac, void* stream,
+              const Real* bnd_lay = nullptr, const Real* bnd_lev = nullptr)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("lw_solver: empty extents");
+    const bool factored = bnd_lay != nullptr;
+    if (factored && (!bnd_lev || !band_lims || flux_up || flux_dn || do_jac))
+        return arg_error("lw_solver: factored sources need both band Planck arrays, band limits and summed outputs");
     if (nmus < 1 || nmus > 4) return arg_error("lw_solver: n_gauss_angles must be 1..4");
     if (ncol * (nlay + 1) * int64_t(sizeof(Real)) >= (int64_t(1) << 32))
         return arg_error("lw_solver: ncol * (nlay + 1) values of one g-point must stay below 4 GiB");
-    if (!secants || !weights || !tau || !lay || !linc || !ldec || !emis || !sfc_src)
+    if (!secants || !weights || !tau || !lay || (!factored && (!linc || !ldec)) || !emis || !sfc_src)
         return arg_error("lw_solver: NULL input");
     const bool spectral = flux_up || flux_dn;
     if (spectral && (!flux_up || !flux_dn)) return arg_error("lw_solver: flux_up and flux_dn go together");
@@ -764,12 +787,17 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
         auto launch = [&](auto kernel, const Real* jac_src, Real* jac_out) {
             hipLaunchKernelGGL(kernel, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, int(nmus), secants,
                                weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux, flux_up, flux_dn,
-                               jac_src, grp, g_up, g_dn, jac_out);
+                               jac_src, grp, g_up, g_dn, jac_out, bnd_lay, bnd_lev);
         };
         if (do_jac)
-            launch(nmus == 1 ? lw_solver_kernel<Real, true, 1> : lw_solver_kernel<Real, true, 4>, sfc_src_jac, g_jac);
+            launch(nmus == 1 ? lw_solver_kernel<Real, true, 1, false> : lw_solver_kernel<Real, true, 4, false>,
+                   sfc_src_jac, g_jac);
+        else if (factored)
+            launch(nmus == 1 ? lw_solver_kernel<Real, false, 1, true> : lw_solver_kernel<Real, false, 4, true>,
+                   nullptr, nullptr);
         else
-            launch(nmus == 1 ? lw_solver_kernel<Real, false, 1> : lw_solver_kernel<Real, false, 4>, nullptr, nullptr);
+            launch(nmus == 1 ? lw_solver_kernel<Real, false, 1, false> : lw_solver_kernel<Real, false, 4, false>,
+                   nullptr, nullptr);
         rc = check_hip(hipGetLastError(), "lw_solver_kernel");
     }
     if (rc == RRTMGP_OK && !spectral)
@@ -867,6 +895,18 @@ extern "C" int rrtmgp_hip_lw_solver_noscat_bnd_##S(int64_t ncol, int64_t nlay, i
                         lev_source_inc, lev_source_dec, sfc_emis, sfc_src, inc_flux, nullptr,        \
                         nullptr, nbnd, band_lims_gpt, bnd_flux_up, bnd_flux_dn, broadband_up,        \
                         broadband_dn, 0, nullptr, nullptr, stream);                                  \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_lw_solver_noscat_bnd_factored_##S(int64_t ncol, int64_t nlay,           \
+    int64_t ngpt, int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,     \
+    const T* planck_frac, const T* band_lay_src, const T* band_lev_src, const T* sfc_emis,          \
+    const T* sfc_src, const T* inc_flux, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,    \
+    T* bnd_flux_dn, T* broadband_up, T* broadband_dn, void* stream)                                 \
+{                                                                                                    \
+    if (!band_lay_src || !band_lev_src) return arg_error("lw_solver_factored: NULL band array");    \
+    return lw_solver<T>(ncol, nlay, ngpt, top_at_1, nmus, secants, weights, tau, planck_frac,       \
+                        nullptr, nullptr, sfc_emis, sfc_src, inc_flux, nullptr, nullptr, nbnd,       \
+                        band_lims_gpt, bnd_flux_up, bnd_flux_dn, broadband_up, broadband_dn, 0,      \
+                        nullptr, nullptr, stream, band_lay_src, band_lev_src);                       \
 }                                                                                                    \
 extern "C" int rrtmgp_hip_sw_solver_2stream_##S(int64_t ncol, int64_t nlay, int64_t ngpt,         \
     int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0, const T* sfc_alb_dir,       \

@@ -10,6 +10,19 @@ class Source_func_lw_gpu : public Optical_props_gpu
 {
     public:
         Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& optical_props);
+        // Factored form (no reference counterpart): lay_source holds the g-points' Planck fractions,
+        // band_lay_source (n_col, n_lay, n_band) and band_lev_source (n_col, n_lay + 1, n_band) the
+        // band Planck functions, and lev_source_inc/dec stay empty.  Gas_optics_rrtmgp_gpu::
+        // gas_optics_with_clouds and Rte_lw_gpu::rte_lw_fluxes take either form; the longwave
+        // driver uses this one when no surrogate is set.
+        Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& optical_props,
+                           bool factored);
+
+        bool is_factored() const { return factored; }
+        Array_gpu<Float,3>& get_band_lay_source() { return band_lay_source; }
+        Array_gpu<Float,3>& get_band_lev_source() { return band_lev_source; }
+        const Array_gpu<Float,3>& get_band_lay_source() const { return band_lay_source; }
+        const Array_gpu<Float,3>& get_band_lev_source() const { return band_lev_source; }
 
         Array_gpu<Float,2>& get_sfc_source()     { return sfc_source;     }
         Array_gpu<Float,2>& get_sfc_source_jac() { return sfc_source_jac; }
@@ -29,6 +42,9 @@ class Source_func_lw_gpu : public Optical_props_gpu
         Array_gpu<Float,3> lay_source;
         Array_gpu<Float,3> lev_source_inc;
         Array_gpu<Float,3> lev_source_dec;
+        bool factored = false;
+        Array_gpu<Float,3> band_lay_source;
+        Array_gpu<Float,3> band_lev_source;
 };
 
 #endif

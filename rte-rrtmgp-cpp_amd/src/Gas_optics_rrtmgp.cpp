@@ -548,6 +548,16 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
     const int sfc_lay = play({1, 1}) > play({1, nlay}) ? 1 : nlay;
     if (cloud && cloud->get_ngpt() != get_nband())
         throw std::runtime_error("Cannot add optical properties with incompatible band - gpoint combination");
+    if (sources.is_factored())
+    {
+        if (use_mlp) throw std::runtime_error("factored longwave sources need the plain gas optics (a surrogate is set)");
+        hip_check(K<Float>::gas_optics_lw_factored(
+                &dev, ncol, nlay, play.ptr(), tlay.ptr(), tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &v, sfc_lay,
+                cloud ? cloud->get_tau().ptr() : nullptr, props.get_tau().ptr(), sources.get_lay_source().ptr(),
+                sources.get_band_lay_source().ptr(), sources.get_band_lev_source().ptr(),
+                sources.get_sfc_source().ptr(), sources.get_sfc_source_jac().ptr(), stream()));
+        return;
+    }
     if (use_mlp)
     {
         const int need = 1 + (ncol + 63) / 64 * nlay;

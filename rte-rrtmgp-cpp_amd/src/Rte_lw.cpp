@@ -73,6 +73,7 @@ void Rte_lw_gpu::rte_lw(
     const int ncol = optical_props->get_ncol();
     const int nlay = optical_props->get_nlay();
     const int ngpt = optical_props->get_ngpt();
+    if (sources.is_factored()) throw std::runtime_error("rte_lw: g-point fluxes need unfactored sources");
     const LwPrep p = lw_prep(*this, optical_props, sfc_emis, n_gauss_angles);
 
     // CPU semantics: a (ncol, nlev, 1) output array selects broadband fluxes (src/Rte_lw.cpp:137).
@@ -106,6 +107,17 @@ void Rte_lw_gpu::rte_lw_fluxes(
     const Float* inc = (inc_flux.size() == 0) ? nullptr : inc_flux.ptr();
     const bool bnd = bnd_flux_up.size() > 0;
     const bool bb = flux_up.size() > 0;
+    if (sources.is_factored())
+    {
+        hip_check(K<Float>::lw_solver_noscat_bnd_factored(
+                ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
+                optical_props->get_tau().ptr(), sources.get_lay_source().ptr(), sources.get_band_lay_source().ptr(),
+                sources.get_band_lev_source().ptr(), p.sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
+                optical_props->get_band_lims_gpoint().ptr(), bnd ? bnd_flux_up.ptr() : nullptr,
+                bnd ? bnd_flux_dn.ptr() : nullptr, bb ? flux_up.ptr() : nullptr, bb ? flux_dn.ptr() : nullptr,
+                stream()));
+        return;
+    }
     hip_check(K<Float>::lw_solver_noscat_bnd(
             ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
             optical_props->get_tau().ptr(), sources.get_lay_source().ptr(),

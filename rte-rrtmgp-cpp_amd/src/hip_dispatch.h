@@ -11,6 +11,8 @@ template<typename T> struct K;
     static constexpr auto kdist_pack = rrtmgp_hip_kdist_pack_##S;  \
     static constexpr auto gas_optics_lw_bands = rrtmgp_hip_gas_optics_lw_bands_##S; \
     static constexpr auto gas_optics_lw_tiles = rrtmgp_hip_gas_optics_lw_tiles_##S; \
+    static constexpr auto gas_optics_lw_factored = rrtmgp_hip_gas_optics_lw_factored_##S; \
+    static constexpr auto lw_solver_noscat_bnd_factored = rrtmgp_hip_lw_solver_noscat_bnd_factored_##S; \
     static constexpr auto gas_optics_sw_tiles = rrtmgp_hip_gas_optics_sw_tiles_##S; \
     static constexpr auto gas_optics_sw_bands = rrtmgp_hip_gas_optics_sw_bands_##S; \
     static constexpr auto gas_optics_lw_adjoint = rrtmgp_hip_gas_optics_lw_adjoint_##S; \

@@ -358,6 +358,30 @@ def test_solvers_column_blocks(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, syn):
         assert rel_err(a[1][n], a[3][n], floor=1e-6) < FP64_RTOL, n
 
 
+@pytest.mark.parametrize("top_at_1", [False, True])
+@pytest.mark.parametrize("prec", [8, 4])
+def test_lw_driver_factored_sources(G, gkd, syn, top_at_1, prec):
+    """The longwave driver runs with factored Planck sources (fractions per g-point, band Planck
+    functions per layer and level); its fluxes against the unfactored pair (gas optics writing
+    the three per-g-point source arrays, then the g-point solver summed to broadband) on the same
+    inputs, at the level of summation-order rounding."""
+    atm = syn.make_atmosphere(47, 70, 50, top_at_1=top_at_1)
+    k = gkd[(True, prec)]
+    da = G.DeviceAtmosphere(atm, precision=prec, cloudy=False)
+    got = G.SolverLW(k).solve(da)
+    got = {n: _host(got[n]) for n in ("flux_up", "flux_dn")}
+    vmr = {g: np.broadcast_to(np.asarray(atm.vmr[g], np.float64), atm.p_lay.shape).copy(order="F") for g in atm.vmr}
+    gc = G.GasConcs(vmr, prec)
+    cd = G.get_col_dry(vmr["h2o"], atm.p_lev, prec)
+    go = G.gas_optics(k, atm, gc, cd)
+    up, dn = G.rte_lw(k, top_at_1, go["tau"], go["lay_src"], go["lev_src_inc"], go["lev_src_dec"], go["sfc_src"],
+                      atm.emis_sfc, do_broadband=True)
+    ref = {"flux_up": _host(up), "flux_dn": _host(dn)}
+    tol = 1e-12 if prec == 8 else 2e-6
+    for n in ref:
+        assert rel_err(got[n], ref[n], floor=1e-3) < tol, n
+
+
 def test_solvers_allsky_fp32_flux_error(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy):
     glw, gsw, rlw, rsw = _solve_both(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy, 4)
     err_lw = max(np.abs(glw[n] - rlw[n]).max() for n in ("flux_up", "flux_dn"))
