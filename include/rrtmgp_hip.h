@@ -384,6 +384,8 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
 /* Planck fraction times its band's Planck function at the layer or level.  The factored pair */
 /* keeps that product unevaluated:                                                              */
 /*   gas_optics_lw_factored   writes tau, planck_frac (ncol, nlay, ngpt), band_lay_src         */
+/*                            (tiles / ntiles as rrtmgp_hip_gas_optics_lw_tiles_*, or NULL / 0 */
+/*                            for every pair)                                                   */
 /*                            (ncol, nlay, nbnd) = B_band(T_lay), band_lev_src (ncol, nlev,    */
 /*                            nbnd) = B_band(T_lev), and sfc_src / sfc_src_jac as the plain    */
 /*                            entry;                                                            */
@@ -392,10 +394,12 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
 /*                            and lev_source_dec = planck_frac band_lev_src (level i), the same*/
 /*                            FP products, so fluxes equal the plain pair's bit for bit.       */
 /* Two arrays of (ncol, nlay, ngpt) are neither written nor read.  Used by the longwave driver */
-/* (Radiation_solver_longwave_gpu) when no surrogate is set.                                    */
+/* (Radiation_solver_longwave_gpu), also with a surrogate (rrtmgp_hip_mlp_gas_optics_lw_factored */
+/* and this entry's tile list for its fallback).                                                */
 #define RRTMGP_DECL_LW_FACTORED(S, T)                                                          \
     int rrtmgp_hip_gas_optics_lw_factored_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,       \
-        int64_t nlay, const T* play, const T* tlay, const T* tlev, const T* tsfc,              \
+        int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay,           \
+        const T* tlev, const T* tsfc,                                                            \
         const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau,        \
         T* tau, T* planck_frac, T* band_lay_src, T* band_lev_src, T* sfc_src, T* sfc_src_jac,  \
         void* stream);                                                                          \
@@ -556,6 +560,15 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* net, int64_t ncol, int64_
                                  double temp_min, double dplanck, int sfc_lay, const float* cloud_tau,
                                  float* tau, float* lay_src, float* lev_src_inc, float* lev_src_dec,
                                  float* sfc_src, float* sfc_src_jac, int* fallback, void* stream);
+/* The same with factored Planck sources (outputs as rrtmgp_hip_gas_optics_lw_factored_f32).  */
+int rrtmgp_hip_mlp_gas_optics_lw_factored(const rrtmgp_mlp_net* net, int64_t ncol, int64_t nlay,
+                                          int64_t ngpt, int64_t nbnd, const int* gpt2band, const float* play,
+                                          const float* tlay, const float* tlev, const float* tsfc,
+                                          const float* col_dry, const rrtmgp_gas_vmr* gases,
+                                          const float* totplnk, int64_t nplanck, double temp_min,
+                                          double dplanck, int sfc_lay, const float* cloud_tau, float* tau,
+                                          float* planck_frac, float* band_lay_src, float* band_lev_src,
+                                          float* sfc_src, float* sfc_src_jac, int* fallback, void* stream);
 int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* net, int64_t ncol, int64_t nlay, int64_t ngpt,
                                  int64_t nbnd, const int* gpt2band, const float* play, const float* tlay,
                                  const float* col_dry, const rrtmgp_gas_vmr* gases, const float* cloud_tau,

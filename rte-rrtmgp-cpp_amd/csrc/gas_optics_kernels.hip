@@ -1756,14 +1756,17 @@ extern "C" int rrtmgp_hip_gas_optics_lw_##S(const rrtmgp_kdist_dev* kd, int64_t 
                             sfc_src, sfc_src_jac, stream);                                         \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_gas_optics_lw_factored_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,     \
-    int64_t nlay, const T* play, const T* tlay, const T* tlev, const T* tsfc, const T* col_dry,     \
-    const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau, T* tau, T* planck_frac,          \
-    T* band_lay_src, T* band_lev_src, T* sfc_src, T* sfc_src_jac, void* stream)                    \
+    int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay, const T* tlev,    \
+    const T* tsfc, const T* col_dry, const rrtmgp_gas_vmr* gases, int sfc_lay, const T* cloud_tau,  \
+    T* tau, T* planck_frac, T* band_lay_src, T* band_lev_src, T* sfc_src, T* sfc_src_jac,          \
+    void* stream)                                                                                   \
 {                                                                                                  \
     if (!band_lay_src || !band_lev_src) return arg_error("gas_optics_lw_factored: NULL band array"); \
+    if (!tiles && ntiles != 0) return arg_error("gas_optics_lw_factored: tile count without a list"); \
     return gas_optics_lw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, tlev, tsfc, col_dry, \
                             gases, sfc_lay, cloud_tau, tau, planck_frac, nullptr, nullptr,         \
-                            sfc_src, sfc_src_jac, stream, nullptr, 0, band_lay_src, band_lev_src); \
+                            sfc_src, sfc_src_jac, stream, tiles, ntiles, band_lay_src,            \
+                            band_lev_src);                                                         \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_gas_optics_lw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,  \
     int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* tlev,    \

@@ -305,7 +305,8 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                            int sfc_lay, const float* __restrict__ ctau, const float* __restrict__ cssa,
                            const float* __restrict__ cg, float* __restrict__ out0, float* __restrict__ out1,
                            float* __restrict__ out2, float* __restrict__ out3, float* __restrict__ sfc_src,
-                           float* __restrict__ sfc_jac, int* __restrict__ fallback)
+                           float* __restrict__ sfc_jac, int* __restrict__ fallback,
+                           float* __restrict__ bnd_lay, float* __restrict__ bnd_lev)
 {
     constexpr int P = MLP_P;
     __shared__ __align__(16) __bf16 xs[P][MLP_INP + 8];
@@ -448,17 +449,28 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
         }
         if constexpr (LW)
         {
-            // interpolate1D over totplnk for the band's Planck values of this column.
+            // interpolate1D over totplnk for the band's Planck values of this column; with factored
+            // sources also the band Planck functions at the layer and its lower level (and the top
+            // level from the top layer).
+            const int64_t icol = col0 + lane, nlev = nlay + 1;
             for (int b = w; b < nbnd; b += MLP_WAVES)
             {
                 const float* tab = sPl + b * nplanck;
+                float bq[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                 {
                     const float v0 = (sT[q][lane] - temp_min) / dplanck;
                     const float fr = v0 - float(int(v0));
                     const int id = min(nplanck - 1, max(1, int(v0) + 1));
-                    sB[q][b][lane] = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+                    bq[q] = tab[id - 1] + fr * (tab[id] - tab[id - 1]);
+                    sB[q][b][lane] = bq[q];
+                }
+                if (bnd_lay && icol < ncol)
+                {
+                    bnd_lay[icol + ncol * ilay + ncl * b] = bq[0];
+                    bnd_lev[icol + ncol * ilay + ncol * nlev * b] = bq[2];
+                    if (ilay == nlay - 1) bnd_lev[icol + ncol * nlay + ncol * nlev * b] = bq[1];
                 }
             }
         }
@@ -531,9 +543,14 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                         const float tau = __expf(y0) * cd;
                         const float pf = y1;
                         store_at(out0 + uo, vofs, ctau ? tau + sC[0][b][pl] : tau);
-                        store_at(out1 + uo, vofs, pf * sB[0][b][pl]);
-                        store_at(out2 + uo, vofs, pf * sB[1][b][pl]);
-                        store_at(out3 + uo, vofs, pf * sB[2][b][pl]);
+                        if (bnd_lay)
+                            store_at(out1 + uo, vofs, pf);               // factored: the fraction only
+                        else
+                        {
+                            store_at(out1 + uo, vofs, pf * sB[0][b][pl]);
+                            store_at(out2 + uo, vofs, pf * sB[1][b][pl]);
+                            store_at(out3 + uo, vofs, pf * sB[2][b][pl]);
+                        }
                         if (at_sfc)
                         {
                             // sfc_jac: pf (B(T_sfc + 1) - B(T_sfc)), as the plain gas optics.
@@ -825,20 +842,23 @@ static int make_mlp_gases(const rrtmgp_gas_vmr* g, int64_t ncol, int64_t nlay, M
     return RRTMGP_OK;
 }
 
-int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd,
-                                 const int* gpt2band, const float* play, const float* tlay, const float* tlev,
-                                 const float* tsfc, const float* col_dry, const rrtmgp_gas_vmr* gases,
-                                 const float* totplnk, int64_t nplanck, double temp_min, double dplanck, int sfc_lay,
-                                 const float* cloud_tau, float* tau, float* lay_src, float* lev_src_inc,
-                                 float* lev_src_dec, float* sfc_src, float* sfc_src_jac, int* fallback, void* stream)
+namespace
+{
+int mlp_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd, const int* gpt2band,
+           const float* play, const float* tlay, const float* tlev, const float* tsfc, const float* col_dry,
+           const rrtmgp_gas_vmr* gases, const float* totplnk, int64_t nplanck, double temp_min, double dplanck,
+           int sfc_lay, const float* cloud_tau, float* tau, float* lay_src, float* lev_src_inc, float* lev_src_dec,
+           float* sfc_src, float* sfc_src_jac, int* fallback, float* bnd_lay, float* bnd_lev, void* stream)
 {
     MlpNet net;
     RRTMGP_TRY(make_mlp_net(n, net));
     MlpGases g;
     RRTMGP_TRY(make_mlp_gases(gases, ncol, nlay, g));
+    const bool factored = bnd_lay != nullptr;
     if (ncol <= 0 || nlay <= 0 || net.no != 2 * ngpt || nbnd < 1 || nbnd > 16 || nplanck < 2 ||
         nplanck * nbnd > MLP_MAXPLANCK || !gpt2band || !play ||
-        !tlay || !tlev || !tsfc || !col_dry || !totplnk || !tau || !lay_src || !lev_src_inc || !lev_src_dec ||
+        !tlay || !tlev || !tsfc || !col_dry || !totplnk || !tau || !lay_src ||
+        (factored ? !bnd_lev : (!lev_src_inc || !lev_src_dec)) ||
         !sfc_src || !sfc_src_jac || !fallback || (sfc_lay != 1 && sfc_lay != nlay))
         return arg_error("mlp_gas_optics_lw: bad arguments");
     const int64_t nunits = ((ncol + 63) / 64) * nlay;
@@ -850,9 +870,36 @@ int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
     hipLaunchKernelGGL(mlp_gas_optics_kernel<true>, dim3(mlp_grid(nunits)), dim3(64 * MLP_WAVES), 0, as_stream(stream), net,
                        ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, tlev, tsfc, col_dry, g, totplnk,
                        int(nplanck), float(temp_min), float(dplanck), sfc_lay, cloud_tau, nullptr, nullptr, tau,
-                       lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac, fallback);
+                       lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac, fallback, bnd_lay, bnd_lev);
     RRTMGP_LAUNCHED("mlp_gas_optics_lw_kernel");
     return RRTMGP_OK;
+}
+}  // namespace
+
+int rrtmgp_hip_mlp_gas_optics_lw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd,
+                                 const int* gpt2band, const float* play, const float* tlay, const float* tlev,
+                                 const float* tsfc, const float* col_dry, const rrtmgp_gas_vmr* gases,
+                                 const float* totplnk, int64_t nplanck, double temp_min, double dplanck, int sfc_lay,
+                                 const float* cloud_tau, float* tau, float* lay_src, float* lev_src_inc,
+                                 float* lev_src_dec, float* sfc_src, float* sfc_src_jac, int* fallback, void* stream)
+{
+    return mlp_lw(n, ncol, nlay, ngpt, nbnd, gpt2band, play, tlay, tlev, tsfc, col_dry, gases, totplnk, nplanck,
+                  temp_min, dplanck, sfc_lay, cloud_tau, tau, lay_src, lev_src_inc, lev_src_dec, sfc_src, sfc_src_jac,
+                  fallback, nullptr, nullptr, stream);
+}
+
+int rrtmgp_hip_mlp_gas_optics_lw_factored(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt,
+                                          int64_t nbnd, const int* gpt2band, const float* play, const float* tlay,
+                                          const float* tlev, const float* tsfc, const float* col_dry,
+                                          const rrtmgp_gas_vmr* gases, const float* totplnk, int64_t nplanck,
+                                          double temp_min, double dplanck, int sfc_lay, const float* cloud_tau,
+                                          float* tau, float* planck_frac, float* band_lay_src, float* band_lev_src,
+                                          float* sfc_src, float* sfc_src_jac, int* fallback, void* stream)
+{
+    if (!band_lay_src || !band_lev_src) return arg_error("mlp_gas_optics_lw_factored: NULL band array");
+    return mlp_lw(n, ncol, nlay, ngpt, nbnd, gpt2band, play, tlay, tlev, tsfc, col_dry, gases, totplnk, nplanck,
+                  temp_min, dplanck, sfc_lay, cloud_tau, tau, planck_frac, nullptr, nullptr, sfc_src, sfc_src_jac,
+                  fallback, band_lay_src, band_lev_src, stream);
 }
 
 int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t nlay, int64_t ngpt, int64_t nbnd,
@@ -876,7 +923,7 @@ int rrtmgp_hip_mlp_gas_optics_sw(const rrtmgp_mlp_net* n, int64_t ncol, int64_t 
     hipLaunchKernelGGL(mlp_gas_optics_kernel<false>, dim3(mlp_grid(nunits)), dim3(64 * MLP_WAVES), 0, as_stream(stream), net,
                        ncol, nlay, int(ngpt), int(nbnd), gpt2band, play, tlay, nullptr, nullptr, col_dry, gs,
                        nullptr, 2, 0.f, 1.f, 1, cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, nullptr, nullptr,
-                       nullptr, fallback);
+                       nullptr, fallback, nullptr, nullptr);
     RRTMGP_LAUNCHED("mlp_gas_optics_sw_kernel");
     return RRTMGP_OK;
 }

@@ -52,7 +52,7 @@ class Radiation_solver_longwave_gpu
         std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist;
         std::shared_ptr<Cloud_optics_gpu> cloud_optics;
         int n_col_block = 1 << 30;
-        int ws_ncol = -1, ws_nlay = -1, ws_bnd = -1, ws_fact = -1;
+        int ws_ncol = -1, ws_nlay = -1, ws_bnd = -1;
         std::unique_ptr<Optical_props_arry_gpu> optical_props;
         std::unique_ptr<Optical_props_1scl_gpu> cloud_props;
         std::unique_ptr<Source_func_lw_gpu> sources;

@@ -14,7 +14,7 @@ class Source_func_lw_gpu : public Optical_props_gpu
         // band_lay_source (n_col, n_lay, n_band) and band_lev_source (n_col, n_lay + 1, n_band) the
         // band Planck functions, and lev_source_inc/dec stay empty.  Gas_optics_rrtmgp_gpu::
         // gas_optics_with_clouds and Rte_lw_gpu::rte_lw_fluxes take either form; the longwave
-        // driver uses this one when no surrogate is set.
+        // driver uses this one.
         Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& optical_props,
                            bool factored);
 

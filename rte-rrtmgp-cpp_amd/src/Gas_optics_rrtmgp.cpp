@@ -21,11 +21,13 @@ namespace
 template<typename T> struct Mlp
 {
     template<typename... A> static int lw(A...) { throw std::invalid_argument("MLP surrogate: FP32 only"); }
+    template<typename... A> static int lw_factored(A...) { throw std::invalid_argument("MLP surrogate: FP32 only"); }
     template<typename... A> static int sw(A...) { throw std::invalid_argument("MLP surrogate: FP32 only"); }
 };
 template<> struct Mlp<float>
 {
     static constexpr auto lw = rrtmgp_hip_mlp_gas_optics_lw;
+    static constexpr auto lw_factored = rrtmgp_hip_mlp_gas_optics_lw_factored;
     static constexpr auto sw = rrtmgp_hip_mlp_gas_optics_sw;
 };
 
@@ -550,10 +552,28 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
         throw std::runtime_error("Cannot add optical properties with incompatible band - gpoint combination");
     if (sources.is_factored())
     {
-        if (use_mlp) throw std::runtime_error("factored longwave sources need the plain gas optics (a surrogate is set)");
+        const Float* ctau = cloud ? cloud->get_tau().ptr() : nullptr;
+        const int* tiles = nullptr;
+        int64_t ntiles = 0;
+        if (use_mlp)
+        {
+            const int need = 1 + (ncol + 63) / 64 * nlay;
+            if (fallback_gpu.size() < need) fallback_gpu = Array_gpu<int,1>({need});
+            const rrtmgp_gas_vmr mv = mlp_vmr_desc(gas_desc, ncol, nlay);
+            hip_check(Mlp<Float>::lw_factored(
+                    &mlp, ncol, nlay, get_ngpt(), get_nband(), gpt2band_gpu.ptr(), play.ptr(), tlay.ptr(),
+                    tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &mv, static_cast<const Float*>(dev.totplnk), dev.nplanck,
+                    dev.temp_ref_min, dev.totplnk_delta, sfc_lay, ctau, props.get_tau().ptr(),
+                    sources.get_lay_source().ptr(), sources.get_band_lay_source().ptr(),
+                    sources.get_band_lev_source().ptr(), sources.get_sfc_source().ptr(),
+                    sources.get_sfc_source_jac().ptr(), fallback_gpu.ptr(), stream()));
+            if (run_fallback_list(ncol, nlay) == 0) return;
+            tiles = fallback_gpu.ptr() + 1;
+            ntiles = n_fallback;
+        }
         hip_check(K<Float>::gas_optics_lw_factored(
-                &dev, ncol, nlay, play.ptr(), tlay.ptr(), tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &v, sfc_lay,
-                cloud ? cloud->get_tau().ptr() : nullptr, props.get_tau().ptr(), sources.get_lay_source().ptr(),
+                &dev, ncol, nlay, tiles, ntiles, play.ptr(), tlay.ptr(), tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &v,
+                sfc_lay, ctau, props.get_tau().ptr(), sources.get_lay_source().ptr(),
                 sources.get_band_lay_source().ptr(), sources.get_band_lev_source().ptr(),
                 sources.get_sfc_source().ptr(), sources.get_sfc_source_jac().ptr(), stream()));
         return;

@@ -67,14 +67,12 @@ void Radiation_solver_longwave_gpu::solve_gpu(
     // Orientation from the first column (src_test/Radiation_solver.cpp:407).
     const Bool top_at_1 = p_lay({1, 1}) < p_lay({1, nlay});
     const int nblk = std::max(1, std::min(n_col_block, ncol));
-    // Factored Planck sources (fractions per g-point, band Planck functions per layer and level)
-    // unless a surrogate writes the full ones: two (ncol, nlay, ngpt) arrays fewer through HBM.
-    const bool factored = !kdist->has_surrogate();
-
-    if (ws_ncol != nblk || ws_nlay != nlay || ws_bnd != int(switch_output_bnd_fluxes) || ws_fact != int(factored))
+    if (ws_ncol != nblk || ws_nlay != nlay || ws_bnd != int(switch_output_bnd_fluxes))
     {
         optical_props = std::make_unique<Optical_props_1scl_gpu>(nblk, nlay, *kdist);
-        sources = std::make_unique<Source_func_lw_gpu>(nblk, nlay, *kdist, factored);
+        // Factored Planck sources (fractions per g-point, band Planck functions per layer and
+        // level): two (ncol, nlay, ngpt) arrays fewer through HBM than the three source arrays.
+        sources = std::make_unique<Source_func_lw_gpu>(nblk, nlay, *kdist, true);
         cloud_props.reset();
         if (cloud_optics) cloud_props = std::make_unique<Optical_props_1scl_gpu>(nblk, nlay, *cloud_optics);
         // Block flux workspaces, used when a block does not span all columns.
@@ -84,7 +82,7 @@ void Radiation_solver_longwave_gpu::solve_gpu(
         const bool pb = partial && switch_output_bnd_fluxes;
         blk_bnd_up = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
         blk_bnd_dn = pb ? Array_gpu<Float,3>({nblk, nlev, nbnd}) : Array_gpu<Float,3>();
-        ws_ncol = nblk; ws_nlay = nlay; ws_bnd = int(switch_output_bnd_fluxes); ws_fact = int(factored);
+        ws_ncol = nblk; ws_nlay = nlay; ws_bnd = int(switch_output_bnd_fluxes);
     }
 
     BlockInputs ws;

@@ -335,6 +335,41 @@ def test_fallback_tiles(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw, lw):
             assert not np.array_equal(a[:, 10], b[:, 10])
 
 
+def test_factored_lw_driver_with_surrogate(S, syn, kd32_lw, ts_lw):
+    """The LW driver's factored sources with the surrogate, and its plain fallback on pairs outside
+    the envelope, against the unfactored surrogate path: gas optics writing the three per-g-point
+    source arrays through the facade, then the g-point solver summed to broadband."""
+    from rrtmgp_amd import api
+    kd = kd32_lw
+    net = S.MlpGasOptics(kd, seed=8)
+    net.fit_normalisation(ts_lw)
+    net.train(ts_lw, steps=50, batch=1024)
+    atm = syn.subset_columns(syn.make_atmosphere(501, 512, 60), np.arange(300))
+    ncol, nlay = atm.p_lay.shape
+    vmr = {g: np.broadcast_to(np.asarray(atm.vmr[g], np.float64), (ncol, nlay)).copy(order="F") for g in atm.vmr}
+    for c, l in [(5, 3), (130, 20)]:
+        vmr["o3"][c, l] *= 1e3                       # two pairs far outside the envelope
+    atm.vmr = vmr
+    S.set_surrogate(kd, net)
+    try:
+        da = api.DeviceAtmosphere(atm, precision=4, cloudy=False)
+        got = api.SolverLW(kd).solve(da)
+        got = {k: got[k].to_host().astype(np.float64) for k in ("flux_up", "flux_dn")}
+        nfb = S.last_fallback_units(kd)
+        gc = api.GasConcs(vmr, 4)
+        cd = api.get_col_dry(vmr["h2o"], atm.p_lev, 4)
+        go = api.gas_optics(kd, atm, gc, cd)
+        up, dn = api.rte_lw(kd, atm.top_at_1, go["tau"], go["lay_src"], go["lev_src_inc"], go["lev_src_dec"],
+                            go["sfc_src"], atm.emis_sfc, do_broadband=True)
+    finally:
+        S.set_surrogate(kd, None)
+    assert nfb == 2
+    ref = {"flux_up": up.to_host().astype(np.float64), "flux_dn": dn.to_host().astype(np.float64)}
+    for k in ref:
+        err = np.abs(got[k] - ref[k]).max() / np.abs(ref[k]).max()
+        assert err < 2e-6, (k, err)
+
+
 def test_training_and_flux_error(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw):
     """Training lowers the loss; the trained surrogate's fluxes stay close to the plain path's."""
     from rrtmgp_amd import api
