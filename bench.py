@@ -216,6 +216,112 @@ def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
     return units / dt, dt, err
 
 
+def precision_study(ncol, nlay, block, ocols, threads):
+    """The FP32 bulk against an all-FP64 GPU run on the same inputs, with 10% of the cloudy layers
+    forced to ssa = 1 - 1e-7, g = 0.85 before delta-scaling (near-conservative scattering): the
+    shortwave through gas optics, cloud optics, delta-scale, add_to and the broadband solver; the
+    longwave through the driver.  Max and RMS flux error (W m-2) and heating-rate error (K/day,
+    from each run's net fluxes differenced in FP64 on the host) over all columns; both GPU runs
+    against the FP64 oracle on the first ocols columns.  Columns go in blocks of `block`."""
+    import oracle as O
+    from rrtmgp_amd import api, device, synthetic as syn
+    kl, ks = syn.make_kdist(True, syn.SEED_KDIST_LW), syn.make_kdist(False, syn.SEED_KDIST_SW)
+    ll, ls = syn.make_cloud_lut(True, syn.SEED_CLOUD_LW), syn.make_cloud_lut(False, syn.SEED_CLOUD_SW)
+    t0 = time.perf_counter()
+    atm = syn.make_atmosphere(syn.SEED_ATMOS[2], ncol, nlay, cloudy=True, conservative_frac=0.1)
+    cons = atm.extra["conservative"]
+    log(f"precision study: {ncol} x {nlay}, {int(cons.sum())} near-conservative cloud layers "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    g_cp = 9.80665 / 1004.64 * 86400.0
+
+    def heating(up, dn, plev):
+        net = up - dn
+        return g_cp * (net[:, 1:] - net[:, :-1]) / (plev[:, 1:] - plev[:, :-1])
+
+    res = {}
+    for prec in (4, 8):
+        t0 = time.perf_counter()
+        kd = api.KDist(ks, AVAILABLE, prec)
+        cloud = api.Cloud(ls, prec)
+        scale = atm.tsi / kd.scalar("tsi")
+        sw = {k: np.empty((ncol, nlay + 1)) for k in ("up", "dn", "dir")}
+        for c0 in range(0, ncol, block):
+            sub = syn.subset_columns(atm, slice(c0, min(ncol, c0 + block)))
+            n = sub.ncol
+            vmr = {g: np.broadcast_to(np.asarray(sub.vmr[g], np.float64), (n, nlay)).copy(order="F")
+                   for g in sub.vmr}
+            gc = api.GasConcs(vmr, prec)
+            cd = api.get_col_dry(vmr["h2o"], sub.p_lev, prec)
+            go = api.gas_optics(kd, sub, gc, cd)
+            ct, cs, cg = cloud.optics(sub.lwp, sub.iwp, sub.rel, sub.rei, True)
+            m = sub.extra["conservative"][:, :, None]
+            dt = api.real_dtype(prec)
+            cs = device.DeviceArray.from_host(np.where(m, 1.0 - 1e-7, cs.to_host()), dt)
+            cg = device.DeviceArray.from_host(np.where(m, 0.85, cg.to_host()), dt)
+            api.delta_scale(ct, cs, cg, prec)
+            api.add_to(kd, go["tau"], ct, go["ssa"], go["g"], cs, cg)
+            toa = device.DeviceArray.from_host(go["toa_src"].to_host() * scale, dt)
+            up, dn, dr = api.rte_sw(kd, sub.top_at_1, go["tau"], go["ssa"], go["g"], sub.mu0, toa, sub.sfc_alb,
+                                    0.5 * sub.sfc_alb, do_broadband=True)
+            for k, v in (("up", up), ("dn", dn), ("dir", dr)):
+                sw[k][c0:c0 + n] = v.to_host()
+            del go, ct, cs, cg, up, dn, dr, toa
+        kdl = api.KDist(kl, AVAILABLE, prec)
+        slw = api.SolverLW(kdl, api.Cloud(ll, prec))
+        slw.set_block(block)
+        lw = slw.solve(api.DeviceAtmosphere(atm, prec))
+        lw = {"up": lw["flux_up"].to_host().astype(np.float64), "dn": lw["flux_dn"].to_host().astype(np.float64)}
+        device.synchronize()
+        res[prec] = {"sw": sw, "lw": lw}
+        log(f"FP{8 * prec} run in {time.perf_counter() - t0:.1f} s")
+        del kd, cloud, kdl, slw
+
+    def errs(a, b):
+        d = a - b
+        return {"max": float(np.abs(d).max()), "rms": float(np.sqrt(np.mean(d * d)))}
+
+    out = {}
+    plev = np.asarray(atm.p_lev, np.float64)
+    for rng, names in (("sw", ("up", "dn", "dir")), ("lw", ("up", "dn"))):
+        f4, f8 = res[4][rng], res[8][rng]
+        for k in names:
+            out[f"{rng}_flux_{k}"] = errs(f4[k], f8[k])
+        out[f"{rng}_flux_net"] = errs(f4["up"] - f4["dn"], f8["up"] - f8["dn"])
+        h4, h8 = heating(f4["up"], f4["dn"], plev), heating(f8["up"], f8["dn"], plev)
+        out[f"{rng}_heating_rate_kday"] = errs(h4, h8)
+        # Layers below 1 hPa: above it the layers are ~1 Pa thick, and the FP32 rounding of the
+        # fluxes alone (~1e-4 W m-2) moves their heating rates by K/day.
+        deep = (0.5 * (plev[:, 1:] + plev[:, :-1])) > 100.0
+        out[f"{rng}_heating_rate_kday_p_gt_1hPa"] = errs(h4[deep], h8[deep])
+    # Oracle on the first ocols columns (SW with the same forced layers).
+    sub = syn.subset_columns(atm, slice(0, ocols))
+    okd = O.Kdist(ks, AVAILABLE)
+    prev = O.set_threads(threads)
+    try:
+        ogo = O.gas_optics(okd, sub)
+        ct, cs, cg = O.Cloud(ls).optics(sub.lwp, sub.iwp, sub.rel, sub.rei, True)
+        m = sub.extra["conservative"][:, :, None]
+        ct, cs, cg = O.delta_scale(ct, np.where(m, 1.0 - 1e-7, cs), np.where(m, 0.85, cg))
+        tau, ssa, g = O.inc_bybnd(ogo["tau"], ct, okd.band_lims, ogo["ssa"], ogo["g"], cs, cg)
+        rup, rdn, rdr = O.rte_sw(tau, ssa, g, sub.mu0, ogo["toa_src"] * (atm.tsi / okd.scalar("tsi")), sub.sfc_alb,
+                                 0.5 * sub.sfc_alb, okd.band_lims, sub.top_at_1)
+        rlw = O.solve_lw(O.Kdist(kl, AVAILABLE), sub, O.Cloud(ll))
+    finally:
+        O.set_threads(prev)
+    oref = {"up": rup.sum(2), "dn": rdn.sum(2), "dir": rdr.sum(2)}
+    oracle = {}
+    for prec in (4, 8):
+        f = res[prec]
+        oracle[f"fp{8 * prec}"] = {
+            "sw_flux_max": max(float(np.abs(f["sw"][k][:ocols] - oref[k]).max()) for k in oref),
+            "lw_flux_max": max(float(np.abs(f["lw"][k][:ocols] - rlw["flux_" + k]).max()) for k in ("up", "dn"))}
+    return {"metric": "precision study: FP32 bulk vs all-FP64 GPU (W m-2, K/day)",
+            "config": {"workload": f"all-sky LW+SW, {ncol} columns x {nlay} layers, 10% of cloud layers at "
+                                   "ssa = 1 - 1e-7, g = 0.85", "ncol": ncol, "nlay": nlay,
+                       "near_conservative_layers": int(cons.sum()), "column_block": block},
+            "fp32_vs_fp64_gpu": out, "vs_fp64_oracle": {"columns": ocols, **oracle}}
+
+
 def copy_calibration():
     """One device copy of 1 GiB (copy_cols_kernel): read and written bytes are known exactly, so
     the FETCH_SIZE / WRITE_SIZE counters of this dispatch calibrate those of the others."""
@@ -236,7 +342,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--ncol", type=int, default=65536)
+    ap.add_argument("--ncol", type=int, default=None, help="columns (65536; 262144 for --precision-study)")
     ap.add_argument("--nlay", type=int, default=128)
     ap.add_argument("--precision", type=int, choices=(32, 64), default=32)
     ap.add_argument("--n-col-block", type=int, default=0, help="columns per solver block (0: all)")
@@ -245,6 +351,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-nn", action="store_true", help="skip the NN gas-optics (surrogate) measurement")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="instead of the throughput line: FP32 vs FP64 flux and heating-rate errors with "
+                         "near-conservative clouds (262144 columns unless --ncol is given)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="launch one untimed 1 GiB device copy first (reference dispatch for PMC counters)")
     args = ap.parse_args()
@@ -257,9 +366,16 @@ def main():
 
     from rrtmgp_amd import api, device, profiling, synthetic as syn
 
+    if args.precision_study:
+        threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        blk = args.n_col_block if args.n_col_block > 0 else 65536
+        res = precision_study(args.ncol or 262144, args.nlay, blk, min(args.cpu_cols, args.ncol or 262144), threads)
+        print(json.dumps(res), flush=True)
+        return
+
     prec = 4 if args.precision == 32 else 8
     E = prec
-    ncol, nlay = args.ncol, args.nlay
+    ncol, nlay = args.ncol or 65536, args.nlay
     log(f"workload: all-sky LW+SW, {ncol} columns x {nlay} layers, FP{args.precision}")
     kl, ks = syn.make_kdist(True, syn.SEED_KDIST_LW), syn.make_kdist(False, syn.SEED_KDIST_SW)
     ll, ls = syn.make_cloud_lut(True, syn.SEED_CLOUD_LW), syn.make_cloud_lut(False, syn.SEED_CLOUD_SW)
