@@ -398,3 +398,22 @@ def test_range_checks_raise(G, gkd, syn, O):
         G.gas_optics(k, atm, G.GasConcs(atm.vmr), O.get_col_dry(atm.vmr["h2o"], atm.p_lev))
     with pytest.raises(ValueError, match="out of range"):
         G.GasConcs({"h2o": np.full((4, 20), 1.5)})
+
+
+def test_precision_study_mode():
+    """bench.py --precision-study at a small size: the FP32 bulk against the all-FP64 GPU run with
+    near-conservative cloud layers (flux errors within 0.05 W m-2; both runs against the oracle)."""
+    import importlib.util
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(root, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    r = bench.precision_study(600, 60, 256, 64, 4)
+    assert r["config"]["near_conservative_layers"] > 0
+    for k, e in r["fp32_vs_fp64_gpu"].items():
+        if "flux" in k:
+            assert e["max"] < 0.05, (k, e)
+    assert r["vs_fp64_oracle"]["fp64"]["sw_flux_max"] < 1e-6
+    assert r["vs_fp64_oracle"]["fp64"]["lw_flux_max"] < 1e-6
+    assert r["vs_fp64_oracle"]["fp32"]["sw_flux_max"] < 0.05
