@@ -382,6 +382,63 @@ def test_lw_driver_factored_sources(G, gkd, syn, top_at_1, prec):
         assert rel_err(got[n], ref[n], floor=1e-3) < tol, n
 
 
+@pytest.mark.parametrize("top_at_1", [False, True])
+@pytest.mark.parametrize("nmu", [1, 3])
+@pytest.mark.parametrize("prec", [8, 4])
+def test_lw_solver_factored_entry(G, top_at_1, nmu, prec):
+    """rrtmgp_hip_lw_solver_noscat_bnd_factored_* against rrtmgp_hip_lw_solver_noscat_bnd_* on
+    sources formed on the host as Planck fraction x band Planck function (layer, level above for
+    lev_source_inc, level below for lev_source_dec), with ragged bands and 1 or 3 angles: band and
+    broadband fluxes equal (FP64) or within FP32 rounding of the same products."""
+    from rrtmgp_amd import _lib
+    rng = np.random.default_rng(17 + nmu)
+    ncol, nlay = 70, 23
+    bl = np.array([[1, 5], [6, 21], [22, 30]], np.int32).T.copy(order="F")      # 5, 16, 9 g-points
+    nbnd, ngpt = bl.shape[1], int(bl.max())
+    band_of = np.concatenate([np.full(bl[1, b] - bl[0, b] + 1, b) for b in range(nbnd)])
+    dt = np.float64 if prec == 8 else np.float32
+    tau = rng.uniform(0, 2, (ncol, nlay, ngpt)).astype(dt)
+    pf = rng.uniform(0, 1, (ncol, nlay, ngpt)).astype(dt)
+    blay = rng.uniform(10, 100, (ncol, nlay, nbnd)).astype(dt)
+    blev = rng.uniform(10, 100, (ncol, nlay + 1, nbnd)).astype(dt)
+    lay = pf * blay[:, :, band_of]
+    linc = pf * blev[:, 1:, band_of]
+    ldec = pf * blev[:, :-1, band_of]
+    emis = rng.uniform(0.8, 1, (ncol, ngpt)).astype(dt)
+    sfc = rng.uniform(10, 100, (ncol, ngpt)).astype(dt)
+    sec = np.empty((ncol, ngpt, nmu), dt)
+    for mu in range(nmu):
+        sec[:, :, mu] = 1.5 + mu
+    w = np.full(nmu, 1.0 / nmu, dt)
+    dev = lambda a: G.DeviceArray.from_host(np.asfortranarray(a), dt)         # noqa: E731
+    d = {k: dev(v) for k, v in dict(tau=tau, pf=pf, blay=blay, blev=blev, lay=lay, linc=linc, ldec=ldec,
+                                    emis=emis, sfc=sfc, sec=sec, w=w).items()}
+    out = {k: G.DeviceArray(sh, dt) for k, sh in (("bu", (ncol, nlay + 1, nbnd)), ("bd", (ncol, nlay + 1, nbnd)),
+                                                  ("u", (ncol, nlay + 1)), ("dn", (ncol, nlay + 1)))}
+    out2 = {k: G.DeviceArray(v.shape, dt) for k, v in out.items()}
+    S = "f64" if prec == 8 else "f32"
+    L = _lib.hip()
+    blp = bl.ctypes.data_as(C.POINTER(C.c_int))
+    i = C.c_int64
+    rc = getattr(L, "rrtmgp_hip_lw_solver_noscat_bnd_" + S)(
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["lay"].ptr, d["linc"].ptr, d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr, None, i(nbnd), blp,
+        out["bu"].ptr, out["bd"].ptr, out["u"].ptr, out["dn"].ptr, None)
+    assert rc == 0
+    rc = getattr(L, "rrtmgp_hip_lw_solver_noscat_bnd_factored_" + S)(
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["pf"].ptr, d["blay"].ptr, d["blev"].ptr, d["emis"].ptr, d["sfc"].ptr, None, i(nbnd), blp,
+        out2["bu"].ptr, out2["bd"].ptr, out2["u"].ptr, out2["dn"].ptr, None)
+    assert rc == 0
+    for k in out:
+        a, b = _host(out2[k]), _host(out[k])
+        if prec == 8:
+            np.testing.assert_array_equal(a, b, err_msg=k)
+        else:
+            # The host formed the FP32 products the kernel forms itself; summation order is equal.
+            assert rel_err(a, b, floor=1.0) < 1e-6, k
+
+
 def test_solvers_allsky_fp32_flux_error(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy):
     glw, gsw, rlw, rsw = _solve_both(G, O, gkd, okd_lw, okd_sw, lut_lw, lut_sw, atm_cloudy, 4)
     err_lw = max(np.abs(glw[n] - rlw[n]).max() for n in ("flux_up", "flux_dn"))
