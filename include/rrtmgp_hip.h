@@ -65,12 +65,27 @@ int rrtmgp_hip_event_elapsed_ms(void* start, void* stop, float* ms);
 
 /* Caching device allocator (the reference's counterparts: allocate_gpu/free_gpu with its own */
 /* pool, include/tools_gpu.h:71-98, Memory_pool_gpu include/mem_pool_gpu.h:12, src_cuda/mem_pool_gpu.cu). */
-/* A freed block is kept and handed out again only to a request on the same device and stream,*/
-/* so stream order covers reuse.  release() synchronizes and frees all unused blocks.         */
+/* A block remembers the stream it was allocated on.  Freed on that stream, it is handed out  */
+/* again only to a request on the same device and stream, so stream order covers reuse.  Freed */
+/* on another stream, it is fenced by an event on each of the two streams and a later request  */
+/* on any stream of the device waits on both before using it.  Streams given to the library    */
+/* must outlive their blocks or be destroyed with rrtmgp_hip_stream_destroy.  release()        */
+/* synchronizes and frees all unused blocks.                                                   */
 int rrtmgp_hip_cache_malloc(void** ptr, size_t bytes, void* stream);
 int rrtmgp_hip_cache_free(void* ptr, void* stream);
 int rrtmgp_hip_cache_release(void);
 int rrtmgp_hip_cache_stats(size_t* live_bytes, size_t* cached_bytes);
+
+/* Array_gpu element operations (reference include/Array.h): fill sets n elements of          */
+/* elem_size bytes (1, 2, 4 or 8) to *value (a host pointer), as Array_gpu::fill and fill_kernel*/
+/* (:296-335, :501-511).  subset copies dst(d_1..d_n) = src(s_1..s_n), s_k = d_k + starts_k - 1 */
+/* (1-based starts, relative to src_offsets when given) or s_k = 1 where src_dims_k == 1 (the  */
+/* spread of Array_gpu::subset, :563-607, subset_kernel :296-322); ndim <= 8.  Ranges outside  */
+/* the source are rejected with RRTMGP_ERR_ARG.                                                */
+int rrtmgp_hip_fill(void* dst, int64_t n, int elem_size, const void* value, void* stream);
+int rrtmgp_hip_subset(int elem_size, int ndim, const void* src, const int64_t* src_dims,
+                      const int64_t* src_offsets, const int64_t* starts, void* dst,
+                      const int64_t* dst_dims, void* stream);
 
 /* Per-kernel timing.  No counterpart in the reference (its timing is external profiling):   */
 /* with timing on, every kernel launch of this library is bracketed by two HIP events on its */

@@ -17,11 +17,16 @@
 // differ the CPU semantics are kept; the decisions are listed at each site.
 //
 // How each part is pinned (tests/test_oracle_*.py): closed forms and invariants for every
-// function (sum rules of the interpolation weights, node reproduction of the tables, linearity,
-// Planck sum rule, transparent / non-scattering / conservative limits of the solvers, a dense
+// function -- sum rules of the interpolation weights, node reproduction of the tables, linearity,
+// the Planck sum rule, transparent / non-scattering / conservative limits of the solvers, a dense
 // linear solve of the two-stream equations for the adding method, by-hand sums for the
-// reductions) plus golden fixtures of the whole path.  The compiled Fortran of the submodule is
-// the one thing nothing here can pin (parity against it is unpinned).
+// reductions -- and, for the Planck-fraction, k-minor and Rayleigh gathers, tables linear in the
+// temperature, eta and pressure nodes with per-g-point (per-contributor) coefficients and
+// distinct lower/upper slopes, whose interpolation has a closed form that a swapped corner, a
+// j0/j1 mix-up, a lost jpress+itropo shift or the wrong flavor or contributor breaks.  The
+// oracle's outputs on two 8-column cases are kept as fixtures (tests/golden/) so that a later
+// change shows.  The compiled Fortran of the submodule is the one thing nothing here can pin
+// (parity against it is unpinned).
 //
 // Build: g++ -O3 -march=x86-64-v3 -ffp-contract=off -fPIC -shared (oracle/build_oracle.py).
 

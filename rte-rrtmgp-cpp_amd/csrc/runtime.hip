@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "cache_policy.h"
 #include "common.h"
 
 namespace rrtmgp_hip
@@ -90,27 +91,23 @@ LaunchTimer::~LaunchTimer()
 
 namespace
 {
-// Caching allocator: blocks freed by their user go to a free list tagged with the device and
-// the stream they were freed on, and are handed out again only to requests on that same
-// device and stream (so stream order protects any reuse).  Nothing is returned to the driver
-// until rrtmgp_hip_cache_release or an out-of-memory retry.
-struct CachedBlock { void* ptr; size_t bytes; int device; hipStream_t stream; };
+// Caching allocator: the reuse rules are those of CachePolicy (cache_policy.h).  Nothing is
+// returned to the driver until rrtmgp_hip_cache_release or an out-of-memory retry.
 std::mutex g_cache_mutex;
-std::vector<CachedBlock> g_cache_free;
-std::vector<CachedBlock> g_cache_live;
+CachePolicy g_cache;
 
-size_t round_block(size_t bytes)
+void destroy_fences(const CacheBlock& b)
 {
-    const size_t q = bytes > (size_t(1) << 20) ? (size_t(2) << 20) : size_t(512);
-    return (bytes + q - 1) / q * q;
+    for (void* e : b.fence)
+        if (e) (void)hipEventDestroy(static_cast<hipEvent_t>(e));
 }
 
 int release_free_blocks()
 {
-    std::vector<CachedBlock> blocks;
+    std::vector<CacheBlock> blocks;
     {
         std::lock_guard<std::mutex> lock(g_cache_mutex);
-        blocks.swap(g_cache_free);
+        blocks = g_cache.drain_free();
     }
     if (blocks.empty()) return RRTMGP_OK;
     int cur = 0;
@@ -119,9 +116,25 @@ int release_free_blocks()
     {
         RRTMGP_HIP(hipSetDevice(b.device));
         RRTMGP_HIP(hipDeviceSynchronize());
+        destroy_fences(b);
         RRTMGP_HIP(hipFree(b.ptr));
     }
     RRTMGP_HIP(hipSetDevice(cur));
+    return RRTMGP_OK;
+}
+
+// Event recorded on a stream, for the fence of a block freed on another stream than its owner.
+int fence_event(hipStream_t s, void** ev)
+{
+    hipEvent_t e = nullptr;
+    RRTMGP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const hipError_t r = hipEventRecord(e, s);
+    if (r != hipSuccess)
+    {
+        (void)hipEventDestroy(e);
+        return check_hip(r, "hipEventRecord (cache fence)");
+    }
+    *ev = e;
     return RRTMGP_OK;
 }
 }  // namespace
@@ -130,24 +143,26 @@ int cache_alloc(void** p, size_t bytes, hipStream_t stream)
 {
     *p = nullptr;
     if (bytes == 0) return RRTMGP_OK;
-    bytes = round_block(bytes);
+    bytes = CachePolicy::round_block(bytes);
     int dev = 0;
     RRTMGP_HIP(hipGetDevice(&dev));
+    CacheBlock b;
+    bool reused = false;
     {
         std::lock_guard<std::mutex> lock(g_cache_mutex);
-        const size_t limit = std::max(2 * bytes, bytes + (size_t(4) << 20));
-        auto best = g_cache_free.end();
-        for (auto it = g_cache_free.begin(); it != g_cache_free.end(); ++it)
-            if (it->device == dev && it->stream == stream && it->bytes >= bytes && it->bytes <= limit &&
-                (best == g_cache_free.end() || it->bytes < best->bytes))
-                best = it;
-        if (best != g_cache_free.end())
-        {
-            *p = best->ptr;
-            g_cache_live.push_back(*best);
-            g_cache_free.erase(best);
-            return RRTMGP_OK;
-        }
+        reused = g_cache.take(dev, stream, bytes, b);
+    }
+    if (reused)
+    {
+        // A fenced block: this stream waits until the uses queued before its free have ended.
+        int rc = RRTMGP_OK;
+        for (void* e : b.fence)
+            if (e && rc == RRTMGP_OK) rc = check_hip(hipStreamWaitEvent(stream, static_cast<hipEvent_t>(e), 0),
+                                                     "hipStreamWaitEvent (cache fence)");
+        destroy_fences(b);
+        if (rc != RRTMGP_OK) return rc;
+        *p = b.ptr;
+        return RRTMGP_OK;
     }
     hipError_t e = hipMalloc(p, bytes);
     if (e == hipErrorOutOfMemory)
@@ -158,24 +173,40 @@ int cache_alloc(void** p, size_t bytes, hipStream_t stream)
     }
     RRTMGP_TRY(check_hip(e, "hipMalloc (cached scratch)"));
     std::lock_guard<std::mutex> lock(g_cache_mutex);
-    g_cache_live.push_back({*p, bytes, dev, stream});
+    g_cache.add_live(*p, bytes, dev, stream);
     return RRTMGP_OK;
 }
 
 int cache_free(void* p, hipStream_t stream)
 {
     if (!p) return RRTMGP_OK;
-    std::lock_guard<std::mutex> lock(g_cache_mutex);
-    for (auto it = g_cache_live.begin(); it != g_cache_live.end(); ++it)
-        if (it->ptr == p)
+    CacheBlock b;
+    bool cross = false;
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mutex);
+        if (!g_cache.find_live(p, stream, b, cross))
+            return arg_error("cache_free: pointer was not allocated by the cache");
+    }
+    void* f0 = nullptr;
+    void* f1 = nullptr;
+    if (cross)
+    {
+        // Freed on another stream than the one it was allocated on: fence both.
+        int cur = 0;
+        RRTMGP_HIP(hipGetDevice(&cur));
+        if (cur != b.device) RRTMGP_HIP(hipSetDevice(b.device));
+        int rc = fence_event(static_cast<hipStream_t>(b.owner), &f0);
+        if (rc == RRTMGP_OK) rc = fence_event(stream, &f1);
+        if (cur != b.device) (void)hipSetDevice(cur);
+        if (rc != RRTMGP_OK)
         {
-            CachedBlock b = *it;
-            b.stream = stream;
-            g_cache_live.erase(it);
-            g_cache_free.push_back(b);
-            return RRTMGP_OK;
+            if (f0) (void)hipEventDestroy(static_cast<hipEvent_t>(f0));
+            return rc;
         }
-    return arg_error("cache_free: pointer was not allocated by the cache");
+    }
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    g_cache.finish_release(p, f0, f1);
+    return RRTMGP_OK;
 }
 
 int retain_pool_memory()
@@ -212,11 +243,8 @@ int rrtmgp_hip_cache_release(void) { return release_free_blocks(); }
 int rrtmgp_hip_cache_stats(size_t* live_bytes, size_t* cached_bytes)
 {
     std::lock_guard<std::mutex> lock(g_cache_mutex);
-    size_t l = 0, c = 0;
-    for (const auto& b : g_cache_live) l += b.bytes;
-    for (const auto& b : g_cache_free) c += b.bytes;
-    if (live_bytes) *live_bytes = l;
-    if (cached_bytes) *cached_bytes = c;
+    if (live_bytes) *live_bytes = g_cache.live_bytes();
+    if (cached_bytes) *cached_bytes = g_cache.cached_bytes();
     return RRTMGP_OK;
 }
 
@@ -333,6 +361,13 @@ int rrtmgp_hip_stream_create(void** stream)
 
 int rrtmgp_hip_stream_destroy(void* stream)
 {
+    // Blocks owned by the stream are handed to the default stream first (after the stream's
+    // work has ended), so that no later fence is recorded on a destroyed stream.
+    RRTMGP_HIP(hipStreamSynchronize(as_stream(stream)));
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mutex);
+        g_cache.forget_stream(stream);
+    }
     RRTMGP_HIP(hipStreamDestroy(as_stream(stream)));
     return RRTMGP_OK;
 }

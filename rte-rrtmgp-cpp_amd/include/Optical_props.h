@@ -68,10 +68,10 @@ class Optical_props_1scl_gpu : public Optical_props_arry_gpu
 
         Array_gpu<Float,3>& get_tau() { return tau; }
         Array_gpu<Float,3>& get_ssa() { throw std::runtime_error("ssa is not available in this class"); }
-        Array_gpu<Float,3>& get_g  () { throw std::runtime_error("g is available in this class"); }
+        Array_gpu<Float,3>& get_g  () { throw std::runtime_error("g is not available in this class"); }
         const Array_gpu<Float,3>& get_tau() const { return tau; }
         const Array_gpu<Float,3>& get_ssa() const { throw std::runtime_error("ssa is not available in this class"); }
-        const Array_gpu<Float,3>& get_g  () const { throw std::runtime_error("g is available in this class"); }
+        const Array_gpu<Float,3>& get_g  () const { throw std::runtime_error("g is not available in this class"); }
 
         void delta_scale(const Array_gpu<Float,3>& forward_frac=Array_gpu<Float,3>()) {}
 

@@ -10,17 +10,20 @@ class Source_func_lw_gpu : public Optical_props_gpu
 {
     public:
         Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& optical_props);
-        // Factored form (no reference counterpart): lay_source holds the g-points' Planck fractions,
-        // band_lay_source (n_col, n_lay, n_band) and band_lev_source (n_col, n_lay + 1, n_band) the
-        // band Planck functions, and lev_source_inc/dec stay empty.  Gas_optics_rrtmgp_gpu::
-        // gas_optics_with_clouds and Rte_lw_gpu::rte_lw_fluxes take either form; the longwave
-        // driver uses this one.
+        // Factored form (no reference counterpart): every source is a g-point's Planck fraction
+        // times its band's Planck function, and the product is left unevaluated.  planck_frac
+        // (n_col, n_lay, n_gpt) holds the fractions, band_lay_source (n_col, n_lay, n_band) and
+        // band_lev_source (n_col, n_lay + 1, n_band) the band Planck functions; lay_source and
+        // lev_source_inc/dec stay empty.  Gas_optics_rrtmgp_gpu::gas_optics_with_clouds and
+        // Rte_lw_gpu::rte_lw_fluxes take either form; the longwave driver uses this one.
         Source_func_lw_gpu(const int n_col, const int n_lay, const Optical_props_gpu& optical_props,
                            bool factored);
 
         bool is_factored() const { return factored; }
+        Array_gpu<Float,3>& get_planck_frac() { return planck_frac; }
         Array_gpu<Float,3>& get_band_lay_source() { return band_lay_source; }
         Array_gpu<Float,3>& get_band_lev_source() { return band_lev_source; }
+        const Array_gpu<Float,3>& get_planck_frac() const { return planck_frac; }
         const Array_gpu<Float,3>& get_band_lay_source() const { return band_lay_source; }
         const Array_gpu<Float,3>& get_band_lev_source() const { return band_lev_source; }
 
@@ -43,6 +46,7 @@ class Source_func_lw_gpu : public Optical_props_gpu
         Array_gpu<Float,3> lev_source_inc;
         Array_gpu<Float,3> lev_source_dec;
         bool factored = false;
+        Array_gpu<Float,3> planck_frac;
         Array_gpu<Float,3> band_lay_source;
         Array_gpu<Float,3> band_lev_source;
 };

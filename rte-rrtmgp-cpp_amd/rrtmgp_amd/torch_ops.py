@@ -69,9 +69,18 @@ def _atmosphere(precision, ncol, nlay, arrays: dict, vmr: dict, clouds):
     return a
 
 
+def _refuse_surrogate(solver):
+    """The backward runs the adjoint of the plain gas optics; a forward through an MLP surrogate
+    would pair fluxes of one function with gradients of another."""
+    if getattr(solver.kd, "_surrogate", None) is not None:
+        raise ValueError("autograd through a k-distribution with an MLP surrogate set is not supported: the "
+                         "adjoint differentiates the plain gas optics; clear the surrogate first")
+
+
 class _Longwave(torch.autograd.Function):
     @staticmethod
     def forward(ctx, solver, other_vmr, clouds, t_lay, t_lev, t_sfc, p_lay, p_lev, h2o, o3, emis_sfc):
+        _refuse_surrogate(solver)
         ncol, nlay = t_lay.shape
         vmr = dict(other_vmr)
         vmr["h2o"], vmr["o3"] = h2o, o3
@@ -98,6 +107,7 @@ class _Shortwave(torch.autograd.Function):
     @staticmethod
     def forward(ctx, solver, other_vmr, clouds, delta_cloud, mu0, tsi_scaling, t_lay, p_lay, p_lev, h2o, o3,
                 sfc_alb):
+        _refuse_surrogate(solver)
         ncol, nlay = t_lay.shape
         vmr = dict(other_vmr)
         vmr["h2o"], vmr["o3"] = h2o, o3

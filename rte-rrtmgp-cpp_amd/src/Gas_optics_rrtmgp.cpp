@@ -564,7 +564,7 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
                     &mlp, ncol, nlay, get_ngpt(), get_nband(), gpt2band_gpu.ptr(), play.ptr(), tlay.ptr(),
                     tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &mv, static_cast<const Float*>(dev.totplnk), dev.nplanck,
                     dev.temp_ref_min, dev.totplnk_delta, sfc_lay, ctau, props.get_tau().ptr(),
-                    sources.get_lay_source().ptr(), sources.get_band_lay_source().ptr(),
+                    sources.get_planck_frac().ptr(), sources.get_band_lay_source().ptr(),
                     sources.get_band_lev_source().ptr(), sources.get_sfc_source().ptr(),
                     sources.get_sfc_source_jac().ptr(), fallback_gpu.ptr(), stream()));
             if (run_fallback_list(ncol, nlay) == 0) return;
@@ -573,7 +573,7 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
         }
         hip_check(K<Float>::gas_optics_lw_factored(
                 &dev, ncol, nlay, tiles, ntiles, play.ptr(), tlay.ptr(), tlev.ptr(), tsfc.ptr(), col_dry.ptr(), &v,
-                sfc_lay, ctau, props.get_tau().ptr(), sources.get_lay_source().ptr(),
+                sfc_lay, ctau, props.get_tau().ptr(), sources.get_planck_frac().ptr(),
                 sources.get_band_lay_source().ptr(), sources.get_band_lev_source().ptr(),
                 sources.get_sfc_source().ptr(), sources.get_sfc_source_jac().ptr(), stream()));
         return;

@@ -13,10 +13,11 @@ using rrtmgp_cpp::stream;
 
 namespace
 {
+// An output array: allocated (and zeroed) when empty, else checked for shape.
 template<int N>
 void ensure(Array_gpu<Float,N>& a, const std::array<int,N>& dims)
 {
-    if (a.size() == 0) { a = Array_gpu<Float,N>(dims); return; }
+    if (a.size() == 0) { a = Array_gpu<Float,N>(dims); a.fill(Float(0)); return; }
     for (int i = 0; i < N; ++i)
         if (a.get_dims()[i] != dims[i]) throw std::invalid_argument("adjoint output has the wrong shape");
 }
@@ -45,6 +46,16 @@ int max_band_gpts(const Gas_optics_rrtmgp_gpu& kd)
     return m;
 }
 
+// The adjoint differentiates the plain table gas optics.  With an MLP surrogate set, the forward
+// solve_gpu runs the network instead, so a gradient from here would belong to another function.
+void refuse_surrogate(const Gas_optics_rrtmgp_gpu& kdist)
+{
+    if (kdist.has_surrogate())
+        throw std::invalid_argument("adjoint: the k-distribution has an MLP gas-optics surrogate set, whose "
+                                    "forward the adjoint does not differentiate; clear it (set_surrogate(NULL)) "
+                                    "before computing gradients");
+}
+
 bool cloudy(const Cloud_optics_gpu* c, const Array_gpu<Float,2>& lwp)
 {
     return c != nullptr && lwp.size() > 0;
@@ -62,6 +73,7 @@ void lw_adjoint_gpu(Gas_optics_rrtmgp_gpu& kdist, Cloud_optics_gpu* cloud_optics
                     Lw_adjoint_out& out)
 {
     if (!kdist.source_is_internal()) throw std::invalid_argument("lw_adjoint_gpu needs a longwave k-distribution");
+    refuse_surrogate(kdist);
     const int ncol = p_lay.dim(1), nlay = p_lay.dim(2), nlev = nlay + 1;
     const int ngpt = kdist.get_ngpt(), nbnd = kdist.get_nband(), ngas = kdist.get_ngas();
     const int64_t ncl = int64_t(ncol) * nlay;
@@ -142,6 +154,7 @@ void sw_adjoint_gpu(Gas_optics_rrtmgp_gpu& kdist, Cloud_optics_gpu* cloud_optics
                     const Array_gpu<Float,2>& g_flux_dir, Sw_adjoint_out& out)
 {
     if (!kdist.source_is_external()) throw std::invalid_argument("sw_adjoint_gpu needs a shortwave k-distribution");
+    refuse_surrogate(kdist);
     const int ncol = p_lay.dim(1), nlay = p_lay.dim(2), nlev = nlay + 1;
     const int ngpt = kdist.get_ngpt(), nbnd = kdist.get_nband(), ngas = kdist.get_ngas();
     const int64_t ncl = int64_t(ncol) * nlay;

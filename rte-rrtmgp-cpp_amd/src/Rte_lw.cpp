@@ -111,7 +111,7 @@ void Rte_lw_gpu::rte_lw_fluxes(
     {
         hip_check(K<Float>::lw_solver_noscat_bnd_factored(
                 ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
-                optical_props->get_tau().ptr(), sources.get_lay_source().ptr(), sources.get_band_lay_source().ptr(),
+                optical_props->get_tau().ptr(), sources.get_planck_frac().ptr(), sources.get_band_lay_source().ptr(),
                 sources.get_band_lev_source().ptr(), p.sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
                 optical_props->get_band_lims_gpoint().ptr(), bnd ? bnd_flux_up.ptr() : nullptr,
                 bnd ? bnd_flux_dn.ptr() : nullptr, bb ? flux_up.ptr() : nullptr, bb ? flux_dn.ptr() : nullptr,

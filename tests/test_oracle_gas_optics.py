@@ -216,3 +216,161 @@ def test_range_checks(O, okd_lw, atm_c0, syn):
     cd = -np.ones((4, 20))
     with pytest.raises(ValueError, match="col_dry is out of range"):
         O.gas_optics(okd_lw, bad, col_dry=cd)
+
+
+# ---------------------------------------------------------------------------------------------
+# Closed forms of the three table gathers that sum rules leave open: the Planck fraction
+# (src_kernels_cuda/gas_optics_rrtmgp_kernels.cu:181-190), k-minor (:451-455) and Rayleigh
+# (:572-616).  Each table is c(index) * (1 + a T_ref(t)/300 + b eta(e) + d p(node)), with c
+# varying over g-points or contributors and the slopes differing between the lower and upper
+# tables.  Multilinear interpolation reproduces a function linear in each node coordinate, so the
+# result is c * (1 + a T/300 + b eta_T + d locp) with eta_T = (1-ftemp) eta_0 + ftemp eta_1
+# (eta_i at temperature node jtemp+i).  A swapped corner, a j0/j1 mix-up, a lost jpress+itropo
+# shift, the wrong flavor or contributor, or a transposed table fails these.
+
+def _interp_np(kd, okd, atm, col_dry):
+    """Independent numpy restatement of the interpolation state (interpolation_kernel,
+    gas_optics_rrtmgp_kernels.cu:245-277) per (col, lay): jtemp, ftemp, itropo, locpress and
+    eta_0/eta_1 per flavor."""
+    tref = kd.temp_ref
+    ntemp, dT = tref.size, tref[1] - tref[0]
+    T = atm.t_lay
+    jtemp = np.clip(((T - (tref[0] - dT)) / dT).astype(int), 1, ntemp - 1)
+    ftemp = (T - tref[jtemp - 1]) / dT
+    lnp = np.log(atm.p_lay)
+    lpr = np.log(kd.press_ref)
+    dlp = (lpr[-1] - lpr[0]) / (lpr.size - 1)
+    locp = 1.0 + (lnp - lpr[0]) / dlp
+    itropo = (~(lnp > np.log(kd.press_ref_trop))).astype(int)
+    names = okd.gas_names
+    col = [col_dry] + [np.broadcast_to(atm.vmr[n], col_dry.shape) * col_dry for n in names]
+    vref = okd.get_real("vmr_ref").reshape(ntemp, len(names) + 1, 2)     # oracle copy, (t, gas, tropo)
+    flav = okd.get_int("flavor").reshape(-1, 2)
+    eta = np.zeros((flav.shape[0], 2) + T.shape)
+    for f, (g1, g2) in enumerate(flav):
+        for i in (0, 1):
+            r = vref[jtemp - 1 + i, g1, itropo] / vref[jtemp - 1 + i, g2, itropo]
+            cm = col[g1] + r * col[g2]
+            eta[f, i] = np.where(cm > 2 * np.finfo(float).tiny, col[g1] / cm, 0.5)
+    return dict(jtemp=jtemp, ftemp=ftemp, itropo=itropo, locp=locp, eta=eta, T=T)
+
+
+def _eta_nodes(kd):
+    neta = kd.kmajor.shape[1]
+    return np.arange(neta) / (neta - 1.0)
+
+
+def test_interp_np_matches_oracle_indices(O, kd_lw, okd_lw, atm_c0, col_dry):
+    # The numpy restatement agrees with the oracle's materialised indices, so the closed forms
+    # below rest on the same cells.
+    s = _interp_np(kd_lw, okd_lw, atm_c0, col_dry)
+    o = O.gas_optics(okd_lw, atm_c0, col_dry=col_dry, interp=True)
+    assert (o["jtemp"] == s["jtemp"]).all()
+    assert (o["tropo"].astype(int) == 1 - s["itropo"]).all()
+    neta = kd_lw.kmajor.shape[1]
+    jeta = np.minimum((s["eta"] * (neta - 1)).astype(int) + 1, neta - 1)
+    # oracle jeta is (itemp, col, lay, flavor); the restatement's is (flavor, itemp, col, lay)
+    assert (o["jeta"] == jeta.transpose(1, 2, 3, 0)).all()
+
+
+def test_planck_fraction_closed_form(O, kd_lw, okd_lw, atm_c0, col_dry):
+    ngpt, neta, npp, ntemp = kd_lw.planck_frac.shape
+    c = 1.0 + 0.01 * np.arange(ngpt)
+    a, b, d = 0.7, 0.9, 0.3
+    tab = c[:, None, None, None] * (1.0 + a * kd_lw.temp_ref[None, None, None, :] / 300.0
+                                    + b * _eta_nodes(kd_lw)[None, :, None, None]
+                                    + d * (np.arange(npp) / npp)[None, None, :, None])
+    kd = dataclasses.replace(kd_lw, planck_frac=np.asfortranarray(tab),
+                             totplnk=np.asfortranarray(np.ones_like(kd_lw.totplnk)))
+    okd = O.Kdist(kd, AVAILABLE)
+    o = O.gas_optics(okd, atm_c0, col_dry=col_dry)
+    s = _interp_np(kd, okd, atm_c0, col_dry)
+    gf = okd.get_int("gpoint_flavor").reshape(-1, 2)
+    expect = np.zeros_like(o["lay_src"])
+    for g in range(ngpt):
+        f = gf[g, s["itropo"]] - 1                       # flavor of g in each cell's half
+        e0 = np.take_along_axis(s["eta"][:, 0], f[None], 0)[0]
+        e1 = np.take_along_axis(s["eta"][:, 1], f[None], 0)[0]
+        eta_t = (1 - s["ftemp"]) * e0 + s["ftemp"] * e1
+        # pressure node (0-based) of the lower corner is jpress - 1 + itropo
+        pn = s["locp"] - 1 + s["itropo"]
+        expect[:, :, g] = c[g] * (1 + a * s["T"] / 300.0 + b * eta_t + d * pn / npp)
+    assert rel_err(o["lay_src"], expect) < 1e-13
+    assert rel_err(o["lev_src_inc"], expect) < 1e-13 and rel_err(o["lev_src_dec"], expect) < 1e-13
+    # surface source: the fraction of the surface layer (layer 1 when p decreases upward)
+    sfc = 0 if atm_c0.p_lay[0, 0] > atm_c0.p_lay[0, -1] else -1
+    assert rel_err(o["sfc_src"], expect[:, sfc, :]) < 1e-13
+
+
+def _minor_scaling(kd, atm, col_dry, side, i):
+    vmr = {k: np.broadcast_to(v, col_dry.shape) for k, v in atm.vmr.items()}
+    gas_of = dict(zip(kd.identifier_minor, kd.gas_minor))
+    s = vmr[gas_of[getattr(kd, f"minor_gases_{side}")[i]]] * col_dry
+    if getattr(kd, f"minor_scales_with_density_{side}")[i]:
+        s = s * atm.p_lay / 100.0 / atm.t_lay
+        sg = getattr(kd, f"scaling_gas_{side}")[i]
+        if sg:
+            frac = vmr[sg] / (1.0 + vmr["h2o"])
+            s = s * ((1.0 - frac) if getattr(kd, f"scale_by_complement_{side}")[i] else frac)
+    return s
+
+
+def test_kminor_closed_form(O, kd_lw, okd_lw, atm_c0, col_dry):
+    slopes = {"lower": (0.8, 1.3), "upper": (-0.4, 2.1)}
+    ch = {"kmajor": np.asfortranarray(np.zeros_like(kd_lw.kmajor))}
+    for side, (a, b) in slopes.items():
+        km = getattr(kd_lw, f"kminor_{side}")                # (ncontrib, neta, ntemp)
+        cc = 1.0 + 0.02 * np.arange(km.shape[0])
+        ch[f"kminor_{side}"] = np.asfortranarray(
+            cc[:, None, None] * (1.0 + a * kd_lw.temp_ref[None, None, :] / 300.0
+                                 + b * _eta_nodes(kd_lw)[None, :, None]))
+    kd = dataclasses.replace(kd_lw, **ch)
+    okd = O.Kdist(kd, AVAILABLE)
+    tau = O.gas_optics(okd, atm_c0, col_dry=col_dry)["tau"]
+    s = _interp_np(kd, okd, atm_c0, col_dry)
+    gf = okd.get_int("gpoint_flavor").reshape(-1, 2)
+    expect = np.zeros_like(tau)
+    for side, itr in (("lower", 0), ("upper", 1)):
+        a, b = slopes[side]
+        mask = s["itropo"] == itr
+        lim = getattr(kd, f"minor_limits_gpt_{side}")
+        start = getattr(kd, f"kminor_start_{side}")
+        cc = 1.0 + 0.02 * np.arange(getattr(kd, f"kminor_{side}").shape[0])
+        for i in range(lim.shape[1]):
+            g0, g1 = lim[:, i]
+            f = gf[g0 - 1, itr] - 1                          # flavor of the interval's first g-point
+            eta_t = (1 - s["ftemp"]) * s["eta"][f, 0] + s["ftemp"] * s["eta"][f, 1]
+            base = (1 + a * s["T"] / 300.0 + b * eta_t) * _minor_scaling(kd, atm_c0, col_dry, side, i)
+            for g in range(g0 - 1, g1):
+                k = start[i] - 1 + (g - (g0 - 1))             # contributor of g-point g
+                expect[:, :, g] += np.where(mask, cc[k] * base, 0.0)
+    assert rel_err(tau, expect, floor=1e-300) < 1e-13
+
+
+def test_rayleigh_closed_form_lower_upper(O, kd_sw, okd_sw, atm_c0, col_dry):
+    ngpt = kd_sw.ngpt
+    par = {"lower": (0.6, 1.7, 1e-27), "upper": (-0.3, 0.5, 3e-27)}
+    ch = {"kmajor": np.asfortranarray(np.zeros_like(kd_sw.kmajor)),
+          "kminor_lower": np.asfortranarray(np.zeros_like(kd_sw.kminor_lower)),
+          "kminor_upper": np.asfortranarray(np.zeros_like(kd_sw.kminor_upper))}
+    cg = 1.0 + 0.03 * np.arange(ngpt)
+    for side, (a, b, k0) in par.items():
+        ch[f"rayl_{side}"] = np.asfortranarray(
+            k0 * cg[:, None, None] * (1.0 + a * kd_sw.temp_ref[None, None, :] / 300.0
+                                      + b * _eta_nodes(kd_sw)[None, :, None]))
+    kd = dataclasses.replace(kd_sw, **ch)
+    okd = O.Kdist(kd, AVAILABLE)
+    o = O.gas_optics(okd, atm_c0, col_dry=col_dry)
+    s = _interp_np(kd, okd, atm_c0, col_dry)
+    gf = okd.get_int("gpoint_flavor").reshape(-1, 2)
+    col = atm_c0.vmr["h2o"] * col_dry + col_dry
+    expect = np.zeros_like(o["tau"])
+    for g in range(ngpt):
+        for side, itr in (("lower", 0), ("upper", 1)):
+            a, b, k0 = par[side]
+            f = gf[g, itr] - 1
+            eta_t = (1 - s["ftemp"]) * s["eta"][f, 0] + s["ftemp"] * s["eta"][f, 1]
+            val = k0 * cg[g] * (1 + a * s["T"] / 300.0 + b * eta_t) * col
+            expect[:, :, g] = np.where(s["itropo"] == itr, val, expect[:, :, g])
+    assert rel_err(o["tau"], expect) < 1e-13
+    assert np.all(o["ssa"] == 1.0)
