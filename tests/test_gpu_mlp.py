@@ -503,3 +503,26 @@ def test_inference_cloudy_top_at_1(S, syn, L, D, kd32_lw, kd32_sw, ts_lw, ts_sw,
         d = np.abs(got[1] - ref_ssa)
         assert d.max() < 2e-2 and np.count_nonzero(d > 1e-4 + 5e-3 * ref_ssa) <= d.size // 10000
         np.testing.assert_allclose(got[2], ref_g, rtol=2e-3, atol=2e-4)
+
+
+def test_adjoint_refuses_surrogate(S, syn, kd32_lw, kd32_sw, ts_lw):
+    """With a surrogate set, solve_gpu runs the network, while the adjoint differentiates the plain gas
+    optics: rrtmgp_api_adjoint_lw/sw and the autograd functions must refuse instead of returning
+    gradients of another function.  With the surrogate cleared, the adjoint runs again."""
+    from rrtmgp_amd import api
+    net = S.MlpGasOptics(kd32_lw, seed=12)
+    net.fit_normalisation(ts_lw)
+    atm = syn.make_atmosphere(502, 8, 30)
+    da = api.DeviceAtmosphere(atm, precision=4, cloudy=False)
+    solver = api.SolverLW(kd32_lw)
+    out = solver.solve(da)
+    g_up = out["flux_up"]
+    S.set_surrogate(kd32_lw, net)
+    try:
+        print("adjoint with a surrogate set: must raise", flush=True)
+        with pytest.raises(TypeError, match="surrogate"):
+            solver.adjoint(da, g_up, out["flux_dn"])
+    finally:
+        S.set_surrogate(kd32_lw, None)
+    g = solver.adjoint(da, g_up, out["flux_dn"])
+    assert np.isfinite(g["t_lay"].to_host()).all()
