@@ -16,13 +16,22 @@ Also reported, on the same JSON line:
                 against the 8 TB/s HBM peak; traffic from rocprofv3 FETCH_SIZE/WRITE_SIZE when a
                 summary of them for this workload is committed under profiles/.
   cpu_baseline  the FP64 CPU oracle (a restatement of the reference CPU path) on the first
-                columns of the same workload, timed on this host.
+                columns of the same workload, timed on this host: serial (one core, as the
+                reference runs) and with all cores (the build's OpenMP over column blocks), each
+                the median of three runs.
   parity        max |flux - FP64 oracle| over those columns (W m-2).
   nn            the NN gas-optics path: LW and SW MLP surrogates trained on the GPU on the plain
                 gas optics, then the same step timed with them, its speed-up over the plain path
                 and its flux RMSE / max error against the plain path (--no-nn skips it).
-  adjoint       adjoint (reverse-mode) time over forward time at 16384 columns (--no-adjoint).
-  kernels       per-kernel launches, time and algorithmic GB/s of the timed steps.
+  large         the same plain (and NN) step at 262144 columns x 128 layers, the size the
+                north-star roofline target is stated at (--no-large skips it).
+  adjoint       adjoint (reverse-mode) time over forward time at 16384 columns, and the peak
+                device memory of the adjoint alone (--no-adjoint).
+  kernels       per-kernel launches and time, from a separate pass of the same number of steps
+                with per-launch HIP events (the timed steps run without them); class-API GB/s
+                (the bytes the reference's class API would move, which a fused kernel may
+                exceed the peak with) and, where profiles/pmc_traffic.json has the kernel, the
+                measured-HBM GB/s.
 """
 from __future__ import annotations
 
@@ -89,8 +98,10 @@ def algorithmic_bytes(ncol, nlay, E, kd_lw, kd_sw, n2d_gases, cloudy):
 
 def adjoint_measure(kl, ks, ll, ls, prec, ncol=16384, nlay=128, reps=3):
     """Forward and adjoint (LW + SW, all-sky) of the 16384-column workload: mean times, their ratio
-    and the device memory pool after the adjoint (the library's caching allocator grows only, so
-    its size is the peak of the run)."""
+    and the adjoint's own peak device memory.  For the latter the library's cache is released after
+    the forward runs, so what the cache holds after one adjoint (it only grows between releases)
+    less what stayed live (the forward's workspaces) is the adjoint's scratch peak; the inputs and
+    cotangents (allocated outside the cache) are added."""
     import ctypes as C
     from rrtmgp_amd import _lib, api, device
     from rrtmgp_amd import synthetic as syn
@@ -111,7 +122,22 @@ def adjoint_measure(kl, ks, ll, ls, prec, ncol=16384, nlay=128, reps=3):
         slw.adjoint(da, cot[0], cot[1])
         ssw.adjoint(da, tsi, cot[2], cot[3], cot[4])
 
+    def stats():
+        live, cached = C.c_size_t(0), C.c_size_t(0)
+        _lib.check_hip(_lib.hip().rrtmgp_hip_cache_stats(C.byref(live), C.byref(cached)))
+        return int(live.value), int(cached.value)
+
     out = {}
+    fwd()
+    device.synchronize()
+    _lib.check_hip(_lib.hip().rrtmgp_hip_cache_release())
+    live0, _ = stats()
+    adj()
+    device.synchronize()
+    live1, cached1 = stats()
+    peak_scratch = live1 + cached1 - live0
+    inputs = sum(a.nbytes for a in vars(da).values() if isinstance(a, device.DeviceArray))
+    inputs += tsi.nbytes + sum(c.nbytes for c in cot)
     for name, fn in (("forward", fwd), ("adjoint", adj)):
         fn()
         device.synchronize()
@@ -120,25 +146,21 @@ def adjoint_measure(kl, ks, ll, ls, prec, ncol=16384, nlay=128, reps=3):
             fn()
         device.synchronize()
         out[name] = (time.perf_counter() - t0) * 1e3 / reps
-    live, cached = C.c_size_t(0), C.c_size_t(0)
-    _lib.check_hip(_lib.hip().rrtmgp_hip_cache_stats(C.byref(live), C.byref(cached)))
     return {"workload": f"all-sky LW+SW, {ncol} columns x {nlay} layers, FP{8 * prec}",
             "forward_ms": round(out["forward"], 3), "adjoint_ms": round(out["adjoint"], 3),
             "adjoint_over_forward": round(out["adjoint"] / out["forward"], 3),
-            "device_pool_bytes": int(live.value + cached.value),
+            "adjoint_peak_scratch_bytes": int(peak_scratch),
+            "adjoint_peak_bytes_with_inputs": int(peak_scratch + inputs),
             "gradients": "T_lay, T_lev, T_sfc, ln p_lay, p_lev, vmr of every gas, emissivity, albedos"}
 
 
-def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain_kernels, train_steps=60000,
-               n_train_atm=32):
-    """The NN gas-optics path: one surrogate per spectral range trained on the GPU on the plain gas
-    optics of n_train_atm x 1024 x 128 perturbed clear-sky pairs (about 4M samples), set on the
-    k-distributions, then the same all-sky step timed and its fluxes compared with the plain path's.
-    Training and the comparison are outside the timed steps."""
-    from rrtmgp_amd import device, profiling, surrogate as S, synthetic as syn
+def train_surrogates(kd_lw, kd_sw, train_steps=60000, n_train_atm=32):
+    """One surrogate per spectral range trained on the GPU on the plain gas optics of
+    n_train_atm x 1024 x 128 perturbed clear-sky pairs (about 4M samples)."""
+    from rrtmgp_amd import device, surrogate as S, synthetic as syn
     t0 = time.perf_counter()
     atms = [syn.make_atmosphere(500 + s, 1024, 128) for s in range(n_train_atm)]
-    nets, losses = {}, {}
+    nets, losses, rows = {}, {}, 0
     for lw, kd in ((True, kd_lw), (False, kd_sw)):
         ts = S.build_training_set(kd, atms, seed=3 if lw else 4)
         net = S.MlpGasOptics(kd, seed=9)
@@ -152,17 +174,29 @@ def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain
     device.synchronize()
     train_s = time.perf_counter() - t0
     log(f"NN: trained LW and SW surrogates in {train_s:.1f} s; normalised losses {losses}")
+    return nets, {"samples_per_range": rows, "steps": train_steps, "batch": 4096, "optimizer": "Adam lr 1e-3",
+                  "seconds_lw_sw": round(train_s, 1), "holdout_loss": losses}
+
+
+def nn_measure(kd_lw, kd_sw, nets, step, out_lw, out_sw, glw, gsw, steps, units, plain_kernels):
+    """The same all-sky step timed with the surrogates set (no per-launch events in the timed
+    steps; a second pass records the gas-optics kernel times), and its fluxes compared with the
+    plain path's.  Training and the comparison are outside the timed steps."""
+    from rrtmgp_amd import device, profiling, surrogate as S
     S.set_surrogate(kd_lw, nets[True])
     S.set_surrogate(kd_sw, nets[False])
     try:
         step()
         device.synchronize()
-        profiling.enable(True)
         t0 = time.perf_counter()
         for _ in range(steps):
             step()
         device.synchronize()
         wall = time.perf_counter() - t0
+        profiling.enable(True)
+        for _ in range(steps):
+            step()
+        device.synchronize()
         profiling.enable(False)
         kt = {k.name: k.avg_ms for k in profiling.collect()}
         fb = {"lw": S.last_fallback_units(kd_lw), "sw": S.last_fallback_units(kd_sw)}
@@ -184,15 +218,14 @@ def nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, steps, units, plain
                               "plain_sw": round(plain_kernels.get("gas_optics_sw_kernel", 0.0), 3),
                               "nn_sw": round(kt.get("mlp_gas_optics_sw_kernel", 0.0), 3)},
             "flux_rmse_wm2": rmse, "flux_err_max_wm2": emax, "fallback_units_last_call": fb,
-            "training": {"samples_per_range": rows, "steps": train_steps, "batch": 4096, "optimizer": "Adam lr 1e-3",
-                         "seconds_lw_sw": round(train_s, 1), "holdout_loss": losses},
             "network": "8 inputs (T, ln p, 5 ln vmr, lower-atmosphere flag) -> 128 -> 128 -> 2 ngpt, softplus, "
                        "BF16 weights/activations, FP32 accumulation"}
 
 
-def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
-    """FP64 oracle on the first ncols columns with `threads` OpenMP threads; returns the timing
-    record and the max flux error of the GPU fluxes over those columns."""
+def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads, runs=3):
+    """FP64 oracle on the first ncols columns with `threads` OpenMP threads, `runs` times; returns
+    the median rate (col-gpt-layers/s), the median seconds and the max flux error of the GPU
+    fluxes over those columns."""
     import oracle as O
     from rrtmgp_amd import synthetic as syn
     sub = syn.subset_columns(atm, slice(0, ncols))
@@ -200,13 +233,16 @@ def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads):
     ocl, ocs = O.Cloud(ll), O.Cloud(ls)
     tsi = np.full(sub.ncol, sub.tsi / oks.scalar("tsi"))
     prev = O.set_threads(threads)
+    secs = []
     try:
-        t0 = time.perf_counter()
-        rlw = O.solve_lw(okl, sub, ocl)
-        rsw = O.solve_sw(oks, sub, ocs, tsi_scaling=tsi)
-        dt = time.perf_counter() - t0
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            rlw = O.solve_lw(okl, sub, ocl)
+            rsw = O.solve_sw(oks, sub, ocs, tsi_scaling=tsi)
+            secs.append(time.perf_counter() - t0)
     finally:
         O.set_threads(prev)
+    dt = float(np.median(secs))
     units = sub.ncol * sub.nlay * (okl.ngpt + oks.ngpt)
     err = {}
     for name, ref, got in (("lw", rlw, glw), ("sw", rsw, gsw)):
@@ -346,7 +382,11 @@ def main():
     ap.add_argument("--nlay", type=int, default=128)
     ap.add_argument("--precision", type=int, choices=(32, 64), default=32)
     ap.add_argument("--n-col-block", type=int, default=0, help="columns per solver block (0: all)")
-    ap.add_argument("--cpu-cols", type=int, default=4096, help="columns of the CPU baseline sample")
+    ap.add_argument("--cpu-cols", type=int, default=4096, help="columns of the all-cores CPU baseline sample")
+    ap.add_argument("--cpu-cols-serial", type=int, default=1024, help="columns of the serial CPU baseline sample")
+    ap.add_argument("--large-ncol", type=int, default=262144, help="columns of the second (large) step")
+    ap.add_argument("--large-steps", type=int, default=5, help="timed steps at the large size")
+    ap.add_argument("--no-large", action="store_true", help="skip the large-size step")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0: OMP_NUM_THREADS, max 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
@@ -374,16 +414,118 @@ def main():
         return
 
     prec = 4 if args.precision == 32 else 8
-    E = prec
     ncol, nlay = args.ncol or 65536, args.nlay
-    log(f"workload: all-sky LW+SW, {ncol} columns x {nlay} layers, FP{args.precision}")
     kl, ks = syn.make_kdist(True, syn.SEED_KDIST_LW), syn.make_kdist(False, syn.SEED_KDIST_SW)
     ll, ls = syn.make_cloud_lut(True, syn.SEED_CLOUD_LW), syn.make_cloud_lut(False, syn.SEED_CLOUD_SW)
+    kd_lw, kd_sw = api.KDist(kl, AVAILABLE, prec), api.KDist(ks, AVAILABLE, prec)
+
+    if args.copy_calibration:
+        copy_calibration()
+        log("calibration copy done (1 GiB read, 1 GiB written)")
+
+    m = measure_step(ncol, nlay, prec, args, kd_lw, kd_sw, ll, ls)
+    atm, units, glw, gsw = m["atm"], m["units"], m["glw"], m["gsw"]
+
+    cpu = None
+    parity = {"finite": m["finite"]}
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        n1 = min(args.cpu_cols_serial, ncol)
+        log(f"CPU baseline: FP64 oracle, serial on {n1} columns, median of 3")
+        r1, s1, err1 = cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, n1, 1)
+        nn_ = min(args.cpu_cols, ncol)
+        log(f"CPU baseline: FP64 oracle, {threads} threads on {nn_} columns, median of 3")
+        rn, sn, err = cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, nn_, threads)
+        cpu = {"value": round(r1, 1), "unit": "col-gpt-layers/s", "cores": 1, "kind": "port",
+               "sample": f"first {n1} of the {ncol} columns x {nlay} layers, all-sky LW+SW, FP64 oracle "
+                         f"(restated reference CPU path, serial as the reference, 12-column blocks), median of "
+                         f"3 runs of {s1:.2f} s",
+               "all_cores": {"value": round(rn, 1), "cores": threads,
+                             "sample": f"first {nn_} columns, OpenMP over 12-column blocks (this build's "
+                                       f"parallelisation, not the reference's), median of 3 runs of {sn:.2f} s"}}
+        parity.update({"columns": nn_, "flux_err_max_wm2": round(max(err.values()), 6),
+                       "per_output": {k: round(v, 6) for k, v in err.items()}})
+        log(f"CPU baseline {r1:.3e} U/s serial, {rn:.3e} U/s on {threads} threads; "
+            f"max flux error {max(err.values()):.3e} W m-2")
+
+    nn, nets = None, None
+    if not args.no_nn and prec == 4:
+        log("NN gas optics: training surrogates and timing the same step with them")
+        nets, training = train_surrogates(kd_lw, kd_sw)
+        nn = nn_measure(kd_lw, kd_sw, nets, m["step"], m["out_lw"], m["out_sw"], glw, gsw, args.steps, units,
+                        m["kernel_avg_ms"])
+        nn["training"] = training
+        nn["speedup_vs_plain"] = round(m["ms_per_step"] / nn["ms_per_step"], 3)
+        log(f"NN path {nn['ms_per_step']:.2f} ms/step ({nn['speedup_vs_plain']:.2f}x); flux RMSE {nn['flux_rmse_wm2']}")
+
+    line = {
+        "metric": METRIC, "value": round(m["value"], 1), "unit": "col-gpt-layers/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(m["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if prec == 4 else "f64",
+        "data": "synthetic: seeded RRTMGP-layout k-distributions, cloud LUTs and profiles",
+        "config": {"workload": f"all-sky LW+SW forward, plain gas optics, {ncol} columns x {nlay} layers",
+                   "ncol": ncol, "nlay": nlay, "ngpt_lw": kd_lw.ngpt, "ngpt_sw": kd_sw.ngpt,
+                   "n_col_block": m["block"], "gpu_ms_per_step": m["gpu_ms_per_step"]},
+        "roofline": m["roofline"], "cpu_baseline": cpu, "parity": parity, "nn": nn,
+    }
+    kernels = m["kernels"]
+    del m, atm, glw, gsw
+
+    large = None
+    if not args.no_large and args.large_ncol > ncol:
+        _release_device_cache()
+        ml = measure_step(args.large_ncol, nlay, prec, args, kd_lw, kd_sw, ll, ls, steps=args.large_steps,
+                          warmup=min(args.warmup, 2))
+        large = {"workload": f"all-sky LW+SW forward, plain gas optics, {args.large_ncol} columns x {nlay} layers",
+                 "value": round(ml["value"], 1), "ms_per_step": round(ml["ms_per_step"], 3),
+                 "steps": args.large_steps, "roofline": ml["roofline"], "finite": ml["finite"],
+                 "kernels": ml["kernels"][:6]}
+        if nets is not None:
+            nl = nn_measure(kd_lw, kd_sw, nets, ml["step"], ml["out_lw"], ml["out_sw"], ml["glw"], ml["gsw"],
+                            args.large_steps, ml["units"], ml["kernel_avg_ms"])
+            nl["speedup_vs_plain"] = round(ml["ms_per_step"] / nl["ms_per_step"], 3)
+            large["nn"] = nl
+            log(f"NN at {args.large_ncol}: {nl['ms_per_step']:.2f} ms/step ({nl['speedup_vs_plain']:.2f}x)")
+        del ml
+        _release_device_cache()
+    line["large"] = large
+
+    adjoint = None
+    if not args.no_adjoint:
+        log("adjoint: LW + SW forward and adjoint at 16384 columns")
+        adjoint = adjoint_measure(kl, ks, ll, ls, prec)
+        log(f"adjoint {adjoint['adjoint_ms']:.1f} ms vs forward {adjoint['forward_ms']:.1f} ms")
+    line["adjoint"] = adjoint
+    line["kernels"] = kernels
+    print(json.dumps(line), flush=True)
+
+
+def _release_device_cache():
+    from rrtmgp_amd import _lib, device
+    device.synchronize()
+    _lib.check_hip(_lib.hip().rrtmgp_hip_cache_release())
+
+
+def _traffic():
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    return None
+
+
+def measure_step(ncol, nlay, prec, args, kd_lw, kd_sw, ll, ls, steps=None, warmup=None):
+    """Set up the all-sky step at ncol columns (inputs resident), warm up, time `steps` steps with
+    no per-launch instrumentation, then run the same number of steps again with per-launch HIP
+    events for the per-kernel table and the roofline of the dominant kernel."""
+    from rrtmgp_amd import api, device, profiling, synthetic as syn
+    steps = steps or args.steps
+    warmup = args.warmup if warmup is None else warmup
+    E = prec
+    log(f"workload: all-sky LW+SW, {ncol} columns x {nlay} layers, FP{8 * prec}")
     t0 = time.perf_counter()
     atm = syn.make_atmosphere(syn.SEED_ATMOS[1], ncol, nlay, cloudy=True)
     log(f"profiles generated in {time.perf_counter() - t0:.1f} s")
-
-    kd_lw, kd_sw = api.KDist(kl, AVAILABLE, prec), api.KDist(ks, AVAILABLE, prec)
     solver_lw = api.SolverLW(kd_lw, api.Cloud(ll, prec))
     solver_sw = api.SolverSW(kd_sw, api.Cloud(ls, prec))
     blk = args.n_col_block if args.n_col_block > 0 else ncol
@@ -402,113 +544,76 @@ def main():
         solver_lw.solve(da, outputs=out_lw)
         solver_sw.solve(da, tsi, outputs=out_sw)
 
-    if args.copy_calibration:
-        copy_calibration()
-        log("calibration copy done (1 GiB read, 1 GiB written)")
-
-    for i in range(max(args.warmup, 1)):
+    for i in range(max(warmup, 1)):
         step()
         device.synchronize()
         log(f"warmup step {i + 1} done")
 
     start, stop = device.Event(), device.Event()
-    profiling.enable(True)
     device.synchronize()
     t0 = time.perf_counter()
     start.record()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     stop.record()
     device.synchronize()
     wall = time.perf_counter() - t0
-    profiling.enable(False)
     gpu_ms = start.elapsed_ms(stop)
+    log(f"{steps} timed steps: {wall * 1e3:.1f} ms wall, {gpu_ms:.1f} ms between events")
+
+    profiling.enable(True)
+    for _ in range(steps):
+        step()
+    device.synchronize()
+    profiling.enable(False)
     kernels = profiling.collect()
-    log(f"{args.steps} timed steps: {wall * 1e3:.1f} ms wall, {gpu_ms:.1f} ms between events")
+    log(f"{steps} profiled steps done ({len(kernels)} kernels)")
 
     units = ncol * nlay * (kd_lw.ngpt + kd_sw.ngpt)
-    ms_per_step = wall * 1e3 / args.steps
-    value = units * args.steps / wall
-
+    ms_per_step = wall * 1e3 / steps
     n2d = sum(1 for v in atm.vmr.values() if v.shape[0] > 1)
     abytes = algorithmic_bytes(ncol, nlay, E, kd_lw, kd_sw, n2d, True)
+    tr = _traffic()
+    tr_ok = tr is not None and tr.get("workload") == {"ncol": ncol, "nlay": nlay, "precision": 8 * prec}
     krows = []
     for k in kernels:
-        total_b = abytes.get(k.name)
-        total_b = total_b * args.steps if total_b is not None else None
-        krows.append({"kernel": k.name, "launches": k.launches, "total_ms": round(k.total_ms, 4),
-                      "avg_ms": round(k.avg_ms, 5),
-                      "algorithmic_GBps": (round(total_b / (k.total_ms * 1e-3) / 1e9, 1)
-                                           if total_b is not None and k.total_ms > 0 else None)})
+        ab = abytes.get(k.name)
+        row = {"kernel": k.name, "launches": k.launches, "total_ms": round(k.total_ms, 4),
+               "avg_ms": round(k.avg_ms, 5),
+               "class_api_GBps": (round(ab * steps / (k.total_ms * 1e-3) / 1e9, 1)
+                                  if ab is not None and k.total_ms > 0 else None)}
+        rec = tr.get("kernels", {}).get(k.name) if tr_ok else None
+        if rec and k.avg_ms > 0:
+            row["hbm_GBps_measured"] = round(rec["hbm_bytes_per_launch"] / (k.avg_ms * 1e-3) / 1e9, 1)
+        if row["class_api_GBps"] is not None and row["class_api_GBps"] > HBM_PEAK_GBS:
+            row["note"] = ("class-API credit above the HBM peak: the fused kernel does not move all the "
+                           "arrays the reference's class API would (see hbm_GBps_measured)")
+        krows.append(row)
     krows.sort(key=lambda r: -r["total_ms"])
-    dom = next((r for r in krows if r["algorithmic_GBps"] is not None), None)
+    dom = next((r for r in krows if r["class_api_GBps"] is not None), None)
     roofline = None
     if dom is None:
         log("no kernel timings were recorded; roofline left empty")
     else:
-        dom_bytes_per_launch = abytes[dom["kernel"]] * args.steps / dom["launches"]
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            with open(TRAFFIC_FILE) as f:
-                tr = json.load(f)
-            rec = tr.get("kernels", {}).get(dom["kernel"])
-            if rec and tr.get("workload") == {"ncol": ncol, "nlay": nlay, "precision": args.precision}:
-                traffic = rec.get("hbm_bytes_per_launch")
+        dom_bytes_per_launch = abytes[dom["kernel"]] * steps / dom["launches"]
+        rec = tr.get("kernels", {}).get(dom["kernel"]) if tr_ok else None
         step_bytes = sum(abytes.values())
         step_gbs = step_bytes / (ms_per_step * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["algorithmic_GBps"],
+        roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": dom["class_api_GBps"],
                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(dom["algorithmic_GBps"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "frac": round(dom["class_api_GBps"] / HBM_PEAK_GBS, 4),
+                    "traffic": rec.get("hbm_bytes_per_launch") if rec else None,
                     "algorithmic_bytes_per_launch": int(dom_bytes_per_launch),
                     "avg_launch_ms": dom["avg_ms"],
                     "step": {"algorithmic_bytes": int(step_bytes), "achieved": round(step_gbs, 1),
                              "frac": round(step_gbs / HBM_PEAK_GBS, 4)}}
-
     glw = {k: v.to_host() for k, v in out_lw.items()}
     gsw = {k: v.to_host() for k, v in out_sw.items()}
     finite = all(np.isfinite(a).all() for a in list(glw.values()) + list(gsw.values()))
-
-    cpu = None
-    parity = {"finite": bool(finite)}
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        ncpu = min(args.cpu_cols, ncol)
-        log(f"CPU baseline: FP64 oracle on {ncpu} columns, {threads} threads")
-        rate, secs, err = cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncpu, threads)
-        cpu = {"value": round(rate, 1), "unit": "col-gpt-layers/s", "cores": threads, "kind": "port",
-               "sample": f"first {ncpu} of the {ncol} columns x {nlay} layers, all-sky LW+SW, FP64 "
-                         f"oracle (restated reference CPU path, 12-column blocks), {secs:.2f} s"}
-        parity.update({"columns": ncpu, "flux_err_max_wm2": round(max(err.values()), 6),
-                       "per_output": {k: round(v, 6) for k, v in err.items()}})
-        log(f"CPU baseline {rate:.3e} U/s in {secs:.2f} s; max flux error {max(err.values()):.3e} W m-2")
-
-    nn = None
-    if not args.no_nn and prec == 4:
-        log("NN gas optics: training surrogates and timing the same step with them")
-        nn = nn_measure(kd_lw, kd_sw, step, out_lw, out_sw, glw, gsw, args.steps, units,
-                        {k.name: k.avg_ms for k in kernels})
-        nn["speedup_vs_plain"] = round(ms_per_step / nn["ms_per_step"], 3)
-        log(f"NN path {nn['ms_per_step']:.2f} ms/step ({nn['speedup_vs_plain']:.2f}x); flux RMSE {nn['flux_rmse_wm2']}")
-
-    adjoint = None
-    if not args.no_adjoint:
-        log("adjoint: LW + SW forward and adjoint at 16384 columns")
-        adjoint = adjoint_measure(kl, ks, ll, ls, prec)
-        log(f"adjoint {adjoint['adjoint_ms']:.1f} ms vs forward {adjoint['forward_ms']:.1f} ms")
-
-    line = {
-        "metric": METRIC, "value": round(value, 1), "unit": "col-gpt-layers/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if prec == 4 else "f64",
-        "data": "synthetic: seeded RRTMGP-layout k-distributions, cloud LUTs and profiles",
-        "config": {"workload": f"all-sky LW+SW forward, plain gas optics, {ncol} columns x {nlay} layers",
-                   "ncol": ncol, "nlay": nlay, "ngpt_lw": kd_lw.ngpt, "ngpt_sw": kd_sw.ngpt,
-                   "n_col_block": blk, "gpu_ms_per_step": round(gpu_ms / args.steps, 3)},
-        "roofline": roofline, "cpu_baseline": cpu, "parity": parity, "nn": nn, "adjoint": adjoint,
-        "kernels": krows,
-    }
-    print(json.dumps(line), flush=True)
+    return {"atm": atm, "units": units, "ms_per_step": ms_per_step, "value": units * steps / wall,
+            "gpu_ms_per_step": round(gpu_ms / steps, 3), "block": blk, "roofline": roofline, "kernels": krows,
+            "kernel_avg_ms": {k.name: k.avg_ms for k in kernels}, "glw": glw, "gsw": gsw, "finite": bool(finite),
+            "step": step, "out_lw": out_lw, "out_sw": out_sw}
 
 
 if __name__ == "__main__":

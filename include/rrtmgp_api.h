@@ -182,6 +182,9 @@ int rrtmgp_api_heating_rate(int ncol, int nlay, const rrtmgp_real* flux_up,
                             const rrtmgp_real* flux_dn, const rrtmgp_real* plev, rrtmgp_real* hr);
 
 /* Radiation_solver_{longwave,shortwave}_gpu (include_test/Radiation_solver.h:33-233).         */
+/* heating_rate (ncol, nlay) may be NULL; when given it receives heating rates [K/day] formed    */
+/* from each layer's net-flux convergence inside the solver, summed over g-points in FP64 (an   */
+/* addition: the reference has no heating rates).                                                */
 int rrtmgp_api_solver_lw_create(void* kdist, void* cloud, void** solver);
 int rrtmgp_api_solver_sw_create(void* kdist, void* cloud, void** solver);
 int rrtmgp_api_solver_destroy(void* solver);
@@ -193,7 +196,7 @@ int rrtmgp_api_solve_lw(void* solver, int ncol, int nlay, int cloudy, int bnd_fl
                         const rrtmgp_real* lwp, const rrtmgp_real* iwp, const rrtmgp_real* rel,
                         const rrtmgp_real* rei, rrtmgp_real* flux_up, rrtmgp_real* flux_dn,
                         rrtmgp_real* flux_net, rrtmgp_real* bnd_up, rrtmgp_real* bnd_dn,
-                        rrtmgp_real* bnd_net);
+                        rrtmgp_real* bnd_net, rrtmgp_real* heating_rate);
 int rrtmgp_api_solve_sw(void* solver, int ncol, int nlay, int cloudy, int bnd_fluxes,
                         int delta_cloud, const rrtmgp_real* play, const rrtmgp_real* plev,
                         const rrtmgp_real* tlay, const rrtmgp_real* tlev, void* gas_concs,
@@ -203,7 +206,7 @@ int rrtmgp_api_solve_sw(void* solver, int ncol, int nlay, int cloudy, int bnd_fl
                         const rrtmgp_real* rel, const rrtmgp_real* rei, rrtmgp_real* flux_up,
                         rrtmgp_real* flux_dn, rrtmgp_real* flux_dir, rrtmgp_real* flux_net,
                         rrtmgp_real* bnd_up, rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir,
-                        rrtmgp_real* bnd_net);
+                        rrtmgp_real* bnd_net, rrtmgp_real* heating_rate);
 
 /* Adjoints of the two drivers above (no reference counterpart): for cotangents g_flux_* of   */
 /* the broadband fluxes (ncol, nlev), the gradients of L = sum(g_flux * flux) with respect to */

@@ -47,6 +47,7 @@ typedef signed char rrtmgp_bool;
 const char* rrtmgp_hip_last_error(void);
 int rrtmgp_hip_device_count(int* n);
 int rrtmgp_hip_set_device(int device);
+int rrtmgp_hip_get_device(int* device);
 int rrtmgp_hip_device_name(char* buf, int len);
 int rrtmgp_hip_malloc(void** ptr, size_t bytes);
 int rrtmgp_hip_free(void* ptr);
@@ -377,20 +378,32 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
 /* g-point fluxes with Fluxes_byband::reduce, src/Fluxes.cpp:142-197): the solvers sum the    */
 /* fluxes over the g-points of each band (band_lims_gpt (2, nbnd), 1-based, host memory) and  */
 /* write band fluxes (ncol, nlev, nbnd) and/or broadband fluxes (ncol, nlev); either set may  */
-/* be NULL.  The SW flux_dn outputs are total (diffuse + direct).                             */
+/* be NULL.  The SW flux_dn outputs are total (diffuse + direct).  They take the surface       */
+/* inputs as the class API holds them, so nothing is expanded per call: sfc_emis and the       */
+/* albedos are (nbnd, ncol) (the reference's Rte_lw/Rte_sw arguments, expanded there by        */
+/* expand_and_transpose, src_cuda/Rte_lw.cu:36-56), and secants / weights are the nmus Gauss   */
+/* secants and weights of the quadrature (device arrays of nmus values; the reference fills    */
+/* (ncol, ngpt, nmus) secants with the same value per angle, rte_solver_kernels.cu:16-32).     */
+/* Heating rates (no reference counterpart): when heating_rate (ncol, nlay) is given, with plev  */
+/* (ncol, nlev), each layer's net-flux convergence is formed per g-point from the layer's own    */
+/* terms -- LW: -fac (S - (1 - T) I_in) per sweep; SW: (dn_dif(top) + up(bottom)) (1 - R - T) + */
+/* dir(top) (1 - Tn) - S_up - S_dn, with 1 - R - T and 1 - Tn formed without cancellation -- and */
+/* summed over g-points in FP64; hr = g0/cp * convergence / |dp| [K/day], g0 = 9.80665 m s-2,    */
+/* cp = 1004.64 J kg-1 K-1.  This is the difference g0/cp (net_i - net_i+1) / (p_i+1 - p_i) of  */
+/* rrtmgp_hip_heating_rate_*, without differencing rounded level fluxes.  NULL: no heating rate.*/
 #define RRTMGP_DECL_SOLVERS_BND(S, T)                                                          \
     int rrtmgp_hip_lw_solver_noscat_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,          \
         int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,           \
         const T* lay_source, const T* lev_source_inc, const T* lev_source_dec,                 \
         const T* sfc_emis, const T* sfc_src, const T* inc_flux, int64_t nbnd,                  \
         const int* band_lims_gpt, T* bnd_flux_up, T* bnd_flux_dn, T* broadband_up,             \
-        T* broadband_dn, void* stream);                                                         \
+        T* broadband_dn, const T* plev, T* heating_rate, void* stream);                         \
     int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,         \
         int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,                    \
         const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,     \
         const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,         \
         T* bnd_flux_dn, T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir,   \
-        void* stream);
+        const T* plev, T* heating_rate, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
 
 /* Longwave with factored Planck sources.  No counterpart in the reference, whose gas optics */
@@ -410,7 +423,8 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
 /*                            FP products, so fluxes equal the plain pair's bit for bit.       */
 /* Two arrays of (ncol, nlay, ngpt) are neither written nor read.  Used by the longwave driver */
 /* (Radiation_solver_longwave_gpu), also with a surrogate (rrtmgp_hip_mlp_gas_optics_lw_factored */
-/* and this entry's tile list for its fallback).                                                */
+/* and this entry's tile list for its fallback).  The solver's secants, weights and sfc_emis are */
+/* those of rrtmgp_hip_lw_solver_noscat_bnd_*: per angle, and (nbnd, ncol).                     */
 #define RRTMGP_DECL_LW_FACTORED(S, T)                                                          \
     int rrtmgp_hip_gas_optics_lw_factored_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,       \
         int64_t nlay, const int* tiles, int64_t ntiles, const T* play, const T* tlay,           \
@@ -422,7 +436,8 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
         int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,           \
         const T* planck_frac, const T* band_lay_src, const T* band_lev_src, const T* sfc_emis,  \
         const T* sfc_src, const T* inc_flux, int64_t nbnd, const int* band_lims_gpt,           \
-        T* bnd_flux_up, T* bnd_flux_dn, T* broadband_up, T* broadband_dn, void* stream);
+        T* bnd_flux_up, T* bnd_flux_dn, T* broadband_up, T* broadband_dn, const T* plev,       \
+        T* heating_rate, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_FACTORED)
 
 /* ------------------------------------------------------------------------------------------ */
@@ -652,6 +667,19 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_COPY_COLS)
     int rrtmgp_hip_count_outside_##S(int64_t n, const T* arr, T lo, T hi, int64_t* count,      \
         void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_COUNT)
+
+/* All input checks of one Gas_optics_rrtmgp::gas_optics call (src/Gas_optics_rrtmgp.cpp:       */
+/* 805-826) in one launch and one device-to-host copy: for each of n <= 8 device arrays of sizes  */
+/* elements (host arrays of pointers, sizes and bounds), counts[k] = values outside [lo[k],      */
+/* hi[k]] (below lo[k] when hi[k] is NaN; NaN counts as outside).  When play is given, also     */
+/* play_ends = {play(1,1), play(1,nlay)} of a (ncol, nlay) array, from which the surface layer  */
+/* is chosen (:1333).  Synchronizes the stream once (the reference's CPU path checks on the     */
+/* host; its GPU path does not check, src_cuda/Gas_optics_rrtmgp.cu:907-943).                  */
+#define RRTMGP_DECL_CHECK(S, T)                                                                \
+    int rrtmgp_hip_check_ranges_##S(int n, const T* const* arrays, const int64_t* sizes,       \
+        const T* lo, const T* hi, int64_t* counts, const T* play, int64_t ncol, int64_t nlay,  \
+        T* play_ends, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_CHECK)
 
 #ifdef __cplusplus
 }

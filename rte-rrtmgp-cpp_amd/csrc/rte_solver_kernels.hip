@@ -98,7 +98,7 @@ int make_groups(int64_t ngpt, int64_t nbnd, const int* band_lims, Groups& g)
 // Writes the LDS slots kk < nslots, components c < NC, summed over the group's g-points, to
 // out[c] at level lev_of(kk) (skipped when negative).
 // red layout: ((kk * NC + c) * SB_GPTS + w) * SB_COLS + lane.
-template<typename Real, int NC, typename LevOf>
+template<typename Real, int NC, int STRIDE = NC, typename LevOf>
 __device__ __forceinline__ void write_group_sums(const Real* red, int nslots, int gcount, int64_t col0,
                                                  int64_t ncol, int64_t nlev, int gi, Real* const (&out)[NC],
                                                  LevOf lev_of)
@@ -113,7 +113,7 @@ __device__ __forceinline__ void write_group_sums(const Real* red, int nslots, in
         const int64_t icol = col0 + lane;
         const int64_t lev = lev_of(kk);
         if (icol >= ncol || lev < 0) continue;
-        const Real* r = red + (int64_t(kk) * NC + c) * SB_GPTS * SB_COLS + lane;
+        const Real* r = red + (int64_t(kk) * STRIDE + c) * SB_GPTS * SB_COLS + lane;
         Real s = r[0];
         for (int w = 1; w < gcount; ++w) s += r[w * SB_COLS];
         out[c][icol + ncol * (lev + nlev * gi)] = s;
@@ -204,10 +204,58 @@ __device__ __forceinline__ void lw_layer(float t, float& T, float& fact)
     fact = (t < .25f) ? t * q : (1.f - T) * rcp_f32(t) - T;
 }
 
+// 1 - e^{-t} given T = e^{-t}, without the cancellation of 1 - T for small t.
+__device__ __forceinline__ double one_minus_T(double t, double) { return -expm1(-t); }
+__device__ __forceinline__ float one_minus_T(float t, float T) { return one_minus_exp_neg_f32(t, T); }
+
+// Heating-rate island: each layer's absorbed flux (net-flux convergence) summed over the group's
+// g-points in FP64, for the LDS slots kk < nslots of component comp (NCH components per slot),
+// written to out[icol + ncol (layer + nlay gi)] for layer lay_of(kk) (skipped when negative).
+template<typename Real, int NCH, typename LayOf>
+__device__ __forceinline__ void write_group_layer_sums(const Real* red, int comp, int nslots, int gcount,
+                                                       int64_t col0, int64_t ncol, int64_t nlay, int gi,
+                                                       double* out, LayOf lay_of)
+{
+    const int tid = threadIdx.y * SB_COLS + threadIdx.x;
+    const int items = nslots * SB_COLS;
+    for (int it = tid; it < items; it += SB_COLS * SB_GPTS)
+    {
+        const int lane = it % SB_COLS;
+        const int kk = it / SB_COLS;
+        const int64_t icol = col0 + lane;
+        const int64_t lay = lay_of(kk);
+        if (icol >= ncol || lay < 0) continue;
+        const Real* r = red + (int64_t(kk) * NCH + comp) * SB_GPTS * SB_COLS + lane;
+        double s = double(r[0]);
+        for (int w = 1; w < gcount; ++w) s += double(r[w * SB_COLS]);
+        out[icol + ncol * (lay + nlay * gi)] = s;
+    }
+}
+
+// Heating rate from the per-group absorbed fluxes of nparts sweeps, (ncol, nlay, ngroups) each:
+// hr = g0 / cp * (sum over parts and groups, in order) / |p(lev+1) - p(lev)|, in FP64.  Equal to
+// g0 / cp (net_i - net_i+1) / (p_i+1 - p_i) with net = dn - up, in either orientation.
+template<typename Real>
+__global__ void absorbed_heating_rate_kernel(int64_t ncol, int64_t nlay, int ngroups, int nparts,
+                                             const double* __restrict__ parts, const Real* __restrict__ plev,
+                                             Real* __restrict__ hr)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncol * nlay) return;
+    const int64_t ncl = ncol * nlay;
+    double s = 0.;
+    for (int g = 0; g < ngroups; ++g)
+        for (int p = 0; p < nparts; ++p)
+            s += parts[i + ncl * (g + int64_t(ngroups) * p)];
+    const double dp = fabs(double(plev[i + ncol]) - double(plev[i]));
+    const double g0 = 9.80665, cp = 1004.64, sec_per_day = 86400.0;
+    hr[i] = Real(g0 / cp * s / dp * sec_per_day);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Longwave: no scattering, linear-in-tau source, nmus <= NMU Gaussian angles.  The layer inputs
 // of the next K-level segment are read while the current segment is computed.
-template<typename Real, bool JAC, int NMU, bool FACT>
+template<typename Real, bool JAC, int NMU, bool FACT, bool HR = false>
 __global__ __launch_bounds__(SB_COLS * SB_GPTS) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 8 : 4)))
 void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int nmus,
                       const Real* __restrict__ secants, const Real* __restrict__ weights,
@@ -218,11 +266,15 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
                       Real* __restrict__ flux_dn, const Real* __restrict__ sfc_src_jac,
                       Groups grp, Real* __restrict__ grp_up, Real* __restrict__ grp_dn,
                       Real* __restrict__ grp_jac, const Real* __restrict__ bnd_lay,
-                      const Real* __restrict__ bnd_lev)
+                      const Real* __restrict__ bnd_lev, int64_t emis_nbnd, int secants_per_angle,
+                      double* __restrict__ absorbed)
 {
     constexpr int K = NMU == 1 ? LwSeg<Real>::K1 : LwSeg<Real>::K4;
     constexpr int NC = JAC ? 2 : 1;
-    __shared__ Real red[K * NC * SB_GPTS * SB_COLS];
+    // With HR, one more LDS component per slot: the layer's absorbed flux of this g-point,
+    // -sum_mu fac (S - (1 - T) I_in), from the sweep's own terms (no difference of level fluxes).
+    constexpr int NCH = NC + (HR ? 1 : 0);
+    __shared__ Real red[K * NCH * SB_GPTS * SB_COLS];
 
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.y));   // wavefront = g-point
@@ -257,7 +309,9 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
         D[mu] = Real(0); fac[mu] = Real(0); I[mu] = Real(0); Ij[mu] = Real(0);
         if (active && mu < nmus)
         {
-            D[mu] = secants[c2 + ncol * ngpt * mu];
+            // Secants per (column, g-point, angle) as the reference's lw_secants_array fills them,
+            // or one secant per angle (the only form the class API can express).
+            D[mu] = secants_per_angle ? secants[mu] : secants[c2 + ncol * ngpt * mu];
             fac[mu] = Real(2) * pi * weights[mu];
             I[mu] = (inc_flux ? inc_flux[c2] : Real(0)) / fac[mu];
         }
@@ -305,7 +359,7 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
             for (int kk = 0; kk < K; ++kk)
             {
                 const int64_t j = s0 + kk;
-                Real f = Real(0), fj = Real(0);
+                Real f = Real(0), fj = Real(0), ab = Real(0);
                 if (active && j <= nlay)
                 {
                     if (j > 0)
@@ -315,7 +369,11 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
                             if (mu < nmus)
                             {
                                 Real T, fact;
-                                lw_layer(ct[kk] * D[mu], T, fact);
+                                const Real t = ct[kk] * D[mu];
+                                lw_layer(t, T, fact);
+                                if constexpr (HR)
+                                    ab -= fac[mu] * (one_minus_T(t, T) * (cx[kk] - I[mu]) +
+                                                     Real(2) * fact * (cl[kk] - cx[kk]));
                                 I[mu] = T * I[mu] + ((Real(1) - T) * cx[kk] + Real(2) * fact * (cl[kk] - cx[kk]));
                                 if (JAC && up) Ij[mu] = T * Ij[mu];
                             }
@@ -331,8 +389,9 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
                 }
                 if (reduce)
                 {
-                    red[(kk * NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
-                    if (JAC && up) red[(kk * NC + 1) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = fj;
+                    red[(kk * NCH) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = f;
+                    if (JAC && up) red[(kk * NCH + 1) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = fj;
+                    if constexpr (HR) red[(kk * NCH + NC) * SB_GPTS * SB_COLS + w * SB_COLS + lane] = ab;
                 }
             }
             if (reduce)
@@ -343,13 +402,24 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
                 if (JAC && up)
                 {
                     Real* const outs[2] = {grp_out, grp_jac};
-                    write_group_sums<Real, 2>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+                    write_group_sums<Real, 2, NCH>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
                 }
                 else
                 {
-                    // NC == 2 keeps the component stride of the Jacobian layout.
+                    // NCH keeps the component stride of the Jacobian / heating-rate layouts.
                     Real* const outs[1] = {grp_out};
-                    write_group_sums_stride<Real, NC>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+                    write_group_sums_stride<Real, NCH>(red, nsl, gcount, col0, ncol, nlev, gi, outs, lev_of);
+                }
+                if constexpr (HR)
+                {
+                    // Sweep level j > 0 closes layer layer_of(j); the down sweep's parts go to
+                    // absorbed[.., 0], the up sweep's to absorbed[.., 1].
+                    double* part = absorbed + (up ? ncol * nlay * int64_t(grp.n) : int64_t(0));
+                    write_group_layer_sums<Real, NCH>(red, NC, nsl, gcount, col0, ncol, nlay, gi, part,
+                                                      [&](int kk) -> int64_t {
+                                                          const int64_t jj = s0 + kk;
+                                                          return jj >= 1 ? int64_t(layer_of(jj)) : int64_t(-1);
+                                                      });
                 }
                 __syncthreads();
             }
@@ -364,7 +434,8 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
     // Surface: reflection of the downward intensity plus emission; then the up sweep with
     // level j counted from the surface (0 = surface).
     {
-        const Real e = active ? emis[c2] : Real(0);
+        // Surface emissivity per (column, g-point), or per (band, column) read directly.
+        const Real e = !active ? Real(0) : (emis_nbnd > 0 ? emis[grp.band[gi] + emis_nbnd * icol] : emis[c2]);
         const Real ss = active ? sfc_src[c2] : Real(0);
         const Real sj = (active && JAC) ? sfc_src_jac[c2] : Real(0);
 #pragma unroll
@@ -388,7 +459,9 @@ __device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, d
 }
 
 // One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
-struct LayerSW { double R, T, SU, SD, Tn; };
+// ab = 1 - R - T (diffuse absorptance) and omTn = 1 - Tn are only read by the heating-rate
+// variant (dead code otherwise).
+struct LayerSW { double R, T, SU, SD, Tn, ab, omTn; };
 
 // 1/x in FP64 for the adding denominators, x in (0, 1]: v_rcp_f64 (relative error up to 4.6e-8
 // on gfx950) and one Newton step, which leaves at most 2.3e-15, far below the FP32 rounding of
@@ -457,7 +530,11 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     const float eps = float(Limits<double>::eps);
     const float rd = fmaxf(eps, fminf(rdf, b));
     const float td = fmaxf(eps, fminf(tdf, b - rd));
-    L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), double(rd) * dir, double(td) * dir, double(Tn)};
+    // 1 - R - T = RT [k (2 - c2) + gamma1 c2 - gamma2 c2 - 2 k e1] = RT [k a^2 + (gamma1 - gamma2) c2] with
+    // gamma1 - gamma2 = 2 (1 - w0): a sum of non-negative terms, free of the cancellation of 1 - R - T.
+    const float ab = RT * (k * a * a + 2.f * one_m_w0 * c2);
+    L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), double(rd) * dir, double(td) * dir, double(Tn),
+         double(ab), double(one_minus_exp_neg_f32(y, Tn))};
     return true;
 }
 
@@ -471,7 +548,7 @@ __device__ __forceinline__ LayerSW sw_layer(Real tau, Real w0, Real g, Real mu0,
         if (sw_layer_fp32(tau, w0, g, mu0, inv_mu0, dir, L)) return L;
     double Rdif, Tdif, Rdir, Tdir, Tn;
     sw_two_stream(double(tau), double(w0), double(g), double(mu0), Rdif, Tdif, Rdir, Tdir, Tn);
-    return {Rdif, Tdif, Rdir * dir, Tdir * dir, Tn};
+    return {Rdif, Tdif, Rdir * dir, Tdir * dir, Tn, 1. - Rdif - Tdif, -expm1(-double(tau) / double(mu0))};
 }
 
 // Column above the next level from the column above this one and the layer between.
@@ -513,7 +590,7 @@ template<> struct Beam<float>
     }
 };
 
-template<typename Real>
+template<typename Real, bool HR = false>
 __global__ __launch_bounds__(SB_COLS * SB_GPTS)
 void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                       const Real* __restrict__ tau, const Real* __restrict__ ssa,
@@ -522,12 +599,17 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                       const Real* __restrict__ inc_dir, const Real* __restrict__ inc_dif,
                       Real* __restrict__ flux_up, Real* __restrict__ flux_dn, Real* __restrict__ flux_dir,
                       Real* __restrict__ ckpt, Groups grp, Real* __restrict__ grp_up,
-                      Real* __restrict__ grp_dn, Real* __restrict__ grp_dir)
+                      Real* __restrict__ grp_dn, Real* __restrict__ grp_dir, int64_t alb_nbnd,
+                      double* __restrict__ absorbed)
 {
     constexpr int K = SwSeg<Real>::K;
     constexpr int NC = 3;
+    // With HR a fourth component per slot: the absorbed flux of the layer below the slot's level,
+    // (dn_dif(top) + up(bottom)) (1 - R - T) + dir(top) (1 - Tn) - SU - SD for this g-point, from the
+    // layer's own terms instead of a difference of level fluxes.
+    constexpr int NCH = NC + (HR ? 1 : 0);
     // Slots 0..K-1: the levels of a segment; slot K: the surface level.
-    __shared__ Real red[(K + 1) * NC * SB_GPTS * SB_COLS];
+    __shared__ Real red[(K + 1) * NCH * SB_GPTS * SB_COLS];
 
     const int lane = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.y));   // wavefront = g-point
@@ -591,9 +673,12 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         }
     }
 
-    // Surface.
-    double A = active ? double(alb_dif[c2]) : 0.;
-    double S = beam.dir * (active ? double(alb_dir[c2]) : 0.);
+    // Upward flux at the bottom of the layer being closed (heating-rate variant).
+    double up_below = 0.;
+    // Surface: albedos per (column, g-point), or per (band, column) read directly.
+    const int64_t ia = alb_nbnd > 0 ? grp.band[gi] + alb_nbnd * icol : c2;
+    double A = active ? double(alb_dif[ia]) : 0.;
+    double S = beam.dir * (active ? double(alb_dir[ia]) : 0.);
     {
         const double dir = beam.dir;
         const double dn = (D + R_up * S) * rcp_f64(1. - R_up * A);
@@ -605,10 +690,11 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         }
         if (reduce)
         {
-            red[((K * NC + 0) * SB_GPTS + w) * SB_COLS + lane] = Real(up);
-            red[((K * NC + 1) * SB_GPTS + w) * SB_COLS + lane] = Real(dn + dir);
-            red[((K * NC + 2) * SB_GPTS + w) * SB_COLS + lane] = Real(dir);
+            red[((K * NCH + 0) * SB_GPTS + w) * SB_COLS + lane] = Real(up);
+            red[((K * NCH + 1) * SB_GPTS + w) * SB_COLS + lane] = Real(dn + dir);
+            red[((K * NCH + 2) * SB_GPTS + w) * SB_COLS + lane] = Real(dir);
         }
+        up_below = up;
     }
 
     // Sweep up, one segment at a time from the bottom.
@@ -619,6 +705,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         if (active)
         {
             Real sR[K], sT[K], sSU[K], sSD[K], sRup[K], sD[K], sDir[K];
+            Real sAb[HR ? K : 1], sOmTn[HR ? K : 1];
             double r_up = double(ck(s, 0)), d = double(ck(s, 1));
             Beam<Real> bm = beam;
             bm.restore(ck(s, 2));
@@ -637,6 +724,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                     sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(bm.dir);
                     const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, bm.dir);
                     sR[kk] = Real(L.R); sT[kk] = Real(L.T); sSU[kk] = Real(L.SU); sSD[kk] = Real(L.SD);
+                    if constexpr (HR) { sAb[kk] = Real(L.ab); sOmTn[kk] = Real(L.omTn); }
                     above_update(L, r_up, d);
                     bm.step(L, tv[kk], inv_m0r);
                 }
@@ -659,9 +747,16 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                     }
                     if (reduce)
                     {
-                        red[((kk * NC + 0) * SB_GPTS + w) * SB_COLS + lane] = Real(up);
-                        red[((kk * NC + 1) * SB_GPTS + w) * SB_COLS + lane] = Real(dn + dd);
-                        red[((kk * NC + 2) * SB_GPTS + w) * SB_COLS + lane] = Real(dd);
+                        red[((kk * NCH + 0) * SB_GPTS + w) * SB_COLS + lane] = Real(up);
+                        red[((kk * NCH + 1) * SB_GPTS + w) * SB_COLS + lane] = Real(dn + dd);
+                        red[((kk * NCH + 2) * SB_GPTS + w) * SB_COLS + lane] = Real(dd);
+                    }
+                    if constexpr (HR)
+                    {
+                        const double ab = (dn + up_below) * double(sAb[kk]) + dd * double(sOmTn[kk]) -
+                                          double(sSU[kk]) - double(sSD[kk]);
+                        red[((kk * NCH + NC) * SB_GPTS + w) * SB_COLS + lane] = Real(ab);
+                        up_below = up;
                     }
                 }
         }
@@ -671,10 +766,14 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
             Real* const outs[3] = {grp_up, grp_dn, grp_dir};
             // The surface slot K goes out with the first (bottom) segment.
             const int nsl = (s == nseg - 1) ? K + 1 : nk;
-            write_group_sums<Real, 3>(red, nsl, gcount, col0, ncol, nlev, gi, outs, [&](int kk) -> int64_t {
+            write_group_sums<Real, 3, NCH>(red, nsl, gcount, col0, ncol, nlev, gi, outs,
+                                                              [&](int kk) -> int64_t {
                 if (kk == K) return lev_of(nlay);
                 return kk < nk ? int64_t(lev_of(j0 + kk)) : int64_t(-1);
             });
+            if constexpr (HR)
+                write_group_layer_sums<Real, NCH>(red, NC, nk, gcount, col0, ncol, nlay, gi, absorbed,
+                                                  [&](int kk) -> int64_t { return int64_t(lay_of(j0 + kk)); });
             __syncthreads();
         }
     }
@@ -714,6 +813,17 @@ int group_sums(int64_t ncv, const Groups& grp, const Real* gf, Real* bnd, Real* 
 }
 
 template<typename Real>
+int absorbed_heating_rate(int64_t ncol, int64_t nlay, int ngroups, int nparts, const double* parts, const Real* plev,
+                          Real* hr, hipStream_t s)
+{
+    RRTMGP_TIMED("absorbed_heating_rate_kernel", s);
+    hipLaunchKernelGGL(absorbed_heating_rate_kernel<Real>, dim3(grid1(ncol * nlay, 256)), dim3(256), 0, s, ncol,
+                       nlay, ngroups, nparts, parts, plev, hr);
+    RRTMGP_LAUNCHED("absorbed_heating_rate_kernel");
+    return RRTMGP_OK;
+}
+
+template<typename Real>
 __global__ void lw_secants_kernel(int64_t n, int64_t ncg, int64_t n_quad, int64_t max_pts,
                                   const Real* __restrict__ gauss_Ds, Real* __restrict__ secants)
 {
@@ -747,9 +857,12 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
               const Real* emis, const Real* sfc_src, const Real* inc_flux, Real* flux_up, Real* flux_dn,
               int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bb_up, Real* bb_dn,
               int do_jac, const Real* sfc_src_jac, Real* flux_up_jac, void* stream,
-              const Real* bnd_lay = nullptr, const Real* bnd_lev = nullptr)
+              const Real* bnd_lay = nullptr, const Real* bnd_lev = nullptr, bool by_band = false,
+              const Real* plev = nullptr, Real* heating_rate = nullptr)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("lw_solver: empty extents");
+    if (heating_rate && (!plev || flux_up || flux_dn || do_jac))
+        return arg_error("lw_solver: heating rates need plev and summed (not g-point) flux outputs");
     const bool factored = bnd_lay != nullptr;
     if (factored && (!bnd_lev || !band_lims || flux_up || flux_dn || do_jac))
         return arg_error("lw_solver: factored sources need both band Planck arrays, band limits and summed outputs");
@@ -764,6 +877,7 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
     if (spectral && (bnd_up || bnd_dn || bb_up || bb_dn)) return arg_error("lw_solver: g-point and summed outputs are exclusive");
     if ((bnd_up || bnd_dn) && (!bnd_up || !bnd_dn || !band_lims)) return arg_error("lw_solver: band fluxes need both arrays and band limits");
     if (do_jac && (!sfc_src_jac || !flux_up_jac || spectral)) return arg_error("lw_solver: the Jacobian needs broadband output");
+    if (by_band && (!band_lims || nbnd < 1)) return arg_error("lw_solver: band surface inputs need band limits");
     hipStream_t s = as_stream(stream);
     Groups grp;
     RRTMGP_TRY(make_groups(ngpt, band_lims ? nbnd : 1, band_lims, grp));
@@ -773,6 +887,7 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
     const dim3 block(SB_COLS, SB_GPTS);
 
     Real *g_up = nullptr, *g_dn = nullptr, *g_jac = nullptr, *scratch = nullptr;
+    double* absorbed = nullptr;
     if (!spectral)
     {
         const int64_t nbuf = (direct ? 0 : 2) + (do_jac ? 1 : 0);
@@ -781,17 +896,30 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
         g_dn = direct ? bnd_dn : scratch + ncv * grp.n;
         if (do_jac) g_jac = scratch + ncv * grp.n * (direct ? 0 : 2);
     }
+    if (heating_rate)
+    {
+        // Per-group absorbed fluxes of the down and the up sweep, FP64, every element written.
+        const int rc0 = scratch_alloc(&absorbed, size_t(ncol * nlay * grp.n * 2), s);
+        if (rc0 != RRTMGP_OK) { scratch_free(scratch, s); return rc0; }
+    }
     int rc = RRTMGP_OK;
     {
         RRTMGP_TIMED("lw_solver_kernel", s);
         auto launch = [&](auto kernel, const Real* jac_src, Real* jac_out) {
             hipLaunchKernelGGL(kernel, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, int(nmus), secants,
                                weights, tau, lay, linc, ldec, emis, sfc_src, inc_flux, flux_up, flux_dn,
-                               jac_src, grp, g_up, g_dn, jac_out, bnd_lay, bnd_lev);
+                               jac_src, grp, g_up, g_dn, jac_out, bnd_lay, bnd_lev, by_band ? nbnd : int64_t(0),
+                               by_band ? 1 : 0, absorbed);
         };
         if (do_jac)
             launch(nmus == 1 ? lw_solver_kernel<Real, true, 1, false> : lw_solver_kernel<Real, true, 4, false>,
                    sfc_src_jac, g_jac);
+        else if (heating_rate && factored)
+            launch(nmus == 1 ? lw_solver_kernel<Real, false, 1, true, true> : lw_solver_kernel<Real, false, 4, true, true>,
+                   nullptr, nullptr);
+        else if (heating_rate)
+            launch(nmus == 1 ? lw_solver_kernel<Real, false, 1, false, true> : lw_solver_kernel<Real, false, 4, false, true>,
+                   nullptr, nullptr);
         else if (factored)
             launch(nmus == 1 ? lw_solver_kernel<Real, false, 1, true> : lw_solver_kernel<Real, false, 4, true>,
                    nullptr, nullptr);
@@ -806,6 +934,9 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
         if (rc == RRTMGP_OK) rc = group_sums<Real>(ncv, grp, g_dn, direct ? nullptr : bnd_dn, bb_dn, s);
         if (rc == RRTMGP_OK && do_jac) rc = group_sums<Real>(ncv, grp, g_jac, nullptr, flux_up_jac, s);
     }
+    if (rc == RRTMGP_OK && heating_rate)
+        rc = absorbed_heating_rate<Real>(ncol, nlay, grp.n, 2, absorbed, plev, heating_rate, s);
+    scratch_free(absorbed, s);
     scratch_free(scratch, s);
     return rc;
 }
@@ -815,9 +946,13 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
               const Real* g, const Real* mu0, const Real* alb_dir, const Real* alb_dif, const Real* inc_dir,
               Real* flux_up, Real* flux_dn, Real* flux_dir, int has_dif_bc, const Real* inc_dif,
               int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bnd_dir,
-              Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream)
+              Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream, bool by_band = false,
+              const Real* plev = nullptr, Real* heating_rate = nullptr)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("sw_solver: empty extents");
+    if (heating_rate && (!plev || flux_up || flux_dn || flux_dir))
+        return arg_error("sw_solver: heating rates need plev and summed (not g-point) flux outputs");
+    if (by_band && (!band_lims || nbnd < 1)) return arg_error("sw_solver: band surface inputs need band limits");
     // 32-bit lane offsets within one g-point's slab of levels or of checkpoints.
     if (ncol * std::max<int64_t>(nlay + 1, 3 * ((nlay + SwSeg<Real>::K - 1) / SwSeg<Real>::K)) *
             int64_t(sizeof(Real)) >= (int64_t(1) << 32))
@@ -841,11 +976,18 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     const dim3 block(SB_COLS, SB_GPTS);
 
     Real *ckpt = nullptr, *scratch = nullptr;
+    double* absorbed = nullptr;
     RRTMGP_TRY(scratch_alloc(&ckpt, size_t(nseg * 3 * ngpt * ncol), s));
     if (!spectral && !direct)
     {
         const int rc0 = scratch_alloc(&scratch, size_t(ncv * grp.n * 3), s);
         if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); return rc0; }
+    }
+    if (heating_rate)
+    {
+        // Per-group absorbed fluxes, FP64, every element written by the kernel.
+        const int rc0 = scratch_alloc(&absorbed, size_t(ncol * nlay * grp.n), s);
+        if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); scratch_free(scratch, s); return rc0; }
     }
     Real* gf[3] = {nullptr, nullptr, nullptr};
     Real* const bo[3] = {bnd_up, bnd_dn, bnd_dir};
@@ -855,13 +997,17 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     int rc = RRTMGP_OK;
     {
         RRTMGP_TIMED("sw_solver_kernel", s);
-        hipLaunchKernelGGL(sw_solver_kernel<Real>, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, tau, ssa, g,
-                           mu0, alb_dir, alb_dif, inc_dir, has_dif_bc ? inc_dif : nullptr, flux_up, flux_dn,
-                           flux_dir, ckpt, grp, gf[0], gf[1], gf[2]);
+        auto kernel = heating_rate ? sw_solver_kernel<Real, true> : sw_solver_kernel<Real, false>;
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir,
+                           has_dif_bc ? inc_dif : nullptr, flux_up, flux_dn, flux_dir, ckpt, grp, gf[0], gf[1], gf[2],
+                           by_band ? nbnd : int64_t(0), absorbed);
         rc = check_hip(hipGetLastError(), "sw_solver_kernel");
     }
     for (int c = 0; c < 3 && rc == RRTMGP_OK && !spectral; ++c)
         rc = group_sums<Real>(ncv, grp, gf[c], direct ? nullptr : bo[c], bbo[c], s);
+    if (rc == RRTMGP_OK && heating_rate)
+        rc = absorbed_heating_rate<Real>(ncol, nlay, grp.n, 1, absorbed, plev, heating_rate, s);
+    scratch_free(absorbed, s);
     scratch_free(ckpt, s);
     scratch_free(scratch, s);
     return rc;
@@ -889,24 +1035,26 @@ extern "C" int rrtmgp_hip_lw_solver_noscat_bnd_##S(int64_t ncol, int64_t nlay, i
     int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,                   \
     const T* lay_source, const T* lev_source_inc, const T* lev_source_dec, const T* sfc_emis,       \
     const T* sfc_src, const T* inc_flux, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,    \
-    T* bnd_flux_dn, T* broadband_up, T* broadband_dn, void* stream)                                 \
+    T* bnd_flux_dn, T* broadband_up, T* broadband_dn, const T* plev, T* heating_rate, void* stream)  \
 {                                                                                                    \
     return lw_solver<T>(ncol, nlay, ngpt, top_at_1, nmus, secants, weights, tau, lay_source,        \
                         lev_source_inc, lev_source_dec, sfc_emis, sfc_src, inc_flux, nullptr,        \
                         nullptr, nbnd, band_lims_gpt, bnd_flux_up, bnd_flux_dn, broadband_up,        \
-                        broadband_dn, 0, nullptr, nullptr, stream);                                  \
+                        broadband_dn, 0, nullptr, nullptr, stream, nullptr, nullptr, true, plev,     \
+                        heating_rate);                                                               \
 }                                                                                                    \
 extern "C" int rrtmgp_hip_lw_solver_noscat_bnd_factored_##S(int64_t ncol, int64_t nlay,           \
     int64_t ngpt, int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,     \
     const T* planck_frac, const T* band_lay_src, const T* band_lev_src, const T* sfc_emis,          \
     const T* sfc_src, const T* inc_flux, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,    \
-    T* bnd_flux_dn, T* broadband_up, T* broadband_dn, void* stream)                                 \
+    T* bnd_flux_dn, T* broadband_up, T* broadband_dn, const T* plev, T* heating_rate, void* stream)  \
 {                                                                                                    \
     if (!band_lay_src || !band_lev_src) return arg_error("lw_solver_factored: NULL band array");    \
     return lw_solver<T>(ncol, nlay, ngpt, top_at_1, nmus, secants, weights, tau, planck_frac,       \
                         nullptr, nullptr, sfc_emis, sfc_src, inc_flux, nullptr, nullptr, nbnd,       \
                         band_lims_gpt, bnd_flux_up, bnd_flux_dn, broadband_up, broadband_dn, 0,      \
-                        nullptr, nullptr, stream, band_lay_src, band_lev_src);                       \
+                        nullptr, nullptr, stream, band_lay_src, band_lev_src, true, plev,            \
+                        heating_rate);                                                               \
 }                                                                                                    \
 extern "C" int rrtmgp_hip_sw_solver_2stream_##S(int64_t ncol, int64_t nlay, int64_t ngpt,         \
     int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0, const T* sfc_alb_dir,       \
@@ -925,12 +1073,13 @@ extern "C" int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, 
     int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0, const T* sfc_alb_dir,       \
     const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc, const T* inc_flux_dif,             \
     int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up, T* bnd_flux_dn, T* bnd_flux_dir,        \
-    T* broadband_up, T* broadband_dn, T* broadband_dir, void* stream)                               \
+    T* broadband_up, T* broadband_dn, T* broadband_dir, const T* plev, T* heating_rate,             \
+    void* stream)                                                                                    \
 {                                                                                                    \
     return sw_solver<T>(ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, sfc_alb_dir, sfc_alb_dif,     \
                         inc_flux_dir, nullptr, nullptr, nullptr, has_dif_bc, inc_flux_dif, nbnd,     \
                         band_lims_gpt, bnd_flux_up, bnd_flux_dn, bnd_flux_dir, broadband_up,         \
-                        broadband_dn, broadband_dir, stream);                                        \
+                        broadband_dn, broadband_dir, stream, true, plev, heating_rate);              \
 }
 
 #define RRTMGP_DEF_SECANTS(S, T)                                                                    \

@@ -301,6 +301,13 @@ int rrtmgp_hip_set_device(int device)
     return RRTMGP_OK;
 }
 
+int rrtmgp_hip_get_device(int* device)
+{
+    if (!device) return arg_error("get_device: NULL output");
+    RRTMGP_HIP(hipGetDevice(device));
+    return RRTMGP_OK;
+}
+
 int rrtmgp_hip_device_name(char* buf, int len)
 {
     int dev = 0;
@@ -453,6 +460,97 @@ int count_outside(int64_t n, const Real* arr, Real lo, Real hi, int64_t* count, 
     return RRTMGP_OK;
 }
 }  // namespace rrtmgp_hip
+
+namespace rrtmgp_hip
+{
+// All range checks of one gas_optics call in one launch: blockIdx.y selects the array.  Block
+// (0, 0) also copies play(1,1) and play(1,nlay), which decide the surface layer.
+constexpr int MAX_CHECKED = 8;
+template<typename Real>
+struct CheckSet
+{
+    int n;
+    const Real* a[MAX_CHECKED];
+    int64_t size[MAX_CHECKED];
+    Real lo[MAX_CHECKED], hi[MAX_CHECKED];
+    int check_hi[MAX_CHECKED];
+};
+
+template<typename Real>
+__global__ void check_ranges_kernel(CheckSet<Real> c, const Real* __restrict__ play, int64_t ncol, int64_t nlay,
+                                    unsigned long long* __restrict__ counts, Real* __restrict__ ends)
+{
+    const int k = blockIdx.y;
+    const Real* a = c.a[k];
+    const Real lo = c.lo[k], hi = c.hi[k];
+    const int chk = c.check_hi[k];
+    unsigned long long local = 0;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < c.size[k]; i += int64_t(gridDim.x) * blockDim.x)
+    {
+        const Real v = a[i];
+        local += ((v >= lo) && (!chk || v <= hi)) ? 0ull : 1ull;
+    }
+    if (local) atomicAdd(counts + k, local);
+    if (blockIdx.x == 0 && k == 0 && threadIdx.x == 0 && play)
+    {
+        ends[0] = play[0];
+        ends[1] = play[ncol * (nlay - 1)];
+    }
+}
+
+template<typename Real>
+int check_ranges(int n, const Real* const* arrays, const int64_t* sizes, const Real* lo, const Real* hi,
+                 int64_t* counts, const Real* play, int64_t ncol, int64_t nlay, Real* play_ends, void* stream)
+{
+    if (n < 1 || n > MAX_CHECKED) return arg_error("check_ranges: 1 to 8 arrays");
+    if (!arrays || !sizes || !lo || !hi || !counts) return arg_error("check_ranges: NULL argument");
+    if (play && (!play_ends || ncol < 1 || nlay < 1)) return arg_error("check_ranges: play needs extents and output");
+    hipStream_t s = as_stream(stream);
+    CheckSet<Real> c{};
+    c.n = n;
+    int64_t most = 1;
+    for (int k = 0; k < n; ++k)
+    {
+        if (sizes[k] > 0 && !arrays[k]) return arg_error("check_ranges: NULL array");
+        c.a[k] = arrays[k]; c.size[k] = sizes[k]; c.lo[k] = lo[k]; c.hi[k] = hi[k];
+        c.check_hi[k] = (hi[k] == hi[k]) ? 1 : 0;
+        most = std::max(most, sizes[k]);
+    }
+    // One scratch block: n counters, then the two pressures.
+    unsigned long long* d = nullptr;
+    RRTMGP_TRY(scratch_alloc(&d, MAX_CHECKED + 2, s));
+    RRTMGP_HIP(hipMemsetAsync(d, 0, MAX_CHECKED * sizeof(unsigned long long), s));
+    Real* ends = reinterpret_cast<Real*>(d + MAX_CHECKED);
+    const unsigned bx = unsigned(std::min<int64_t>(1024, (most + 255) / 256));
+    {
+        RRTMGP_TIMED("check_ranges_kernel", s);
+        hipLaunchKernelGGL(check_ranges_kernel<Real>, dim3(bx, unsigned(n)), dim3(256), 0, s, c, play, ncol, nlay,
+                           d, ends);
+        RRTMGP_LAUNCHED("check_ranges_kernel");
+    }
+    unsigned long long h[MAX_CHECKED + 2];
+    RRTMGP_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, s));
+    RRTMGP_HIP(hipStreamSynchronize(s));
+    scratch_free(d, s);
+    for (int k = 0; k < n; ++k) counts[k] = int64_t(h[k]);
+    if (play) std::memcpy(play_ends, h + MAX_CHECKED, 2 * sizeof(Real));
+    return RRTMGP_OK;
+}
+}  // namespace rrtmgp_hip
+
+extern "C" int rrtmgp_hip_check_ranges_f32(int n, const float* const* arrays, const int64_t* sizes, const float* lo,
+                                           const float* hi, int64_t* counts, const float* play, int64_t ncol,
+                                           int64_t nlay, float* play_ends, void* stream)
+{
+    return rrtmgp_hip::check_ranges<float>(n, arrays, sizes, lo, hi, counts, play, ncol, nlay, play_ends, stream);
+}
+
+extern "C" int rrtmgp_hip_check_ranges_f64(int n, const double* const* arrays, const int64_t* sizes,
+                                           const double* lo, const double* hi, int64_t* counts, const double* play,
+                                           int64_t ncol, int64_t nlay, double* play_ends, void* stream)
+{
+    return rrtmgp_hip::check_ranges<double>(n, arrays, sizes, lo, hi, counts, play, ncol, nlay, play_ends, stream);
+}
 
 extern "C" int rrtmgp_hip_count_outside_f32(int64_t n, const float* arr, float lo, float hi, int64_t* count, void* stream)
 {

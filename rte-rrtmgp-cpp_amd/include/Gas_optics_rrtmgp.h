@@ -265,9 +265,11 @@ class Gas_optics_rrtmgp_gpu : public Gas_optics_gpu
         void upload();
         void set_solar_variability(const Float mg_index, const Float sb_index);
         void make_vmr_desc(const Gas_concs_gpu& gas_desc, int ncol, int nlay, rrtmgp_gas_vmr& out) const;
-        void check_inputs(const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev,
-                          const Array_gpu<Float,2>& tlay, const Array_gpu<Float,2>& col_dry,
-                          const Array_gpu<Float,2>* tlev, const Array_gpu<Float,1>* tsfc) const;
+        // The range checks of src/Gas_optics_rrtmgp.cpp:805-826 (when enabled) and the surface
+        // layer of :1333, from one device pass and one read-back; returns sfc_lay.
+        int check_inputs(const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev,
+                         const Array_gpu<Float,2>& tlay, const Array_gpu<Float,2>& col_dry,
+                         const Array_gpu<Float,2>* tlev, const Array_gpu<Float,1>* tsfc) const;
 };
 
 #endif

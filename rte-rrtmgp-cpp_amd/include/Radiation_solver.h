@@ -27,7 +27,9 @@ class Radiation_solver_longwave_gpu
 
         // Arrays are device arrays of the full domain: (ncol, nlay|nlev) profiles, t_sfc (ncol),
         // emis_sfc (nbnd, ncol); col_dry may be empty (computed from h2o); cloud arrays are used
-        // when switch_cloud_optics.  Outputs (ncol, nlev[, nbnd]).
+        // when switch_cloud_optics.  Outputs (ncol, nlev[, nbnd]).  lw_heating_rate (ncol, nlay),
+        // when given, receives heating rates [K/day] from the layers' net-flux convergence formed
+        // inside the solver and summed in FP64 (an addition: the reference has no heating rates).
         void solve_gpu(
                 const bool switch_cloud_optics, const bool switch_output_bnd_fluxes,
                 const Gas_concs_gpu& gas_concs,
@@ -40,7 +42,8 @@ class Radiation_solver_longwave_gpu
                 Array_gpu<Float,2>& lw_flux_up, Array_gpu<Float,2>& lw_flux_dn,
                 Array_gpu<Float,2>& lw_flux_net,
                 Array_gpu<Float,3>& lw_bnd_flux_up, Array_gpu<Float,3>& lw_bnd_flux_dn,
-                Array_gpu<Float,3>& lw_bnd_flux_net);
+                Array_gpu<Float,3>& lw_bnd_flux_net,
+                Array_gpu<Float,2>* lw_heating_rate = nullptr);
 
         void set_n_col_block(int n) { n_col_block = n; }
         int get_n_gpt() const { return kdist->get_ngpt(); }
@@ -58,6 +61,7 @@ class Radiation_solver_longwave_gpu
         std::unique_ptr<Source_func_lw_gpu> sources;
         Array_gpu<Float,2> blk_flux_up, blk_flux_dn;
         Array_gpu<Float,3> blk_bnd_up, blk_bnd_dn;
+        Array_gpu<Float,2> blk_hr;
         Array_gpu<Float,2> col_dry_ws;
 };
 
@@ -67,7 +71,8 @@ class Radiation_solver_shortwave_gpu
         Radiation_solver_shortwave_gpu(std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist,
                                        std::shared_ptr<Cloud_optics_gpu> cloud_optics);
 
-        // sfc_alb_dir/dif (nbnd, ncol), tsi_scaling and mu0 (ncol).
+        // sfc_alb_dir/dif (nbnd, ncol), tsi_scaling and mu0 (ncol); sw_heating_rate as for the
+        // longwave solver.
         void solve_gpu(
                 const bool switch_cloud_optics, const bool switch_output_bnd_fluxes,
                 const bool switch_delta_cloud,
@@ -82,7 +87,8 @@ class Radiation_solver_shortwave_gpu
                 Array_gpu<Float,2>& sw_flux_up, Array_gpu<Float,2>& sw_flux_dn,
                 Array_gpu<Float,2>& sw_flux_dn_dir, Array_gpu<Float,2>& sw_flux_net,
                 Array_gpu<Float,3>& sw_bnd_flux_up, Array_gpu<Float,3>& sw_bnd_flux_dn,
-                Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net);
+                Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net,
+                Array_gpu<Float,2>* sw_heating_rate = nullptr);
 
         void set_n_col_block(int n) { n_col_block = n; }
         int get_n_gpt() const { return kdist->get_ngpt(); }
@@ -101,6 +107,7 @@ class Radiation_solver_shortwave_gpu
         Array_gpu<Float,2> toa_src;
         Array_gpu<Float,2> blk_flux_up, blk_flux_dn, blk_flux_dir;
         Array_gpu<Float,3> blk_bnd_up, blk_bnd_dn, blk_bnd_dir;
+        Array_gpu<Float,2> blk_hr;
         Array_gpu<Float,2> col_dry_ws;
 };
 

@@ -322,6 +322,25 @@ def rte_sw(kd: KDist, top_at_1, tau, ssa, g, mu0, inc_flux_dir, sfc_alb_dir, sfc
     return up, dn, dr
 
 
+def rte_sw_bnd(kd: KDist, top_at_1, tau, ssa, g, mu0, inc_flux_dir, sfc_alb_dir, sfc_alb_dif, p_lev=None):
+    """Broadband SW fluxes summed inside the solver (rrtmgp_hip_sw_solver_2stream_bnd_*), with
+    albedos (nbnd, ncol); given p_lev (ncol, nlev), also heating rates (ncol, nlay) [K/day] from the
+    layers' absorbed flux summed over g-points in FP64.  Returns a dict of DeviceArrays."""
+    dt = real_dtype(kd.precision)
+    ncol, nlay, ng = tau.shape
+    mu0, inc, ad, af, pl = (_dev(a, dt) for a in (mu0, inc_flux_dir, sfc_alb_dir, sfc_alb_dif, p_lev))
+    o = {k: DeviceArray((ncol, nlay + 1), dt) for k in ("flux_up", "flux_dn", "flux_dn_dir")}
+    o["heating_rate"] = DeviceArray((ncol, nlay), dt) if pl is not None else None
+    fn = getattr(_lib.hip(), "rrtmgp_hip_sw_solver_2stream_bnd_" + ("f64" if kd.precision == 8 else "f32"))
+    bl = kd.band_lims
+    i = C.c_int64
+    _lib.check_hip(fn(i(ncol), i(nlay), i(ng), C.c_int(int(top_at_1)), tau.ptr, ssa.ptr, g.ptr, mu0.ptr, ad.ptr,
+                      af.ptr, inc.ptr, C.c_int(0), None, i(kd.nbnd), bl.ctypes.data_as(C.POINTER(C.c_int)), None,
+                      None, None, o["flux_up"].ptr, o["flux_dn"].ptr, o["flux_dn_dir"].ptr, _p(pl),
+                      _p(o["heating_rate"]), None))
+    return {k: v for k, v in o.items() if v is not None}
+
+
 def fluxes_reduce(kd: KDist, gpt_up, gpt_dn, gpt_dir=None, byband=False):
     dt = real_dtype(kd.precision)
     ncol, nlev, _ = gpt_up.shape
@@ -392,7 +411,9 @@ class SolverLW(_Solver):
         self.h = h
         self.out = None
 
-    def solve(self, da: DeviceAtmosphere, bnd=False, col_dry=None, outputs=None):
+    def solve(self, da: DeviceAtmosphere, bnd=False, col_dry=None, outputs=None, heating_rate=False):
+        """Broadband (and with bnd, band) fluxes; with heating_rate, also o["heating_rate"]
+        (ncol, nlay) [K/day] from the solver's FP64 sums of the layers' net-flux convergence."""
         dt = real_dtype(self.precision)
         ncol, nlay = da.ncol, da.nlay
         nlev = nlay + 1
@@ -401,6 +422,8 @@ class SolverLW(_Solver):
             o = {k: DeviceArray((ncol, nlev), dt) for k in ("flux_up", "flux_dn", "flux_net")}
             for k in ("bnd_flux_up", "bnd_flux_dn", "bnd_flux_net"):
                 o[k] = DeviceArray((ncol, nlev, self.kd.nbnd), dt) if bnd else None
+        if heating_rate and o.get("heating_rate") is None:
+            o["heating_rate"] = DeviceArray((ncol, nlay), dt)
         cloudy = da.cloudy and self.cloud is not None
         self._check(self.L.rrtmgp_api_solve_lw(
             self.h, ncol, nlay, int(cloudy), int(bnd), da.p_lay.ptr, da.p_lev.ptr, da.t_lay.ptr,
@@ -408,7 +431,7 @@ class SolverLW(_Solver):
             _p(da.lwp) if cloudy else None, _p(da.iwp) if cloudy else None,
             _p(da.rel) if cloudy else None, _p(da.rei) if cloudy else None,
             o["flux_up"].ptr, o["flux_dn"].ptr, o["flux_net"].ptr, _p(o.get("bnd_flux_up")),
-            _p(o.get("bnd_flux_dn")), _p(o.get("bnd_flux_net"))))
+            _p(o.get("bnd_flux_dn")), _p(o.get("bnd_flux_net")), _p(o.get("heating_rate"))))
         return o
 
 
@@ -445,7 +468,9 @@ class SolverSW(_Solver):
         self._check(self.L.rrtmgp_api_solver_sw_create(kd.h, cloud.h if cloud else None, C.byref(h)))
         self.h = h
 
-    def solve(self, da: DeviceAtmosphere, tsi_scaling, bnd=False, delta_cloud=True, col_dry=None, outputs=None):
+    def solve(self, da: DeviceAtmosphere, tsi_scaling, bnd=False, delta_cloud=True, col_dry=None, outputs=None,
+              heating_rate=False):
+        """As SolverLW.solve, with the direct flux; heating_rate as there."""
         dt = real_dtype(self.precision)
         ncol, nlay = da.ncol, da.nlay
         nlev = nlay + 1
@@ -455,6 +480,8 @@ class SolverSW(_Solver):
             o = {k: DeviceArray((ncol, nlev), dt) for k in ("flux_up", "flux_dn", "flux_dn_dir", "flux_net")}
             for k in ("bnd_flux_up", "bnd_flux_dn", "bnd_flux_dn_dir", "bnd_flux_net"):
                 o[k] = DeviceArray((ncol, nlev, self.kd.nbnd), dt) if bnd else None
+        if heating_rate and o.get("heating_rate") is None:
+            o["heating_rate"] = DeviceArray((ncol, nlay), dt)
         cloudy = da.cloudy and self.cloud is not None
         self._check(self.L.rrtmgp_api_solve_sw(
             self.h, ncol, nlay, int(cloudy), int(bnd), int(delta_cloud), da.p_lay.ptr, da.p_lev.ptr,
@@ -463,7 +490,7 @@ class SolverSW(_Solver):
             _p(da.rel) if cloudy else None, _p(da.rei) if cloudy else None,
             o["flux_up"].ptr, o["flux_dn"].ptr, o["flux_dn_dir"].ptr, o["flux_net"].ptr,
             _p(o.get("bnd_flux_up")), _p(o.get("bnd_flux_dn")), _p(o.get("bnd_flux_dn_dir")),
-            _p(o.get("bnd_flux_net"))))
+            _p(o.get("bnd_flux_net")), _p(o.get("heating_rate"))))
         return o
 
     def adjoint(self, da: DeviceAtmosphere, tsi_scaling, g_flux_up, g_flux_dn, g_flux_dir, delta_cloud=True,

@@ -496,23 +496,29 @@ void Gas_optics_rrtmgp_gpu::make_vmr_desc(const Gas_concs_gpu& gas_desc, int nco
 }
 
 
-void Gas_optics_rrtmgp_gpu::check_inputs(const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev,
-                                         const Array_gpu<Float,2>& tlay, const Array_gpu<Float,2>& col_dry,
-                                         const Array_gpu<Float,2>* tlev, const Array_gpu<Float,1>* tsfc) const
+int Gas_optics_rrtmgp_gpu::check_inputs(const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev,
+                                        const Array_gpu<Float,2>& tlay, const Array_gpu<Float,2>& col_dry,
+                                        const Array_gpu<Float,2>* tlev, const Array_gpu<Float,1>* tsfc) const
 {
-    if (!check_ranges) return;
-    auto outside = [](const auto& a, Float lo, Float hi) {
-        int64_t n = 0;
-        hip_check(K<Float>::count_outside(a.size(), a.ptr(), lo, hi, &n, stream()));
-        return n > 0;
-    };
     const Float nan = std::nan("");
-    if (outside(play, press_ref_min, press_ref_max)) throw std::range_error("play is out of range");
-    if (outside(plev, press_ref_min, press_ref_max)) throw std::range_error("plev is out of range");
-    if (outside(tlay, temp_ref_min, temp_ref_max)) throw std::range_error("tlay is out of range");
-    if (tlev && outside(*tlev, temp_ref_min, temp_ref_max)) throw std::range_error("tlev is out of range");
-    if (tsfc && outside(*tsfc, temp_ref_min, temp_ref_max)) throw std::range_error("tsfc is out of range");
-    if (outside(col_dry, Float(0.), nan)) throw std::range_error("col_dry is out of range");
+    // In the reference's order of the checks (and of its messages).
+    const char* what[6] = {"play", "plev", "tlay", "tlev", "tsfc", "col_dry"};
+    const Float* arrays[6] = {play.ptr(), plev.ptr(), tlay.ptr(), tlev ? tlev->ptr() : nullptr,
+                              tsfc ? tsfc->ptr() : nullptr, col_dry.ptr()};
+    int64_t sizes[6] = {play.size(), plev.size(), tlay.size(), tlev ? tlev->size() : 0,
+                        tsfc ? tsfc->size() : 0, col_dry.size()};
+    const Float lo[6] = {press_ref_min, press_ref_min, temp_ref_min, temp_ref_min, temp_ref_min, Float(0.)};
+    const Float hi[6] = {press_ref_max, press_ref_max, temp_ref_max, temp_ref_max, temp_ref_max, nan};
+    if (!check_ranges)
+        for (auto& n : sizes) n = 0;
+    int64_t counts[6] = {0, 0, 0, 0, 0, 0};
+    Float ends[2] = {Float(0.), Float(0.)};
+    const int ncol = play.dim(1), nlay = play.dim(2);
+    hip_check(K<Float>::check_ranges(6, arrays, sizes, lo, hi, counts, play.ptr(), ncol, nlay, ends, stream()));
+    for (int k = 0; k < 6; ++k)
+        if (counts[k] > 0) throw std::range_error(std::string(what[k]) + " is out of range");
+    // Surface layer from the first column (src/Gas_optics_rrtmgp.cpp:1333).
+    return ends[0] > ends[1] ? 1 : nlay;
 }
 
 
@@ -543,11 +549,9 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
 {
     if (!source_is_internal()) throw std::runtime_error("longwave gas_optics called on a shortwave k-distribution");
     const int ncol = play.dim(1), nlay = play.dim(2);
-    check_inputs(play, plev, tlay, col_dry, &tlev, &tsfc);
+    const int sfc_lay = check_inputs(play, plev, tlay, col_dry, &tlev, &tsfc);
     rrtmgp_gas_vmr v;
     make_vmr_desc(gas_desc, ncol, nlay, v);
-    // Surface layer from the first column (src/Gas_optics_rrtmgp.cpp:1333).
-    const int sfc_lay = play({1, 1}) > play({1, nlay}) ? 1 : nlay;
     if (cloud && cloud->get_ngpt() != get_nband())
         throw std::runtime_error("Cannot add optical properties with incompatible band - gpoint combination");
     if (sources.is_factored())
@@ -615,7 +619,7 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
 {
     if (krayl.size() == 0) throw std::runtime_error("shortwave gas_optics called on a longwave k-distribution");
     const int ncol = play.dim(1), nlay = play.dim(2);
-    check_inputs(play, plev, tlay, col_dry, nullptr, nullptr);
+    (void)check_inputs(play, plev, tlay, col_dry, nullptr, nullptr);
     rrtmgp_gas_vmr v;
     make_vmr_desc(gas_desc, ncol, nlay, v);
     if (cloud && cloud->get_ngpt() != get_nband())

@@ -59,8 +59,9 @@ void Radiation_solver_longwave_gpu::solve_gpu(
         const Array_gpu<Float,2>& rel, const Array_gpu<Float,2>& rei,
         Array_gpu<Float,2>& lw_flux_up, Array_gpu<Float,2>& lw_flux_dn, Array_gpu<Float,2>& lw_flux_net,
         Array_gpu<Float,3>& lw_bnd_flux_up, Array_gpu<Float,3>& lw_bnd_flux_dn,
-        Array_gpu<Float,3>& lw_bnd_flux_net)
+        Array_gpu<Float,3>& lw_bnd_flux_net, Array_gpu<Float,2>* lw_heating_rate)
 {
+    const bool hr = lw_heating_rate != nullptr && lw_heating_rate->size() > 0;
     const int ncol = p_lay.dim(1), nlay = p_lay.dim(2), nlev = nlay + 1;
     const int ngpt = kdist->get_ngpt(), nbnd = kdist->get_nband();
     if (switch_cloud_optics && !cloud_optics) throw std::runtime_error("cloud optics requested but not configured");
@@ -109,10 +110,12 @@ void Radiation_solver_longwave_gpu::solve_gpu(
                 bu = Array_gpu<Float,3>({n, nlev, nbnd}); bd = Array_gpu<Float,3>({n, nlev, nbnd});
                 bn = Array_gpu<Float,3>({n, nlev, nbnd});
             }
+            Array_gpu<Float,2> th = hr ? Array_gpu<Float,2>({n, nlay}) : Array_gpu<Float,2>();
             tail.solve_gpu(switch_cloud_optics, switch_output_bnd_fluxes, gsub, pl, pv, tl, tv, cd, ts, em,
-                           a, b, c, d, fu, fd, fn, bu, bd, bn);
+                           a, b, c, d, fu, fd, fn, bu, bd, bn, hr ? &th : nullptr);
             scatter(fu.ptr(), n, nlev, lw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, lw_flux_dn.ptr(), ncol, c0);
+            if (hr) scatter(th.ptr(), n, nlay, lw_heating_rate->ptr(), ncol, c0);
             if (switch_output_bnd_fluxes)
             {
                 scatter(bu.ptr(), n, int64_t(nlev) * nbnd, lw_bnd_flux_up.ptr(), ncol, c0);
@@ -155,11 +158,18 @@ void Radiation_solver_longwave_gpu::solve_gpu(
         Array_gpu<Float,3> no_bnd;
         Array_gpu<Float,3>& bu = switch_output_bnd_fluxes ? (whole ? lw_bnd_flux_up : blk_bnd_up) : no_bnd;
         Array_gpu<Float,3>& bd = switch_output_bnd_fluxes ? (whole ? lw_bnd_flux_dn : blk_bnd_dn) : no_bnd;
-        rte.rte_lw_fluxes(optical_props, top_at_1, *sources, em, Array_gpu<Float,2>(), bu, bd, fu, fd, 1);
+        Array_gpu<Float,2>* hb = nullptr;
+        if (hr)
+        {
+            if (!whole && (blk_hr.dim(1) != n || blk_hr.dim(2) != nlay)) blk_hr = Array_gpu<Float,2>({n, nlay});
+            hb = whole ? lw_heating_rate : &blk_hr;
+        }
+        rte.rte_lw_fluxes(optical_props, top_at_1, *sources, em, Array_gpu<Float,2>(), bu, bd, fu, fd, 1, &pv, hb);
         if (!whole)
         {
             scatter(fu.ptr(), n, nlev, lw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, lw_flux_dn.ptr(), ncol, c0);
+            if (hr) scatter(blk_hr.ptr(), n, nlay, lw_heating_rate->ptr(), ncol, c0);
             if (switch_output_bnd_fluxes)
             {
                 const int64_t rows = int64_t(nlev) * nbnd;
@@ -193,8 +203,10 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         Array_gpu<Float,2>& sw_flux_up, Array_gpu<Float,2>& sw_flux_dn,
         Array_gpu<Float,2>& sw_flux_dn_dir, Array_gpu<Float,2>& sw_flux_net,
         Array_gpu<Float,3>& sw_bnd_flux_up, Array_gpu<Float,3>& sw_bnd_flux_dn,
-        Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net)
+        Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net,
+        Array_gpu<Float,2>* sw_heating_rate)
 {
+    const bool hr = sw_heating_rate != nullptr && sw_heating_rate->size() > 0;
     const int ncol = p_lay.dim(1), nlay = p_lay.dim(2), nlev = nlay + 1;
     const int ngpt = kdist->get_ngpt(), nbnd = kdist->get_nband();
     if (switch_cloud_optics && !cloud_optics) throw std::runtime_error("cloud optics requested but not configured");
@@ -243,8 +255,10 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
                 bu = Array_gpu<Float,3>({n, nlev, nbnd}); bd = Array_gpu<Float,3>({n, nlev, nbnd});
                 bdir = Array_gpu<Float,3>({n, nlev, nbnd}); bn = Array_gpu<Float,3>({n, nlev, nbnd});
             }
+            Array_gpu<Float,2> th = hr ? Array_gpu<Float,2>({n, nlay}) : Array_gpu<Float,2>();
             tail.solve_gpu(switch_cloud_optics, switch_output_bnd_fluxes, switch_delta_cloud, gsub, pl, pv, tl, tv,
-                           cd, ad, af, tsi, m0, a, b, c, d, fu, fd, fdir, fn, bu, bd, bdir, bn);
+                           cd, ad, af, tsi, m0, a, b, c, d, fu, fd, fdir, fn, bu, bd, bdir, bn, hr ? &th : nullptr);
+            if (hr) scatter(th.ptr(), n, nlay, sw_heating_rate->ptr(), ncol, c0);
             scatter(fu.ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
             scatter(fdir.ptr(), n, nlev, sw_flux_dn_dir.ptr(), ncol, c0);
@@ -298,9 +312,17 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         Array_gpu<Float,3>& bu = switch_output_bnd_fluxes ? (whole ? sw_bnd_flux_up : blk_bnd_up) : no_bnd;
         Array_gpu<Float,3>& bd = switch_output_bnd_fluxes ? (whole ? sw_bnd_flux_dn : blk_bnd_dn) : no_bnd;
         Array_gpu<Float,3>& br = switch_output_bnd_fluxes ? (whole ? sw_bnd_flux_dn_dir : blk_bnd_dir) : no_bnd;
-        rte.rte_sw_fluxes(optical_props, top_at_1, m0, toa_src, ad, af, Array_gpu<Float,2>(), bu, bd, br, fu, fd, fr);
+        Array_gpu<Float,2>* hb = nullptr;
+        if (hr)
+        {
+            if (!whole && (blk_hr.dim(1) != n || blk_hr.dim(2) != nlay)) blk_hr = Array_gpu<Float,2>({n, nlay});
+            hb = whole ? sw_heating_rate : &blk_hr;
+        }
+        rte.rte_sw_fluxes(optical_props, top_at_1, m0, toa_src, ad, af, Array_gpu<Float,2>(), bu, bd, br, fu, fd, fr,
+                          &pv, hb);
         if (!whole)
         {
+            if (hr) scatter(blk_hr.ptr(), n, nlay, sw_heating_rate->ptr(), ncol, c0);
             scatter(fu.ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
             scatter(fr.ptr(), n, nlev, sw_flux_dn_dir.ptr(), ncol, c0);

@@ -1,6 +1,9 @@
 // Longwave driver (reference src_cuda/Rte_lw.cu:81-152, CPU src/Rte_lw.cpp:90-162).
 #include "Rte_lw.h"
 
+#include <map>
+#include <mutex>
+
 #include "Optical_props.h"
 #include "Source_functions.h"
 #include "hip_dispatch.h"
@@ -23,6 +26,32 @@ const Float gauss_wts[4][4] = {{0.5, 0., 0., 0.},
 
 namespace
 {
+// The Gauss tables on the device, [secants (4 x 4) | weights (4 x 4)], row n-1 holding the
+// n-point quadrature: uploaded once per device and kept for the life of the process (32 values),
+// so that a solve neither allocates nor copies them.
+const Float* gauss_tables_device()
+{
+    static std::mutex m;
+    static std::map<int, Float*> tables;
+    int dev = 0;
+    hip_check(rrtmgp_hip_get_device(&dev));
+    std::lock_guard<std::mutex> lock(m);
+    auto it = tables.find(dev);
+    if (it != tables.end()) return it->second;
+    Float h[32];
+    for (int n = 0; n < 4; ++n)
+        for (int i = 0; i < 4; ++i)
+        {
+            h[4 * n + i] = gauss_Ds[n][i];
+            h[16 + 4 * n + i] = gauss_wts[n][i];
+        }
+    void* p = nullptr;
+    hip_check(rrtmgp_hip_malloc(&p, sizeof(h)));
+    hip_check(rrtmgp_hip_memcpy_h2d(p, h, sizeof(h), nullptr));
+    tables[dev] = static_cast<Float*>(p);
+    return static_cast<Float*>(p);
+}
+
 // Per-call solver inputs: surface emissivity by g-point, secants (ncol, ngpt, nmus) and weights.
 struct LwPrep
 {
@@ -97,35 +126,49 @@ void Rte_lw_gpu::rte_lw_fluxes(
         const Array_gpu<Float,2>& inc_flux,
         Array_gpu<Float,3>& bnd_flux_up, Array_gpu<Float,3>& bnd_flux_dn,
         Array_gpu<Float,2>& flux_up, Array_gpu<Float,2>& flux_dn,
-        const int n_gauss_angles)
+        const int n_gauss_angles,
+        const Array_gpu<Float,2>* p_lev, Array_gpu<Float,2>* heating_rate)
 {
     const int ncol = optical_props->get_ncol();
     const int nlay = optical_props->get_nlay();
     const int ngpt = optical_props->get_ngpt();
     const int nbnd = optical_props->get_nband();
-    const LwPrep p = lw_prep(*this, optical_props, sfc_emis, n_gauss_angles);
+    const bool hr = heating_rate != nullptr && heating_rate->size() > 0;
+    if (hr && (!p_lev || p_lev->dim(1) != ncol || p_lev->dim(2) != nlay + 1 || heating_rate->dim(1) != ncol ||
+               heating_rate->dim(2) != nlay))
+        throw std::runtime_error("rte_lw: heating rates need p_lev (ncol, nlev) and heating_rate (ncol, nlay)");
+    const Float* plev = hr ? p_lev->ptr() : nullptr;
+    Float* hrp = hr ? heating_rate->ptr() : nullptr;
+    if (n_gauss_angles < 1 || n_gauss_angles > 4)
+        throw std::runtime_error("n_gauss_angles must be between 1 and 4");
+    if (sfc_emis.dim(1) != nbnd || sfc_emis.dim(2) != ncol)
+        throw std::runtime_error("rte_lw: sfc_emis must be (nband, ncol)");
+    // The band-summed solvers read the emissivity per band and one secant per angle.
+    const Float* gt = gauss_tables_device();
+    const Float* secants = gt + 4 * (n_gauss_angles - 1);
+    const Float* weights = gt + 16 + 4 * (n_gauss_angles - 1);
     const Float* inc = (inc_flux.size() == 0) ? nullptr : inc_flux.ptr();
     const bool bnd = bnd_flux_up.size() > 0;
     const bool bb = flux_up.size() > 0;
     if (sources.is_factored())
     {
         hip_check(K<Float>::lw_solver_noscat_bnd_factored(
-                ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
+                ncol, nlay, ngpt, top_at_1, n_gauss_angles, secants, weights,
                 optical_props->get_tau().ptr(), sources.get_planck_frac().ptr(), sources.get_band_lay_source().ptr(),
-                sources.get_band_lev_source().ptr(), p.sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
+                sources.get_band_lev_source().ptr(), sfc_emis.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
                 optical_props->get_band_lims_gpoint().ptr(), bnd ? bnd_flux_up.ptr() : nullptr,
                 bnd ? bnd_flux_dn.ptr() : nullptr, bb ? flux_up.ptr() : nullptr, bb ? flux_dn.ptr() : nullptr,
-                stream()));
+                plev, hrp, stream()));
         return;
     }
     hip_check(K<Float>::lw_solver_noscat_bnd(
-            ncol, nlay, ngpt, top_at_1, n_gauss_angles, p.secants.ptr(), p.weights.ptr(),
+            ncol, nlay, ngpt, top_at_1, n_gauss_angles, secants, weights,
             optical_props->get_tau().ptr(), sources.get_lay_source().ptr(),
             sources.get_lev_source_inc().ptr(), sources.get_lev_source_dec().ptr(),
-            p.sfc_emis_gpt.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
+            sfc_emis.ptr(), sources.get_sfc_source().ptr(), inc, nbnd,
             optical_props->get_band_lims_gpoint().ptr(), bnd ? bnd_flux_up.ptr() : nullptr,
             bnd ? bnd_flux_dn.ptr() : nullptr, bb ? flux_up.ptr() : nullptr,
-            bb ? flux_dn.ptr() : nullptr, stream()));
+            bb ? flux_dn.ptr() : nullptr, plev, hrp, stream()));
 }
 
 void Rte_lw_gpu::expand_and_transpose(

@@ -580,11 +580,12 @@ int rrtmgp_api_solve_lw(void* h, int ncol, int nlay, int cloudy, int bnd, const 
                         const rrtmgp_real* emis_sfc, const rrtmgp_real* lwp, const rrtmgp_real* iwp,
                         const rrtmgp_real* rel, const rrtmgp_real* rei, rrtmgp_real* flux_up,
                         rrtmgp_real* flux_dn, rrtmgp_real* flux_net, rrtmgp_real* bnd_up,
-                        rrtmgp_real* bnd_dn, rrtmgp_real* bnd_net)
+                        rrtmgp_real* bnd_dn, rrtmgp_real* bnd_net, rrtmgp_real* heating_rate)
 {
     return guarded([&] {
         auto* s = static_cast<Solver*>(h);
         if (!s->lw) throw std::invalid_argument("not a longwave solver");
+        auto hr = heating_rate ? view<2>(heating_rate, {ncol, nlay}) : Array_gpu<Float,2>();
         const int nlev = nlay + 1, nbnd = s->lw->get_n_bnd();
         auto fu = view<2>(flux_up, {ncol, nlev}), fd = view<2>(flux_dn, {ncol, nlev}), fn = view<2>(flux_net, {ncol, nlev});
         auto bu = bnd ? view<3>(bnd_up, {ncol, nlev, nbnd}) : Array_gpu<Float,3>();
@@ -598,7 +599,7 @@ int rrtmgp_api_solve_lw(void* h, int ncol, int nlay, int cloudy, int bnd, const 
                          cloudy ? view<2>(iwp, {ncol, nlay}) : Array_gpu<Float,2>(),
                          cloudy ? view<2>(rel, {ncol, nlay}) : Array_gpu<Float,2>(),
                          cloudy ? view<2>(rei, {ncol, nlay}) : Array_gpu<Float,2>(),
-                         fu, fd, fn, bu, bd, bn);
+                         fu, fd, fn, bu, bd, bn, heating_rate ? &hr : nullptr);
     });
 }
 
@@ -609,11 +610,13 @@ int rrtmgp_api_solve_sw(void* h, int ncol, int nlay, int cloudy, int bnd, int de
                         const rrtmgp_real* tsi_scaling, const rrtmgp_real* mu0, const rrtmgp_real* lwp,
                         const rrtmgp_real* iwp, const rrtmgp_real* rel, const rrtmgp_real* rei,
                         rrtmgp_real* flux_up, rrtmgp_real* flux_dn, rrtmgp_real* flux_dir, rrtmgp_real* flux_net,
-                        rrtmgp_real* bnd_up, rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir, rrtmgp_real* bnd_net)
+                        rrtmgp_real* bnd_up, rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir, rrtmgp_real* bnd_net,
+                        rrtmgp_real* heating_rate)
 {
     return guarded([&] {
         auto* s = static_cast<Solver*>(h);
         if (!s->sw) throw std::invalid_argument("not a shortwave solver");
+        auto hr = heating_rate ? view<2>(heating_rate, {ncol, nlay}) : Array_gpu<Float,2>();
         const int nlev = nlay + 1, nbnd = s->sw->get_n_bnd();
         auto fu = view<2>(flux_up, {ncol, nlev}), fd = view<2>(flux_dn, {ncol, nlev});
         auto fdir = view<2>(flux_dir, {ncol, nlev}), fn = view<2>(flux_net, {ncol, nlev});
@@ -630,7 +633,7 @@ int rrtmgp_api_solve_sw(void* h, int ncol, int nlay, int cloudy, int bnd, int de
                          cloudy ? view<2>(iwp, {ncol, nlay}) : Array_gpu<Float,2>(),
                          cloudy ? view<2>(rel, {ncol, nlay}) : Array_gpu<Float,2>(),
                          cloudy ? view<2>(rei, {ncol, nlay}) : Array_gpu<Float,2>(),
-                         fu, fd, fdir, fn, bu, bd, bdir, bn);
+                         fu, fd, fdir, fn, bu, bd, bdir, bn, heating_rate ? &hr : nullptr);
     });
 }
 

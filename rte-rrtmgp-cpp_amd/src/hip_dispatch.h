@@ -48,6 +48,7 @@ template<typename T> struct K;
     static constexpr auto net_byband_full = rrtmgp_hip_net_byband_full_##S; \
     static constexpr auto heating_rate = rrtmgp_hip_heating_rate_##S;    \
     static constexpr auto count_outside = rrtmgp_hip_count_outside_##S;  \
+    static constexpr auto check_ranges = rrtmgp_hip_check_ranges_##S;    \
     static constexpr auto spread_col = rrtmgp_hip_spread_col_##S;        \
     static constexpr auto scale_by_col = rrtmgp_hip_scale_by_col_##S;    \
     static constexpr auto copy_cols = rrtmgp_hip_copy_cols_##S;

@@ -404,15 +404,19 @@ def test_lw_solver_factored_entry(G, top_at_1, nmu, prec):
     lay = pf * blay[:, :, band_of]
     linc = pf * blev[:, 1:, band_of]
     ldec = pf * blev[:, :-1, band_of]
-    emis = rng.uniform(0.8, 1, (ncol, ngpt)).astype(dt)
+    # The band-summed entries take emissivity per (band, column) and one secant per angle; the
+    # reference-ABI entry takes them expanded to (column, g-point[, angle]).
+    emis_b = rng.uniform(0.8, 1, (nbnd, ncol)).astype(dt)
+    emis = np.ascontiguousarray(emis_b[band_of, :].T)
     sfc = rng.uniform(10, 100, (ncol, ngpt)).astype(dt)
+    sec1 = (1.5 + np.arange(nmu)).astype(dt)
     sec = np.empty((ncol, ngpt, nmu), dt)
     for mu in range(nmu):
-        sec[:, :, mu] = 1.5 + mu
+        sec[:, :, mu] = sec1[mu]
     w = np.full(nmu, 1.0 / nmu, dt)
     dev = lambda a: G.DeviceArray.from_host(np.asfortranarray(a), dt)         # noqa: E731
     d = {k: dev(v) for k, v in dict(tau=tau, pf=pf, blay=blay, blev=blev, lay=lay, linc=linc, ldec=ldec,
-                                    emis=emis, sfc=sfc, sec=sec, w=w).items()}
+                                    emis=emis, emis_b=emis_b, sfc=sfc, sec=sec, sec1=sec1, w=w).items()}
     out = {k: G.DeviceArray(sh, dt) for k, sh in (("bu", (ncol, nlay + 1, nbnd)), ("bd", (ncol, nlay + 1, nbnd)),
                                                   ("u", (ncol, nlay + 1)), ("dn", (ncol, nlay + 1)))}
     out2 = {k: G.DeviceArray(v.shape, dt) for k, v in out.items()}
@@ -421,15 +425,24 @@ def test_lw_solver_factored_entry(G, top_at_1, nmu, prec):
     blp = bl.ctypes.data_as(C.POINTER(C.c_int))
     i = C.c_int64
     rc = getattr(L, "rrtmgp_hip_lw_solver_noscat_bnd_" + S)(
-        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
-        d["lay"].ptr, d["linc"].ptr, d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr, None, i(nbnd), blp,
-        out["bu"].ptr, out["bd"].ptr, out["u"].ptr, out["dn"].ptr, None)
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec1"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["lay"].ptr, d["linc"].ptr, d["ldec"].ptr, d["emis_b"].ptr, d["sfc"].ptr, None, i(nbnd), blp,
+        out["bu"].ptr, out["bd"].ptr, out["u"].ptr, out["dn"].ptr, None, None, None)
     assert rc == 0
     rc = getattr(L, "rrtmgp_hip_lw_solver_noscat_bnd_factored_" + S)(
-        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
-        d["pf"].ptr, d["blay"].ptr, d["blev"].ptr, d["emis"].ptr, d["sfc"].ptr, None, i(nbnd), blp,
-        out2["bu"].ptr, out2["bd"].ptr, out2["u"].ptr, out2["dn"].ptr, None)
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec1"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["pf"].ptr, d["blay"].ptr, d["blev"].ptr, d["emis_b"].ptr, d["sfc"].ptr, None, i(nbnd), blp,
+        out2["bu"].ptr, out2["bd"].ptr, out2["u"].ptr, out2["dn"].ptr, None, None, None)
     assert rc == 0
+    # The reference-ABI entry on the expanded emissivity and secants, summed to broadband.
+    ref_u, ref_d = G.DeviceArray((ncol, nlay + 1), dt), G.DeviceArray((ncol, nlay + 1), dt)
+    rc = getattr(L, "rrtmgp_hip_lw_solver_noscat_" + S)(
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["lay"].ptr, d["linc"].ptr, d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr, None, None, None,
+        C.c_int(1), ref_u.ptr, ref_d.ptr, C.c_int(0), None, None, None)
+    assert rc == 0
+    for a, b in ((out["u"], ref_u), (out["dn"], ref_d)):
+        assert rel_err(_host(a), _host(b), floor=1.0) < (1e-13 if prec == 8 else 1e-6)
     for k in out:
         a, b = _host(out2[k]), _host(out[k])
         if prec == 8:
