@@ -281,6 +281,7 @@ def precision_study(ncol, nlay, block, ocols, threads):
         cloud = api.Cloud(ls, prec)
         scale = atm.tsi / kd.scalar("tsi")
         sw = {k: np.empty((ncol, nlay + 1)) for k in ("up", "dn", "dir")}
+        sw["hr"] = np.empty((ncol, nlay))
         for c0 in range(0, ncol, block):
             sub = syn.subset_columns(atm, slice(c0, min(ncol, c0 + block)))
             n = sub.ncol
@@ -297,16 +298,17 @@ def precision_study(ncol, nlay, block, ocols, threads):
             api.delta_scale(ct, cs, cg, prec)
             api.add_to(kd, go["tau"], ct, go["ssa"], go["g"], cs, cg)
             toa = device.DeviceArray.from_host(go["toa_src"].to_host() * scale, dt)
-            up, dn, dr = api.rte_sw(kd, sub.top_at_1, go["tau"], go["ssa"], go["g"], sub.mu0, toa, sub.sfc_alb,
-                                    0.5 * sub.sfc_alb, do_broadband=True)
-            for k, v in (("up", up), ("dn", dn), ("dir", dr)):
-                sw[k][c0:c0 + n] = v.to_host()
-            del go, ct, cs, cg, up, dn, dr, toa
+            o = api.rte_sw_bnd(kd, sub.top_at_1, go["tau"], go["ssa"], go["g"], sub.mu0, toa, sub.sfc_alb,
+                               0.5 * sub.sfc_alb, p_lev=sub.p_lev)
+            for k, v in (("up", "flux_up"), ("dn", "flux_dn"), ("dir", "flux_dn_dir"), ("hr", "heating_rate")):
+                sw[k][c0:c0 + n] = o[v].to_host()
+            del go, ct, cs, cg, o, toa
         kdl = api.KDist(kl, AVAILABLE, prec)
         slw = api.SolverLW(kdl, api.Cloud(ll, prec))
         slw.set_block(block)
-        lw = slw.solve(api.DeviceAtmosphere(atm, prec))
-        lw = {"up": lw["flux_up"].to_host().astype(np.float64), "dn": lw["flux_dn"].to_host().astype(np.float64)}
+        lw = slw.solve(api.DeviceAtmosphere(atm, prec), heating_rate=True)
+        lw = {"up": lw["flux_up"].to_host().astype(np.float64), "dn": lw["flux_dn"].to_host().astype(np.float64),
+              "hr": lw["heating_rate"].to_host().astype(np.float64)}
         device.synchronize()
         res[prec] = {"sw": sw, "lw": lw}
         log(f"FP{8 * prec} run in {time.perf_counter() - t0:.1f} s")
@@ -323,12 +325,16 @@ def precision_study(ncol, nlay, block, ocols, threads):
         for k in names:
             out[f"{rng}_flux_{k}"] = errs(f4[k], f8[k])
         out[f"{rng}_flux_net"] = errs(f4["up"] - f4["dn"], f8["up"] - f8["dn"])
+        # Heating rates from the solvers' FP64 island (each layer's net-flux convergence formed per
+        # g-point and summed in FP64), on all layers.
+        out[f"{rng}_heating_rate_kday"] = errs(f4["hr"], f8["hr"])
+        # For comparison: heating rates differenced on the host from each run's rounded level fluxes.
+        # Above 1 hPa the layers are ~1 Pa thick, and the FP32 rounding of the fluxes alone
+        # (~1e-4 W m-2) moves those by K/day.
         h4, h8 = heating(f4["up"], f4["dn"], plev), heating(f8["up"], f8["dn"], plev)
-        out[f"{rng}_heating_rate_kday"] = errs(h4, h8)
-        # Layers below 1 hPa: above it the layers are ~1 Pa thick, and the FP32 rounding of the
-        # fluxes alone (~1e-4 W m-2) moves their heating rates by K/day.
         deep = (0.5 * (plev[:, 1:] + plev[:, :-1])) > 100.0
-        out[f"{rng}_heating_rate_kday_p_gt_1hPa"] = errs(h4[deep], h8[deep])
+        out[f"{rng}_heating_rate_kday_from_level_fluxes"] = errs(h4, h8)
+        out[f"{rng}_heating_rate_kday_from_level_fluxes_p_gt_1hPa"] = errs(h4[deep], h8[deep])
     # Oracle on the first ocols columns (SW with the same forced layers).
     sub = syn.subset_columns(atm, slice(0, ocols))
     okd = O.Kdist(ks, AVAILABLE)

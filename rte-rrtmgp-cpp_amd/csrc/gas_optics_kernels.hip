@@ -848,10 +848,12 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
         g_c2 += g_cm * r;
         if (mix)
         {
+            // d eta / d c1 = r c2 / cm^2 = (1 - eta) / cm and d eta / d c2 = -eta r / cm, formed without
+            // squaring cm: column amounts reach ~1e24 molecules cm-2, whose square overflows FP32.
             const Real g_eta = g_feta * Real(k.neta - 1);
-            const Real icm2 = Real(1) / (cm * cm);
-            g_c1 += g_eta * (cm - c1) * icm2;
-            g_c2 -= g_eta * c1 * r * icm2;
+            const Real icm = Real(1) / cm;
+            g_c1 += g_eta * (r * c2 * icm) * icm;
+            g_c2 -= g_eta * eta * r * icm;
         }
     }
     dcol(gas1, g_c1);

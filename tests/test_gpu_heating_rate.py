@@ -4,7 +4,8 @@ The solvers form each layer's net-flux convergence per g-point from the layer's 
 it over g-points in FP64 (rrtmgp_hip_{lw_solver_noscat,sw_solver_2stream}_bnd_*), instead of
 differencing rounded level fluxes.  Checks:
   * FP64 build against the oracle's heating rates (FP64 differences of FP64 fluxes) through the
-    drivers, both orientations, with and without column blocks: within 1e-7 K/day + 1e-8 relative;
+    drivers, both orientations, with and without column blocks: within 1e-8 relative plus what the
+    1e-10 relative flux agreement allows over the layer (g0/cp 2e-10 max|F| / |dp|);
   * FP32 build against the FP64 build with 10% of the cloud layers near conservative scattering
     (ssa = 1 - 1e-7, g = 0.85): within 0.05 K/day on every layer, including the ~1 Pa layers at the
     model top, where differencing FP32 level fluxes is off by K/day.
@@ -36,11 +37,15 @@ def test_driver_heating_rate_fp64(syn, O, okd_lw, okd_sw, kd_lw, kd_sw, lut_lw, 
     gsw = ssw.solve(da, tsi, heating_rate=True)
     rlw = O.solve_lw(okd_lw, atm, O.Cloud(lut_lw))
     rsw = O.solve_sw(okd_sw, atm, O.Cloud(lut_sw), tsi_scaling=tsi)
+    dp = np.abs(np.diff(atm.p_lev, axis=1))
     for name, g, r in (("lw", glw, rlw), ("sw", gsw, rsw)):
         want = O.heating_rate(r["flux_dn"] - r["flux_up"], atm.p_lev)
         got = g["heating_rate"].to_host()
-        err = np.abs(got - want) - 1e-8 * np.abs(want)
-        assert err.max() < 1e-7, (name, float(np.abs(got - want).max()))
+        # The HIP FP64 path meets the oracle's fluxes to 1e-10 relative; the heating rates may differ
+        # by that much of the flux over the layer's pressure thickness.
+        fmax = max(np.abs(r["flux_up"]).max(), np.abs(r["flux_dn"]).max(), 1.0)
+        tol = 1e-8 * np.abs(want) + C_HR * 2e-10 * fmax / dp
+        assert np.all(np.abs(got - want) <= tol), (name, float((np.abs(got - want) / tol).max()))
         # the fluxes are unchanged by asking for heating rates
         assert np.abs(g["flux_up"].to_host() - r["flux_up"]).max() < 1e-9
 
