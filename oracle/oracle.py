@@ -71,7 +71,7 @@ def lib():
         L.oracle_cloud_create.argtypes = [C.c_int] + [C.c_double] * 4 + [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_cloud_destroy.argtypes = [C.c_void_p]
         for name in ("oracle_gas_optics", "oracle_cloud_optics", "oracle_solve_lw", "oracle_solve_sw",
-                     "oracle_rte_lw", "oracle_rte_sw"):
+                     "oracle_rte_lw", "oracle_rte_sw", "oracle_rte_lw_rescaled", "oracle_lw_solver_2stream"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -339,6 +339,40 @@ def rte_lw(tau, lay_src, lev_inc, lev_dec, sfc_src, sfc_emis, band_lims, top_at_
                                n_gauss_angles, int(do_broadband), ptr(up), ptr(dn),
                                ptr(None if sfc_src_jac is None else _f64(sfc_src_jac)), ptr(jac)))
     return (up, dn, jac) if jac is not None else (up, dn)
+
+
+def rte_lw_rescaled(tau, ssa, g, lay_src, lev_inc, lev_dec, sfc_src, sfc_emis, band_lims, top_at_1,
+                    inc_flux=None, n_gauss_angles=1, do_broadband=False, sfc_src_jac=None):
+    """rte_lw with the Tang et al. (2018) scattering rescaling (do_rescaling = true)."""
+    tau, ssa, g, lay_src, lev_inc, lev_dec, sfc_src, sfc_emis = (
+        _f64(a) for a in (tau, ssa, g, lay_src, lev_inc, lev_dec, sfc_src, sfc_emis))
+    band_lims = _i32(band_lims)
+    ncol, nlay, ngpt = tau.shape
+    nlev = nlay + 1
+    shape = (ncol, nlev) if do_broadband else (ncol, nlev, ngpt)
+    up = np.zeros(shape, order="F")
+    dn = np.zeros(shape, order="F")
+    jac = np.zeros((ncol, nlev), order="F") if sfc_src_jac is not None else None
+    _check(lib().oracle_rte_lw_rescaled(ncol, nlay, ngpt, band_lims.shape[1], ptr(band_lims, C.c_int),
+                                        int(top_at_1), ptr(tau), ptr(ssa), ptr(g), ptr(lay_src), ptr(lev_inc),
+                                        ptr(lev_dec), ptr(sfc_src), ptr(sfc_emis),
+                                        ptr(None if inc_flux is None else _f64(inc_flux)), n_gauss_angles,
+                                        int(do_broadband), ptr(up), ptr(dn),
+                                        ptr(None if sfc_src_jac is None else _f64(sfc_src_jac)), ptr(jac)))
+    return (up, dn, jac) if jac is not None else (up, dn)
+
+
+def lw_solver_2stream(tau, ssa, g, lev_inc, lev_dec, sfc_emis, sfc_src, top_at_1, inc_flux=None):
+    """lw_solver_2stream: spectral fluxes (ncol, nlay+1, ngpt); sfc_emis, sfc_src, inc_flux (ncol, ngpt)."""
+    tau, ssa, g, lev_inc, lev_dec, sfc_emis, sfc_src = (
+        _f64(a) for a in (tau, ssa, g, lev_inc, lev_dec, sfc_emis, sfc_src))
+    ncol, nlay, ngpt = tau.shape
+    up = np.zeros((ncol, nlay + 1, ngpt), order="F")
+    dn = np.zeros((ncol, nlay + 1, ngpt), order="F")
+    _check(lib().oracle_lw_solver_2stream(ncol, nlay, ngpt, int(top_at_1), ptr(tau), ptr(ssa), ptr(g),
+                                          ptr(lev_inc), ptr(lev_dec), ptr(sfc_emis), ptr(sfc_src),
+                                          ptr(None if inc_flux is None else _f64(inc_flux)), ptr(up), ptr(dn)))
+    return up, dn
 
 
 def rte_sw(tau, ssa, g, mu0, inc_flux_dir, sfc_alb_dir, sfc_alb_dif, band_lims, top_at_1,

@@ -877,15 +877,72 @@ void increment(size_t n, double* tau1, double* ssa1, double* g1, const double* t
 
 const double pi = std::acos(-1.0);
 
+// lw_transport_1rescl (F90:731-822): the up transport with the adjustment of Tang et al. (2018)
+// Eq. 18b/20 for the linear-in-tau source, then a second down transport with that of Eq. 19/21,
+// overwriting the first.  The up pass reads the first down pass's radiance at the top of each
+// layer in both orientations; the second down pass reads the up radiance at level ilev of layer
+// ilev, which is the layer's top when top_at_1 and its bottom otherwise (F90:773-776, 806-809),
+// as the reference does.  The Jacobian is transported without adjustment.
+void lw_transport_1rescl(int ncol, int nlay, bool top_at_1, const double* trans, const double* src_dn,
+                         const double* src_up, double* up, double* dn, const double* An, const double* Cn,
+                         bool do_jac, double* jac)
+{
+    auto L = [ncol](int i, int k) { return i + size_t(ncol)*k; };
+    if (top_at_1)
+    {
+        for (int il = nlay - 1; il >= 0; --il)
+        {
+            for (int i = 0; i < ncol; ++i)
+            {
+                const size_t l = L(i, il);
+                const double adj = Cn[l] * (An[l] * dn[l] - trans[l] * src_dn[l] - src_up[l]);
+                up[l] = trans[l] * up[L(i, il+1)] + src_up[l] + adj;
+            }
+            if (do_jac) for (int i = 0; i < ncol; ++i) jac[L(i, il)] = trans[L(i, il)] * jac[L(i, il+1)];
+        }
+        for (int il = 0; il < nlay; ++il)
+            for (int i = 0; i < ncol; ++i)
+            {
+                const size_t l = L(i, il);
+                const double adj = Cn[l] * (An[l] * up[l] - trans[l] * src_up[l] - src_dn[l]);
+                dn[L(i, il+1)] = trans[l] * dn[l] + src_dn[l] + adj;
+            }
+    }
+    else
+    {
+        for (int il = 0; il < nlay; ++il)
+        {
+            for (int i = 0; i < ncol; ++i)
+            {
+                const size_t l = L(i, il);
+                const double adj = Cn[l] * (An[l] * dn[L(i, il+1)] - trans[l] * src_dn[l] - src_up[l]);
+                up[L(i, il+1)] = trans[l] * up[l] + src_up[l] + adj;
+            }
+            if (do_jac) for (int i = 0; i < ncol; ++i) jac[L(i, il+1)] = trans[L(i, il)] * jac[L(i, il)];
+        }
+        for (int il = nlay - 1; il >= 0; --il)
+            for (int i = 0; i < ncol; ++i)
+            {
+                const size_t l = L(i, il);
+                const double adj = Cn[l] * (An[l] * up[l] - trans[l] * src_up[l] - src_dn[l]);
+                dn[l] = trans[l] * dn[L(i, il+1)] + src_dn[l] + adj;
+            }
+    }
+}
+
 // lw_source_noscat (F90:616-653), transport dn (:659-689) and up (:691-723), and the g-point
 // loop of lw_solver_noscat (F90:58-260).  Fluxes are (ncol, nlay+1, ngpt) or broadband
 // (ncol, nlay+1) when do_broadband.  Jacobian is the broadband sfc-T Jacobian (F90:236-256).
+// do_rescaling (F90:168-200, Tang et al. 2018): tau_loc = tau D (1 - w + w (1 - g)/2), the
+// adjustment coefficients Cn = 0.4 w (1 - g)/2 / scaleTau and An = 1 - trans^2, and the up and
+// second down transport of lw_transport_1rescl (F90:731-822) in place of the up transport.
 void lw_solver_noscat(int ncol, int nlay, int ngpt, bool top_at_1, const double* D, double weight,
                       const double* tau, const double* lay_source, const double* lev_source_inc,
                       const double* lev_source_dec, const double* sfc_emis, const double* sfc_src,
                       const double* inc_flux, double* flux_up, double* flux_dn, bool do_broadband,
                       double* bb_up, double* bb_dn, bool do_jac, const double* sfc_src_jac,
-                      double* flux_up_jac)
+                      double* flux_up_jac, bool do_rescaling = false, const double* ssa = nullptr,
+                      const double* g = nullptr)
 {
     const int nlev = nlay + 1;
     const int top_level = top_at_1 ? 0 : nlay;
@@ -895,8 +952,14 @@ void lw_solver_noscat(int ncol, int nlay, int ngpt, bool top_at_1, const double*
     const double tau_thresh = std::sqrt(std::numeric_limits<double>::epsilon());
     const size_t ncl = size_t(ncol) * nlay, ncv = size_t(ncol) * nlev;
 
-    std::vector<double> tau_loc(ncl), trans(ncl), src_dn(ncl), src_up(ncl);
+    std::vector<double> tau_loc(ncl), trans(ncl), src_dn(ncl), src_up(ncl), An, Cn;
     std::vector<double> loc_up(ncv), loc_dn(ncv), jac(ncv);
+    if (do_rescaling)
+    {
+        if (!ssa || !g) throw std::runtime_error("lw_solver_noscat: rescaling needs ssa and g");
+        An.resize(ncl);
+        Cn.resize(ncl);
+    }
     if (do_broadband) { std::fill(bb_up, bb_up + ncv, 0.); std::fill(bb_dn, bb_dn + ncv, 0.); }
     if (do_jac) std::fill(flux_up_jac, flux_up_jac + ncv, 0.);
 
@@ -910,7 +973,16 @@ void lw_solver_noscat(int ncol, int nlay, int ngpt, bool top_at_1, const double*
         {
             const size_t icol = i % ncol;
             tau_loc[i] = tau[i + ncl*igpt] * D[icol + size_t(ncol)*igpt];
+            if (do_rescaling)
+            {
+                const double ssal = ssa[i + ncl*igpt];
+                const double wb = ssal * (1. - g[i + ncl*igpt]) * 0.5;
+                const double scale_tau = 1. - ssal + wb;
+                Cn[i] = 0.4 * wb / scale_tau;
+                tau_loc[i] *= scale_tau;
+            }
             trans[i] = std::exp(-tau_loc[i]);
+            if (do_rescaling) An[i] = 1. - trans[i] * trans[i];
         }
         for (size_t i = 0; i < ncl; ++i)
         {
@@ -939,7 +1011,10 @@ void lw_solver_noscat(int ncol, int nlay, int ngpt, bool top_at_1, const double*
             gup[s] = gdn[s] * (1. - e) + e * sfc_src[icol + size_t(ncol)*igpt];
             if (do_jac) jac[s] = e * sfc_src_jac[icol + size_t(ncol)*igpt];
         }
-        if (top_at_1)
+        if (do_rescaling)
+            lw_transport_1rescl(ncol, nlay, top_at_1, trans.data(), src_dn.data(), src_up.data(), gup, gdn,
+                                An.data(), Cn.data(), do_jac, jac.data());
+        else if (top_at_1)
             for (int ilev = nlay - 1; ilev >= 0; --ilev)
                 for (int icol = 0; icol < ncol; ++icol)
                 {
@@ -983,7 +1058,8 @@ void rte_lw(int ncol, int nlay, int ngpt, int nbnd, const int* band_lims, bool t
             const double* tau, const double* lay_src, const double* lev_inc, const double* lev_dec,
             const double* sfc_src, const double* sfc_emis_bnd, const double* inc_flux, int nmus,
             bool do_broadband, double* flux_up, double* flux_dn, const double* sfc_src_jac,
-            double* flux_up_jac)
+            double* flux_up_jac, bool do_rescaling = false, const double* ssa = nullptr,
+            const double* g = nullptr)
 {
     if (nmus < 1 || nmus > 4) throw std::runtime_error("n_gauss_angles must be 1..4");
     const int nlev = nlay + 1;
@@ -1007,13 +1083,13 @@ void rte_lw(int ncol, int nlay, int ngpt, int nbnd, const int* band_lims, bool t
         if (imu == 0)
             lw_solver_noscat(ncol, nlay, ngpt, top_at_1, D.data(), w, tau, lay_src, lev_inc, lev_dec,
                              emis.data(), sfc_src, inc_flux, flux_up, flux_dn, do_broadband,
-                             flux_up, flux_dn, do_jac, sfc_src_jac, flux_up_jac);
+                             flux_up, flux_dn, do_jac, sfc_src_jac, flux_up_jac, do_rescaling, ssa, g);
         else
         {
             tup.assign(nout, 0.); tdn.assign(nout, 0.); tjac.assign(ncv, 0.);
             lw_solver_noscat(ncol, nlay, ngpt, top_at_1, D.data(), w, tau, lay_src, lev_inc, lev_dec,
                              emis.data(), sfc_src, inc_flux, tup.data(), tdn.data(), do_broadband,
-                             tup.data(), tdn.data(), do_jac, sfc_src_jac, tjac.data());
+                             tup.data(), tdn.data(), do_jac, sfc_src_jac, tjac.data(), do_rescaling, ssa, g);
             for (size_t i = 0; i < nout; ++i) { flux_up[i] += tup[i]; flux_dn[i] += tdn[i]; }
             if (do_jac) for (size_t i = 0; i < ncv; ++i) flux_up_jac[i] += tjac[i];
         }
@@ -1127,6 +1203,81 @@ void adding(int ncol, int nlay, bool top_at_1, const double* albedo_sfc, const d
                                      src_dn[L(i, il)]) * denom[L(i, il)];
                 flux_up[L(i, il)] = flux_dn[L(i, il)] * albedo[L(i, il)] + src[L(i, il)];
             }
+    }
+}
+
+// lw_solver_2stream (F90:386-453): per g-point, lw_combine_sources (F90:896-920), lw_two_stream
+// (F90:832-887, Fu et al. 1997 coefficients with the diffusivity secant 1.66), lw_source_2str
+// (F90:928-978, Toon et al. 1989 Eqs. 26-27; layer sources zero where tau <= 1e-8), then the
+// adding method with surface albedo 1 - sfc_emis and flux_dn(top) = inc_flux.  sfc_emis, sfc_src
+// and inc_flux are (ncol, ngpt); fluxes (ncol, nlay+1, ngpt).  lay_source is an argument of the
+// reference but no part of its arithmetic.
+void lw_solver_2stream(int ncol, int nlay, int ngpt, bool top_at_1, const double* tau, const double* ssa,
+                       const double* g, const double* lev_source_inc, const double* lev_source_dec,
+                       const double* sfc_emis, const double* sfc_src, const double* inc_flux, double* flux_up,
+                       double* flux_dn)
+{
+    const double k_min = 1.e-12;
+    const double lw_diff_sec = 1.66;
+    const int nlev = nlay + 1;
+    const size_t ncl = size_t(ncol) * nlay, ncv = size_t(ncol) * nlev;
+    auto L = [ncol](int i, int k) { return i + size_t(ncol)*k; };
+    std::vector<double> lev(ncv), rdif(ncl), tdif(ncl), g1(ncl), g2(ncl), sdn(ncl), sup(ncl), ssfc(ncol), alb(ncol);
+    for (int igpt = 0; igpt < ngpt; ++igpt)
+    {
+        const double* inc = lev_source_inc + ncl*igpt;
+        const double* dec = lev_source_dec + ncl*igpt;
+        const double* tg = tau + ncl*igpt;
+        for (int i = 0; i < ncol; ++i)
+        {
+            lev[L(i, 0)] = dec[L(i, 0)];
+            lev[L(i, nlay)] = inc[L(i, nlay-1)];
+        }
+        for (int il = 1; il < nlay; ++il)
+            for (int i = 0; i < ncol; ++i)
+                lev[L(i, il)] = std::sqrt(dec[L(i, il)] * inc[L(i, il-1)]);
+        for (size_t j = 0; j < ncl; ++j)
+        {
+            const double w0 = ssa[j + ncl*igpt], gg = g[j + ncl*igpt];
+            g1[j] = lw_diff_sec * (1. - 0.5 * w0 * (1. + gg));
+            g2[j] = lw_diff_sec * 0.5 * w0 * (1. - gg);
+            const double k = std::sqrt(std::max((g1[j] - g2[j]) * (g1[j] + g2[j]), k_min));
+            const double e = std::exp(-tg[j] * k);
+            const double e2 = e * e;
+            const double rt = 1. / (k * (1. + e2) + g1[j] * (1. - e2));
+            rdif[j] = rt * g2[j] * (1. - e2);
+            tdif[j] = rt * 2. * k * e;
+        }
+        for (int il = 0; il < nlay; ++il)
+            for (int i = 0; i < ncol; ++i)
+            {
+                const size_t j = L(i, il);
+                const double top = top_at_1 ? lev[L(i, il)] : lev[L(i, il+1)];
+                const double bot = top_at_1 ? lev[L(i, il+1)] : lev[L(i, il)];
+                if (tg[j] > 1.e-8)
+                {
+                    const double Z = (bot - top) / (tg[j] * (g1[j] + g2[j]));
+                    const double zup_top = Z + top, zup_bot = Z + bot, zdn_top = -Z + top, zdn_bot = -Z + bot;
+                    sup[j] = pi * (zup_top - rdif[j] * zdn_top - tdif[j] * zup_bot);
+                    sdn[j] = pi * (zdn_bot - rdif[j] * zup_bot - tdif[j] * zdn_top);
+                }
+                else
+                {
+                    sup[j] = 0.;
+                    sdn[j] = 0.;
+                }
+            }
+        for (int i = 0; i < ncol; ++i)
+        {
+            ssfc[i] = pi * sfc_emis[i + size_t(ncol)*igpt] * sfc_src[i + size_t(ncol)*igpt];
+            alb[i] = 1. - sfc_emis[i + size_t(ncol)*igpt];
+        }
+        double* up = flux_up + ncv*igpt;
+        double* dn = flux_dn + ncv*igpt;
+        const int top_level = top_at_1 ? 0 : nlay;
+        for (int i = 0; i < ncol; ++i) dn[L(i, top_level)] = inc_flux ? inc_flux[i + size_t(ncol)*igpt] : 0.;
+        adding(ncol, nlay, top_at_1, alb.data(), rdif.data(), tdif.data(), sdn.data(), sup.data(), ssfc.data(),
+               up, dn);
     }
 }
 
@@ -1477,6 +1628,34 @@ int oracle_rte_lw(int ncol, int nlay, int ngpt, int nbnd, const int* band_lims, 
         rte_lw(ncol, nlay, ngpt, nbnd, band_lims, top_at_1 != 0, tau, lay_src, lev_inc, lev_dec,
                sfc_src, sfc_emis, inc_flux, n_gauss_angles, do_broadband != 0, flux_up, flux_dn,
                sfc_src_jac, flux_up_jac);
+    });
+}
+
+// rte_lw_solver_noscat (include/rrtmgp_kernels.h:150-158) with do_rescaling = true: Gauss
+// quadrature over n_gauss_angles as oracle_rte_lw, with the Tang et al. rescaling by ssa, g.
+int oracle_rte_lw_rescaled(int ncol, int nlay, int ngpt, int nbnd, const int* band_lims, int top_at_1,
+                           const double* tau, const double* ssa, const double* g, const double* lay_src,
+                           const double* lev_inc, const double* lev_dec, const double* sfc_src,
+                           const double* sfc_emis, const double* inc_flux, int n_gauss_angles, int do_broadband,
+                           double* flux_up, double* flux_dn, const double* sfc_src_jac, double* flux_up_jac)
+{
+    return guarded([&] {
+        rte_lw(ncol, nlay, ngpt, nbnd, band_lims, top_at_1 != 0, tau, lay_src, lev_inc, lev_dec,
+               sfc_src, sfc_emis, inc_flux, n_gauss_angles, do_broadband != 0, flux_up, flux_dn,
+               sfc_src_jac, flux_up_jac, true, ssa, g);
+    });
+}
+
+// lw_solver_2stream (F90:386-453), spectral fluxes (ncol, nlay+1, ngpt); sfc_emis, sfc_src and
+// inc_flux (NULL = 0) per (ncol, ngpt) as the Fortran kernel takes them.
+int oracle_lw_solver_2stream(int ncol, int nlay, int ngpt, int top_at_1, const double* tau, const double* ssa,
+                             const double* g, const double* lev_inc, const double* lev_dec,
+                             const double* sfc_emis, const double* sfc_src, const double* inc_flux,
+                             double* flux_up, double* flux_dn)
+{
+    return guarded([&] {
+        lw_solver_2stream(ncol, nlay, ngpt, top_at_1 != 0, tau, ssa, g, lev_inc, lev_dec, sfc_emis, sfc_src,
+                          inc_flux, flux_up, flux_dn);
     });
 }
 

@@ -374,3 +374,178 @@ def test_solve_threads_identical(O, okd_lw, okd_sw, lut_lw, lut_sw, syn):
             O.solve_lw(okd_lw, bad)
     finally:
         O.set_threads(prev)
+
+
+# --------------------------------------------------------------------- LW rescaling, LW 2-stream
+def test_lw_rescaling_reduces_to_noscat(O):
+    """Without scattering the rescaled solver is the no-scattering solver; with g = 1 (Cn = 0) it is
+    the no-scattering solver on the scaled optical depth tau (1 - ssa)."""
+    rng = np.random.default_rng(11)
+    tau, lay, linc, ldec, sfc, emis, bl = _lw_case(rng)
+    z = np.zeros_like(tau)
+    for nmu in (1, 3):
+        up, dn = O.rte_lw(tau, lay, linc, ldec, sfc, emis, bl, top_at_1=True, n_gauss_angles=nmu)
+        up2, dn2 = O.rte_lw_rescaled(tau, z, rng.uniform(0, 1, tau.shape), lay, linc, ldec, sfc, emis, bl,
+                                     top_at_1=True, n_gauss_angles=nmu)
+        assert np.array_equal(up, up2) and np.array_equal(dn, dn2)
+        ssa = rng.uniform(0, 1, tau.shape)
+        for top_at_1 in (True, False):
+            up, dn = O.rte_lw(tau * (1 - ssa), lay, linc, ldec, sfc, emis, bl, top_at_1=top_at_1, n_gauss_angles=nmu)
+            up2, dn2 = O.rte_lw_rescaled(tau, ssa, np.ones_like(tau), lay, linc, ldec, sfc, emis, bl,
+                                         top_at_1=top_at_1, n_gauss_angles=nmu)
+            assert rel_err(up2, up) < 1e-13 and rel_err(dn2, dn, floor=1e-12) < 1e-13
+
+
+@pytest.mark.parametrize("top_at_1", [True, False])
+def test_lw_rescaling_single_layer_reflection(O, top_at_1):
+    """One non-emitting layer over a black surface lit from above.  With g = -1 the optical depth is
+    not scaled and Cn = 0.4 ssa: the up pass reflects Cn An of the incident radiance at the layer
+    top (An = 1 - T^2); the second down pass adds Cn An times the up radiance of level ilev, which
+    is the layer's top (the reflected radiance) when top_at_1 and its bottom (zero) otherwise."""
+    tau0, w, F = 0.7, 0.6, 50.0
+    T = np.exp(-1.66 * tau0)
+    An, Cn = 1 - T * T, 0.4 * w
+    arr = lambda v: np.full((1, 1, 1), v)
+    z = arr(0.0)
+    up, dn = O.rte_lw_rescaled(arr(tau0), arr(w), arr(-1.0), z, z, z, np.zeros((1, 1)), np.ones((1, 1)),
+                               np.array([[1], [1]], dtype=np.int32), top_at_1, inc_flux=np.full((1, 1), F))
+    top, bot = (0, 1) if top_at_1 else (1, 0)
+    assert abs(up[0, top, 0] - Cn * An * F) < 1e-13 * F and up[0, bot, 0] == 0.0
+    want_dn = T * F + (Cn * An * Cn * An * F if top_at_1 else 0.0)
+    assert abs(dn[0, bot, 0] - want_dn) < 1e-13 * F and dn[0, top, 0] == F
+
+
+def test_lw_rescaling_jacobian_and_broadband(O):
+    """The surface-temperature Jacobian is transported without the adjustment, i.e. as the
+    no-scattering Jacobian on the scaled optical depth; broadband fluxes are the g-point sums."""
+    rng = np.random.default_rng(12)
+    tau, lay, linc, ldec, sfc, emis, bl = _lw_case(rng)
+    ssa, g = rng.uniform(0, 1, tau.shape), rng.uniform(0, 0.9, tau.shape)
+    jac_src = rng.uniform(0.01, 0.1, sfc.shape)
+    scale = 1 - ssa + ssa * (1 - g) * 0.5
+    bu, bd, jac = O.rte_lw_rescaled(tau, ssa, g, lay, linc, ldec, sfc, emis, bl, True, do_broadband=True,
+                                    sfc_src_jac=jac_src)
+    _, _, jac0 = O.rte_lw(tau * scale, lay, linc, ldec, sfc, emis, bl, True, do_broadband=True, sfc_src_jac=jac_src)
+    assert rel_err(jac, jac0) < 1e-13
+    up, dn = O.rte_lw_rescaled(tau, ssa, g, lay, linc, ldec, sfc, emis, bl, True)
+    assert rel_err(bu, up.sum(2)) < 1e-13 and rel_err(bd, dn.sum(2), floor=1e-12) < 1e-13
+
+
+def _lw2(O, tau, ssa, g, lev_top, lev_bot, emis, sfc, top_at_1=True, inc=None):
+    """lw_solver_2stream with level sources given per layer top/bottom (a column-constant source
+    profile combines to those values exactly: sqrt(B B) = B)."""
+    ncol, nlay, ngpt = tau.shape
+    return O.lw_solver_2stream(tau, ssa, g, lev_bot if top_at_1 else lev_top, lev_top if top_at_1 else lev_bot,
+                               emis, sfc, top_at_1, inc_flux=inc)
+
+
+def test_lw_2stream_isothermal_equilibrium(O):
+    """An isothermal column at B over a black surface at B, lit from above by pi B: up = dn = pi B on
+    every level, whatever the scattering (thermodynamic equilibrium); without scattering and without
+    incident flux, dn = pi B (1 - exp(-1.66 tau_above))."""
+    rng = np.random.default_rng(13)
+    ncol, nlay, ngpt = 3, 6, 4
+    tau = rng.uniform(0.0, 2.0, (ncol, nlay, ngpt))
+    ssa, g = rng.uniform(0, 1, tau.shape), rng.uniform(0, 0.9, tau.shape)
+    B = 3.0
+    src = np.full(tau.shape, B)
+    one, sfc = np.ones((ncol, ngpt)), np.full((ncol, ngpt), B)
+    for top_at_1 in (True, False):
+        up, dn = O.lw_solver_2stream(tau, ssa, g, src, src, one, sfc, top_at_1, inc_flux=np.full((ncol, ngpt), PI * B))
+        assert rel_err(up, np.full(up.shape, PI * B)) < 1e-13 and rel_err(dn, np.full(dn.shape, PI * B)) < 1e-13
+    up, dn = O.lw_solver_2stream(tau, 0 * ssa, g, src, src, one, sfc, True)
+    tau_above = np.concatenate([np.zeros((ncol, 1, ngpt)), np.cumsum(tau, axis=1)], axis=1)
+    assert rel_err(up, np.full(up.shape, PI * B)) < 1e-13
+    assert rel_err(dn, PI * B * (1 - np.exp(-1.66 * tau_above)), floor=1e-3) < 1e-12
+
+
+@pytest.mark.parametrize("tau0", [1e-4, 0.1, 1.0, 8.0])
+@pytest.mark.parametrize("top_at_1", [True, False])
+def test_lw_2stream_single_layer_linear_source(O, tau0, top_at_1):
+    """No scattering, one layer with a Planck source linear in optical depth between the level
+    values: the emitted fluxes equal the quadrature of the diffusivity-approximation integrals
+    pi int_0^tau B(t) 1.66 exp(-1.66 t) dt (up, at the top) and with exp(-1.66 (tau - t)) (down,
+    at the bottom)."""
+    from scipy.integrate import quad
+    Bt, Bb, Bs = 2.0, 5.0, 4.0
+    B = lambda t: Bt + (Bb - Bt) * t / tau0
+    e_up = quad(lambda t: B(t) * 1.66 * np.exp(-1.66 * t), 0, tau0, epsabs=0, epsrel=1e-13)[0]
+    e_dn = quad(lambda t: B(t) * 1.66 * np.exp(-1.66 * (tau0 - t)), 0, tau0, epsabs=0, epsrel=1e-13)[0]
+    arr = lambda v: np.full((1, 1, 1), v)
+    up, dn = _lw2(O, arr(tau0), arr(0.0), arr(0.5), arr(Bt), arr(Bb), np.ones((1, 1)), np.full((1, 1), Bs), top_at_1)
+    top, bot = (0, 1) if top_at_1 else (1, 0)
+    T = np.exp(-1.66 * tau0)
+    assert abs(up[0, top, 0] - PI * (e_up + T * Bs)) < 1e-11 * PI * Bb
+    assert abs(dn[0, bot, 0] - PI * e_dn) < 1e-11 * PI * Bb
+    assert up[0, bot, 0] == PI * Bs and dn[0, top, 0] == 0.0
+
+
+def test_lw_2stream_layer_doubling(O):
+    """Two-stream reflectance and transmittance of homogeneous layers obey the doubling rule: a
+    layer of 2 tau reflects and transmits as two layers of tau combined (Rdif, Tdif of the
+    Meador-Weaver solution); with no emission and a black surface, up(top) = R F, dn(sfc) = T F."""
+    rng = np.random.default_rng(14)
+    n = 6
+    tau, w, gg = rng.uniform(0.05, 3.0, n), rng.uniform(0.0, 1.0, n), rng.uniform(0.0, 0.9, n)
+    F = 100.0
+    z2 = np.zeros((n, 2, 1))
+    one, zero, inc = np.ones((n, 1)), np.zeros((n, 1)), np.full((n, 1), F)
+    col = lambda a, k: np.repeat(a[:, None, None], k, axis=1)
+    up1, dn1 = O.lw_solver_2stream(col(2 * tau, 1), col(w, 1), col(gg, 1), z2[:, :1], z2[:, :1], one, zero, True, inc)
+    up2, dn2 = O.lw_solver_2stream(col(tau, 2), col(w, 2), col(gg, 2), z2, z2, one, zero, True, inc)
+    assert rel_err(up1[:, 0, 0], up2[:, 0, 0]) < 1e-12 and rel_err(dn1[:, 1, 0], dn2[:, 2, 0]) < 1e-12
+    # and the semi-infinite limit R = gamma2 / (gamma1 + k) of the Fu et al. coefficients
+    g1 = 1.66 * (1 - 0.5 * w * (1 + gg))
+    g2 = 1.66 * 0.5 * w * (1 - gg)
+    k = np.sqrt(np.maximum((g1 - g2) * (g1 + g2), 1e-12))
+    upi, _ = O.lw_solver_2stream(col(np.full(n, 60.0), 1), col(w, 1), col(gg, 1), z2[:, :1], z2[:, :1], one, zero,
+                                 True, inc)
+    assert rel_err(upi[:, 0, 0], F * g2 / (g1 + k), floor=1e-30) < 1e-12
+
+
+def test_lw_2stream_conservative_scattering_conserves_energy(O):
+    """ssa = 1 and no emission: the net flux is constant with height (to the k_min floor)."""
+    rng = np.random.default_rng(15)
+    ncol, nlay, ngpt = 2, 8, 3
+    tau = rng.uniform(0.1, 3.0, (ncol, nlay, ngpt))
+    g = rng.uniform(0, 0.9, tau.shape)
+    z = np.zeros_like(tau)
+    emis = np.full((ncol, ngpt), 0.6)
+    up, dn = O.lw_solver_2stream(tau, np.ones_like(tau), g, z, z, emis, np.zeros((ncol, ngpt)), True,
+                                 inc_flux=np.full((ncol, ngpt), 200.0))
+    net = dn - up
+    assert np.all(net[:, -1] > 10.0)
+    assert rel_err(net, np.broadcast_to(net[:, -1:, :], net.shape)) < 1e-5
+
+
+@pytest.mark.parametrize("top_at_1", [True, False])
+def test_lw_2stream_level_sources_combine(O, top_at_1):
+    """No scattering, three layers whose edge sources come from one level profile (lev_inc of a
+    layer = the profile at its increasing-index level, lev_dec at its decreasing-index level): each
+    interior level combines to the profile value, so the top up and surface down fluxes are the
+    layer emissions, each the quadrature of its linear-in-tau integral, transmitted through the
+    layers between."""
+    from scipy.integrate import quad
+    Bl = np.array([1.0, 2.5, 6.0, 3.0])      # level profile by increasing index
+    tau = np.array([0.3, 1.2, 0.5])
+    Bs = 4.0
+    nlay = 3
+    # Order the layers top to bottom for the integrals.
+    order = range(nlay) if top_at_1 else range(nlay - 1, -1, -1)
+    lay_tb = [(Bl[l], Bl[l + 1]) if top_at_1 else (Bl[l + 1], Bl[l]) for l in order]
+    t_tb = [tau[l] for l in order]
+
+    def emit(bt, bb, t0, up):
+        B = lambda t: bt + (bb - bt) * t / t0
+        w = (lambda t: np.exp(-1.66 * t)) if up else (lambda t: np.exp(-1.66 * (t0 - t)))
+        return quad(lambda t: B(t) * 1.66 * w(t), 0, t0, epsabs=0, epsrel=1e-13)[0]
+
+    T = np.exp(-1.66 * np.array(t_tb))
+    up_top = Bs * np.prod(T) + sum(emit(*lay_tb[k], t_tb[k], True) * np.prod(T[:k]) for k in range(nlay))
+    dn_sfc = sum(emit(*lay_tb[k], t_tb[k], False) * np.prod(T[k + 1:]) for k in range(nlay))
+    arr = lambda v: np.asarray(v, float)[None, :, None]
+    up, dn = O.lw_solver_2stream(arr(tau), arr(np.zeros(3)), arr(np.full(3, 0.5)), arr(Bl[1:]), arr(Bl[:-1]),
+                                 np.ones((1, 1)), np.full((1, 1), Bs), top_at_1)
+    top, sfc = (0, nlay) if top_at_1 else (nlay, 0)
+    assert abs(up[0, top, 0] - PI * up_top) < 1e-11 * PI * 6.0
+    assert abs(dn[0, sfc, 0] - PI * dn_sfc) < 1e-11 * PI * 6.0
