@@ -81,6 +81,17 @@ typedef struct
     int nsize_ice; const double* lut_extice; const double* lut_ssaice; const double* lut_asyice;
 } rrtmgp_cloud_lut;
 
+/* Aerosol LUT as load_and_init_aerosol_optics reads it (src_test/Radiation_solver.cpp:329-366) */
+/* for the Aerosol_optics constructor (include/Aerosol_optics.h:20-24).                        */
+typedef struct
+{
+    int nbnd, nhum, nphobic, nphilic;
+    const double* band_lims_wvn;     /* (2, nbnd)                                              */
+    const double* rh_upper;          /* (nhum): relative_humidity2                             */
+    const double* mext_phobic; const double* ssa_phobic; const double* g_phobic;   /* (nbnd, nphobic) */
+    const double* mext_philic; const double* ssa_philic; const double* g_philic;   /* (nbnd, nhum, nphilic) */
+} rrtmgp_aerosol_lut;
+
 /* Gas concentrations: n named device arrays with dims (1|ncol, 1|nlay).                      */
 typedef struct
 {
@@ -146,6 +157,15 @@ int rrtmgp_api_cloud_optics(void* cloud, int ncol, int nlay, const rrtmgp_real* 
                             int two_stream, int delta_scale, rrtmgp_real* tau, rrtmgp_real* ssa,
                             rrtmgp_real* g);
 
+/* Aerosol_optics_gpu (include/Aerosol_optics.h:55-101): aerosol_concs is a gas-concentration */
+/* handle holding aermr01..aermr11 (Aerosol_concs_gpu = Gas_concs_gpu, as in the reference);    */
+/* rh (ncol, nlay), plev (ncol, nlay+1); outputs (ncol, nlay, nbnd), delta-scaled when asked.  */
+int rrtmgp_api_aerosol_create(const rrtmgp_aerosol_lut* lut, void** aerosol);
+int rrtmgp_api_aerosol_destroy(void* aerosol);
+int rrtmgp_api_aerosol_optics(void* aerosol, int ncol, int nlay, void* aerosol_concs, const rrtmgp_real* rh,
+                              const rrtmgp_real* plev, int delta_scale, rrtmgp_real* tau, rrtmgp_real* ssa,
+                              rrtmgp_real* g);
+
 /* add_to and delta_scale (include/Optical_props.h:174-176,256).  nspec_in is ngpt (same grid) */
 /* or nbnd (band-resolved input); ssa/g NULL for 1scl.                                        */
 int rrtmgp_api_add_to(void* kdist, int ncol, int nlay, int nspec_in, rrtmgp_real* tau,
@@ -187,6 +207,8 @@ int rrtmgp_api_heating_rate(int ncol, int nlay, const rrtmgp_real* flux_up,
 /* addition: the reference has no heating rates).                                                */
 int rrtmgp_api_solver_lw_create(void* kdist, void* cloud, void** solver);
 int rrtmgp_api_solver_sw_create(void* kdist, void* cloud, void** solver);
+/* Shortwave solver with aerosol optics as well (aerosol may be NULL: as solver_sw_create).    */
+int rrtmgp_api_solver_sw_create_aerosol(void* kdist, void* cloud, void* aerosol, void** solver);
 int rrtmgp_api_solver_destroy(void* solver);
 int rrtmgp_api_solver_set_block(void* solver, int n_col_block);
 int rrtmgp_api_solve_lw(void* solver, int ncol, int nlay, int cloudy, int bnd_fluxes,
@@ -207,6 +229,21 @@ int rrtmgp_api_solve_sw(void* solver, int ncol, int nlay, int cloudy, int bnd_fl
                         rrtmgp_real* flux_dn, rrtmgp_real* flux_dir, rrtmgp_real* flux_net,
                         rrtmgp_real* bnd_up, rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir,
                         rrtmgp_real* bnd_net, rrtmgp_real* heating_rate);
+
+/* rrtmgp_api_solve_sw with aerosols (src_test/Radiation_solver.cu:794-810): aerosol != 0      */
+/* adds the aerosol optical properties of rh (ncol, nlay) and aerosol_concs (a gas-concentration */
+/* handle of aermr01..aermr11) after the clouds, delta-scaled when delta_aerosol != 0.          */
+int rrtmgp_api_solve_sw_aerosol(void* solver, int ncol, int nlay, int cloudy, int aerosol, int bnd_fluxes,
+                                int delta_cloud, int delta_aerosol, const rrtmgp_real* play,
+                                const rrtmgp_real* plev, const rrtmgp_real* tlay, const rrtmgp_real* tlev,
+                                void* gas_concs, const rrtmgp_real* col_dry, const rrtmgp_real* sfc_alb_dir,
+                                const rrtmgp_real* sfc_alb_dif, const rrtmgp_real* tsi_scaling,
+                                const rrtmgp_real* mu0, const rrtmgp_real* lwp, const rrtmgp_real* iwp,
+                                const rrtmgp_real* rel, const rrtmgp_real* rei, const rrtmgp_real* rh,
+                                void* aerosol_concs, rrtmgp_real* flux_up, rrtmgp_real* flux_dn,
+                                rrtmgp_real* flux_dir, rrtmgp_real* flux_net, rrtmgp_real* bnd_up,
+                                rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir, rrtmgp_real* bnd_net,
+                                rrtmgp_real* heating_rate);
 
 /* Adjoints of the two drivers above (no reference counterpart): for cotangents g_flux_* of   */
 /* the broadband fluxes (ncol, nlev), the gradients of L = sum(g_flux * flux) with respect to */

@@ -307,6 +307,23 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_FILL_GASES)
         int delta_scale, T* tau, T* ssa, T* g, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_CLOUD)
 
+/* Aerosol optics (Aerosol_optics_gpu::aerosol_optics, src_cuda/Aerosol_optics.cu; CPU        */
+/* src/Aerosol_optics.cpp:26-224).  aermr holds the 11 mass mixing ratios aermr01..aermr11    */
+/* (ngas = 11; dim1 1|ncol, dim2 1|nlay), rh (ncol, nlay), plev (ncol, nlay+1).  Tables as    */
+/* the class holds them: rh_upper (nhum), mext/ssa/g_phobic (nbnd, nphobic >= 11),            */
+/* mext/ssa/g_philic (nbnd, nhum, nphilic >= 5).  Species, table columns and summation order  */
+/* of the CPU path; layer mass |dp| / 9.81 (the GPU path's; equal to the CPU path's for levels */
+/* ordered from the surface up); rh above every class bound takes the last class (the         */
+/* reference reads past the table).  Outputs tau, ssa, g (ncol, nlay, nbnd), delta-scaled     */
+/* (rte_delta_scale_2str_k) when delta_scale != 0.                                             */
+#define RRTMGP_DECL_AEROSOL(S, T)                                                              \
+    int rrtmgp_hip_aerosol_optics_##S(int64_t ncol, int64_t nlay, int64_t nbnd, int64_t nhum, \
+        int64_t nphobic, int64_t nphilic, const rrtmgp_gas_vmr* aermr, const T* rh,            \
+        const T* plev, const T* rh_upper, const T* mext_phobic, const T* ssa_phobic,           \
+        const T* g_phobic, const T* mext_philic, const T* ssa_philic, const T* g_philic,       \
+        int delta_scale, T* tau, T* ssa, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_AEROSOL)
+
 /* ------------------------------------------------------------------------------------------ */
 /* Optical-property arithmetic (Optical_props_kernels_cuda, include_kernels_cuda/             */
 /* optical_props_kernels_cuda.h:31-55; Fortran rte_increment_*, rte_inc_*_bybnd,              */
@@ -440,6 +457,34 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
         T* heating_rate, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_FACTORED)
 
+/* Longwave solvers with scattering.                                                          */
+/*   lw_solver_noscat_rescaled: rte_lw_solver_noscat with do_rescaling = 1 and its ssa, g      */
+/*       arguments (include/rrtmgp_kernels.h:148-158; src_kernels/mo_rte_solver_kernels.F90    */
+/*       :168-200 and lw_transport_1rescl :731-822, Tang et al. 2018).  Other arguments as      */
+/*       rrtmgp_hip_lw_solver_noscat_*: secants (ncol, ngpt, nmus), sfc_emis, sfc_src and       */
+/*       inc_flux (NULL = 0) (ncol, ngpt), spectral fluxes (ncol, nlev, ngpt) or, with          */
+/*       do_broadband, broadband sums (ncol, nlev); the Jacobian needs do_broadband and is      */
+/*       transported without the rescaling adjustment, as the reference's.                      */
+/*   lw_solver_2stream: the two-stream longwave solver lw_solver_2stream (F90:386-453), whose   */
+/*       argument list it keeps (lay_source is accepted and unused, as there): level sources   */
+/*       combined by lw_combine_sources (:896-920), Fu et al. (1997) coefficients with the      */
+/*       diffusivity secant 1.66 (:832-887), Toon et al. (1989) layer sources (:928-978) and    */
+/*       the adding method with surface albedo 1 - sfc_emis (:1117-1227).  sfc_emis, sfc_src,   */
+/*       inc_flux (NULL = 0, a flux, not a radiance) (ncol, ngpt); fluxes (ncol, nlev, ngpt).   */
+/* Neither solver exists on the reference's GPU path.                                          */
+#define RRTMGP_DECL_LW_SCATTERING(S, T)                                                        \
+    int rrtmgp_hip_lw_solver_noscat_rescaled_##S(int64_t ncol, int64_t nlay, int64_t ngpt,    \
+        int top_at_1, int64_t nmus, const T* secants, const T* weights, const T* tau,           \
+        const T* ssa, const T* g, const T* lay_source, const T* lev_source_inc,                 \
+        const T* lev_source_dec, const T* sfc_emis, const T* sfc_src, const T* inc_flux,        \
+        T* flux_up, T* flux_dn, int do_broadband, T* broadband_up, T* broadband_dn,             \
+        int do_jacobians, const T* sfc_src_jac, T* flux_up_jac, void* stream);                  \
+    int rrtmgp_hip_lw_solver_2stream_##S(int64_t ncol, int64_t nlay, int64_t ngpt,            \
+        int top_at_1, const T* tau, const T* ssa, const T* g, const T* lay_source,              \
+        const T* lev_source_inc, const T* lev_source_dec, const T* sfc_emis, const T* sfc_src,  \
+        const T* inc_flux, T* flux_up, T* flux_dn, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_LW_SCATTERING)
+
 /* ------------------------------------------------------------------------------------------ */
 /* Adjoints.  No counterpart in the reference (its LW path has only the surface-temperature   */
 /* Jacobian, src/Rte_lw.cpp:140).                                                              */
@@ -476,7 +521,9 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_ADJOINT)
 /* Reverse mode of the fused gas optics (rrtmgp_hip_gas_optics_{lw,sw}_bands_* for one band):  */
 /* given the cotangents of the band's outputs ((ncol, nlay, ngb) with ngb = the band's         */
 /* g-points; g_sfc_src (ncol, ngb)), ADDS the gradients with respect to the column amounts     */
-/* d_col_gas (ncol, nlay, ngas+1; gas 0 = dry air, i.e. col_dry), T_lay and ln p_lay           */
+/* d_col_gas (ncol, nlay, ngas+2: slot 0 dry air as a column, slots 1..ngas the gases, slot    */
+/* ngas+1 the derivative with respect to col_dry at fixed volume mixing ratios, formed from the */
+/* degree-1 homogeneity of every column quantity in col_dry), T_lay and ln p_lay               */
 /* (ncol, nlay), and in the longwave T_lev at each layer's level ilay (d_tlev_lo) and          */
 /* ilay+1 (d_tlev_hi) (ncol, nlay) and T_sfc (ncol).  The SW cotangents are those of the       */
 /* outputs after the optional band cloud add_to.  Call band by band on one stream.             */
@@ -499,9 +546,10 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_ADJOINT)
         T* d_plev, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GO_ADJOINT)
 
-/* Assembly steps of the adjoint driver.  col_chain: from d_col_gas (ncol, nlay, ngas+1) to    */
+/* Assembly steps of the adjoint driver.  col_chain: from d_col_gas (ncol, nlay, ngas+2) to    */
 /* d_vmr (ncol, nlay, ngas, overwritten; gases in the descriptor's order) and d_col_dry        */
-/* (ncol, nlay, overwritten).  levels: d_tlev (ncol, nlev) from the per-layer contributions    */
+/* (ncol, nlay, overwritten; slot ngas+1).  levels: d_tlev (ncol, nlev) from the per-layer     */
+/* contributions                                                                               */
 /* of rrtmgp_hip_gas_optics_lw_adjoint_*.  gpt_to_band: band sums of (ncol, ngpt) values into  */
 /* (nbnd, ncol) (band_lims_gpt device, 1-based).                                               */
 #define RRTMGP_DECL_ADJ_HELPERS(S, T)                                                          \

@@ -71,7 +71,8 @@ def lib():
         L.oracle_cloud_create.argtypes = [C.c_int] + [C.c_double] * 4 + [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
         L.oracle_cloud_destroy.argtypes = [C.c_void_p]
         for name in ("oracle_gas_optics", "oracle_cloud_optics", "oracle_solve_lw", "oracle_solve_sw",
-                     "oracle_rte_lw", "oracle_rte_sw", "oracle_rte_lw_rescaled", "oracle_lw_solver_2stream"):
+                     "oracle_rte_lw", "oracle_rte_sw", "oracle_rte_lw_rescaled", "oracle_lw_solver_2stream",
+                     "oracle_aerosol_optics"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -288,6 +289,28 @@ class Cloud:
         _check(lib().oracle_cloud_optics(self.h, ncol, nlay, ptr(lwp), ptr(iwp), ptr(rel), ptr(rei),
                                          int(two_stream), ptr(tau), ptr(ssa), ptr(g)))
         return (tau, ssa, g) if two_stream else tau
+
+
+AEROSOL_FIELDS = tuple(f"aermr{i:02d}" for i in range(1, 12))
+
+
+def aerosol_optics(aermr, rh, plev, lut):
+    """Aerosol_optics::aerosol_optics: aermr maps aermr01..aermr11 to (ncol, nlay) mass mixing ratios;
+    lut holds rh_upper (nhum), mext/ssa/g_phobic (nbnd, nphobic), mext/ssa/g_philic (nbnd, nhum, nphilic).
+    Returns tau, ssa, g (ncol, nlay, nbnd)."""
+    rh, plev = _f64(rh), _f64(plev)
+    ncol, nlay = rh.shape
+    fields = np.stack([np.broadcast_to(np.asarray(aermr[n], dtype=np.float64), (ncol, nlay)).T
+                       for n in AEROSOL_FIELDS])           # (11, nlay, ncol) C order = 11 Fortran fields
+    fields = np.ascontiguousarray(fields)
+    tabs = [_f64(lut[n]) for n in ("rh_upper", "mext_phobic", "ssa_phobic", "g_phobic",
+                                   "mext_philic", "ssa_philic", "g_philic")]
+    nbnd, nphobic = tabs[1].shape
+    _, nhum, nphilic = tabs[4].shape
+    tau, ssa, g = (np.zeros((ncol, nlay, nbnd), order="F") for _ in range(3))
+    _check(lib().oracle_aerosol_optics(ncol, nlay, nbnd, nhum, nphobic, nphilic, ptr(fields), ptr(rh), ptr(plev),
+                                       *[ptr(t) for t in tabs], ptr(tau), ptr(ssa), ptr(g)))
+    return tau, ssa, g
 
 
 def delta_scale(tau, ssa, g):

@@ -813,6 +813,80 @@ void cloud_optics(const Cloud& c, int ncol, int nlay, const double* lwp, const d
 }
 
 // ---------------------------------------------------------------------------------------------
+// Aerosol optics: src/Aerosol_optics.cpp:26-224.
+//
+// rh_class (:26-36): the first 1-based class whose upper bound is >= rh.  The reference reads
+// past the table when rh exceeds every bound; here the last class is taken.
+int aerosol_rh_class(double rh, const double* rh_upper, int nhum)
+{
+    int ihum = 1;
+    while (ihum < nhum && rh_upper[ihum - 1] < rh) ++ihum;
+    return ihum;
+}
+
+// Species of compute_all_from_table (:49-146), in its summation order: the mass-mixing-ratio
+// field aermrNN, whether the species is hydrophilic (table indexed by humidity class), and its
+// 1-based column of the hydrophobic or hydrophilic table.
+struct AerosolSpecies { int field; bool philic; int column; };
+const AerosolSpecies aerosol_species[11] = {
+    {1, true, 1},    // SS1
+    {2, true, 2},    // SS2
+    {3, true, 3},    // SS3
+    {4, false, 1},   // DU1
+    {5, false, 8},   // DU2
+    {6, false, 6},   // DU3
+    {8, false, 10},  // OM1
+    {7, true, 4},    // OM2
+    {9, false, 11},  // BC1
+    {10, false, 11}, // BC2
+    {11, true, 5},   // SU
+};
+
+// compute_all_from_table (:38-150) and Aerosol_optics::aerosol_optics (:168-224) for (ncol, nlay)
+// mixing ratios aermr[0..10], rh (ncol, nlay) and plev (ncol, nlay+1); tables mext/ssa/g_phobic
+// (nbnd, nphobic) and _philic (nbnd, nhum, nphilic); outputs (ncol, nlay, nbnd).  The layer mass
+// is |p(ilay) - p(ilay+1)| / 9.81: the CPU path takes the signed difference (:55), which is
+// the same for levels ordered from the surface up and negative the other way round; the GPU path
+// takes its magnitude (src_cuda/Aerosol_optics.cu:67), which is kept here.
+void aerosol_optics(int ncol, int nlay, int nbnd, int nhum, int nphobic, int nphilic, const double* const* aermr,
+                    const double* rh, const double* plev, const double* rh_upper, const double* mext_phobic,
+                    const double* ssa_phobic, const double* g_phobic, const double* mext_philic,
+                    const double* ssa_philic, const double* g_philic, double* tau, double* ssa, double* g)
+{
+    for (const auto& s : aerosol_species)
+        if (s.column > (s.philic ? nphilic : nphobic))
+            throw std::runtime_error("aerosol_optics: table has too few species");
+    const double eps = std::numeric_limits<double>::epsilon();
+    const size_t ncl = size_t(ncol) * nlay;
+    for (int ibnd = 0; ibnd < nbnd; ++ibnd)
+        for (int ilay = 0; ilay < nlay; ++ilay)
+            for (int icol = 0; icol < ncol; ++icol)
+            {
+                const size_t i2 = icol + size_t(ncol) * ilay;
+                const double dpg = std::abs(plev[i2] - plev[i2 + ncol]) / 9.81;
+                const int ihum = aerosol_rh_class(rh[i2], rh_upper, nhum);
+                double t = 0., ts = 0., tsg = 0.;
+                for (const auto& s : aerosol_species)
+                {
+                    const size_t it = s.philic
+                        ? ibnd + size_t(nbnd) * ((ihum - 1) + size_t(nhum) * (s.column - 1))
+                        : ibnd + size_t(nbnd) * (s.column - 1);
+                    const double mext = s.philic ? mext_philic[it] : mext_phobic[it];
+                    const double sa = s.philic ? ssa_philic[it] : ssa_phobic[it];
+                    const double ga = s.philic ? g_philic[it] : g_phobic[it];
+                    const double od = aermr[s.field - 1][i2] * dpg * mext;
+                    t += od;
+                    ts += od * sa;
+                    tsg += od * sa * ga;
+                }
+                const size_t o = i2 + ncl * ibnd;
+                tau[o] = t;
+                ssa[o] = ts / std::max(t, eps);
+                g[o] = tsg / std::max(ts, eps);
+            }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Optical-property arithmetic: src_kernels_cuda/optical_props_kernels.cu, eps = 3*DBL_MIN
 // (optical_props_kernels_launchers.cu:80,123,172).
 
@@ -1597,6 +1671,22 @@ int oracle_cloud_optics(void* h, int ncol, int nlay, const double* lwp, const do
 {
     return guarded([&] { cloud_optics(*static_cast<Cloud*>(h), ncol, nlay, lwp, iwp, rel, rei,
                                       two_stream != 0, tau, ssa, g); });
+}
+
+// Aerosol_optics::aerosol_optics (src/Aerosol_optics.cpp:168-224); aermr holds the 11 (ncol, nlay)
+// fields aermr01..aermr11 back to back.
+int oracle_aerosol_optics(int ncol, int nlay, int nbnd, int nhum, int nphobic, int nphilic, const double* aermr,
+                          const double* rh, const double* plev, const double* rh_upper,
+                          const double* mext_phobic, const double* ssa_phobic, const double* g_phobic,
+                          const double* mext_philic, const double* ssa_philic, const double* g_philic,
+                          double* tau, double* ssa, double* g)
+{
+    return guarded([&] {
+        const double* fields[11];
+        for (int k = 0; k < 11; ++k) fields[k] = aermr + size_t(k) * ncol * nlay;
+        aerosol_optics(ncol, nlay, nbnd, nhum, nphobic, nphilic, fields, rh, plev, rh_upper, mext_phobic,
+                       ssa_phobic, g_phobic, mext_philic, ssa_philic, g_philic, tau, ssa, g);
+    });
 }
 
 void oracle_delta_scale_2str(int ncol, int nlay, int ngpt, double* tau, double* ssa, double* g)

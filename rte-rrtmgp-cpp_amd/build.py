@@ -28,7 +28,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-I", INC_ROOT, "-I", CSRC, "-Wno-unused-result"]
 
 KERNEL_SOURCES = ["runtime.hip", "array_kernels.hip", "gas_optics_kernels.hip", "optical_props_kernels.hip",
-                  "rte_solver_kernels.hip", "rte_adjoint_kernels.hip",
+                  "rte_solver_kernels.hip", "rte_lw_scattering_kernels.hip", "rte_adjoint_kernels.hip",
                   "mlp_kernels.hip"]
 CPP_SOURCES = []   # filled when the class layer sources exist
 

@@ -822,6 +822,13 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
     const int gas1 = bd.gas[s][0], gas2 = bd.gas[s][1];
     const Real c1 = col_lds(sg, gas1, icol, ilay, cd), c2 = col_lds(sg, gas2, icol, ilay, cd);
     auto dcol = [&](int gas, Real v) { d_col[idx + ncl * gas] += v; };
+    // g_dry: the derivative with respect to col_dry at fixed volume mixing ratios, times col_dry.
+    // Every column the gas optics reads is a vmr times col_dry, and each quantity formed from
+    // them is homogeneous in col_dry: col_mix, the minor scalings and col_ray of degree 1, eta and
+    // the scaling fraction X of degree 0.  So g_dry = sum over the degree-1 quantities of their
+    // cotangent times their value; the eta and X paths, whose per-gas terms cancel exactly in
+    // the sum over gases but not in rounded arithmetic, do not enter.
+    Real g_dry = Real(0);
     Real g_ftemp = Real(0), g_fp = Real(0), g_c1 = Real(0), g_c2 = Real(0);
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -846,6 +853,7 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
         const Real g_cm = acc[GA_CM + i];
         g_c1 += g_cm;
         g_c2 += g_cm * r;
+        g_dry += g_cm * cm;
         if (mix)
         {
             // d eta / d c1 = r c2 / cm^2 = (1 - eta) / cm and d eta / d c2 = -eta r / cm, formed without
@@ -871,6 +879,7 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
         if (!md.dens)
         {
             dcol(md.gas, g_sc);
+            g_dry += g_sc * base;
             continue;
         }
         const Real P = Real(0.01) * pv / tv;
@@ -882,6 +891,7 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
             F = md.compl_ ? Real(1) - X : X;
         }
         dcol(md.gas, g_sc * P * F);
+        g_dry += g_sc * base * P * F;
         const Real g_P = g_sc * base * F;
         g_lnp += g_P * P;
         g_t -= g_P * P / tv;
@@ -909,7 +919,9 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
     {
         dcol(k.idx_h2o, acc[GA_X]);      // col_ray = col(h2o) + col_dry
         dcol(0, acc[GA_X]);
+        g_dry += acc[GA_X] * (col_lds(sg, k.idx_h2o, icol, ilay, cd) + cd);
     }
+    dcol(k.ngas + 1, g_dry / cd);
     d_tlay[idx] += g_t;
     d_lnp[idx] += g_lnp;
 }

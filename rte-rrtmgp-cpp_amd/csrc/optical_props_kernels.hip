@@ -77,6 +77,91 @@ __global__ void cloud_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd,
     }
 }
 
+// Aerosol optics (src/Aerosol_optics.cpp:38-224): per layer, 11 species with mass mixing
+// ratios aermr01..aermr11, each with its column of the hydrophobic table (nbnd, nphobic) or, for
+// hydrophilic species, of the humidity class's slice of (nbnd, nhum, nphilic).  Species in the
+// summation order of compute_all_from_table (:49-146): field (0-based), hydrophilic, column
+// (0-based).
+struct AerosolSpeciesRef { int field, philic, column; };
+constexpr AerosolSpeciesRef aerosol_species_order[11] = {
+    {0, 1, 0},    // SS1: aermr01, hydrophilic 1
+    {1, 1, 1},    // SS2: aermr02, hydrophilic 2
+    {2, 1, 2},    // SS3: aermr03, hydrophilic 3
+    {3, 0, 0},    // DU1: aermr04, hydrophobic 1
+    {4, 0, 7},    // DU2: aermr05, hydrophobic 8
+    {5, 0, 5},    // DU3: aermr06, hydrophobic 6
+    {7, 0, 9},    // OM1: aermr08, hydrophobic 10
+    {6, 1, 3},    // OM2: aermr07, hydrophilic 4
+    {8, 0, 10},   // BC1: aermr09, hydrophobic 11
+    {9, 0, 10},   // BC2: aermr10, hydrophobic 11
+    {10, 1, 4},   // SU:  aermr11, hydrophilic 5
+};
+constexpr int AEROSOL_NSPECIES = 11;
+
+template<typename Real>
+struct AerosolMix
+{
+    const Real* p[AEROSOL_NSPECIES];
+    int64_t scol[AEROSOL_NSPECIES], slay[AEROSOL_NSPECIES];
+};
+
+// One thread per (column, layer); the band loop writes (ncol, nlay, nbnd) with columns fastest.
+// The layer mass is |p(ilay) - p(ilay+1)| / 9.81 (src_cuda/Aerosol_optics.cu:67; the CPU path's
+// signed difference agrees for levels ordered from the surface up).  The humidity class is the
+// first whose upper bound is >= rh, the last when rh exceeds them all.  delta_scale != 0 applies
+// delta_scale_2str_k (optical_props_kernels.cu:139-160) to the result.
+template<typename Real>
+__global__ void aerosol_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd, int64_t nhum, AerosolMix<Real> mix,
+                                      const Real* __restrict__ rh, const Real* __restrict__ plev,
+                                      const Real* __restrict__ rh_upper, const Real* __restrict__ mext_phobic,
+                                      const Real* __restrict__ ssa_phobic, const Real* __restrict__ g_phobic,
+                                      const Real* __restrict__ mext_philic, const Real* __restrict__ ssa_philic,
+                                      const Real* __restrict__ g_philic, int delta_scale, Real* __restrict__ tau,
+                                      Real* __restrict__ ssa, Real* __restrict__ g)
+{
+    const int64_t ncl = ncol * nlay;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ncl) return;
+    const int64_t icol = i % ncol, ilay = i / ncol;
+    const Real dpg = fabs(plev[i] - plev[i + ncol]) / Real(9.81);
+    const Real r = rh[i];
+    int64_t ihum = 0;
+    while (ihum + 1 < nhum && rh_upper[ihum] < r) ++ihum;
+    Real m[AEROSOL_NSPECIES];
+#pragma unroll
+    for (int k = 0; k < AEROSOL_NSPECIES; ++k) m[k] = mix.p[k][icol * mix.scol[k] + ilay * mix.slay[k]];
+    const Real eps = Limits<Real>::eps;
+    const Real op_eps = Real(3) * Limits<Real>::tiny;
+    for (int64_t b = 0; b < nbnd; ++b)
+    {
+        Real t = Real(0), ts = Real(0), tsg = Real(0);
+#pragma unroll
+        for (int k = 0; k < AEROSOL_NSPECIES; ++k)
+        {
+            const AerosolSpeciesRef sp = aerosol_species_order[k];
+            const int64_t it = sp.philic ? b + nbnd * (ihum + nhum * sp.column) : b + nbnd * sp.column;
+            const Real mext = sp.philic ? mext_philic[it] : mext_phobic[it];
+            const Real sa = sp.philic ? ssa_philic[it] : ssa_phobic[it];
+            const Real ga = sp.philic ? g_philic[it] : g_phobic[it];
+            const Real od = m[sp.field] * dpg * mext;
+            t += od;
+            ts += od * sa;
+            tsg += od * sa * ga;
+        }
+        Real s = ts / max(t, eps);
+        Real gg = tsg / max(ts, eps);
+        if (delta_scale)
+        {
+            const Real f = gg * gg, wf = s * f;
+            t *= (Real(1) - wf);
+            s = (s - wf) / max(op_eps, Real(1) - wf);
+            gg = (gg - f) / max(op_eps, Real(1) - f);
+        }
+        const int64_t o = i + ncl * b;
+        tau[o] = t; ssa[o] = s; g[o] = gg;
+    }
+}
+
 template<typename Real>
 __global__ void increment_kernel(int64_t n, Real* __restrict__ tau1, Real* __restrict__ ssa1,
                                  Real* __restrict__ g1, const Real* __restrict__ tau2,
@@ -303,6 +388,37 @@ extern "C" int rrtmgp_hip_cloud_optics_##S(int64_t ncol, int64_t nlay, int64_t n
                        lut_ssaice, lut_asyice, clwp, ciwp, reliq, reice, two_stream, delta_scale,   \
                        tau, ssa, g);                                                                \
     RRTMGP_LAUNCHED("cloud_optics_kernel");                                                         \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_aerosol_optics_##S(int64_t ncol, int64_t nlay, int64_t nbnd,            \
+    int64_t nhum, int64_t nphobic, int64_t nphilic, const rrtmgp_gas_vmr* aermr, const T* rh,       \
+    const T* plev, const T* rh_upper, const T* mext_phobic, const T* ssa_phobic, const T* g_phobic, \
+    const T* mext_philic, const T* ssa_philic, const T* g_philic, int delta_scale, T* tau, T* ssa,  \
+    T* g, void* stream)                                                                              \
+{                                                                                                    \
+    if (ncol <= 0 || nlay <= 0 || nbnd <= 0 || nhum <= 0) return arg_error("aerosol_optics: empty extents"); \
+    if (nphobic < 11 || nphilic < 5)                                                                 \
+        return arg_error("aerosol_optics: tables need 11 hydrophobic and 5 hydrophilic species");  \
+    if (!aermr || aermr->ngas != 11) return arg_error("aerosol_optics: 11 mixing ratios (aermr01..aermr11) expected"); \
+    if (!rh || !plev || !rh_upper || !mext_phobic || !ssa_phobic || !g_phobic || !mext_philic ||    \
+        !ssa_philic || !g_philic || !tau || !ssa || !g)                                              \
+        return arg_error("aerosol_optics: NULL array");                                             \
+    AerosolMix<T> mix;                                                                               \
+    for (int k = 0; k < AEROSOL_NSPECIES; ++k)                                                       \
+    {                                                                                                \
+        const int64_t d1 = aermr->dim1[k], d2 = aermr->dim2[k];                                      \
+        if (!aermr->vmr[k] || !((d1 == 1 || d1 == ncol) && (d2 == 1 || d2 == nlay)))                \
+            return arg_error("aerosol_optics: aermr" + std::to_string(k + 1) + " is NULL or not (1|ncol, 1|nlay)"); \
+        mix.p[k] = static_cast<const T*>(aermr->vmr[k]);                                             \
+        mix.scol[k] = d1 == 1 ? 0 : 1;                                                               \
+        mix.slay[k] = d2 == 1 ? 0 : d1;                                                              \
+    }                                                                                                \
+    const int64_t n = ncol * nlay;                                                                   \
+    RRTMGP_TIMED("aerosol_optics_kernel", as_stream(stream));                                        \
+    hipLaunchKernelGGL(aerosol_optics_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), \
+                       ncol, nlay, nbnd, nhum, mix, rh, plev, rh_upper, mext_phobic, ssa_phobic,     \
+                       g_phobic, mext_philic, ssa_philic, g_philic, delta_scale, tau, ssa, g);       \
+    RRTMGP_LAUNCHED("aerosol_optics_kernel");                                                       \
     return RRTMGP_OK;                                                                                \
 }                                                                                                    \
 extern "C" int rrtmgp_hip_increment_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau_inout,   \

@@ -566,19 +566,11 @@ __global__ void col_chain_kernel(int64_t ncol, int64_t nlay, int ngas, const rrt
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t ncl = ncol * nlay;
     if (i >= ncl) return;
-    const int64_t icol = i % ncol, ilay = i / ncol;
     const Real cd = col_dry[i];
-    Real g = d_col[i];
     for (int gas = 0; gas < ngas; ++gas)
-    {
-        const int64_t d1 = gases.dim1[gas];
-        const int64_t vi = (d1 == 1 ? 0 : icol) + (gases.dim2[gas] == 1 ? 0 : d1 * ilay);
-        const Real v = static_cast<const Real*>(gases.vmr[gas])[vi];
-        const Real dc = d_col[i + ncl * (gas + 1)];
-        d_vmr[i + ncl * gas] = dc * cd;
-        g += dc * v;
-    }
-    d_col_dry[i] = g;
+        d_vmr[i + ncl * gas] = d_col[i + ncl * (gas + 1)] * cd;
+    // The derivative at fixed mixing ratios, formed by the gas-optics adjoint (slot ngas + 1).
+    d_col_dry[i] = d_col[i + ncl * (ngas + 1)];
 }
 
 // d_tlev (ncol, nlev): level i receives d_lo of layer i and d_hi of layer i-1.

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "common.h"
+#include "solver_math.h"
 #include "two_stream.h"
 
 namespace rrtmgp_hip
@@ -141,72 +142,6 @@ __device__ __forceinline__ void write_group_sums_stride(const Real* red, int nsl
         out[0][icol + ncol * (lev + nlev * gi)] = s;
     }
 }
-
-// FP32 building blocks on the hardware approximations (v_exp_f32, v_rcp_f32, v_sqrt_f32, each
-// within about 1 ulp), which cost one or two instructions instead of the IEEE sequences.
-__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
-
-// e^{-y} for y >= 0 through 2^x.
-__device__ __forceinline__ float exp_neg_f32(float y) { return __builtin_amdgcn_exp2f(-y * 1.44269504088896341f); }
-
-// 1 - e^{-y} for 0 <= y < 0.25 from its series (truncation < y^8/8! relative).
-__device__ __forceinline__ float one_minus_exp_neg_series_f32(float y)
-{
-    float t = 1.f / 720.f - y * (1.f / 5040.f);
-    t = 1.f / 120.f - y * t;
-    t = 1.f / 24.f - y * t;
-    t = 1.f / 6.f - y * t;
-    t = .5f - y * t;
-    t = 1.f - y * t;
-    return y * t;
-}
-
-// 1 - e^{-y} for y >= 0 given e = e^{-y}: the series below 0.25, else 1 - e.
-__device__ __forceinline__ float one_minus_exp_neg_f32(float y, float e)
-{
-    return (y < .25f) ? one_minus_exp_neg_series_f32(y) : 1.f - e;
-}
-
-// Array element i at a uniform base with a 32-bit lane offset: addressed as scalar base +
-// 32-bit VGPR offset (the launchers keep the offsets below 2^32 bytes).
-template<typename Real>
-__device__ __forceinline__ const Real& at32(const Real* base, uint32_t i)
-{
-    return *reinterpret_cast<const Real*>(reinterpret_cast<const char*>(base) + i * uint32_t(sizeof(Real)));
-}
-
-template<typename Real>
-__device__ __forceinline__ Real& at32(Real* base, uint32_t i)
-{
-    return *reinterpret_cast<Real*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(Real)));
-}
-
-// Layer transmittance T = e^{-t} and the linear-in-tau source factor of lw_source_noscat
-// (F90:616-653): fact = (1 - T)/t - T, with t (1/2 - t/3) below sqrt(epsilon).  The FP32 build
-// evaluates fact for t < 1/4 from its series t/2 - t^2/3 + t^3/8 - t^4/30 + t^5/144 - t^6/840
-// + t^7/5760 (the reference's small-t form is its first two terms; truncation < 3e-9 relative),
-// which avoids the cancellation of (1 - T)/t - T, and uses the hardware exp2 and reciprocal.
-__device__ __forceinline__ void lw_layer(double t, double& T, double& fact)
-{
-    T = exp(-t);
-    fact = (t > sqrt(Limits<double>::eps)) ? (1. - T) / t - T : t * (.5 - (1. / 3.) * t);
-}
-
-__device__ __forceinline__ void lw_layer(float t, float& T, float& fact)
-{
-    T = exp_neg_f32(t);
-    float q = 1.f / 840.f - t * (1.f / 5760.f);
-    q = 1.f / 144.f - t * q;
-    q = 1.f / 30.f - t * q;
-    q = 1.f / 8.f - t * q;
-    q = 1.f / 3.f - t * q;
-    q = .5f - t * q;
-    fact = (t < .25f) ? t * q : (1.f - T) * rcp_f32(t) - T;
-}
-
-// 1 - e^{-t} given T = e^{-t}, without the cancellation of 1 - T for small t.
-__device__ __forceinline__ double one_minus_T(double t, double) { return -expm1(-t); }
-__device__ __forceinline__ float one_minus_T(float t, float T) { return one_minus_exp_neg_f32(t, T); }
 
 // Heating-rate island: each layer's absorbed flux (net-flux convergence) summed over the group's
 // g-points in FP64, for the LDS slots kk < nslots of component comp (NCH components per slot),
@@ -463,15 +398,6 @@ __device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, d
 // variant (dead code otherwise).
 struct LayerSW { double R, T, SU, SD, Tn, ab, omTn; };
 
-// 1/x in FP64 for the adding denominators, x in (0, 1]: v_rcp_f64 (relative error up to 4.6e-8
-// on gfx950) and one Newton step, which leaves at most 2.3e-15, far below the FP32 rounding of
-// the fluxes and of the carried states' inputs.
-__device__ __forceinline__ double rcp_f64(double x)
-{
-    const double r = __builtin_amdgcn_rcp(x);
-    return fma(r, fma(-x, r, 1.), r);
-}
-
 // The layer properties in FP32, rearranged so that FP32 keeps its accuracy:
 //   k^2 = 2 (1 - w0) (gamma1 + gamma2)          (gamma1 - gamma2 = 2 (1 - w0) exactly),
 //   1 - e^{-2k tau} = a (2 - a) with a = 1 - e^{-k tau},
@@ -483,8 +409,8 @@ __device__ __forceinline__ double rcp_f64(double x)
 // where every O(1) term that cancels in the reference form has been cancelled analytically.  G is
 // e^{-tau/mu0} (tau/mu0) (e^x - 1)/x with x = tau (1 - k mu0) / mu0, from a series when |x| <= 1/2
 // and as the difference quotient otherwise.  a keeps its series for small k tau: where k is small
-// RT ~ 1/(2k) amplifies the absolute error of 1 - e^{-k tau}.  1 - e^{-tau/mu0} only bounds
-// Rdir + Tdir and is taken as it comes.  The transmittance e^{-tau/mu0} of this layer enters G
+// RT ~ 1/(2k) amplifies the absolute error of 1 - e^{-k tau}.  1 - e^{-tau/mu0}, the bound of
+// Rdir + Tdir, keeps its series too (see the clamp).  The transmittance e^{-tau/mu0} of this layer enters G
 // and Tdir only; the direct beam itself is carried by the caller (Beam<float>).  Returns false
 // where FP64 is kept: near conservative scattering (1 - w0 < 1e-6).
 __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, float mu0, float inv_mu0, double dir,
@@ -525,8 +451,10 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     const float rdf = s * ((alpha2 + k * gamma3) * c2 + 2.f * (k * gamma3 - alpha2 * k_mu) * e1 * G);
     const float tdf = -s * ((alpha1 - k * gamma4) * c2 * Tn - 2.f * k * (gamma4 + mu0 * alpha1) * G);
     // Clamps of rte_solver_kernels.cu:581-583 with the FP64 epsilon of the island they replace
-    // (representable in FP32).
-    const float b = 1.f - Tn;
+    // (representable in FP32).  The bound 1 - e^{-tau/mu0} is taken from its series for small
+    // tau/mu0: near conservative scattering Rdir + Tdir reaches it, so the clamp is active, and
+    // 1 - Tn in FP32 would cut Tdir of a thin layer by up to the FP32 spacing near 1 (6e-8).
+    const float b = one_minus_exp_neg_f32(y, Tn);
     const float eps = float(Limits<double>::eps);
     const float rd = fmaxf(eps, fminf(rdf, b));
     const float td = fmaxf(eps, fminf(tdf, b - rd));
@@ -534,7 +462,7 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
     // gamma1 - gamma2 = 2 (1 - w0): a sum of non-negative terms, free of the cancellation of 1 - R - T.
     const float ab = RT * (k * a * a + 2.f * one_m_w0 * c2);
     L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), double(rd) * dir, double(td) * dir, double(Tn),
-         double(ab), double(one_minus_exp_neg_f32(y, Tn))};
+         double(ab), double(b)};
     return true;
 }
 
@@ -975,18 +903,22 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     const dim3 grid(grid1(ncol, SB_COLS), unsigned(grp.n));
     const dim3 block(SB_COLS, SB_GPTS);
 
+    // Every element of the checkpoints, group partials and heating-rate parts is written by the
+    // kernel before it is read, so none of them needs clearing.
     Real *ckpt = nullptr, *scratch = nullptr;
     double* absorbed = nullptr;
-    RRTMGP_TRY(scratch_alloc(&ckpt, size_t(nseg * 3 * ngpt * ncol), s));
-    if (!spectral && !direct)
+    const size_t n_ck = size_t(nseg * 3 * ngpt * ncol);
+    const size_t n_part = (!spectral && !direct) ? size_t(ncv * grp.n * 3) : 0;
+    const size_t n_abs = heating_rate ? size_t(ncol * nlay * grp.n) : 0;
+    RRTMGP_TRY(scratch_alloc(&ckpt, n_ck, s));
+    if (n_part)
     {
-        const int rc0 = scratch_alloc(&scratch, size_t(ncv * grp.n * 3), s);
+        const int rc0 = scratch_alloc(&scratch, n_part, s);
         if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); return rc0; }
     }
-    if (heating_rate)
+    if (n_abs)
     {
-        // Per-group absorbed fluxes, FP64, every element written by the kernel.
-        const int rc0 = scratch_alloc(&absorbed, size_t(ncol * nlay * grp.n), s);
+        const int rc0 = scratch_alloc(&absorbed, n_abs, s);
         if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); scratch_free(scratch, s); return rc0; }
     }
     Real* gf[3] = {nullptr, nullptr, nullptr};

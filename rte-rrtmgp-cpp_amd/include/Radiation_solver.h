@@ -10,6 +10,7 @@
 
 #include <memory>
 
+#include "Aerosol_optics.h"
 #include "Array.h"
 #include "Cloud_optics.h"
 #include "Fluxes.h"
@@ -69,7 +70,8 @@ class Radiation_solver_shortwave_gpu
 {
     public:
         Radiation_solver_shortwave_gpu(std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist,
-                                       std::shared_ptr<Cloud_optics_gpu> cloud_optics);
+                                       std::shared_ptr<Cloud_optics_gpu> cloud_optics,
+                                       std::shared_ptr<Aerosol_optics_gpu> aerosol_optics = nullptr);
 
         // sfc_alb_dir/dif (nbnd, ncol), tsi_scaling and mu0 (ncol); sw_heating_rate as for the
         // longwave solver.
@@ -90,20 +92,48 @@ class Radiation_solver_shortwave_gpu
                 Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net,
                 Array_gpu<Float,2>* sw_heating_rate = nullptr);
 
+        // With aerosols (src_test/Radiation_solver.cu:794-810): after the clouds, the aerosol
+        // optical properties from rh (ncol, nlay) and aerosol_concs (aermr01..aermr11), delta-scaled
+        // when switch_delta_aerosol, are added to the gas + cloud properties.  The aerosol
+        // concentrations of a column block are those of its own columns (the reference's driver
+        // takes columns 1..n of every block, src_test/Radiation_solver.cpp:769, which only
+        // matters for column-varying aerosols and blocked runs).
+        void solve_gpu(
+                const bool switch_cloud_optics, const bool switch_aerosol_optics,
+                const bool switch_output_bnd_fluxes, const bool switch_delta_cloud,
+                const bool switch_delta_aerosol,
+                const Gas_concs_gpu& gas_concs,
+                const Array_gpu<Float,2>& p_lay, const Array_gpu<Float,2>& p_lev,
+                const Array_gpu<Float,2>& t_lay, const Array_gpu<Float,2>& t_lev,
+                const Array_gpu<Float,2>& col_dry,
+                const Array_gpu<Float,2>& sfc_alb_dir, const Array_gpu<Float,2>& sfc_alb_dif,
+                const Array_gpu<Float,1>& tsi_scaling, const Array_gpu<Float,1>& mu0,
+                const Array_gpu<Float,2>& lwp, const Array_gpu<Float,2>& iwp,
+                const Array_gpu<Float,2>& rel, const Array_gpu<Float,2>& rei,
+                const Array_gpu<Float,2>& rh, const Aerosol_concs_gpu& aerosol_concs,
+                Array_gpu<Float,2>& sw_flux_up, Array_gpu<Float,2>& sw_flux_dn,
+                Array_gpu<Float,2>& sw_flux_dn_dir, Array_gpu<Float,2>& sw_flux_net,
+                Array_gpu<Float,3>& sw_bnd_flux_up, Array_gpu<Float,3>& sw_bnd_flux_dn,
+                Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net,
+                Array_gpu<Float,2>* sw_heating_rate = nullptr);
+
         void set_n_col_block(int n) { n_col_block = n; }
         int get_n_gpt() const { return kdist->get_ngpt(); }
         int get_n_bnd() const { return kdist->get_nband(); }
         Float get_tsi() const { return kdist->get_tsi(); }
         Gas_optics_rrtmgp_gpu& get_kdist() { return *kdist; }
         Cloud_optics_gpu* get_cloud_optics() { return cloud_optics.get(); }
+        Aerosol_optics_gpu* get_aerosol_optics() { return aerosol_optics.get(); }
 
     private:
         std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist;
         std::shared_ptr<Cloud_optics_gpu> cloud_optics;
+        std::shared_ptr<Aerosol_optics_gpu> aerosol_optics;
         int n_col_block = 1 << 30;
         int ws_ncol = -1, ws_nlay = -1, ws_bnd = -1;
         std::unique_ptr<Optical_props_arry_gpu> optical_props;
         std::unique_ptr<Optical_props_2str_gpu> cloud_props;
+        std::unique_ptr<Optical_props_2str_gpu> aerosol_props;
         Array_gpu<Float,2> toa_src;
         Array_gpu<Float,2> blk_flux_up, blk_flux_dn, blk_flux_dir;
         Array_gpu<Float,3> blk_bnd_up, blk_bnd_dn, blk_bnd_dir;

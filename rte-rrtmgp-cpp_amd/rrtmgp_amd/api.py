@@ -53,6 +53,12 @@ class CloudLutC(C.Structure):
                 ("nsize_ice", C.c_int), ("lut_extice", _dp), ("lut_ssaice", _dp), ("lut_asyice", _dp)]
 
 
+class AerosolLutC(C.Structure):
+    _fields_ = [("nbnd", C.c_int), ("nhum", C.c_int), ("nphobic", C.c_int), ("nphilic", C.c_int),
+                ("band_lims_wvn", _dp), ("rh_upper", _dp), ("mext_phobic", _dp), ("ssa_phobic", _dp),
+                ("g_phobic", _dp), ("mext_philic", _dp), ("ssa_philic", _dp), ("g_philic", _dp)]
+
+
 class ApiGases(C.Structure):
     _fields_ = [("n", C.c_int), ("names", C.c_char_p), ("data", C.POINTER(C.c_void_p)),
                 ("dims", C.POINTER(C.c_int64))]
@@ -243,6 +249,47 @@ class Cloud(_Handle):
         self._check(self.L.rrtmgp_api_cloud_optics(self.h, ncol, nlay, lwp.ptr, iwp.ptr, rel.ptr, rei.ptr,
                                                    int(two_stream), int(delta_scale), tau.ptr, _p(ssa), _p(g)))
         return (tau, ssa, g) if two_stream else tau
+
+
+AEROSOL_FIELDS = tuple(f"aermr{i:02d}" for i in range(1, 12))
+
+
+class Aerosol(_Handle):
+    """Aerosol_optics_gpu from a dict of the tables load_and_init_aerosol_optics reads
+    (src_test/Radiation_solver.cpp:329-366): rh_upper (nhum), mext/ssa/g_phobic (nbnd, nphobic),
+    mext/ssa/g_philic (nbnd, nhum, nphilic); band_lims_wvn (2, nbnd) is optional (the reference's
+    loader leaves it unset)."""
+
+    def __init__(self, lut: dict, precision=8):
+        self.precision = precision
+        names = ("rh_upper", "mext_phobic", "ssa_phobic", "g_phobic", "mext_philic", "ssa_philic", "g_philic")
+        k = {n: np.asfortranarray(np.asarray(lut[n], dtype=np.float64)) for n in names}
+        nbnd, nphobic = k["mext_phobic"].shape
+        _, nhum, nphilic = k["mext_philic"].shape
+        blw = lut.get("band_lims_wvn")
+        k["band_lims_wvn"] = np.asfortranarray(np.asarray(
+            blw if blw is not None else np.stack([np.arange(nbnd), np.arange(1, nbnd + 1)]), dtype=np.float64))
+        self._keep = k
+        s = AerosolLutC(nbnd, nhum, nphobic, nphilic, _ptr(k["band_lims_wvn"]), *[_ptr(k[n]) for n in names])
+        h = C.c_void_p()
+        self._check(self.L.rrtmgp_api_aerosol_create(C.byref(s), C.byref(h)))
+        self.h = h
+        self.nbnd = nbnd
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.rrtmgp_api_aerosol_destroy(self.h)
+            self.h = None
+
+    def optics(self, concs: GasConcs, rh, plev, delta_scale=False):
+        """tau, ssa, g (ncol, nlay, nbnd) of the aerosol mixing ratios held by concs."""
+        dt = real_dtype(self.precision)
+        rh, plev = _dev(rh, dt), _dev(plev, dt)
+        ncol, nlay = rh.shape
+        out = [DeviceArray((ncol, nlay, self.nbnd), dt) for _ in range(3)]
+        self._check(self.L.rrtmgp_api_aerosol_optics(self.h, ncol, nlay, concs.h, rh.ptr, plev.ptr,
+                                                     int(delta_scale), *[o.ptr for o in out]))
+        return tuple(out)
 
 
 def get_col_dry(vmr_h2o, plev, precision=8):
@@ -461,16 +508,19 @@ class SolverLW(_Solver):
 class SolverSW(_Solver):
     """Radiation_solver_shortwave_gpu."""
 
-    def __init__(self, kd: KDist, cloud: Cloud | None = None):
+    def __init__(self, kd: KDist, cloud: Cloud | None = None, aerosol: Aerosol | None = None):
         self.precision = kd.precision
-        self.kd, self.cloud = kd, cloud
+        self.kd, self.cloud, self.aerosol = kd, cloud, aerosol
         h = C.c_void_p()
-        self._check(self.L.rrtmgp_api_solver_sw_create(kd.h, cloud.h if cloud else None, C.byref(h)))
+        self._check(self.L.rrtmgp_api_solver_sw_create_aerosol(kd.h, cloud.h if cloud else None,
+                                                               aerosol.h if aerosol else None, C.byref(h)))
         self.h = h
 
     def solve(self, da: DeviceAtmosphere, tsi_scaling, bnd=False, delta_cloud=True, col_dry=None, outputs=None,
-              heating_rate=False):
-        """As SolverLW.solve, with the direct flux; heating_rate as there."""
+              heating_rate=False, aerosol=None, delta_aerosol=False):
+        """As SolverLW.solve, with the direct flux; heating_rate as there.  aerosol = (rh, concs):
+        relative humidity (ncol, nlay) and a GasConcs of aermr01..aermr11, added after the clouds
+        (delta-scaled with delta_aerosol)."""
         dt = real_dtype(self.precision)
         ncol, nlay = da.ncol, da.nlay
         nlev = nlay + 1
@@ -483,11 +533,15 @@ class SolverSW(_Solver):
         if heating_rate and o.get("heating_rate") is None:
             o["heating_rate"] = DeviceArray((ncol, nlay), dt)
         cloudy = da.cloudy and self.cloud is not None
-        self._check(self.L.rrtmgp_api_solve_sw(
-            self.h, ncol, nlay, int(cloudy), int(bnd), int(delta_cloud), da.p_lay.ptr, da.p_lev.ptr,
+        rh, aconcs = (None, None) if aerosol is None else aerosol
+        rh = _dev(rh, dt) if aerosol is not None else None
+        self._check(self.L.rrtmgp_api_solve_sw_aerosol(
+            self.h, ncol, nlay, int(cloudy), int(aerosol is not None), int(bnd), int(delta_cloud),
+            int(delta_aerosol), da.p_lay.ptr, da.p_lev.ptr,
             da.t_lay.ptr, da.t_lev.ptr, da.gas_concs.h, _p(col_dry), da.sfc_alb.ptr, da.sfc_alb.ptr,
             tsi.ptr, da.mu0.ptr, _p(da.lwp) if cloudy else None, _p(da.iwp) if cloudy else None,
             _p(da.rel) if cloudy else None, _p(da.rei) if cloudy else None,
+            _p(rh), aconcs.h if aconcs is not None else None,
             o["flux_up"].ptr, o["flux_dn"].ptr, o["flux_dn_dir"].ptr, o["flux_net"].ptr,
             _p(o.get("bnd_flux_up")), _p(o.get("bnd_flux_dn")), _p(o.get("bnd_flux_dn_dir")),
             _p(o.get("bnd_flux_net")), _p(o.get("heating_rate"))))

@@ -195,6 +195,28 @@ def cloud_from_schema(s: Schema) -> syn.CloudLUT:
 
 
 # ---------------------------------------------------------------------------------------------
+# aerosol look-up tables (src_test/Radiation_solver.cpp:329-366)
+
+def aerosol_lut(ds) -> dict:
+    """The tables load_and_init_aerosol_optics reads from aerosol_optics.nc, in the layouts the
+    Aerosol_optics constructor takes: rh_upper (nhum) from relative_humidity2, mext/ssa/g_phobic
+    (nbnd, nphobic) and mext/ssa/g_philic (nbnd, nhum, nphilic) from the *_sw_* variables.  ds is a
+    netcdf.Dataset (or anything with get_dimension_size and get_variable(name, dims))."""
+    n_band = ds.get_dimension_size("band_sw")
+    n_hum = ds.get_dimension_size("relative_humidity")
+    n_philic = ds.get_dimension_size("hydrophilic")
+    n_phobic = ds.get_dimension_size("hydrophobic")
+    out = {"rh_upper": np.asarray(ds.get_variable("relative_humidity2", (n_hum,)), dtype=np.float64)}
+    for key, name in (("mext", "mass_ext"), ("ssa", "ssa"), ("g", "asymmetry")):
+        phobic = ds.get_variable(f"{name}_sw_hydrophobic", (n_phobic, n_band))
+        philic = ds.get_variable(f"{name}_sw_hydrophilic", (n_philic, n_hum, n_band))
+        # C order over the NetCDF dimensions = Fortran order over the reversed ones.
+        out[f"{key}_phobic"] = np.asfortranarray(np.asarray(phobic, dtype=np.float64).T)
+        out[f"{key}_philic"] = np.asfortranarray(np.asarray(philic, dtype=np.float64).T)
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
 # bundles
 
 def write_bundle(path: str, s: Schema):

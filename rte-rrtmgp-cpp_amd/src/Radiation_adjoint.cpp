@@ -106,7 +106,7 @@ void lw_adjoint_gpu(Gas_optics_rrtmgp_gpu& kdist, Cloud_optics_gpu* cloud_optics
     w_h({1}) = Float(0.5);
     Array_gpu<Float,1> weights(w_h);
 
-    Array_gpu<Float,3> d_col({ncol, nlay, ngas + 1});
+    Array_gpu<Float,3> d_col({ncol, nlay, ngas + 2});
     Array_gpu<Float,2> d_lo({ncol, nlay}), d_hi({ncol, nlay}), d_col_dry({ncol, nlay}), d_emis_gpt({ncol, ngpt});
     zero(d_col); zero(d_lo); zero(d_hi); zero(out.d_t_lay); zero(out.d_t_sfc); zero(out.d_lnp_lay); zero(out.d_p_lev);
 
@@ -184,7 +184,7 @@ void sw_adjoint_gpu(Gas_optics_rrtmgp_gpu& kdist, Cloud_optics_gpu* cloud_optics
     hip_check(K<Float>::scale_by_col(ncol, ngpt, toa.ptr(), tsi_scaling.ptr(), stream()));
     Array_gpu<Float,2> ad_gpt = expand(kdist, sfc_alb_dir, ncol), af_gpt = expand(kdist, sfc_alb_dif, ncol);
 
-    Array_gpu<Float,3> d_col({ncol, nlay, ngas + 1});
+    Array_gpu<Float,3> d_col({ncol, nlay, ngas + 2});
     Array_gpu<Float,2> d_col_dry({ncol, nlay}), d_ad({ncol, ngpt}), d_af({ncol, ngpt});
     zero(d_col); zero(out.d_t_lay); zero(out.d_lnp_lay); zero(out.d_p_lev);
 

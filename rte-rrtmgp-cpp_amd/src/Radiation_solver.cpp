@@ -38,7 +38,7 @@ void scatter(const Float* blk, int n, int64_t rows, Float* full, int ncol, int c
 
 struct BlockInputs
 {
-    Array_gpu<Float,2> p_lay, p_lev, t_lay, t_lev, col_dry, lwp, iwp, rel, rei;
+    Array_gpu<Float,2> p_lay, p_lev, t_lay, t_lev, col_dry, lwp, iwp, rel, rei, rh;
 };
 }  // namespace
 
@@ -186,8 +186,9 @@ void Radiation_solver_longwave_gpu::solve_gpu(
 
 
 Radiation_solver_shortwave_gpu::Radiation_solver_shortwave_gpu(
-        std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist, std::shared_ptr<Cloud_optics_gpu> cloud_optics) :
-    kdist(std::move(kdist)), cloud_optics(std::move(cloud_optics))
+        std::shared_ptr<Gas_optics_rrtmgp_gpu> kdist, std::shared_ptr<Cloud_optics_gpu> cloud_optics,
+        std::shared_ptr<Aerosol_optics_gpu> aerosol_optics) :
+    kdist(std::move(kdist)), cloud_optics(std::move(cloud_optics)), aerosol_optics(std::move(aerosol_optics))
 {}
 
 void Radiation_solver_shortwave_gpu::solve_gpu(
@@ -206,10 +207,38 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net,
         Array_gpu<Float,2>* sw_heating_rate)
 {
+    solve_gpu(switch_cloud_optics, false, switch_output_bnd_fluxes, switch_delta_cloud, false, gas_concs, p_lay,
+              p_lev, t_lay, t_lev, col_dry, sfc_alb_dir, sfc_alb_dif, tsi_scaling, mu0, lwp, iwp, rel, rei,
+              Array_gpu<Float,2>(), Aerosol_concs_gpu(), sw_flux_up, sw_flux_dn, sw_flux_dn_dir, sw_flux_net,
+              sw_bnd_flux_up, sw_bnd_flux_dn, sw_bnd_flux_dn_dir, sw_bnd_flux_net, sw_heating_rate);
+}
+
+void Radiation_solver_shortwave_gpu::solve_gpu(
+        const bool switch_cloud_optics, const bool switch_aerosol_optics, const bool switch_output_bnd_fluxes,
+        const bool switch_delta_cloud, const bool switch_delta_aerosol,
+        const Gas_concs_gpu& gas_concs,
+        const Array_gpu<Float,2>& p_lay, const Array_gpu<Float,2>& p_lev,
+        const Array_gpu<Float,2>& t_lay, const Array_gpu<Float,2>& t_lev,
+        const Array_gpu<Float,2>& col_dry,
+        const Array_gpu<Float,2>& sfc_alb_dir, const Array_gpu<Float,2>& sfc_alb_dif,
+        const Array_gpu<Float,1>& tsi_scaling, const Array_gpu<Float,1>& mu0,
+        const Array_gpu<Float,2>& lwp, const Array_gpu<Float,2>& iwp,
+        const Array_gpu<Float,2>& rel, const Array_gpu<Float,2>& rei,
+        const Array_gpu<Float,2>& rh, const Aerosol_concs_gpu& aerosol_concs,
+        Array_gpu<Float,2>& sw_flux_up, Array_gpu<Float,2>& sw_flux_dn,
+        Array_gpu<Float,2>& sw_flux_dn_dir, Array_gpu<Float,2>& sw_flux_net,
+        Array_gpu<Float,3>& sw_bnd_flux_up, Array_gpu<Float,3>& sw_bnd_flux_dn,
+        Array_gpu<Float,3>& sw_bnd_flux_dn_dir, Array_gpu<Float,3>& sw_bnd_flux_net,
+        Array_gpu<Float,2>* sw_heating_rate)
+{
     const bool hr = sw_heating_rate != nullptr && sw_heating_rate->size() > 0;
     const int ncol = p_lay.dim(1), nlay = p_lay.dim(2), nlev = nlay + 1;
     const int ngpt = kdist->get_ngpt(), nbnd = kdist->get_nband();
     if (switch_cloud_optics && !cloud_optics) throw std::runtime_error("cloud optics requested but not configured");
+    if (switch_aerosol_optics && !aerosol_optics)
+        throw std::runtime_error("aerosol optics requested but not configured");
+    if (switch_aerosol_optics && (rh.dim(1) != ncol || rh.dim(2) != nlay))
+        throw std::runtime_error("aerosol optics: rh must be (ncol, nlay)");
     const Bool top_at_1 = p_lay({1, 1}) < p_lay({1, nlay});
     const int nblk = std::max(1, std::min(n_col_block, ncol));
 
@@ -218,6 +247,8 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         optical_props = std::make_unique<Optical_props_2str_gpu>(nblk, nlay, *kdist);
         cloud_props.reset();
         if (cloud_optics) cloud_props = std::make_unique<Optical_props_2str_gpu>(nblk, nlay, *cloud_optics);
+        aerosol_props.reset();
+        if (aerosol_optics) aerosol_props = std::make_unique<Optical_props_2str_gpu>(nblk, nlay, *aerosol_optics);
         toa_src = Array_gpu<Float,2>({nblk, ngpt});
         const bool partial = nblk < ncol;
         blk_flux_up = partial ? Array_gpu<Float,2>({nblk, nlev}) : Array_gpu<Float,2>();
@@ -237,8 +268,12 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         const int n = std::min(nblk, ncol - c0);
         if (n != nblk)
         {
-            Radiation_solver_shortwave_gpu tail(kdist, cloud_optics);
+            Radiation_solver_shortwave_gpu tail(kdist, cloud_optics, aerosol_optics);
             Gas_concs_gpu gsub(gas_concs, c0 + 1, n);
+            Aerosol_concs_gpu asub = switch_aerosol_optics ? Aerosol_concs_gpu(aerosol_concs, c0 + 1, n)
+                                                           : Aerosol_concs_gpu();
+            Array_gpu<Float,2> rh_ws;
+            Array_gpu<Float,2> rhb = switch_aerosol_optics ? block_of(rh, c0, n, rh_ws) : Array_gpu<Float,2>();
             Array_gpu<Float,2> pl = block_of(p_lay, c0, n, ws.p_lay), pv = block_of(p_lev, c0, n, ws.p_lev);
             Array_gpu<Float,2> tl = block_of(t_lay, c0, n, ws.t_lay), tv = block_of(t_lev, c0, n, ws.t_lev);
             Array_gpu<Float,2> cd = block_of(col_dry, c0, n, ws.col_dry);
@@ -256,8 +291,9 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
                 bdir = Array_gpu<Float,3>({n, nlev, nbnd}); bn = Array_gpu<Float,3>({n, nlev, nbnd});
             }
             Array_gpu<Float,2> th = hr ? Array_gpu<Float,2>({n, nlay}) : Array_gpu<Float,2>();
-            tail.solve_gpu(switch_cloud_optics, switch_output_bnd_fluxes, switch_delta_cloud, gsub, pl, pv, tl, tv,
-                           cd, ad, af, tsi, m0, a, b, c, d, fu, fd, fdir, fn, bu, bd, bdir, bn, hr ? &th : nullptr);
+            tail.solve_gpu(switch_cloud_optics, switch_aerosol_optics, switch_output_bnd_fluxes, switch_delta_cloud,
+                           switch_delta_aerosol, gsub, pl, pv, tl, tv, cd, ad, af, tsi, m0, a, b, c, d, rhb, asub,
+                           fu, fd, fdir, fn, bu, bd, bdir, bn, hr ? &th : nullptr);
             if (hr) scatter(th.ptr(), n, nlay, sw_heating_rate->ptr(), ncol, c0);
             scatter(fu.ptr(), n, nlev, sw_flux_up.ptr(), ncol, c0);
             scatter(fd.ptr(), n, nlev, sw_flux_dn.ptr(), ncol, c0);
@@ -302,6 +338,15 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
             cloud = cloud_props.get();
         }
         kdist->gas_optics_with_clouds(pl, pv, tl, gc, *optical_props, toa_src, cd, cloud);
+        if (switch_aerosol_optics)
+        {
+            std::unique_ptr<Aerosol_concs_gpu> asub;
+            if (!whole) asub = std::make_unique<Aerosol_concs_gpu>(aerosol_concs, c0 + 1, n);
+            Array_gpu<Float,2> rhb = block_of(rh, c0, n, ws.rh);
+            const Aerosol_concs_gpu& ac = whole ? aerosol_concs : *asub;
+            aerosol_optics->compute(ac, rhb, pv, switch_delta_aerosol, *aerosol_props);
+            add_to(dynamic_cast<Optical_props_2str_gpu&>(*optical_props), *aerosol_props);
+        }
         // TSI scaling of the incoming flux (src_test/Radiation_solver.cpp:739-743).
         hip_check(K<Float>::scale_by_col(n, ngpt, toa_src.ptr(), tsi.ptr(), stream()));
 

@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "Aerosol_optics.h"
 #include "Cloud_optics.h"
 #include "Fluxes.h"
 #include "Gas_concs.h"
@@ -78,6 +79,7 @@ Array<Bool,1> bool_arr(const signed char* p, int n)
 
 struct Kdist { std::shared_ptr<Gas_optics_rrtmgp_gpu> k; bool is_lw; };
 struct Cloud { std::shared_ptr<Cloud_optics_gpu> c; };
+struct Aerosol { std::shared_ptr<Aerosol_optics_gpu> a; };
 struct Solver { std::unique_ptr<Radiation_solver_longwave_gpu> lw; std::unique_ptr<Radiation_solver_shortwave_gpu> sw; };
 
 template<int N>
@@ -363,6 +365,40 @@ int rrtmgp_api_cloud_create(const rrtmgp_cloud_lut* l, void** out)
 
 int rrtmgp_api_cloud_destroy(void* h) { delete static_cast<Cloud*>(h); return RRTMGP_OK; }
 
+int rrtmgp_api_aerosol_create(const rrtmgp_aerosol_lut* l, void** out)
+{
+    return guarded([&] {
+        if (!l || !out) throw std::invalid_argument("aerosol_create: NULL argument");
+        std::unique_ptr<Aerosol> a(new Aerosol);
+        a->a = std::make_shared<Aerosol_optics_gpu>(
+            real_arr<2>(l->band_lims_wvn, {2, l->nbnd}), real_arr<1>(l->rh_upper, {l->nhum}),
+            real_arr<2>(l->mext_phobic, {l->nbnd, l->nphobic}), real_arr<2>(l->ssa_phobic, {l->nbnd, l->nphobic}),
+            real_arr<2>(l->g_phobic, {l->nbnd, l->nphobic}),
+            real_arr<3>(l->mext_philic, {l->nbnd, l->nhum, l->nphilic}),
+            real_arr<3>(l->ssa_philic, {l->nbnd, l->nhum, l->nphilic}),
+            real_arr<3>(l->g_philic, {l->nbnd, l->nhum, l->nphilic}));
+        *out = a.release();
+    });
+}
+
+int rrtmgp_api_aerosol_destroy(void* h) { delete static_cast<Aerosol*>(h); return RRTMGP_OK; }
+
+int rrtmgp_api_aerosol_optics(void* h, int ncol, int nlay, void* concs, const rrtmgp_real* rh,
+                              const rrtmgp_real* plev, int delta_scale, rrtmgp_real* tau, rrtmgp_real* ssa,
+                              rrtmgp_real* g)
+{
+    return guarded([&] {
+        auto& a = *static_cast<Aerosol*>(h)->a;
+        const int nb = a.get_nband();
+        Optical_props_2str_gpu p(0, 0, a);
+        p.get_tau() = Array_gpu<Float,3>(tau, {ncol, nlay, nb});
+        p.get_ssa() = Array_gpu<Float,3>(ssa, {ncol, nlay, nb});
+        p.get_g() = Array_gpu<Float,3>(g, {ncol, nlay, nb});
+        a.compute(*static_cast<Gas_concs_gpu*>(concs), view<2>(rh, {ncol, nlay}), view<2>(plev, {ncol, nlay + 1}),
+                  delta_scale != 0, p);
+    });
+}
+
 int rrtmgp_api_cloud_optics(void* h, int ncol, int nlay, const rrtmgp_real* lwp, const rrtmgp_real* iwp,
                             const rrtmgp_real* rel, const rrtmgp_real* rei, int two_stream, int delta_scale,
                             rrtmgp_real* tau, rrtmgp_real* ssa, rrtmgp_real* g)
@@ -563,6 +599,16 @@ int rrtmgp_api_solver_sw_create(void* kd, void* cloud, void** out)
     });
 }
 
+int rrtmgp_api_solver_sw_create_aerosol(void* kd, void* cloud, void* aerosol, void** out)
+{
+    return guarded([&] {
+        std::unique_ptr<Solver> s(new Solver);
+        s->sw.reset(new Radiation_solver_shortwave_gpu(K_(kd).k, cloud ? static_cast<Cloud*>(cloud)->c : nullptr,
+                                                       aerosol ? static_cast<Aerosol*>(aerosol)->a : nullptr));
+        *out = s.release();
+    });
+}
+
 int rrtmgp_api_solver_destroy(void* h) { delete static_cast<Solver*>(h); return RRTMGP_OK; }
 
 int rrtmgp_api_solver_set_block(void* h, int n)
@@ -637,6 +683,44 @@ int rrtmgp_api_solve_sw(void* h, int ncol, int nlay, int cloudy, int bnd, int de
     });
 }
 
+
+int rrtmgp_api_solve_sw_aerosol(void* h, int ncol, int nlay, int cloudy, int aerosol, int bnd, int delta_cloud,
+                                int delta_aerosol, const rrtmgp_real* play, const rrtmgp_real* plev,
+                                const rrtmgp_real* tlay, const rrtmgp_real* tlev, void* gas_concs,
+                                const rrtmgp_real* col_dry, const rrtmgp_real* sfc_alb_dir,
+                                const rrtmgp_real* sfc_alb_dif, const rrtmgp_real* tsi_scaling,
+                                const rrtmgp_real* mu0, const rrtmgp_real* lwp, const rrtmgp_real* iwp,
+                                const rrtmgp_real* rel, const rrtmgp_real* rei, const rrtmgp_real* rh,
+                                void* aerosol_concs, rrtmgp_real* flux_up, rrtmgp_real* flux_dn,
+                                rrtmgp_real* flux_dir, rrtmgp_real* flux_net, rrtmgp_real* bnd_up,
+                                rrtmgp_real* bnd_dn, rrtmgp_real* bnd_dir, rrtmgp_real* bnd_net,
+                                rrtmgp_real* heating_rate)
+{
+    return guarded([&] {
+        auto* s = static_cast<Solver*>(h);
+        if (!s->sw) throw std::invalid_argument("not a shortwave solver");
+        if (aerosol && (!rh || !aerosol_concs)) throw std::invalid_argument("aerosols need rh and aerosol_concs");
+        auto hr = heating_rate ? view<2>(heating_rate, {ncol, nlay}) : Array_gpu<Float,2>();
+        const int nlev = nlay + 1, nbnd = s->sw->get_n_bnd();
+        auto fu = view<2>(flux_up, {ncol, nlev}), fd = view<2>(flux_dn, {ncol, nlev});
+        auto fdir = view<2>(flux_dir, {ncol, nlev}), fn = view<2>(flux_net, {ncol, nlev});
+        auto bu = bnd ? view<3>(bnd_up, {ncol, nlev, nbnd}) : Array_gpu<Float,3>();
+        auto bd = bnd ? view<3>(bnd_dn, {ncol, nlev, nbnd}) : Array_gpu<Float,3>();
+        auto bdir = bnd ? view<3>(bnd_dir, {ncol, nlev, nbnd}) : Array_gpu<Float,3>();
+        auto bn = bnd ? view<3>(bnd_net, {ncol, nlev, nbnd}) : Array_gpu<Float,3>();
+        auto c = [&](const rrtmgp_real* p) { return cloudy ? view<2>(p, {ncol, nlay}) : Array_gpu<Float,2>(); };
+        const Aerosol_concs_gpu empty;
+        s->sw->solve_gpu(cloudy != 0, aerosol != 0, bnd != 0, delta_cloud != 0, delta_aerosol != 0,
+                         *static_cast<Gas_concs_gpu*>(gas_concs),
+                         view<2>(play, {ncol, nlay}), view<2>(plev, {ncol, nlev}), view<2>(tlay, {ncol, nlay}),
+                         view<2>(tlev, {ncol, nlev}), col_dry ? view<2>(col_dry, {ncol, nlay}) : Array_gpu<Float,2>(),
+                         view<2>(sfc_alb_dir, {nbnd, ncol}), view<2>(sfc_alb_dif, {nbnd, ncol}),
+                         view<1>(tsi_scaling, {ncol}), view<1>(mu0, {ncol}), c(lwp), c(iwp), c(rel), c(rei),
+                         aerosol ? view<2>(rh, {ncol, nlay}) : Array_gpu<Float,2>(),
+                         aerosol ? *static_cast<Gas_concs_gpu*>(aerosol_concs) : empty,
+                         fu, fd, fdir, fn, bu, bd, bdir, bn, heating_rate ? &hr : nullptr);
+    });
+}
 
 int rrtmgp_api_adjoint_lw(void* h, int ncol, int nlay, int cloudy, const rrtmgp_real* play,
                           const rrtmgp_real* plev, const rrtmgp_real* tlay, const rrtmgp_real* tlev,

@@ -32,6 +32,7 @@ template<typename T> struct K;
     static constexpr auto get_col_dry = rrtmgp_hip_get_col_dry_##S;      \
     static constexpr auto fill_col_gas = rrtmgp_hip_fill_col_gas_##S;    \
     static constexpr auto cloud_optics = rrtmgp_hip_cloud_optics_##S;    \
+    static constexpr auto aerosol_optics = rrtmgp_hip_aerosol_optics_##S; \
     static constexpr auto increment = rrtmgp_hip_increment_##S;          \
     static constexpr auto inc_bybnd = rrtmgp_hip_inc_bybnd_##S;          \
     static constexpr auto delta_scale_2str = rrtmgp_hip_delta_scale_2str_##S; \

@@ -1,0 +1,193 @@
+"""GPU parity of the longwave solvers with scattering (rrtmgp_hip_lw_solver_noscat_rescaled_*,
+rrtmgp_hip_lw_solver_2stream_*) against the FP64 oracle, through the thin C ABI.
+
+Inputs: the oracle's LW gas optics of a synthetic atmosphere (so sources and optical depths have
+their real spread over 256 g-points and 60 layers), with ssa and g drawn per element as a cloud
+would give them, including zero-ssa elements.
+Bars: FP64 within 1e-10 relative; FP32 within 1e-4 relative of the largest flux of the column
+(FP32 rounding of inputs and sources, summed down the column).
+
+The FP64 cases keep every layer's optical depth outside (1e-8, 1e-5).  The stress case adds
+layers with tau just above the 1e-8 threshold of lw_source_2str (some g-points reach it by
+themselves at the model top) and near-conservative scattering (ssa = 1 - 1e-7).  There the reference's expressions cancel
+catastrophically: Z = (bot - top) / (tau (g1 + g2)) reaches 1e7 while 1 + Rdif - Tdif is O(tau),
+and g1 - g2 is formed by subtracting O(1) terms.  On such a column the FP64 oracle is 2e-6
+relative away from the same expressions evaluated in 80-bit arithmetic, and two FP64 evaluations
+that differ only in the exp and sqrt implementations differ by 3e-8 -- so no FP64 bar below
+~1e-6 can hold there, for any implementation of the reference's arithmetic.  The stress case is
+therefore held to the FP32 bar, in both builds; the FP32 build evaluates these layers in a
+rearranged form free of the cancellation.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+_GAUSS = {1: ([1.66], [0.5]), 3: ([1.09719858, 1.69338507, 4.70941630], [0.2009319137, 0.2292411064, 0.0698269799])}
+
+
+@pytest.fixture(scope="module")
+def G():
+    from rrtmgp_amd import api, device
+    if device.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return api
+
+
+def _case(syn, O, okd_lw, top_at_1, seed=41, ncol=40, nlay=60, stress=False):
+    atm = syn.make_atmosphere(seed, ncol, nlay, top_at_1=top_at_1)
+    go = O.gas_optics(okd_lw, atm)
+    rng = np.random.default_rng(seed)
+    shape = go["tau"].shape
+    ssa = rng.uniform(0.0, 0.999, shape)
+    g = rng.uniform(0.0, 0.9, shape)
+    ssa[rng.uniform(size=shape) < 0.05] = 0.0
+    tau = go["tau"].copy()
+    thin = rng.uniform(size=shape) < 0.05
+    if stress:
+        ssa[rng.uniform(size=shape) < 0.02] = 1.0 - 1e-7
+        tau[thin] *= 1e-6                                        # tau just above 1e-8 in places
+    else:
+        # Below the source threshold; so are the layers whose own optical depth lies where the
+        # Toon source cancels (1e-8 < tau < 1e-5, reached at the model top by weak g-points).
+        tau[thin | ((tau > 1e-8) & (tau < 1e-5))] = 1e-9
+    bl = np.asarray(okd_lw.band_lims)
+    band_of = np.concatenate([np.full(bl[1, b] - bl[0, b] + 1, b) for b in range(bl.shape[1])])
+    emis_gpt = np.ascontiguousarray(np.asarray(atm.emis_sfc)[band_of, :].T)
+    inc = rng.uniform(0.0, 0.5, (shape[0], shape[2]))
+    return dict(atm=atm, go=go, tau=tau, ssa=ssa, g=g, bl=bl, emis_gpt=emis_gpt, inc=inc)
+
+
+def _dev(G, a, dt):
+    return G.DeviceArray.from_host(np.asfortranarray(np.asarray(a, dtype=dt)), dt)
+
+
+def _host(d):
+    return d.to_host().astype(np.float64)
+
+
+def _col_scale(ref):
+    """Largest |flux| of each column, broadcast to ref's shape (FP32 error yardstick)."""
+    m = np.abs(ref).reshape(ref.shape[0], -1).max(axis=1)
+    return np.broadcast_to(m.reshape((-1,) + (1,) * (ref.ndim - 1)), ref.shape)
+
+
+def _check(got, ref, prec, what):
+    if prec == 8:
+        assert rel_err(got, ref, floor=1e-6) < 1e-10, what
+    else:
+        err = np.abs(got - ref) / _col_scale(ref)
+        assert err.max() < 1e-4, f"{what}: {err.max():.3e}"
+
+
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("nmu", [1, 3])
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_lw_rescaled_spectral(G, syn, O, okd_lw, top_at_1, nmu, prec):
+    from rrtmgp_amd import _lib
+    c = _case(syn, O, okd_lw, top_at_1)
+    go, tau = c["go"], c["tau"]
+    ncol, nlay, ngpt = tau.shape
+    dt = np.float64 if prec == 8 else np.float32
+    ds, ws = _GAUSS[nmu]
+    sec = np.empty((ncol, ngpt, nmu))
+    for mu in range(nmu):
+        sec[:, :, mu] = ds[mu]
+    d = {k: _dev(G, v, dt) for k, v in dict(sec=sec, w=ws, tau=tau, ssa=c["ssa"], g=c["g"], lay=go["lay_src"],
+                                            linc=go["lev_src_inc"], ldec=go["lev_src_dec"], emis=c["emis_gpt"],
+                                            sfc=go["sfc_src"], inc=c["inc"]).items()}
+    up, dn = G.DeviceArray((ncol, nlay + 1, ngpt), dt), G.DeviceArray((ncol, nlay + 1, ngpt), dt)
+    S = "f64" if prec == 8 else "f32"
+    i = C.c_int64
+    _lib.check_hip(getattr(_lib.hip(), "rrtmgp_hip_lw_solver_noscat_rescaled_" + S)(
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["ssa"].ptr, d["g"].ptr, d["lay"].ptr, d["linc"].ptr, d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr,
+        d["inc"].ptr, up.ptr, dn.ptr, C.c_int(0), None, None, C.c_int(0), None, None, None))
+    # The inputs as the device holds them (FP32-rounded in the FP32 build).
+    r = lambda a: np.asarray(a, dtype=dt).astype(np.float64)     # noqa: E731
+    rup, rdn = O.rte_lw_rescaled(r(tau), r(c["ssa"]), r(c["g"]), r(go["lay_src"]), r(go["lev_src_inc"]),
+                                 r(go["lev_src_dec"]), r(go["sfc_src"]), r(c["atm"].emis_sfc), c["bl"], top_at_1,
+                                 inc_flux=r(c["inc"]), n_gauss_angles=nmu)
+    _check(_host(up), rup, prec, "flux_up")
+    _check(_host(dn), rdn, prec, "flux_dn")
+
+
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_lw_rescaled_broadband_jacobian_fp64(G, syn, O, okd_lw, top_at_1):
+    from rrtmgp_amd import _lib
+    c = _case(syn, O, okd_lw, top_at_1, seed=43)
+    go, tau = c["go"], c["tau"]
+    ncol, nlay, ngpt = tau.shape
+    nmu = 3
+    ds, ws = _GAUSS[nmu]
+    sec = np.empty((ncol, ngpt, nmu))
+    for mu in range(nmu):
+        sec[:, :, mu] = ds[mu]
+    jac_src = np.random.default_rng(3).uniform(0.01, 0.1, (ncol, ngpt))
+    dt = np.float64
+    d = {k: _dev(G, v, dt) for k, v in dict(sec=sec, w=ws, tau=tau, ssa=c["ssa"], g=c["g"], lay=go["lay_src"],
+                                            linc=go["lev_src_inc"], ldec=go["lev_src_dec"], emis=c["emis_gpt"],
+                                            sfc=go["sfc_src"], jsrc=jac_src).items()}
+    out = {k: G.DeviceArray((ncol, nlay + 1), dt) for k in ("up", "dn", "jac")}
+    i = C.c_int64
+    _lib.check_hip(_lib.hip().rrtmgp_hip_lw_solver_noscat_rescaled_f64(
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), i(nmu), d["sec"].ptr, d["w"].ptr, d["tau"].ptr,
+        d["ssa"].ptr, d["g"].ptr, d["lay"].ptr, d["linc"].ptr, d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr,
+        None, None, None, C.c_int(1), out["up"].ptr, out["dn"].ptr, C.c_int(1), d["jsrc"].ptr, out["jac"].ptr,
+        None))
+    bu, bd, jac = O.rte_lw_rescaled(tau, c["ssa"], c["g"], go["lay_src"], go["lev_src_inc"], go["lev_src_dec"],
+                                    go["sfc_src"], c["atm"].emis_sfc, c["bl"], top_at_1, n_gauss_angles=nmu,
+                                    do_broadband=True, sfc_src_jac=jac_src)
+    assert rel_err(_host(out["up"]), bu) < 1e-10
+    assert rel_err(_host(out["dn"]), bd, floor=1e-6) < 1e-10
+    assert rel_err(_host(out["jac"]), jac) < 1e-10
+
+
+@pytest.mark.parametrize("stress", [False, True])
+@pytest.mark.parametrize("prec", [8, 4])
+@pytest.mark.parametrize("top_at_1", [False, True])
+def test_lw_2stream(G, syn, O, okd_lw, top_at_1, prec, stress):
+    from rrtmgp_amd import _lib
+    c = _case(syn, O, okd_lw, top_at_1, seed=45, stress=stress)
+    go, tau = c["go"], c["tau"]
+    ncol, nlay, ngpt = tau.shape
+    dt = np.float64 if prec == 8 else np.float32
+    d = {k: _dev(G, v, dt) for k, v in dict(tau=tau, ssa=c["ssa"], g=c["g"], lay=go["lay_src"],
+                                            linc=go["lev_src_inc"], ldec=go["lev_src_dec"], emis=c["emis_gpt"],
+                                            sfc=go["sfc_src"], inc=c["inc"]).items()}
+    up, dn = G.DeviceArray((ncol, nlay + 1, ngpt), dt), G.DeviceArray((ncol, nlay + 1, ngpt), dt)
+    S = "f64" if prec == 8 else "f32"
+    i = C.c_int64
+    _lib.check_hip(getattr(_lib.hip(), "rrtmgp_hip_lw_solver_2stream_" + S)(
+        i(ncol), i(nlay), i(ngpt), C.c_int(int(top_at_1)), d["tau"].ptr, d["ssa"].ptr, d["g"].ptr, d["lay"].ptr,
+        d["linc"].ptr, d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr, d["inc"].ptr, up.ptr, dn.ptr, None))
+    r = lambda a: np.asarray(a, dtype=dt).astype(np.float64)     # noqa: E731
+    rup, rdn = O.lw_solver_2stream(r(tau), r(c["ssa"]), r(c["g"]), r(go["lev_src_inc"]), r(go["lev_src_dec"]),
+                                   r(c["emis_gpt"]), r(go["sfc_src"]), top_at_1, inc_flux=r(c["inc"]))
+    _check(_host(up), rup, 4 if stress else prec, "flux_up")
+    _check(_host(dn), rdn, 4 if stress else prec, "flux_dn")
+
+
+def test_lw_2stream_without_incident_flux(G, syn, O, okd_lw):
+    """inc_flux = NULL is zero incident flux at the top."""
+    from rrtmgp_amd import _lib
+    c = _case(syn, O, okd_lw, True, seed=47, ncol=9, nlay=20)
+    go, tau = c["go"], c["tau"]
+    ncol, nlay, ngpt = tau.shape
+    dt = np.float64
+    d = {k: _dev(G, v, dt) for k, v in dict(tau=tau, ssa=c["ssa"], g=c["g"], linc=go["lev_src_inc"],
+                                            ldec=go["lev_src_dec"], emis=c["emis_gpt"], sfc=go["sfc_src"]).items()}
+    up, dn = G.DeviceArray((ncol, nlay + 1, ngpt), dt), G.DeviceArray((ncol, nlay + 1, ngpt), dt)
+    i = C.c_int64
+    _lib.check_hip(_lib.hip().rrtmgp_hip_lw_solver_2stream_f64(
+        i(ncol), i(nlay), i(ngpt), C.c_int(1), d["tau"].ptr, d["ssa"].ptr, d["g"].ptr, None, d["linc"].ptr,
+        d["ldec"].ptr, d["emis"].ptr, d["sfc"].ptr, None, up.ptr, dn.ptr, None))
+    rup, rdn = O.lw_solver_2stream(tau, c["ssa"], c["g"], go["lev_src_inc"], go["lev_src_dec"], c["emis_gpt"],
+                                   go["sfc_src"], True)
+    assert rel_err(_host(up), rup) < 1e-10
+    assert rel_err(_host(dn), rdn, floor=1e-6) < 1e-10
+    assert np.all(_host(dn)[:, 0, :] == 0.0)

@@ -482,7 +482,8 @@ def test_precision_study_mode():
     r = bench.precision_study(600, 60, 256, 64, 4)
     assert r["config"]["near_conservative_layers"] > 0
     for k, e in r["fp32_vs_fp64_gpu"].items():
-        if "flux" in k:
+        # Fluxes (W m-2); the heating rates (K/day) are held by tests/test_gpu_heating_rate.py.
+        if "flux" in k and "heating" not in k:
             assert e["max"] < 0.05, (k, e)
     assert r["vs_fp64_oracle"]["fp64"]["sw_flux_max"] < 1e-6
     assert r["vs_fp64_oracle"]["fp64"]["lw_flux_max"] < 1e-6
