@@ -423,6 +423,26 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
         const T* plev, T* heating_rate, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SOLVERS_BND)
 
+/* rrtmgp_hip_sw_solver_2stream_bnd_* without allocation.  No counterpart in the reference: it  */
+/* is for callers that cannot allocate device memory (code under HIP graph capture, serving     */
+/* loops, harnesses that forbid hipMalloc).  Same arguments, kernels and results as the          */
+/* allocating entry; its scratch (SW checkpoints, group partial fluxes, FP64 heating-rate parts) */
+/* comes from a fixed 64 MiB device array that the library holds on each device for the life   */
+/* of the process, and it calls no hipMalloc, hipFree or allocating routine.  A call whose       */
+/* scratch exceeds 64 MiB returns RRTMGP_ERR_ARG before any launch (at 128 layers and 224       */
+/* g-points in FP32, about 86 KB per column plus 14 KB with heating rates: ~650 columns per      */
+/* call; half as many in FP64).                                                                 */
+/* Calls that overlap on one device share the array and race: serialise them (one stream, or    */
+/* events).  Every scratch element is written before it is read, so nothing needs clearing.     */
+#define RRTMGP_DECL_SW_NOALLOC(S, T)                                                           \
+    int rrtmgp_hip_sw_solver_2stream_bnd_noalloc_##S(int64_t ncol, int64_t nlay, int64_t ngpt, \
+        int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,                    \
+        const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,     \
+        const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,         \
+        T* bnd_flux_dn, T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir,   \
+        const T* plev, T* heating_rate, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_NOALLOC)
+
 /* Longwave with factored Planck sources.  No counterpart in the reference, whose gas optics */
 /* writes lay_source, lev_source_inc and lev_source_dec per g-point (Source_func_lw,          */
 /* include/Source_functions.h:73-99) for Rte_lw to read.  Every one of them is a g-point's     */

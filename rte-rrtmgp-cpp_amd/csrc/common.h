@@ -44,6 +44,20 @@ class LaunchTimer
         hipEvent_t start_ = nullptr;
 };
 
+// While a TimingSuspend lives, launches on this thread are not timed (no events are created),
+// whatever rrtmgp_hip_timing_enable says.
+class TimingSuspend
+{
+    public:
+        TimingSuspend();
+        ~TimingSuspend();
+        TimingSuspend(const TimingSuspend&) = delete;
+        TimingSuspend& operator=(const TimingSuspend&) = delete;
+
+    private:
+        bool prev_;
+};
+
 #define RRTMGP_CAT2(a, b) a##b
 #define RRTMGP_CAT(a, b) RRTMGP_CAT2(a, b)
 #define RRTMGP_TIMED(name, stream) \

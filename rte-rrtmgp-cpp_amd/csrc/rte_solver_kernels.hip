@@ -30,6 +30,8 @@
 // FP64 for near-conservative layers, with the FP64 k_min = 1e-12 and epsilon.  Inputs, outputs
 // and checkpoints keep the build's precision.
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -869,13 +871,43 @@ int lw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, int64_t nm
     return rc;
 }
 
+// Scratch of the entry points without allocation (rrtmgp_hip_sw_solver_2stream_bnd_noalloc_*):
+// checkpoints, group partial fluxes and heating-rate parts of one call, carved from this array
+// in 256-byte aligned pieces.  Each device holds its own copy for the life of the process; its
+// address on a device comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr size_t SW_NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char sw_scratch[SW_NOALLOC_SCRATCH_BYTES];
+
+// Device address of sw_scratch on the current device, looked up once per device.
+int noalloc_scratch(void** out)
+{
+    constexpr int max_dev = 64;
+    static std::mutex m;
+    static void* addr[max_dev] = {};
+    int dev = 0;
+    RRTMGP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= max_dev) return arg_error("sw_solver (no allocation): device index out of range");
+    std::lock_guard<std::mutex> lock(m);
+    if (!addr[dev])
+    {
+        void* p = nullptr;
+        RRTMGP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(sw_scratch)));
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return RRTMGP_OK;
+}
+
+// With workspace != nullptr the solver's scratch is carved from it (at most workspace_bytes) and
+// nothing is allocated; otherwise it comes from the library's caching allocator.
 template<typename Real>
 int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real* tau, const Real* ssa,
               const Real* g, const Real* mu0, const Real* alb_dir, const Real* alb_dif, const Real* inc_dir,
               Real* flux_up, Real* flux_dn, Real* flux_dir, int has_dif_bc, const Real* inc_dif,
               int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bnd_dir,
               Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream, bool by_band = false,
-              const Real* plev = nullptr, Real* heating_rate = nullptr)
+              const Real* plev = nullptr, Real* heating_rate = nullptr, unsigned char* workspace = nullptr,
+              size_t workspace_bytes = 0)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("sw_solver: empty extents");
     if (heating_rate && (!plev || flux_up || flux_dn || flux_dir))
@@ -910,16 +942,33 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     const size_t n_ck = size_t(nseg * 3 * ngpt * ncol);
     const size_t n_part = (!spectral && !direct) ? size_t(ncv * grp.n * 3) : 0;
     const size_t n_abs = heating_rate ? size_t(ncol * nlay * grp.n) : 0;
-    RRTMGP_TRY(scratch_alloc(&ckpt, n_ck, s));
-    if (n_part)
+    if (workspace)
     {
-        const int rc0 = scratch_alloc(&scratch, n_part, s);
-        if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); return rc0; }
+        auto round256 = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t need = round256(n_ck * sizeof(Real)) + round256(n_part * sizeof(Real)) + round256(n_abs * sizeof(double));
+        if (need > workspace_bytes)
+            return arg_error("sw_solver (no allocation): this call needs " + std::to_string(need) +
+                             " bytes of scratch; the entry point without allocation holds " +
+                             std::to_string(workspace_bytes) + " (use fewer columns per call)");
+        ckpt = reinterpret_cast<Real*>(workspace);
+        scratch = n_part ? reinterpret_cast<Real*>(workspace + round256(n_ck * sizeof(Real))) : nullptr;
+        absorbed = n_abs ? reinterpret_cast<double*>(workspace + round256(n_ck * sizeof(Real)) +
+                                                     round256(n_part * sizeof(Real)))
+                         : nullptr;
     }
-    if (n_abs)
+    else
     {
-        const int rc0 = scratch_alloc(&absorbed, n_abs, s);
-        if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); scratch_free(scratch, s); return rc0; }
+        RRTMGP_TRY(scratch_alloc(&ckpt, n_ck, s));
+        if (n_part)
+        {
+            const int rc0 = scratch_alloc(&scratch, n_part, s);
+            if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); return rc0; }
+        }
+        if (n_abs)
+        {
+            const int rc0 = scratch_alloc(&absorbed, n_abs, s);
+            if (rc0 != RRTMGP_OK) { scratch_free(ckpt, s); scratch_free(scratch, s); return rc0; }
+        }
     }
     Real* gf[3] = {nullptr, nullptr, nullptr};
     Real* const bo[3] = {bnd_up, bnd_dn, bnd_dir};
@@ -939,10 +988,30 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
         rc = group_sums<Real>(ncv, grp, gf[c], direct ? nullptr : bo[c], bbo[c], s);
     if (rc == RRTMGP_OK && heating_rate)
         rc = absorbed_heating_rate<Real>(ncol, nlay, grp.n, 1, absorbed, plev, heating_rate, s);
-    scratch_free(absorbed, s);
-    scratch_free(ckpt, s);
-    scratch_free(scratch, s);
+    if (!workspace)
+    {
+        scratch_free(absorbed, s);
+        scratch_free(ckpt, s);
+        scratch_free(scratch, s);
+    }
     return rc;
+}
+
+template<typename Real>
+int sw_solver_noalloc(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real* tau, const Real* ssa,
+                      const Real* g, const Real* mu0, const Real* alb_dir, const Real* alb_dif, const Real* inc_dir,
+                      int has_dif_bc, const Real* inc_dif, int64_t nbnd, const int* band_lims, Real* bnd_up,
+                      Real* bnd_dn, Real* bnd_dir, Real* bb_up, Real* bb_dn, Real* bb_dir, const Real* plev,
+                      Real* heating_rate, void* stream)
+{
+    void* ws = nullptr;
+    RRTMGP_TRY(noalloc_scratch(&ws));
+    // Launch timing would create events: it is held off for this call.
+    const TimingSuspend no_timing;
+    return sw_solver<Real>(ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir, nullptr,
+                           nullptr, nullptr, has_dif_bc, inc_dif, nbnd, band_lims, bnd_up, bnd_dn, bnd_dir, bb_up,
+                           bb_dn, bb_dir, stream, true, plev, heating_rate, static_cast<unsigned char*>(ws),
+                           SW_NOALLOC_SCRATCH_BYTES);
 }
 }  // namespace rrtmgp_hip
 
@@ -1013,6 +1082,23 @@ extern "C" int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, 
                         band_lims_gpt, bnd_flux_up, bnd_flux_dn, bnd_flux_dir, broadband_up,         \
                         broadband_dn, broadband_dir, stream, true, plev, heating_rate);              \
 }
+
+#define RRTMGP_DEF_SW_NOALLOC(S, T)                                                                 \
+extern "C" int rrtmgp_hip_sw_solver_2stream_bnd_noalloc_##S(int64_t ncol, int64_t nlay,            \
+    int64_t ngpt, int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,               \
+    const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,             \
+    const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up, T* bnd_flux_dn,  \
+    T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir, const T* plev,             \
+    T* heating_rate, void* stream)                                                                   \
+{                                                                                                    \
+    return sw_solver_noalloc<T>(ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, sfc_alb_dir,          \
+                                sfc_alb_dif, inc_flux_dir, has_dif_bc, inc_flux_dif, nbnd,           \
+                                band_lims_gpt, bnd_flux_up, bnd_flux_dn, bnd_flux_dir, broadband_up, \
+                                broadband_dn, broadband_dir, plev, heating_rate, stream);            \
+}
+
+RRTMGP_DEF_SW_NOALLOC(f32, float)
+RRTMGP_DEF_SW_NOALLOC(f64, double)
 
 #define RRTMGP_DEF_SECANTS(S, T)                                                                    \
 extern "C" int rrtmgp_hip_lw_secants_array_##S(int64_t ncol, int64_t ngpt, int64_t n_gauss_quad,  \

@@ -72,9 +72,17 @@ void clear_records()
 }
 }  // namespace
 
+namespace
+{
+thread_local bool g_timing_suspended = false;
+}
+
+TimingSuspend::TimingSuspend() : prev_(g_timing_suspended) { g_timing_suspended = true; }
+TimingSuspend::~TimingSuspend() { g_timing_suspended = prev_; }
+
 LaunchTimer::LaunchTimer(const char* name, hipStream_t stream) : name_(name), stream_(stream)
 {
-    if (!g_timing_on.load(std::memory_order_relaxed)) return;
+    if (g_timing_suspended || !g_timing_on.load(std::memory_order_relaxed)) return;
     start_ = pooled_event();
     if (start_ && hipEventRecord(start_, stream_) != hipSuccess) start_ = nullptr;
 }
