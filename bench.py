@@ -255,7 +255,8 @@ def cpu_baseline(atm, kl, ks, ll, ls, glw, gsw, ncols, threads, runs=3):
 def precision_study(ncol, nlay, block, ocols, threads):
     """The FP32 bulk against an all-FP64 GPU run on the same inputs, with 10% of the cloudy layers
     forced to ssa = 1 - 1e-7, g = 0.85 before delta-scaling (near-conservative scattering): the
-    shortwave through gas optics, cloud optics, delta-scale, add_to and the broadband solver; the
+    shortwave through gas optics, cloud optics, delta-scale, add_to and the broadband solver, all in
+    co-albedo form (1 - ssa carried in place of ssa, as the shortwave driver runs them); the
     longwave through the driver.  Max and RMS flux error (W m-2) and heating-rate error (K/day,
     from each run's net fluxes differenced in FP64 on the host) over all columns; both GPU runs
     against the FP64 oracle on the first ocols columns.  Columns go in blocks of `block`."""
@@ -289,20 +290,24 @@ def precision_study(ncol, nlay, block, ocols, threads):
                    for g in sub.vmr}
             gc = api.GasConcs(vmr, prec)
             cd = api.get_col_dry(vmr["h2o"], sub.p_lev, prec)
-            go = api.gas_optics(kd, sub, gc, cd)
-            ct, cs, cg = cloud.optics(sub.lwp, sub.iwp, sub.rel, sub.rei, True)
+            # The whole shortwave chain in co-albedo form (1 - ssa), as the driver runs it.
+            go = api.gas_optics(kd, sub, gc, cd, coalbedo=True)
+            ct, cc, cg = (a.to_host().astype(np.float64)
+                          for a in cloud.optics(sub.lwp, sub.iwp, sub.rel, sub.rei, True, coalbedo=True))
             m = sub.extra["conservative"][:, :, None]
+            cc, cg = np.where(m, 1e-7, cc), np.where(m, 0.85, cg)
+            f = cg * cg                                  # delta-scaling in co-albedo form, FP64
+            d = (1.0 - f) + f * cc
+            ct, cc, cg = ct * d, cc / np.maximum(d, 1e-300), (cg - f) / np.maximum(1.0 - f, 1e-12)
             dt = api.real_dtype(prec)
-            cs = device.DeviceArray.from_host(np.where(m, 1.0 - 1e-7, cs.to_host()), dt)
-            cg = device.DeviceArray.from_host(np.where(m, 0.85, cg.to_host()), dt)
-            api.delta_scale(ct, cs, cg, prec)
-            api.add_to(kd, go["tau"], ct, go["ssa"], go["g"], cs, cg)
+            ct, cc, cg = (device.DeviceArray.from_host(a, dt) for a in (ct, cc, cg))
+            api.add_to(kd, go["tau"], ct, go["coalbedo"], go["g"], cc, cg, coalbedo=True)
             toa = device.DeviceArray.from_host(go["toa_src"].to_host() * scale, dt)
-            o = api.rte_sw_bnd(kd, sub.top_at_1, go["tau"], go["ssa"], go["g"], sub.mu0, toa, sub.sfc_alb,
-                               0.5 * sub.sfc_alb, p_lev=sub.p_lev)
+            o = api.rte_sw_bnd(kd, sub.top_at_1, go["tau"], go["coalbedo"], go["g"], sub.mu0, toa, sub.sfc_alb,
+                               0.5 * sub.sfc_alb, p_lev=sub.p_lev, coalbedo=True)
             for k, v in (("up", "flux_up"), ("dn", "flux_dn"), ("dir", "flux_dn_dir"), ("hr", "heating_rate")):
                 sw[k][c0:c0 + n] = o[v].to_host()
-            del go, ct, cs, cg, o, toa
+            del go, ct, cc, cg, o, toa
         kdl = api.KDist(kl, AVAILABLE, prec)
         slw = api.SolverLW(kdl, api.Cloud(ll, prec))
         slw.set_block(block)

@@ -137,6 +137,13 @@ int rrtmgp_api_gas_optics_sw(void* kdist, int ncol, int nlay, const rrtmgp_real*
                              const rrtmgp_real* plev, const rrtmgp_real* tlay, void* gas_concs,
                              const rrtmgp_real* col_dry, rrtmgp_real* tau, rrtmgp_real* ssa,
                              rrtmgp_real* g, rrtmgp_real* toa_src);
+/* As rrtmgp_api_gas_optics_sw, writing the co-albedo 1 - ssa in place of ssa (no reference      */
+/* counterpart: the shortwave chain in co-albedo form, which FP32 needs to resolve absorption    */
+/* below 6e-8 of extinction; not with a surrogate).                                              */
+int rrtmgp_api_gas_optics_sw_coalbedo(void* kdist, int ncol, int nlay, const rrtmgp_real* play,
+                                      const rrtmgp_real* plev, const rrtmgp_real* tlay, void* gas_concs,
+                                      const rrtmgp_real* col_dry, rrtmgp_real* tau, rrtmgp_real* coalbedo,
+                                      rrtmgp_real* g, rrtmgp_real* toa_src);
 
 /* The materializing kernels of the reference launch chain on this k-distribution's device    */
 /* tables: fill_gases_kernel (col_gas (ncol, nlay, ngas+1)) and interpolation (outputs as     */
@@ -149,7 +156,8 @@ int rrtmgp_api_kdist_interpolation(void* kdist, int ncol, int nlay, const rrtmgp
                                    rrtmgp_real* col_mix, signed char* tropo, int* jeta,
                                    int* jpress);
 
-/* Cloud_optics_gpu (include/Cloud_optics.h:84-128). */
+/* Cloud_optics_gpu (include/Cloud_optics.h:84-128).  two_stream: 0 absorption only, 1 tau,   */
+/* ssa, g, 2 tau, co-albedo, g (delta-scaled in co-albedo form when delta_scale != 0).           */
 int rrtmgp_api_cloud_create(const rrtmgp_cloud_lut* lut, void** cloud);
 int rrtmgp_api_cloud_destroy(void* cloud);
 int rrtmgp_api_cloud_optics(void* cloud, int ncol, int nlay, const rrtmgp_real* lwp,
@@ -159,18 +167,24 @@ int rrtmgp_api_cloud_optics(void* cloud, int ncol, int nlay, const rrtmgp_real* 
 
 /* Aerosol_optics_gpu (include/Aerosol_optics.h:55-101): aerosol_concs is a gas-concentration */
 /* handle holding aermr01..aermr11 (Aerosol_concs_gpu = Gas_concs_gpu, as in the reference);    */
-/* rh (ncol, nlay), plev (ncol, nlay+1); outputs (ncol, nlay, nbnd), delta-scaled when asked.  */
+/* rh (ncol, nlay), plev (ncol, nlay+1); outputs (ncol, nlay, nbnd), delta-scaled when asked,  */
+/* with the co-albedo 1 - ssa in place of ssa when coalbedo != 0.                               */
 int rrtmgp_api_aerosol_create(const rrtmgp_aerosol_lut* lut, void** aerosol);
 int rrtmgp_api_aerosol_destroy(void* aerosol);
 int rrtmgp_api_aerosol_optics(void* aerosol, int ncol, int nlay, void* aerosol_concs, const rrtmgp_real* rh,
-                              const rrtmgp_real* plev, int delta_scale, rrtmgp_real* tau, rrtmgp_real* ssa,
-                              rrtmgp_real* g);
+                              const rrtmgp_real* plev, int delta_scale, int coalbedo, rrtmgp_real* tau,
+                              rrtmgp_real* ssa, rrtmgp_real* g);
 
 /* add_to and delta_scale (include/Optical_props.h:174-176,256).  nspec_in is ngpt (same grid) */
 /* or nbnd (band-resolved input); ssa/g NULL for 1scl.                                        */
 int rrtmgp_api_add_to(void* kdist, int ncol, int nlay, int nspec_in, rrtmgp_real* tau,
                       rrtmgp_real* ssa, rrtmgp_real* g, const rrtmgp_real* tau_in,
                       const rrtmgp_real* ssa_in, const rrtmgp_real* g_in);
+/* add_to of band-resolved (ncol, nlay, nbnd) properties onto (ncol, nlay, ngpt) ones, both in    */
+/* co-albedo form (rrtmgp_hip_inc_bybnd_coalbedo_*; no reference counterpart).                   */
+int rrtmgp_api_add_to_coalbedo(void* kdist, int ncol, int nlay, rrtmgp_real* tau, rrtmgp_real* coalbedo,
+                               rrtmgp_real* g, const rrtmgp_real* tau_in, const rrtmgp_real* coalbedo_in,
+                               const rrtmgp_real* g_in);
 int rrtmgp_api_delta_scale(int ncol, int nlay, int ngpt, rrtmgp_real* tau, rrtmgp_real* ssa,
                            rrtmgp_real* g);
 

@@ -199,6 +199,22 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_LW)
         T* tau, T* ssa, T* g, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_SW)
 
+/* Co-albedo forms (no reference counterpart).  In FP32, ssa near 1 resolves 1 - ssa only to     */
+/* 6e-8, while the absorption of a Rayleigh-dominated layer at the model top is a few 1e-9 of its */
+/* extinction: stored as ssa it is lost or multiplied, and the heating of the top layers with it. */
+/* These entries carry the co-albedo 1 - ssa in place of ssa, formed from absorption optical     */
+/* depths, through the shortwave chain: gas optics (+ band cloud add_to), cloud optics           */
+/* (rrtmgp_hip_cloud_optics_* with two_stream = 2), aerosol optics (coalbedo = 1), the band      */
+/* add_to (rrtmgp_hip_inc_bybnd_coalbedo_*) and the solver                                      */
+/* (rrtmgp_hip_sw_solver_2stream_bnd_coalbedo_*).  Arguments are those of the ssa entries, with  */
+/* co-albedos where those take ssa; results are the same quantities.                            */
+#define RRTMGP_DECL_GAS_OPTICS_SW_CO(S, T)                                                     \
+    int rrtmgp_hip_gas_optics_sw_coalbedo_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,        \
+        int64_t nlay, const T* play, const T* tlay, const T* col_dry,                            \
+        const rrtmgp_gas_vmr* gases, const T* cloud_tau, const T* cloud_coalbedo,               \
+        const T* cloud_g, T* tau, T* coalbedo, T* g, void* stream);
+RRTMGP_DECLARE_BOTH(RRTMGP_DECL_GAS_OPTICS_SW_CO)
+
 /* The fused gas optics restricted to listed (64-column tile, layer) pairs: tiles[i] =         */
 /* ilay * ((ncol + 63)/64) + tile, 0-based, for i < ntiles (ntiles = 0 computes nothing);      */
 /* outputs outside the listed pairs are left as they are.  No reference counterpart: the       */
@@ -296,6 +312,7 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_FILL_GASES)
 /* Cloud optics (Cloud_optics_gpu::cloud_optics, src_cuda/Cloud_optics.cu:181-329; CPU       */
 /* src/Cloud_optics.cpp:72-232).  LUTs are (nsize, nbnd) device arrays (ice: the roughness    */
 /* category the host class selected).  two_stream = 0 writes the 1scl absorption tau only;    */
+/* two_stream = 2 writes the co-albedo 1 - ssa in place of ssa (see the co-albedo forms);     */
 /* delta_scale != 0 applies delta_scale_2str_k to the 2str result in the same pass.           */
 /* ------------------------------------------------------------------------------------------ */
 #define RRTMGP_DECL_CLOUD(S, T)                                                                \
@@ -315,13 +332,14 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_CLOUD)
 /* of the CPU path; layer mass |dp| / 9.81 (the GPU path's; equal to the CPU path's for levels */
 /* ordered from the surface up); rh above every class bound takes the last class (the         */
 /* reference reads past the table).  Outputs tau, ssa, g (ncol, nlay, nbnd), delta-scaled     */
-/* (rte_delta_scale_2str_k) when delta_scale != 0.                                             */
+/* (rte_delta_scale_2str_k) when delta_scale != 0; coalbedo != 0 writes 1 - ssa in place of   */
+/* ssa (see the co-albedo forms).                                                              */
 #define RRTMGP_DECL_AEROSOL(S, T)                                                              \
     int rrtmgp_hip_aerosol_optics_##S(int64_t ncol, int64_t nlay, int64_t nbnd, int64_t nhum, \
         int64_t nphobic, int64_t nphilic, const rrtmgp_gas_vmr* aermr, const T* rh,            \
         const T* plev, const T* rh_upper, const T* mext_phobic, const T* ssa_phobic,           \
         const T* g_phobic, const T* mext_philic, const T* ssa_philic, const T* g_philic,       \
-        int delta_scale, T* tau, T* ssa, T* g, void* stream);
+        int delta_scale, int coalbedo, T* tau, T* ssa, T* g, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_AEROSOL)
 
 /* ------------------------------------------------------------------------------------------ */
@@ -337,7 +355,10 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_AEROSOL)
         T* ssa_inout, T* g_inout, const T* tau_in, const T* ssa_in, const T* g_in,             \
         int64_t nbnd, const int* band_lims_gpt, void* stream);                                 \
     int rrtmgp_hip_delta_scale_2str_##S(int64_t ncol, int64_t nlay, int64_t ngpt, T* tau,     \
-        T* ssa, T* g, void* stream);
+        T* ssa, T* g, void* stream);                                                            \
+    int rrtmgp_hip_inc_bybnd_coalbedo_##S(int64_t ncol, int64_t nlay, int64_t ngpt,            \
+        T* tau_inout, T* coalbedo_inout, T* g_inout, const T* tau_in, const T* coalbedo_in,    \
+        const T* g_in, int64_t nbnd, const int* band_lims_gpt, void* stream);
 RRTMGP_DECLARE_BOTH(RRTMGP_DECL_OPTPROPS)
 
 /* ------------------------------------------------------------------------------------------ */
@@ -417,6 +438,12 @@ RRTMGP_DECLARE_BOTH(RRTMGP_DECL_SW_SOLVER)
         T* broadband_dn, const T* plev, T* heating_rate, void* stream);                         \
     int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt,         \
         int top_at_1, const T* tau, const T* ssa, const T* g, const T* mu0,                    \
+        const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,     \
+        const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,         \
+        T* bnd_flux_dn, T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir,   \
+        const T* plev, T* heating_rate, void* stream);                                          \
+    int rrtmgp_hip_sw_solver_2stream_bnd_coalbedo_##S(int64_t ncol, int64_t nlay,             \
+        int64_t ngpt, int top_at_1, const T* tau, const T* coalbedo, const T* g, const T* mu0,  \
         const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,     \
         const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up,         \
         T* bnd_flux_dn, T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir,   \

@@ -114,6 +114,14 @@ template<> struct Limits<double>
     static constexpr double k_min = 1.e-12;
 };
 
+// Cut-off of the optical-property rules: combine_abs_and_rayleigh's ssa = tau_ray / tau for
+// tau > 2 eps (src/Gas_optics_rrtmgp.cpp:378), and the max(tau, eps) guards of cloud and aerosol
+// optics (src/Cloud_optics.cpp:159-171).  It is the FP64 epsilon in both precisions: with FP32's
+// (1.2e-7) every layer with tau < 2.4e-7, which near a 1 Pa model top is most g-points of the top
+// ~10 layers, would count as purely absorbing (ssa = 0), moving FP32 heating rates there by
+// ~0.5 K/day and the pressure gradients of the adjoint by tens of percent.
+constexpr double prop_eps = DBL_EPSILON;
+
 // 8 XCDs with private L2 each: work-group i is dispatched to XCD i % 8.  Remap the flat
 // work-group id so that consecutive work units (which share k-table slices) land on one XCD.
 constexpr int n_xcd = 8;

@@ -697,7 +697,7 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ col_dry_a, const Real* __restrict__ ctau_a,
                           const Real* __restrict__ cssa_a, const Real* __restrict__ cg_a,
-                          Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ gout)
+                          Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ gout, int coalb)
 {
     static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
@@ -750,7 +750,7 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const int64_t icol = un.col0 + lane;
     if (icol >= ncol) return;
     const int64_t idx = icol + ncol * ilay;
-    const Real eps2 = Real(2) * Limits<Real>::eps;
+    const Real eps2 = Real(2 * prop_eps);
     const Real op_eps = Real(3) * Limits<Real>::tiny;
     Real t2 = Real(0), s2 = Real(0), g2 = Real(0);
     if (ctau_a)
@@ -761,8 +761,31 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const int gbase = bd.g0 - k.bands[grid.b0].g0 + un.c0;
     for (int row = w; row < un.ncg; row += GO_THREADS / 64)
     {
+        const Real tabs = s_abs[row][lane];
         const Real tray = s_ray[row][lane];
-        const Real tt = s_abs[row][lane] + tray;
+        const Real tt = tabs + tray;
+        const int64_t o = idx + ncl * (gbase + row);
+        if (coalb)
+        {
+            // The same with the co-albedo 1 - ssa in place of ssa, for the cloud input and the
+            // output: formed from the absorption optical depths, it keeps the precision of the
+            // absorption that 1 - ssa in FP32 loses once it falls below 6e-8.
+            Real tt_out = tt;
+            Real co = (tt > eps2) ? tabs / tt : Real(1);
+            Real gg_out = Real(0);
+            if (ctau_a)
+            {
+                const Real tau12 = tt + t2;
+                const Real tauscat12 = (tt * (Real(1) - co)) + (t2 * (Real(1) - s2));
+                gg_out = (t2 * (Real(1) - s2) * g2) / max(tauscat12, op_eps);
+                co = (tau12 > op_eps) ? ((tt * co) + (t2 * s2)) / tau12 : Real(1) - tauscat12 / op_eps;
+                tt_out = tau12;
+            }
+            tau[o] = tt_out;
+            ssa[o] = co;
+            gout[o] = gg_out;
+            continue;
+        }
         // The CPU combine rule (Gas_optics_rrtmgp.cpp:365-385).
         Real tt_out = tt;
         Real ss = (tt > eps2) ? tray / tt : Real(0);
@@ -776,7 +799,6 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
             ss = tauscat12 / max(op_eps, tau12);
             tt_out = tau12;
         }
-        const int64_t o = idx + ncl * (gbase + row);
         tau[o] = tt_out;
         ssa[o] = ss;
         gout[o] = gg_out;
@@ -999,7 +1021,7 @@ void gas_optics_adjoint_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t 
     for (int a = 0; a < GA_N; ++a) acc[a] = Real(0);
 #pragma unroll
     for (int q = 0; q < GO_MAXM; ++q) s_sc[q][threadIdx.x] = Real(0);
-    const Real eps2 = Real(2) * Limits<Real>::eps;
+    const Real eps2 = Real(2 * prop_eps);
     const Real op_eps = Real(3) * Limits<Real>::tiny;
 
     for (int c0 = 0; c0 < bd.ng; c0 += GO_GPTS)
@@ -1335,7 +1357,7 @@ __global__ void combine_kernel(int64_t n, const Real* __restrict__ tau_abs, cons
     if (i >= n) return;
     const Real tr = tau_ray[i];
     const Real tt = tau_abs[i] + tr;
-    ssa[i] = (tt > Real(2) * Limits<Real>::eps) ? tr / tt : Real(0);
+    ssa[i] = (tt > Real(2 * prop_eps)) ? tr / tt : Real(0);
     tau[i] = tt;
     g[i] = Real(0);
 }
@@ -1656,7 +1678,7 @@ template<typename Real>
 int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t ncol, int64_t nlay, const Real* play,
                   const Real* tlay, const Real* col_dry, const rrtmgp_gas_vmr* gases,
                   const Real* ctau, const Real* cssa, const Real* cg, Real* tau, Real* ssa, Real* g,
-                  void* stream, const int* tiles = nullptr, int64_t ntiles = 0)
+                  void* stream, const int* tiles = nullptr, int64_t ntiles = 0, bool coalb = false)
 {
     RRTMGP_TRY(check_kd_fused(kd, false));
     if (ncol <= 0 || nlay <= 0) return arg_error("ncol and nlay must be positive");
@@ -1674,7 +1696,7 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
     RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
     hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
                        as_stream(stream), k, gs, ncol, nlay, grid, play, tlay, col_dry, ctau, cssa, cg,
-                       tau, ssa, g);
+                       tau, ssa, g, coalb ? 1 : 0);
     RRTMGP_LAUNCHED("gas_optics_sw_kernel");
     return RRTMGP_OK;
 }
@@ -1797,6 +1819,15 @@ extern "C" int rrtmgp_hip_gas_optics_sw_##S(const rrtmgp_kdist_dev* kd, int64_t 
 {                                                                                                  \
     return gas_optics_sw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, col_dry, gases,      \
                             cloud_tau, cloud_ssa, cloud_g, tau, ssa, g, stream);                   \
+}                                                                                                  \
+extern "C" int rrtmgp_hip_gas_optics_sw_coalbedo_##S(const rrtmgp_kdist_dev* kd, int64_t ncol,   \
+    int64_t nlay, const T* play, const T* tlay, const T* col_dry, const rrtmgp_gas_vmr* gases,      \
+    const T* cloud_tau, const T* cloud_coalbedo, const T* cloud_g, T* tau, T* coalbedo, T* g,       \
+    void* stream)                                                                                   \
+{                                                                                                  \
+    return gas_optics_sw<T>(kd, 0, kd ? kd->nbnd : 0, ncol, nlay, play, tlay, col_dry, gases,      \
+                            cloud_tau, cloud_coalbedo, cloud_g, tau, coalbedo, g, stream, nullptr, \
+                            0, true);                                                              \
 }                                                                                                  \
 extern "C" int rrtmgp_hip_gas_optics_sw_bands_##S(const rrtmgp_kdist_dev* kd, int64_t band_start,  \
     int64_t band_count, int64_t ncol, int64_t nlay, const T* play, const T* tlay, const T* col_dry, \

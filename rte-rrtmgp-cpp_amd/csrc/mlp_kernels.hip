@@ -568,7 +568,7 @@ void mlp_gas_optics_kernel(MlpNet net, int64_t ncol, int64_t nlay, int ngpt, int
                     {
                         const float tabs = __expf(y0) * cd, tray = __expf(y1) * cd;
                         const float tt = tabs + tray;
-                        const float eps2 = 2.f * FLT_EPSILON, op_eps = 3.f * FLT_MIN;
+                        const float eps2 = float(2 * prop_eps), op_eps = 3.f * FLT_MIN;
                         float ss = tt > eps2 ? __fdividef(tray, tt) : 0.f, tt_out = tt, gg = 0.f;
                         if (ctau)
                         {

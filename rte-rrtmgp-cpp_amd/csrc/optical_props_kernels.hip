@@ -9,10 +9,13 @@
 namespace rrtmgp_hip
 {
 // Table lookup of compute_all_from_table (src/Cloud_optics.cpp:80-101) for one element.
+// taua, when given, receives the absorption optical depth tau (1 - ssa), with 1 - ssa interpolated
+// from the table's own 1 - ssa (exact in either precision for ssa >= 1/2) instead of formed as
+// tau - taussa.
 template<typename Real>
 __device__ __forceinline__ void cloud_lookup(Real cwp, Real re, int64_t nsteps, Real step, Real offset,
                                              const Real* et, const Real* st, const Real* at, int64_t ibnd,
-                                             Real& tau, Real& taussa, Real& taussag)
+                                             Real& tau, Real& taussa, Real& taussag, Real* taua = nullptr)
 {
     if (cwp > Real(0))
     {
@@ -22,13 +25,16 @@ __device__ __forceinline__ void cloud_lookup(Real cwp, Real re, int64_t nsteps, 
         const Real fint = (re - offset) / step - Real(index - 1);
         const int64_t b = nsteps * ibnd;
         const Real e0 = et[b + index - 1], s0 = st[b + index - 1], a0 = at[b + index - 1];
+        const Real ds = st[b + index] - s0;
         tau = cwp * (e0 + fint * (et[b + index] - e0));
-        taussa = tau * (s0 + fint * (st[b + index] - s0));
+        taussa = tau * (s0 + fint * ds);
         taussag = taussa * (a0 + fint * (at[b + index] - a0));
+        if (taua) *taua = tau * ((Real(1) - s0) - fint * ds);
     }
     else
     {
         tau = Real(0); taussa = Real(0); taussag = Real(0);
+        if (taua) *taua = Real(0);
     }
 }
 
@@ -47,15 +53,38 @@ __global__ void cloud_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd,
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= ncl) return;
     const Real lw = clwp[i], iw = ciwp[i], rl = rel[i], ri = rei[i];
-    const Real eps = Limits<Real>::eps;
+    const Real eps = Real(prop_eps);
     const Real op_eps = Real(3) * Limits<Real>::tiny;
     for (int64_t b = 0; b < nbnd; ++b)
     {
-        Real lt, ls, lg, it, is, ig;
-        cloud_lookup(lw, rl, liq_nsteps, liq_step, liq_lwr, extl, ssal, asyl, b, lt, ls, lg);
-        cloud_lookup(iw, ri, ice_nsteps, ice_step, ice_lwr, exti, ssai, asyi, b, it, is, ig);
+        Real lt, ls, lg, it, is, ig, la = Real(0), ia = Real(0);
+        cloud_lookup(lw, rl, liq_nsteps, liq_step, liq_lwr, extl, ssal, asyl, b, lt, ls, lg,
+                     two_stream == 2 ? &la : nullptr);
+        cloud_lookup(iw, ri, ice_nsteps, ice_step, ice_lwr, exti, ssai, asyi, b, it, is, ig,
+                     two_stream == 2 ? &ia : nullptr);
         const int64_t o = i + ncl * b;
-        if (two_stream)
+        if (two_stream == 2)
+        {
+            // As below, with the co-albedo 1 - ssa written in place of ssa, formed from the
+            // absorption optical depth (tau - tau ssa per phase, each from 1 - the table's ssa):
+            // it keeps the precision that 1 - ssa in FP32 loses near conservative scattering.
+            // Delta-scaling in the same terms: 1 - ssa' = (1 - ssa) / (1 - f + f (1 - ssa)).
+            Real t = lt + it;
+            const Real ts = ls + is, tsg = lg + ig;
+            const Real ta = la + ia;
+            Real co = (t > eps) ? ta / t : Real(1) - ts / eps;
+            Real gg = tsg / max(ts, eps);
+            if (delta_scale)
+            {
+                const Real f = gg * gg;
+                const Real d = (Real(1) - f) + f * co;         // 1 - ssa f
+                t *= d;
+                co = co / max(op_eps, d);
+                gg = (gg - f) / max(op_eps, Real(1) - f);
+            }
+            tau[o] = t; ssa[o] = co; g[o] = gg;
+        }
+        else if (two_stream)
         {
             // Cloud_optics::cloud_optics 2str (src/Cloud_optics.cpp:159-171).
             Real t = lt + it;
@@ -116,8 +145,8 @@ __global__ void aerosol_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd, 
                                       const Real* __restrict__ rh_upper, const Real* __restrict__ mext_phobic,
                                       const Real* __restrict__ ssa_phobic, const Real* __restrict__ g_phobic,
                                       const Real* __restrict__ mext_philic, const Real* __restrict__ ssa_philic,
-                                      const Real* __restrict__ g_philic, int delta_scale, Real* __restrict__ tau,
-                                      Real* __restrict__ ssa, Real* __restrict__ g)
+                                      const Real* __restrict__ g_philic, int delta_scale, int coalb,
+                                      Real* __restrict__ tau, Real* __restrict__ ssa, Real* __restrict__ g)
 {
     const int64_t ncl = ncol * nlay;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -130,11 +159,11 @@ __global__ void aerosol_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd, 
     Real m[AEROSOL_NSPECIES];
 #pragma unroll
     for (int k = 0; k < AEROSOL_NSPECIES; ++k) m[k] = mix.p[k][icol * mix.scol[k] + ilay * mix.slay[k]];
-    const Real eps = Limits<Real>::eps;
+    const Real eps = Real(prop_eps);
     const Real op_eps = Real(3) * Limits<Real>::tiny;
     for (int64_t b = 0; b < nbnd; ++b)
     {
-        Real t = Real(0), ts = Real(0), tsg = Real(0);
+        Real t = Real(0), ts = Real(0), tsg = Real(0), ta = Real(0);
 #pragma unroll
         for (int k = 0; k < AEROSOL_NSPECIES; ++k)
         {
@@ -147,9 +176,26 @@ __global__ void aerosol_optics_kernel(int64_t ncol, int64_t nlay, int64_t nbnd, 
             t += od;
             ts += od * sa;
             tsg += od * sa * ga;
+            ta += od * (Real(1) - sa);
+        }
+        Real gg = tsg / max(ts, eps);
+        if (coalb)
+        {
+            // The co-albedo 1 - ssa from the absorption optical depth (see cloud_optics_kernel).
+            Real co = (t > eps) ? ta / t : Real(1) - ts / eps;
+            if (delta_scale)
+            {
+                const Real f = gg * gg;
+                const Real d = (Real(1) - f) + f * co;
+                t *= d;
+                co = co / max(op_eps, d);
+                gg = (gg - f) / max(op_eps, Real(1) - f);
+            }
+            const int64_t o = i + ncl * b;
+            tau[o] = t; ssa[o] = co; g[o] = gg;
+            continue;
         }
         Real s = ts / max(t, eps);
-        Real gg = tsg / max(ts, eps);
         if (delta_scale)
         {
             const Real f = gg * gg, wf = s * f;
@@ -185,7 +231,8 @@ template<typename Real>
 __global__ void inc_bybnd_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, Real* __restrict__ tau1,
                                  Real* __restrict__ ssa1, Real* __restrict__ g1,
                                  const Real* __restrict__ tau2, const Real* __restrict__ ssa2,
-                                 const Real* __restrict__ g2, int64_t nbnd, const int* __restrict__ band_lims)
+                                 const Real* __restrict__ g2, int64_t nbnd, const int* __restrict__ band_lims,
+                                 int coalb = 0)
 {
     const int64_t ncl = ncol * nlay;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -199,6 +246,17 @@ __global__ void inc_bybnd_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, Real*
         {
             const int64_t ib = cl + ncl * b;
             if (!ssa1) { tau1[i] = tau1[i] + tau2[ib]; continue; }
+            if (coalb)
+            {
+                // ssa1, ssa2 hold co-albedos 1 - ssa: the same increment in those terms.
+                const Real t1 = tau1[i], t2 = tau2[ib], c1 = ssa1[i], c2 = ssa2[ib];
+                const Real tau12 = t1 + t2;
+                const Real tauscat12 = (t1 * (Real(1) - c1)) + (t2 * (Real(1) - c2));
+                g1[i] = ((t1 * (Real(1) - c1) * g1[i]) + (t2 * (Real(1) - c2) * g2[ib])) / max(tauscat12, eps);
+                ssa1[i] = (tau12 > eps) ? ((t1 * c1) + (t2 * c2)) / tau12 : Real(1) - tauscat12 / eps;
+                tau1[i] = tau12;
+                continue;
+            }
             const Real t1 = tau1[i], t2 = tau2[ib], s1 = ssa1[i], s2 = ssa2[ib];
             const Real tau12 = t1 + t2;
             const Real tauscat12 = (t1 * s1) + (t2 * s2);
@@ -393,8 +451,8 @@ extern "C" int rrtmgp_hip_cloud_optics_##S(int64_t ncol, int64_t nlay, int64_t n
 extern "C" int rrtmgp_hip_aerosol_optics_##S(int64_t ncol, int64_t nlay, int64_t nbnd,            \
     int64_t nhum, int64_t nphobic, int64_t nphilic, const rrtmgp_gas_vmr* aermr, const T* rh,       \
     const T* plev, const T* rh_upper, const T* mext_phobic, const T* ssa_phobic, const T* g_phobic, \
-    const T* mext_philic, const T* ssa_philic, const T* g_philic, int delta_scale, T* tau, T* ssa,  \
-    T* g, void* stream)                                                                              \
+    const T* mext_philic, const T* ssa_philic, const T* g_philic, int delta_scale, int coalbedo,    \
+    T* tau, T* ssa, T* g, void* stream)                                                              \
 {                                                                                                    \
     if (ncol <= 0 || nlay <= 0 || nbnd <= 0 || nhum <= 0) return arg_error("aerosol_optics: empty extents"); \
     if (nphobic < 11 || nphilic < 5)                                                                 \
@@ -417,7 +475,8 @@ extern "C" int rrtmgp_hip_aerosol_optics_##S(int64_t ncol, int64_t nlay, int64_t
     RRTMGP_TIMED("aerosol_optics_kernel", as_stream(stream));                                        \
     hipLaunchKernelGGL(aerosol_optics_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream), \
                        ncol, nlay, nbnd, nhum, mix, rh, plev, rh_upper, mext_phobic, ssa_phobic,     \
-                       g_phobic, mext_philic, ssa_philic, g_philic, delta_scale, tau, ssa, g);       \
+                       g_phobic, mext_philic, ssa_philic, g_philic, delta_scale, coalbedo, tau, ssa, \
+                       g);                                                                           \
     RRTMGP_LAUNCHED("aerosol_optics_kernel");                                                       \
     return RRTMGP_OK;                                                                                \
 }                                                                                                    \
@@ -440,6 +499,20 @@ extern "C" int rrtmgp_hip_inc_bybnd_##S(int64_t ncol, int64_t nlay, int64_t ngpt
     hipLaunchKernelGGL(inc_bybnd_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
                        ncol, nlay, ngpt, tau_inout, ssa_inout, g_inout, tau_in, ssa_in, g_in, nbnd, \
                        band_lims_gpt);                                                               \
+    RRTMGP_LAUNCHED("inc_bybnd_kernel");                                                            \
+    return RRTMGP_OK;                                                                                \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_inc_bybnd_coalbedo_##S(int64_t ncol, int64_t nlay, int64_t ngpt,       \
+    T* tau_inout, T* coalbedo_inout, T* g_inout, const T* tau_in, const T* coalbedo_in,             \
+    const T* g_in, int64_t nbnd, const int* band_lims_gpt, void* stream)                             \
+{                                                                                                    \
+    if (!coalbedo_inout || !coalbedo_in || !g_inout || !g_in)                                        \
+        return arg_error("inc_bybnd_coalbedo: two-stream arrays required");                         \
+    const int64_t n = ncol * nlay * ngpt;                                                            \
+    RRTMGP_TIMED("inc_bybnd_kernel", as_stream(stream));                                             \
+    hipLaunchKernelGGL(inc_bybnd_kernel<T>, dim3(grid1(n, 256)), dim3(256), 0, as_stream(stream),   \
+                       ncol, nlay, ngpt, tau_inout, coalbedo_inout, g_inout, tau_in, coalbedo_in,   \
+                       g_in, nbnd, band_lims_gpt, 1);                                                \
     RRTMGP_LAUNCHED("inc_bybnd_kernel");                                                            \
     return RRTMGP_OK;                                                                                \
 }                                                                                                    \

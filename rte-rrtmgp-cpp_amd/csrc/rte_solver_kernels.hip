@@ -415,10 +415,9 @@ struct LayerSW { double R, T, SU, SD, Tn, ab, omTn; };
 // Rdir + Tdir, keeps its series too (see the clamp).  The transmittance e^{-tau/mu0} of this layer enters G
 // and Tdir only; the direct beam itself is carried by the caller (Beam<float>).  Returns false
 // where FP64 is kept: near conservative scattering (1 - w0 < 1e-6).
-__device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, float mu0, float inv_mu0, double dir,
-                                              LayerSW& L)
+__device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float one_m_w0, float g, float mu0, float inv_mu0,
+                                              double dir, LayerSW& L)
 {
-    const float one_m_w0 = 1.f - w0;
     if (one_m_w0 < 1e-6f) return false;
     const float gamma1 = (8.f - w0 * (5.f + 3.f * g)) * .25f;
     const float gamma2 = 3.f * (w0 * (1.f - g)) * .25f;
@@ -469,15 +468,19 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float g, floa
 }
 
 // FP32 builds take the FP32 form where it is accurate and the FP64 island elsewhere; FP64
-// builds evaluate the reference expressions in FP64.
-template<typename Real>
-__device__ __forceinline__ LayerSW sw_layer(Real tau, Real w0, Real g, Real mu0, Real inv_mu0, double dir)
+// builds evaluate the reference expressions in FP64.  With CO the second input is the
+// co-albedo 1 - w0 (exact in the caller's precision, where w0 near 1 is not: FP32 resolves
+// 1 - w0 only to 6e-8).
+template<typename Real, bool CO = false>
+__device__ __forceinline__ LayerSW sw_layer(Real tau, Real w_in, Real g, Real mu0, Real inv_mu0, double dir)
 {
     LayerSW L;
+    const Real w0 = CO ? Real(1) - w_in : w_in;
     if constexpr (std::is_same<Real, float>::value)
-        if (sw_layer_fp32(tau, w0, g, mu0, inv_mu0, dir, L)) return L;
+        if (sw_layer_fp32(tau, w0, CO ? w_in : 1.f - w_in, g, mu0, inv_mu0, dir, L)) return L;
+    const double w0d = CO ? 1. - double(w_in) : double(w_in);
     double Rdif, Tdif, Rdir, Tdir, Tn;
-    sw_two_stream(double(tau), double(w0), double(g), double(mu0), Rdif, Tdif, Rdir, Tdir, Tn);
+    sw_two_stream(double(tau), w0d, double(g), double(mu0), Rdif, Tdif, Rdir, Tdir, Tn);
     return {Rdif, Tdif, Rdir * dir, Tdir * dir, Tn, 1. - Rdif - Tdif, -expm1(-double(tau) / double(mu0))};
 }
 
@@ -520,7 +523,8 @@ template<> struct Beam<float>
     }
 };
 
-template<typename Real, bool HR = false>
+// CO: the ssa argument holds the co-albedo 1 - ssa (see sw_layer).
+template<typename Real, bool HR = false, bool CO = false>
 __global__ __launch_bounds__(SB_COLS * SB_GPTS)
 void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                       const Real* __restrict__ tau, const Real* __restrict__ ssa,
@@ -596,7 +600,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
-                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, beam.dir);
+                    const LayerSW L = sw_layer<Real, CO>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, beam.dir);
                     above_update(L, R_up, D);
                     beam.step(L, tv[kk], inv_m0r);
                 }
@@ -652,7 +656,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                 if (kk < nk)
                 {
                     sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(bm.dir);
-                    const LayerSW L = sw_layer<Real>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, bm.dir);
+                    const LayerSW L = sw_layer<Real, CO>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, bm.dir);
                     sR[kk] = Real(L.R); sT[kk] = Real(L.T); sSU[kk] = Real(L.SU); sSD[kk] = Real(L.SD);
                     if constexpr (HR) { sAb[kk] = Real(L.ab); sOmTn[kk] = Real(L.omTn); }
                     above_update(L, r_up, d);
@@ -907,7 +911,7 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
               int64_t nbnd, const int* band_lims, Real* bnd_up, Real* bnd_dn, Real* bnd_dir,
               Real* bb_up, Real* bb_dn, Real* bb_dir, void* stream, bool by_band = false,
               const Real* plev = nullptr, Real* heating_rate = nullptr, unsigned char* workspace = nullptr,
-              size_t workspace_bytes = 0)
+              size_t workspace_bytes = 0, bool coalbedo = false)
 {
     if (ncol <= 0 || nlay <= 0 || ngpt <= 0) return arg_error("sw_solver: empty extents");
     if (heating_rate && (!plev || flux_up || flux_dn || flux_dir))
@@ -978,7 +982,8 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     int rc = RRTMGP_OK;
     {
         RRTMGP_TIMED("sw_solver_kernel", s);
-        auto kernel = heating_rate ? sw_solver_kernel<Real, true> : sw_solver_kernel<Real, false>;
+        auto kernel = heating_rate ? (coalbedo ? sw_solver_kernel<Real, true, true> : sw_solver_kernel<Real, true>)
+                                   : (coalbedo ? sw_solver_kernel<Real, false, true> : sw_solver_kernel<Real, false>);
         hipLaunchKernelGGL(kernel, grid, block, 0, s, ncol, nlay, ngpt, top_at_1, tau, ssa, g, mu0, alb_dir, alb_dif, inc_dir,
                            has_dif_bc ? inc_dif : nullptr, flux_up, flux_dn, flux_dir, ckpt, grp, gf[0], gf[1], gf[2],
                            by_band ? nbnd : int64_t(0), absorbed);
@@ -1081,6 +1086,19 @@ extern "C" int rrtmgp_hip_sw_solver_2stream_bnd_##S(int64_t ncol, int64_t nlay, 
                         inc_flux_dir, nullptr, nullptr, nullptr, has_dif_bc, inc_flux_dif, nbnd,     \
                         band_lims_gpt, bnd_flux_up, bnd_flux_dn, bnd_flux_dir, broadband_up,         \
                         broadband_dn, broadband_dir, stream, true, plev, heating_rate);              \
+}                                                                                                    \
+extern "C" int rrtmgp_hip_sw_solver_2stream_bnd_coalbedo_##S(int64_t ncol, int64_t nlay,           \
+    int64_t ngpt, int top_at_1, const T* tau, const T* coalbedo, const T* g, const T* mu0,          \
+    const T* sfc_alb_dir, const T* sfc_alb_dif, const T* inc_flux_dir, int has_dif_bc,             \
+    const T* inc_flux_dif, int64_t nbnd, const int* band_lims_gpt, T* bnd_flux_up, T* bnd_flux_dn,  \
+    T* bnd_flux_dir, T* broadband_up, T* broadband_dn, T* broadband_dir, const T* plev,             \
+    T* heating_rate, void* stream)                                                                   \
+{                                                                                                    \
+    return sw_solver<T>(ncol, nlay, ngpt, top_at_1, tau, coalbedo, g, mu0, sfc_alb_dir,            \
+                        sfc_alb_dif, inc_flux_dir, nullptr, nullptr, nullptr, has_dif_bc,            \
+                        inc_flux_dif, nbnd, band_lims_gpt, bnd_flux_up, bnd_flux_dn, bnd_flux_dir,   \
+                        broadband_up, broadband_dn, broadband_dir, stream, true, plev, heating_rate, \
+                        nullptr, 0, true);                                                           \
 }
 
 #define RRTMGP_DEF_SW_NOALLOC(S, T)                                                                 \

@@ -27,12 +27,13 @@ class Aerosol_optics_gpu : public Optical_props_gpu
                 const Array_gpu<Float,2>& rh, const Array_gpu<Float,2>& plev,
                 Optical_props_2str_gpu& optical_props);
 
-        // The same, with delta_scale() applied in the same pass when delta_scale is set; the
-        // concentrations are only read.
+        // The same, with delta_scale() applied in the same pass when delta_scale is set, and the
+        // co-albedo 1 - ssa written in place of ssa when coalbedo is set; the concentrations are
+        // only read.
         void compute(
                 const Aerosol_concs_gpu& aerosol_concs,
                 const Array_gpu<Float,2>& rh, const Array_gpu<Float,2>& plev,
-                bool delta_scale, Optical_props_2str_gpu& optical_props);
+                bool delta_scale, Optical_props_2str_gpu& optical_props, bool coalbedo = false);
 
     private:
 

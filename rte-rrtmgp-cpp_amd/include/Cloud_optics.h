@@ -32,6 +32,13 @@ class Cloud_optics_gpu : public Optical_props_gpu
                 const Array_gpu<Float,2>& reliq, const Array_gpu<Float,2>& reice,
                 Optical_props_2str_gpu& optical_props);
 
+        // Two-stream variant writing the co-albedo 1 - ssa in place of ssa (delta-scaled when
+        // delta_scale), for the shortwave driver's co-albedo chain.
+        void cloud_optics_coalbedo(
+                const Array_gpu<Float,2>& clwp, const Array_gpu<Float,2>& ciwp,
+                const Array_gpu<Float,2>& reliq, const Array_gpu<Float,2>& reice,
+                bool delta_scale, Optical_props_2str_gpu& optical_props);
+
     private:
         void run(const Array_gpu<Float,2>& clwp, const Array_gpu<Float,2>& ciwp,
                  const Array_gpu<Float,2>& reliq, const Array_gpu<Float,2>& reice,

@@ -133,7 +133,9 @@ class Gas_optics_rrtmgp_gpu : public Gas_optics_gpu
                 Array_gpu<Float,2>& toa_src, const Array_gpu<Float,2>& col_dry);
 
         // Fused variants that also add band-resolved cloud properties (an add_to folded into
-        // the gas-optics write).  cloud may be nullptr.
+        // the gas-optics write).  cloud may be nullptr.  The shortwave one with coalbedo writes
+        // the co-albedo 1 - ssa in place of ssa, and reads the cloud's as co-albedo too
+        // (rrtmgp_hip_gas_optics_sw_coalbedo_*; plain gas optics only, not with a surrogate).
         void gas_optics_with_clouds(
                 const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev,
                 const Array_gpu<Float,2>& tlay, const Array_gpu<Float,1>& tsfc,
@@ -144,7 +146,8 @@ class Gas_optics_rrtmgp_gpu : public Gas_optics_gpu
                 const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev,
                 const Array_gpu<Float,2>& tlay, const Gas_concs_gpu& gas_desc,
                 Optical_props_arry_gpu& optical_props, Array_gpu<Float,2>& toa_src,
-                const Array_gpu<Float,2>& col_dry, const Optical_props_arry_gpu* cloud);
+                const Array_gpu<Float,2>& col_dry, const Optical_props_arry_gpu* cloud,
+                bool coalbedo = false);
 
         // gas concentrations as the rrtmgp_gas_vmr descriptor of the kernels, in this
         // k-distribution's gas order

@@ -13,6 +13,8 @@ class Optical_props_1scl_gpu;
 class Optical_props_2str_gpu;
 void add_to(Optical_props_1scl_gpu& op_inout, const Optical_props_1scl_gpu& op_in);
 void add_to(Optical_props_2str_gpu& op_inout, const Optical_props_2str_gpu& op_in);
+// add_to of band-resolved two-stream properties whose ssa arrays both hold co-albedos 1 - ssa.
+void add_to_coalbedo(Optical_props_2str_gpu& op_inout, const Optical_props_2str_gpu& op_in);
 
 class Optical_props_gpu
 {

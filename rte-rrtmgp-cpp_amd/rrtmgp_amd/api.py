@@ -239,7 +239,8 @@ class Cloud(_Handle):
             self.L.rrtmgp_api_cloud_destroy(self.h)
             self.h = None
 
-    def optics(self, lwp, iwp, rel, rei, two_stream, delta_scale=False):
+    def optics(self, lwp, iwp, rel, rei, two_stream, delta_scale=False, coalbedo=False):
+        """tau (and with two_stream ssa, g) per band; with coalbedo the second array is 1 - ssa."""
         dt = real_dtype(self.precision)
         lwp, iwp, rel, rei = (_dev(a, dt) for a in (lwp, iwp, rel, rei))
         ncol, nlay = lwp.shape
@@ -247,7 +248,8 @@ class Cloud(_Handle):
         ssa = DeviceArray((ncol, nlay, self.nbnd), dt) if two_stream else None
         g = DeviceArray((ncol, nlay, self.nbnd), dt) if two_stream else None
         self._check(self.L.rrtmgp_api_cloud_optics(self.h, ncol, nlay, lwp.ptr, iwp.ptr, rel.ptr, rei.ptr,
-                                                   int(two_stream), int(delta_scale), tau.ptr, _p(ssa), _p(g)))
+                                                   2 if two_stream and coalbedo else int(two_stream),
+                                                   int(delta_scale), tau.ptr, _p(ssa), _p(g)))
         return (tau, ssa, g) if two_stream else tau
 
 
@@ -281,14 +283,15 @@ class Aerosol(_Handle):
             self.L.rrtmgp_api_aerosol_destroy(self.h)
             self.h = None
 
-    def optics(self, concs: GasConcs, rh, plev, delta_scale=False):
-        """tau, ssa, g (ncol, nlay, nbnd) of the aerosol mixing ratios held by concs."""
+    def optics(self, concs: GasConcs, rh, plev, delta_scale=False, coalbedo=False):
+        """tau, ssa, g (ncol, nlay, nbnd) of the aerosol mixing ratios held by concs; with coalbedo
+        the second array is 1 - ssa."""
         dt = real_dtype(self.precision)
         rh, plev = _dev(rh, dt), _dev(plev, dt)
         ncol, nlay = rh.shape
         out = [DeviceArray((ncol, nlay, self.nbnd), dt) for _ in range(3)]
         self._check(self.L.rrtmgp_api_aerosol_optics(self.h, ncol, nlay, concs.h, rh.ptr, plev.ptr,
-                                                     int(delta_scale), *[o.ptr for o in out]))
+                                                     int(delta_scale), int(coalbedo), *[o.ptr for o in out]))
         return tuple(out)
 
 
@@ -302,8 +305,9 @@ def get_col_dry(vmr_h2o, plev, precision=8):
     return out
 
 
-def gas_optics(kd: KDist, atm, gas_concs: GasConcs, col_dry):
-    """Gas_optics_rrtmgp_gpu::gas_optics (LW or SW by k-distribution); returns DeviceArrays."""
+def gas_optics(kd: KDist, atm, gas_concs: GasConcs, col_dry, coalbedo=False):
+    """Gas_optics_rrtmgp_gpu::gas_optics (LW or SW by k-distribution); returns DeviceArrays.  With
+    coalbedo (SW) o["coalbedo"] = 1 - ssa replaces o["ssa"]."""
     dt = real_dtype(kd.precision)
     ncol, nlay = atm.p_lay.shape
     ng = kd.ngpt
@@ -321,18 +325,24 @@ def gas_optics(kd: KDist, atm, gas_concs: GasConcs, col_dry):
             o["tau"].ptr, o["lay_src"].ptr, o["lev_src_inc"].ptr, o["lev_src_dec"].ptr,
             o["sfc_src"].ptr, o["sfc_src_jac"].ptr))
     else:
-        o["ssa"] = DeviceArray((ncol, nlay, ng), dt)
+        key = "coalbedo" if coalbedo else "ssa"
+        o[key] = DeviceArray((ncol, nlay, ng), dt)
         o["g"] = DeviceArray((ncol, nlay, ng), dt)
         o["toa_src"] = DeviceArray((ncol, ng), dt)
-        kd._check(kd.L.rrtmgp_api_gas_optics_sw(
-            kd.h, ncol, nlay, play.ptr, plev.ptr, tlay.ptr, gas_concs.h, cd.ptr, o["tau"].ptr,
-            o["ssa"].ptr, o["g"].ptr, o["toa_src"].ptr))
+        fn = kd.L.rrtmgp_api_gas_optics_sw_coalbedo if coalbedo else kd.L.rrtmgp_api_gas_optics_sw
+        kd._check(fn(kd.h, ncol, nlay, play.ptr, plev.ptr, tlay.ptr, gas_concs.h, cd.ptr, o["tau"].ptr,
+                     o[key].ptr, o["g"].ptr, o["toa_src"].ptr))
     return o
 
 
-def add_to(kd: KDist, tau, tau_in, ssa=None, g=None, ssa_in=None, g_in=None):
-    """add_to(op_inout, op_in): same spectral grid or band-resolved input; in place."""
+def add_to(kd: KDist, tau, tau_in, ssa=None, g=None, ssa_in=None, g_in=None, coalbedo=False):
+    """add_to(op_inout, op_in): same spectral grid or band-resolved input; in place.  With coalbedo
+    ssa and ssa_in hold 1 - ssa and the input is band-resolved."""
     ncol, nlay, _ = tau.shape
+    if coalbedo:
+        kd._check(kd.L.rrtmgp_api_add_to_coalbedo(kd.h, ncol, nlay, tau.ptr, ssa.ptr, g.ptr, tau_in.ptr,
+                                                  ssa_in.ptr, g_in.ptr))
+        return
     kd._check(kd.L.rrtmgp_api_add_to(kd.h, ncol, nlay, tau_in.shape[2], tau.ptr, _p(ssa), _p(g),
                                      tau_in.ptr, _p(ssa_in), _p(g_in)))
 
@@ -369,16 +379,19 @@ def rte_sw(kd: KDist, top_at_1, tau, ssa, g, mu0, inc_flux_dir, sfc_alb_dir, sfc
     return up, dn, dr
 
 
-def rte_sw_bnd(kd: KDist, top_at_1, tau, ssa, g, mu0, inc_flux_dir, sfc_alb_dir, sfc_alb_dif, p_lev=None):
+def rte_sw_bnd(kd: KDist, top_at_1, tau, ssa, g, mu0, inc_flux_dir, sfc_alb_dir, sfc_alb_dif, p_lev=None,
+               coalbedo=False):
     """Broadband SW fluxes summed inside the solver (rrtmgp_hip_sw_solver_2stream_bnd_*), with
     albedos (nbnd, ncol); given p_lev (ncol, nlev), also heating rates (ncol, nlay) [K/day] from the
-    layers' absorbed flux summed over g-points in FP64.  Returns a dict of DeviceArrays."""
+    layers' absorbed flux summed over g-points in FP64.  With coalbedo ssa holds 1 - ssa.  Returns a
+    dict of DeviceArrays."""
     dt = real_dtype(kd.precision)
     ncol, nlay, ng = tau.shape
     mu0, inc, ad, af, pl = (_dev(a, dt) for a in (mu0, inc_flux_dir, sfc_alb_dir, sfc_alb_dif, p_lev))
     o = {k: DeviceArray((ncol, nlay + 1), dt) for k in ("flux_up", "flux_dn", "flux_dn_dir")}
     o["heating_rate"] = DeviceArray((ncol, nlay), dt) if pl is not None else None
-    fn = getattr(_lib.hip(), "rrtmgp_hip_sw_solver_2stream_bnd_" + ("f64" if kd.precision == 8 else "f32"))
+    fn = getattr(_lib.hip(), "rrtmgp_hip_sw_solver_2stream_bnd_" + ("coalbedo_" if coalbedo else "")
+                 + ("f64" if kd.precision == 8 else "f32"))
     bl = kd.band_lims
     i = C.c_int64
     _lib.check_hip(fn(i(ncol), i(nlay), i(ng), C.c_int(int(top_at_1)), tau.ptr, ssa.ptr, g.ptr, mu0.ptr, ad.ptr,

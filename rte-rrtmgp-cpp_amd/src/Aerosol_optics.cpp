@@ -29,7 +29,8 @@ Aerosol_optics_gpu::Aerosol_optics_gpu(
 }
 
 void Aerosol_optics_gpu::compute(const Aerosol_concs_gpu& concs, const Array_gpu<Float,2>& rh,
-                                 const Array_gpu<Float,2>& plev, bool delta, Optical_props_2str_gpu& p)
+                                 const Array_gpu<Float,2>& plev, bool delta, Optical_props_2str_gpu& p,
+                                 bool coalbedo)
 {
     const int ncol = rh.dim(1), nlay = rh.dim(2);
     if (plev.dim(1) != ncol || plev.dim(2) != nlay + 1)
@@ -50,8 +51,8 @@ void Aerosol_optics_gpu::compute(const Aerosol_concs_gpu& concs, const Array_gpu
     hip_check(K<Float>::aerosol_optics(
             ncol, nlay, get_nband(), nhum, nphobic, nphilic, &mix, rh.ptr(), plev.ptr(), rh_upper_gpu.ptr(),
             mext_phobic_gpu.ptr(), ssa_phobic_gpu.ptr(), g_phobic_gpu.ptr(), mext_philic_gpu.ptr(),
-            ssa_philic_gpu.ptr(), g_philic_gpu.ptr(), delta ? 1 : 0, p.get_tau().ptr(), p.get_ssa().ptr(), p.get_g().ptr(),
-            stream()));
+            ssa_philic_gpu.ptr(), g_philic_gpu.ptr(), delta ? 1 : 0, coalbedo ? 1 : 0, p.get_tau().ptr(),
+            p.get_ssa().ptr(), p.get_g().ptr(), stream()));
 }
 
 void Aerosol_optics_gpu::aerosol_optics(Aerosol_concs_gpu& concs, const Array_gpu<Float,2>& rh,

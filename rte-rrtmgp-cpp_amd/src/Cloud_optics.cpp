@@ -71,3 +71,10 @@ void Cloud_optics_gpu::cloud_optics_delta_scaled(const Array_gpu<Float,2>& clwp,
 {
     run(clwp, ciwp, reliq, reice, 1, 1, p.get_tau(), p.get_ssa().ptr(), p.get_g().ptr());
 }
+
+void Cloud_optics_gpu::cloud_optics_coalbedo(const Array_gpu<Float,2>& clwp, const Array_gpu<Float,2>& ciwp,
+                                             const Array_gpu<Float,2>& reliq, const Array_gpu<Float,2>& reice,
+                                             bool delta_scale, Optical_props_2str_gpu& p)
+{
+    run(clwp, ciwp, reliq, reice, 2, delta_scale ? 1 : 0, p.get_tau(), p.get_ssa().ptr(), p.get_g().ptr());
+}

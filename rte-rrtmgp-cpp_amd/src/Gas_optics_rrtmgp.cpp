@@ -615,9 +615,10 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
 void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
         const Array_gpu<Float,2>& play, const Array_gpu<Float,2>& plev, const Array_gpu<Float,2>& tlay,
         const Gas_concs_gpu& gas_desc, Optical_props_arry_gpu& props, Array_gpu<Float,2>& toa_src,
-        const Array_gpu<Float,2>& col_dry, const Optical_props_arry_gpu* cloud)
+        const Array_gpu<Float,2>& col_dry, const Optical_props_arry_gpu* cloud, bool coalbedo)
 {
     if (krayl.size() == 0) throw std::runtime_error("shortwave gas_optics called on a longwave k-distribution");
+    if (coalbedo && use_mlp) throw std::invalid_argument("co-albedo gas optics with a surrogate set");
     const int ncol = play.dim(1), nlay = play.dim(2);
     (void)check_inputs(play, plev, tlay, col_dry, nullptr, nullptr);
     rrtmgp_gas_vmr v;
@@ -641,6 +642,10 @@ void Gas_optics_rrtmgp_gpu::gas_optics_with_clouds(
                                                     props.get_tau().ptr(), props.get_ssa().ptr(),
                                                     props.get_g().ptr(), stream()));
     }
+    else if (coalbedo)
+        hip_check(K<Float>::gas_optics_sw_coalbedo(&dev, ncol, nlay, play.ptr(), tlay.ptr(), col_dry.ptr(), &v, ct,
+                                                   cs, cg, props.get_tau().ptr(), props.get_ssa().ptr(),
+                                                   props.get_g().ptr(), stream()));
     else
         hip_check(K<Float>::gas_optics_sw(&dev, ncol, nlay, play.ptr(), tlay.ptr(), col_dry.ptr(), &v, ct, cs, cg,
                                           props.get_tau().ptr(), props.get_ssa().ptr(), props.get_g().ptr(),

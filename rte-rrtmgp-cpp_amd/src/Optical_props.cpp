@@ -85,3 +85,13 @@ void add_to(Optical_props_2str_gpu& a, const Optical_props_2str_gpu& b)
                                       a.get_band_lims_gpoint_gpu().ptr(), stream()));
     }
 }
+
+void add_to_coalbedo(Optical_props_2str_gpu& a, const Optical_props_2str_gpu& b)
+{
+    if (b.get_ngpt() != a.get_nband())
+        throw std::runtime_error("add_to_coalbedo: the input must be band-resolved on the output's bands");
+    hip_check(K<Float>::inc_bybnd_coalbedo(a.get_ncol(), a.get_nlay(), a.get_ngpt(), a.get_tau().ptr(),
+                                           a.get_ssa().ptr(), a.get_g().ptr(), b.get_tau().ptr(), b.get_ssa().ptr(),
+                                           b.get_g().ptr(), a.get_nband(), a.get_band_lims_gpoint_gpu().ptr(),
+                                           stream()));
+}

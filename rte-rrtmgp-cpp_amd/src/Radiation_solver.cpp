@@ -328,24 +328,31 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
         else
             cd = block_of(col_dry, c0, n, ws.col_dry);
 
+        // With the plain gas optics the whole chain carries the co-albedo 1 - ssa in the ssa
+        // arrays (FP32 keeps absorption far below 6e-8 of extinction only that way); a surrogate
+        // writes ssa.
+        const bool co = !kdist->has_surrogate();
         const Optical_props_arry_gpu* cloud = nullptr;
         if (switch_cloud_optics)
         {
             auto a = block_of(lwp, c0, n, ws.lwp), b = block_of(iwp, c0, n, ws.iwp);
             auto c = block_of(rel, c0, n, ws.rel), d = block_of(rei, c0, n, ws.rei);
-            if (switch_delta_cloud) cloud_optics->cloud_optics_delta_scaled(a, b, c, d, *cloud_props);
+            if (co) cloud_optics->cloud_optics_coalbedo(a, b, c, d, switch_delta_cloud, *cloud_props);
+            else if (switch_delta_cloud) cloud_optics->cloud_optics_delta_scaled(a, b, c, d, *cloud_props);
             else cloud_optics->cloud_optics(a, b, c, d, *cloud_props);
             cloud = cloud_props.get();
         }
-        kdist->gas_optics_with_clouds(pl, pv, tl, gc, *optical_props, toa_src, cd, cloud);
+        kdist->gas_optics_with_clouds(pl, pv, tl, gc, *optical_props, toa_src, cd, cloud, co);
         if (switch_aerosol_optics)
         {
             std::unique_ptr<Aerosol_concs_gpu> asub;
             if (!whole) asub = std::make_unique<Aerosol_concs_gpu>(aerosol_concs, c0 + 1, n);
             Array_gpu<Float,2> rhb = block_of(rh, c0, n, ws.rh);
             const Aerosol_concs_gpu& ac = whole ? aerosol_concs : *asub;
-            aerosol_optics->compute(ac, rhb, pv, switch_delta_aerosol, *aerosol_props);
-            add_to(dynamic_cast<Optical_props_2str_gpu&>(*optical_props), *aerosol_props);
+            aerosol_optics->compute(ac, rhb, pv, switch_delta_aerosol, *aerosol_props, co);
+            auto& ops = dynamic_cast<Optical_props_2str_gpu&>(*optical_props);
+            if (co) add_to_coalbedo(ops, *aerosol_props);
+            else add_to(ops, *aerosol_props);
         }
         // TSI scaling of the incoming flux (src_test/Radiation_solver.cpp:739-743).
         hip_check(K<Float>::scale_by_col(n, ngpt, toa_src.ptr(), tsi.ptr(), stream()));
@@ -364,7 +371,7 @@ void Radiation_solver_shortwave_gpu::solve_gpu(
             hb = whole ? sw_heating_rate : &blk_hr;
         }
         rte.rte_sw_fluxes(optical_props, top_at_1, m0, toa_src, ad, af, Array_gpu<Float,2>(), bu, bd, br, fu, fd, fr,
-                          &pv, hb);
+                          &pv, hb, co);
         if (!whole)
         {
             if (hr) scatter(blk_hr.ptr(), n, nlay, sw_heating_rate->ptr(), ncol, c0);

@@ -52,7 +52,7 @@ void Rte_sw_gpu::rte_sw_fluxes(
         Array_gpu<Float,3>& bnd_flux_up, Array_gpu<Float,3>& bnd_flux_dn,
         Array_gpu<Float,3>& bnd_flux_dir,
         Array_gpu<Float,2>& flux_up, Array_gpu<Float,2>& flux_dn, Array_gpu<Float,2>& flux_dir,
-        const Array_gpu<Float,2>* p_lev, Array_gpu<Float,2>* heating_rate)
+        const Array_gpu<Float,2>* p_lev, Array_gpu<Float,2>* heating_rate, bool coalbedo)
 {
     const int ncol = optical_props->get_ncol();
     const int nlay = optical_props->get_nlay();
@@ -70,7 +70,8 @@ void Rte_sw_gpu::rte_sw_fluxes(
     const bool has_dif_bc = inc_flux_dif.size() > 0;
     const bool bnd = bnd_flux_up.size() > 0;
     const bool bb = flux_up.size() > 0;
-    hip_check(K<Float>::sw_solver_2stream_bnd(
+    auto solver = coalbedo ? K<Float>::sw_solver_2stream_bnd_coalbedo : K<Float>::sw_solver_2stream_bnd;
+    hip_check(solver(
             ncol, nlay, ngpt, top_at_1, optical_props->get_tau().ptr(), optical_props->get_ssa().ptr(),
             optical_props->get_g().ptr(), mu0.ptr(), sfc_alb_dir.ptr(), sfc_alb_dif.ptr(),
             inc_flux_dir.ptr(), has_dif_bc, has_dif_bc ? inc_flux_dif.ptr() : nullptr, nbnd,

@@ -297,9 +297,11 @@ int rrtmgp_api_gas_optics_lw(void* h, int ncol, int nlay, const rrtmgp_real* pla
     });
 }
 
-int rrtmgp_api_gas_optics_sw(void* h, int ncol, int nlay, const rrtmgp_real* play, const rrtmgp_real* plev,
-                             const rrtmgp_real* tlay, void* gas_concs, const rrtmgp_real* col_dry,
-                             rrtmgp_real* tau, rrtmgp_real* ssa, rrtmgp_real* g, rrtmgp_real* toa_src)
+namespace
+{
+int gas_optics_sw(void* h, int ncol, int nlay, const rrtmgp_real* play, const rrtmgp_real* plev,
+                  const rrtmgp_real* tlay, void* gas_concs, const rrtmgp_real* col_dry,
+                  rrtmgp_real* tau, rrtmgp_real* ssa, rrtmgp_real* g, rrtmgp_real* toa_src, bool coalbedo)
 {
     return guarded([&] {
         auto& k = *K_(h).k;
@@ -322,8 +324,23 @@ int rrtmgp_api_gas_optics_sw(void* h, int ncol, int nlay, const rrtmgp_real* pla
         auto toa = view<2>(toa_src, {ncol, ngpt});
         k.gas_optics_with_clouds(view<2>(play, {ncol, nlay}), view<2>(plev, {ncol, nlay + 1}),
                                  view<2>(tlay, {ncol, nlay}), *static_cast<Gas_concs_gpu*>(gas_concs), props,
-                                 toa, view<2>(col_dry, {ncol, nlay}), nullptr);
+                                 toa, view<2>(col_dry, {ncol, nlay}), nullptr, coalbedo);
     });
+}
+}
+
+int rrtmgp_api_gas_optics_sw(void* h, int ncol, int nlay, const rrtmgp_real* play, const rrtmgp_real* plev,
+                             const rrtmgp_real* tlay, void* gas_concs, const rrtmgp_real* col_dry,
+                             rrtmgp_real* tau, rrtmgp_real* ssa, rrtmgp_real* g, rrtmgp_real* toa_src)
+{
+    return gas_optics_sw(h, ncol, nlay, play, plev, tlay, gas_concs, col_dry, tau, ssa, g, toa_src, false);
+}
+
+int rrtmgp_api_gas_optics_sw_coalbedo(void* h, int ncol, int nlay, const rrtmgp_real* play, const rrtmgp_real* plev,
+                                      const rrtmgp_real* tlay, void* gas_concs, const rrtmgp_real* col_dry,
+                                      rrtmgp_real* tau, rrtmgp_real* coalbedo, rrtmgp_real* g, rrtmgp_real* toa_src)
+{
+    return gas_optics_sw(h, ncol, nlay, play, plev, tlay, gas_concs, col_dry, tau, coalbedo, g, toa_src, true);
 }
 
 int rrtmgp_api_kdist_fill_col_gas(void* h, int ncol, int nlay, void* gas_concs, const rrtmgp_real* col_dry,
@@ -384,8 +401,8 @@ int rrtmgp_api_aerosol_create(const rrtmgp_aerosol_lut* l, void** out)
 int rrtmgp_api_aerosol_destroy(void* h) { delete static_cast<Aerosol*>(h); return RRTMGP_OK; }
 
 int rrtmgp_api_aerosol_optics(void* h, int ncol, int nlay, void* concs, const rrtmgp_real* rh,
-                              const rrtmgp_real* plev, int delta_scale, rrtmgp_real* tau, rrtmgp_real* ssa,
-                              rrtmgp_real* g)
+                              const rrtmgp_real* plev, int delta_scale, int coalbedo, rrtmgp_real* tau,
+                              rrtmgp_real* ssa, rrtmgp_real* g)
 {
     return guarded([&] {
         auto& a = *static_cast<Aerosol*>(h)->a;
@@ -395,7 +412,7 @@ int rrtmgp_api_aerosol_optics(void* h, int ncol, int nlay, void* concs, const rr
         p.get_ssa() = Array_gpu<Float,3>(ssa, {ncol, nlay, nb});
         p.get_g() = Array_gpu<Float,3>(g, {ncol, nlay, nb});
         a.compute(*static_cast<Gas_concs_gpu*>(concs), view<2>(rh, {ncol, nlay}), view<2>(plev, {ncol, nlay + 1}),
-                  delta_scale != 0, p);
+                  delta_scale != 0, p, coalbedo != 0);
     });
 }
 
@@ -412,7 +429,10 @@ int rrtmgp_api_cloud_optics(void* h, int ncol, int nlay, const rrtmgp_real* lwp,
             p.get_tau() = Array_gpu<Float,3>(tau, {ncol, nlay, nb});
             p.get_ssa() = Array_gpu<Float,3>(ssa, {ncol, nlay, nb});
             p.get_g() = Array_gpu<Float,3>(g, {ncol, nlay, nb});
-            if (delta_scale)
+            if (two_stream == 2)
+                c.cloud_optics_coalbedo(view<2>(lwp, {ncol, nlay}), view<2>(iwp, {ncol, nlay}),
+                                        view<2>(rel, {ncol, nlay}), view<2>(rei, {ncol, nlay}), delta_scale != 0, p);
+            else if (delta_scale)
                 c.cloud_optics_delta_scaled(view<2>(lwp, {ncol, nlay}), view<2>(iwp, {ncol, nlay}),
                                             view<2>(rel, {ncol, nlay}), view<2>(rei, {ncol, nlay}), p);
             else
@@ -458,6 +478,25 @@ int rrtmgp_api_add_to(void* h, int ncol, int nlay, int nspec_in, rrtmgp_real* ta
             b.get_tau() = view<3>(tau_in, {ncol, nlay, nspec_in});
             add_to(a, b);
         }
+    });
+}
+
+int rrtmgp_api_add_to_coalbedo(void* h, int ncol, int nlay, rrtmgp_real* tau, rrtmgp_real* coalbedo,
+                               rrtmgp_real* g, const rrtmgp_real* tau_in, const rrtmgp_real* coalbedo_in,
+                               const rrtmgp_real* g_in)
+{
+    return guarded([&] {
+        auto& k = *K_(h).k;
+        const int ngpt = k.get_ngpt(), nbnd = k.get_nband();
+        const Optical_props_gpu band_only(k.get_band_lims_wavenumber());
+        Optical_props_2str_gpu a(0, 0, k), b(0, 0, band_only);
+        a.get_tau() = Array_gpu<Float,3>(tau, {ncol, nlay, ngpt});
+        a.get_ssa() = Array_gpu<Float,3>(coalbedo, {ncol, nlay, ngpt});
+        a.get_g() = Array_gpu<Float,3>(g, {ncol, nlay, ngpt});
+        b.get_tau() = view<3>(tau_in, {ncol, nlay, nbnd});
+        b.get_ssa() = view<3>(coalbedo_in, {ncol, nlay, nbnd});
+        b.get_g() = view<3>(g_in, {ncol, nlay, nbnd});
+        add_to_coalbedo(a, b);
     });
 }
 

@@ -24,6 +24,7 @@ template<typename T> struct K;
     static constexpr auto adjoint_levels = rrtmgp_hip_adjoint_levels_##S; \
     static constexpr auto gpt_to_band = rrtmgp_hip_gpt_to_band_##S; \
     static constexpr auto gas_optics_sw = rrtmgp_hip_gas_optics_sw_##S;  \
+    static constexpr auto gas_optics_sw_coalbedo = rrtmgp_hip_gas_optics_sw_coalbedo_##S; \
     static constexpr auto interpolation = rrtmgp_hip_interpolation_##S;  \
     static constexpr auto compute_tau_absorption = rrtmgp_hip_compute_tau_absorption_##S; \
     static constexpr auto compute_tau_rayleigh = rrtmgp_hip_compute_tau_rayleigh_##S;     \
@@ -35,6 +36,7 @@ template<typename T> struct K;
     static constexpr auto aerosol_optics = rrtmgp_hip_aerosol_optics_##S; \
     static constexpr auto increment = rrtmgp_hip_increment_##S;          \
     static constexpr auto inc_bybnd = rrtmgp_hip_inc_bybnd_##S;          \
+    static constexpr auto inc_bybnd_coalbedo = rrtmgp_hip_inc_bybnd_coalbedo_##S; \
     static constexpr auto delta_scale_2str = rrtmgp_hip_delta_scale_2str_##S; \
     static constexpr auto apply_bc = rrtmgp_hip_apply_bc_##S;            \
     static constexpr auto expand_and_transpose = rrtmgp_hip_expand_and_transpose_##S; \
@@ -43,6 +45,7 @@ template<typename T> struct K;
     static constexpr auto sw_solver_2stream = rrtmgp_hip_sw_solver_2stream_##S; \
     static constexpr auto lw_solver_noscat_bnd = rrtmgp_hip_lw_solver_noscat_bnd_##S; \
     static constexpr auto sw_solver_2stream_bnd = rrtmgp_hip_sw_solver_2stream_bnd_##S; \
+    static constexpr auto sw_solver_2stream_bnd_coalbedo = rrtmgp_hip_sw_solver_2stream_bnd_coalbedo_##S; \
     static constexpr auto sum_broadband = rrtmgp_hip_sum_broadband_##S;  \
     static constexpr auto net_broadband = rrtmgp_hip_net_broadband_##S;  \
     static constexpr auto sum_byband = rrtmgp_hip_sum_byband_##S;        \

@@ -45,7 +45,7 @@ def c3(syn, kd_lw):
 def _clear_of_nodes(x, x0, dx, step):
     """True where x lies more than six standard deviations of the step from the nodes x0 + n dx."""
     f = np.mod((x - x0) / dx, 1.0)
-    return np.minimum(f, 1.0 - f) * dx > 6.0 * np.abs(step)
+    return np.minimum(f, 1.0 - f) * abs(dx) > 6.0 * np.abs(step)
 
 
 def _relative_directions(rng, a, lw, kd):
@@ -165,9 +165,10 @@ def test_adjoint_c3_workload(c3, syn, O, okd_lw, okd_sw, kd_lw, kd_sw, lut_lw, l
     trials = [(k, {j: (v_all[j] if j == k else 0 * v_all[j]) for j in v_all}) for k in v_all]
     trials.append(("all", v_all))
     for name, v in trials:
+        scale = sum(float(np.sum(np.abs(g[k] * v[k]))) for k in v)
+        assert scale > 0, name
         fd = (loss(_perturbed(sub, v, H)) - loss(_perturbed(sub, v, -H))) / (2 * H)
         ad = sum(float(np.sum(g[k] * v[k])) for k in v)
-        scale = sum(float(np.sum(np.abs(g[k] * v[k]))) for k in v)
         print(f"  direction {name}: adjoint {ad:.8e}, central difference {fd:.8e}, "
               f"|ad - fd| / sum|g v| = {abs(ad - fd) / scale:.2e}", flush=True)
         assert abs(ad - fd) <= 1e-3 * scale, (name, ad, fd, scale)

@@ -51,24 +51,36 @@ def test_driver_heating_rate_fp64(syn, O, okd_lw, okd_sw, kd_lw, kd_sw, lut_lw, 
         assert rel_err(g["flux_dn"].to_host(), r["flux_dn"], floor=1e-6) < 1e-10
 
 
+def _forced_cloud_coalbedo(cloud, atm, cons, dt):
+    """Cloud optics in co-albedo form (tau, 1 - ssa, g), forced near-conservative (1 - ssa = 1e-7,
+    g = 0.85) where cons is set, then delta-scaled in FP64 on the host (f = g^2; tau (1 - f ssa),
+    with 1 - f ssa = (1 - f) + f (1 - ssa); (1 - ssa) / (1 - f ssa); (g - f) / (1 - f)) and uploaded."""
+    from rrtmgp_amd import device
+    ct, cc, cg = (a.to_host().astype(np.float64)
+                  for a in cloud.optics(atm.lwp, atm.iwp, atm.rel, atm.rei, True, coalbedo=True))
+    m = cons[:, :, None]
+    cc = np.where(m, 1e-7, cc)
+    cg = np.where(m, 0.85, cg)
+    f = cg * cg
+    d = (1.0 - f) + f * cc
+    ct, cc, cg = ct * d, cc / np.maximum(d, np.finfo(np.float64).eps), (cg - f) / np.maximum(1.0 - f, 1e-12)
+    return tuple(device.DeviceArray.from_host(a, dt) for a in (ct, cc, cg))
+
+
 def _sw_forced(api, device, syn, atm, kd, cloud, prec, cons):
     """SW gas optics + cloud optics with forced near-conservative layers, delta-scaled and added,
-    then the band-summed solver with heating rates."""
+    then the band-summed solver with heating rates; the whole chain in co-albedo form."""
     dt = api.real_dtype(prec)
     ncol, nlay = atm.p_lay.shape
     vmr = {g: np.broadcast_to(np.asarray(atm.vmr[g], np.float64), (ncol, nlay)).copy(order="F") for g in atm.vmr}
     gc = api.GasConcs(vmr, prec)
     cd = api.get_col_dry(vmr["h2o"], atm.p_lev, prec)
-    go = api.gas_optics(kd, atm, gc, cd)
-    ct, cs, cg = cloud.optics(atm.lwp, atm.iwp, atm.rel, atm.rei, True)
-    m = cons[:, :, None]
-    cs = device.DeviceArray.from_host(np.where(m, 1.0 - 1e-7, cs.to_host()), dt)
-    cg = device.DeviceArray.from_host(np.where(m, 0.85, cg.to_host()), dt)
-    api.delta_scale(ct, cs, cg, prec)
-    api.add_to(kd, go["tau"], ct, go["ssa"], go["g"], cs, cg)
+    go = api.gas_optics(kd, atm, gc, cd, coalbedo=True)
+    ct, cc, cg = _forced_cloud_coalbedo(cloud, atm, cons, dt)
+    api.add_to(kd, go["tau"], ct, go["coalbedo"], go["g"], cc, cg, coalbedo=True)
     toa = device.DeviceArray.from_host(go["toa_src"].to_host() * (atm.tsi / kd.scalar("tsi")), dt)
-    return api.rte_sw_bnd(kd, atm.top_at_1, go["tau"], go["ssa"], go["g"], atm.mu0, toa, atm.sfc_alb,
-                          0.5 * atm.sfc_alb, p_lev=atm.p_lev)
+    return api.rte_sw_bnd(kd, atm.top_at_1, go["tau"], go["coalbedo"], go["g"], atm.mu0, toa, atm.sfc_alb,
+                          0.5 * atm.sfc_alb, p_lev=atm.p_lev, coalbedo=True)
 
 
 def test_heating_rate_fp32_island(syn, kd_lw, kd_sw, lut_lw, lut_sw):

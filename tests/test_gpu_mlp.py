@@ -269,8 +269,9 @@ def test_inference_matches_numpy(S, syn, kd32_lw, kd32_sw, ts_lw, ts_sw, lw):
         # In log space, as for LW: exp() turns a rare BF16 rounding flip into ~1% of tau.
         big = tt > 1e-30
         assert_close_log(np.log(tau[big]), np.log(tt[big]), "ln tau")
-        # ssa = tau_ray / tau where tau > 2 FLT_EPSILON, else 0 (the plain combine rule).
-        eps2 = 2 * np.finfo(np.float32).eps
+        # ssa = tau_ray / tau where tau > 2 epsilon (FP64's, in both precisions), else 0 (the plain
+        # combine rule).
+        eps2 = 2 * np.finfo(np.float64).eps
         ref_ssa = np.where(tt > eps2, tray / np.where(tt > 0, tt, 1.0), 0.0)
         near = np.abs(tt - eps2) < 1e-2 * eps2                       # threshold ties: either side
         d = np.abs(np.where(near, 0, ssa) - np.where(near, 0, ref_ssa))
@@ -492,7 +493,7 @@ def test_inference_cloudy_top_at_1(S, syn, L, D, kd32_lw, kd32_sw, ts_lw, ts_sw,
         tabs = np.exp(y[:, 0::2]) * cdf
         tray = np.exp(y[:, 1::2]) * cdf
         tt = tabs + tray
-        eps2 = 2 * np.finfo(np.float32).eps
+        eps2 = 2 * np.finfo(np.float64).eps
         ss = np.where(tt > eps2, tray / np.where(tt > 0, tt, 1.0), 0.0)
         t2, s2, g2 = cl(ctau), cl(cssa), cl(cgg)
         tau12 = tt + t2

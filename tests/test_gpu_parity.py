@@ -140,10 +140,10 @@ def _eta_edge_mask(O, okd, atm, neta):
 def test_gas_optics_fp32(G, gkd, okd_lw, okd_sw, atm_c0, O, syn, lw):
     """FP32 build against the FP64 oracle: 2e-4 relative (floor 1e-6 of the field's maximum).
 
-    Two FP32 semantics of the reference are reproduced rather than FP64's: the CPU combine rule
-    sets ssa = 0 where tau <= 2 epsilon (src/Gas_optics_rrtmgp.cpp:365-385), with epsilon the
-    FP32 one; and the eta = 1 jump (see _eta_edge_mask).  Elements within 1e-3 of the ssa
-    threshold, where either side is right, are skipped."""
+    The eta = 1 jump is an FP32 semantic of the reference reproduced rather than FP64's (see
+    _eta_edge_mask).  The CPU combine rule (ssa = 0 where tau <= 2 epsilon,
+    src/Gas_optics_rrtmgp.cpp:365-385) is taken with FP64's epsilon in both precisions, so the
+    thin top layers keep their single-scattering albedo in FP32 too; the test counts them."""
     okd = okd_lw if lw else okd_sw
     ref, got = _gas_optics_both(G, gkd, okd, atm_c0, lw, 4, O)
     edge = _eta_edge_mask(O, okd, atm_c0, syn.NETA)
@@ -151,13 +151,9 @@ def test_gas_optics_fp32(G, gkd, okd_lw, okd_sw, atm_c0, O, syn, lw):
     assert edge.mean() < 0.01
     keep3 = ~edge
     if not lw:
-        thr = 2.0 * np.finfo(np.float32).eps
-        tau = ref["tau"]
-        ref = dict(ref)
-        ref["ssa"] = np.where(tau > thr, ref["ssa"], 0.0)
-        ref["g"] = np.where(tau > thr, ref["g"], 0.0)
-        keep3 &= np.abs(tau / thr - 1.0) > 1e-3
-        print(f"layers below the FP32 combine threshold: {int((tau <= thr).sum())}")
+        thin = int((ref["tau"] <= 2.0 * np.finfo(np.float32).eps).sum())
+        print(f"layers below FP32's 2 epsilon, ssa kept: {thin}")
+        assert thin > 0
     for n in ref:
         scale = np.abs(ref[n]).max()
         keep = keep3 if ref[n].ndim == 3 else np.ones(ref[n].shape, dtype=bool)
