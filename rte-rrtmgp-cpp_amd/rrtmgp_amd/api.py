@@ -485,6 +485,7 @@ class SolverLW(_Solver):
         if heating_rate and o.get("heating_rate") is None:
             o["heating_rate"] = DeviceArray((ncol, nlay), dt)
         cloudy = da.cloudy and self.cloud is not None
+        col_dry = _dev(col_dry, dt)
         self._check(self.L.rrtmgp_api_solve_lw(
             self.h, ncol, nlay, int(cloudy), int(bnd), da.p_lay.ptr, da.p_lev.ptr, da.t_lay.ptr,
             da.t_lev.ptr, da.t_sfc.ptr, da.gas_concs.h, _p(col_dry), da.emis_sfc.ptr,
@@ -530,10 +531,11 @@ class SolverSW(_Solver):
         self.h = h
 
     def solve(self, da: DeviceAtmosphere, tsi_scaling, bnd=False, delta_cloud=True, col_dry=None, outputs=None,
-              heating_rate=False, aerosol=None, delta_aerosol=False):
+              heating_rate=False, aerosol=None, delta_aerosol=False, sfc_alb_dif=None):
         """As SolverLW.solve, with the direct flux; heating_rate as there.  aerosol = (rh, concs):
         relative humidity (ncol, nlay) and a GasConcs of aermr01..aermr11, added after the clouds
-        (delta-scaled with delta_aerosol)."""
+        (delta-scaled with delta_aerosol).  The diffuse albedo (nbnd, ncol) defaults to the direct
+        one (da.sfc_alb)."""
         dt = real_dtype(self.precision)
         ncol, nlay = da.ncol, da.nlay
         nlev = nlay + 1
@@ -548,10 +550,12 @@ class SolverSW(_Solver):
         cloudy = da.cloudy and self.cloud is not None
         rh, aconcs = (None, None) if aerosol is None else aerosol
         rh = _dev(rh, dt) if aerosol is not None else None
+        col_dry = _dev(col_dry, dt)
+        af = da.sfc_alb if sfc_alb_dif is None else _dev(sfc_alb_dif, dt)
         self._check(self.L.rrtmgp_api_solve_sw_aerosol(
             self.h, ncol, nlay, int(cloudy), int(aerosol is not None), int(bnd), int(delta_cloud),
             int(delta_aerosol), da.p_lay.ptr, da.p_lev.ptr,
-            da.t_lay.ptr, da.t_lev.ptr, da.gas_concs.h, _p(col_dry), da.sfc_alb.ptr, da.sfc_alb.ptr,
+            da.t_lay.ptr, da.t_lev.ptr, da.gas_concs.h, _p(col_dry), da.sfc_alb.ptr, af.ptr,
             tsi.ptr, da.mu0.ptr, _p(da.lwp) if cloudy else None, _p(da.iwp) if cloudy else None,
             _p(da.rel) if cloudy else None, _p(da.rei) if cloudy else None,
             _p(rh), aconcs.h if aconcs is not None else None,

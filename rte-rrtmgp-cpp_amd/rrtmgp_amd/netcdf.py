@@ -12,7 +12,13 @@ h5py, so this module reads the two on-disk formats those files come in:
   layout version 4), the deflate, shuffle and Fletcher-32 filters, fixed-point, floating-point and
   fixed-length string types.  A variable's dimension names come from its DIMENSION_LIST attribute
   (object references to the dimension-scale datasets, held in the global heap).
-* NetCDF-3 classic and 64-bit-offset files, through scipy.io.netcdf_file.
+* NetCDF-3: classic (CDF-1), 64-bit offset (CDF-2) and 64-bit data (CDF-5) files, read here
+  without scipy.
+
+`Writer` writes the NetCDF-3 formats (CDF-2, or CDF-5 once a fixed-size variable passes the
+4 GiB that CDF-2 allows all but its last variable); the driver's output file
+(test_rte_rrtmgp.cpp:332-611 writes it as NetCDF-4, with the same dimensions and variables) is
+written through it.
 
 `Dataset(path)` exposes `dimensions` (name -> size), `variables` (name -> `Variable`) and
 `get_variable(name, dims)`, the counterpart of Netcdf_handle::get_variable with its size check.
@@ -760,16 +766,11 @@ class Dataset:
             raise FormatError(f"{path}: neither NetCDF-4 (HDF5) nor NetCDF-3")
 
     def _open_classic(self, path: str):
-        import scipy.io
-        f = scipy.io.netcdf_file(path, "r", mmap=False)
-        for name, size in f.dimensions.items():
-            self.dimensions[name] = int(size) if size is not None else 0
-        for name, v in f.variables.items():
-            arr = np.array(v.data)
-            attrs = {k: (val.decode() if isinstance(val, bytes) else val) for k, val in v._attributes.items()}
-            self.variables[name] = Variable(name, tuple(v.dimensions), tuple(arr.shape),
-                                            (lambda a=arr: a), attrs)
-        f.close()
+        h = _read_classic_header(path)
+        self.dimensions.update({name: size for name, size in h["dims"]})
+        for v in h["vars"]:
+            self.variables[v["name"]] = Variable(v["name"], v["dims"], v["shape"],
+                                                 (lambda v=v: _read_classic_var(path, h, v)), v["attrs"])
 
     def _open_hdf5(self, data: bytes):
         h = _Hdf5(data)
@@ -840,6 +841,264 @@ class Dataset:
             s = b"".join(r.tolist()).split(b"\0")[0].decode("utf-8")
             out.append(s.strip() if trim else s)
         return out
+
+
+# ---------------------------------------------------------------------------------------------
+# NetCDF-3 (classic, 64-bit offset, 64-bit data): big-endian header of dimension, attribute and
+# variable lists, then the fixed-size variables in order (4-byte aligned) and the records.
+
+_NC_TYPES = {1: ">i1", 2: "S1", 3: ">i2", 4: ">i4", 5: ">f4", 6: ">f8", 7: ">u1", 8: ">u2", 9: ">u4",
+             10: ">i8", 11: ">u8"}
+_NC_CODES = {"i1": 1, "S1": 2, "i2": 3, "i4": 4, "f4": 5, "f8": 6, "u1": 7, "u2": 8, "u4": 9, "i8": 10,
+             "u8": 11}
+_TAG_DIM, _TAG_VAR, _TAG_ATT = 10, 11, 12
+
+
+def _pad4(n: int) -> int:
+    return (n + 3) & ~3
+
+
+def _read_classic_header(path: str) -> dict:
+    with open(path, "rb") as fh:
+        head = fh.read(4)
+        if head[:3] != b"CDF" or head[3] not in (1, 2, 5):
+            raise FormatError(f"{path}: not a NetCDF-3 file")
+        version = head[3]
+        # The header ends before the first variable's data; read generously and grow on demand.
+        size = 1 << 16
+        while True:
+            fh.seek(0)
+            buf = fh.read(size)
+            try:
+                return _parse_classic_header(buf, version, path)
+            except IndexError:
+                if len(buf) < size:
+                    raise FormatError(f"{path}: truncated NetCDF-3 header")
+                size *= 4
+
+
+def _parse_classic_header(buf: bytes, version: int, path: str) -> dict:
+    pos = 4
+    big = version == 5
+
+    def take(n):
+        nonlocal pos
+        if pos + n > len(buf):
+            raise IndexError
+        b = buf[pos:pos + n]
+        pos += n
+        return b
+
+    def i32():
+        return struct.unpack(">i", take(4))[0]
+
+    def nonneg():
+        return struct.unpack(">q", take(8))[0] if big else struct.unpack(">i", take(4))[0]
+
+    def offset():
+        return struct.unpack(">q", take(8))[0] if version in (2, 5) else struct.unpack(">i", take(4))[0]
+
+    def name():
+        n = nonneg()
+        return take(_pad4(n))[:n].decode("utf-8")
+
+    def atts():
+        tag, n = i32(), nonneg()
+        if tag == 0 and n == 0:
+            return {}
+        if tag != _TAG_ATT:
+            raise FormatError(f"{path}: bad attribute list tag {tag}")
+        out = {}
+        for _ in range(n):
+            k = name()
+            t = i32()
+            cnt = nonneg()
+            dt = np.dtype(_NC_TYPES[t])
+            raw = take(_pad4(cnt * dt.itemsize))[:cnt * dt.itemsize]
+            if t == 2:
+                out[k] = raw.split(b"\0")[0].decode("utf-8", "replace")
+            else:
+                a = np.frombuffer(raw, dt).astype(dt.newbyteorder("="))
+                out[k] = a[0] if a.size == 1 else a
+        return out
+
+    numrecs = nonneg()
+    dims = []
+    tag, n = i32(), nonneg()
+    if not (tag == 0 and n == 0):
+        if tag != _TAG_DIM:
+            raise FormatError(f"{path}: bad dimension list tag {tag}")
+        for _ in range(n):
+            dims.append((name(), nonneg()))
+    gatts = atts()
+    tag, n = i32(), nonneg()
+    vars_ = []
+    if not (tag == 0 and n == 0):
+        if tag != _TAG_VAR:
+            raise FormatError(f"{path}: bad variable list tag {tag}")
+        for _ in range(n):
+            vname = name()
+            nd = nonneg()
+            ids = [nonneg() for _ in range(nd)]
+            vatts = atts()
+            t = i32()
+            vsize = nonneg()
+            begin = offset()
+            vdims = tuple(dims[i][0] for i in ids)
+            rec = bool(ids) and dims[ids[0]][1] == 0
+            shape = tuple((numrecs if (rec and j == 0) else dims[i][1]) for j, i in enumerate(ids))
+            vars_.append(dict(name=vname, dims=vdims, shape=shape, type=t, vsize=vsize, begin=begin,
+                              record=rec, attrs=vatts))
+    recsize = sum(v["vsize"] for v in vars_ if v["record"])
+    if sum(1 for v in vars_ if v["record"]) == 1:
+        v = next(v for v in vars_ if v["record"])
+        recsize = int(np.prod(v["shape"][1:], dtype=np.int64)) * np.dtype(_NC_TYPES[v["type"]]).itemsize
+    return dict(version=version, numrecs=numrecs, dims=dims, gatts=gatts, vars=vars_, recsize=recsize)
+
+
+def _read_classic_var(path: str, h: dict, v: dict) -> np.ndarray:
+    dt = np.dtype(_NC_TYPES[v["type"]])
+    with open(path, "rb") as fh:
+        if not v["record"]:
+            count = int(np.prod(v["shape"], dtype=np.int64))
+            fh.seek(v["begin"])
+            raw = fh.read(count * dt.itemsize)
+            if len(raw) != count * dt.itemsize:
+                raise FormatError(f"{path}: variable {v['name']} runs past the end of the file")
+            a = np.frombuffer(raw, dt).reshape(v["shape"])
+        else:
+            per = int(np.prod(v["shape"][1:], dtype=np.int64)) * dt.itemsize
+            parts = []
+            for r in range(h["numrecs"]):
+                fh.seek(v["begin"] + r * h["recsize"])
+                parts.append(fh.read(per))
+            a = np.frombuffer(b"".join(parts), dt).reshape(v["shape"])
+    if dt.kind in "iuf":
+        a = a.astype(dt.newbyteorder("="))
+    return a.copy()
+
+
+class Writer:
+    """A NetCDF-3 file written in one go: add dimensions and variables, then close().
+
+    Mirrors the parts of Netcdf_file / Netcdf_variable the driver uses (add_dimension,
+    add_variable, insert of the whole variable); variables are C-ordered numpy arrays whose shape
+    matches their dimensions.  The file is CDF-2 (64-bit offset) unless a fixed-size variable other
+    than the last passes 4 GiB, then CDF-5."""
+
+    def __init__(self, path: str, version: int | None = None):
+        if version not in (None, 2, 5):
+            raise ValueError("NetCDF-3 version must be 2 (64-bit offset) or 5 (64-bit data)")
+        self.path = path
+        self.version = version
+        self.dims: list = []
+        self.vars: list = []
+        self.gatts: dict = {}
+
+    def add_dimension(self, name: str, size: int):
+        if any(n == name for n, _ in self.dims):
+            if dict(self.dims)[name] != size:
+                raise ValueError(f"dimension {name!r} exists with another size")
+            return
+        if size <= 0:
+            raise ValueError(f"dimension {name!r}: size {size} (record dimensions are not written)")
+        self.dims.append((name, int(size)))
+
+    def add_variable(self, name: str, dims: tuple, data, attrs: dict | None = None):
+        d = dict(self.dims)
+        for n in dims:
+            if n not in d:
+                raise KeyError(f"variable {name!r}: no dimension {n!r}")
+        a = np.asarray(data)
+        if a.dtype.kind == "f" and a.dtype.itemsize not in (4, 8):
+            a = a.astype(np.float64)
+        if a.dtype == np.bool_:
+            a = a.astype(np.int8)
+        shape = tuple(d[n] for n in dims)
+        if a.size != int(np.prod(shape, dtype=np.int64)):
+            raise ValueError(f"variable {name!r}: {a.size} values for dimensions {dims} {shape}")
+        code = a.dtype.kind + str(a.dtype.itemsize) if a.dtype.kind != "S" else "S1"
+        if code not in _NC_CODES:
+            raise TypeError(f"variable {name!r}: type {a.dtype} has no NetCDF-3 counterpart")
+        self.vars.append((name, tuple(dims), np.ascontiguousarray(a.reshape(shape)), attrs or {}))
+
+    def close(self):
+        big = any(v[2].nbytes >= (1 << 32) - 4 for v in self.vars[:-1])
+        version = self.version or (5 if big else 2)
+        if version == 2 and big:
+            raise ValueError("a fixed-size variable other than the last passes CDF-2's 4 GiB")
+        dim_index = {n: i for i, (n, _) in enumerate(self.dims)}
+
+        def nonneg(x):
+            return struct.pack(">q", x) if version == 5 else struct.pack(">i", x)
+
+        def name(s):
+            b = s.encode("utf-8")
+            return nonneg(len(b)) + b + b"\0" * (_pad4(len(b)) - len(b))
+
+        def atts(d):
+            if not d:
+                return struct.pack(">i", 0) + nonneg(0)
+            out = struct.pack(">i", _TAG_ATT) + nonneg(len(d))
+            for k, val in d.items():
+                if isinstance(val, str):
+                    raw, t, n = val.encode("utf-8"), 2, len(val.encode("utf-8"))
+                else:
+                    a = np.atleast_1d(np.asarray(val))
+                    code = a.dtype.kind + str(a.dtype.itemsize)
+                    t = _NC_CODES[code]
+                    raw, n = a.astype(np.dtype(_NC_TYPES[t])).tobytes(), a.size
+                out += name(k) + struct.pack(">i", t) + nonneg(n) + raw + b"\0" * (_pad4(len(raw)) - len(raw))
+            return out
+
+        def header(begins):
+            h = b"CDF" + bytes([version]) + nonneg(0)
+            if self.dims:
+                h += struct.pack(">i", _TAG_DIM) + nonneg(len(self.dims))
+                for n, size in self.dims:
+                    h += name(n) + nonneg(size)
+            else:
+                h += struct.pack(">i", 0) + nonneg(0)
+            h += atts(self.gatts)
+            if self.vars:
+                h += struct.pack(">i", _TAG_VAR) + nonneg(len(self.vars))
+                for (vn, vd, a, va), b in zip(self.vars, begins):
+                    code = a.dtype.kind + str(a.dtype.itemsize) if a.dtype.kind != "S" else "S1"
+                    vsize = _pad4(a.nbytes)
+                    if version == 2 and vsize >= (1 << 32):
+                        vsize = (1 << 32) - 1          # the last variable of a CDF-2 file
+                    h += name(vn) + nonneg(len(vd)) + b"".join(nonneg(dim_index[x]) for x in vd)
+                    h += atts(va) + struct.pack(">i", _NC_CODES[code]) + nonneg(vsize) + struct.pack(">q", b)
+            else:
+                h += struct.pack(">i", 0) + nonneg(0)
+            return h
+
+        hlen = len(header([0] * len(self.vars)))
+        begins, off = [], hlen
+        for _, _, a, _ in self.vars:
+            begins.append(off)
+            off += _pad4(a.nbytes)
+        h = header(begins)
+        assert len(h) == hlen
+        with open(self.path, "wb") as fh:
+            fh.write(h)
+            for _, _, a, _ in self.vars:
+                if a.dtype.kind == "S":
+                    raw = a.astype("S1").tobytes()
+                else:
+                    raw = a.astype(a.dtype.newbyteorder(">")).tobytes()
+                fh.write(raw)
+                fh.write(b"\0" * (_pad4(len(raw)) - len(raw)))
+
+
+def write_schema(path: str, schema, version: int | None = None):
+    """A coefficients.Schema (dimensions and C-ordered variables) as a NetCDF-3 file."""
+    w = Writer(path, version)
+    for name, size in schema.dims.items():
+        w.add_dimension(name, size)
+    for name, (dims, a) in schema.vars.items():
+        w.add_variable(name, dims, a)
+    w.close()
 
 
 def to_schema(ds: Dataset):

@@ -143,3 +143,46 @@ def test_not_netcdf(tmp_path):
     p.write_bytes(b"this is not a netcdf file at all" * 10)
     with pytest.raises(netcdf.FormatError):
         netcdf.Dataset(str(p))
+
+
+@pytest.mark.parametrize("version", [2, 5])
+def test_writer_round_trip(tmp_path, version):
+    """Writer output read back by this module's NetCDF-3 reader; CDF-2 also by scipy."""
+    path = str(tmp_path / f"out{version}.nc")
+    w = netcdf.Writer(path, version=version)
+    w.add_dimension("x", 5)
+    w.add_dimension("y", 1)
+    w.add_dimension("lev", 4)
+    w.add_dimension("pair", 2)
+    up = np.random.default_rng(3).normal(size=(4, 1, 5))
+    w.add_variable("lw_flux_up", ("lev", "y", "x"), up)
+    w.add_variable("sw_flux_dn", ("lev", "y", "x"), up.astype(np.float32), {"units": "W m-2"})
+    w.add_variable("lw_band_lims_gpt", ("x", "pair"), np.arange(10, dtype=np.int32).reshape(5, 2))
+    w.add_variable("tsi", (), np.float64(1360.5))
+    w.close()
+    ds = netcdf.Dataset(path)
+    assert ds.dimensions == {"x": 5, "y": 1, "lev": 4, "pair": 2}
+    assert ds.variables["lw_flux_up"].dimensions == ("lev", "y", "x")
+    assert np.array_equal(ds.get_variable("lw_flux_up"), up)
+    assert ds.get_variable("sw_flux_dn").dtype == np.float32
+    assert np.array_equal(ds.get_variable("sw_flux_dn"), up.astype(np.float32))
+    assert ds.variables["sw_flux_dn"].attributes["units"] == "W m-2"
+    assert np.array_equal(ds.get_variable("lw_band_lims_gpt"), np.arange(10).reshape(5, 2))
+    assert float(ds.get_variable("tsi")) == 1360.5
+    if version == 2:
+        import scipy.io
+        f = scipy.io.netcdf_file(path, "r", mmap=False)
+        assert np.array_equal(f.variables["lw_flux_up"][:], up)
+        assert f.variables["lw_band_lims_gpt"].dimensions == ("x", "pair")
+        f.close()
+
+
+def test_writer_checks(tmp_path):
+    w = netcdf.Writer(str(tmp_path / "bad.nc"))
+    w.add_dimension("x", 3)
+    with pytest.raises(KeyError):
+        w.add_variable("a", ("lay",), np.zeros(3))
+    with pytest.raises(ValueError):
+        w.add_variable("a", ("x",), np.zeros(4))
+    with pytest.raises(ValueError):
+        w.add_dimension("x", 4)
