@@ -25,9 +25,11 @@
 // levels, instead of four layer arrays written and read back.
 //
 // Precision.  The adding denominators 1/(1 - R A) and the carried adding states (A, S, R_up, D,
-// the direct beam) are FP64 in every build.  The two-stream layer coefficients are FP64 in the
-// FP64 build; the FP32 build evaluates them in a rearranged FP32 form (sw_layer_fp32) and keeps
-// FP64 for near-conservative layers, with the FP64 k_min = 1e-12 and epsilon.  Inputs, outputs
+// the direct beam) are in the build's precision; the FP32 build forms the direct beam from the
+// optical depth summed in FP64.  The two-stream layer coefficients are FP64 in the FP64 build;
+// the FP32 build evaluates them in a rearranged FP32 form (sw_layer_fp32), with the FP64
+// k_min = 1e-12 and epsilon, and keeps FP64 for near-conservative layers when given ssa (whose
+// FP32 value cannot resolve 1 - ssa below 6e-8), not when given the co-albedo.  Inputs, outputs
 // and checkpoints keep the build's precision.
 #include <algorithm>
 #include <mutex>
@@ -52,9 +54,10 @@ constexpr int MAX_GROUPS = 256;
 template<typename Real> struct LwSeg;
 template<> struct LwSeg<float> { static constexpr int K1 = 6, K4 = 4; };
 template<> struct LwSeg<double> { static constexpr int K1 = 4, K4 = 4; };
-template<typename Real> struct SwSeg;
-template<> struct SwSeg<float> { static constexpr int K = 4; };
-template<> struct SwSeg<double> { static constexpr int K = 4; };
+// The SW kernel's wide variant (FP32, co-albedo input, no heating rates) fits 64 VGPRs with
+// K = 3, so two 1024-thread work-groups share a CU; the others keep K = 4 at up to 128 VGPRs.
+constexpr bool sw_wide(bool fp32, bool co, bool hr) { return fp32 && co && !hr; }
+constexpr int sw_seg_k(bool fp32, bool co, bool hr) { return sw_wide(fp32, co, hr) ? 3 : 4; }
 
 // G-point groups: consecutive g-points of one band, at most SB_GPTS each.
 struct Groups
@@ -395,10 +398,11 @@ __device__ __forceinline__ void sw_two_stream(double tau, double w0, double g, d
     sw_two_stream_t<double>(tau, w0, g, mu0, Rdif, Tdif, Rdir, Tdir, Tnoscat);
 }
 
-// One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance.
-// ab = 1 - R - T (diffuse absorptance) and omTn = 1 - Tn are only read by the heating-rate
-// variant (dead code otherwise).
-struct LayerSW { double R, T, SU, SD, Tn, ab, omTn; };
+// One layer of the top-down sweep: diffuse R/T, the direct-beam sources and transmittance, in
+// the precision St of the adding states.  ab = 1 - R - T (diffuse absorptance) and omTn = 1 - Tn
+// are only read by the heating-rate variant (dead code otherwise).
+template<typename St>
+struct LayerSW { St R, T, SU, SD, Tn, ab, omTn; };
 
 // The layer properties in FP32, rearranged so that FP32 keeps its accuracy:
 //   k^2 = 2 (1 - w0) (gamma1 + gamma2)          (gamma1 - gamma2 = 2 (1 - w0) exactly),
@@ -415,10 +419,11 @@ struct LayerSW { double R, T, SU, SD, Tn, ab, omTn; };
 // Rdir + Tdir, keeps its series too (see the clamp).  The transmittance e^{-tau/mu0} of this layer enters G
 // and Tdir only; the direct beam itself is carried by the caller (Beam<float>).  Returns false
 // where FP64 is kept: near conservative scattering (1 - w0 < 1e-6).
+template<bool ISLAND, typename St>
 __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float one_m_w0, float g, float mu0, float inv_mu0,
-                                              double dir, LayerSW& L)
+                                              St dir, LayerSW<St>& L)
 {
-    if (one_m_w0 < 1e-6f) return false;
+    if (ISLAND && one_m_w0 < 1e-6f) return false;
     const float gamma1 = (8.f - w0 * (5.f + 3.f * g)) * .25f;
     const float gamma2 = 3.f * (w0 * (1.f - g)) * .25f;
     const float k = __builtin_amdgcn_sqrtf(fmaxf(2.f * one_m_w0 * (gamma1 + gamma2), 1e-12f));
@@ -462,32 +467,48 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float one_m_w
     // 1 - R - T = RT [k (2 - c2) + gamma1 c2 - gamma2 c2 - 2 k e1] = RT [k a^2 + (gamma1 - gamma2) c2] with
     // gamma1 - gamma2 = 2 (1 - w0): a sum of non-negative terms, free of the cancellation of 1 - R - T.
     const float ab = RT * (k * a * a + 2.f * one_m_w0 * c2);
-    L = {double(RT * gamma2 * c2), double(RT * 2.f * k * e1), double(rd) * dir, double(td) * dir, double(Tn),
-         double(ab), double(b)};
+    L = {St(RT * gamma2 * c2), St(RT * 2.f * k * e1), St(rd) * dir, St(td) * dir, St(Tn), St(ab), St(b)};
     return true;
 }
 
 // FP32 builds take the FP32 form where it is accurate and the FP64 island elsewhere; FP64
 // builds evaluate the reference expressions in FP64.  With CO the second input is the
-// co-albedo 1 - w0 (exact in the caller's precision, where w0 near 1 is not: FP32 resolves
-// 1 - w0 only to 6e-8).
-template<typename Real, bool CO = false>
-__device__ __forceinline__ LayerSW sw_layer(Real tau, Real w_in, Real g, Real mu0, Real inv_mu0, double dir)
+// co-albedo 1 - w0, exact in the caller's precision, where w0 near 1 is not (FP32 resolves
+// 1 - w0 only to 6e-8).  Given it exactly, the FP32 form keeps its accuracy down to conservative
+// scattering: against the two-stream in 80-bit arithmetic over tau in [1e-10, 3e3],
+// 1 - w0 in [0, 1e-5], g in [0, 0.95], mu0 in [0.02, 1], its largest errors in R, T, Rdir and
+// Tdir are 2-4e-6 for 1 - w0 below 1e-6 as above it, so the CO form has no FP64 island.
+template<typename Real, bool CO, typename St>
+__device__ __forceinline__ LayerSW<St> sw_layer(Real tau, Real w_in, Real g, Real mu0, Real inv_mu0, St dir)
 {
-    LayerSW L;
+    LayerSW<St> L;
     const Real w0 = CO ? Real(1) - w_in : w_in;
     if constexpr (std::is_same<Real, float>::value)
-        if (sw_layer_fp32(tau, w0, CO ? w_in : 1.f - w_in, g, mu0, inv_mu0, dir, L)) return L;
+    {
+        if constexpr (CO)
+        {
+            sw_layer_fp32<false>(tau, w0, w_in, g, mu0, inv_mu0, dir, L);
+            return L;
+        }
+        else if (sw_layer_fp32<true>(tau, w0, 1.f - w_in, g, mu0, inv_mu0, dir, L))
+            return L;
+    }
     const double w0d = CO ? 1. - double(w_in) : double(w_in);
     double Rdif, Tdif, Rdir, Tdir, Tn;
     sw_two_stream(double(tau), w0d, double(g), double(mu0), Rdif, Tdif, Rdir, Tdir, Tn);
-    return {Rdif, Tdif, Rdir * dir, Tdir * dir, Tn, 1. - Rdif - Tdif, -expm1(-double(tau) / double(mu0))};
+    return {St(Rdif), St(Tdif), St(Rdir * double(dir)), St(Tdir * double(dir)), St(Tn), St(1. - Rdif - Tdif),
+            St(-expm1(-double(tau) / double(mu0)))};
 }
 
+// Reciprocal of an adding denominator 1 - R A in (0, 1].
+__device__ __forceinline__ double rcp_den(double x) { return rcp_f64(x); }
+__device__ __forceinline__ float rcp_den(float x) { return rcp_f32(x); }
+
 // Column above the next level from the column above this one and the layer between.
-__device__ __forceinline__ void above_update(const LayerSW& L, double& R_up, double& D)
+template<typename St>
+__device__ __forceinline__ void above_update(const LayerSW<St>& L, St& R_up, St& D)
 {
-    const double den = rcp_f64(1. - L.R * R_up);
+    const St den = rcp_den(St(1) - L.R * R_up);
     D = L.SD + L.T * (D + R_up * L.SU) * den;
     R_up = L.R + L.T * L.T * R_up * den;
 }
@@ -497,35 +518,39 @@ __device__ __forceinline__ void above_update(const LayerSW& L, double& R_up, dou
 // dir0 e^{-sum tau/mu0} from the optical depth summed in FP64, so the rounding of the hardware
 // exponential stays that of one evaluation instead of compounding down the column.  The
 // checkpoint value is the beam (FP64 builds) or the summed optical depth (FP32 builds).
-template<typename Real> struct Beam;
-template<> struct Beam<double>
+template<typename Real, typename St> struct Beam;
+template<> struct Beam<double, double>
 {
     double dir;
     __device__ void init(double dir0) { dir = dir0; }
-    __device__ void step(const LayerSW& L, double, double) { dir *= L.Tn; }
+    __device__ void step(const LayerSW<double>& L, double, double) { dir *= L.Tn; }
     __device__ double saved() const { return dir; }
     __device__ void restore(double v) { dir = v; }
 };
-template<> struct Beam<float>
+template<typename St> struct Beam<float, St>
 {
-    double dir0, ycum, dir;
-    __device__ void init(double d0) { dir0 = d0; ycum = 0.; dir = d0; }
-    __device__ void step(const LayerSW&, float tau, float inv_mu0)
+    double ycum;
+    St dir0, dir;
+    __device__ void init(St d0) { dir0 = d0; ycum = 0.; dir = d0; }
+    __device__ void step(const LayerSW<St>&, float tau, float inv_mu0)
     {
         ycum += double(tau * inv_mu0);
-        dir = dir0 * double(exp_neg_f32(float(ycum)));
+        dir = dir0 * St(exp_neg_f32(float(ycum)));
     }
     __device__ float saved() const { return float(ycum); }
     __device__ void restore(float v)
     {
         ycum = double(v);
-        dir = dir0 * double(exp_neg_f32(v));
+        dir = dir0 * St(exp_neg_f32(v));
     }
 };
 
-// CO: the ssa argument holds the co-albedo 1 - ssa (see sw_layer).
+// CO: the ssa argument holds the co-albedo 1 - ssa (see sw_layer).  The adding states and
+// denominators are carried in the build's precision (with FP32 states the FP32 build stays within
+// 2e-3 W m-2 of the FP64 build, near-conservative cloud layers included).
 template<typename Real, bool HR = false, bool CO = false>
 __global__ __launch_bounds__(SB_COLS * SB_GPTS)
+__attribute__((amdgpu_waves_per_eu(sw_wide(sizeof(Real) == 4, CO, HR) ? 8 : 4)))
 void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                       const Real* __restrict__ tau, const Real* __restrict__ ssa,
                       const Real* __restrict__ gasy, const Real* __restrict__ mu0,
@@ -536,7 +561,8 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                       Real* __restrict__ grp_dn, Real* __restrict__ grp_dir, int64_t alb_nbnd,
                       double* __restrict__ absorbed)
 {
-    constexpr int K = SwSeg<Real>::K;
+    using St = Real;
+    constexpr int K = sw_seg_k(sizeof(Real) == 4, CO, HR);
     constexpr int NC = 3;
     // With HR a fourth component per slot: the absorbed flux of the layer below the slot's level,
     // (dn_dif(top) + up(bottom)) (1 - R - T) + dir(top) (1 - Tn) - SU - SD for this g-point, from the
@@ -574,13 +600,13 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
 
     const Real m0r = active ? mu0[icol] : Real(1);
     const Real inv_m0r = Real(1) / m0r;
-    const double m0 = double(m0r);
-    double R_up = 0., D = 0.;
-    Beam<Real> beam;
-    beam.init(active ? double(inc_dir[c2]) * m0 : 0.);
+    const St m0 = St(m0r);
+    St R_up = St(0), D = St(0);
+    Beam<Real, St> beam;
+    beam.init(active ? St(inc_dir[c2]) * m0 : St(0));
     if (active)
     {
-        D = inc_dif ? double(inc_dif[c2]) : 0.;
+        D = inc_dif ? St(inc_dif[c2]) : St(0);
         // Sweep down: the column above every level; checkpoints at the top of each segment.  The
         // inputs of a segment are all requested before its dependent chain starts.
         for (int64_t s = 0; s < nseg; ++s)
@@ -600,7 +626,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
             for (int kk = 0; kk < K; ++kk)
                 if (kk < nk)
                 {
-                    const LayerSW L = sw_layer<Real, CO>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, beam.dir);
+                    const LayerSW<St> L = sw_layer<Real, CO>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, beam.dir);
                     above_update(L, R_up, D);
                     beam.step(L, tv[kk], inv_m0r);
                 }
@@ -608,15 +634,15 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
     }
 
     // Upward flux at the bottom of the layer being closed (heating-rate variant).
-    double up_below = 0.;
+    St up_below = St(0);
     // Surface: albedos per (column, g-point), or per (band, column) read directly.
     const int64_t ia = alb_nbnd > 0 ? grp.band[gi] + alb_nbnd * icol : c2;
-    double A = active ? double(alb_dif[ia]) : 0.;
-    double S = beam.dir * (active ? double(alb_dir[ia]) : 0.);
+    St A = active ? St(alb_dif[ia]) : St(0);
+    St S = beam.dir * (active ? St(alb_dir[ia]) : St(0));
     {
-        const double dir = beam.dir;
-        const double dn = (D + R_up * S) * rcp_f64(1. - R_up * A);
-        const double up = A * dn + S;
+        const St dir = beam.dir;
+        const St dn = (D + R_up * S) * rcp_den(St(1) - R_up * A);
+        const St up = A * dn + S;
         if (active && up_g)
         {
             const uint32_t o = ucol + uncol * lev_of(nlay);
@@ -640,8 +666,8 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         {
             Real sR[K], sT[K], sSU[K], sSD[K], sRup[K], sD[K], sDir[K];
             Real sAb[HR ? K : 1], sOmTn[HR ? K : 1];
-            double r_up = double(ck(s, 0)), d = double(ck(s, 1));
-            Beam<Real> bm = beam;
+            St r_up = St(ck(s, 0)), d = St(ck(s, 1));
+            Beam<Real, St> bm = beam;
             bm.restore(ck(s, 2));
             Real tv[K], wv[K], gv[K];
 #pragma unroll
@@ -656,7 +682,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                 if (kk < nk)
                 {
                     sRup[kk] = Real(r_up); sD[kk] = Real(d); sDir[kk] = Real(bm.dir);
-                    const LayerSW L = sw_layer<Real, CO>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, bm.dir);
+                    const LayerSW<St> L = sw_layer<Real, CO>(tv[kk], wv[kk], gv[kk], m0r, inv_m0r, bm.dir);
                     sR[kk] = Real(L.R); sT[kk] = Real(L.T); sSU[kk] = Real(L.SU); sSD[kk] = Real(L.SD);
                     if constexpr (HR) { sAb[kk] = Real(L.ab); sOmTn[kk] = Real(L.omTn); }
                     above_update(L, r_up, d);
@@ -666,14 +692,14 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
             for (int kk = K - 1; kk >= 0; --kk)
                 if (kk < nk)
                 {
-                    const double R = sR[kk], T = sT[kk];
-                    const double den = rcp_f64(1. - R * A);
-                    S = double(sSU[kk]) + T * (S + A * double(sSD[kk])) * den;
+                    const St R = sR[kk], T = sT[kk];
+                    const St den = rcp_den(St(1) - R * A);
+                    S = St(sSU[kk]) + T * (S + A * St(sSD[kk])) * den;
                     A = R + T * T * A * den;
-                    const double ru = sRup[kk];
-                    const double dn = (double(sD[kk]) + ru * S) * rcp_f64(1. - ru * A);
-                    const double up = A * dn + S;
-                    const double dd = sDir[kk];
+                    const St ru = sRup[kk];
+                    const St dn = (St(sD[kk]) + ru * S) * rcp_den(St(1) - ru * A);
+                    const St up = A * dn + S;
+                    const St dd = sDir[kk];
                     if (up_g)
                     {
                         const uint32_t o = ucol + uncol * lev_of(j0 + kk);
@@ -687,8 +713,8 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                     }
                     if constexpr (HR)
                     {
-                        const double ab = (dn + up_below) * double(sAb[kk]) + dd * double(sOmTn[kk]) -
-                                          double(sSU[kk]) - double(sSD[kk]);
+                        const St ab = (dn + up_below) * St(sAb[kk]) + dd * St(sOmTn[kk]) - St(sSU[kk]) -
+                                      St(sSD[kk]);
                         red[((kk * NCH + NC) * SB_GPTS + w) * SB_COLS + lane] = Real(ab);
                         up_below = up;
                     }
@@ -918,8 +944,8 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
         return arg_error("sw_solver: heating rates need plev and summed (not g-point) flux outputs");
     if (by_band && (!band_lims || nbnd < 1)) return arg_error("sw_solver: band surface inputs need band limits");
     // 32-bit lane offsets within one g-point's slab of levels or of checkpoints.
-    if (ncol * std::max<int64_t>(nlay + 1, 3 * ((nlay + SwSeg<Real>::K - 1) / SwSeg<Real>::K)) *
-            int64_t(sizeof(Real)) >= (int64_t(1) << 32))
+    const int K = sw_seg_k(sizeof(Real) == 4, coalbedo, heating_rate != nullptr);
+    if (ncol * std::max<int64_t>(nlay + 1, 3 * ((nlay + K - 1) / K)) * int64_t(sizeof(Real)) >= (int64_t(1) << 32))
         return arg_error("sw_solver: ncol * (nlay + 1) values of one g-point must stay below 4 GiB");
     if (!tau || !ssa || !g || !mu0 || !alb_dir || !alb_dif || !inc_dir) return arg_error("sw_solver: NULL input");
     const bool spectral = flux_up || flux_dn || flux_dir;
@@ -933,7 +959,6 @@ int sw_solver(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, const Real
     hipStream_t s = as_stream(stream);
     Groups grp;
     RRTMGP_TRY(make_groups(ngpt, band_lims ? nbnd : 1, band_lims, grp));
-    constexpr int K = SwSeg<Real>::K;
     const int64_t ncv = ncol * (nlay + 1), nseg = (nlay + K - 1) / K;
     const bool direct = want_bnd && grp.n == int(nbnd);
     const dim3 grid(grid1(ncol, SB_COLS), unsigned(grp.n));
