@@ -114,6 +114,34 @@ def test_gas_optics_fp64(G, gkd, okd_lw, okd_sw, atm_c0, O, lw):
         assert rel_err(got[n], ref[n], floor=1e-300) < FP64_RTOL, n
 
 
+@pytest.mark.parametrize("lw", [True, False])
+@pytest.mark.parametrize("precision", [8, 4])
+def test_gas_optics_heterogeneous_tiles(G, gkd, okd_lw, okd_sw, syn, O, lw, precision):
+    """A work-group of the fused kernels takes 64 columns of one layer.  In tile 0 the columns
+    alternate between temperatures 60 K apart and layer pressures a factor 2.9 apart, so the
+    columns of a work-group read far-apart k-table nodes and flavors' eta ranges; tile 1 is a
+    plain tile.  Against the oracle (FP64: 1e-12; FP32: 2e-4 of each field, floor 1e-6 of its
+    maximum, outside the eta = 1 jump)."""
+    okd = okd_lw if lw else okd_sw
+    atm = syn.make_atmosphere(syn.SEED_ATMOS[1], 128, 60)
+    tmin, tmax = float(np.min(okd.src.temp_ref)), float(np.max(okd.src.temp_ref))
+    pmin, pmax = float(np.min(okd.src.press_ref)), float(np.max(okd.src.press_ref))
+    sign = np.where(np.arange(64) % 2 == 0, 1.0, -1.0)[:, None]
+    atm.t_lay[:64] = np.clip(atm.t_lay[:64] + 30.0 * sign, tmin + 1, tmax - 1)
+    atm.t_lev[:64] = np.clip(atm.t_lev[:64] + 30.0 * sign, tmin + 1, tmax - 1)
+    f = np.where(sign > 0, 1.7, 1.0 / 1.7)
+    atm.p_lay[:64] = np.clip(atm.p_lay[:64] * f, 1.01 * pmin, 0.99 * pmax)     # p_lev (col_dry) kept
+    ref, got = _gas_optics_both(G, gkd, okd, atm, lw, precision, O)
+    if precision == 8:
+        for n in ref:
+            assert rel_err(got[n], ref[n], floor=1e-300) < FP64_RTOL, n
+    else:
+        keep = ~_eta_edge_mask(O, okd, atm, syn.NETA)
+        for n in ref:
+            k3 = keep if ref[n].ndim == 3 else np.ones(ref[n].shape, dtype=bool)
+            assert rel_err(got[n][k3], ref[n][k3], floor=1e-6 * np.abs(ref[n]).max()) < 2e-4, n
+
+
 def _eta_edge_mask(O, okd, atm, neta):
     """(ncol, nlay, ngpt) mask of g-points whose FP64 eta lies within FP32 rounding below 1.
 
