@@ -402,20 +402,97 @@ struct PairState
     Real aux[5];         // LW: B(T_lay), B(T_lev above), B(T_lev below), B(T_sfc), B(T_sfc + 1); SW: col_ray
 };
 
-// Phase 1 for one pair (lane = column).
-template<typename Real, bool LW>
-__device__ __forceinline__ void pair_state(const Kd<Real>& k, const GasRef<Real>* sg, const GoBand& bd, int ibnd,
-                                           int64_t icol, int64_t ilay, int64_t ncol, const Real* play,
-                                           const Real* tlay, const Real* tlev, const Real* tsfc,
-                                           const Real* col_dry_a, bool at_sfc, PairState<Real>& st)
+// The per-(column, layer) inputs of phase 1, read from memory ...
+template<typename Real>
+struct PairInGlobal
 {
-    const int64_t idx = icol + ncol * ilay;
-    const Real pv = play[idx], tv = tlay[idx], cd = col_dry_a[idx];
+    const GasRef<Real>* sg;
+    int64_t icol, ilay, ncol;
+    const Real *play, *tlay, *tlev, *tsfc, *col_dry;
+    __device__ Real p() const { return play[icol + ncol * ilay]; }
+    __device__ Real t() const { return tlay[icol + ncol * ilay]; }
+    __device__ Real cd() const { return col_dry[icol + ncol * ilay]; }
+    __device__ Real col(int gas, Real cdv) const { return col_lds(sg, gas, icol, ilay, cdv); }
+    __device__ Real t_hi() const { return tlev[icol + ncol * (ilay + 1)]; }
+    __device__ Real t_lo() const { return tlev[icol + ncol * ilay]; }
+    __device__ Real t_sfc() const { return tsfc[icol]; }
+};
+
+// ... or from the work-group's LDS rows, filled by all its threads at once (one memory round trip
+// instead of the dependent loads of one wavefront): row 0 p, 1 T, 2 col_dry, 3 .. 3 + ngas - 1
+// the vmr of gases 1 .. ngas, then (longwave) T of the level below and above and T_sfc.
+constexpr int GO_IN_FIXED = 3;
+
+template<typename Real>
+struct PairInLds
+{
+    const Real* rows;
+    int lane, ngas;
+    __device__ Real v(int r) const { return rows[r * GO_COLS + lane]; }
+    __device__ Real p() const { return v(0); }
+    __device__ Real t() const { return v(1); }
+    __device__ Real cd() const { return v(2); }
+    __device__ Real col(int gas, Real cdv) const { return gas == 0 ? cdv : v(GO_IN_FIXED + gas - 1) * cdv; }
+    __device__ Real t_lo() const { return v(GO_IN_FIXED + ngas); }
+    __device__ Real t_hi() const { return v(GO_IN_FIXED + ngas + 1); }
+    __device__ Real t_sfc() const { return v(GO_IN_FIXED + ngas + 2); }
+};
+
+template<bool LW>
+__host__ __device__ constexpr int go_in_rows(int ngas) { return GO_IN_FIXED + ngas + (LW ? 3 : 0); }
+
+// Fills the LDS input rows of the work-group's 64 columns of layer ilay (all threads; the
+// caller synchronises before reading them).
+template<typename Real, bool LW>
+__device__ __forceinline__ void stage_pair_inputs(const GasRef<Real>* sg, int ngas, int64_t col0, int64_t ilay,
+                                                  int64_t ncol, int64_t nlay, const Real* play, const Real* tlay,
+                                                  const Real* tlev, const Real* tsfc, const Real* col_dry,
+                                                  Real* rows)
+{
+    // All loads first (row r is uniform over a wavefront), then the LDS stores.
+    constexpr int NIT = 2 * GO_GPTS * GO_ROW / GO_THREADS;
+    const int n = go_in_rows<LW>(ngas) * GO_COLS;
+    Real v[NIT];
+#pragma unroll
+    for (int i = 0; i < NIT; ++i)
+    {
+        const int e = int(threadIdx.x) + i * GO_THREADS;
+        v[i] = Real(0);
+        if (e >= n) continue;
+        const int r = e / GO_COLS, c = e % GO_COLS;
+        const int64_t icol = min(col0 + c, ncol - 1);           // padding lanes repeat the last column
+        const int64_t idx = icol + ncol * ilay;
+        if (r == 0) v[i] = play[idx];
+        else if (r == 1) v[i] = tlay[idx];
+        else if (r == 2) v[i] = col_dry[idx];
+        else if (r < GO_IN_FIXED + ngas)
+        {
+            const GasRef<Real>& g = sg[r - GO_IN_FIXED];
+            v[i] = g.p[icol * g.scol + ilay * g.slay];
+        }
+        else if (r == GO_IN_FIXED + ngas) v[i] = tlev[idx];
+        else if (r == GO_IN_FIXED + ngas + 1) v[i] = tlev[icol + ncol * (ilay + 1)];
+        else v[i] = tsfc[icol];
+    }
+#pragma unroll
+    for (int i = 0; i < NIT; ++i)
+    {
+        const int e = int(threadIdx.x) + i * GO_THREADS;
+        if (e < n) rows[e] = v[i];
+    }
+}
+
+// Phase 1 for one pair (lane = column).
+template<typename Real, bool LW, typename In>
+__device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& bd, int ibnd, const In& in,
+                                              bool at_sfc, PairState<Real>& st)
+{
+    const Real pv = in.p(), tv = in.t(), cd = in.cd();
     const CL<Real> c = interp_cl(k, pv, tv);
     const int s = c.itropo;
     const int gas1 = bd.gas[s][0], gas2 = bd.gas[s][1];
     FL<Real> f;
-    interp_flav(k, c, gas1, gas2, col_lds(sg, gas1, icol, ilay, cd), col_lds(sg, gas2, icol, ilay, cd), f);
+    interp_flav(k, c, gas1, gas2, in.col(gas1, cd), in.col(gas2, cd), f);
     const int gp = bd.gp, np1 = k.npres + 1, nte = k.ntemp * k.neta;
     const int jp = c.jpress + c.itropo - 1;
 #pragma unroll
@@ -439,7 +516,7 @@ __device__ __forceinline__ void pair_state(const Kd<Real>& k, const GasRef<Real>
         {
             const GoMinor md = bd.minor[s][q];
             sc = minor_scaling_of(
-                    [&](int gas) { return col_lds(sg, gas, icol, ilay, cd); },
+                    [&](int gas) { return in.col(gas, cd); },
                     md.gas, md.dens != 0, md.scal_gas, md.compl_ != 0, k.idx_h2o, pv, tv, cd);
         }
         st.scaling[q] = sc;
@@ -449,16 +526,27 @@ __device__ __forceinline__ void pair_state(const Kd<Real>& k, const GasRef<Real>
     {
         const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
         st.aux[0] = interpolate1D(tv, k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-        st.aux[1] = interpolate1D(tlev[icol + ncol * (ilay + 1)], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-        st.aux[2] = interpolate1D(tlev[idx], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        st.aux[1] = interpolate1D(in.t_hi(), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+        st.aux[2] = interpolate1D(in.t_lo(), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
         if (at_sfc)
         {
-            st.aux[3] = interpolate1D(tsfc[icol], k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
-            st.aux[4] = interpolate1D(tsfc[icol] + Real(1), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+            const Real ts = in.t_sfc();
+            st.aux[3] = interpolate1D(ts, k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+            st.aux[4] = interpolate1D(ts + Real(1), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
         }
     }
     else
-        st.aux[0] = col_lds(sg, k.idx_h2o, icol, ilay, cd) + cd;
+        st.aux[0] = in.col(k.idx_h2o, cd) + cd;
+}
+
+template<typename Real, bool LW>
+__device__ __forceinline__ void pair_state(const Kd<Real>& k, const GasRef<Real>* sg, const GoBand& bd, int ibnd,
+                                           int64_t icol, int64_t ilay, int64_t ncol, const Real* play,
+                                           const Real* tlay, const Real* tlev, const Real* tsfc,
+                                           const Real* col_dry_a, bool at_sfc, PairState<Real>& st)
+{
+    const PairInGlobal<Real> in{sg, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a};
+    pair_state_in<Real, LW>(k, bd, ibnd, in, at_sfc, st);
 }
 
 // Entries i..i+3 of a band-packed table (quads are 4-element aligned).  The table base is
@@ -592,8 +680,10 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
 {
     static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
-    __shared__ Real s_tau[GO_GPTS][GO_ROW];
-    __shared__ Real s_pf[GO_GPTS][GO_ROW];
+    // Phase 1's input rows, then (after phase 1) phase 2's results per g-point row.
+    __shared__ Real s_buf[2 * GO_GPTS * GO_ROW];
+    Real (*const s_tau)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf);
+    Real (*const s_pf)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf + GO_GPTS * GO_ROW);
     __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
     const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
     if (u >= grid.nunits) return;                       // uniform over the work-group
@@ -603,17 +693,26 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int64_t ilay = un.ilay, ncl = ncol * nlay;
     const bool at_sfc = (ilay == sfc_lay - 1);
+    const bool staged_in = go_in_rows<true>(k.ngas) * GO_COLS <= 2 * GO_GPTS * GO_ROW;
 
     if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
     __syncthreads();
+    if (staged_in)
+    {
+        stage_pair_inputs<Real, true>(sg, k.ngas, un.col0, ilay, ncol, nlay, play, tlay, tlev, tsfc, col_dry_a,
+                                      s_buf);
+        __syncthreads();
+    }
     if (w == 0)
     {
         const int64_t icol = un.col0 + lane;
-        if (icol < ncol)
+        if (icol >= ncol)
+            sst[lane].flags = 0;
+        else if (staged_in)
+            pair_state_in<Real, true>(k, bd, un.ibnd, PairInLds<Real>{s_buf, lane, k.ngas}, at_sfc, sst[lane]);
+        else
             pair_state<Real, true>(k, sg, bd, un.ibnd, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a,
                                    at_sfc, sst[lane]);
-        else
-            sst[lane].flags = 0;
     }
     __syncthreads();
 
@@ -701,8 +800,10 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
 {
     static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
-    __shared__ Real s_abs[GO_GPTS][GO_ROW];
-    __shared__ Real s_ray[GO_GPTS][GO_ROW];
+    // Phase 1's input rows, then (after phase 1) phase 2's results per g-point row.
+    __shared__ Real s_buf[2 * GO_GPTS * GO_ROW];
+    Real (*const s_abs)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf);
+    Real (*const s_ray)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf + GO_GPTS * GO_ROW);
     __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
     const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
     if (u >= grid.nunits) return;
@@ -711,17 +812,26 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     const GoBand& bd = k.bands[un.ibnd];
     const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int64_t ilay = un.ilay, ncl = ncol * nlay;
+    const bool staged_in = go_in_rows<false>(k.ngas) * GO_COLS <= 2 * GO_GPTS * GO_ROW;
 
     if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
     __syncthreads();
+    if (staged_in)
+    {
+        stage_pair_inputs<Real, false>(sg, k.ngas, un.col0, ilay, ncol, nlay, play, tlay, nullptr, nullptr,
+                                       col_dry_a, s_buf);
+        __syncthreads();
+    }
     if (w == 0)
     {
         const int64_t icol = un.col0 + lane;
-        if (icol < ncol)
+        if (icol >= ncol)
+            sst[lane].flags = 0;
+        else if (staged_in)
+            pair_state_in<Real, false>(k, bd, un.ibnd, PairInLds<Real>{s_buf, lane, k.ngas}, false, sst[lane]);
+        else
             pair_state<Real, false>(k, sg, bd, un.ibnd, icol, ilay, ncol, play, tlay, nullptr, nullptr,
                                     col_dry_a, false, sst[lane]);
-        else
-            sst[lane].flags = 0;
     }
     __syncthreads();
 
