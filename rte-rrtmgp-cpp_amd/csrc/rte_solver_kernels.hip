@@ -413,7 +413,7 @@ struct LayerSW { St R, T, SU, SD, Tn, ab, omTn; };
 //   Rdir = w0 RT [(alpha2 + k gamma3)(2a - a^2) + 2 (k gamma3 - alpha2 k mu0) e^{-k tau} G] / (1 + k mu0)
 //   Tdir = -w0 RT [(alpha1 - k gamma4)(2a - a^2) e^{-tau/mu0} - 2k (gamma4 + mu0 alpha1) G] / (1 + k mu0)
 // where every O(1) term that cancels in the reference form has been cancelled analytically.  G is
-// e^{-tau/mu0} (tau/mu0) (e^x - 1)/x with x = tau (1 - k mu0) / mu0, from a series when |x| <= 1/2
+// e^{-tau/mu0} (tau/mu0) (e^x - 1)/x with x = tau (1 - k mu0) / mu0, from a series when |x| <= 1/4
 // and as the difference quotient otherwise.  a keeps its series for small k tau: where k is small
 // RT ~ 1/(2k) amplifies the absolute error of 1 - e^{-k tau}.  1 - e^{-tau/mu0}, the bound of
 // Rdir + Tdir, keeps its series too (see the clamp).  The transmittance e^{-tau/mu0} of this layer enters G
@@ -440,15 +440,15 @@ __device__ __forceinline__ bool sw_layer_fp32(float tau, float w0, float one_m_w
     const float Tn = exp_neg_f32(y);
     const float ek = 1.f - k_mu;
     const float x = y * ek;
-    float phi = 1.f / 40320.f;
-    phi = 1.f / 5040.f + x * phi;
-    phi = 1.f / 720.f + x * phi;
+    // (e^x - 1) / x below |x| = 1/4 (truncation x^6/7! < 5e-8); above, the difference quotient
+    // loses at most 2^-23 / |x| < 5e-7 relative.
+    float phi = 1.f / 720.f;
     phi = 1.f / 120.f + x * phi;
     phi = 1.f / 24.f + x * phi;
     phi = 1.f / 6.f + x * phi;
     phi = .5f + x * phi;
     phi = 1.f + x * phi;
-    const float G = (fabsf(x) <= .5f) ? Tn * y * phi : (e1 - Tn) * rcp_f32(ek);
+    const float G = (fabsf(x) <= .25f) ? Tn * y * phi : (e1 - Tn) * rcp_f32(ek);
     // One reciprocal for RT = 1 / (k (2 - c2) + gamma1 c2) and 1 / (1 + k mu0).
     const float opk = 1.f + k_mu;
     const float q = rcp_f32((k * (2.f - c2) + gamma1 * c2) * opk);
@@ -515,15 +515,17 @@ __device__ __forceinline__ void above_update(const LayerSW<St>& L, St& R_up, St&
 
 // The direct beam below each layer of the top-down sweeps.  FP64 builds multiply the layer
 // transmittances as the reference does (F90 sw_solver_2stream).  FP32 builds evaluate
-// dir0 e^{-sum tau/mu0} from the optical depth summed in FP64, so the rounding of the hardware
-// exponential stays that of one evaluation instead of compounding down the column.  The
-// checkpoint value is the beam (FP64 builds) or the summed optical depth (FP32 builds).
+// dir0 e^{-sum tau/mu0} from the optical depth summed in FP64 at every checkpoint segment start
+// and multiply the layer transmittances within a segment, so the rounding of the hardware
+// exponential does not compound down the column.  The checkpoint value is the beam (FP64
+// builds) or the summed optical depth (FP32 builds).
 template<typename Real, typename St> struct Beam;
 template<> struct Beam<double, double>
 {
     double dir;
     __device__ void init(double dir0) { dir = dir0; }
     __device__ void step(const LayerSW<double>& L, double, double) { dir *= L.Tn; }
+    __device__ void refresh() {}
     __device__ double saved() const { return dir; }
     __device__ void restore(double v) { dir = v; }
 };
@@ -532,11 +534,15 @@ template<typename St> struct Beam<float, St>
     double ycum;
     St dir0, dir;
     __device__ void init(St d0) { dir0 = d0; ycum = 0.; dir = d0; }
-    __device__ void step(const LayerSW<St>&, float tau, float inv_mu0)
+    // Within a checkpoint segment the beam is carried by the layers' own transmittances (at most
+    // K - 1 roundings); at each segment start it is re-formed from the summed optical depth, the
+    // value the replay of that segment restores.
+    __device__ void step(const LayerSW<St>& L, float tau, float inv_mu0)
     {
         ycum += double(tau * inv_mu0);
-        dir = dir0 * St(exp_neg_f32(float(ycum)));
+        dir *= L.Tn;
     }
+    __device__ void refresh() { dir = dir0 * St(exp_neg_f32(float(ycum))); }
     __device__ float saved() const { return float(ycum); }
     __device__ void restore(float v)
     {
@@ -613,6 +619,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
         {
             const int64_t j0 = s * K;
             const int nk = int(std::min<int64_t>(K, nlay - j0));
+            beam.refresh();
             ck(s, 0) = Real(R_up); ck(s, 1) = Real(D); ck(s, 2) = Real(beam.saved());
             Real tv[K], wv[K], gv[K];
 #pragma unroll
