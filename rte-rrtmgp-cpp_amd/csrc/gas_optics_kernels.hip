@@ -27,6 +27,11 @@ constexpr int GO_THREADS = 256;   // work-group: 4 wavefronts
 constexpr int GO_COLS = 64;       // columns of a work-group's layer
 constexpr int GO_GPTS = 16;       // g-points per chunk: the lanes of one pair
 constexpr int GO_ROW = 68;        // LDS row length (64 columns + padding)
+
+// Unpadded (g-point row, column) LDS tiles of the forward kernels: column c of row r sits at
+// c ^ 16 * (r / 4 % 4), so phase 2's stores (four rows gq + e, 16 columns each, per instruction)
+// and phase 3's row reads fall on distinct banks.
+__host__ __device__ constexpr int go_swz(int r, int c) { return c ^ (((r >> 2) & 3) << 4); }
 constexpr int GO_MAXM = 8;        // minor intervals per band and half of the atmosphere
 
 // One minor interval of a band (one half of the atmosphere), as gas indices of the col_gas
@@ -293,18 +298,18 @@ __device__ __forceinline__ Real minor_scaling_of(const ColOf& col, int gas, bool
 // band's slots with no per-g-point interval test, and a g-point outside an interval adds an
 // exact zero, so the sum equals the reference's sum over the intervals containing the g-point.
 
-// Grid of the fused kernels: units (chunk, layer, 64-column tile), chunk = (band, 16-g-point
-// part of the band), ordered so that consecutive units share a chunk.
+// Grid of the fused kernels: one work-group per (64-column tile, layer) unit, tile fastest; the
+// work-group runs the launch's chunks (band, 16-g-point part of the band) one after the other.
 // A launch covers bands b0 .. b0+nbnd-1; its outputs hold their g-points only, counted from the
-// first g-point of band b0.
-// Units are (64-column tile, layer, band chunk) triples, tile fastest.  With a tile list the
-// (tile, layer) pairs are tiles[0 .. nlist-1] (each ilay * ncoltiles + tile) instead of all of them.
+// first g-point of band b0.  With a tile list the units are tiles[0 .. nlist-1] (each
+// ilay * ncoltiles + tile) instead of all of them.
 struct GoGrid
 {
-    int64_t ncoltiles, nlay, cpb, nunits;
-    int b0;
+    int64_t ncoltiles, nlay, cpb;
+    int b0, nbnd;
     const int* tiles;
     int64_t nlist;
+    int64_t ntl;           // (tile, layer) units: the work-groups of the forward kernels
 };
 
 inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t b0, int64_t nbnd, int64_t max_band_gpts,
@@ -316,49 +321,29 @@ inline GoGrid make_go_grid(int64_t ncol, int64_t nlay, int64_t b0, int64_t nbnd,
     t.cpb = std::max<int64_t>(1, (max_band_gpts + GO_GPTS - 1) / GO_GPTS);
     t.tiles = tiles;
     t.nlist = tiles ? nlist : 0;
-    t.nunits = (tiles ? nlist : t.ncoltiles * nlay) * nbnd * t.cpb;
+    t.ntl = tiles ? nlist : t.ncoltiles * nlay;
     t.b0 = int(b0);
+    t.nbnd = int(nbnd);
     return t;
 }
 
-struct GoUnit
+// (64-column tile, layer) unit u of the forward kernels: tile fastest, or the listed ids
+// (ilay * ncoltiles + tile).  ok is false for a listed id outside the grid.
+struct GoTl
 {
-    int ibnd, c0, ncg;   // band, first g-point of the chunk within the band, g-points in the chunk
     int64_t ilay, col0;
+    bool ok;
 };
 
-// The launchers keep nunits below 2^31, so the decomposition runs in 32-bit arithmetic.
-__device__ __forceinline__ GoUnit go_unit(const GoBand* bands, const GoGrid& t, int64_t u64)
+__device__ __forceinline__ GoTl go_tl(const GoGrid& t, int64_t u64)
 {
-    GoUnit r;
-    const uint32_t u = uint32_t(u64), nct = uint32_t(t.ncoltiles), nl = uint32_t(t.nlay);
-    const uint32_t cpb = uint32_t(t.cpb);
-    uint32_t chunk;
-    if (t.tiles)
-    {
-        const uint32_t nls = uint32_t(t.nlist);
-        chunk = u / nls;
-        const uint32_t id = uint32_t(t.tiles[u - chunk * nls]);
-        const uint32_t ilay = id / nct;
-        r.col0 = int64_t(id - ilay * nct) * GO_COLS;
-        r.ilay = ilay;
-        if (ilay >= nl)                                  // not a tile of this grid: no work
-        {
-            r.ibnd = t.b0; r.c0 = 0; r.ncg = 0; r.ilay = 0;
-            return r;
-        }
-    }
-    else
-    {
-        const uint32_t rest = u / nct;
-        r.col0 = int64_t(u - rest * nct) * GO_COLS;
-        chunk = rest / nl;
-        r.ilay = rest - chunk * nl;
-    }
-    const int cb = int(chunk / cpb);
-    r.c0 = GO_GPTS * int(chunk - uint32_t(cb) * cpb);
-    r.ibnd = t.b0 + cb;
-    r.ncg = min(bands[r.ibnd].ng - r.c0, GO_GPTS);
+    GoTl r;
+    const uint32_t u = uint32_t(u64), nct = uint32_t(t.ncoltiles);
+    const uint32_t id = t.tiles ? uint32_t(t.tiles[u]) : u;
+    const uint32_t ilay = id / nct;
+    r.ilay = ilay;
+    r.col0 = int64_t(id - ilay * nct) * GO_COLS;
+    r.ok = ilay < uint32_t(t.nlay);
     return r;
 }
 
@@ -419,9 +404,11 @@ struct PairInGlobal
 };
 
 // ... or from the work-group's LDS rows, filled by all its threads at once (one memory round trip
-// instead of the dependent loads of one wavefront): row 0 p, 1 T, 2 col_dry, 3 .. 3 + ngas - 1
-// the vmr of gases 1 .. ngas, then (longwave) T of the level below and above and T_sfc.
+// instead of the dependent loads of one wavefront) and read by every chunk of the work-group:
+// row 0 p, 1 T, 2 col_dry, 3 .. 3 + ngas - 1 the vmr of gases 1 .. ngas, then (longwave) T of the
+// level below and above and T_sfc.
 constexpr int GO_IN_FIXED = 3;
+constexpr int GO_IN_MAXROWS = GO_IN_FIXED + RRTMGP_MAX_GAS + 3;   // the most input rows of a launch
 
 template<typename Real>
 struct PairInLds
@@ -450,7 +437,7 @@ __device__ __forceinline__ void stage_pair_inputs(const GasRef<Real>* sg, int ng
                                                   Real* rows)
 {
     // All loads first (row r is uniform over a wavefront), then the LDS stores.
-    constexpr int NIT = 2 * GO_GPTS * GO_ROW / GO_THREADS;
+    constexpr int NIT = (GO_IN_MAXROWS * GO_COLS + GO_THREADS - 1) / GO_THREADS;
     const int n = go_in_rows<LW>(ngas) * GO_COLS;
     Real v[NIT];
 #pragma unroll
@@ -667,7 +654,7 @@ __device__ __forceinline__ void rayleigh4(const Kd<Real>& k, const GoBand& bd, c
 //   phase 2 (lane = (pair, quad of g-points)): absorption and Planck fraction -> LDS;
 //   phase 3 (lane = column, wavefront = g-point row): sources, cloud add_to, coalesced stores.
 template<typename Real>
-__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 7 : 4)))
 void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, GoGrid grid,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ tlev, const Real* __restrict__ tsfc,
@@ -680,118 +667,125 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
 {
     static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
-    // Phase 1's input rows, then (after phase 1) phase 2's results per g-point row.
-    __shared__ Real s_buf[2 * GO_GPTS * GO_ROW];
-    Real (*const s_tau)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf);
-    Real (*const s_pf)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf + GO_GPTS * GO_ROW);
-    __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
-    const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
-    if (u >= grid.nunits) return;                       // uniform over the work-group
-    const GoUnit un = go_unit(k.bands, grid, u);
-    if (un.ncg <= 0) return;                            // uniform: an empty chunk of a short band
-    const GoBand& bd = k.bands[un.ibnd];
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int64_t ilay = un.ilay, ncl = ncol * nlay;
+    __shared__ __align__(16) Real s_tau[GO_GPTS][GO_COLS];
+    __shared__ Real s_pf[GO_GPTS][GO_COLS];
+    // The gas pointers are read by the input staging only: they share s_tau's LDS.
+    static_assert(sizeof(GasRef<Real>) * RRTMGP_MAX_GAS <= sizeof(Real) * GO_GPTS * GO_COLS, "sg in s_tau");
+    GasRef<Real>* const sg = reinterpret_cast<GasRef<Real>*>(&s_tau[0][0]);
+    extern __shared__ __align__(16) unsigned char go_dyn[];
+    Real* const s_in = reinterpret_cast<Real*>(go_dyn);      // phase 1's input rows (launch-sized)
+    const int64_t u = xcd_remap(blockIdx.x, grid.ntl);
+    if (u >= grid.ntl) return;                          // uniform over the work-group
+    const GoTl tl = go_tl(grid, u);
+    if (!tl.ok) return;                                 // uniform: a listed id outside the grid
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+    const int64_t ilay = tl.ilay, ncl = ncol * nlay;
     const bool at_sfc = (ilay == sfc_lay - 1);
-    const bool staged_in = go_in_rows<true>(k.ngas) * GO_COLS <= 2 * GO_GPTS * GO_ROW;
 
     if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
     __syncthreads();
-    if (staged_in)
+    stage_pair_inputs<Real, true>(sg, k.ngas, tl.col0, ilay, ncol, nlay, play, tlay, tlev, tsfc, col_dry_a, s_in);
+    __syncthreads();
+    // Every chunk (16 g-points of a band) of the launch's bands for this tile and layer: phase 1's
+    // inputs are loaded once and the work-group's launch and staging are shared by all of them.
+    const int nchunk = grid.nbnd * int(grid.cpb);
+    for (int cb = 0; cb < nchunk; ++cb)
     {
-        stage_pair_inputs<Real, true>(sg, k.ngas, un.col0, ilay, ncol, nlay, play, tlay, tlev, tsfc, col_dry_a,
-                                      s_buf);
+        const int ln = lane;
+        const int64_t icol = tl.col0 + ln;
+        const int64_t idx = icol + ncol * ilay;
+        const int ibnd = grid.b0 + cb / int(grid.cpb);
+        const GoBand& bd = k.bands[ibnd];
+        const int c0 = GO_GPTS * (cb % int(grid.cpb));
+        const int ncg = min(bd.ng - c0, GO_GPTS);
+        if (ncg <= 0) continue;                         // uniform: an empty chunk of a short band
+        if (w == 0 && c0 == 0)                          // pair states hold for all of a band's chunks
+        {
+            if (icol >= ncol)
+                sst[ln].flags = 0;
+            else
+                pair_state_in<Real, true>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, at_sfc, sst[ln]);
+        }
         __syncthreads();
-    }
-    if (w == 0)
-    {
-        const int64_t icol = un.col0 + lane;
-        if (icol >= ncol)
-            sst[lane].flags = 0;
-        else if (staged_in)
-            pair_state_in<Real, true>(k, bd, un.ibnd, PairInLds<Real>{s_buf, lane, k.ngas}, at_sfc, sst[lane]);
-        else
-            pair_state<Real, true>(k, sg, bd, un.ibnd, icol, ilay, ncol, play, tlay, tlev, tsfc, col_dry_a,
-                                   at_sfc, sst[lane]);
-    }
-    __syncthreads();
 
-    {
-        const int pi = w * 16 + lane / 4;               // pair
-        const int gq = 4 * (lane % 4);                  // first g-point of the quad in the chunk
-        const PairState<Real>& st = sst[pi];
-        Real t4[4] = {Real(0), Real(0), Real(0), Real(0)};
-        Real p4[4] = {Real(0), Real(0), Real(0), Real(0)};
-        if ((st.flags & 1) && gq < un.ncg)
         {
-            const uint32_t gl = uint32_t(un.c0 + gq);
-            absorption4(k, bd, st, gl, t4);
-            // Bounds the table reads in flight (and so the registers) to one gather group.
-            __builtin_amdgcn_sched_barrier(0);
-            planck_fraction4(k, bd, st, gl, p4);
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-        {
-            s_tau[gq + e][pi] = t4[e];
-            s_pf[gq + e][pi] = p4[e];
-        }
-    }
-    __syncthreads();
-
-    const int64_t icol = un.col0 + lane;
-    if (icol >= ncol) return;
-    const PairState<Real>& st = sst[lane];
-    const int64_t idx = icol + ncol * ilay;
-    const Real ctau = cloud_tau ? cloud_tau[idx + ncl * un.ibnd] : Real(0);
-    const int gbase = bd.g0 - k.bands[grid.b0].g0 + un.c0;
-    if (bnd_lay)
-    {
-        // Factored sources: Planck fractions per g-point, band Planck values per layer and level
-        // (written once, by the band's first chunk; level nlay by the top layer).
-        if (w == 0 && un.c0 == 0)
-        {
-            const int64_t nlev = nlay + 1, bb = un.ibnd - grid.b0;
-            bnd_lay[idx + ncl * bb] = st.aux[0];
-            bnd_lev[icol + ncol * ilay + ncol * nlev * bb] = st.aux[2];
-            if (ilay == nlay - 1) bnd_lev[icol + ncol * nlay + ncol * nlev * bb] = st.aux[1];
-        }
-        for (int row = w; row < un.ncg; row += GO_THREADS / 64)
-        {
-            const int gr = gbase + row;
-            const int64_t o = idx + ncl * gr;
-            const Real pf = s_pf[row][lane];
-            tau[o] = cloud_tau ? s_tau[row][lane] + ctau : s_tau[row][lane];
-            lay_src[o] = pf;
-            if (at_sfc)
+            const int pi = w * 16 + ln / 4;           // pair
+            const int gq = 4 * (ln % 4);              // first g-point of the quad in the chunk
+            const PairState<Real>& st = sst[pi];
+            Real t4[4] = {Real(0), Real(0), Real(0), Real(0)};
+            Real p4[4] = {Real(0), Real(0), Real(0), Real(0)};
+            if ((st.flags & 1) && gq < ncg)
             {
-                sfc_src[icol + ncol * gr] = pf * st.aux[3];
-                sfc_jac[icol + ncol * gr] = pf * (st.aux[4] - st.aux[3]);
+                const uint32_t gl = uint32_t(c0 + gq);
+                absorption4(k, bd, st, gl, t4);
+                // Bounds the table reads in flight (and so the registers) to one gather group.
+                __builtin_amdgcn_sched_barrier(0);
+                planck_fraction4(k, bd, st, gl, p4);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+            {
+                s_tau[gq + e][go_swz(gq + e, pi)] = t4[e];
+                s_pf[gq + e][go_swz(gq + e, pi)] = p4[e];
             }
         }
-        return;
-    }
-    for (int row = w; row < un.ncg; row += GO_THREADS / 64)
-    {
-        const int gr = gbase + row;
-        const int64_t o = idx + ncl * gr;
-        const Real pf = s_pf[row][lane];
-        tau[o] = cloud_tau ? s_tau[row][lane] + ctau : s_tau[row][lane];
-        lay_src[o] = pf * st.aux[0];
-        lev_inc[o] = pf * st.aux[1];
-        lev_dec[o] = pf * st.aux[2];
-        if (at_sfc)
+        __syncthreads();
+
+        if (icol < ncol)
         {
-            sfc_src[icol + ncol * gr] = pf * st.aux[3];
-            sfc_jac[icol + ncol * gr] = pf * (st.aux[4] - st.aux[3]);
+            const PairState<Real>& st = sst[ln];
+            const Real ctau = cloud_tau ? cloud_tau[idx + ncl * ibnd] : Real(0);
+            const int gbase = bd.g0 - k.bands[grid.b0].g0 + c0;
+            if (bnd_lay)
+            {
+                // Factored sources: Planck fractions per g-point, band Planck values per layer and
+                // level (written once, by the band's first chunk; level nlay by the top layer).
+                if (w == 0 && c0 == 0)
+                {
+                    const int64_t nlev = nlay + 1, bb = ibnd - grid.b0;
+                    bnd_lay[idx + ncl * bb] = st.aux[0];
+                    bnd_lev[icol + ncol * ilay + ncol * nlev * bb] = st.aux[2];
+                    if (ilay == nlay - 1) bnd_lev[icol + ncol * nlay + ncol * nlev * bb] = st.aux[1];
+                }
+                for (int row = w; row < ncg; row += GO_THREADS / 64)
+                {
+                    const int gr = gbase + row;
+                    const int64_t o = idx + ncl * gr;
+                    const Real pf = s_pf[row][go_swz(row, ln)];
+                    tau[o] = cloud_tau ? s_tau[row][go_swz(row, ln)] + ctau : s_tau[row][go_swz(row, ln)];
+                    lay_src[o] = pf;
+                    if (at_sfc)
+                    {
+                        sfc_src[icol + ncol * gr] = pf * st.aux[3];
+                        sfc_jac[icol + ncol * gr] = pf * (st.aux[4] - st.aux[3]);
+                    }
+                }
+            }
+            else
+                for (int row = w; row < ncg; row += GO_THREADS / 64)
+                {
+                    const int gr = gbase + row;
+                    const int64_t o = idx + ncl * gr;
+                    const Real pf = s_pf[row][go_swz(row, ln)];
+                    tau[o] = cloud_tau ? s_tau[row][go_swz(row, ln)] + ctau : s_tau[row][go_swz(row, ln)];
+                    lay_src[o] = pf * st.aux[0];
+                    lev_inc[o] = pf * st.aux[1];
+                    lev_dec[o] = pf * st.aux[2];
+                    if (at_sfc)
+                    {
+                        sfc_src[icol + ncol * gr] = pf * st.aux[3];
+                        sfc_jac[icol + ncol * gr] = pf * (st.aux[4] - st.aux[3]);
+                    }
+                }
         }
+        __syncthreads();                                // the next chunk rewrites sst and the rows
     }
 }
 
 // Fused shortwave gas optics: absorption + Rayleigh + combine (+ band cloud add_to), in the
 // same three phases.
 template<typename Real>
-__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ __launch_bounds__(GO_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 8 : 4)))
 void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay, GoGrid grid,
                           const Real* __restrict__ play, const Real* __restrict__ tlay,
                           const Real* __restrict__ col_dry_a, const Real* __restrict__ ctau_a,
@@ -800,118 +794,124 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
 {
     static_assert(GO_GPTS == 16 && GO_THREADS == 4 * GO_COLS, "lane layout of phase 2");
     __shared__ PairState<Real> sst[GO_COLS];
-    // Phase 1's input rows, then (after phase 1) phase 2's results per g-point row.
-    __shared__ Real s_buf[2 * GO_GPTS * GO_ROW];
-    Real (*const s_abs)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf);
-    Real (*const s_ray)[GO_ROW] = reinterpret_cast<Real (*)[GO_ROW]>(s_buf + GO_GPTS * GO_ROW);
-    __shared__ GasRef<Real> sg[RRTMGP_MAX_GAS];
-    const int64_t u = xcd_remap(blockIdx.x, grid.nunits);
-    if (u >= grid.nunits) return;
-    const GoUnit un = go_unit(k.bands, grid, u);
-    if (un.ncg <= 0) return;
-    const GoBand& bd = k.bands[un.ibnd];
-    const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int64_t ilay = un.ilay, ncl = ncol * nlay;
-    const bool staged_in = go_in_rows<false>(k.ngas) * GO_COLS <= 2 * GO_GPTS * GO_ROW;
+    __shared__ __align__(16) Real s_abs[GO_GPTS][GO_COLS];
+    __shared__ Real s_ray[GO_GPTS][GO_COLS];
+    // The gas pointers are read by the input staging only: they share s_abs's LDS.
+    static_assert(sizeof(GasRef<Real>) * RRTMGP_MAX_GAS <= sizeof(Real) * GO_GPTS * GO_COLS, "sg in s_abs");
+    GasRef<Real>* const sg = reinterpret_cast<GasRef<Real>*>(&s_abs[0][0]);
+    extern __shared__ __align__(16) unsigned char go_dyn[];
+    Real* const s_in = reinterpret_cast<Real*>(go_dyn);      // phase 1's input rows (launch-sized)
+    const int64_t u = xcd_remap(blockIdx.x, grid.ntl);
+    if (u >= grid.ntl) return;
+    const GoTl tl = go_tl(grid, u);
+    if (!tl.ok) return;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
+    const int64_t ilay = tl.ilay, ncl = ncol * nlay;
+    const Real eps2 = Real(2 * prop_eps);
+    const Real op_eps = Real(3) * Limits<Real>::tiny;
 
     if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
     __syncthreads();
-    if (staged_in)
+    stage_pair_inputs<Real, false>(sg, k.ngas, tl.col0, ilay, ncol, nlay, play, tlay, nullptr, nullptr, col_dry_a,
+                                   s_in);
+    __syncthreads();
+    const int nchunk = grid.nbnd * int(grid.cpb);
+    for (int cb = 0; cb < nchunk; ++cb)
     {
-        stage_pair_inputs<Real, false>(sg, k.ngas, un.col0, ilay, ncol, nlay, play, tlay, nullptr, nullptr,
-                                       col_dry_a, s_buf);
+        const int ln = lane;
+        const int64_t icol = tl.col0 + ln;
+        const int64_t idx = icol + ncol * ilay;
+        const int ibnd = grid.b0 + cb / int(grid.cpb);
+        const GoBand& bd = k.bands[ibnd];
+        const int c0 = GO_GPTS * (cb % int(grid.cpb));
+        const int ncg = min(bd.ng - c0, GO_GPTS);
+        if (ncg <= 0) continue;
+        if (w == 0 && c0 == 0)                          // pair states hold for all of a band's chunks
+        {
+            if (icol >= ncol)
+                sst[ln].flags = 0;
+            else
+                pair_state_in<Real, false>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, false, sst[ln]);
+        }
         __syncthreads();
-    }
-    if (w == 0)
-    {
-        const int64_t icol = un.col0 + lane;
-        if (icol >= ncol)
-            sst[lane].flags = 0;
-        else if (staged_in)
-            pair_state_in<Real, false>(k, bd, un.ibnd, PairInLds<Real>{s_buf, lane, k.ngas}, false, sst[lane]);
-        else
-            pair_state<Real, false>(k, sg, bd, un.ibnd, icol, ilay, ncol, play, tlay, nullptr, nullptr,
-                                    col_dry_a, false, sst[lane]);
-    }
-    __syncthreads();
 
-    {
-        const int pi = w * 16 + lane / 4;
-        const int gq = 4 * (lane % 4);
-        const PairState<Real>& st = sst[pi];
-        Real t4[4] = {Real(0), Real(0), Real(0), Real(0)};
-        Real r4[4] = {Real(0), Real(0), Real(0), Real(0)};
-        if ((st.flags & 1) && gq < un.ncg)
         {
-            const uint32_t gl = uint32_t(un.c0 + gq);
-            absorption4(k, bd, st, gl, t4);
-            __builtin_amdgcn_sched_barrier(0);
-            rayleigh4(k, bd, st, gl, r4);
-        }
+            const int pi = w * 16 + ln / 4;
+            const int gq = 4 * (ln % 4);
+            const PairState<Real>& st = sst[pi];
+            Real t4[4] = {Real(0), Real(0), Real(0), Real(0)};
+            Real r4[4] = {Real(0), Real(0), Real(0), Real(0)};
+            if ((st.flags & 1) && gq < ncg)
+            {
+                const uint32_t gl = uint32_t(c0 + gq);
+                absorption4(k, bd, st, gl, t4);
+                __builtin_amdgcn_sched_barrier(0);
+                rayleigh4(k, bd, st, gl, r4);
+            }
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-        {
-            s_abs[gq + e][pi] = t4[e];
-            s_ray[gq + e][pi] = r4[e];
+            for (int e = 0; e < 4; ++e)
+            {
+                s_abs[gq + e][go_swz(gq + e, pi)] = t4[e];
+                s_ray[gq + e][go_swz(gq + e, pi)] = r4[e];
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    const int64_t icol = un.col0 + lane;
-    if (icol >= ncol) return;
-    const int64_t idx = icol + ncol * ilay;
-    const Real eps2 = Real(2 * prop_eps);
-    const Real op_eps = Real(3) * Limits<Real>::tiny;
-    Real t2 = Real(0), s2 = Real(0), g2 = Real(0);
-    if (ctau_a)
-    {
-        const int64_t ib = idx + ncl * un.ibnd;
-        t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
-    }
-    const int gbase = bd.g0 - k.bands[grid.b0].g0 + un.c0;
-    for (int row = w; row < un.ncg; row += GO_THREADS / 64)
-    {
-        const Real tabs = s_abs[row][lane];
-        const Real tray = s_ray[row][lane];
-        const Real tt = tabs + tray;
-        const int64_t o = idx + ncl * (gbase + row);
-        if (coalb)
+        if (icol < ncol)
         {
-            // The same with the co-albedo 1 - ssa in place of ssa, for the cloud input and the
-            // output: formed from the absorption optical depths, it keeps the precision of the
-            // absorption that 1 - ssa in FP32 loses once it falls below 6e-8.
-            Real tt_out = tt;
-            Real co = (tt > eps2) ? tabs / tt : Real(1);
-            Real gg_out = Real(0);
+            Real t2 = Real(0), s2 = Real(0), g2 = Real(0);
             if (ctau_a)
             {
-                const Real tau12 = tt + t2;
-                const Real tauscat12 = (tt * (Real(1) - co)) + (t2 * (Real(1) - s2));
-                gg_out = (t2 * (Real(1) - s2) * g2) / max(tauscat12, op_eps);
-                co = (tau12 > op_eps) ? ((tt * co) + (t2 * s2)) / tau12 : Real(1) - tauscat12 / op_eps;
-                tt_out = tau12;
+                const int64_t ib = idx + ncl * ibnd;
+                t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
             }
-            tau[o] = tt_out;
-            ssa[o] = co;
-            gout[o] = gg_out;
-            continue;
+            const int gbase = bd.g0 - k.bands[grid.b0].g0 + c0;
+            for (int row = w; row < ncg; row += GO_THREADS / 64)
+            {
+                const Real tabs = s_abs[row][go_swz(row, ln)];
+                const Real tray = s_ray[row][go_swz(row, ln)];
+                const Real tt = tabs + tray;
+                const int64_t o = idx + ncl * (gbase + row);
+                if (coalb)
+                {
+                    // The same with the co-albedo 1 - ssa in place of ssa, for the cloud input and
+                    // the output: formed from the absorption optical depths, it keeps the precision
+                    // of the absorption that 1 - ssa in FP32 loses once it falls below 6e-8.
+                    Real tt_out = tt;
+                    Real co = (tt > eps2) ? tabs / tt : Real(1);
+                    Real gg_out = Real(0);
+                    if (ctau_a)
+                    {
+                        const Real tau12 = tt + t2;
+                        const Real tauscat12 = (tt * (Real(1) - co)) + (t2 * (Real(1) - s2));
+                        gg_out = (t2 * (Real(1) - s2) * g2) / max(tauscat12, op_eps);
+                        co = (tau12 > op_eps) ? ((tt * co) + (t2 * s2)) / tau12 : Real(1) - tauscat12 / op_eps;
+                        tt_out = tau12;
+                    }
+                    tau[o] = tt_out;
+                    ssa[o] = co;
+                    gout[o] = gg_out;
+                    continue;
+                }
+                // The CPU combine rule (Gas_optics_rrtmgp.cpp:365-385).
+                Real tt_out = tt;
+                Real ss = (tt > eps2) ? tray / tt : Real(0);
+                Real gg_out = Real(0);
+                if (ctau_a)
+                {
+                    // inc_2stream_by_2stream_bybnd (optical_props_kernels.cu:99-136).
+                    const Real tau12 = tt + t2;
+                    const Real tauscat12 = (tt * ss) + (t2 * s2);
+                    gg_out = ((tt * ss * Real(0)) + (t2 * s2 * g2)) / max(tauscat12, op_eps);
+                    ss = tauscat12 / max(op_eps, tau12);
+                    tt_out = tau12;
+                }
+                tau[o] = tt_out;
+                ssa[o] = ss;
+                gout[o] = gg_out;
+            }
         }
-        // The CPU combine rule (Gas_optics_rrtmgp.cpp:365-385).
-        Real tt_out = tt;
-        Real ss = (tt > eps2) ? tray / tt : Real(0);
-        Real gg_out = Real(0);
-        if (ctau_a)
-        {
-            // inc_2stream_by_2stream_bybnd (optical_props_kernels.cu:99-136).
-            const Real tau12 = tt + t2;
-            const Real tauscat12 = (tt * ss) + (t2 * s2);
-            gg_out = ((tt * ss * Real(0)) + (t2 * s2 * g2)) / max(tauscat12, op_eps);
-            ss = tauscat12 / max(op_eps, tau12);
-            tt_out = tau12;
-        }
-        tau[o] = tt_out;
-        ssa[o] = ss;
-        gout[o] = gg_out;
+        __syncthreads();                                // the next chunk rewrites sst and the rows
     }
 }
 
@@ -1566,8 +1566,8 @@ int check_kd_fused(const rrtmgp_kdist_dev* kd, bool lw)
 
 int check_go_grid(const GoGrid& g)
 {
-    if (g.nunits >= (int64_t(1) << 31))
-        return arg_error("fused gas optics: too many (column tile, layer, chunk) units for one launch; "
+    if (g.ntl >= (int64_t(1) << 31))
+        return arg_error("fused gas optics: too many (column tile, layer) units for one launch; "
                          "split the columns into blocks");
     return RRTMGP_OK;
 }
@@ -1777,7 +1777,8 @@ int gas_optics_lw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
     const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts, tiles, ntiles);
     RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_lw_kernel", as_stream(stream));
-    hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
+    const size_t dyn_lw = size_t(go_in_rows<true>(kd->ngas)) * GO_COLS * sizeof(Real);   // phase 1's input rows
+    hipLaunchKernelGGL(gas_optics_lw_kernel<Real>, dim3(xcd_grid(grid.ntl)), dim3(GO_THREADS), dyn_lw,
                        as_stream(stream), k, g, ncol, nlay, grid, play, tlay, tlev, tsfc, col_dry,
                        sfc_lay, cloud_tau, tau, lay_src, lev_inc, lev_dec, sfc_src, sfc_jac, bnd_lay, bnd_lev);
     RRTMGP_LAUNCHED("gas_optics_lw_kernel");
@@ -1804,7 +1805,8 @@ int gas_optics_sw(const rrtmgp_kdist_dev* kd, int64_t b0, int64_t nb, int64_t nc
     const GoGrid grid = make_go_grid(ncol, nlay, b0, nb, kd->max_band_gpts, tiles, ntiles);
     RRTMGP_TRY(check_go_grid(grid));
     RRTMGP_TIMED("gas_optics_sw_kernel", as_stream(stream));
-    hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.nunits)), dim3(GO_THREADS), 0,
+    const size_t dyn_sw = size_t(go_in_rows<false>(kd->ngas)) * GO_COLS * sizeof(Real);
+    hipLaunchKernelGGL(gas_optics_sw_kernel<Real>, dim3(xcd_grid(grid.ntl)), dim3(GO_THREADS), dyn_sw,
                        as_stream(stream), k, gs, ncol, nlay, grid, play, tlay, col_dry, ctau, cssa, cg,
                        tau, ssa, g, coalb ? 1 : 0);
     RRTMGP_LAUNCHED("gas_optics_sw_kernel");
