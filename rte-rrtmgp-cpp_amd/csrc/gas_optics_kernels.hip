@@ -736,6 +736,8 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
             const PairState<Real>& st = sst[ln];
             const Real ctau = cloud_tau ? cloud_tau[idx + ncl * ibnd] : Real(0);
             const int gbase = bd.g0 - k.bands[grid.b0].g0 + c0;
+            // The g-point outputs are streamed (non-temporal stores): the solvers read them back
+            // long after any cache could hold them.
             if (bnd_lay)
             {
                 // Factored sources: Planck fractions per g-point, band Planck values per layer and
@@ -752,8 +754,9 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                     const int gr = gbase + row;
                     const int64_t o = idx + ncl * gr;
                     const Real pf = s_pf[row][go_swz(row, ln)];
-                    tau[o] = cloud_tau ? s_tau[row][go_swz(row, ln)] + ctau : s_tau[row][go_swz(row, ln)];
-                    lay_src[o] = pf;
+                    const Real tv = s_tau[row][go_swz(row, ln)];
+                    __builtin_nontemporal_store(cloud_tau ? tv + ctau : tv, &tau[o]);
+                    __builtin_nontemporal_store(pf, &lay_src[o]);
                     if (at_sfc)
                     {
                         sfc_src[icol + ncol * gr] = pf * st.aux[3];
@@ -767,10 +770,11 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                     const int gr = gbase + row;
                     const int64_t o = idx + ncl * gr;
                     const Real pf = s_pf[row][go_swz(row, ln)];
-                    tau[o] = cloud_tau ? s_tau[row][go_swz(row, ln)] + ctau : s_tau[row][go_swz(row, ln)];
-                    lay_src[o] = pf * st.aux[0];
-                    lev_inc[o] = pf * st.aux[1];
-                    lev_dec[o] = pf * st.aux[2];
+                    const Real tv = s_tau[row][go_swz(row, ln)];
+                    __builtin_nontemporal_store(cloud_tau ? tv + ctau : tv, &tau[o]);
+                    __builtin_nontemporal_store(pf * st.aux[0], &lay_src[o]);
+                    __builtin_nontemporal_store(pf * st.aux[1], &lev_inc[o]);
+                    __builtin_nontemporal_store(pf * st.aux[2], &lev_dec[o]);
                     if (at_sfc)
                     {
                         sfc_src[icol + ncol * gr] = pf * st.aux[3];
@@ -888,9 +892,9 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                         co = (tau12 > op_eps) ? ((tt * co) + (t2 * s2)) / tau12 : Real(1) - tauscat12 / op_eps;
                         tt_out = tau12;
                     }
-                    tau[o] = tt_out;
-                    ssa[o] = co;
-                    gout[o] = gg_out;
+                    __builtin_nontemporal_store(tt_out, &tau[o]);
+                    __builtin_nontemporal_store(co, &ssa[o]);
+                    __builtin_nontemporal_store(gg_out, &gout[o]);
                     continue;
                 }
                 // The CPU combine rule (Gas_optics_rrtmgp.cpp:365-385).
@@ -906,9 +910,9 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                     ss = tauscat12 / max(op_eps, tau12);
                     tt_out = tau12;
                 }
-                tau[o] = tt_out;
-                ssa[o] = ss;
-                gout[o] = gg_out;
+                __builtin_nontemporal_store(tt_out, &tau[o]);
+                __builtin_nontemporal_store(ss, &ssa[o]);
+                __builtin_nontemporal_store(gg_out, &gout[o]);
             }
         }
         __syncthreads();                                // the next chunk rewrites sst and the rows

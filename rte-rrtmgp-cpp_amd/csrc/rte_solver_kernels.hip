@@ -627,7 +627,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                 if (kk < nk)
                 {
                     const uint32_t l = ucol + uncol * lay_of(j0 + kk);
-                    tv[kk] = at32(tau_g, l); wv[kk] = at32(ssa_g, l); gv[kk] = at32(gasy_g, l);
+                    tv[kk] = ld_nt(tau_g, l); wv[kk] = ld_nt(ssa_g, l); gv[kk] = ld_nt(gasy_g, l);
                 }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
@@ -682,7 +682,7 @@ void sw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1,
                 if (kk < nk)
                 {
                     const uint32_t l = ucol + uncol * lay_of(j0 + kk);
-                    tv[kk] = at32(tau_g, l); wv[kk] = at32(ssa_g, l); gv[kk] = at32(gasy_g, l);
+                    tv[kk] = ld_nt(tau_g, l); wv[kk] = ld_nt(ssa_g, l); gv[kk] = ld_nt(gasy_g, l);
                 }
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)

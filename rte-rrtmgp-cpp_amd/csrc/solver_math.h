@@ -44,6 +44,14 @@ __device__ __forceinline__ Real& at32(Real* base, uint32_t i)
     return *reinterpret_cast<Real*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(Real)));
 }
 
+// The same element read as a stream (non-temporal): inputs read once per sweep that no later
+// access finds in the cache.
+template<typename Real>
+__device__ __forceinline__ Real ld_nt(const Real* base, uint32_t i)
+{
+    return __builtin_nontemporal_load(&at32(base, i));
+}
+
 // Layer transmittance T = e^{-t} and the linear-in-tau source factor of lw_source_noscat
 // (F90:616-653): fact = (1 - T)/t - T, with t (1/2 - t/3) below sqrt(epsilon).  The FP32 build
 // evaluates fact for t < 1/4 from its series t/2 - t^2/3 + t^3/8 - t^4/30 + t^5/144 - t^6/840
