@@ -36,10 +36,12 @@ __device__ __forceinline__ float softplus_fast(float z)
     return fmaxf(z, 0.f) + __logf(1.f + __expf(-fabsf(z)));
 }
 
-// Store at a wave-uniform base plus a 32-bit lane offset (scalar base, one VGPR of offset).
+// Store at a wave-uniform base plus a 32-bit lane offset (scalar base, one VGPR of offset),
+// streamed (non-temporal): the g-point outputs are read back by the solvers long after any cache
+// could hold them.
 __device__ __forceinline__ void store_at(float* base, uint32_t i, float v)
 {
-    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(float))) = v;
+    __builtin_nontemporal_store(v, reinterpret_cast<float*>(reinterpret_cast<char*>(base) + i * uint32_t(sizeof(float))));
 }
 __device__ __forceinline__ float sigmoid(float z) { return 1.f / (1.f + expf(-z)); }
 
