@@ -260,8 +260,11 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
     // Inputs of levels s0 .. s0+K-1 of a sweep (level j > 0 ends layer layer_of(j)).  With factored
     // sources the level source of layer L is at level L + lsh (lev_src_inc: L + 1, lev_src_dec: L),
     // and the products are the gas optics' own (no contraction), so the sources are bit-identical.
+    // The up sweep (nt) reads tau and the Planck fractions for the last time and streams them; the
+    // down sweep's reads stay cacheable, so the bottom layers it read last are partly still in the
+    // last-level cache when the up sweep starts there.
     Real bt[K], bl[K], bx[K];
-    auto fetch = [&](int64_t s0, const Real* lev_g, int lsh, auto layer_of) {
+    auto fetch = [&](int64_t s0, const Real* lev_g, int lsh, auto layer_of, bool nt) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk)
         {
@@ -271,11 +274,11 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
             {
                 const uint32_t L = uint32_t(layer_of(j));
                 const uint32_t l = ucol + uncol * L;
-                bt[kk] = at32(tau_g, l);
+                bt[kk] = nt ? ld_nt(tau_g, l) : at32(tau_g, l);
                 if constexpr (FACT)
                 {
 #pragma clang fp contract(off)
-                    const Real pf = at32(lay_g, l);
+                    const Real pf = nt ? ld_nt(lay_g, l) : at32(lay_g, l);
                     bl[kk] = pf * at32(blay_g, l);
                     bx[kk] = pf * at32(blev_g, ucol + uncol * (L + uint32_t(lsh)));
                 }
@@ -288,13 +291,13 @@ void lw_solver_kernel(int64_t ncol, int64_t nlay, int64_t ngpt, int top_at_1, in
     };
     // One sweep over levels 0..nlay; lev_out(j) is the output level of sweep level j.
     auto sweep = [&](const Real* lev_g, int lsh, auto layer_of, auto lev_out, Real* flux_g, Real* grp_out, bool up) {
-        fetch(0, lev_g, lsh, layer_of);
+        fetch(0, lev_g, lsh, layer_of, up);
         for (int64_t s0 = 0; s0 <= nlay; s0 += K)
         {
             Real ct[K], cl[K], cx[K];
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) { ct[kk] = bt[kk]; cl[kk] = bl[kk]; cx[kk] = bx[kk]; }
-            if (s0 + K <= nlay) fetch(s0 + K, lev_g, lsh, layer_of);
+            if (s0 + K <= nlay) fetch(s0 + K, lev_g, lsh, layer_of, up);
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
             {
