@@ -469,13 +469,14 @@ __device__ __forceinline__ void stage_pair_inputs(const GasRef<Real>* sg, int ng
     }
 }
 
-// Phase 1 for one pair (lane = column).
-template<typename Real, bool LW, typename In>
-__device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& bd, int ibnd, const In& in,
-                                              bool at_sfc, PairState<Real>& st)
+// Phase 1 for one pair (lane = column), in parts: the interpolation state of the major gathers
+// (ps_major), the minor scalings of slots q0, q0 + dq, ... (ps_minor), and the band Planck values
+// (LW) or the Rayleigh column (SW) (ps_aux).  pair_state_in runs them all; the fused forward
+// kernels share them out over their four wavefronts (pair_state_split).
+template<typename Real, typename In>
+__device__ __forceinline__ void ps_major(const Kd<Real>& k, const GoBand& bd, const CL<Real>& c, const In& in, Real cd,
+                                         PairState<Real>& st)
 {
-    const Real pv = in.p(), tv = in.t(), cd = in.cd();
-    const CL<Real> c = interp_cl(k, pv, tv);
     const int s = c.itropo;
     const int gas1 = bd.gas[s][0], gas2 = bd.gas[s][1];
     FL<Real> f;
@@ -495,8 +496,15 @@ __device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& b
         st.fminor[i][0] = f.fminor[i][0];
         st.fminor[i][1] = f.fminor[i][1];
     }
+    st.flags = 1;
+}
+
+template<typename Real, typename In>
+__device__ __forceinline__ void ps_minor(const Kd<Real>& k, const GoBand& bd, int s, const In& in, Real pv, Real tv,
+                                         Real cd, int q0, int dq, PairState<Real>& st)
+{
     const int nq = bd.nq[s];
-    for (int q = 0; q < bd.nslot; ++q)
+    for (int q = q0; q < bd.nslot; q += dq)
     {
         Real sc = Real(0);
         if (q < nq)
@@ -508,7 +516,12 @@ __device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& b
         }
         st.scaling[q] = sc;
     }
-    st.flags = 1;
+}
+
+template<typename Real, bool LW, typename In>
+__device__ __forceinline__ void ps_aux(const Kd<Real>& k, int ibnd, const In& in, Real tv, Real cd, bool at_sfc,
+                                       PairState<Real>& st)
+{
     if constexpr (LW)
     {
         const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
@@ -524,6 +537,43 @@ __device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& b
     }
     else
         st.aux[0] = in.col(k.idx_h2o, cd) + cd;
+}
+
+template<typename Real, bool LW, typename In>
+__device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& bd, int ibnd, const In& in,
+                                              bool at_sfc, PairState<Real>& st)
+{
+    const Real pv = in.p(), tv = in.t(), cd = in.cd();
+    const CL<Real> c = interp_cl(k, pv, tv);
+    ps_major(k, bd, c, in, cd, st);
+    ps_minor(k, bd, c.itropo, in, pv, tv, cd, 0, 1, st);
+    ps_aux<Real, LW>(k, ibnd, in, tv, cd, at_sfc, st);
+}
+
+// Phase 1 of the fused forward kernels on wavefront w (uniform) of four, every wavefront taking all
+// 64 columns: w = 0 the major interpolation state; LW: w = 2 the Planck values, w = 1 and 3 the even
+// and odd minor slots; SW: w = 1, 2, 3 every third minor slot, w = 1 also the Rayleigh column.  The
+// parts write disjoint fields of the pair state.
+template<typename Real, bool LW, typename In>
+__device__ __forceinline__ void pair_state_split(const Kd<Real>& k, const GoBand& bd, int ibnd, const In& in,
+                                                 bool at_sfc, PairState<Real>& st, int w)
+{
+    const Real pv = in.p(), tv = in.t(), cd = in.cd();
+    const CL<Real> c = interp_cl(k, pv, tv);
+    if (w == 0)
+        ps_major(k, bd, c, in, cd, st);
+    else if constexpr (LW)
+    {
+        if (w == 2)
+            ps_aux<Real, true>(k, ibnd, in, tv, cd, at_sfc, st);
+        else
+            ps_minor(k, bd, c.itropo, in, pv, tv, cd, w == 1 ? 0 : 1, 2, st);
+    }
+    else
+    {
+        ps_minor(k, bd, c.itropo, in, pv, tv, cd, w - 1, 3, st);
+        if (w == 1) ps_aux<Real, false>(k, ibnd, in, tv, cd, false, st);
+    }
 }
 
 template<typename Real, bool LW>
@@ -649,8 +699,11 @@ __device__ __forceinline__ void rayleigh4(const Kd<Real>& k, const GoBand& bd, c
 }
 
 // ---------------------------------------------------------------------------------------------
-// Fused longwave gas optics: tau (+ cloud absorption per band) and Planck sources.
-//   phase 1 (wavefront 0, lane = column): pair states -> LDS;
+// Fused longwave gas optics: tau (+ cloud absorption per band) and Planck sources.  A work-group
+// takes one (64-column tile, layer) unit: it stages the columns' inputs in LDS once, then runs every
+// 16-g-point chunk of the launch's bands in three phases:
+//   phase 1 (first chunk of a band; lane = column, the parts of the pair state spread over the four
+//   wavefronts): pair states -> LDS;
 //   phase 2 (lane = (pair, quad of g-points)): absorption and Planck fraction -> LDS;
 //   phase 3 (lane = column, wavefront = g-point row): sources, cloud add_to, coalesced stores.
 template<typename Real>
@@ -699,12 +752,14 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         const int c0 = GO_GPTS * (cb % int(grid.cpb));
         const int ncg = min(bd.ng - c0, GO_GPTS);
         if (ncg <= 0) continue;                         // uniform: an empty chunk of a short band
-        if (w == 0 && c0 == 0)                          // pair states hold for all of a band's chunks
+        if (c0 == 0)                                    // pair states hold for all of a band's chunks
         {
             if (icol >= ncol)
-                sst[ln].flags = 0;
+            {
+                if (w == 0) sst[ln].flags = 0;
+            }
             else
-                pair_state_in<Real, true>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, at_sfc, sst[ln]);
+                pair_state_split<Real, true>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, at_sfc, sst[ln], w);
         }
         __syncthreads();
 
@@ -830,12 +885,14 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         const int c0 = GO_GPTS * (cb % int(grid.cpb));
         const int ncg = min(bd.ng - c0, GO_GPTS);
         if (ncg <= 0) continue;
-        if (w == 0 && c0 == 0)                          // pair states hold for all of a band's chunks
+        if (c0 == 0)                                    // pair states hold for all of a band's chunks
         {
             if (icol >= ncol)
-                sst[ln].flags = 0;
+            {
+                if (w == 0) sst[ln].flags = 0;
+            }
             else
-                pair_state_in<Real, false>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, false, sst[ln]);
+                pair_state_split<Real, false>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, false, sst[ln], w);
         }
         __syncthreads();
 
