@@ -189,6 +189,12 @@ __device__ __forceinline__ CL<Real> interp_cl(const Kd<Real>& k, Real p, Real t)
     return c;
 }
 
+// fmod(x, 1) without the general remainder routine: x - trunc(x) is exact in floating point (the
+// difference of x and its integer part is representable), and taking x's sign keeps fmod's signed
+// zero for negative integers; infinities and NaN give NaN as fmod does.
+template<typename Real>
+__device__ __forceinline__ Real frac1(Real x) { return copysign(x - trunc(x), x); }
+
 // Per-flavor part of interpolation_kernel (.cu:254-291).
 template<typename Real>
 __device__ __forceinline__ void interp_flav(const Kd<Real>& k, const CL<Real>& c, int gas1, int gas2,
@@ -206,7 +212,7 @@ __device__ __forceinline__ void interp_flav(const Kd<Real>& k, const CL<Real>& c
         const Real loceta = eta * Real(k.neta - 1);
         // The lower clamp only matters for non-finite inputs; it keeps every gather in bounds.
         f.jeta[itemp] = max(1, min(int(loceta) + 1, k.neta - 1));
-        const Real feta = fmod(loceta, Real(1));
+        const Real feta = frac1(loceta);
         const Real ftemp_term = Real(1 - itemp) + Real(2 * itemp - 1) * c.ftemp;
         const Real m0 = (Real(1) - feta) * ftemp_term;
         const Real m1 = feta * ftemp_term;
@@ -1032,7 +1038,7 @@ __device__ __forceinline__ void pair_state_backward(const Kd<Real>& k, const Gas
         const Real cm = c1 + r * c2;
         const bool mix = cm > Real(2) * Limits<Real>::tiny;
         const Real eta = mix ? c1 / cm : Real(0.5);
-        const Real feta = fmod(eta * Real(k.neta - 1), Real(1));
+        const Real feta = frac1(eta * Real(k.neta - 1));
         const Real ftt = i == 0 ? Real(1) - c.ftemp : c.ftemp;
         const Real m0 = (Real(1) - feta) * ftt, m1 = feta * ftt;
         const Real* gfm = acc + GA_FM + 4 * i;
