@@ -255,14 +255,29 @@ __device__ __forceinline__ Real planck_fraction(const Kd<Real>& k, const CL<Real
          + (f.fmajor[1][0] * p1[0] + f.fmajor[1][1] * p1[se] + f.fmajor[1][2] * p1[sp] + f.fmajor[1][3] * p1[sp + se]);
 }
 
-// interpolate1D (.cu:1-12).
+// interpolate1D (.cu:1-12), in two steps: the position of val in the table (index and fraction),
+// which the fused kernels compute once per column for all bands, and the lookup.
+template<typename Real>
+__device__ __forceinline__ void interp1d_pos(Real val, Real offset, Real delta, int len, int& idx, Real& frac)
+{
+    const Real val0 = (val - offset) / delta;
+    frac = val0 - Real(int(val0));
+    idx = min(len - 1, max(1, int(val0) + 1));
+}
+
+template<typename Real>
+__device__ __forceinline__ Real interp1d_at(const Real* table, int idx, Real frac)
+{
+    return table[idx - 1] + frac * (table[idx] - table[idx - 1]);
+}
+
 template<typename Real>
 __device__ __forceinline__ Real interpolate1D(Real val, Real offset, Real delta, int len, const Real* table)
 {
-    const Real val0 = (val - offset) / delta;
-    const Real frac = val0 - Real(int(val0));
-    const int idx = min(len - 1, max(1, int(val0) + 1));
-    return table[idx - 1] + frac * (table[idx] - table[idx - 1]);
+    int idx;
+    Real frac;
+    interp1d_pos(val, offset, delta, len, idx, frac);
+    return interp1d_at(table, idx, frac);
 }
 
 // Scaling of one minor interval at one (col, lay) (.cu:388-420); col(gas) gives the column
@@ -556,22 +571,66 @@ __device__ __forceinline__ void pair_state_in(const Kd<Real>& k, const GoBand& b
     ps_aux<Real, LW>(k, ibnd, in, tv, cd, at_sfc, st);
 }
 
+// Band-independent state of a column, computed once per work-group by the fused kernels and kept
+// in LDS (element [i][column]): ci = jtemp, jpress, itropo (+ LW: Planck-table index of T_lay, T_hi,
+// T_lo); cf = ftemp, fpress (+ LW: the Planck-table fractions).
+template<typename Real, bool LW>
+struct ColState
+{
+    int ci[LW ? 6 : 3][GO_COLS];
+    Real cf[LW ? 5 : 2][GO_COLS];
+};
+
+// Fills ColState for the 64 columns (wavefront 0: the interpolation; LW, wavefront 1: the Planck
+// positions); the caller synchronises before it is read.
+template<typename Real, bool LW, typename In>
+__device__ __forceinline__ void col_state(const Kd<Real>& k, const In& in, int w, int lane, ColState<Real, LW>& cs)
+{
+    if (w == 0)
+    {
+        const CL<Real> c = interp_cl(k, in.p(), in.t());
+        cs.ci[0][lane] = c.jtemp; cs.ci[1][lane] = c.jpress; cs.ci[2][lane] = c.itropo;
+        cs.cf[0][lane] = c.ftemp; cs.cf[1][lane] = c.fpress;
+    }
+    else if constexpr (LW)
+    {
+        if (w == 1)
+        {
+            const Real t[3] = {in.t(), in.t_hi(), in.t_lo()};
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                interp1d_pos(t[q], k.temp_ref_min, k.totplnk_delta, k.nplanck, cs.ci[3 + q][lane], cs.cf[2 + q][lane]);
+        }
+    }
+}
+
 // Phase 1 of the fused forward kernels on wavefront w (uniform) of four, every wavefront taking all
 // 64 columns: w = 0 the major interpolation state; LW: w = 2 the Planck values, w = 1 and 3 the even
 // and odd minor slots; SW: w = 1, 2, 3 every third minor slot, w = 1 also the Rayleigh column.  The
 // parts write disjoint fields of the pair state.
 template<typename Real, bool LW, typename In>
 __device__ __forceinline__ void pair_state_split(const Kd<Real>& k, const GoBand& bd, int ibnd, const In& in,
-                                                 bool at_sfc, PairState<Real>& st, int w)
+                                                 const ColState<Real, LW>& cs, int lane, bool at_sfc,
+                                                 PairState<Real>& st, int w)
 {
     const Real pv = in.p(), tv = in.t(), cd = in.cd();
-    const CL<Real> c = interp_cl(k, pv, tv);
+    const CL<Real> c{cs.ci[0][lane], cs.ci[1][lane], cs.ci[2][lane], cs.cf[0][lane], cs.cf[1][lane]};
     if (w == 0)
         ps_major(k, bd, c, in, cd, st);
     else if constexpr (LW)
     {
         if (w == 2)
-            ps_aux<Real, true>(k, ibnd, in, tv, cd, at_sfc, st);
+        {
+            const Real* tab = k.totplnk + int64_t(ibnd) * k.nplanck;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) st.aux[q] = interp1d_at(tab, cs.ci[3 + q][lane], cs.cf[2 + q][lane]);
+            if (at_sfc)
+            {
+                const Real ts = in.t_sfc();
+                st.aux[3] = interpolate1D(ts, k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+                st.aux[4] = interpolate1D(ts + Real(1), k.temp_ref_min, k.totplnk_delta, k.nplanck, tab);
+            }
+        }
         else
             ps_minor(k, bd, c.itropo, in, pv, tv, cd, w == 1 ? 0 : 1, 2, st);
     }
@@ -728,6 +787,7 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     __shared__ PairState<Real> sst[GO_COLS];
     __shared__ __align__(16) Real s_tau[GO_GPTS][GO_COLS];
     __shared__ Real s_pf[GO_GPTS][GO_COLS];
+    __shared__ ColState<Real, true> s_cs;
     // The gas pointers are read by the input staging only: they share s_tau's LDS.
     static_assert(sizeof(GasRef<Real>) * RRTMGP_MAX_GAS <= sizeof(Real) * GO_GPTS * GO_COLS, "sg in s_tau");
     GasRef<Real>* const sg = reinterpret_cast<GasRef<Real>*>(&s_tau[0][0]);
@@ -744,6 +804,8 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     if (threadIdx.x == 0) stage_gases(g, k.ngas, sg);
     __syncthreads();
     stage_pair_inputs<Real, true>(sg, k.ngas, tl.col0, ilay, ncol, nlay, play, tlay, tlev, tsfc, col_dry_a, s_in);
+    __syncthreads();
+    col_state<Real, true>(k, PairInLds<Real>{s_in, lane, k.ngas}, w, lane, s_cs);
     __syncthreads();
     // Every chunk (16 g-points of a band) of the launch's bands for this tile and layer: phase 1's
     // inputs are loaded once and the work-group's launch and staging are shared by all of them.
@@ -765,7 +827,8 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                 if (w == 0) sst[ln].flags = 0;
             }
             else
-                pair_state_split<Real, true>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, at_sfc, sst[ln], w);
+                pair_state_split<Real, true>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, s_cs, ln, at_sfc,
+                                             sst[ln], w);
         }
         __syncthreads();
 
@@ -861,6 +924,7 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     __shared__ PairState<Real> sst[GO_COLS];
     __shared__ __align__(16) Real s_abs[GO_GPTS][GO_COLS];
     __shared__ Real s_ray[GO_GPTS][GO_COLS];
+    __shared__ ColState<Real, false> s_cs;
     // The gas pointers are read by the input staging only: they share s_abs's LDS.
     static_assert(sizeof(GasRef<Real>) * RRTMGP_MAX_GAS <= sizeof(Real) * GO_GPTS * GO_COLS, "sg in s_abs");
     GasRef<Real>* const sg = reinterpret_cast<GasRef<Real>*>(&s_abs[0][0]);
@@ -880,6 +944,8 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
     stage_pair_inputs<Real, false>(sg, k.ngas, tl.col0, ilay, ncol, nlay, play, tlay, nullptr, nullptr, col_dry_a,
                                    s_in);
     __syncthreads();
+    col_state<Real, false>(k, PairInLds<Real>{s_in, lane, k.ngas}, w, lane, s_cs);
+    __syncthreads();
     const int nchunk = grid.nbnd * int(grid.cpb);
     for (int cb = 0; cb < nchunk; ++cb)
     {
@@ -898,7 +964,8 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
                 if (w == 0) sst[ln].flags = 0;
             }
             else
-                pair_state_split<Real, false>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, false, sst[ln], w);
+                pair_state_split<Real, false>(k, bd, ibnd, PairInLds<Real>{s_in, ln, k.ngas}, s_cs, ln, false,
+                                              sst[ln], w);
         }
         __syncthreads();
 
