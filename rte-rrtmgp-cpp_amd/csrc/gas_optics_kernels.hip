@@ -832,6 +832,8 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         }
         __syncthreads();
 
+        // The band cloud optical depth of phase 3's column, requested before the gathers.
+        const Real ctau = (cloud_tau && icol < ncol) ? cloud_tau[idx + ncl * ibnd] : Real(0);
         {
             const int pi = w * 16 + ln / 4;           // pair
             const int gq = 4 * (ln % 4);              // first g-point of the quad in the chunk
@@ -858,7 +860,6 @@ void gas_optics_lw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         if (icol < ncol)
         {
             const PairState<Real>& st = sst[ln];
-            const Real ctau = cloud_tau ? cloud_tau[idx + ncl * ibnd] : Real(0);
             const int gbase = bd.g0 - k.bands[grid.b0].g0 + c0;
             // The g-point outputs are streamed (non-temporal stores): the solvers read them back
             // long after any cache could hold them.
@@ -969,6 +970,13 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
         }
         __syncthreads();
 
+        // The band cloud properties of phase 3's column, requested before the gathers.
+        Real t2 = Real(0), s2 = Real(0), g2 = Real(0);
+        if (ctau_a && icol < ncol)
+        {
+            const int64_t ib = idx + ncl * ibnd;
+            t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
+        }
         {
             const int pi = w * 16 + ln / 4;
             const int gq = 4 * (ln % 4);
@@ -993,12 +1001,6 @@ void gas_optics_sw_kernel(Kd<Real> k, Gases<Real> g, int64_t ncol, int64_t nlay,
 
         if (icol < ncol)
         {
-            Real t2 = Real(0), s2 = Real(0), g2 = Real(0);
-            if (ctau_a)
-            {
-                const int64_t ib = idx + ncl * ibnd;
-                t2 = ctau_a[ib]; s2 = cssa_a[ib]; g2 = cg_a[ib];
-            }
             const int gbase = bd.g0 - k.bands[grid.b0].g0 + c0;
             for (int row = w; row < ncg; row += GO_THREADS / 64)
             {
